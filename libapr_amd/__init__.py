@@ -1,0 +1,373 @@
+"""libapr_amd -- MI355X (gfx950) implementation of LibAPR's APR hot path.
+
+Python binding of the C-ABI in include/aprh.h (libapr_amd/_lib/libaprh.so).
+The classes mirror the reference's host interface for this path:
+
+    APRParameters         src/algorithm/APRParameters.hpp:11-70
+    APRConverter          src/algorithm/APRConverter.hpp:34-134  (get_apr)
+    APR                   src/data_structures/APR/APR.hpp (GenInfo + LinearAccess)
+    ParticleData          src/data_structures/APR/particles/ParticleData.hpp (sample_image)
+
+Arrays are numpy (host) or device pointers (``int`` addresses, e.g. from
+``torch.Tensor.data_ptr()``); volumes are indexed [z, x, y] with y fastest,
+matching PixelData's memory order.
+
+The HIP library is required: importing works without it, but every entry
+point raises ``RuntimeError`` when libaprh.so has not been built
+(``python -m libapr_amd.build``).  There is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libaprh.so")
+
+APRH_OK = 0
+APRH_ERR_INVALID_ARGUMENT = 1
+APRH_ERR_OVERFLOW = 2
+APRH_ERR_BAD_DIMS = 3
+APRH_ERR_UNSUPPORTED = 4
+APRH_ERR_OUT_OF_MEMORY = 5
+APRH_ERR_HIP = 6
+
+STAGES = ("offset", "smooth_bspline", "gradient", "local_intensity_scale", "solve_for_apr",
+          "generate_data_structures", "sample")
+
+
+class AprhError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[aprh {code}] {msg}")
+        self.code = code
+
+
+class AprhOverflowError(AprhError, ValueError):
+    """u16 IIR under/overflow -- the reference throws std::invalid_argument
+    (ComputeGradient.hpp:409-412)."""
+
+
+class AprhParams(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("dx", "dy", "dz", "psfx", "psfy", "psfz", "Ip_th", "lambda_", "rel_error",
+                                         "sigma_th", "sigma_th_max", "grad_th")] + \
+               [(n, C.c_int) for n in ("reflect_bc_lis", "neighborhood_optimization", "constant_intensity_scale")]
+
+
+class AprhAprInfo(C.Structure):
+    _fields_ = [("l_min", C.c_int32), ("l_max", C.c_int32), ("org_dims", C.c_int64 * 3),
+                ("total_number_particles", C.c_uint64), ("xz_end_size", C.c_uint64), ("level_xz_size", C.c_uint64),
+                ("bspline_offset", C.c_float)]
+
+
+_lib = None
+vp = C.c_void_p
+i64 = C.c_int64
+
+
+def lib():
+    """The loaded libaprh.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libaprh.so not built ({LIB_PATH}); run `python -m libapr_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    L.aprh_last_error.restype = C.c_char_p
+    L.aprh_context_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.aprh_context_destroy.argtypes = [vp]
+    L.aprh_context_set_keep_intermediates.argtypes = [vp, C.c_int]
+    L.aprh_context_set_timing.argtypes = [vp, C.c_int]
+    L.aprh_context_stage_times.argtypes = [vp, C.POINTER(C.c_float)]
+    L.aprh_context_release_cache.argtypes = [vp]
+    L.aprh_get_apr_u16.argtypes = [vp, vp, i64, i64, i64, C.POINTER(AprhParams), C.POINTER(vp)]
+    L.aprh_apr_from_grad_lis.argtypes = [vp, vp, vp, vp, i64, i64, i64, C.POINTER(AprhParams), C.c_float, C.POINTER(vp)]
+    L.aprh_apr_info_get.argtypes = [vp, C.POINTER(AprhAprInfo)]
+    L.aprh_apr_copy_structure.argtypes = [vp, vp, vp, vp]
+    L.aprh_apr_device_structure.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
+    L.aprh_apr_intermediate.argtypes = [vp, C.c_int, vp]
+    L.aprh_apr_particle_cell_tree.argtypes = [vp, C.c_int, vp, C.POINTER(C.c_int64)]
+    L.aprh_apr_destroy.argtypes = [vp]
+    L.aprh_sample_image_u16.argtypes = [vp, vp, vp, vp]
+    L.aprh_bspline_filt_rec_u16.argtypes = [vp, vp, i64, i64, i64, C.c_int, C.c_float, C.c_float]
+    L.aprh_bspline_filt_rec_f32.argtypes = [vp, vp, i64, i64, i64, C.c_int, C.c_float, C.c_float]
+    L.aprh_inv_bspline_f32.argtypes = [vp, vp, i64, i64, i64, C.c_int]
+    L.aprh_gradient_ds_u16.argtypes = [vp, vp, vp, vp, i64, i64, i64, C.c_float, C.c_float, C.c_float]
+    L.aprh_sat_mean_f32.argtypes = [vp, vp, i64, i64, i64, C.c_int, C.c_int]
+    L.aprh_local_intensity_scale.argtypes = [vp, vp, vp, i64, i64, i64, C.POINTER(AprhParams)]
+    L.aprh_synth_blobs_host.argtypes = [vp, i64, i64, i64, C.c_uint64, i64, C.c_float, C.c_float, C.c_float, C.c_float,
+                                        C.c_float, C.c_int]
+    L.aprh_synth_blobs_device.argtypes = [vp, vp, i64, i64, i64, C.c_uint64, i64, C.c_float, C.c_float, C.c_float,
+                                          C.c_float, C.c_float, C.c_int]
+    L.aprh_malloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
+    L.aprh_free.argtypes = [vp, vp]
+    L.aprh_memcpy.argtypes = [vp, vp, vp, C.c_size_t]
+    _lib = L
+    return L
+
+
+def _check(code):
+    if code != APRH_OK:
+        msg = lib().aprh_last_error().decode(errors="replace")
+        if code == APRH_ERR_OVERFLOW:
+            raise AprhOverflowError(code, msg)
+        raise AprhError(code, msg)
+
+
+def _ptr(a):
+    """host numpy array -> pointer; int -> device address"""
+    if isinstance(a, (int, np.integer)):
+        return C.c_void_p(int(a))
+    if hasattr(a, "data_ptr"):
+        return C.c_void_p(a.data_ptr())
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return C.c_void_p(a.ctypes.data)
+
+
+class APRParameters:
+    """APRParameters (APRParameters.hpp:11-70); defaults of the reference."""
+
+    def __init__(self, **kw):
+        self.dx = self.dy = self.dz = 1.0
+        self.psfx = self.psfy = self.psfz = 2.0
+        self.Ip_th = 0.0
+        self.lambda_ = 3.0
+        self.rel_error = 0.1
+        self.sigma_th = 0.0
+        self.sigma_th_max = 0.0
+        self.grad_th = 0.0
+        self.reflect_bc_lis = True
+        self.neighborhood_optimization = True
+        self.constant_intensity_scale = False
+        for k, v in kw.items():
+            setattr(self, "lambda_" if k == "lambda" else k, v)
+
+    def c(self):
+        return AprhParams(self.dx, self.dy, self.dz, self.psfx, self.psfy, self.psfz, self.Ip_th, self.lambda_,
+                          self.rel_error, self.sigma_th, self.sigma_th_max, self.grad_th, int(self.reflect_bc_lis),
+                          int(self.neighborhood_optimization), int(self.constant_intensity_scale))
+
+
+class Context:
+    """One HIP stream on one device (aprh_context)."""
+
+    def __init__(self, device=0, keep_intermediates=False, timing=False):
+        h = vp()
+        _check(lib().aprh_context_create(device, C.byref(h)))
+        self.h = h
+        self.device = device
+        if keep_intermediates:
+            _check(lib().aprh_context_set_keep_intermediates(h, 1))
+        if timing:
+            _check(lib().aprh_context_set_timing(h, 1))
+
+    def set_keep_intermediates(self, on):
+        _check(lib().aprh_context_set_keep_intermediates(self.h, int(on)))
+
+    def set_timing(self, on):
+        _check(lib().aprh_context_set_timing(self.h, int(on)))
+
+    def stage_times(self):
+        ms = (C.c_float * 7)()
+        _check(lib().aprh_context_stage_times(self.h, ms))
+        return dict(zip(STAGES, list(ms)))
+
+    def release_cache(self):
+        _check(lib().aprh_context_release_cache(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().aprh_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class APR:
+    """Device-resident APR: GenInfo + LinearAccess (APR.hpp:18-61)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx, self.h = ctx, handle
+        info = AprhAprInfo()
+        _check(lib().aprh_apr_info_get(handle, C.byref(info)))
+        self.l_min, self.l_max = info.l_min, info.l_max
+        self.org_dims = tuple(info.org_dims)          # (y, x, z)
+        self.total = info.total_number_particles
+        self.xz_end_size = info.xz_end_size
+        self.level_xz_size = info.level_xz_size
+        self.bspline_offset = info.bspline_offset
+        self._struct = None
+
+    def total_number_particles(self):
+        return self.total
+
+    def level_min(self):
+        return self.l_min
+
+    def level_max(self):
+        return self.l_max
+
+    def structure(self):
+        """(level_xz_vec u64, xz_end_vec u64, y_vec u16) copied to the host"""
+        if self._struct is None:
+            lxz = np.zeros(self.level_xz_size, np.uint64)
+            xze = np.zeros(self.xz_end_size, np.uint64)
+            yv = np.zeros(max(self.total, 1), np.uint16)
+            _check(lib().aprh_apr_copy_structure(self.h, _ptr(lxz), _ptr(xze), _ptr(yv)))
+            self._struct = (lxz, xze, yv[:self.total])
+        return self._struct
+
+    def device_structure(self):
+        xze, yv = vp(), vp()
+        _check(lib().aprh_apr_device_structure(self.h, C.byref(xze), C.byref(yv)))
+        return xze.value, yv.value
+
+    def intermediate(self, which):
+        y, x, z = self.org_dims
+        yd, xd, zd = (y + 1) // 2, (x + 1) // 2, (z + 1) // 2
+        shapes = {0: ((z, x, y), np.uint16), 1: ((zd, xd, yd), np.uint16), 2: ((zd, xd, yd), np.float32),
+                  3: ((zd, xd, yd), np.uint16), 4: ((zd, xd, yd), np.float32), 5: ((zd, xd, yd), np.float32)}
+        shp, dt = shapes[which]
+        out = np.zeros(shp, dt)
+        _check(lib().aprh_apr_intermediate(self.h, which, _ptr(out)))
+        return out
+
+    def particle_cell_tree(self, level):
+        dims = (C.c_int64 * 3)()
+        _check(lib().aprh_apr_particle_cell_tree(self.h, level, None, dims))
+        y, x, z = dims
+        out = np.zeros((z, x, y), np.uint8)
+        _check(lib().aprh_apr_particle_cell_tree(self.h, level, _ptr(out), dims))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().aprh_apr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class APRConverter:
+    """APRConverter<uint16_t> (APRConverter.hpp:34-134)."""
+
+    def __init__(self, ctx=None):
+        self.par = APRParameters()
+        self.ctx = ctx or default_context()
+
+    def get_apr(self, image, dims=None):
+        """image: numpy uint16 [z, x, y] or a device address with dims=(y, x, z)"""
+        if dims is None:
+            z, x, y = image.shape
+            if image.dtype != np.uint16:
+                raise TypeError("APRConverter<uint16_t> needs a uint16 image")
+        else:
+            y, x, z = dims
+        h = vp()
+        p = self.par.c()
+        _check(lib().aprh_get_apr_u16(self.ctx.h, _ptr(image), y, x, z, C.byref(p), C.byref(h)))
+        return APR(self.ctx, h)
+
+    def get_apr_custom_grad_scale(self, grad_ds, lis_ds, intensity_ds, dims, bspline_offset=0.0):
+        y, x, z = dims
+        h = vp()
+        p = self.par.c()
+        _check(lib().aprh_apr_from_grad_lis(self.ctx.h, _ptr(np.ascontiguousarray(grad_ds, np.uint16)),
+                                            _ptr(np.ascontiguousarray(lis_ds, np.float32)),
+                                            _ptr(np.ascontiguousarray(intensity_ds, np.float32)), y, x, z, C.byref(p),
+                                            bspline_offset, C.byref(h)))
+        return APR(self.ctx, h)
+
+
+class ParticleData:
+    """ParticleData<uint16_t>::sample_image (ParticleData.hpp:122-127)."""
+
+    def __init__(self):
+        self.data = None
+
+    def sample_image(self, apr, image, out=None):
+        if out is None:
+            out = np.zeros(max(apr.total, 1), np.uint16)
+        _check(lib().aprh_sample_image_u16(apr.ctx.h, apr.h, _ptr(image), _ptr(out)))
+        self.data = out[:apr.total] if isinstance(out, np.ndarray) else out
+        return self.data
+
+
+# ---------------- stage-level entry points ----------------
+
+def bspline_filt_rec(image, axis, lam=3.0, tol=1e-4, ctx=None):
+    """in place on a numpy [z, x, y] array (u16 or f32)"""
+    ctx = ctx or default_context()
+    z, x, y = image.shape
+    f = lib().aprh_bspline_filt_rec_u16 if image.dtype == np.uint16 else lib().aprh_bspline_filt_rec_f32
+    _check(f(ctx.h, _ptr(image), y, x, z, axis, lam, tol))
+    return image
+
+
+def inv_bspline(image, axis, ctx=None):
+    ctx = ctx or default_context()
+    z, x, y = image.shape
+    _check(lib().aprh_inv_bspline_f32(ctx.h, _ptr(image), y, x, z, axis))
+    return image
+
+
+def gradient_ds(image, h=(1.0, 1.0, 1.0), ctx=None, with_mean=True):
+    ctx = ctx or default_context()
+    z, x, y = image.shape
+    g = np.zeros(((z + 1) // 2, (x + 1) // 2, (y + 1) // 2), np.uint16)
+    m = np.zeros(g.shape, np.float32) if with_mean else None
+    _check(lib().aprh_gradient_ds_u16(ctx.h, _ptr(image), _ptr(g), _ptr(m) if m is not None else None, y, x, z,
+                                      h[0], h[1], h[2]))
+    return g, m
+
+
+def sat_mean(image, axis, offset, ctx=None):
+    ctx = ctx or default_context()
+    z, x, y = image.shape
+    _check(lib().aprh_sat_mean_f32(ctx.h, _ptr(image), y, x, z, axis, offset))
+    return image
+
+
+def local_intensity_scale(intensity, par=None, ctx=None):
+    ctx = ctx or default_context()
+    par = par or APRParameters()
+    z, x, y = intensity.shape
+    out = np.zeros_like(intensity, dtype=np.float32)
+    p = par.c()
+    _check(lib().aprh_local_intensity_scale(ctx.h, _ptr(np.ascontiguousarray(intensity, np.float32)), _ptr(out), y, x,
+                                            z, C.byref(p)))
+    return out
+
+
+def synth_blobs(shape_zxy, seed, n_blobs, sigma=(2.0, 6.0), amp=(500.0, 3000.0), background=100.0, noise=True,
+                out=None, ctx=None):
+    """Synthetic uint16 blob volume; host (numpy) or device (out = device address)."""
+    z, x, y = shape_zxy
+    if out is None or isinstance(out, np.ndarray):
+        if out is None:
+            out = np.zeros((z, x, y), np.uint16)
+        _check(lib().aprh_synth_blobs_host(_ptr(out), y, x, z, seed, n_blobs, sigma[0], sigma[1], amp[0], amp[1],
+                                           background, int(noise)))
+        return out
+    ctx = ctx or default_context()
+    _check(lib().aprh_synth_blobs_device(ctx.h, _ptr(out), y, x, z, seed, n_blobs, sigma[0], sigma[1], amp[0], amp[1],
+                                         background, int(noise)))
+    return out

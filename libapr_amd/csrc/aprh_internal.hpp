@@ -1,0 +1,221 @@
+/*
+ * aprh_internal.hpp -- shared internals of libaprh (HIP, gfx950).
+ *
+ * Error handling: every failure throws aprh::Error carrying an aprh_status
+ * code; the extern "C" layer catches it, stores the message for
+ * aprh_last_error() and returns the code.
+ *
+ * Memory: each context owns a caching device allocator (size-bucketed free
+ * lists) so repeated calls on same-sized volumes perform no hipMalloc.
+ */
+#ifndef APRH_INTERNAL_HPP
+#define APRH_INTERNAL_HPP
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/aprh.h"
+
+namespace aprh {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void throw_hip(hipError_t e, const char* what, const char* file, int line);
+
+#define HIP_CHECK(x)                                                   \
+    do {                                                               \
+        hipError_t e_ = (x);                                           \
+        if (e_ != hipSuccess) ::aprh::throw_hip(e_, #x, __FILE__, __LINE__); \
+    } while (0)
+
+#define HIP_LAUNCH_CHECK() HIP_CHECK(hipGetLastError())
+
+/* size-bucketed caching allocator (per context / device) */
+class Pool {
+public:
+    explicit Pool(int device) : device_(device) {}
+    ~Pool();
+    void* get(size_t bytes);
+    void put(void* p);
+    void release_all();
+    size_t bytes_cached() const { return cached_; }
+
+private:
+    static size_t bucket(size_t b);
+    int device_;
+    std::multimap<size_t, void*> free_;
+    std::map<void*, size_t> live_;
+    size_t cached_ = 0;
+};
+
+/* device buffer leased from a Pool; returned on destruction */
+class Buf {
+public:
+    Buf() {}
+    Buf(Pool* pool, size_t bytes) { reset(pool, bytes); }
+    ~Buf() { release(); }
+    Buf(const Buf&) = delete;
+    Buf& operator=(const Buf&) = delete;
+    Buf(Buf&& o) noexcept { *this = std::move(o); }
+    Buf& operator=(Buf&& o) noexcept {
+        if (this != &o) {
+            release();
+            pool_ = o.pool_; p_ = o.p_; bytes_ = o.bytes_;
+            o.pool_ = nullptr; o.p_ = nullptr; o.bytes_ = 0;
+        }
+        return *this;
+    }
+    void reset(Pool* pool, size_t bytes) {
+        release();
+        pool_ = pool;
+        bytes_ = bytes;
+        p_ = bytes ? pool->get(bytes) : nullptr;
+    }
+    void release() {
+        if (p_ && pool_) pool_->put(p_);
+        p_ = nullptr; bytes_ = 0;
+    }
+    template <typename T> T* as() const { return (T*)p_; }
+    void* ptr() const { return p_; }
+    size_t bytes() const { return bytes_; }
+
+private:
+    Pool* pool_ = nullptr;
+    void* p_ = nullptr;
+    size_t bytes_ = 0;
+};
+
+/* timings of the conversion stages, named after APRConverter's timers */
+enum Stage { ST_OFFSET = 0, ST_SMOOTH, ST_GRADIENT, ST_LIS, ST_SOLVE, ST_STRUCTURE, ST_SAMPLE, ST_COUNT };
+
+}  // namespace aprh
+
+struct aprh_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    aprh::Pool* pool = nullptr;
+    int keep_intermediates = 0;
+    int timing = 0;
+    hipEvent_t ev[aprh::ST_COUNT + 1] = {};
+    float stage_ms[aprh::ST_COUNT] = {};
+    int* d_flags = nullptr;   /* small device scratch: overflow counters etc. */
+};
+
+/* device-resident APR: GenInfo + LinearAccess (+ optional intermediates) */
+struct aprh_apr {
+    aprh_context* ctx = nullptr;
+    int l_min = 1, l_max = 1;
+    int64_t org[3] = {0, 0, 0};
+    std::vector<int64_t> y_num, x_num, z_num;      /* per level, index 0..l_max */
+    std::vector<uint64_t> level_xz_vec;            /* host copy (l_max+2)       */
+    uint64_t total = 0;
+    uint64_t n_xz = 0;                             /* size of xz_end_vec        */
+    float bspline_offset = 0;
+    aprh::Buf xz_end;                              /* u64[n_xz]                 */
+    aprh::Buf y_vec;                               /* u16[total]                */
+    /* optional stage outputs (context keep_intermediates) */
+    aprh::Buf smoothed, grad_raw, lis_raw, grad, lis, levels;
+    std::vector<aprh::Buf> pct;                    /* final particle-cell tree  */
+    std::vector<int64_t> pct_dims;                 /* 3 per pct level           */
+};
+
+namespace aprh {
+
+/* level geometry of GenInfo::init (GenInfo.hpp:33-65) */
+void geninfo_init(aprh_apr& a, int64_t y, int64_t x, int64_t z);
+
+/* does ptr refer to device memory? */
+bool is_device_ptr(const void* p);
+
+/* copy helpers that accept host or device pointers on either side */
+void copy_any(aprh_context* ctx, void* dst, const void* src, size_t bytes);
+
+/* B-spline coefficients computed on the host (ComputeGradient.hpp:256-330) */
+struct IIRCoeffs {
+    float b1, b2, nf;
+    int mL;
+    float bc1[64], bc2[64], bc3[64], bc4[64];
+};
+IIRCoeffs iir_coeffs(float lambda, float tol, int64_t len);
+
+/* ---- stage launchers (k_iir.hip) ---- */
+void launch_minmax_offset(aprh_context* ctx, const uint16_t* img, size_t n, float lambda, float* d_offset, uint32_t* d_minmax);
+void launch_offset_copy(aprh_context* ctx, const uint16_t* in, uint16_t* out, size_t n, const float* d_offset);
+template <typename T>
+void launch_iir_y(aprh_context* ctx, const T* in, T* out, int64_t y, int64_t x, int64_t z, const IIRCoeffs& c,
+                  const float* d_offset, int* d_err);
+template <typename T>
+void launch_iir_outer(aprh_context* ctx, T* img, int64_t y, int64_t x, int64_t z, int axis, const IIRCoeffs& c, int* d_err);
+
+/* ---- gradient / inverse B-spline / LIS (k_grad_lis.hip) ---- */
+void launch_grad_ds_u16(aprh_context* ctx, const uint16_t* img, uint16_t* grad, float* mean, int64_t y, int64_t x, int64_t z,
+                        float hx, float hy, float hz);
+void launch_inv_bspline(aprh_context* ctx, float* m, float* tmp, int64_t y, int64_t x, int64_t z, bool ay, bool ax, bool az);
+struct LisParams {
+    int win[6];
+    bool active[3];
+    float var_rescale;
+    int pad[3];
+};
+LisParams lis_params(const aprh_params& par, int64_t yd, int64_t xd, int64_t zd);
+/* full local intensity scale over the reflect-padded array A (see lis_padded_elems) */
+void launch_lis(aprh_context* ctx, const float* intensity, float* A, float* unused, int64_t yd, int64_t xd, int64_t zd,
+                const LisParams& lp);
+size_t lis_padded_elems(int64_t yd, int64_t xd, int64_t zd, const LisParams& lp);
+void launch_unpad_rescale(aprh_context* ctx, const float* A, const LisParams& lp, int64_t yd, int64_t xd, int64_t zd, float* out);
+/* constant-scale branch of get_local_intensity_scale, on the device */
+void launch_constant_scale(aprh_context* ctx, float* m, size_t n);
+void launch_u8_to_f32(aprh_context* ctx, const uint8_t* in, float* out, size_t n);
+void launch_sat_mean(aprh_context* ctx, float* m, int64_t y, int64_t x, int64_t z, int axis, int off);
+/* rescale + unpad of A, applyParameters and computeLevels (level map u8);
+   optional outputs: sigma_raw (local_intensity_scale_step), sigma_out (after sigma_th), grad_keep */
+void launch_levels_padded(aprh_context* ctx, const float* A, const LisParams& lp, int64_t yd, int64_t xd, int64_t zd,
+                          const float* inten, uint16_t* grad, float* sigma_raw, float* sigma_out, uint8_t* level,
+                          const aprh_params& par, const float* d_offset, float mult, uint16_t* grad_keep);
+
+/* ---- pulling scheme / structure / sampling (k_structure.hip) ---- */
+void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, const aprh_params& par);
+void launch_sample_u16(aprh_context* ctx, const aprh_apr& a, const uint16_t* img, uint16_t* parts);
+
+/* ---- synthetic data (k_synth.hip) ---- */
+void synth_device(aprh_context* ctx, uint16_t* out, int64_t y, int64_t x, int64_t z, uint64_t seed, int64_t n_blobs, float smin,
+                  float smax, float amin, float amax, float background, int noise);
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+void set_last_error(const std::string& s);
+
+}  // namespace aprh
+
+/* exception -> status translation used by every extern "C" entry point */
+#define APRH_TRY try {
+#define APRH_CATCH                                              \
+    }                                                           \
+    catch (aprh::Error & e) {                                   \
+        aprh::set_last_error(e.what());                         \
+        return e.code;                                          \
+    }                                                           \
+    catch (std::bad_alloc&) {                                   \
+        aprh::set_last_error("host allocation failed");         \
+        return APRH_ERR_OUT_OF_MEMORY;                          \
+    }                                                           \
+    catch (std::exception & e) {                                \
+        aprh::set_last_error(e.what());                         \
+        return APRH_ERR_HIP;                                    \
+    }                                                           \
+    return APRH_OK;
+
+namespace aprh {
+
+}  // namespace aprh
+
+#endif

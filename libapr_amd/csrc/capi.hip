@@ -1,0 +1,463 @@
+/*
+ * capi.hip -- extern "C" entry points of libaprh: the get_apr pipeline
+ * (APRConverter<uint16_t>::get_apr, APRConverter.hpp:534-582), sampling, the
+ * stage-level entry points and the APR accessors.
+ */
+#include <algorithm>
+#include <cmath>
+
+#include "aprh_internal.hpp"
+
+using namespace aprh;
+
+namespace {
+
+/* device copy of a caller buffer (no copy when it already is device memory) */
+struct DevIn {
+    const void* ptr = nullptr;
+    Buf own;
+    DevIn(aprh_context* ctx, const void* p, size_t bytes) {
+        if (is_device_ptr(p)) {
+            ptr = p;
+        } else {
+            own.reset(ctx->pool, bytes);
+            HIP_CHECK(hipMemcpyAsync(own.ptr(), p, bytes, hipMemcpyHostToDevice, ctx->stream));
+            ptr = own.ptr();
+        }
+    }
+};
+
+/* device output that is copied to the caller's host buffer on finish() */
+struct DevOut {
+    void* ptr = nullptr;
+    void* host = nullptr;
+    size_t bytes = 0;
+    Buf own;
+    DevOut(aprh_context* ctx, void* p, size_t b) : bytes(b) {
+        if (is_device_ptr(p)) {
+            ptr = p;
+        } else {
+            own.reset(ctx->pool, b);
+            ptr = own.ptr();
+            host = p;
+        }
+    }
+    void finish(aprh_context* ctx) {
+        if (host) HIP_CHECK(hipMemcpyAsync(host, ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    }
+};
+
+void check_ctx(aprh_context* ctx) {
+    if (!ctx) throw Error(APRH_ERR_INVALID_ARGUMENT, "context is NULL");
+    HIP_CHECK(hipSetDevice(ctx->device));
+}
+
+void check_dims(int64_t y, int64_t x, int64_t z) {
+    if (y <= 0 || x <= 0 || z <= 0) throw Error(APRH_ERR_BAD_DIMS, "dimensions must be positive");
+    if (y > 65535 || x > 65535 || z > 65535) throw Error(APRH_ERR_UNSUPPORTED, "dimensions above 65535 (u16 y_vec / iterator limits)");
+}
+
+struct StageTimer {
+    aprh_context* ctx;
+    explicit StageTimer(aprh_context* c) : ctx(c) {
+        for (int i = 0; i < ST_COUNT; ++i) ctx->stage_ms[i] = 0.0f;
+    }
+    void mark(int i) {
+        if (ctx->timing) HIP_CHECK(hipEventRecord(ctx->ev[i], ctx->stream));
+    }
+    void collect(int first, int last) {
+        if (!ctx->timing) return;
+        HIP_CHECK(hipEventSynchronize(ctx->ev[last]));
+        for (int i = first; i < last; ++i) HIP_CHECK(hipEventElapsedTime(&ctx->stage_ms[i], ctx->ev[i], ctx->ev[i + 1]));
+    }
+};
+
+float level_mult(const aprh_params& par, int l_max) {
+    /* computeLevels (LocalParticleCellSet.hpp:128-147) */
+    const float min_dim = std::min(par.dy, std::min(par.dx, par.dz));
+    const float level_factor = (float)(std::pow(2.0, (double)l_max) * (double)min_dim);
+    return level_factor / par.rel_error;
+}
+
+void keep_copy(aprh_context* ctx, Buf& dst, const void* src, size_t bytes) {
+    dst.reset(ctx->pool, bytes);
+    HIP_CHECK(hipMemcpyAsync(dst.ptr(), src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+}
+
+/* levels + pulling scheme + structure from a downsampled gradient, an unpadded
+   scale (A with zero pads, var_rescale 1) or the padded LIS array */
+void solve(aprh_context* ctx, aprh_apr& a, const aprh_params& par, const float* A, const LisParams& lp, const float* inten,
+           uint16_t* grad, const float* d_off, StageTimer& tm) {
+    const int64_t yd = cdiv(a.org[0], 2), xd = cdiv(a.org[1], 2), zd = cdiv(a.org[2], 2);
+    const size_t nd = (size_t)(yd * xd * zd);
+    Buf level(ctx->pool, nd);
+    const bool keep = ctx->keep_intermediates != 0;
+    float* sig_raw = nullptr;
+    float* sig = nullptr;
+    uint16_t* gk = nullptr;
+    if (keep) {
+        a.lis_raw.reset(ctx->pool, nd * 4);
+        a.lis.reset(ctx->pool, nd * 4);
+        a.grad.reset(ctx->pool, nd * 2);
+        sig_raw = a.lis_raw.as<float>();
+        sig = a.lis.as<float>();
+        gk = a.grad.as<uint16_t>();
+    }
+    launch_levels_padded(ctx, A, lp, yd, xd, zd, inten, grad, sig_raw, sig, level.as<uint8_t>(), par, d_off, level_mult(par, a.l_max),
+                         gk);
+    if (keep) {
+        /* levels as float, like local_scale_temp after computeLevels */
+        a.levels.reset(ctx->pool, nd * 4);
+        launch_u8_to_f32(ctx, level.as<uint8_t>(), a.levels.as<float>(), nd);
+    }
+    tm.mark(ST_SOLVE);
+    build_structure(ctx, a, level.as<uint8_t>(), par);
+    tm.mark(ST_STRUCTURE);
+}
+
+void get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x, int64_t z, const aprh_params& par, aprh_apr* a) {
+    StageTimer tm(ctx);
+    const size_t n = (size_t)(y * x * z);
+    const int64_t yd = cdiv(y, 2), xd = cdiv(x, 2), zd = cdiv(z, 2);
+    const size_t nd = (size_t)(yd * xd * zd);
+    hipStream_t st = ctx->stream;
+    DevIn in(ctx, image, n * 2);
+    geninfo_init(*a, y, x, z);
+    tm.mark(ST_OFFSET);
+    float* d_off = (float*)ctx->d_flags;
+    uint32_t* d_mm = (uint32_t*)ctx->d_flags + 2;
+    int* d_err = ctx->d_flags + 4;
+    HIP_CHECK(hipMemsetAsync(d_err, 0, 3 * sizeof(int), st));
+    /* computeL: bspline offset (APRConverter.hpp:213-224) */
+    launch_minmax_offset(ctx, (const uint16_t*)in.ptr, n, par.lambda, d_off, d_mm);
+    tm.mark(ST_SMOOTH);
+    /* smoothing (ComputeGradient.hpp:173-205) with the offset copy fused into the first pass */
+    Buf img(ctx->pool, n * 2);
+    uint16_t* I = img.as<uint16_t>();
+    const float tol = 0.0001f;
+    if (par.lambda > 0 && y > 2) {
+        launch_iir_y<uint16_t>(ctx, (const uint16_t*)in.ptr, I, y, x, z, iir_coeffs(par.lambda, tol, y), d_off, d_err + 0);
+    } else {
+        launch_offset_copy(ctx, (const uint16_t*)in.ptr, I, n, d_off);
+    }
+    if (par.lambda > 0 && x > 2) launch_iir_outer<uint16_t>(ctx, I, y, x, z, 1, iir_coeffs(par.lambda, tol, x), d_err + 1);
+    if (par.lambda > 0 && z > 2) launch_iir_outer<uint16_t>(ctx, I, y, x, z, 2, iir_coeffs(par.lambda, tol, z), d_err + 2);
+    tm.mark(ST_GRADIENT);
+    /* gradient magnitude + 2x2x2 max (ComputeGradient.hpp:92-93), mean downsample and inverse B-spline (:95-118) */
+    Buf grad(ctx->pool, nd * 2);
+    Buf mean(ctx->pool, nd * 4);
+    HIP_CHECK(hipMemsetAsync(grad.ptr(), 0, nd * 2, st));
+    launch_grad_ds_u16(ctx, I, grad.as<uint16_t>(), mean.as<float>(), y, x, z, par.dx, par.dy, par.dz);
+    int herr[3] = {0, 0, 0};
+    HIP_CHECK(hipMemcpyAsync(herr, d_err, sizeof(herr), hipMemcpyDeviceToHost, st));
+    if (ctx->keep_intermediates) {
+        keep_copy(ctx, a->smoothed, I, n * 2);
+        keep_copy(ctx, a->grad_raw, grad.ptr(), nd * 2);
+    }
+    img.release();
+    if (par.lambda > 0) {
+        Buf tmp(ctx->pool, nd * 4);
+        launch_inv_bspline(ctx, mean.as<float>(), tmp.as<float>(), yd, xd, zd, y > 2, x > 2, z > 2);
+    }
+    HIP_CHECK(hipStreamSynchronize(st));
+    static const char* axis_name[3] = {"y", "x", "z"};
+    for (int i = 0; i < 3; ++i)
+        if (herr[i])
+            throw Error(APRH_ERR_OVERFLOW, std::string("integer under-/overflow encountered in ComputeGradient::bspline_filt_rec_") +
+                                               axis_name[i] +
+                                               " - try squashing the input image to a narrower range or use APRConverter<float>");
+    tm.mark(ST_LIS);
+    /* local intensity scale (LocalIntensityScale.hpp:26-150) */
+    LisParams lp = lis_params(par, yd, xd, zd);
+    const int nact = (int)lp.active[0] + (int)lp.active[1] + (int)lp.active[2];
+    Buf A;
+    const float* Ap;
+    LisParams lpa = lp;
+    if (par.constant_intensity_scale || nact == 0) {
+        A.reset(ctx->pool, nd * 4);
+        HIP_CHECK(hipMemcpyAsync(A.ptr(), mean.ptr(), nd * 4, hipMemcpyDeviceToDevice, st));
+        launch_constant_scale(ctx, A.as<float>(), nd);
+        lpa.pad[0] = lpa.pad[1] = lpa.pad[2] = 0;
+        lpa.var_rescale = 1.0f;
+        Ap = A.as<float>();
+    } else {
+        A.reset(ctx->pool, lis_padded_elems(yd, xd, zd, lp) * 4);
+        launch_lis(ctx, mean.as<float>(), A.as<float>(), nullptr, yd, xd, zd, lp);
+        Ap = A.as<float>();
+    }
+    /* applyParameters + solveForAPR + generateDatastructures */
+    solve(ctx, *a, par, Ap, lpa, mean.as<float>(), grad.as<uint16_t>(), d_off, tm);
+    HIP_CHECK(hipMemcpyAsync(&a->bspline_offset, d_off, 4, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    tm.collect(ST_OFFSET, ST_STRUCTURE);
+}
+
+}  // namespace
+
+extern "C" {
+
+int aprh_get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x, int64_t z, const aprh_params* par,
+                     aprh_apr** out) {
+    APRH_TRY
+    check_ctx(ctx);
+    if (!image || !par || !out) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    check_dims(y, x, z);
+    aprh_apr* a = new aprh_apr();
+    a->ctx = ctx;
+    try {
+        get_apr_u16(ctx, image, y, x, z, *par, a);
+    } catch (...) {
+        (void)hipStreamSynchronize(ctx->stream);
+        delete a;
+        throw;
+    }
+    *out = a;
+    APRH_CATCH
+}
+
+int aprh_apr_from_grad_lis(aprh_context* ctx, const uint16_t* grad_ds, const float* lis_ds, const float* intensity_ds, int64_t y,
+                           int64_t x, int64_t z, const aprh_params* par, float bspline_offset, aprh_apr** out) {
+    APRH_TRY
+    check_ctx(ctx);
+    if (!grad_ds || !lis_ds || !intensity_ds || !par || !out) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    check_dims(y, x, z);
+    const size_t nd = (size_t)(cdiv(y, 2) * cdiv(x, 2) * cdiv(z, 2));
+    aprh_apr* a = new aprh_apr();
+    a->ctx = ctx;
+    try {
+        geninfo_init(*a, y, x, z);
+        a->bspline_offset = bspline_offset;
+        DevIn g(ctx, grad_ds, nd * 2), l(ctx, lis_ds, nd * 4), it(ctx, intensity_ds, nd * 4);
+        Buf gcopy(ctx->pool, nd * 2);
+        HIP_CHECK(hipMemcpyAsync(gcopy.ptr(), g.ptr, nd * 2, hipMemcpyDeviceToDevice, ctx->stream));
+        float* d_off = (float*)ctx->d_flags;
+        HIP_CHECK(hipMemcpyAsync(d_off, &bspline_offset, 4, hipMemcpyHostToDevice, ctx->stream));
+        LisParams lp{};
+        lp.var_rescale = 1.0f;
+        StageTimer tm(ctx);
+        solve(ctx, *a, *par, (const float*)l.ptr, lp, (const float*)it.ptr, gcopy.as<uint16_t>(), d_off, tm);
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+        (void)hipStreamSynchronize(ctx->stream);
+        delete a;
+        throw;
+    }
+    *out = a;
+    APRH_CATCH
+}
+
+int aprh_apr_info_get(const aprh_apr* a, aprh_apr_info* info) {
+    APRH_TRY
+    if (!a || !info) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    info->l_min = a->l_min;
+    info->l_max = a->l_max;
+    for (int i = 0; i < 3; ++i) info->org_dims[i] = a->org[i];
+    info->total_number_particles = a->total;
+    info->xz_end_size = a->n_xz;
+    info->level_xz_size = a->level_xz_vec.size();
+    info->bspline_offset = a->bspline_offset;
+    APRH_CATCH
+}
+
+int aprh_apr_copy_structure(const aprh_apr* a, uint64_t* level_xz_vec, uint64_t* xz_end_vec, uint16_t* y_vec) {
+    APRH_TRY
+    if (!a) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL apr");
+    check_ctx(a->ctx);
+    if (level_xz_vec) std::copy(a->level_xz_vec.begin(), a->level_xz_vec.end(), level_xz_vec);
+    if (xz_end_vec) copy_any(a->ctx, xz_end_vec, a->xz_end.ptr(), a->n_xz * 8);
+    if (y_vec) copy_any(a->ctx, y_vec, a->y_vec.ptr(), a->total * 2);
+    HIP_CHECK(hipStreamSynchronize(a->ctx->stream));
+    APRH_CATCH
+}
+
+int aprh_apr_device_structure(const aprh_apr* a, const uint64_t** xz_end_vec, const uint16_t** y_vec) {
+    APRH_TRY
+    if (!a) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL apr");
+    if (xz_end_vec) *xz_end_vec = a->xz_end.as<uint64_t>();
+    if (y_vec) *y_vec = a->y_vec.as<uint16_t>();
+    APRH_CATCH
+}
+
+int aprh_apr_intermediate(const aprh_apr* a, int which, void* dst) {
+    APRH_TRY
+    if (!a || !dst) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    check_ctx(a->ctx);
+    const aprh::Buf* b = nullptr;
+    switch (which) {
+        case 0: b = &a->smoothed; break;
+        case 1: b = &a->grad_raw; break;
+        case 2: b = &a->lis_raw; break;
+        case 3: b = &a->grad; break;
+        case 4: b = &a->lis; break;
+        case 5: b = &a->levels; break;
+        default: throw Error(APRH_ERR_INVALID_ARGUMENT, "unknown intermediate");
+    }
+    if (!b->ptr()) throw Error(APRH_ERR_INVALID_ARGUMENT, "intermediate not kept (aprh_context_set_keep_intermediates)");
+    copy_any(a->ctx, dst, b->ptr(), b->bytes());
+    HIP_CHECK(hipStreamSynchronize(a->ctx->stream));
+    APRH_CATCH
+}
+
+int aprh_apr_particle_cell_tree(const aprh_apr* a, int level, uint8_t* dst, int64_t* dims3) {
+    APRH_TRY
+    if (!a) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL apr");
+    check_ctx(a->ctx);
+    if (level < 0 || level >= (int)a->pct.size() || !a->pct[level].ptr())
+        throw Error(APRH_ERR_INVALID_ARGUMENT, "particle-cell tree level not kept");
+    if (dims3)
+        for (int i = 0; i < 3; ++i) dims3[i] = a->pct_dims[3 * level + i];
+    if (dst) {
+        copy_any(a->ctx, dst, a->pct[level].ptr(),
+                 (size_t)(a->pct_dims[3 * level] * a->pct_dims[3 * level + 1] * a->pct_dims[3 * level + 2]));
+        HIP_CHECK(hipStreamSynchronize(a->ctx->stream));
+    }
+    APRH_CATCH
+}
+
+int aprh_apr_destroy(aprh_apr* a) {
+    APRH_TRY
+    if (!a) return APRH_OK;
+    if (a->ctx) {
+        (void)hipSetDevice(a->ctx->device);
+        (void)hipStreamSynchronize(a->ctx->stream);
+    }
+    delete a;
+    APRH_CATCH
+}
+
+int aprh_sample_image_u16(aprh_context* ctx, const aprh_apr* a, const uint16_t* image, uint16_t* parts) {
+    APRH_TRY
+    check_ctx(ctx);
+    if (!a || !image || !parts) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    const size_t n = (size_t)(a->org[0] * a->org[1] * a->org[2]);
+    StageTimer tm(ctx);
+    tm.mark(ST_SAMPLE);
+    DevIn in(ctx, image, n * 2);
+    DevOut out(ctx, parts, std::max<uint64_t>(a->total, 1) * 2);
+    launch_sample_u16(ctx, *a, (const uint16_t*)in.ptr, (uint16_t*)out.ptr);
+    tm.mark(ST_COUNT);
+    out.finish(ctx);
+    tm.collect(ST_SAMPLE, ST_COUNT);
+    APRH_CATCH
+}
+
+/* ---------------- stage-level entry points ---------------- */
+
+int aprh_bspline_filt_rec_u16(aprh_context* ctx, uint16_t* image, int64_t y, int64_t x, int64_t z, int axis, float lambda,
+                              float tol) {
+    APRH_TRY
+    check_ctx(ctx);
+    if (!image) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL image");
+    check_dims(y, x, z);
+    const int64_t len = axis == 0 ? y : axis == 1 ? x : z;
+    if (len < 2) throw Error(APRH_ERR_BAD_DIMS, "filtered axis must have at least 2 samples");
+    const size_t n = (size_t)(y * x * z);
+    DevOut io(ctx, image, n * 2);
+    if (io.host) HIP_CHECK(hipMemcpyAsync(io.ptr, image, n * 2, hipMemcpyHostToDevice, ctx->stream));
+    int* d_err = ctx->d_flags + 4;
+    HIP_CHECK(hipMemsetAsync(d_err, 0, 4, ctx->stream));
+    IIRCoeffs c = iir_coeffs(lambda, tol, len);
+    if (axis == 0) launch_iir_y<uint16_t>(ctx, (const uint16_t*)io.ptr, (uint16_t*)io.ptr, y, x, z, c, nullptr, d_err);
+    else launch_iir_outer<uint16_t>(ctx, (uint16_t*)io.ptr, y, x, z, axis, c, d_err);
+    int herr = 0;
+    HIP_CHECK(hipMemcpyAsync(&herr, d_err, 4, hipMemcpyDeviceToHost, ctx->stream));
+    io.finish(ctx);
+    if (herr) throw Error(APRH_ERR_OVERFLOW, "integer under-/overflow encountered in ComputeGradient::bspline_filt_rec");
+    APRH_CATCH
+}
+
+int aprh_bspline_filt_rec_f32(aprh_context* ctx, float* image, int64_t y, int64_t x, int64_t z, int axis, float lambda, float tol) {
+    APRH_TRY
+    check_ctx(ctx);
+    if (!image) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL image");
+    check_dims(y, x, z);
+    const int64_t len = axis == 0 ? y : axis == 1 ? x : z;
+    if (len < 2) throw Error(APRH_ERR_BAD_DIMS, "filtered axis must have at least 2 samples");
+    const size_t n = (size_t)(y * x * z);
+    DevOut io(ctx, image, n * 4);
+    if (io.host) HIP_CHECK(hipMemcpyAsync(io.ptr, image, n * 4, hipMemcpyHostToDevice, ctx->stream));
+    int* d_err = ctx->d_flags + 4;
+    IIRCoeffs c = iir_coeffs(lambda, tol, len);
+    if (axis == 0) launch_iir_y<float>(ctx, (const float*)io.ptr, (float*)io.ptr, y, x, z, c, nullptr, d_err);
+    else launch_iir_outer<float>(ctx, (float*)io.ptr, y, x, z, axis, c, d_err);
+    io.finish(ctx);
+    APRH_CATCH
+}
+
+int aprh_inv_bspline_f32(aprh_context* ctx, float* image, int64_t y, int64_t x, int64_t z, int axis) {
+    APRH_TRY
+    check_ctx(ctx);
+    if (!image) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL image");
+    check_dims(y, x, z);
+    const int64_t len = axis == 0 ? y : axis == 1 ? x : z;
+    if (len < 2) throw Error(APRH_ERR_BAD_DIMS, "filtered axis must have at least 2 samples");
+    const size_t n = (size_t)(y * x * z);
+    DevOut io(ctx, image, n * 4);
+    if (io.host) HIP_CHECK(hipMemcpyAsync(io.ptr, image, n * 4, hipMemcpyHostToDevice, ctx->stream));
+    Buf tmp(ctx->pool, n * 4);
+    launch_inv_bspline(ctx, (float*)io.ptr, tmp.as<float>(), y, x, z, axis == 0, axis == 1, axis == 2);
+    io.finish(ctx);
+    APRH_CATCH
+}
+
+int aprh_gradient_ds_u16(aprh_context* ctx, const uint16_t* image, uint16_t* grad_ds, float* mean_ds, int64_t y, int64_t x, int64_t z,
+                         float hx, float hy, float hz) {
+    APRH_TRY
+    check_ctx(ctx);
+    if (!image || !grad_ds) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    check_dims(y, x, z);
+    const size_t n = (size_t)(y * x * z);
+    const size_t nd = (size_t)(cdiv(y, 2) * cdiv(x, 2) * cdiv(z, 2));
+    DevIn in(ctx, image, n * 2);
+    DevOut g(ctx, grad_ds, nd * 2);
+    if (g.host) HIP_CHECK(hipMemcpyAsync(g.ptr, grad_ds, nd * 2, hipMemcpyHostToDevice, ctx->stream));
+    if (mean_ds) {
+        DevOut m(ctx, mean_ds, nd * 4);
+        launch_grad_ds_u16(ctx, (const uint16_t*)in.ptr, (uint16_t*)g.ptr, (float*)m.ptr, y, x, z, hx, hy, hz);
+        m.finish(ctx);
+    } else {
+        launch_grad_ds_u16(ctx, (const uint16_t*)in.ptr, (uint16_t*)g.ptr, nullptr, y, x, z, hx, hy, hz);
+    }
+    g.finish(ctx);
+    APRH_CATCH
+}
+
+int aprh_sat_mean_f32(aprh_context* ctx, float* image, int64_t y, int64_t x, int64_t z, int axis, int offset) {
+    APRH_TRY
+    check_ctx(ctx);
+    if (!image) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL image");
+    check_dims(y, x, z);
+    if (offset < 1 || offset > 6) throw Error(APRH_ERR_UNSUPPORTED, "offset must be 1..6");
+    const size_t n = (size_t)(y * x * z);
+    DevOut io(ctx, image, n * 4);
+    if (io.host) HIP_CHECK(hipMemcpyAsync(io.ptr, image, n * 4, hipMemcpyHostToDevice, ctx->stream));
+    launch_sat_mean(ctx, (float*)io.ptr, y, x, z, axis, offset);
+    io.finish(ctx);
+    APRH_CATCH
+}
+
+int aprh_local_intensity_scale(aprh_context* ctx, const float* intensity_ds, float* scale_ds, int64_t y, int64_t x, int64_t z,
+                               const aprh_params* par) {
+    APRH_TRY
+    check_ctx(ctx);
+    if (!intensity_ds || !scale_ds || !par) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    check_dims(y, x, z);
+    const size_t n = (size_t)(y * x * z);
+    DevIn in(ctx, intensity_ds, n * 4);
+    DevOut out(ctx, scale_ds, n * 4);
+    LisParams lp = lis_params(*par, y, x, z);
+    const int nact = (int)lp.active[0] + (int)lp.active[1] + (int)lp.active[2];
+    if (par->constant_intensity_scale || nact == 0) {
+        HIP_CHECK(hipMemcpyAsync(out.ptr, in.ptr, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+        launch_constant_scale(ctx, (float*)out.ptr, n);
+    } else {
+        Buf A(ctx->pool, lis_padded_elems(y, x, z, lp) * 4);
+        launch_lis(ctx, (const float*)in.ptr, A.as<float>(), nullptr, y, x, z, lp);
+        launch_unpad_rescale(ctx, A.as<float>(), lp, y, x, z, (float*)out.ptr);
+    }
+    out.finish(ctx);
+    APRH_CATCH
+}
+
+}  // extern "C"

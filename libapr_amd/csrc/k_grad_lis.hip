@@ -1,0 +1,513 @@
+/*
+ * k_grad_lis.hip -- downsampled gradient, inverse B-spline, local intensity
+ * scale and level computation of the conversion pipeline (gfx950).
+ *
+ *  - k_grad_ds: one lane per downsampled cell; gradient magnitude of its
+ *    2x2x2 voxels (ComputeGradient.hpp:975-1051, double squares/sqrt like the
+ *    reference) fused with the 2x2x2 mean downsample (ComputeGradient.hpp:95-98).
+ *  - inverse B-spline: one pass per axis, out of place (ComputeGradient.hpp:784-964).
+ *  - local intensity scale (LocalIntensityScale.hpp:26-494): the six
+ *    separable running-sum means over the reflect-padded array.  The y mean
+ *    is a per-line sequential prefix sum, so lines are staged through an LDS
+ *    tile (64 lines per wave) and each lane walks its line with a register
+ *    delay line of 2*offset+2 prefix sums; the x / z means run one lane per
+ *    y with the delay line along the outer axis.  Every float operation keeps
+ *    the reference's order and precision (float vs double sub-expressions).
+ *  - k_levels: applyParameters + computeLevels fused (APRConverter.hpp:249-297,
+ *    LocalParticleCellSet.hpp:128-171).
+ */
+#include <algorithm>
+#include <cmath>
+
+#include "aprh_internal.hpp"
+
+namespace aprh {
+
+namespace {
+
+__device__ __forceinline__ uint16_t d2u16(double v) {
+    if (!(v > -2147483648.0 && v < 2147483648.0)) return 0;
+    return (uint16_t)(int32_t)v;
+}
+
+__device__ __forceinline__ uint32_t f2u32(float v) {
+    if (!(v > -9.2233720368547758e18f && v < 9.2233720368547758e18f)) return 0u;
+    return (uint32_t)(int64_t)v;
+}
+
+__device__ __forceinline__ double sq_term(int d, float h2) {
+    const double q = (double)((float)d / h2);
+    return q * q;
+}
+
+__global__ __launch_bounds__(256) void k_grad_ds(const uint16_t* __restrict__ img, uint16_t* grad, float* mean, int64_t y_num,
+                                                 int64_t x_num, int64_t z_num, int64_t yd_num, float h2x, float h2y, float h2z) {
+    const int64_t yd = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (yd >= yd_num) return;
+    const int64_t xd = blockIdx.y, zd = blockIdx.z;
+    const int64_t xd_num = gridDim.y;
+    const int64_t xy = x_num * y_num;
+    uint16_t g = grad[(zd * xd_num + xd) * yd_num + yd];
+    const int64_t y0 = 2 * yd, y1 = min(2 * yd + 1, y_num - 1);
+    for (int64_t z = 2 * zd; z <= min(2 * zd + 1, z_num - 1); ++z) {
+        const int64_t zm = z > 0 ? z - 1 : 0, zp = min(z + 1, z_num - 1);
+        for (int64_t x = 2 * xd; x <= min(2 * xd + 1, x_num - 1); ++x) {
+            const int64_t xm = x > 0 ? x - 1 : 0, xp = min(x + 1, x_num - 1);
+            const uint16_t* cen = img + z * xy + x * y_num;
+            const uint16_t* left = img + z * xy + xm * y_num;
+            const uint16_t* right = img + z * xy + xp * y_num;
+            const uint16_t* up = img + zm * xy + x * y_num;
+            const uint16_t* down = img + zp * xy + x * y_num;
+            for (int w = 0; w < 2; ++w) {
+                const int64_t y = w ? y1 : y0;
+                uint16_t t;
+                if (y_num >= 2) {
+                    int dyv;
+                    if (y == 0) dyv = (int)cen[1] - (int)cen[0];
+                    else if (y == y_num - 1) dyv = (int)cen[y_num - 1] - (int)cen[y_num - 2];
+                    else dyv = (int)cen[y + 1] - (int)cen[y - 1];
+                    const double s = sq_term((int)right[y] - (int)left[y], h2x) + sq_term((int)down[y] - (int)up[y], h2z) +
+                                     sq_term(dyv, h2y);
+                    t = d2u16(sqrt(s));
+                } else {
+                    t = 0;
+                }
+                g = t > g ? t : g;
+            }
+        }
+    }
+    grad[(zd * xd_num + xd) * yd_num + yd] = g;
+    if (mean) {
+        /* downsample(sum, /8.0): clamped indices, fixed reduce order (PixelData.hpp:944-962) */
+        const int64_t shx = min(2 * xd + 1, x_num - 1), shz = min(2 * zd + 1, z_num - 1), shy = y1;
+        const int64_t z0 = 2 * zd, x0 = 2 * xd;
+        float s = (float)img[z0 * xy + x0 * y_num + y0];
+        s = s + (float)img[z0 * xy + x0 * y_num + shy];
+        s = s + (float)img[z0 * xy + shx * y_num + y0];
+        s = s + (float)img[z0 * xy + shx * y_num + shy];
+        s = s + (float)img[shz * xy + x0 * y_num + y0];
+        s = s + (float)img[shz * xy + x0 * y_num + shy];
+        s = s + (float)img[shz * xy + shx * y_num + y0];
+        s = s + (float)img[shz * xy + shx * y_num + shy];
+        mean[(zd * xd_num + xd) * yd_num + yd] = (float)((double)s / 8.0);
+    }
+}
+
+/* ---------------- inverse cubic B-spline, [1 4 1]/6 ---------------- */
+__global__ void k_inv_y(const float* __restrict__ in, float* out, int64_t y_num, size_t n) {
+    const float a1 = (float)(1.0 / 6.0), a2 = (float)(4.0 / 6.0), a3 = (float)(1.0 / 6.0);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int64_t k = (int64_t)(i % (size_t)y_num);
+        const float* l = in + (i - (size_t)k);
+        float v;
+        if (k == 0) {
+            v = a2 * l[0];
+            v += (a1 + a3) * l[1];
+        } else if (k == y_num - 1) {
+            v = (a1 + a3) * l[y_num - 2];
+            v += a2 * l[y_num - 1];
+        } else {
+            v = a1 * l[k - 1] + a2 * l[k] + a3 * l[k + 1];
+        }
+        out[i] = v;
+    }
+}
+
+/* x / z: out[0] = a1*in[1] + a2*in[0] + a3*in[1]; last = (a1+a3)*in[n-2] + a2*in[n-1] */
+__global__ void k_inv_outer(const float* __restrict__ in, float* out, int64_t n_axis, int64_t stride, size_t n) {
+    const float a1 = (float)(1.0 / 6.0), a2 = (float)(4.0 / 6.0), a3 = (float)(1.0 / 6.0);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int64_t j = (int64_t)((i / (size_t)stride) % (size_t)n_axis);
+        float v;
+        if (j == n_axis - 1) {
+            v = (a1 + a3) * in[i - stride];
+            v += a2 * in[i];
+        } else if (j == 0) {
+            v = a1 * in[i + stride] + a2 * in[i] + a3 * in[i + stride];
+        } else {
+            v = a1 * in[i - stride] + a2 * in[i] + a3 * in[i + stride];
+        }
+        out[i] = v;
+    }
+}
+
+/* ---------------- local intensity scale ---------------- */
+__device__ __forceinline__ int64_t refl(int64_t i, int64_t n) {
+    /* reflection about the edge sample (paddPixels, PixelData.hpp:1070-1176) */
+    if (i < 0) return -i;
+    if (i >= n) return 2 * (n - 1) - i;
+    return i;
+}
+
+struct PadGeom {
+    int64_t yd, xd, zd;       /* unpadded (downsampled) dims */
+    int64_t py, px, pz;       /* pads                         */
+    int64_t yp, xp, zp;       /* padded dims                  */
+};
+
+/* source sample of the y-mean pass: mode 0 = padded copy of the intensity,
+   mode 1 = |A - padded intensity| (calc_abs_diff, LocalIntensityScale.hpp:188-196) */
+__device__ __forceinline__ float ysrc(int mode, const float* __restrict__ inten, const float* A, const PadGeom& g, int64_t line,
+                                      int64_t yy) {
+    const int64_t xp = line % g.xp, zp = line / g.xp;
+    const float b = inten[(refl(zp - g.pz, g.zd) * g.xd + refl(xp - g.px, g.xd)) * g.yd + refl(yy - g.py, g.yd)];
+    if (mode == 0) return b;
+    return fabsf(A[line * g.yp + yy] - b);
+}
+
+/* y running-sum mean with offset O over padded lines (LocalIntensityScale.hpp:311-363);
+   O == 0 means "axis inactive": copy only.  Output written to A (may alias the source). */
+template <int O>
+__global__ __launch_bounds__(64) void k_mean_y(int mode, const float* __restrict__ inten, float* A, PadGeom g) {
+    constexpr int TY = 64;
+    constexpr int H = 2 * O + 2;
+    __shared__ float tin[64][TY + 1];
+    __shared__ float tout[64][TY + 1];
+    const int lane = threadIdx.x;
+    const int64_t n_lines = g.xp * g.zp;
+    const int64_t line0 = (int64_t)blockIdx.x * 64;
+    const int64_t me = line0 + lane;
+    const bool valid = me < n_lines;
+    const int64_t yn = g.yp;
+    const float d = (float)(2 * O + 1);
+    float h[H > 0 ? H : 1];
+    for (int i = 0; i < H; ++i) h[i] = 0.0f;
+    float t = 0.0f;
+    /* pre-consume samples [0, O) */
+    if (valid) {
+        for (int j = 0; j < O; ++j) {
+            t += ysrc(mode, inten, A, g, me, j);
+            for (int i = 0; i + 1 < H; ++i) h[i] = h[i + 1];
+            h[H - 1] = t;
+        }
+    }
+    __syncthreads();
+    for (int64_t p0 = 0; p0 < yn; p0 += TY) {
+        /* inputs for outputs [p0, p0+64) are samples [p0+O, p0+O+64) */
+        for (int l = 0; l < 64; ++l) {
+            const int64_t ln = line0 + l;
+            const int64_t yy = p0 + O + lane;
+            if (ln < n_lines && yy < yn) tin[l][lane] = ysrc(mode, inten, A, g, ln, yy);
+        }
+        __syncthreads();
+        if (valid) {
+            for (int k = 0; k < TY; ++k) {
+                const int64_t q = p0 + k;
+                if (q >= yn) break;
+                if (O == 0) {
+                    tout[lane][k] = tin[lane][k];
+                    continue;
+                }
+                if (q + O < yn) {
+                    t += tin[lane][k];
+                    for (int i = 0; i + 1 < H; ++i) h[i] = h[i + 1];
+                    h[H - 1] = t;
+                }
+                if (q < yn - O) {
+                    float v = (q <= O) ? 0.0f : -h[0] / d;
+                    v += h[H - 1] / d;
+                    if (q >= 1 && q <= O) v = (float)((double)v * ((double)d / ((double)(q + O) + 1.0)));
+                    if (q == 0) v = (float)((double)v * ((double)d / ((double)O + 1.0)));
+                    tout[lane][k] = v;
+                }
+            }
+        }
+        __syncthreads();
+        for (int l = 0; l < 64; ++l) {
+            const int64_t ln = line0 + l;
+            const int64_t q = p0 + lane;
+            if (ln < n_lines && q < yn - O) A[ln * yn + q] = tout[l][lane];
+        }
+        __syncthreads();
+    }
+    /* right-hand end (LocalIntensityScale.hpp:346-352) */
+    if (valid && O > 0) {
+        const float tot = h[H - 1];
+        for (int r = 0; r < O; ++r) {
+            const int64_t q = yn - O + r;
+            if (q < 0) continue;
+            float v = (q <= O) ? 0.0f : -h[1 + r] / d;
+            const float cnt = (float)(r + 1);
+            v *= d;
+            v += tot;
+            v = (float)((double)v * (1.0 / (double)(d - cnt)));
+            if (q >= 1 && q <= O) v = (float)((double)v * ((double)d / ((double)(q + O) + 1.0)));
+            if (q == 0) v = (float)((double)v * ((double)d / ((double)O + 1.0)));
+            A[me * yn + q] = v;
+        }
+    }
+}
+
+/* x / z running-sum mean with offset O (LocalIntensityScale.hpp:365-494), in place;
+   thread = one y of one line along the outer axis (length n, stride s) */
+template <int O>
+__global__ __launch_bounds__(256) void k_mean_outer(float* A, int64_t y_num, int64_t n, int64_t s, int64_t outer_stride) {
+    constexpr int H = 2 * O + 2;
+    constexpr int W = 2 * O + 1;
+    const int64_t y = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= y_num) return;
+    float* base = A + (int64_t)blockIdx.y * outer_stride + y;
+    float h[H];
+    for (int i = 0; i < H; ++i) h[i] = 0.0f;
+    float c = 0.0f;
+    for (int64_t j = 0; j < n; ++j) {
+        const float x = base[j * s];
+        c = (j == 0) ? x : x + c;
+        for (int i = 0; i + 1 < H; ++i) h[i] = h[i + 1];
+        h[H - 1] = c;
+        const int64_t i = j - O;
+        if (i < 0) continue;
+        if (i <= O) {
+            base[i * s] = c / (float)(i + O + 1);
+        } else if (i < n - O) {
+            base[i * s] = (c - h[0]) / (float)W;
+        }
+    }
+    /* right-hand end: (C[last] - C[i-O-1]) / (n - i + O) */
+    for (int r = 0; r < O; ++r) {
+        const int64_t i = n - O + r;
+        base[i * s] = (h[H - 1] - h[1 + r]) / (float)(n - i + O);
+    }
+}
+
+template <int O>
+void mean_y(aprh_context* ctx, int mode, const float* inten, float* A, const PadGeom& g) {
+    const int64_t lines = g.xp * g.zp;
+    hipLaunchKernelGGL(k_mean_y<O>, dim3((unsigned)cdiv(lines, 64)), dim3(64), 0, ctx->stream, mode, inten, A, g);
+    HIP_LAUNCH_CHECK();
+}
+
+void mean_y_dispatch(aprh_context* ctx, int O, int mode, const float* inten, float* A, const PadGeom& g) {
+    switch (O) {
+        case 0: mean_y<0>(ctx, mode, inten, A, g); break;
+        case 1: mean_y<1>(ctx, mode, inten, A, g); break;
+        case 2: mean_y<2>(ctx, mode, inten, A, g); break;
+        case 3: mean_y<3>(ctx, mode, inten, A, g); break;
+        case 4: mean_y<4>(ctx, mode, inten, A, g); break;
+        case 5: mean_y<5>(ctx, mode, inten, A, g); break;
+        case 6: mean_y<6>(ctx, mode, inten, A, g); break;
+        default: throw Error(APRH_ERR_UNSUPPORTED, "mean window offset > 6");
+    }
+}
+
+template <int O>
+void mean_outer(aprh_context* ctx, float* A, int64_t y_num, int64_t n, int64_t s, int64_t outer_stride, int64_t n_outer) {
+    hipLaunchKernelGGL(k_mean_outer<O>, dim3((unsigned)cdiv(y_num, 256), (unsigned)n_outer), dim3(256), 0, ctx->stream, A, y_num, n,
+                       s, outer_stride);
+    HIP_LAUNCH_CHECK();
+}
+
+void mean_outer_dispatch(aprh_context* ctx, int O, float* A, int64_t y_num, int64_t n, int64_t s, int64_t outer_stride,
+                         int64_t n_outer) {
+    if (n < 2 * O + 2) throw Error(APRH_ERR_UNSUPPORTED, "mean window wider than the axis");
+    switch (O) {
+        case 1: mean_outer<1>(ctx, A, y_num, n, s, outer_stride, n_outer); break;
+        case 2: mean_outer<2>(ctx, A, y_num, n, s, outer_stride, n_outer); break;
+        case 3: mean_outer<3>(ctx, A, y_num, n, s, outer_stride, n_outer); break;
+        case 4: mean_outer<4>(ctx, A, y_num, n, s, outer_stride, n_outer); break;
+        case 5: mean_outer<5>(ctx, A, y_num, n, s, outer_stride, n_outer); break;
+        case 6: mean_outer<6>(ctx, A, y_num, n, s, outer_stride, n_outer); break;
+        default: throw Error(APRH_ERR_UNSUPPORTED, "mean window offset must be 1..6");
+    }
+}
+
+/* x mean over A (dims y,x,z): outer = z; z mean: outer = x */
+void mean_x(aprh_context* ctx, int O, float* A, int64_t y, int64_t x, int64_t z) {
+    mean_outer_dispatch(ctx, O, A, y, x, y, x * y, z);
+}
+void mean_z(aprh_context* ctx, int O, float* A, int64_t y, int64_t x, int64_t z) {
+    mean_outer_dispatch(ctx, O, A, y, z, x * y, y, x);
+}
+
+/* applyParameters + computeLevels, one lane per downsampled cell */
+__global__ void k_levels(const float* __restrict__ A, PadGeom g, float var_rescale, const float* __restrict__ inten,
+                         uint16_t* grad, float* sigma_raw, float* sigma_out, uint8_t* level, const float* d_off, float Ip_th,
+                         float sigma_th, float sigma_th_max, float grad_th, float mult, uint16_t* grad_keep) {
+    const float ipth = Ip_th + *d_off;
+    const size_t n = (size_t)(g.yd * g.xd * g.zd);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int64_t y = (int64_t)(i % (size_t)g.yd);
+        const int64_t r = (int64_t)(i / (size_t)g.yd);
+        const int64_t x = r % g.xd, z = r / g.xd;
+        /* rescale_var + unpad (LocalIntensityScale.hpp:173-184, PixelData.hpp:1189-1210) */
+        float s = A[((z + g.pz) * g.xp + (x + g.px)) * g.yp + (y + g.py)] * var_rescale;
+        if (sigma_raw) sigma_raw[i] = s;
+        uint16_t gv = grad[i];
+        /* threshold_gradient (ComputeGradient.hpp:159-171) */
+        if (inten[i] <= ipth) gv = 0;
+        if (s < sigma_th) s = (s < sigma_th_max) ? 60000.0f : sigma_th;
+        if ((float)gv < grad_th) gv = 0;
+        if (sigma_out) sigma_out[i] = s;
+        if (grad_keep) grad_keep[i] = gv;
+        const float L = (float)gv / s;
+        const uint32_t u = f2u32(L * mult);
+        level[i] = (uint8_t)(u == 0u ? 0 : 31 - __clz(u));
+    }
+}
+
+/* rescale_var + unpaddPixels (LocalIntensityScale.hpp:173-184, PixelData.hpp:1189-1210) */
+__global__ void k_unpad_rescale(const float* __restrict__ A, PadGeom g, float var_rescale, float* out) {
+    const size_t n = (size_t)(g.yd * g.xd * g.zd);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int64_t y = (int64_t)(i % (size_t)g.yd);
+        const int64_t r = (int64_t)(i / (size_t)g.yd);
+        const int64_t x = r % g.xd, z = r / g.xd;
+        out[i] = A[((z + g.pz) * g.xp + (x + g.px)) * g.yp + (y + g.py)] * var_rescale;
+    }
+}
+
+/* constant local intensity scale (LocalIntensityScale.hpp:131-148): a sequential double
+   sum in index order, kept sequential (single lane) to reproduce its rounding */
+__global__ void k_constant_scale_sum(const float* __restrict__ m, size_t n, float* out_val) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    float mn = 660000.0f;
+    double sum = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        const float t = m[i];
+        sum += (double)t;
+        if (t < mn) mn = t;
+    }
+    *out_val = (float)(sum / (double)(float)n - (double)mn);
+}
+
+__global__ void k_fill(float* m, size_t n, const float* __restrict__ val) {
+    const float v = *val;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) m[i] = v;
+}
+
+PadGeom pad_geom(int64_t yd, int64_t xd, int64_t zd, const LisParams& lp) {
+    PadGeom g;
+    g.yd = yd; g.xd = xd; g.zd = zd;
+    g.py = lp.pad[0]; g.px = lp.pad[1]; g.pz = lp.pad[2];
+    g.yp = yd + 2 * g.py; g.xp = xd + 2 * g.px; g.zp = zd + 2 * g.pz;
+    return g;
+}
+
+}  // namespace
+
+void launch_grad_ds_u16(aprh_context* ctx, const uint16_t* img, uint16_t* grad, float* mean, int64_t y, int64_t x, int64_t z,
+                        float hx, float hy, float hz) {
+    const int64_t yd = cdiv(y, 2), xd = cdiv(x, 2), zd = cdiv(z, 2);
+    if (xd > 65535 || zd > 65535) throw Error(APRH_ERR_UNSUPPORTED, "x or z dimension too large");
+    hipLaunchKernelGGL(k_grad_ds, dim3((unsigned)cdiv(yd, 256), (unsigned)xd, (unsigned)zd), dim3(256), 0, ctx->stream, img, grad,
+                       mean, y, x, z, yd, 2.0f * hx, 2.0f * hy, 2.0f * hz);
+    HIP_LAUNCH_CHECK();
+}
+
+void launch_inv_bspline(aprh_context* ctx, float* m, float* tmp, int64_t y, int64_t x, int64_t z, bool ay, bool ax, bool az) {
+    /* ping-pong m -> tmp -> m ...; result always ends in m */
+    const size_t n = (size_t)(y * x * z);
+    float* src = m;
+    float* dst = tmp;
+    const dim3 grid(4096), block(256);
+    if (ay) {
+        hipLaunchKernelGGL(k_inv_y, grid, block, 0, ctx->stream, src, dst, y, n);
+        HIP_LAUNCH_CHECK();
+        std::swap(src, dst);
+    }
+    if (ax) {
+        hipLaunchKernelGGL(k_inv_outer, grid, block, 0, ctx->stream, src, dst, x, y, n);
+        HIP_LAUNCH_CHECK();
+        std::swap(src, dst);
+    }
+    if (az) {
+        hipLaunchKernelGGL(k_inv_outer, grid, block, 0, ctx->stream, src, dst, z, x * y, n);
+        HIP_LAUNCH_CHECK();
+        std::swap(src, dst);
+    }
+    if (src != m) HIP_CHECK(hipMemcpyAsync(m, src, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+}
+
+LisParams lis_params(const aprh_params& par, int64_t yd, int64_t xd, int64_t zd) {
+    /* get_window_alt (LocalIntensityScale.hpp:250-303) */
+    static const double r3[6] = {12.8214, 26.1256, 40.2795, 23.3692, 36.2061, 27.0385};
+    static const double r2[6] = {13.2421, 28.7069, 52.0385, 24.4272, 34.9565, 21.1891};
+    static const double r1[6] = {13.9040, 31.2843, 57.4037, 30.3767, 45.9930, 29.0890};
+    static const int w1[] = {1, 1, 1, 2, 2, 3}, w2[] = {2, 3, 4, 4, 5, 6};
+    int psf_ind = (int)std::max((float)(std::round((double)(par.psfx / par.dx)) - 1), 0.0f);
+    psf_ind = std::min(psf_ind, 5);
+    const int wv = std::max(w1[psf_ind], w2[psf_ind]);
+    LisParams lp;
+    const int64_t dims[3] = {yd, xd, zd};
+    int nact = 0;
+    for (int a = 0; a < 3; ++a) {
+        lp.active[a] = dims[a] > wv;
+        lp.win[a] = lp.active[a] ? w1[psf_ind] : 0;
+        lp.win[a + 3] = lp.active[a] ? w2[psf_ind] : 0;
+        nact += lp.active[a];
+        /* paddPixels clamps the pad to dim-1 (PixelData.hpp:1083-1099) */
+        int p = std::max(lp.win[a], lp.win[a + 3]);
+        lp.pad[a] = dims[a] > 1 ? (int)std::min<int64_t>(p, dims[a] - 1) : 0;
+    }
+    lp.var_rescale = (float)(nact == 3 ? r3[psf_ind] : nact == 2 ? r2[psf_ind] : r1[psf_ind]);
+    return lp;
+}
+
+void launch_sat_mean(aprh_context* ctx, float* m, int64_t y, int64_t x, int64_t z, int axis, int off) {
+    if (axis == 0) {
+        LisParams lp{};
+        PadGeom g = pad_geom(y, x, z, lp);
+        (void)g;
+        /* y mean of an unpadded array: treat it as a padded array with zero pads whose source is itself */
+        PadGeom gg;
+        gg.yd = y; gg.xd = x; gg.zd = z; gg.py = gg.px = gg.pz = 0; gg.yp = y; gg.xp = x; gg.zp = z;
+        if (y < 2 * off + 2) throw Error(APRH_ERR_UNSUPPORTED, "mean window wider than the axis");
+        mean_y_dispatch(ctx, off, 0, m, m, gg);
+    } else if (axis == 1) {
+        mean_x(ctx, off, m, y, x, z);
+    } else {
+        mean_z(ctx, off, m, y, x, z);
+    }
+}
+
+void launch_lis(aprh_context* ctx, const float* intensity, float* sigma_padded, float* /*unused*/, int64_t yd, int64_t xd,
+                int64_t zd, const LisParams& lp) {
+    PadGeom g = pad_geom(yd, xd, zd, lp);
+    float* A = sigma_padded;
+    /* first mean (windows win[0..2]) of the padded intensity */
+    mean_y_dispatch(ctx, lp.active[0] ? lp.win[0] : 0, 0, intensity, A, g);
+    if (lp.active[1]) mean_x(ctx, lp.win[1], A, g.yp, g.xp, g.zp);
+    if (lp.active[2]) mean_z(ctx, lp.win[2], A, g.yp, g.xp, g.zp);
+    /* |mean - intensity| then the second mean (windows win[3..5]) */
+    mean_y_dispatch(ctx, lp.active[0] ? lp.win[3] : 0, 1, intensity, A, g);
+    if (lp.active[1]) mean_x(ctx, lp.win[4], A, g.yp, g.xp, g.zp);
+    if (lp.active[2]) mean_z(ctx, lp.win[5], A, g.yp, g.xp, g.zp);
+}
+
+void launch_levels_padded(aprh_context* ctx, const float* A, const LisParams& lp, int64_t yd, int64_t xd, int64_t zd,
+                          const float* inten, uint16_t* grad, float* sigma_raw, float* sigma_out, uint8_t* level,
+                          const aprh_params& par, const float* d_offset, float mult, uint16_t* grad_keep) {
+    PadGeom g = pad_geom(yd, xd, zd, lp);
+    hipLaunchKernelGGL(k_levels, dim3(4096), dim3(256), 0, ctx->stream, A, g, lp.var_rescale, inten, grad, sigma_raw, sigma_out, level,
+                       d_offset, par.Ip_th, par.sigma_th, par.sigma_th_max, par.grad_th, mult, grad_keep);
+    HIP_LAUNCH_CHECK();
+}
+
+void launch_unpad_rescale(aprh_context* ctx, const float* A, const LisParams& lp, int64_t yd, int64_t xd, int64_t zd, float* out) {
+    PadGeom g = pad_geom(yd, xd, zd, lp);
+    hipLaunchKernelGGL(k_unpad_rescale, dim3(4096), dim3(256), 0, ctx->stream, A, g, lp.var_rescale, out);
+    HIP_LAUNCH_CHECK();
+}
+
+__global__ void k_u8_to_f32(const uint8_t* __restrict__ in, float* out, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) out[i] = (float)in[i];
+}
+
+void launch_u8_to_f32(aprh_context* ctx, const uint8_t* in, float* out, size_t n) {
+    hipLaunchKernelGGL(k_u8_to_f32, dim3(4096), dim3(256), 0, ctx->stream, in, out, n);
+    HIP_LAUNCH_CHECK();
+}
+
+void launch_constant_scale(aprh_context* ctx, float* m, size_t n) {
+    float* d_val = (float*)ctx->d_flags + 32;
+    hipLaunchKernelGGL(k_constant_scale_sum, dim3(1), dim3(64), 0, ctx->stream, m, n, d_val);
+    HIP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_fill, dim3(4096), dim3(256), 0, ctx->stream, m, n, d_val);
+    HIP_LAUNCH_CHECK();
+}
+
+size_t lis_padded_elems(int64_t yd, int64_t xd, int64_t zd, const LisParams& lp) {
+    return (size_t)((yd + 2 * lp.pad[0]) * (xd + 2 * lp.pad[1]) * (zd + 2 * lp.pad[2]));
+}
+
+}  // namespace aprh

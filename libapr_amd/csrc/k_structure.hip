@@ -1,0 +1,511 @@
+/*
+ * k_structure.hip -- pulling scheme, linear particle structure and particle
+ * sampling on gfx950.
+ *
+ *  - level pyramid + particle-cell seeds: 2x2x2 max downsample of the level
+ *    map fused with PullingScheme::fill (LocalParticleCellSet.hpp:58-91,
+ *    PullingScheme.hpp:140-168).
+ *  - pulling scheme: one in-place pass per level, top to bottom, fusing
+ *    set_ascendant_neighbours, set_filler and fill_neighbours/fill_parent
+ *    (PullingScheme.hpp:111-393).  The per-cell predicates only test states
+ *    that the pass cannot create or destroy (ASCENDANT; SEED-or-PROPOGATE),
+ *    so concurrent in-place updates give the sequential result.
+ *  - linear structure (LinearAccess.hpp:210-524): equivalence optimisation
+ *    level by level, then per-64-cell-segment counts, one exclusive scan over
+ *    all segments in particle order, and a wave-per-segment fill of y_vec
+ *    using ballot prefix counts; xz_end_vec is read off the segment scan.
+ *  - sampling (ParticleData.hpp:221-261): mean pyramid of the image, then a
+ *    row-parallel gather.
+ */
+#include <algorithm>
+#include <cmath>
+
+#include "aprh_internal.hpp"
+
+namespace aprh {
+
+namespace {
+
+enum : uint8_t { EMPTY = 0, SEED = 1, BOUNDARY = 2, FILLER = 3, UPSEED = 4, ASC = 8, PROP = 15, ASCN = 16 };
+
+struct Dims {
+    int64_t y, x, z;
+};
+
+/* level map of level l from level l+1 (max of the clamped 2x2x2 block) + seeds of level l */
+__global__ void k_level_down(const uint8_t* __restrict__ fine, Dims fd, uint8_t* coarse, Dims cd, uint8_t* pct, int level, int mode) {
+    const size_t n = (size_t)(cd.y * cd.x * cd.z);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int64_t y = (int64_t)(i % (size_t)cd.y);
+        const int64_t r = (int64_t)(i / (size_t)cd.y);
+        const int64_t x = r % cd.x, z = r / cd.x;
+        const int64_t y1 = min(2 * y + 1, fd.y - 1), x1 = min(2 * x + 1, fd.x - 1), z1 = min(2 * z + 1, fd.z - 1);
+        const int64_t y0 = 2 * y, x0 = 2 * x, z0 = 2 * z;
+        uint8_t m = fine[(z0 * fd.x + x0) * fd.y + y0];
+        m = max(m, fine[(z0 * fd.x + x0) * fd.y + y1]);
+        m = max(m, fine[(z0 * fd.x + x1) * fd.y + y0]);
+        m = max(m, fine[(z0 * fd.x + x1) * fd.y + y1]);
+        m = max(m, fine[(z1 * fd.x + x0) * fd.y + y0]);
+        m = max(m, fine[(z1 * fd.x + x0) * fd.y + y1]);
+        m = max(m, fine[(z1 * fd.x + x1) * fd.y + y0]);
+        m = max(m, fine[(z1 * fd.x + x1) * fd.y + y1]);
+        if (coarse) coarse[i] = m;
+        bool seed = (mode == 0) ? (m >= level) : (mode == 1) ? (m <= level) : (m == level);
+        pct[i] = seed ? SEED : EMPTY;
+    }
+}
+
+/* seeds of the top level straight from the level map (PullingScheme::fill, k == l_max) */
+__global__ void k_seed(const uint8_t* __restrict__ lev, uint8_t* pct, size_t n, int level, int mode) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const uint8_t m = lev[i];
+        bool seed = (mode == 0) ? (m >= level) : (mode == 1) ? (m <= level) : (m == level);
+        pct[i] = seed ? SEED : EMPTY;
+    }
+}
+
+/* one pulling-scheme level (PullingScheme.hpp:111-132 loop body) */
+__global__ void k_pull_level(uint8_t* P, Dims d, uint8_t* Pc, Dims dc, uint8_t* Pp, Dims dp, int is_top, int has_parent) {
+    const size_t n = (size_t)(d.y * d.x * d.z);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int64_t y = (int64_t)(i % (size_t)d.y);
+        const int64_t r = (int64_t)(i / (size_t)d.y);
+        const int64_t x = r % d.x, z = r / d.x;
+        const uint8_t old = P[i];
+        bool any_asc = false, any_sp = false;
+        const int64_t za = z > 0 ? z - 1 : 0, zb = z < d.z - 1 ? z + 1 : z;
+        const int64_t xa = x > 0 ? x - 1 : 0, xb = x < d.x - 1 ? x + 1 : x;
+        const int64_t ya = y > 0 ? y - 1 : 0, yb = y < d.y - 1 ? y + 1 : y;
+        for (int64_t zz = za; zz <= zb; ++zz)
+            for (int64_t xx = xa; xx <= xb; ++xx)
+                for (int64_t yy = ya; yy <= yb; ++yy) {
+                    const uint8_t s = P[(zz * d.x + xx) * d.y + yy];
+                    any_asc |= (s == ASC);
+                    any_sp |= (s == SEED) | (s == PROP);
+                }
+        uint8_t nw = old;
+        if (!is_top) {
+            /* set_ascendant_neighbours then fill_neighbours */
+            if (old == EMPTY) nw = any_asc ? ASCN : (any_sp ? BOUNDARY : EMPTY);
+            else if (old == SEED) nw = any_asc ? PROP : SEED;
+        } else {
+            if (old == EMPTY && any_sp) nw = BOUNDARY;
+        }
+        if (nw != old) P[i] = nw;
+        /* set_filler: empty children of ASCENDANTNEIGHBOUR / PROPOGATE become FILLER */
+        if (!is_top && (nw == ASCN || nw == PROP)) {
+            const int64_t cz = (z == d.z - 1 && (dc.z & 1)) ? 1 : 2;
+            const int64_t cx = (x == d.x - 1 && (dc.x & 1)) ? 1 : 2;
+            const int64_t cy = (y == d.y - 1 && (dc.y & 1)) ? 1 : 2;
+            for (int64_t a = 0; a < cz; ++a)
+                for (int64_t b = 0; b < cx; ++b)
+                    for (int64_t c = 0; c < cy; ++c) {
+                        uint8_t* ch = &Pc[((2 * z + a) * dc.x + (2 * x + b)) * dc.y + (2 * y + c)];
+                        if (*ch == EMPTY) *ch = FILLER;
+                    }
+        }
+        /* fill_parent */
+        if (has_parent && (nw == SEED || nw == PROP || nw == ASC)) {
+            uint8_t* p = &Pp[((z / 2) * dp.x + (x / 2)) * dp.y + (y / 2)];
+            if (*p != SEED) *p = ASC;
+        }
+    }
+}
+
+/* equivalence optimisation step for one level (LinearAccess.hpp:265-294) */
+__global__ void k_upseed(uint8_t* P, Dims d, const uint8_t* __restrict__ Pp, Dims dp, int min_type) {
+    const size_t n = (size_t)(d.y * d.x * d.z);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int64_t y = (int64_t)(i % (size_t)d.y);
+        const int64_t r = (int64_t)(i / (size_t)d.y);
+        const int64_t x = r % d.x, z = r / d.x;
+        const uint8_t s = Pp[((z / 2) * dp.x + (x / 2)) * dp.y + (y / 2)];
+        if (s > 0 && s <= min_type) P[i] = UPSEED;
+    }
+}
+
+/* segment table: particle-order list of (level row, 64-cell segment) */
+constexpr int SEG = 64;
+constexpr int MAXL = 24;
+struct SegTable {
+    int n_levels;                 /* APR levels l_min..l_max           */
+    int l_min, l_max;
+    int64_t seg_begin[MAXL + 1];  /* first segment id of each APR level (index l - l_min) */
+    int64_t row_begin[MAXL];      /* xz_end index of row 0 of level    */
+    int64_t nseg_row[MAXL];       /* segments per row                  */
+    int64_t rows_x[MAXL];         /* x_num of the APR level            */
+    int64_t map_off[MAXL];        /* byte offset of the pct map used   */
+    int64_t map_y[MAXL], map_x[MAXL]; /* dims of that pct map           */
+    int64_t y_max;                /* y_num at l_max                    */
+    int min_type;
+};
+
+__device__ __forceinline__ int find_level(const SegTable& t, int64_t s) {
+    int li = 0;
+    while (li + 1 < t.n_levels && s >= t.seg_begin[li + 1]) ++li;
+    return li;
+}
+
+/* weight of one pct cell for APR level li (0 or 1; 0..2 at l_max) */
+__device__ __forceinline__ int cell_weight(const SegTable& t, int li, uint8_t st, int64_t yy) {
+    if (li == t.n_levels - 1) {
+        if (st > 0 && st <= t.min_type) return 1 + ((2 * yy < t.y_max - 1) ? 1 : 0);
+        return 0;
+    }
+    return (st > t.min_type && st <= UPSEED) ? 1 : 0;
+}
+
+/* locate segment s: level index, pct row start pointer, y0 and row length */
+__device__ __forceinline__ const uint8_t* seg_row(const SegTable& t, const uint8_t* maps, int64_t s, int& li, int64_t& y0,
+                                                  int64_t& ylen, int64_t& xzrow) {
+    li = find_level(t, s);
+    const int64_t local = s - t.seg_begin[li];
+    const int64_t row = local / t.nseg_row[li];
+    const int64_t seg = local % t.nseg_row[li];
+    xzrow = row;
+    y0 = seg * SEG;
+    ylen = t.map_y[li];
+    int64_t prow;
+    if (li == t.n_levels - 1) {
+        /* max-level row (z, x) reads pct row (z/2, x/2) of level l_max - 1 */
+        const int64_t zm = row / t.rows_x[li], xm = row % t.rows_x[li];
+        prow = (zm / 2) * t.map_x[li] + xm / 2;
+    } else {
+        prow = row;
+    }
+    return maps + t.map_off[li] + prow * ylen;
+}
+
+__global__ void k_seg_count(const uint8_t* __restrict__ maps, SegTable t, int64_t nseg, uint32_t* cnt) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    int li;
+    int64_t y0, ylen, row;
+    const uint8_t* p = seg_row(t, maps, s, li, y0, ylen, row);
+    const int64_t y1 = min(y0 + SEG, ylen);
+    uint32_t c = 0;
+    for (int64_t yy = y0; yy < y1; ++yy) c += cell_weight(t, li, p[yy], yy);
+    cnt[s] = c;
+}
+
+/* exclusive scan, three phases over blocks of 1024 */
+__global__ __launch_bounds__(1024) void k_scan_blocks(const uint32_t* __restrict__ in, uint64_t* out, uint64_t* block_sums, int64_t n) {
+    __shared__ uint64_t sh[1024];
+    const int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+    uint64_t v = i < n ? in[i] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        uint64_t a = threadIdx.x >= (unsigned)o ? sh[threadIdx.x - o] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += a;
+        __syncthreads();
+    }
+    if (i < n) out[i] = sh[threadIdx.x] - v;
+    if (threadIdx.x == 1023) block_sums[blockIdx.x] = sh[1023];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_sums(uint64_t* block_sums, int64_t nb, uint64_t* total) {
+    /* single block: exclusive scan of the block sums (serial chunks per thread) */
+    __shared__ uint64_t sh[1024];
+    const int64_t per = (nb + 1023) / 1024;
+    const int64_t b = (int64_t)threadIdx.x * per;
+    uint64_t s = 0;
+    for (int64_t k = b; k < min(b + per, nb); ++k) s += block_sums[k];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        uint64_t a = threadIdx.x >= (unsigned)o ? sh[threadIdx.x - o] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += a;
+        __syncthreads();
+    }
+    uint64_t run = sh[threadIdx.x] - s;
+    for (int64_t k = b; k < min(b + per, nb); ++k) {
+        uint64_t v = block_sums[k];
+        block_sums[k] = run;
+        run += v;
+    }
+    if (threadIdx.x == 1023) *total = sh[1023];
+}
+
+__global__ void k_scan_add(uint64_t* out, const uint64_t* __restrict__ block_sums, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += block_sums[i / 1024];
+}
+
+/* xz_end_vec: inclusive count at the end of each row = offset of the row's last segment + its count */
+__global__ void k_xz_end(const uint64_t* __restrict__ seg_off, const uint32_t* __restrict__ cnt, SegTable t, uint64_t* xz_end,
+                         int64_t n_rows_total, const int64_t* __restrict__ dummy) {
+    (void)dummy;
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rows_total) return;
+    /* r enumerates rows of levels l_min..l_max in order */
+    int li = 0;
+    int64_t acc = 0;
+    for (; li < t.n_levels; ++li) {
+        const int64_t nrows = (t.seg_begin[li + 1] - t.seg_begin[li]) / t.nseg_row[li];
+        if (r < acc + nrows) break;
+        acc += nrows;
+    }
+    const int64_t row = r - acc;
+    const int64_t s_last = t.seg_begin[li] + row * t.nseg_row[li] + t.nseg_row[li] - 1;
+    xz_end[t.row_begin[li] + row] = seg_off[s_last] + cnt[s_last];
+}
+
+/* wave per segment: y values of the qualifying cells in order */
+__global__ __launch_bounds__(256) void k_seg_fill(const uint8_t* __restrict__ maps, SegTable t, int64_t nseg,
+                                                  const uint64_t* __restrict__ seg_off, uint16_t* y_vec) {
+    const int64_t s = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const int lane = threadIdx.x & 63;
+    if (s >= nseg) return;
+    int li;
+    int64_t y0, ylen, row;
+    const uint8_t* p = seg_row(t, maps, s, li, y0, ylen, row);
+    const int64_t yy = y0 + lane;
+    int w = 0;
+    if (yy < ylen) w = cell_weight(t, li, p[yy], yy);
+    const uint64_t b1 = __ballot(w >= 1);
+    const uint64_t b2 = __ballot(w >= 2);
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint64_t pos = seg_off[s] + __popcll(b1 & lt) + __popcll(b2 & lt);
+    if (w == 0) return;
+    if (li == t.n_levels - 1) {
+        y_vec[pos] = (uint16_t)(2 * yy);
+        if (w == 2) y_vec[pos + 1] = (uint16_t)(2 * yy + 1);
+    } else {
+        y_vec[pos] = (uint16_t)yy;
+    }
+}
+
+/* mean pyramid level (downsamplePyramid with sum then x/8.0f, u16 store) */
+__global__ void k_pyr_down(const uint16_t* __restrict__ fine, Dims fd, uint16_t* coarse, Dims cd) {
+    const size_t n = (size_t)(cd.y * cd.x * cd.z);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int64_t y = (int64_t)(i % (size_t)cd.y);
+        const int64_t r = (int64_t)(i / (size_t)cd.y);
+        const int64_t x = r % cd.x, z = r / cd.x;
+        const int64_t y1 = min(2 * y + 1, fd.y - 1), x1 = min(2 * x + 1, fd.x - 1), z1 = min(2 * z + 1, fd.z - 1);
+        const int64_t y0 = 2 * y, x0 = 2 * x, z0 = 2 * z;
+        float s = (float)fine[(z0 * fd.x + x0) * fd.y + y0];
+        s = s + (float)fine[(z0 * fd.x + x0) * fd.y + y1];
+        s = s + (float)fine[(z0 * fd.x + x1) * fd.y + y0];
+        s = s + (float)fine[(z0 * fd.x + x1) * fd.y + y1];
+        s = s + (float)fine[(z1 * fd.x + x0) * fd.y + y0];
+        s = s + (float)fine[(z1 * fd.x + x0) * fd.y + y1];
+        s = s + (float)fine[(z1 * fd.x + x1) * fd.y + y0];
+        s = s + (float)fine[(z1 * fd.x + x1) * fd.y + y1];
+        const float m = s / 8.0f;
+        coarse[i] = (uint16_t)(int32_t)m;
+    }
+}
+
+struct LevelPtrs {
+    const uint16_t* img[MAXL];
+    int64_t y[MAXL], x[MAXL];
+    int64_t row_begin[MAXL];
+    int64_t nrows[MAXL];
+    int64_t rows_prefix[MAXL + 1];
+    int n_levels;
+};
+
+/* one wave per row; lanes stride the row's particles */
+__global__ __launch_bounds__(256) void k_gather(LevelPtrs L, const uint64_t* __restrict__ xz_end, const uint16_t* __restrict__ y_vec,
+                                                uint16_t* parts, int64_t total_rows) {
+    const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / 64;
+    for (int64_t r = wid; r < total_rows; r += nw) {
+        int li = 0;
+        while (li + 1 < L.n_levels && r >= L.rows_prefix[li + 1]) ++li;
+        const int64_t row = r - L.rows_prefix[li];
+        const int64_t xzi = L.row_begin[li] + row;
+        const uint64_t b = xz_end[xzi - 1], e = xz_end[xzi];
+        const int64_t z = row / L.x[li], x = row % L.x[li];
+        const uint16_t* plane = L.img[li] + (z * L.x[li] + x) * L.y[li];
+        for (uint64_t i = b + lane; i < e; i += 64) parts[i] = plane[y_vec[i]];
+    }
+}
+
+}  // namespace
+
+void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, const aprh_params& par) {
+    hipStream_t st = ctx->stream;
+    const int lmax = a.l_max, lmin = a.l_min;
+    const dim3 grid(4096), block(256);
+    if (lmax <= 2) {
+        /* LinearAccess.hpp:229-260: every pixel is a particle at l_max */
+        const int64_t yL = a.y_num[lmax], xL = a.x_num[lmax], zL = a.z_num[lmax];
+        std::vector<uint64_t> xz(a.n_xz, 0);
+        uint64_t c = 0;
+        for (int64_t z = 0; z < zL; ++z)
+            for (int64_t x = 0; x < xL; ++x) {
+                c += (uint64_t)yL;
+                xz[a.level_xz_vec[lmax] + z * xL + x] = c;
+            }
+        /* the reference leaves the lower rows at 0 and skips the partial sum */
+        std::vector<uint16_t> yv(c);
+        uint64_t k = 0;
+        for (int64_t z = 0; z < zL; ++z)
+            for (int64_t x = 0; x < xL; ++x)
+                for (int64_t y = 0; y < yL; ++y) yv[k++] = (uint16_t)y;
+        a.total = c;
+        a.xz_end.reset(ctx->pool, a.n_xz * 8);
+        a.y_vec.reset(ctx->pool, std::max<uint64_t>(c, 1) * 2);
+        HIP_CHECK(hipMemcpyAsync(a.xz_end.ptr(), xz.data(), a.n_xz * 8, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(a.y_vec.ptr(), yv.data(), c * 2, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        return;
+    }
+    /* particle-cell tree levels lmin .. top (= lmax - 1), one contiguous u8 buffer */
+    const int top = lmax - 1;
+    std::vector<int64_t> off(top + 2, 0);
+    for (int l = lmin; l <= top; ++l) off[l + 1] = off[l] + a.y_num[l] * a.x_num[l] * a.z_num[l];
+    const int64_t pct_bytes = off[top + 1];
+    Buf pct(ctx->pool, (size_t)pct_bytes);
+    Buf lev(ctx->pool, (size_t)pct_bytes);
+    uint8_t* P = pct.as<uint8_t>();
+    uint8_t* LV = lev.as<uint8_t>();
+    auto D = [&](int l) { return Dims{a.y_num[l], a.x_num[l], a.z_num[l]}; };
+    auto mode_of = [&](int l) { return l == top ? 0 : (l == lmin ? 1 : 2); };
+    /* seeds of the top level straight from the level map, then the max pyramid */
+    {
+        const size_t n = (size_t)(a.y_num[top] * a.x_num[top] * a.z_num[top]);
+        hipLaunchKernelGGL(k_seed, grid, block, 0, st, level_top, P + off[top], n, top, mode_of(top));
+        HIP_LAUNCH_CHECK();
+    }
+    const uint8_t* fine = level_top;
+    for (int l = top - 1; l >= lmin; --l) {
+        hipLaunchKernelGGL(k_level_down, grid, block, 0, st, fine, D(l + 1), LV + off[l], D(l), P + off[l], l, mode_of(l));
+        HIP_LAUNCH_CHECK();
+        fine = LV + off[l];
+    }
+    lev.release();
+    /* pulling scheme, top to bottom */
+    for (int l = top; l >= lmin; --l) {
+        const bool has_parent = (l - 1) >= lmin;
+        const bool is_top = (l == top);
+        uint8_t* Pc = is_top ? nullptr : P + off[l + 1];
+        uint8_t* Pp = has_parent ? P + off[l - 1] : nullptr;
+        hipLaunchKernelGGL(k_pull_level, grid, block, 0, st, P + off[l], D(l), Pc, is_top ? Dims{1, 1, 1} : D(l + 1), Pp,
+                           has_parent ? D(l - 1) : Dims{1, 1, 1}, is_top ? 1 : 0, has_parent ? 1 : 0);
+        HIP_LAUNCH_CHECK();
+    }
+    const int min_type = par.neighborhood_optimization ? 1 : 2;
+    /* equivalence optimisation, levels upward in order */
+    for (int l = lmin + 1; l <= top; ++l) {
+        hipLaunchKernelGGL(k_upseed, grid, block, 0, st, P + off[l], D(l), P + off[l - 1], D(l - 1), min_type);
+        HIP_LAUNCH_CHECK();
+    }
+    /* segment table over APR levels lmin..lmax */
+    SegTable t;
+    std::memset(&t, 0, sizeof(t));
+    t.l_min = lmin;
+    t.l_max = lmax;
+    t.n_levels = lmax - lmin + 1;
+    if (t.n_levels > MAXL) throw Error(APRH_ERR_UNSUPPORTED, "too many levels");
+    t.min_type = min_type;
+    t.y_max = a.y_num[lmax];
+    int64_t sb = 0;
+    for (int l = lmin; l <= lmax; ++l) {
+        const int li = l - lmin;
+        const int ml = (l == lmax) ? top : l;   /* pct map providing the cells */
+        t.seg_begin[li] = sb;
+        t.row_begin[li] = (int64_t)a.level_xz_vec[l];
+        t.nseg_row[li] = cdiv(a.y_num[ml], SEG);
+        t.rows_x[li] = a.x_num[l];
+        t.map_off[li] = off[ml];
+        t.map_y[li] = a.y_num[ml];
+        t.map_x[li] = a.x_num[ml];
+        sb += a.x_num[l] * a.z_num[l] * t.nseg_row[li];
+    }
+    t.seg_begin[t.n_levels] = sb;
+    const int64_t nseg = sb;
+    Buf cnt(ctx->pool, (size_t)nseg * 4);
+    Buf segoff(ctx->pool, (size_t)nseg * 8);
+    const int64_t nb = cdiv(nseg, 1024);
+    Buf bsum(ctx->pool, (size_t)(nb + 1) * 8);
+    hipLaunchKernelGGL(k_seg_count, dim3((unsigned)cdiv(nseg, 256)), dim3(256), 0, st, P, t, nseg, cnt.as<uint32_t>());
+    HIP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(1024), 0, st, cnt.as<uint32_t>(), segoff.as<uint64_t>(),
+                       bsum.as<uint64_t>(), nseg);
+    HIP_LAUNCH_CHECK();
+    uint64_t* d_total = (uint64_t*)ctx->d_flags + 8;
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, bsum.as<uint64_t>(), nb, d_total);
+    HIP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)cdiv(nseg, 256)), dim3(256), 0, st, segoff.as<uint64_t>(), bsum.as<uint64_t>(), nseg);
+    HIP_LAUNCH_CHECK();
+    /* xz_end_vec */
+    a.xz_end.reset(ctx->pool, a.n_xz * 8);
+    HIP_CHECK(hipMemsetAsync(a.xz_end.ptr(), 0, a.n_xz * 8, st));
+    int64_t nrows = 0;
+    for (int l = lmin; l <= lmax; ++l) nrows += a.x_num[l] * a.z_num[l];
+    hipLaunchKernelGGL(k_xz_end, dim3((unsigned)cdiv(nrows, 256)), dim3(256), 0, st, segoff.as<uint64_t>(), cnt.as<uint32_t>(), t,
+                       a.xz_end.as<uint64_t>(), nrows, (const int64_t*)nullptr);
+    HIP_LAUNCH_CHECK();
+    uint64_t total = 0;
+    HIP_CHECK(hipMemcpyAsync(&total, d_total, 8, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    a.total = total;
+    a.y_vec.reset(ctx->pool, std::max<uint64_t>(total, 1) * 2);
+    hipLaunchKernelGGL(k_seg_fill, dim3((unsigned)cdiv(nseg * 64, 256)), dim3(256), 0, st, P, t, nseg, segoff.as<uint64_t>(),
+                       a.y_vec.as<uint16_t>());
+    HIP_LAUNCH_CHECK();
+    if (ctx->keep_intermediates) {
+        a.pct.clear();
+        a.pct_dims.assign(3 * (top + 1), 0);
+        a.pct.resize(top + 1);
+        for (int l = lmin; l <= top; ++l) {
+            const size_t nb2 = (size_t)(off[l + 1] - off[l]);
+            a.pct[l].reset(ctx->pool, nb2);
+            HIP_CHECK(hipMemcpyAsync(a.pct[l].ptr(), P + off[l], nb2, hipMemcpyDeviceToDevice, st));
+            a.pct_dims[3 * l] = a.y_num[l];
+            a.pct_dims[3 * l + 1] = a.x_num[l];
+            a.pct_dims[3 * l + 2] = a.z_num[l];
+        }
+    }
+    HIP_CHECK(hipStreamSynchronize(st));
+}
+
+void launch_sample_u16(aprh_context* ctx, const aprh_apr& a, const uint16_t* img, uint16_t* parts) {
+    hipStream_t st = ctx->stream;
+    const int lmax = a.l_max, lmin = a.l_min;
+    /* pyramid levels lmin..lmax-1 in one buffer */
+    std::vector<int64_t> off(lmax + 1, 0);
+    int64_t tot = 0;
+    for (int l = lmin; l < lmax; ++l) {
+        off[l] = tot;
+        tot += a.y_num[l] * a.x_num[l] * a.z_num[l];
+    }
+    Buf pyr(ctx->pool, (size_t)std::max<int64_t>(tot, 1) * 2);
+    uint16_t* Pyr = pyr.as<uint16_t>();
+    const uint16_t* fine = img;
+    for (int l = lmax - 1; l >= lmin; --l) {
+        hipLaunchKernelGGL(k_pyr_down, dim3(4096), dim3(256), 0, st, fine, Dims{a.y_num[l + 1], a.x_num[l + 1], a.z_num[l + 1]},
+                           Pyr + off[l], Dims{a.y_num[l], a.x_num[l], a.z_num[l]});
+        HIP_LAUNCH_CHECK();
+        fine = Pyr + off[l];
+    }
+    LevelPtrs L;
+    std::memset(&L, 0, sizeof(L));
+    L.n_levels = lmax - lmin + 1;
+    int64_t rp = 0;
+    for (int l = lmin; l <= lmax; ++l) {
+        const int li = l - lmin;
+        L.img[li] = (l == lmax) ? img : Pyr + off[l];
+        L.y[li] = a.y_num[l];
+        L.x[li] = a.x_num[l];
+        L.row_begin[li] = (int64_t)a.level_xz_vec[l];
+        L.nrows[li] = a.x_num[l] * a.z_num[l];
+        L.rows_prefix[li] = rp;
+        rp += L.nrows[li];
+    }
+    L.rows_prefix[L.n_levels] = rp;
+    if (a.total > 0) {
+        hipLaunchKernelGGL(k_gather, dim3(8192), dim3(256), 0, st, L, a.xz_end.as<uint64_t>(), a.y_vec.as<uint16_t>(), parts, rp);
+        HIP_LAUNCH_CHECK();
+    }
+    HIP_CHECK(hipStreamSynchronize(st));
+}
+
+}  // namespace aprh

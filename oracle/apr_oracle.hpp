@@ -1,0 +1,1127 @@
+/*
+ * apr_oracle.hpp -- CPU restatement of LibAPR's APR hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity oracle: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
+ * only as the checker / CPU baseline.  The product (libapr_amd) never
+ * includes, links or calls it.
+ *
+ * Every function restates one reference function and cites the reference
+ * file:line it follows.  Floating-point expressions keep the reference's
+ * operand order and its C++ type promotions (float vs double sub-expressions,
+ * float->integer truncations) so that, compiled with -ffp-contract=off and
+ * without -ffast-math, results are bit-identical to the reference built the
+ * same way.  Float->u16 / float->u32 conversions reproduce what g++ emits on
+ * x86-64 (truncation through a 32-bit resp. 64-bit signed integer).
+ *
+ * Pinning of each part (checked in tests/): see the header of
+ * oracle_capi.cpp and the tests that call each entry point.
+ */
+#ifndef APR_ORACLE_HPP
+#define APR_ORACLE_HPP
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <stdexcept>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/aprh_params.h"
+
+namespace oracle {
+
+/* ------------------------------------------------------------------------ */
+/* Dense y-fastest volume, idx = z*x_num*y_num + x*y_num + y                  */
+/* (PixelData.hpp:509,521)                                                    */
+/* ------------------------------------------------------------------------ */
+template <typename T>
+struct Mesh {
+    int64_t y_num = 0, x_num = 0, z_num = 0;
+    std::vector<T> v;
+    Mesh() {}
+    Mesh(int64_t y, int64_t x, int64_t z, T init = T(0)) { resize(y, x, z, init); }
+    void resize(int64_t y, int64_t x, int64_t z, T init = T(0)) {
+        y_num = y; x_num = x; z_num = z;
+        v.assign((size_t)(y * x * z), init);
+    }
+    size_t size() const { return v.size(); }
+    T& at(int64_t y, int64_t x, int64_t z) { return v[(size_t)(z * x_num * y_num + x * y_num + y)]; }
+    const T& at(int64_t y, int64_t x, int64_t z) const { return v[(size_t)(z * x_num * y_num + x * y_num + y)]; }
+};
+
+/* x86-64 g++ conversion semantics for out-of-contract values. */
+static inline uint16_t f2u16(float v) {
+    if (!(v > -2147483648.0f && v < 2147483648.0f)) return 0;
+    return (uint16_t)(int32_t)v;
+}
+static inline uint16_t d2u16(double v) {
+    if (!(v > -2147483648.0 && v < 2147483648.0)) return 0;
+    return (uint16_t)(int32_t)v;
+}
+static inline uint32_t f2u32(float v) {
+    if (!(v > -9.2233720368547758e18f && v < 9.2233720368547758e18f)) return 0;
+    return (uint32_t)(int64_t)v;
+}
+
+template <typename T> struct Store;
+template <> struct Store<uint16_t> {
+    static uint16_t cvt(float v) { return f2u16(v); }
+    /* ComputeGradient.hpp:226-244: std::round then range check, value kept as float */
+    static float rnd(float v, size_t& err) {
+        v = std::round(v);
+        if (v < 0.0f || v > 65535.0f) err++;
+        return v;
+    }
+};
+template <> struct Store<float> {
+    static float cvt(float v) { return v; }
+    static float rnd(float v, size_t&) { return v; }
+};
+
+/* ------------------------------------------------------------------------ */
+/* B-spline smoothing coefficients (ComputeGradient.hpp:256-330).             */
+/* The reference calls the global double-precision libm functions on float   */
+/* operands; each step below states that promotion explicitly.               */
+/* ------------------------------------------------------------------------ */
+struct BsplineCoeffs {
+    float b1 = 0, b2 = 0, norm_factor = 0;
+    size_t k0 = 0, minLen = 0;
+    std::vector<float> bc1, bc2, bc3, bc4;
+};
+
+static inline float impulse_resp_f(float k, float rho, float omg) {
+    /* ComputeGradient.hpp:213-217 */
+    float ak = std::fabs(k);
+    double num = std::pow((double)rho, (double)ak) * std::sin((double)((ak + 1.0f) * omg));
+    return (float)(num / std::sin((double)omg));
+}
+
+static inline float impulse_resp_back_f(float k, float rho, float omg, float gamma, float c0) {
+    /* ComputeGradient.hpp:219-223 */
+    float ak = std::fabs(k);
+    double p = std::pow((double)rho, (double)ak);
+    double t1 = (double)c0 * p;
+    double t2 = std::cos((double)(omg * ak)) + (double)gamma * std::sin((double)(omg * ak));
+    double r2 = std::pow((double)rho, 2.0);
+    double den = std::pow(1.0 - 2.0 * (double)rho * std::cos((double)omg) + r2, 2.0);
+    return (float)((t1 * t2) * (1.0 / den));
+}
+
+static inline BsplineCoeffs bspline_coeffs(float lambda, float tol, size_t len) {
+    BsplineCoeffs c;
+    /* eq 4.6 / 4.5 (ComputeGradient.hpp:256-258) */
+    float f96 = 96.0f * lambda;
+    float f24 = 24.0f * lambda;
+    float f144p3 = 3.0f + 144.0f * lambda;
+    float xi = (float)((double)(1.0f - f96) + (double)f24 * std::sqrt((double)f144p3));
+    float inv_xi = 1.0f / xi;
+    double d1 = (double)(f24 - 1.0f) - std::sqrt((double)xi);
+    double d2 = d1 / (double)f24;
+    double d4 = (double)(48.0f * lambda) + (double)f24 * std::sqrt((double)f144p3);
+    float rho = (float)(d2 * std::sqrt((double)inv_xi * d4));
+    float omg = (float)std::atan(std::sqrt((double)(inv_xi * (144.0f * lambda - 1.0f))));
+    /* eq 4.8 (ComputeGradient.hpp:260-261) */
+    double pr2 = std::pow((double)rho, 2.0);
+    double co = std::cos((double)omg);
+    float r2f = 2.0f * rho;
+    double c0d = (((1.0 + pr2) / (1.0 - pr2)) * (1.0 - (double)r2f * co + pr2)) / (1.0 + (double)r2f * co + pr2);
+    float c0 = (float)c0d;
+    float gamma = (float)(((1.0 - pr2) / (1.0 + pr2)) * (1.0 / std::tan((double)omg)));
+    /* ComputeGradient.hpp:263-264 */
+    c.b1 = (float)((double)r2f * co);
+    c.b2 = (float)(-std::pow((double)rho, 2.0));
+    /* ComputeGradient.hpp:271-274 */
+    size_t k0 = (size_t)std::ceil(std::fabs(std::log((double)tol) / std::log((double)rho)));
+    c.k0 = k0;
+    c.minLen = std::min(k0, len);
+    c.norm_factor = (float)std::pow(1.0 - 2.0 * (double)rho * co + pr2, 2.0);
+
+    std::vector<float> f(k0 + 3), b(k0 + 3);
+    for (size_t k = 0; k < k0 + 3; ++k) f[k] = impulse_resp_f((float)k, rho, omg);
+    for (size_t k = 0; k < k0 + 3; ++k) b[k] = impulse_resp_back_f((float)k, rho, omg, gamma, c0);
+    const size_t mL = c.minLen;
+    /* ComputeGradient.hpp:287-330 */
+    c.bc1.assign(k0, 0.0f);
+    c.bc1[1] = f[0];
+    for (size_t k = 0; k < k0; ++k) c.bc1[k] += f[k + 1];
+    for (size_t k = mL; k < k0; ++k) c.bc1[mL - 1] += c.bc1[k];
+    c.bc2.assign(k0, 0.0f);
+    for (size_t k = 0; k < k0; ++k) c.bc2[k] = f[k];
+    for (size_t k = mL; k < k0; ++k) c.bc2[mL - 1] += c.bc2[k];
+    c.bc3.assign(k0, 0.0f);
+    c.bc3[0] = b[1];
+    for (size_t k = 0; k < k0 - 1; ++k) c.bc3[k + 1] += b[k] + b[k + 2];
+    for (size_t k = mL; k < k0; ++k) c.bc3[mL - 1] += c.bc3[k];
+    c.bc4.assign(k0, 0.0f);
+    c.bc4[0] = b[0];
+    for (size_t k = 1; k < k0; ++k) c.bc4[k] += 2.0f * b[k];
+    for (size_t k = mL; k < k0; ++k) c.bc4[mL - 1] += c.bc4[k];
+    return c;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Recursive smoothing-spline filters                                        */
+/* ------------------------------------------------------------------------ */
+
+/* bspline_filt_rec_y (ComputeGradient.hpp:248-413). Causal state kept      */
+/* unrounded, outputs rounded; anti-causal state restarts from stored/norm.  */
+template <typename T>
+void bspline_filt_rec_y(Mesh<T>& img, float lambda, float tol, size_t& err) {
+    const int64_t z_num = img.z_num, x_num = img.x_num, y_num = img.y_num;
+    BsplineCoeffs c = bspline_coeffs(lambda, tol, (size_t)y_num);
+    const float b1 = c.b1, b2 = c.b2, nf = c.norm_factor;
+    const int64_t mL = (int64_t)c.minLen;
+    size_t e = 0;
+#pragma omp parallel for reduction(+ : e) schedule(static)
+    for (int64_t z = 0; z < z_num; ++z) {
+        for (int64_t x = 0; x < x_num; ++x) {
+            T* line = img.v.data() + z * x_num * y_num + x * y_num;
+            float t1 = 0, t2 = 0, t3 = 0, t4 = 0;
+            for (int64_t k = 0; k < mL; ++k) {
+                t1 += c.bc1[k] * (float)line[k];
+                t2 += c.bc2[k] * (float)line[k];
+            }
+            for (int64_t k = 0; k < mL; ++k) {
+                t3 += c.bc3[k] * (float)line[y_num - 1 - k];
+                t4 += c.bc4[k] * (float)line[y_num - 1 - k];
+            }
+            line[0] = Store<T>::cvt(t2);
+            line[1] = Store<T>::cvt(t1);
+            for (int64_t k = 2; k < y_num; ++k) {
+                float t = t1 * b1 + t2 * b2 + (float)line[k];
+                line[k] = Store<T>::cvt(Store<T>::rnd(t, e));
+                t2 = t1;
+                t1 = t;
+            }
+            line[y_num - 2] = Store<T>::cvt(Store<T>::rnd(t3 * nf, e));
+            line[y_num - 1] = Store<T>::cvt(Store<T>::rnd(t4 * nf, e));
+        }
+    }
+#pragma omp parallel for reduction(+ : e) schedule(static)
+    for (int64_t z = 0; z < z_num; ++z) {
+        for (int64_t x = 0; x < x_num; ++x) {
+            T* line = img.v.data() + z * x_num * y_num + x * y_num;
+            float t2 = (float)line[y_num - 1] / nf;
+            float t1 = (float)line[y_num - 2] / nf;
+            for (int64_t k = y_num - 3; k >= 0; --k) {
+                float t = t1 * b1 + t2 * b2 + (float)line[k];
+                line[k] = Store<T>::cvt(Store<T>::rnd(t * nf, e));
+                t2 = t1;
+                t1 = t;
+            }
+        }
+    }
+    err += e;
+}
+
+/* Shared body of bspline_filt_rec_x (ComputeGradient.hpp:601-779) and       */
+/* bspline_filt_rec_z (:415-599): lines along an outer axis of length n with  */
+/* element stride s; each line is a vector over y. The causal recursion keeps */
+/* the *rounded* output as state from the third sample on.                   */
+template <typename T>
+static void bspline_outer_axis(T* base, int64_t n, int64_t s, int64_t y_num, const BsplineCoeffs& c,
+                               std::vector<float>& v1, std::vector<float>& v2, std::vector<float>& v3,
+                               std::vector<float>& v4, size_t& e) {
+    const float b1 = c.b1, b2 = c.b2, nf = c.norm_factor;
+    const int64_t mL = (int64_t)c.minLen;
+    std::fill(v1.begin(), v1.end(), 0.0f);
+    std::fill(v2.begin(), v2.end(), 0.0f);
+    std::fill(v3.begin(), v3.end(), 0.0f);
+    std::fill(v4.begin(), v4.end(), 0.0f);
+    for (int64_t j = 0; j < mL; ++j) {
+        const T* a = base + j * s;
+        const T* bk = base + (n - 1 - j) * s;
+        for (int64_t k = 0; k < y_num; ++k) {
+            v1[k] += c.bc1[j] * (float)a[k];
+            v2[k] += c.bc2[j] * (float)a[k];
+            v3[k] += c.bc3[j] * (float)bk[k];
+            v4[k] += c.bc4[j] * (float)bk[k];
+        }
+    }
+    for (int64_t k = 0; k < y_num; ++k) base[k] = Store<T>::cvt(Store<T>::rnd(v2[k], e));
+    for (int64_t k = 0; k < y_num; ++k) base[s + k] = Store<T>::cvt(Store<T>::rnd(v1[k], e));
+    for (int64_t j = 2; j < n; ++j) {
+        T* row = base + j * s;
+        for (int64_t k = 0; k < y_num; ++k) {
+            v2[k] = Store<T>::rnd((float)row[k] + b1 * v1[k] + b2 * v2[k], e);
+        }
+        std::swap(v1, v2);
+        for (int64_t k = 0; k < y_num; ++k) row[k] = Store<T>::cvt(v1[k]);
+    }
+    for (int64_t k = 0; k < y_num; ++k) base[(n - 1) * s + k] = Store<T>::cvt(Store<T>::rnd(v4[k] * nf, e));
+    for (int64_t k = 0; k < y_num; ++k) base[(n - 2) * s + k] = Store<T>::cvt(Store<T>::rnd(v3[k] * nf, e));
+    for (int64_t j = n - 3; j >= 0; --j) {
+        T* row = base + j * s;
+        for (int64_t k = 0; k < y_num; ++k) {
+            float t = ((float)row[k] + b1 * v3[k]) + b2 * v4[k];
+            row[k] = Store<T>::cvt(Store<T>::rnd(t * nf, e));
+            v4[k] = v3[k];
+            v3[k] = t;
+        }
+    }
+}
+
+template <typename T>
+void bspline_filt_rec_x(Mesh<T>& img, float lambda, float tol, size_t& err) {
+    const int64_t z_num = img.z_num, x_num = img.x_num, y_num = img.y_num;
+    BsplineCoeffs c = bspline_coeffs(lambda, tol, (size_t)x_num);
+    size_t e = 0;
+#pragma omp parallel reduction(+ : e)
+    {
+        std::vector<float> v1(y_num), v2(y_num), v3(y_num), v4(y_num);
+#pragma omp for schedule(static)
+        for (int64_t z = 0; z < z_num; ++z)
+            bspline_outer_axis(img.v.data() + z * x_num * y_num, x_num, y_num, y_num, c, v1, v2, v3, v4, e);
+    }
+    err += e;
+}
+
+template <typename T>
+void bspline_filt_rec_z(Mesh<T>& img, float lambda, float tol, size_t& err) {
+    const int64_t z_num = img.z_num, x_num = img.x_num, y_num = img.y_num;
+    BsplineCoeffs c = bspline_coeffs(lambda, tol, (size_t)z_num);
+    size_t e = 0;
+#pragma omp parallel reduction(+ : e)
+    {
+        std::vector<float> v1(y_num), v2(y_num), v3(y_num), v4(y_num);
+#pragma omp for schedule(static)
+        for (int64_t x = 0; x < x_num; ++x)
+            bspline_outer_axis(img.v.data() + x * y_num, z_num, x_num * y_num, y_num, c, v1, v2, v3, v4, e);
+    }
+    err += e;
+}
+
+/* get_smooth_bspline_3D (ComputeGradient.hpp:173-205) + overflow exceptions */
+/* (ComputeGradient.hpp:409-412, :595-598, :775-778).                        */
+template <typename T>
+void smooth_bspline_3d(Mesh<T>& img, float lambda) {
+    const float tol = 0.0001f;
+    size_t err = 0;
+    if (img.y_num > 2) {
+        bspline_filt_rec_y(img, lambda, tol, err);
+        if (err) throw std::invalid_argument("integer under-/overflow encountered in bspline_filt_rec_y");
+    }
+    if (img.x_num > 2) {
+        bspline_filt_rec_x(img, lambda, tol, err);
+        if (err) throw std::invalid_argument("integer under-/overflow encountered in bspline_filt_rec_x");
+    }
+    if (img.z_num > 2) {
+        bspline_filt_rec_z(img, lambda, tol, err);
+        if (err) throw std::invalid_argument("integer under-/overflow encountered in bspline_filt_rec_z");
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Inverse cubic B-spline [1,4,1]/6 (ComputeGradient.hpp:784-964)            */
+/* ------------------------------------------------------------------------ */
+static inline void inv_bspline_y(Mesh<float>& m) {
+    const int64_t z_num = m.z_num, x_num = m.x_num, y_num = m.y_num;
+    const float a1 = (float)(1.0 / 6.0), a2 = (float)(4.0 / 6.0), a3 = (float)(1.0 / 6.0);
+#pragma omp parallel
+    {
+        std::vector<float> t(y_num);
+#pragma omp for schedule(static)
+        for (int64_t z = 0; z < z_num; ++z) {
+            for (int64_t x = 0; x < x_num; ++x) {
+                float* l = m.v.data() + z * x_num * y_num + x * y_num;
+                for (int64_t k = 0; k < y_num; ++k) t[k] = l[k];
+                l[0] = a2 * t[0];
+                l[0] += (a1 + a3) * t[1];
+                for (int64_t k = 1; k < y_num - 1; ++k) l[k] = a1 * t[k - 1] + a2 * t[k] + a3 * t[k + 1];
+                l[y_num - 1] = (a1 + a3) * t[y_num - 2];
+                l[y_num - 1] += a2 * t[y_num - 1];
+            }
+        }
+    }
+}
+
+/* x (ComputeGradient.hpp:898-964) and z (:828-895) share one form:          */
+/* out[0] = a1*in[1] + a2*in[0] + a3*in[1]; out[j] = a1*in[j-1]+a2*in[j]+a3*in[j+1];
+   out[n-1] = (a1+a3)*in[n-2] + a2*in[n-1].                                  */
+static inline void inv_bspline_outer(float* base, int64_t n, int64_t s, int64_t y_num, std::vector<float>& t1,
+                                     std::vector<float>& t2, std::vector<float>& t3) {
+    const float a1 = (float)(1.0 / 6.0), a2 = (float)(4.0 / 6.0), a3 = (float)(1.0 / 6.0);
+    for (int64_t k = 0; k < y_num; ++k) {
+        t1[k] = base[s + k];
+        t2[k] = base[k];
+    }
+    for (int64_t j = 0; j < n - 1; ++j) {
+        for (int64_t k = 0; k < y_num; ++k) t3[k] = base[(j + 1) * s + k];
+        for (int64_t k = 0; k < y_num; ++k) base[j * s + k] = a1 * t1[k] + a2 * t2[k] + a3 * t3[k];
+        for (int64_t k = 0; k < y_num; ++k) {
+            t1[k] = t2[k];
+            t2[k] = t3[k];
+        }
+    }
+    for (int64_t k = 0; k < y_num; ++k) {
+        float v = (a1 + a3) * t1[k];
+        v += a2 * t2[k];
+        base[(n - 1) * s + k] = v;
+    }
+}
+
+static inline void inv_bspline_x(Mesh<float>& m) {
+    const int64_t z_num = m.z_num, x_num = m.x_num, y_num = m.y_num;
+#pragma omp parallel
+    {
+        std::vector<float> t1(y_num), t2(y_num), t3(y_num);
+#pragma omp for schedule(static)
+        for (int64_t z = 0; z < z_num; ++z) inv_bspline_outer(m.v.data() + z * x_num * y_num, x_num, y_num, y_num, t1, t2, t3);
+    }
+}
+
+static inline void inv_bspline_z(Mesh<float>& m) {
+    const int64_t z_num = m.z_num, x_num = m.x_num, y_num = m.y_num;
+#pragma omp parallel
+    {
+        std::vector<float> t1(y_num), t2(y_num), t3(y_num);
+#pragma omp for schedule(static)
+        for (int64_t x = 0; x < x_num; ++x) inv_bspline_outer(m.v.data() + x * y_num, z_num, x_num * y_num, y_num, t1, t2, t3);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Gradient magnitude + 2x2x2 max downsample (ComputeGradient.hpp:975-1051). */
+/* Central differences in float, squared/summed/sqrt in double, truncated.   */
+/* ------------------------------------------------------------------------ */
+template <typename S>
+static inline S grad_store(double v);
+template <> inline uint16_t grad_store<uint16_t>(double v) { return d2u16(v); }
+template <> inline float grad_store<float>(double v) { return (float)v; }
+
+template <typename S>
+void grad_mag_ds(const Mesh<S>& in, Mesh<S>& grad, float hx, float hy, float hz) {
+    const int64_t z_num = in.z_num, x_num = in.x_num, y_num = in.y_num;
+    const int64_t x_ds = grad.x_num, y_ds = grad.y_num, z_ds = grad.z_num;
+    const int64_t xy = x_num * y_num;
+    const float h2x = 2.0f * hx, h2y = 2.0f * hy, h2z = 2.0f * hz;
+    /* (S - S) is int for uint16 and float for float; then / (2*h) in float */
+    auto term = [](float d, float h2) { return std::pow((double)(d / h2), 2.0); };
+    auto df = [](S a, S b) { return (float)(a - b); };
+#pragma omp parallel
+    {
+        std::vector<S> tmp(y_num);
+#pragma omp for schedule(static)
+        for (int64_t zd = 0; zd < z_ds; ++zd) {
+            for (int64_t z = 2 * zd; z <= std::min(2 * zd + 1, z_num - 1); ++z) {
+                const int64_t zm = z > 0 ? z - 1 : 0, zp = std::min(z + 1, z_num - 1);
+                for (int64_t x = 0; x < x_num; ++x) {
+                    const S* up = in.v.data() + zm * xy + x * y_num;
+                    const S* down = in.v.data() + zp * xy + x * y_num;
+                    const S* left = in.v.data() + z * xy + (x > 0 ? x - 1 : 0) * y_num;
+                    const S* right = in.v.data() + z * xy + std::min(x + 1, x_num - 1) * y_num;
+                    const S* cen = in.v.data() + z * xy + x * y_num;
+                    if (y_num >= 2) {
+                        tmp[0] = grad_store<S>(std::sqrt(term(df(right[0], left[0]), h2x) +
+                                                         term(df(down[0], up[0]), h2z) +
+                                                         term(df(cen[1], cen[0]), h2y)));
+                        int64_t e = y_num - 1;
+                        tmp[e] = grad_store<S>(std::sqrt(term(df(right[e], left[e]), h2x) +
+                                                         term(df(down[e], up[e]), h2z) +
+                                                         term(df(cen[e], cen[e - 1]), h2y)));
+                    } else {
+                        tmp[0] = 0;
+                    }
+                    for (int64_t y = 1; y < y_num - 1; ++y) {
+                        tmp[y] = grad_store<S>(std::sqrt(term(df(right[y], left[y]), h2x) +
+                                                         term(df(down[y], up[y]), h2z) +
+                                                         term(df(cen[y + 1], cen[y - 1]), h2y)));
+                    }
+                    const int64_t z2 = z / 2, x2 = x / 2;
+                    for (int64_t k = 0; k < y_ds; ++k) {
+                        const int64_t ks = std::min(2 * k + 1, y_num - 1);
+                        S& g = grad.v[(size_t)(z2 * x_ds * y_ds + x2 * y_ds + k)];
+                        g = std::max(tmp[2 * k], std::max(tmp[ks], g));
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* 2x2x2 downsample with fixed reduce order (PixelData.hpp:915-967)          */
+/* ------------------------------------------------------------------------ */
+template <typename T, typename S, typename R, typename C>
+void downsample(const Mesh<T>& in, Mesh<S>& out, R reduce, C cop) {
+    const int64_t z_num = in.z_num, x_num = in.x_num, y_num = in.y_num;
+    const int64_t zd = (int64_t)std::ceil(z_num / 2.0), xd = (int64_t)std::ceil(x_num / 2.0), yd = (int64_t)std::ceil(y_num / 2.0);
+    out.resize(yd, xd, zd);
+    const int64_t xy = x_num * y_num;
+#pragma omp parallel for schedule(static)
+    for (int64_t z = 0; z < zd; ++z) {
+        for (int64_t x = 0; x < xd; ++x) {
+            const int64_t shx = std::min(2 * x + 1, x_num - 1), shz = std::min(2 * z + 1, z_num - 1);
+            for (int64_t y = 0; y < yd; ++y) {
+                const int64_t shy = std::min(2 * y + 1, y_num - 1);
+                const T* a = in.v.data();
+                out.v[(size_t)(z * xd * yd + x * yd + y)] = cop(
+                    reduce(reduce(reduce(reduce(reduce(reduce(reduce(a[2 * z * xy + 2 * x * y_num + 2 * y], a[2 * z * xy + 2 * x * y_num + shy]),
+                                                              a[2 * z * xy + shx * y_num + 2 * y]),
+                                                       a[2 * z * xy + shx * y_num + shy]),
+                                                a[shz * xy + 2 * x * y_num + 2 * y]),
+                                         a[shz * xy + 2 * x * y_num + shy]),
+                                  a[shz * xy + shx * y_num + 2 * y]),
+                           a[shz * xy + shx * y_num + shy]));
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Local intensity scale (LocalIntensityScale.hpp:26-494)                    */
+/* ------------------------------------------------------------------------ */
+struct LisWindow {
+    float var_rescale;
+    int win[6];
+    bool active[3];
+};
+
+/* get_window_alt (LocalIntensityScale.hpp:250-303) */
+static inline LisWindow lis_window(const aprh_params& par, int64_t y, int64_t x, int64_t z) {
+    static const double r3[6] = {12.8214, 26.1256, 40.2795, 23.3692, 36.2061, 27.0385};
+    static const double r2[6] = {13.2421, 28.7069, 52.0385, 24.4272, 34.9565, 21.1891};
+    static const double r1[6] = {13.9040, 31.2843, 57.4037, 30.3767, 45.9930, 29.0890};
+    static const int w1[] = {1, 1, 1, 2, 2, 3}, w2[] = {2, 3, 4, 4, 5, 6};
+    int psf_ind = (int)std::max((float)(std::round((double)(par.psfx / par.dx)) - 1), 0.0f);
+    psf_ind = std::min(psf_ind, 5);
+    int wv = std::max(w1[psf_ind], w2[psf_ind]);
+    LisWindow w;
+    for (int i = 0; i < 6; ++i) w.win[i] = 0;
+    int64_t dims[3] = {y, x, z};
+    int nact = 0;
+    for (int a = 0; a < 3; ++a) {
+        w.active[a] = dims[a] > wv;
+        if (w.active[a]) {
+            w.win[a] = w1[psf_ind];
+            w.win[a + 3] = w2[psf_ind];
+            nact++;
+        }
+    }
+    w.var_rescale = (float)(nact == 3 ? r3[psf_ind] : nact == 2 ? r2[psf_ind] : r1[psf_ind]);
+    return w;
+}
+
+/* paddPixels (PixelData.hpp:1070-1176): reflect about the edge pixel */
+template <typename T>
+void padd(const Mesh<T>& in, Mesh<T>& out, int sy, int sx, int sz) {
+    sy = in.y_num > 1 ? std::min<int64_t>(sy, in.y_num - 1) : 0;
+    sx = in.x_num > 1 ? std::min<int64_t>(sx, in.x_num - 1) : 0;
+    sz = in.z_num > 1 ? std::min<int64_t>(sz, in.z_num - 1) : 0;
+    out.resize(in.y_num + 2 * sy, in.x_num + 2 * sx, in.z_num + 2 * sz);
+    for (int64_t j = 0; j < in.z_num; ++j)
+        for (int64_t i = 0; i < in.x_num; ++i)
+            for (int64_t k = 0; k < in.y_num; ++k) out.at(k + sy, i + sx, j + sz) = in.at(k, i, j);
+    if (in.y_num > 1) {
+        for (int64_t j = 0; j < out.z_num; ++j)
+            for (int64_t i = 0; i < out.x_num; ++i) {
+                for (int64_t k = 0; k < sy; ++k) out.at(k, i, j) = out.at(2 * sy - k, i, j);
+                int64_t idx = sy + 2;
+                for (int64_t k = out.y_num - sy; k < out.y_num; ++k) out.at(k, i, j) = out.at(out.y_num - idx++, i, j);
+            }
+    }
+    if (in.x_num > 1) {
+        for (int64_t j = 0; j < out.z_num; ++j) {
+            for (int64_t i = 0; i < sx; ++i)
+                for (int64_t k = 0; k < out.y_num; ++k) out.at(k, i, j) = out.at(k, 2 * sx - i, j);
+            int64_t idx = sx + 2;
+            for (int64_t i = out.x_num - sx; i < out.x_num; ++i) {
+                for (int64_t k = 0; k < out.y_num; ++k) out.at(k, i, j) = out.at(k, out.x_num - idx, j);
+                idx++;
+            }
+        }
+    }
+    if (in.z_num > 1) {
+        for (int64_t j = 0; j < sz; ++j)
+            for (int64_t i = 0; i < out.x_num; ++i)
+                for (int64_t k = 0; k < out.y_num; ++k) out.at(k, i, j) = out.at(k, i, 2 * sz - j);
+        for (int64_t j = out.z_num - sz; j < out.z_num; ++j) {
+            int64_t idx = sz + 2 + j - (out.z_num - sz);
+            for (int64_t i = 0; i < out.x_num; ++i)
+                for (int64_t k = 0; k < out.y_num; ++k) out.at(k, i, j) = out.at(k, i, out.z_num - idx);
+        }
+    }
+}
+
+/* calc_sat_mean_y (LocalIntensityScale.hpp:311-363) */
+static inline void sat_mean_y(Mesh<float>& m, int64_t off) {
+    const int64_t z_num = m.z_num, x_num = m.x_num, y_num = m.y_num;
+    const float divisor = (float)(2 * off + 1);
+#pragma omp parallel
+    {
+        std::vector<float> tv(y_num);
+#pragma omp for schedule(static)
+        for (int64_t j = 0; j < z_num; ++j) {
+            for (int64_t i = 0; i < x_num; ++i) {
+                float* l = m.v.data() + j * x_num * y_num + i * y_num;
+                float t = 0;
+                for (int64_t k = 0; k < y_num; ++k) {
+                    t += l[k];
+                    tv[k] = t;
+                }
+                for (int64_t k = 0; k <= off; ++k) l[k] = 0;
+                for (int64_t k = off + 1; k < y_num; ++k) l[k] = -tv[k - off - 1] / divisor;
+                for (int64_t k = 0; k < y_num - off; ++k) l[k] += tv[k + off] / divisor;
+                float counter = 0;
+                for (int64_t k = y_num - off; k < y_num; ++k) {
+                    counter++;
+                    l[k] *= divisor;
+                    l[k] += tv[y_num - 1];
+                    l[k] = (float)((double)l[k] * (1.0 / (double)(divisor - counter)));
+                }
+                for (int64_t k = 1; k <= off; ++k) l[k] = (float)((double)l[k] * ((double)divisor / ((double)(k + off) + 1.0)));
+                l[0] = (float)((double)l[0] * ((double)divisor / ((double)off + 1.0)));
+            }
+        }
+    }
+}
+
+/* calc_sat_mean_x / _z (LocalIntensityScale.hpp:365-494): running sum in a  */
+/* ring of 2*off+1 lines; shared body over an outer axis of length n.        */
+static inline void sat_mean_outer(float* base, int64_t n, int64_t s, int64_t y_num, int64_t off, std::vector<float>& tv) {
+    const int64_t w = 2 * off + 1;
+    for (int64_t k = 0; k < y_num; ++k) tv[k] = base[k];
+    for (int64_t i = 1; i < w; ++i)
+        for (int64_t k = 0; k < y_num; ++k) tv[i * y_num + k] = base[i * s + k] + tv[(i - 1) * y_num + k];
+    for (int64_t i = 0; i < off + 1; ++i)
+        for (int64_t k = 0; k < y_num; ++k) base[i * s + k] = tv[(i + off) * y_num + k] / (float)(i + off + 1);
+    int64_t cur = off + 1, im = 0;
+    for (int64_t i = off + 1; i < n - off; ++i) {
+        im = (cur + off) % w;
+        int64_t pm = (cur + off - 1) % w;
+        for (int64_t k = 0; k < y_num; ++k) {
+            float t = base[(i + off) * s + k] + tv[pm * y_num + k];
+            base[i * s + k] = (t - tv[im * y_num + k]) / (float)w;
+            tv[im * y_num + k] = t;
+        }
+        cur = (cur + 1) % w;
+    }
+    cur = (cur + off) % w;
+    for (int64_t i = n - off; i < n; ++i) {
+        for (int64_t k = 0; k < y_num; ++k) base[i * s + k] = (tv[im * y_num + k] - tv[cur * y_num + k]) / (float)(n - i + off);
+        cur = (cur + 1) % w;
+    }
+}
+
+static inline void sat_mean_x(Mesh<float>& m, int64_t off) {
+    const int64_t z_num = m.z_num, x_num = m.x_num, y_num = m.y_num;
+#pragma omp parallel
+    {
+        std::vector<float> tv(y_num * (2 * off + 1), 0.0f);
+#pragma omp for schedule(static)
+        for (int64_t j = 0; j < z_num; ++j) sat_mean_outer(m.v.data() + j * x_num * y_num, x_num, y_num, y_num, off, tv);
+    }
+}
+
+static inline void sat_mean_z(Mesh<float>& m, int64_t off) {
+    const int64_t z_num = m.z_num, x_num = m.x_num, y_num = m.y_num;
+#pragma omp parallel
+    {
+        std::vector<float> tv(y_num * (2 * off + 1), 0.0f);
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < x_num; ++i) sat_mean_outer(m.v.data() + i * y_num, z_num, x_num * y_num, y_num, off, tv);
+    }
+}
+
+/* get_local_intensity_scale (LocalIntensityScale.hpp:26-150).               */
+/* In: lis = downsampled smoothed intensities.  Out: lis = sigma,             */
+/* intensity = copy of the input intensities (the Ip_th reference).           */
+static inline void local_intensity_scale(Mesh<float>& lis, Mesh<float>& intensity, const aprh_params& par) {
+    LisWindow w = lis_window(par, lis.y_num, lis.x_num, lis.z_num);
+    const int nact = (int)w.active[0] + (int)w.active[1] + (int)w.active[2];
+    if (par.constant_intensity_scale || nact == 0) {
+        float min_val = 660000;
+        double sum = 0;
+        for (size_t i = 0; i < lis.size(); ++i) {
+            float t = lis.v[i];
+            sum += t;
+            if (t < min_val) min_val = t;
+        }
+        float numel = (float)(lis.y_num * lis.x_num * lis.z_num);
+        float sv = (float)(sum / (double)numel - (double)min_val);
+        intensity = lis;  /* local_scale_temp2 untouched by the reference here; it holds the pre-LIS copy made by the caller */
+        for (auto& v : lis.v) v = sv;
+        return;
+    }
+    Mesh<float> copy = lis;
+    if (par.reflect_bc_lis) {
+        Mesh<float> a, b;
+        padd(copy, a, std::max(w.win[0], w.win[3]), std::max(w.win[1], w.win[4]), std::max(w.win[2], w.win[5]));
+        padd(copy, b, std::max(w.win[0], w.win[3]), std::max(w.win[1], w.win[4]), std::max(w.win[2], w.win[5]));
+        if (w.active[0]) sat_mean_y(a, w.win[0]);
+        if (w.active[1]) sat_mean_x(a, w.win[1]);
+        if (w.active[2]) sat_mean_z(a, w.win[2]);
+        for (size_t i = 0; i < a.size(); ++i) a.v[i] = std::fabs(a.v[i] - b.v[i]);
+        if (w.active[0]) sat_mean_y(a, w.win[3]);
+        if (w.active[1]) sat_mean_x(a, w.win[4]);
+        if (w.active[2]) sat_mean_z(a, w.win[5]);
+        for (size_t i = 0; i < a.size(); ++i) a.v[i] = a.v[i] * w.var_rescale;
+        /* unpaddPixels (PixelData.hpp:1189-1210) */
+        const int64_t py = (a.y_num - copy.y_num) / 2, px = (a.x_num - copy.x_num) / 2, pz = (a.z_num - copy.z_num) / 2;
+        for (int64_t j = 0; j < copy.z_num; ++j)
+            for (int64_t i = 0; i < copy.x_num; ++i)
+                for (int64_t k = 0; k < copy.y_num; ++k) lis.at(k, i, j) = a.at(k + py, i + px, j + pz);
+        intensity = copy;
+    } else {
+        Mesh<float> b = lis;
+        Mesh<float>& a = lis;
+        if (w.active[0]) sat_mean_y(a, w.win[0]);
+        if (w.active[1]) sat_mean_x(a, w.win[1]);
+        if (w.active[2]) sat_mean_z(a, w.win[2]);
+        for (size_t i = 0; i < a.size(); ++i) a.v[i] = std::fabs(a.v[i] - b.v[i]);
+        if (w.active[0]) sat_mean_y(a, w.win[3]);
+        if (w.active[1]) sat_mean_x(a, w.win[4]);
+        if (w.active[2]) sat_mean_z(a, w.win[5]);
+        for (size_t i = 0; i < a.size(); ++i) a.v[i] = a.v[i] * w.var_rescale;
+        intensity = b;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* GenInfo (GenInfo.hpp:33-65)                                               */
+/* ------------------------------------------------------------------------ */
+struct GenInfo {
+    int l_min = 1, l_max = 1;
+    int64_t org[3] = {0, 0, 0};
+    std::vector<int64_t> y_num, x_num, z_num;
+    void init(int64_t y, int64_t x, int64_t z) {
+        org[0] = y; org[1] = x; org[2] = z;
+        int64_t md = std::max(std::max(x, y), z);
+        int lm = std::max(1, (int)std::ceil(std::log2((double)md)));
+        l_min = 1;
+        l_max = lm;
+        y_num.assign(lm + 1, 0);
+        x_num.assign(lm + 1, 0);
+        z_num.assign(lm + 1, 0);
+        for (int l = l_min; l <= l_max; ++l) {
+            double cs = std::round(std::pow(2.0, (double)(l_max - l)));
+            y_num[l] = (int64_t)std::ceil((double)y / cs);
+            x_num[l] = (int64_t)std::ceil((double)x / cs);
+            z_num[l] = (int64_t)std::ceil((double)z / cs);
+        }
+    }
+    /* GenInfo::init_tree (GenInfo.hpp:68-99) */
+    void init_tree(int64_t y, int64_t x, int64_t z) {
+        org[0] = y; org[1] = x; org[2] = z;
+        int64_t md = std::max(std::max(y, x), z);
+        int lm = (int)std::ceil(std::log2((double)md));
+        l_min = 0;
+        l_max = std::max(lm - 1, 0);
+        y_num.assign(l_max + 1, 0);
+        x_num.assign(l_max + 1, 0);
+        z_num.assign(l_max + 1, 0);
+        for (int l = l_min; l <= l_max; ++l) {
+            double cs = std::round(std::pow(2.0, (double)(l_max - l + 1)));
+            y_num[l] = (int64_t)std::ceil((double)y / cs);
+            x_num[l] = (int64_t)std::ceil((double)x / cs);
+            z_num[l] = (int64_t)std::ceil((double)z / cs);
+        }
+    }
+};
+
+/* LinearAccess (LinearAccess.hpp:33-54) */
+struct Linear {
+    std::vector<uint64_t> level_xz_vec, xz_end_vec;
+    std::vector<uint16_t> y_vec;
+    uint64_t total = 0;
+    void init_xz(const GenInfo& g) {
+        uint64_t ct = 1;
+        level_xz_vec.assign(g.l_max + 2, 0);
+        level_xz_vec[0] = 1;
+        for (int i = 0; i <= g.l_max; ++i) {
+            ct += (uint64_t)(g.x_num[i] * g.z_num[i]);
+            level_xz_vec[i + 1] = ct;
+        }
+        xz_end_vec.assign(ct, 0);
+    }
+    uint64_t row_begin(int level, int64_t z, int64_t x, const GenInfo& g) const {
+        return xz_end_vec[level_xz_vec[level] + z * g.x_num[level] + x - 1];
+    }
+    uint64_t row_end(int level, int64_t z, int64_t x, const GenInfo& g) const {
+        return xz_end_vec[level_xz_vec[level] + z * g.x_num[level] + x];
+    }
+};
+
+/* ------------------------------------------------------------------------ */
+/* Pulling scheme (PullingScheme.hpp:16-393)                                 */
+/* ------------------------------------------------------------------------ */
+enum : uint8_t { PS_EMPTY = 0, PS_SEED = 1, PS_BOUNDARY = 2, PS_FILLER = 3, PS_UPSEED = 4, PS_ASC = 8, PS_PROP = 15, PS_ASCN = 16 };
+
+struct PullingScheme {
+    int l_min = 1, l_max = 0;  /* l_max = GenInfo.l_max - 1 */
+    std::vector<Mesh<uint8_t>> pct;
+
+    void init(const GenInfo& g) {
+        l_max = g.l_max - 1;
+        l_min = g.l_min;
+        pct.assign(std::max(l_max + 1, 0), Mesh<uint8_t>());
+        for (int l = l_min; l <= l_max; ++l) {
+            double cs = std::round(std::pow(2.0, (double)(l_max - l + 1)));
+            pct[l].resize((int64_t)std::ceil((double)g.org[0] / cs), (int64_t)std::ceil((double)g.org[1] / cs),
+                          (int64_t)std::ceil((double)g.org[2] / cs), PS_EMPTY);
+        }
+    }
+
+    /* fill (PullingScheme.hpp:140-168) */
+    void fill(int k, const Mesh<float>& in) {
+        auto& m = pct[k].v;
+        const float kf = (float)k;
+        if (k == l_max) {
+            for (size_t i = 0; i < in.size(); ++i) if (in.v[i] >= kf) m[i] = PS_SEED;
+        } else if (k == l_min) {
+            for (size_t i = 0; i < in.size(); ++i) if (in.v[i] <= kf) m[i] = PS_SEED;
+        } else {
+            for (size_t i = 0; i < in.size(); ++i) if (in.v[i] == kf) m[i] = PS_SEED;
+        }
+    }
+
+    static void bounds(int64_t v, int64_t last, int64_t& lo, int64_t& hi) {
+        lo = (v == 0) ? 0 : -1;
+        hi = (v == last) ? 1 : 2;
+    }
+
+    /* set_ascendant_neighbours (PullingScheme.hpp:250-289) */
+    void set_ascendant_neighbours(int level) {
+        Mesh<uint8_t>& m = pct[level];
+        for (int64_t j = 0; j < m.z_num; ++j) {
+            int64_t zl, zh; bounds(j, m.z_num - 1, zl, zh);
+            for (int64_t i = 0; i < m.x_num; ++i) {
+                int64_t xl, xh; bounds(i, m.x_num - 1, xl, xh);
+                for (int64_t k = 0; k < m.y_num; ++k) {
+                    if (m.at(k, i, j) != PS_ASC) continue;
+                    int64_t yl, yh; bounds(k, m.y_num - 1, yl, yh);
+                    for (int64_t a = zl; a < zh; ++a)
+                        for (int64_t b = xl; b < xh; ++b)
+                            for (int64_t c = yl; c < yh; ++c) {
+                                uint8_t& n = m.at(k + c, i + b, j + a);
+                                if (n == PS_EMPTY) n = PS_ASCN;
+                                if (n == PS_SEED) n = PS_PROP;
+                            }
+                }
+            }
+        }
+    }
+
+    /* set_filler (PullingScheme.hpp:291-343) */
+    void set_filler(int level) {
+        Mesh<uint8_t>& m = pct[level];
+        Mesh<uint8_t>& c = pct[level + 1];
+        for (int64_t j = 0; j < m.z_num; ++j) {
+            int64_t cz = (j == m.z_num - 1 && (c.z_num % 2)) ? 1 : 2;
+            for (int64_t i = 0; i < m.x_num; ++i) {
+                int64_t cx = (i == m.x_num - 1 && (c.x_num % 2)) ? 1 : 2;
+                for (int64_t k = 0; k < m.y_num; ++k) {
+                    int64_t cy = (k == m.y_num - 1 && (c.y_num % 2)) ? 1 : 2;
+                    uint8_t s = m.at(k, i, j);
+                    if (s == PS_ASCN || s == PS_PROP) {
+                        for (int64_t a = 2 * j; a < 2 * j + cz; ++a)
+                            for (int64_t b = 2 * i; b < 2 * i + cx; ++b)
+                                for (int64_t d = 2 * k; d < 2 * k + cy; ++d)
+                                    if (c.at(d, b, a) == PS_EMPTY) c.at(d, b, a) = PS_FILLER;
+                    }
+                }
+            }
+        }
+    }
+
+    /* fill_neighbours + fill_parent (PullingScheme.hpp:345-393) */
+    void fill_neighbours(int level) {
+        Mesh<uint8_t>& m = pct[level];
+        for (int64_t j = 0; j < m.z_num; ++j) {
+            int64_t zl, zh; bounds(j, m.z_num - 1, zl, zh);
+            for (int64_t i = 0; i < m.x_num; ++i) {
+                int64_t xl, xh; bounds(i, m.x_num - 1, xl, xh);
+                for (int64_t k = 0; k < m.y_num; ++k) {
+                    uint8_t s = m.at(k, i, j);
+                    if (s == PS_SEED || s == PS_PROP) {
+                        int64_t yl, yh; bounds(k, m.y_num - 1, yl, yh);
+                        for (int64_t a = zl; a < zh; ++a)
+                            for (int64_t b = xl; b < xh; ++b)
+                                for (int64_t c = yl; c < yh; ++c) {
+                                    uint8_t& n = m.at(k + c, i + b, j + a);
+                                    if (n == PS_EMPTY) n = PS_BOUNDARY;
+                                }
+                        fill_parent(j, i, k, m.x_num, m.y_num, level - 1);
+                    } else if (s == PS_ASC) {
+                        fill_parent(j, i, k, m.x_num, m.y_num, level - 1);
+                    }
+                }
+            }
+        }
+    }
+
+    void fill_parent(int64_t j, int64_t i, int64_t k, int64_t x_num, int64_t y_num, int nl) {
+        if (nl >= l_min) {
+            int64_t nx = (x_num + 1) / 2, ny = (y_num + 1) / 2;
+            uint8_t& p = pct[nl].v[(size_t)((j / 2) * nx * ny + (i / 2) * ny + (k / 2))];
+            if (p != PS_SEED) p = PS_ASC;
+        }
+    }
+
+    /* pulling_scheme_main (PullingScheme.hpp:111-132) */
+    void run() {
+        for (int level = l_max; level >= l_min; --level) {
+            if (level != l_max) {
+                set_ascendant_neighbours(level);
+                set_filler(level);
+            }
+            fill_neighbours(level);
+        }
+    }
+};
+
+/* computeLevels + compute_level_for_array + asmlog_2                        */
+/* (LocalParticleCellSet.hpp:128-171). lis becomes the level map (as float). */
+static inline void compute_levels(const Mesh<uint16_t>& grad, Mesh<float>& lis, int max_level, float rel_error, float dx, float dy,
+                                  float dz) {
+    for (size_t i = 0; i < grad.size(); ++i) lis.v[i] = (float)grad.v[i] / lis.v[i];
+    float min_dim = std::min(dy, std::min(dx, dz));
+    float level_factor = (float)(std::pow(2.0, (double)max_level) * (double)min_dim);
+    const float mult = level_factor / rel_error;
+    for (size_t i = 0; i < lis.size(); ++i) {
+        uint32_t u = f2u32(lis.v[i] * mult);
+        lis.v[i] = (float)(u == 0 ? 0u : (uint32_t)(31 - __builtin_clz(u)));
+    }
+}
+
+/* get_local_particle_cell_set (LocalParticleCellSet.hpp:58-91) */
+static inline void local_particle_cell_set(PullingScheme& ps, Mesh<float>& levels) {
+    ps.fill(ps.l_max, levels);
+    Mesh<float> cur = levels, nxt;
+    for (int l = ps.l_max - 1; l >= ps.l_min; --l) {
+        downsample(cur, nxt, [](float a, float b) { return std::max(a, b); }, [](float a) { return a; });
+        ps.fill(l, nxt);
+        std::swap(cur, nxt);
+    }
+}
+
+/* LinearAccess::initialize_linear_structure (LinearAccess.hpp:210-524) */
+static inline void linear_structure(const GenInfo& g, Linear& L, std::vector<Mesh<uint8_t>>& pmap, bool nbhd_opt) {
+    const uint8_t min_type = nbhd_opt ? 1 : 2;
+    const uint8_t UPS = PS_UPSEED;
+    L.init_xz(g);
+    const int lmax = g.l_max, lmin = g.l_min;
+    if (lmax <= 2) {
+        const uint64_t ls = L.level_xz_vec[lmax];
+        uint64_t counter = 0;
+        for (int64_t z = 0; z < g.z_num[lmax]; ++z)
+            for (int64_t x = 0; x < g.x_num[lmax]; ++x) {
+                counter += (uint64_t)g.y_num[lmax];
+                L.xz_end_vec[ls + z * g.x_num[lmax] + x] = counter;
+            }
+        L.y_vec.assign(counter, 0);
+        counter = 0;
+        for (int64_t z = 0; z < g.z_num[lmax]; ++z)
+            for (int64_t x = 0; x < g.x_num[lmax]; ++x)
+                for (int64_t y = 0; y < g.y_num[lmax]; ++y) L.y_vec[counter++] = (uint16_t)y;
+        L.total = counter;
+        return;
+    }
+    /* step 1: equivalence optimisation, levels processed upward in order */
+    for (int level = lmin + 1; level < lmax; ++level) {
+        const int64_t xL = g.x_num[level], zL = g.z_num[level], yL = g.y_num[level];
+        const int64_t xU = g.x_num[level - 1], yU = g.y_num[level - 1];
+        for (int64_t z = 0; z < zL; ++z)
+            for (int64_t x = 0; x < xL; ++x) {
+                const int64_t ods = (x / 2) * yU + (z / 2) * yU * xU, o = x * yL + z * yL * xL;
+                for (int64_t y = 0; y < yU; ++y) {
+                    uint8_t s = pmap[level - 1].v[ods + y];
+                    if (s > 0 && s <= min_type) {
+                        int64_t y2p = std::min(2 * y + 1, yL - 1);
+                        pmap[level].v[o + 2 * y] = UPS;
+                        pmap[level].v[o + y2p] = UPS;
+                    }
+                }
+            }
+    }
+    /* step 2: row counts */
+    for (int level = lmin; level < lmax - 1; ++level) {
+        const int64_t xL = g.x_num[level], zL = g.z_num[level], yL = g.y_num[level];
+        const uint64_t ls = L.level_xz_vec[level];
+        for (int64_t z = 0; z < zL; ++z)
+            for (int64_t x = 0; x < xL; ++x) {
+                const int64_t opc = z * xL + x, opm = yL * opc;
+                uint64_t c = 0;
+                for (int64_t y = 0; y < yL; ++y) {
+                    uint8_t s = pmap[level].v[opm + y];
+                    if (s > min_type && s <= UPS) c++;
+                }
+                L.xz_end_vec[ls + opc] = c;
+            }
+    }
+    const int lm1 = lmax - 1;
+    const int64_t xL = g.x_num[lm1], zL = g.z_num[lm1], yL = g.y_num[lm1], yLm = g.y_num[lm1 + 1];
+    const uint64_t ls1 = L.level_xz_vec[lm1];
+    std::vector<uint64_t> tmax((size_t)(zL * xL), 0);
+    for (int64_t z = 0; z < zL; ++z)
+        for (int64_t x = 0; x < xL; ++x) {
+            const int64_t opc = z * xL + x, opm = yL * opc;
+            uint64_t c = 0, cl = 0;
+            for (int64_t y = 0; y < yL; ++y) {
+                uint8_t s = pmap[lm1].v[opm + y];
+                if (s > min_type && s <= UPS) c++;
+                else if (s > 0 && s <= min_type) {
+                    cl++;
+                    if (2 * y < yLm - 1) cl++;
+                }
+            }
+            L.xz_end_vec[ls1 + opc] = c;
+            tmax[opc] = cl;
+        }
+    const int64_t xLm = g.x_num[lmax], zLm = g.z_num[lmax];
+    const uint64_t lsm = L.level_xz_vec[lmax];
+    for (int64_t z = 0; z < zLm; ++z)
+        for (int64_t x = 0; x < xLm; ++x) L.xz_end_vec[lsm + z * xLm + x] = tmax[(z / 2) * xL + x / 2];
+    std::partial_sum(L.xz_end_vec.begin(), L.xz_end_vec.end(), L.xz_end_vec.begin());
+    L.total = L.xz_end_vec.back();
+    L.y_vec.assign(L.total, 0);
+    for (int level = lmin; level < lmax - 1; ++level) {
+        const int64_t xl = g.x_num[level], zl = g.z_num[level], yl = g.y_num[level];
+        const uint64_t ls = L.level_xz_vec[level];
+        for (int64_t z = 0; z < zl; ++z)
+            for (int64_t x = 0; x < xl; ++x) {
+                const int64_t opc = z * xl + x, opm = yl * opc;
+                uint64_t c = 0, oy = L.xz_end_vec[ls + opc - 1];
+                for (int64_t y = 0; y < yl; ++y) {
+                    uint8_t s = pmap[level].v[opm + y];
+                    if (s > min_type && s <= UPS) L.y_vec[oy + c++] = (uint16_t)y;
+                }
+            }
+    }
+    for (int64_t z = 0; z < zL; ++z)
+        for (int64_t x = 0; x < xL; ++x) {
+            const int64_t opc = z * xL + x, opcm = (z * 2) * xLm + x * 2, opm = yL * opc;
+            uint64_t c = 0, cl = 0;
+            uint64_t oy = L.xz_end_vec[ls1 + opc - 1], oym = L.xz_end_vec[lsm + opcm - 1];
+            for (int64_t y = 0; y < yL; ++y) {
+                uint8_t s = pmap[lm1].v[opm + y];
+                if (s > min_type && s <= UPS) L.y_vec[oy + c++] = (uint16_t)y;
+                else if (s > 0 && s <= min_type) {
+                    L.y_vec[oym + cl++] = (uint16_t)(2 * y);
+                    if (2 * y < yLm - 1) L.y_vec[oym + cl++] = (uint16_t)(2 * y + 1);
+                }
+            }
+        }
+    for (int64_t z = 0; z < zLm; ++z)
+        for (int64_t x = 0; x < xLm; ++x) {
+            if ((z % 2) != 0 || (x % 2) != 0) {
+                const int64_t om = z * xLm + x, omf = (z / 2) * 2 * xLm + (x / 2) * 2;
+                uint64_t bf = L.xz_end_vec[lsm + omf - 1], ef = L.xz_end_vec[lsm + omf], b = L.xz_end_vec[lsm + om - 1];
+                std::copy(L.y_vec.begin() + bf, L.y_vec.begin() + ef, L.y_vec.begin() + b);
+            }
+        }
+}
+
+/* ------------------------------------------------------------------------ */
+/* get_apr (APRConverter.hpp:534-582) for ImageType = uint16_t               */
+/* ------------------------------------------------------------------------ */
+struct AprResult {
+    GenInfo info;
+    Linear lin;
+    float bspline_offset = 0;
+    /* intermediates (for stage-level parity) */
+    Mesh<uint16_t> smoothed;   /* after the three IIR passes            */
+    Mesh<uint16_t> grad;       /* after thresholds (input to levels)     */
+    Mesh<uint16_t> grad_raw;   /* gradient_step.tif equivalent           */
+    Mesh<float> lis_raw;       /* local_intensity_scale_step.tif equiv.  */
+    Mesh<float> lis;           /* after sigma_th rescale                 */
+    Mesh<float> levels;        /* computeLevels output                   */
+    std::vector<Mesh<uint8_t>> pct;  /* final particle-cell tree        */
+};
+
+/* compute_bspline_offset (AutoParameters.hpp:165-196) for uint16 */
+static inline float bspline_offset_u16(const uint16_t* img, size_t n, float lambda) {
+    if (lambda <= 0) return 0;
+    uint16_t mn = 65535, mx = 0;
+    for (size_t i = 0; i < n; ++i) {
+        mn = std::min(mn, img[i]);
+        mx = std::max(mx, img[i]);
+    }
+    float off = (float)((65535 - ((int)mx - (int)mn)) / 2 - (int)mn);
+    return std::max(std::min(off, 100.f), -100.f);
+}
+
+/* The stages after computeL: applyParameters, solveForAPR,                  */
+/* generateDatastructures (APRConverter.hpp:249-363).                        */
+static inline void apr_from_grad_lis(AprResult& r, Mesh<uint16_t>& grad, Mesh<float>& lis, Mesh<float>& intensity,
+                                     const aprh_params& par, bool keep) {
+    /* applyParameters (APRConverter.hpp:249-297) */
+    const float ipth = par.Ip_th + r.bspline_offset;
+    for (size_t i = 0; i < intensity.size(); ++i) if (intensity.v[i] <= ipth) grad.v[i] = 0;
+    for (size_t i = 0; i < grad.size(); ++i) {
+        float s = lis.v[i];
+        if (s < par.sigma_th) {
+            s = (s < par.sigma_th_max) ? 60000.0f : par.sigma_th;
+            lis.v[i] = s;
+        }
+    }
+    for (size_t i = 0; i < grad.size(); ++i) if ((float)grad.v[i] < par.grad_th) grad.v[i] = 0;
+    if (keep) { r.grad = grad; r.lis = lis; }
+    /* solveForAPR (APRConverter.hpp:300-336) */
+    compute_levels(grad, lis, r.info.l_max, par.rel_error, par.dx, par.dy, par.dz);
+    if (keep) r.levels = lis;
+    PullingScheme ps;
+    ps.init(r.info);
+    local_particle_cell_set(ps, lis);
+    ps.run();
+    /* generateDatastructures (APRConverter.hpp:338-363) */
+    linear_structure(r.info, r.lin, ps.pct, par.neighborhood_optimization != 0);
+    if (keep) r.pct = ps.pct;
+}
+
+static inline void get_apr_u16(AprResult& r, const Mesh<uint16_t>& input, const aprh_params& par, bool keep) {
+    r.info.init(input.y_num, input.x_num, input.z_num);
+    const int64_t zd = (int64_t)std::ceil(1.0 * input.z_num / 2.0), xd = (int64_t)std::ceil(1.0 * input.x_num / 2.0),
+                  yd = (int64_t)std::ceil(1.0 * input.y_num / 2.0);
+    Mesh<uint16_t> grad(yd, xd, zd, 0);
+    /* computeL (APRConverter.hpp:194-247) */
+    Mesh<uint16_t> img(input.y_num, input.x_num, input.z_num);
+    r.bspline_offset = bspline_offset_u16(input.v.data(), input.size(), par.lambda);
+    const float off = r.bspline_offset;
+    for (size_t i = 0; i < input.size(); ++i) img.v[i] = f2u16((float)input.v[i] + off);
+    /* ComputeGradient::get_gradient (ComputeGradient.hpp:70-119) */
+    if (par.lambda > 0) smooth_bspline_3d(img, par.lambda);
+    if (keep) r.smoothed = img;
+    grad_mag_ds(img, grad, par.dx, par.dy, par.dz);
+    Mesh<float> lis;
+    downsample(img, lis, [](float a, float b) { return a + b; }, [](float a) { return (float)((double)a / 8.0); });
+    if (par.lambda > 0) {
+        if (img.y_num > 2) inv_bspline_y(lis);
+        if (img.x_num > 2) inv_bspline_x(lis);
+        if (img.z_num > 2) inv_bspline_z(lis);
+    }
+    if (keep) r.grad_raw = grad;
+    Mesh<float> intensity;
+    local_intensity_scale(lis, intensity, par);
+    if (keep) r.lis_raw = lis;
+    apr_from_grad_lis(r, grad, lis, intensity, par, keep);
+}
+
+/* ParticleData::sample_image (ParticleData.hpp:122-127, :221-261) with the  */
+/* mean pyramid of downsamplePyramid (PixelData.hpp:1036-1058).              */
+template <typename ImageType, typename PartType>
+void sample_image(const GenInfo& g, const Linear& L, const Mesh<ImageType>& img, std::vector<PartType>& parts) {
+    std::vector<Mesh<ImageType>> pyr(g.l_max + 1);
+    pyr[g.l_max] = img;
+    for (int level = g.l_max; level > g.l_min; --level)
+        downsample(pyr[level], pyr[level - 1], [](float a, float b) { return a + b; },
+                   [](float a) { return Store<ImageType>::cvt(a / 8.0f); });
+    parts.assign(L.total, 0);
+    for (int level = g.l_min; level <= g.l_max; ++level) {
+#pragma omp parallel for schedule(dynamic)
+        for (int64_t z = 0; z < g.z_num[level]; ++z)
+            for (int64_t x = 0; x < g.x_num[level]; ++x)
+                for (uint64_t i = L.row_begin(level, z, x, g); i < L.row_end(level, z, x, g); ++i)
+                    parts[i] = (PartType)pyr[level].at(L.y_vec[i], x, z);
+    }
+}
+
+}  // namespace oracle
+
+#endif

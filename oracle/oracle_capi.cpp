@@ -1,0 +1,238 @@
+/*
+ * oracle_capi.cpp -- C entry points of the CPU oracle (liboracle.so).
+ *
+ * TEST INFRASTRUCTURE ONLY (see apr_oracle.hpp).  The same symbol set is
+ * exported by oracle/_ref/libref_*.so, built from the reference's own headers
+ * by ref_driver.cpp, so a test can run one check against either library.
+ *
+ * How each part is pinned (tests/test_oracle_*.py):
+ *   IIR y/x/z, inverse B-spline      reference KAT arrays + _ref bit equality
+ *   gradient magnitude + max ds      reference corner KATs + _ref bit equality
+ *                                    + integer closed form floor(sqrt(k)/2)
+ *   SAT means, padding               reference KAT arrays + _ref bit equality
+ *   full get_apr structure, sampling sphere_120 / sphere_diff_dims fixtures
+ *                                    (level/x/y/z/pc images) + _ref equality
+ */
+#include <cstdio>
+#include <cstring>
+#include <exception>
+
+#include "apr_oracle.hpp"
+
+using namespace oracle;
+
+namespace {
+struct Handle {
+    AprResult r;
+    int64_t dims[3];
+};
+
+template <typename T>
+Mesh<T> wrap(const T* p, int64_t y, int64_t x, int64_t z) {
+    Mesh<T> m;
+    m.y_num = y; m.x_num = x; m.z_num = z;
+    m.v.assign(p, p + (size_t)(y * x * z));
+    return m;
+}
+template <typename T>
+void unwrap(const Mesh<T>& m, T* p) {
+    std::memcpy(p, m.v.data(), m.v.size() * sizeof(T));
+}
+void set_err(char* buf, int len, const char* msg) {
+    if (buf && len > 0) {
+        std::snprintf(buf, (size_t)len, "%s", msg);
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int orc_version(void) { return 1; }
+
+/* returns the overflow count of the pass (ComputeGradient.hpp:234-244) */
+uint64_t orc_bspline_u16(uint16_t* img, int64_t y, int64_t x, int64_t z, int axis, float lambda, float tol) {
+    Mesh<uint16_t> m = wrap(img, y, x, z);
+    size_t err = 0;
+    if (axis == 0) bspline_filt_rec_y(m, lambda, tol, err);
+    else if (axis == 1) bspline_filt_rec_x(m, lambda, tol, err);
+    else bspline_filt_rec_z(m, lambda, tol, err);
+    unwrap(m, img);
+    return err;
+}
+
+uint64_t orc_bspline_f32(float* img, int64_t y, int64_t x, int64_t z, int axis, float lambda, float tol) {
+    Mesh<float> m = wrap(img, y, x, z);
+    size_t err = 0;
+    if (axis == 0) bspline_filt_rec_y(m, lambda, tol, err);
+    else if (axis == 1) bspline_filt_rec_x(m, lambda, tol, err);
+    else bspline_filt_rec_z(m, lambda, tol, err);
+    unwrap(m, img);
+    return err;
+}
+
+/* B-spline coefficient bit patterns: out = [b1, b2, norm, k0, minLen, bc1[k0], bc2[k0], bc3[k0], bc4[k0]] */
+int orc_bspline_coeffs(float lambda, float tol, int64_t len, float* out, int cap) {
+    BsplineCoeffs c = bspline_coeffs(lambda, tol, (size_t)len);
+    int need = 5 + 4 * (int)c.k0;
+    if (cap < need) return -need;
+    out[0] = c.b1; out[1] = c.b2; out[2] = c.norm_factor; out[3] = (float)c.k0; out[4] = (float)c.minLen;
+    for (size_t k = 0; k < c.k0; ++k) {
+        out[5 + k] = c.bc1[k];
+        out[5 + c.k0 + k] = c.bc2[k];
+        out[5 + 2 * c.k0 + k] = c.bc3[k];
+        out[5 + 3 * c.k0 + k] = c.bc4[k];
+    }
+    return need;
+}
+
+void orc_inv_bspline_f32(float* img, int64_t y, int64_t x, int64_t z, int axis) {
+    Mesh<float> m = wrap(img, y, x, z);
+    if (axis == 0) inv_bspline_y(m);
+    else if (axis == 1) inv_bspline_x(m);
+    else inv_bspline_z(m);
+    unwrap(m, img);
+}
+
+/* grad must hold ceil(y/2)*ceil(x/2)*ceil(z/2) values, pre-initialised by the caller */
+void orc_grad_mag_ds_u16(const uint16_t* in, uint16_t* grad, int64_t y, int64_t x, int64_t z, float hx, float hy, float hz) {
+    Mesh<uint16_t> m = wrap(in, y, x, z);
+    Mesh<uint16_t> g = wrap(grad, (y + 1) / 2, (x + 1) / 2, (z + 1) / 2);
+    grad_mag_ds(m, g, hx, hy, hz);
+    unwrap(g, grad);
+}
+
+void orc_grad_mag_ds_f32(const float* in, float* grad, int64_t y, int64_t x, int64_t z, float hx, float hy, float hz) {
+    Mesh<float> m = wrap(in, y, x, z);
+    Mesh<float> g = wrap(grad, (y + 1) / 2, (x + 1) / 2, (z + 1) / 2);
+    grad_mag_ds(m, g, hx, hy, hz);
+    unwrap(g, grad);
+}
+
+/* the gradient stage's mean downsample: sum in float, x / 8.0 in double (ComputeGradient.hpp:95-98) */
+void orc_downsample_mean_u16_f32(const uint16_t* in, float* out, int64_t y, int64_t x, int64_t z) {
+    Mesh<uint16_t> m = wrap(in, y, x, z);
+    Mesh<float> o;
+    downsample(m, o, [](float a, float b) { return a + b; }, [](float a) { return (float)((double)a / 8.0); });
+    unwrap(o, out);
+}
+
+/* generic downsample used by tests: op 0 = mean (x/8.0f), 1 = max */
+void orc_downsample_f32(const float* in, float* out, int64_t y, int64_t x, int64_t z, int op) {
+    Mesh<float> m = wrap(in, y, x, z);
+    Mesh<float> o;
+    if (op == 0) downsample(m, o, [](float a, float b) { return a + b; }, [](float a) { return a / 8.0f; });
+    else downsample(m, o, [](float a, float b) { return std::max(a, b); }, [](float a) { return a; });
+    unwrap(o, out);
+}
+
+void orc_sat_mean_f32(float* img, int64_t y, int64_t x, int64_t z, int axis, int64_t offset) {
+    Mesh<float> m = wrap(img, y, x, z);
+    if (axis == 0) sat_mean_y(m, offset);
+    else if (axis == 1) sat_mean_x(m, offset);
+    else sat_mean_z(m, offset);
+    unwrap(m, img);
+}
+
+/* out must hold the padded size; pads are clipped like paddPixels */
+void orc_pad_f32(const float* in, float* out, int64_t y, int64_t x, int64_t z, int sy, int sx, int sz) {
+    Mesh<float> m = wrap(in, y, x, z), o;
+    padd(m, o, sy, sx, sz);
+    unwrap(o, out);
+}
+
+/* lis: in = downsampled smoothed intensities, out = local intensity scale; intensity: out copy */
+void orc_local_intensity_scale(float* lis, float* intensity, int64_t y, int64_t x, int64_t z, const aprh_params* par) {
+    Mesh<float> m = wrap(lis, y, x, z), it;
+    local_intensity_scale(m, it, *par);
+    unwrap(m, lis);
+    if (intensity) unwrap(it, intensity);
+}
+
+void* orc_get_apr_u16(const uint16_t* img, int64_t y, int64_t x, int64_t z, const aprh_params* par, int keep, char* errbuf,
+                      int errlen) {
+    try {
+        Handle* h = new Handle();
+        h->dims[0] = y; h->dims[1] = x; h->dims[2] = z;
+        Mesh<uint16_t> m = wrap(img, y, x, z);
+        get_apr_u16(h->r, m, *par, keep != 0);
+        return h;
+    } catch (std::exception& e) {
+        set_err(errbuf, errlen, e.what());
+        return nullptr;
+    }
+}
+
+/* get_apr_custom_grad_scale (APRConverter.hpp:169-191): structure from given   */
+/* downsampled gradient and local intensity scale; intensity = Ip_th reference  */
+void* orc_apr_from_grad_lis(const uint16_t* grad, const float* lis, const float* intensity, int64_t y, int64_t x, int64_t z,
+                            const aprh_params* par, float bspline_offset) {
+    Handle* h = new Handle();
+    h->dims[0] = y; h->dims[1] = x; h->dims[2] = z;
+    h->r.info.init(y, x, z);
+    h->r.bspline_offset = bspline_offset;
+    const int64_t yd = (y + 1) / 2, xd = (x + 1) / 2, zd = (z + 1) / 2;
+    Mesh<uint16_t> g = wrap(grad, yd, xd, zd);
+    Mesh<float> l = wrap(lis, yd, xd, zd);
+    Mesh<float> it = wrap(intensity, yd, xd, zd);
+    apr_from_grad_lis(h->r, g, l, it, *par, true);
+    return h;
+}
+
+void orc_apr_info(void* hp, int64_t* out) {
+    Handle* h = (Handle*)hp;
+    out[0] = h->r.info.l_min;
+    out[1] = h->r.info.l_max;
+    out[2] = (int64_t)h->r.lin.total;
+    out[3] = (int64_t)h->r.lin.xz_end_vec.size();
+    out[4] = (int64_t)h->r.lin.level_xz_vec.size();
+    std::memcpy(&out[5], &h->r.bspline_offset, sizeof(float));
+}
+
+void orc_apr_get(void* hp, uint64_t* level_xz, uint64_t* xz_end, uint16_t* y_vec) {
+    Handle* h = (Handle*)hp;
+    std::memcpy(level_xz, h->r.lin.level_xz_vec.data(), h->r.lin.level_xz_vec.size() * 8);
+    std::memcpy(xz_end, h->r.lin.xz_end_vec.data(), h->r.lin.xz_end_vec.size() * 8);
+    std::memcpy(y_vec, h->r.lin.y_vec.data(), h->r.lin.y_vec.size() * 2);
+}
+
+/* which: 0 smoothed(u16,N) 1 grad_raw(u16,Nd) 2 lis_raw(f32,Nd) 3 grad(u16,Nd) 4 lis(f32,Nd) 5 levels(f32,Nd) */
+int orc_apr_intermediate(void* hp, int which, void* dst) {
+    Handle* h = (Handle*)hp;
+    switch (which) {
+        case 0: if (h->r.smoothed.size() == 0) return -1; unwrap(h->r.smoothed, (uint16_t*)dst); return 0;
+        case 1: if (h->r.grad_raw.size() == 0) return -1; unwrap(h->r.grad_raw, (uint16_t*)dst); return 0;
+        case 2: if (h->r.lis_raw.size() == 0) return -1; unwrap(h->r.lis_raw, (float*)dst); return 0;
+        case 3: if (h->r.grad.size() == 0) return -1; unwrap(h->r.grad, (uint16_t*)dst); return 0;
+        case 4: if (h->r.lis.size() == 0) return -1; unwrap(h->r.lis, (float*)dst); return 0;
+        case 5: if (h->r.levels.size() == 0) return -1; unwrap(h->r.levels, (float*)dst); return 0;
+        default: return -1;
+    }
+}
+
+/* final particle-cell tree level l (u8, dims of level l) */
+int orc_apr_pct(void* hp, int level, uint8_t* dst) {
+    Handle* h = (Handle*)hp;
+    if (level < 0 || level >= (int)h->r.pct.size() || h->r.pct[level].size() == 0) return -1;
+    unwrap(h->r.pct[level], dst);
+    return 0;
+}
+
+void orc_sample_image_u16(void* hp, const uint16_t* img, uint16_t* parts) {
+    Handle* h = (Handle*)hp;
+    Mesh<uint16_t> m = wrap(img, h->dims[0], h->dims[1], h->dims[2]);
+    std::vector<uint16_t> p;
+    sample_image(h->r.info, h->r.lin, m, p);
+    std::memcpy(parts, p.data(), p.size() * 2);
+}
+
+void orc_apr_free(void* hp) { delete (Handle*)hp; }
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+}  // extern "C"

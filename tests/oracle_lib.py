@@ -1,0 +1,150 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) and of the reference
+build (oracle/_ref/libref_ieee.so, libref_rel.so) -- both export the same C
+symbols (oracle/oracle_capi.cpp, oracle/ref_driver.cpp).
+
+Test infrastructure: imported only by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+
+u16p = np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+i64 = C.c_int64
+
+
+class AprhParams(C.Structure):
+    """mirror of aprh_params (include/aprh_params.h)"""
+    _fields_ = [(n, C.c_float) for n in ("dx", "dy", "dz", "psfx", "psfy", "psfz", "Ip_th", "lambda_", "rel_error",
+                                         "sigma_th", "sigma_th_max", "grad_th")] + \
+               [(n, C.c_int) for n in ("reflect_bc_lis", "neighborhood_optimization", "constant_intensity_scale")]
+
+
+def params(**kw):
+    p = AprhParams(1, 1, 1, 2, 2, 2, 0, 3.0, 0.1, 0, 0, 0, 1, 1, 0)
+    for k, v in kw.items():
+        setattr(p, "lambda_" if k == "lambda" else k, v)
+    return p
+
+
+def build(ref=True):
+    import subprocess
+    target = "all" if ref else "liboracle.so"
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR, target], check=True)
+
+
+def _bind(lib):
+    lib.orc_bspline_u16.argtypes = [u16p, i64, i64, i64, C.c_int, C.c_float, C.c_float]
+    lib.orc_bspline_u16.restype = C.c_uint64
+    lib.orc_bspline_f32.argtypes = [f32p, i64, i64, i64, C.c_int, C.c_float, C.c_float]
+    lib.orc_bspline_f32.restype = C.c_uint64
+    lib.orc_bspline_coeffs.argtypes = [C.c_float, C.c_float, i64, f32p, C.c_int]
+    lib.orc_inv_bspline_f32.argtypes = [f32p, i64, i64, i64, C.c_int]
+    lib.orc_grad_mag_ds_u16.argtypes = [u16p, u16p, i64, i64, i64, C.c_float, C.c_float, C.c_float]
+    lib.orc_grad_mag_ds_f32.argtypes = [f32p, f32p, i64, i64, i64, C.c_float, C.c_float, C.c_float]
+    lib.orc_downsample_mean_u16_f32.argtypes = [u16p, f32p, i64, i64, i64]
+    lib.orc_downsample_f32.argtypes = [f32p, f32p, i64, i64, i64, C.c_int]
+    lib.orc_sat_mean_f32.argtypes = [f32p, i64, i64, i64, C.c_int, i64]
+    lib.orc_pad_f32.argtypes = [f32p, f32p, i64, i64, i64, C.c_int, C.c_int, C.c_int]
+    lib.orc_local_intensity_scale.argtypes = [f32p, C.c_void_p, i64, i64, i64, C.POINTER(AprhParams)]
+    lib.orc_get_apr_u16.argtypes = [u16p, i64, i64, i64, C.POINTER(AprhParams), C.c_int, C.c_char_p, C.c_int]
+    lib.orc_get_apr_u16.restype = C.c_void_p
+    lib.orc_apr_from_grad_lis.argtypes = [u16p, f32p, f32p, i64, i64, i64, C.POINTER(AprhParams), C.c_float]
+    lib.orc_apr_from_grad_lis.restype = C.c_void_p
+    lib.orc_apr_info.argtypes = [C.c_void_p, np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")]
+    lib.orc_apr_get.argtypes = [C.c_void_p, u64p, u64p, u16p]
+    lib.orc_apr_intermediate.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+    lib.orc_apr_intermediate.restype = C.c_int
+    lib.orc_apr_pct.argtypes = [C.c_void_p, C.c_int, u8p]
+    lib.orc_apr_pct.restype = C.c_int
+    lib.orc_sample_image_u16.argtypes = [C.c_void_p, u16p, u16p]
+    lib.orc_apr_free.argtypes = [C.c_void_p]
+    lib.orc_num_threads.restype = C.c_int
+    return lib
+
+
+_LIBS = {}
+
+
+def load(which="oracle"):
+    """which: 'oracle' (restatement), 'ref' (reference, IEEE build), 'ref_rel' (reference, Release flags)"""
+    if which not in _LIBS:
+        path = {"oracle": os.path.join(ORACLE_DIR, "liboracle.so"),
+                "ref": os.path.join(ORACLE_DIR, "_ref", "libref_ieee.so"),
+                "ref_rel": os.path.join(ORACLE_DIR, "_ref", "libref_rel.so")}[which]
+        if not os.path.exists(path):
+            return None
+        _LIBS[which] = _bind(C.CDLL(path))
+    return _LIBS[which]
+
+
+class OracleAPR:
+    """Result of get_apr on one of the CPU libraries."""
+
+    def __init__(self, lib, handle, dims):
+        self.lib, self.h, self.dims = lib, handle, dims
+        info = np.zeros(6, np.int64)
+        lib.orc_apr_info(handle, info)
+        self.l_min, self.l_max, self.total = int(info[0]), int(info[1]), int(info[2])
+        self.n_xz, self.n_lxz = int(info[3]), int(info[4])
+        self.bspline_offset = float(np.frombuffer(info[5:6].tobytes()[:4], np.float32)[0])
+        self.level_xz_vec = np.zeros(self.n_lxz, np.uint64)
+        self.xz_end_vec = np.zeros(self.n_xz, np.uint64)
+        self.y_vec = np.zeros(max(self.total, 1), np.uint16)
+        lib.orc_apr_get(handle, self.level_xz_vec, self.xz_end_vec, self.y_vec)
+        self.y_vec = self.y_vec[:self.total]
+
+    def intermediate(self, which):
+        y, x, z = self.dims
+        yd, xd, zd = (y + 1) // 2, (x + 1) // 2, (z + 1) // 2
+        shapes = {0: ((z, x, y), np.uint16), 1: ((zd, xd, yd), np.uint16), 2: ((zd, xd, yd), np.float32),
+                  3: ((zd, xd, yd), np.uint16), 4: ((zd, xd, yd), np.float32), 5: ((zd, xd, yd), np.float32)}
+        shp, dt = shapes[which]
+        out = np.zeros(shp, dt)
+        if self.lib.orc_apr_intermediate(self.h, which, out.ctypes.data) != 0:
+            return None
+        return out
+
+    def pct(self, level, shape):
+        out = np.zeros(shape, np.uint8)
+        if self.lib.orc_apr_pct(self.h, level, out.reshape(-1)) != 0:
+            return None
+        return out
+
+    def sample(self, img):
+        parts = np.zeros(max(self.total, 1), np.uint16)
+        self.lib.orc_sample_image_u16(self.h, np.ascontiguousarray(img).reshape(-1), parts)
+        return parts[:self.total]
+
+    def __del__(self):
+        try:
+            self.lib.orc_apr_free(self.h)
+        except Exception:
+            pass
+
+
+def get_apr(img_zxy, par, which="oracle", keep=True):
+    """img_zxy: numpy uint16 array of shape (z, x, y) (y fastest)."""
+    lib = load(which)
+    z, x, y = img_zxy.shape
+    err = C.create_string_buffer(512)
+    h = lib.orc_get_apr_u16(np.ascontiguousarray(img_zxy).reshape(-1), y, x, z, C.byref(par), int(keep), err, 512)
+    if not h:
+        raise ValueError(err.value.decode())
+    return OracleAPR(lib, h, (y, x, z))
+
+
+def apr_from_grad_lis(grad, lis, intensity, dims, par, offset, which="oracle"):
+    lib = load(which)
+    y, x, z = dims
+    h = lib.orc_apr_from_grad_lis(np.ascontiguousarray(grad, np.uint16).reshape(-1),
+                                  np.ascontiguousarray(lis, np.float32).reshape(-1),
+                                  np.ascontiguousarray(intensity, np.float32).reshape(-1), y, x, z, C.byref(par), offset)
+    return OracleAPR(lib, h, dims)

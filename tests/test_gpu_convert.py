@@ -1,0 +1,93 @@
+"""GPU parity of the conversion path (get_apr + sample_image) against the CPU
+oracle, through the C-ABI.  Bit-exact for every integer / index result and for
+the float stage outputs (the kernels keep the reference's FP32 order)."""
+import numpy as np
+import pytest
+
+import libapr_amd as A
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # (z, x, y), n_blobs, seed
+    ((32, 64, 64), 20, 0),
+    ((61, 63, 65), 30, 1),
+    ((3, 37, 200), 10, 2),
+    ((128, 256, 256), 500, 0),   # the 256x256x128 conversion workload
+]
+
+
+def _par(O):
+    return O.params(rel_error=0.1, sigma_th=50, Ip_th=120, grad_th=10)
+
+
+def _apar():
+    return A.APRParameters(rel_error=0.1, sigma_th=50, Ip_th=120, grad_th=10)
+
+
+@pytest.mark.parametrize("shape,nb,seed", CASES)
+def test_get_apr_matches_oracle(oracle, ctx, shape, nb, seed):
+    img = A.synth_blobs(shape, seed, nb)
+    ref = oracle.get_apr(img, _par(oracle), "oracle")
+    conv = A.APRConverter(ctx)
+    conv.par = _apar()
+    apr = conv.get_apr(img)
+    print(f"case {shape} P={apr.total} oracle P={ref.total}", flush=True)
+    names = ["smoothed", "grad_raw", "lis_raw", "grad", "lis", "levels"]
+    bad = []
+    for w, nm in enumerate(names):
+        g, o = apr.intermediate(w), ref.intermediate(w)
+        nmis = int((g != o).sum())
+        if nmis:
+            idx = np.argwhere(g != o)[:3]
+            bad.append(f"{nm}: {nmis}/{g.size} e.g. {idx.tolist()} gpu={g[tuple(idx[0])]} oracle={o[tuple(idx[0])]}")
+    assert not bad, "\n".join(bad)
+    assert apr.total == ref.total
+    lxz, xze, yv = apr.structure()
+    assert np.array_equal(lxz, ref.level_xz_vec)
+    assert np.array_equal(xze, ref.xz_end_vec)
+    assert np.array_equal(yv, ref.y_vec)
+    parts = A.ParticleData().sample_image(apr, img)
+    assert np.array_equal(parts, ref.sample(img))
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("dtype", [np.uint16, np.float32])
+def test_bspline_axis(oracle, ctx, axis, dtype):
+    rng = np.random.default_rng(axis)
+    img = (rng.random((9, 23, 47)) * 2000 + 200).astype(dtype)
+    g = img.copy()
+    A.bspline_filt_rec(g, axis, 3.0, 1e-4, ctx)
+    o = img.copy()
+    lib = oracle.load("oracle")
+    z, x, y = img.shape
+    (lib.orc_bspline_u16 if dtype == np.uint16 else lib.orc_bspline_f32)(o.reshape(-1), y, x, z, axis, 3.0, 1e-4)
+    assert np.array_equal(g.view(np.uint8), o.view(np.uint8))
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("off", [1, 2, 3, 6])
+def test_sat_mean_axis(oracle, ctx, axis, off):
+    rng = np.random.default_rng(7 + off)
+    img = (rng.random((19, 21, 23)) * 100).astype(np.float32)
+    g = img.copy()
+    A.sat_mean(g, axis, off, ctx)
+    o = img.copy()
+    z, x, y = img.shape
+    oracle.load("oracle").orc_sat_mean_f32(o.reshape(-1), y, x, z, axis, off)
+    assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
+
+
+def test_gradient_sqrt_exact(oracle, ctx):
+    # extreme differences exercise the double sqrt + truncation
+    rng = np.random.default_rng(3)
+    img = rng.integers(0, 65535, size=(8, 16, 40), dtype=np.uint16)
+    g, m = A.gradient_ds(img, ctx=ctx)
+    z, x, y = img.shape
+    go = np.zeros_like(g)
+    mo = np.zeros_like(m)
+    lib = oracle.load("oracle")
+    lib.orc_grad_mag_ds_u16(img.reshape(-1), go.reshape(-1), y, x, z, 1.0, 1.0, 1.0)
+    lib.orc_downsample_mean_u16_f32(img.reshape(-1), mo.reshape(-1), y, x, z)
+    assert np.array_equal(g, go)
+    assert np.array_equal(m.view(np.uint32), mo.view(np.uint32))
