@@ -58,11 +58,13 @@ int aprh_context_destroy(aprh_context* ctx);
    inside each aprh_apr for inspection; replaces APRParameters::output_steps
    (APRConverter.hpp:231-244) */
 int aprh_context_set_keep_intermediates(aprh_context* ctx, int keep);
-/* per-stage device time of the last get_apr / sample (ms), stage order:
-   offset, smooth_bspline, gradient, local_intensity_scale, solve_for_apr,
-   generate_data_structures, sample  (APRConverter timers, APRConverter.hpp:61-65) */
+/* per-stage device time (ms, HIP events on the context stream) of the last get_apr /
+   sample_image; replaces APRConverter's timers (APRConverter.hpp:61-65).  Stages are
+   named by aprh_stage_name(i), i < aprh_stage_count(). */
 int aprh_context_set_timing(aprh_context* ctx, int on);
-int aprh_context_stage_times(aprh_context* ctx, float* ms7);
+int aprh_context_stage_times(aprh_context* ctx, float* ms, int n);
+int aprh_stage_count(void);
+const char* aprh_stage_name(int i);
 int aprh_context_release_cache(aprh_context* ctx);
 
 /* ---- conversion ---- */

@@ -32,8 +32,6 @@ APRH_ERR_UNSUPPORTED = 4
 APRH_ERR_OUT_OF_MEMORY = 5
 APRH_ERR_HIP = 6
 
-STAGES = ("offset", "smooth_bspline", "gradient", "local_intensity_scale", "solve_for_apr",
-          "generate_data_structures", "sample")
 
 
 class AprhError(RuntimeError):
@@ -77,7 +75,9 @@ def lib():
     L.aprh_context_destroy.argtypes = [vp]
     L.aprh_context_set_keep_intermediates.argtypes = [vp, C.c_int]
     L.aprh_context_set_timing.argtypes = [vp, C.c_int]
-    L.aprh_context_stage_times.argtypes = [vp, C.POINTER(C.c_float)]
+    L.aprh_context_stage_times.argtypes = [vp, C.POINTER(C.c_float), C.c_int]
+    L.aprh_stage_name.restype = C.c_char_p
+    L.aprh_stage_name.argtypes = [C.c_int]
     L.aprh_context_release_cache.argtypes = [vp]
     L.aprh_get_apr_u16.argtypes = [vp, vp, i64, i64, i64, C.POINTER(AprhParams), C.POINTER(vp)]
     L.aprh_apr_from_grad_lis.argtypes = [vp, vp, vp, vp, i64, i64, i64, C.POINTER(AprhParams), C.c_float, C.POINTER(vp)]
@@ -168,9 +168,10 @@ class Context:
         _check(lib().aprh_context_set_timing(self.h, int(on)))
 
     def stage_times(self):
-        ms = (C.c_float * 7)()
-        _check(lib().aprh_context_stage_times(self.h, ms))
-        return dict(zip(STAGES, list(ms)))
+        n = lib().aprh_stage_count()
+        ms = (C.c_float * n)()
+        _check(lib().aprh_context_stage_times(self.h, ms, n))
+        return {lib().aprh_stage_name(i).decode(): ms[i] for i in range(n)}
 
     def release_cache(self):
         _check(lib().aprh_context_release_cache(self.h))

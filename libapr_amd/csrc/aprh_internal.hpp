@@ -94,8 +94,21 @@ private:
     size_t bytes_ = 0;
 };
 
-/* timings of the conversion stages, named after APRConverter's timers */
-enum Stage { ST_OFFSET = 0, ST_SMOOTH, ST_GRADIENT, ST_LIS, ST_SOLVE, ST_STRUCTURE, ST_SAMPLE, ST_COUNT };
+/* timed stages of get_apr / sample_image (APRConverter's timers, split per kernel group) */
+enum Stage {
+    ST_OFFSET = 0,   /* min/max + bspline offset               */
+    ST_SMOOTH_Y,     /* bspline_filt_rec_y (offset copy fused)  */
+    ST_SMOOTH_X,     /* bspline_filt_rec_x                      */
+    ST_SMOOTH_Z,     /* bspline_filt_rec_z                      */
+    ST_GRADIENT,     /* gradient magnitude + max/mean ds        */
+    ST_INV_BSPLINE,  /* calc_inv_bspline_y/x/z                  */
+    ST_LIS,          /* local intensity scale                   */
+    ST_LEVELS,       /* applyParameters + computeLevels         */
+    ST_PULLING,      /* level pyramid + pulling scheme          */
+    ST_STRUCTURE,    /* linear structure                        */
+    ST_SAMPLE,       /* sample_image                            */
+    ST_COUNT
+};
 
 }  // namespace aprh
 
@@ -106,6 +119,7 @@ struct aprh_context {
     int keep_intermediates = 0;
     int timing = 0;
     hipEvent_t ev[aprh::ST_COUNT + 1] = {};
+    bool ev_marked[aprh::ST_COUNT + 1] = {};
     float stage_ms[aprh::ST_COUNT] = {};
     int* d_flags = nullptr;   /* small device scratch: overflow counters etc. */
 };
@@ -191,6 +205,17 @@ void synth_device(aprh_context* ctx, uint16_t* out, int64_t y, int64_t x, int64_
                   float smax, float amin, float amax, float background, int noise);
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+/* record the start of stage s (the end of the previous one) when timing is on */
+inline void mark(aprh_context* ctx, int s) {
+    if (ctx->timing) {
+        HIP_CHECK(hipEventRecord(ctx->ev[s], ctx->stream));
+        ctx->ev_marked[s] = true;
+    }
+}
+/* convert marked event pairs [first, last) into stage times */
+void collect_times(aprh_context* ctx, int first, int last);
+void reset_times(aprh_context* ctx, int first, int last);
 
 void set_last_error(const std::string& s);
 

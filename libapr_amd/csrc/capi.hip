@@ -58,21 +58,6 @@ void check_dims(int64_t y, int64_t x, int64_t z) {
     if (y > 65535 || x > 65535 || z > 65535) throw Error(APRH_ERR_UNSUPPORTED, "dimensions above 65535 (u16 y_vec / iterator limits)");
 }
 
-struct StageTimer {
-    aprh_context* ctx;
-    explicit StageTimer(aprh_context* c) : ctx(c) {
-        for (int i = 0; i < ST_COUNT; ++i) ctx->stage_ms[i] = 0.0f;
-    }
-    void mark(int i) {
-        if (ctx->timing) HIP_CHECK(hipEventRecord(ctx->ev[i], ctx->stream));
-    }
-    void collect(int first, int last) {
-        if (!ctx->timing) return;
-        HIP_CHECK(hipEventSynchronize(ctx->ev[last]));
-        for (int i = first; i < last; ++i) HIP_CHECK(hipEventElapsedTime(&ctx->stage_ms[i], ctx->ev[i], ctx->ev[i + 1]));
-    }
-};
-
 float level_mult(const aprh_params& par, int l_max) {
     /* computeLevels (LocalParticleCellSet.hpp:128-147) */
     const float min_dim = std::min(par.dy, std::min(par.dx, par.dz));
@@ -88,7 +73,7 @@ void keep_copy(aprh_context* ctx, Buf& dst, const void* src, size_t bytes) {
 /* levels + pulling scheme + structure from a downsampled gradient, an unpadded
    scale (A with zero pads, var_rescale 1) or the padded LIS array */
 void solve(aprh_context* ctx, aprh_apr& a, const aprh_params& par, const float* A, const LisParams& lp, const float* inten,
-           uint16_t* grad, const float* d_off, StageTimer& tm) {
+           uint16_t* grad, const float* d_off) {
     const int64_t yd = cdiv(a.org[0], 2), xd = cdiv(a.org[1], 2), zd = cdiv(a.org[2], 2);
     const size_t nd = (size_t)(yd * xd * zd);
     Buf level(ctx->pool, nd);
@@ -111,27 +96,25 @@ void solve(aprh_context* ctx, aprh_apr& a, const aprh_params& par, const float* 
         a.levels.reset(ctx->pool, nd * 4);
         launch_u8_to_f32(ctx, level.as<uint8_t>(), a.levels.as<float>(), nd);
     }
-    tm.mark(ST_SOLVE);
     build_structure(ctx, a, level.as<uint8_t>(), par);
-    tm.mark(ST_STRUCTURE);
 }
 
 void get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x, int64_t z, const aprh_params& par, aprh_apr* a) {
-    StageTimer tm(ctx);
+    reset_times(ctx, ST_OFFSET, ST_SAMPLE);
     const size_t n = (size_t)(y * x * z);
     const int64_t yd = cdiv(y, 2), xd = cdiv(x, 2), zd = cdiv(z, 2);
     const size_t nd = (size_t)(yd * xd * zd);
     hipStream_t st = ctx->stream;
     DevIn in(ctx, image, n * 2);
     geninfo_init(*a, y, x, z);
-    tm.mark(ST_OFFSET);
+    mark(ctx, ST_OFFSET);
     float* d_off = (float*)ctx->d_flags;
     uint32_t* d_mm = (uint32_t*)ctx->d_flags + 2;
     int* d_err = ctx->d_flags + 4;
     HIP_CHECK(hipMemsetAsync(d_err, 0, 3 * sizeof(int), st));
     /* computeL: bspline offset (APRConverter.hpp:213-224) */
     launch_minmax_offset(ctx, (const uint16_t*)in.ptr, n, par.lambda, d_off, d_mm);
-    tm.mark(ST_SMOOTH);
+    mark(ctx, ST_SMOOTH_Y);
     /* smoothing (ComputeGradient.hpp:173-205) with the offset copy fused into the first pass */
     Buf img(ctx->pool, n * 2);
     uint16_t* I = img.as<uint16_t>();
@@ -141,9 +124,11 @@ void get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x,
     } else {
         launch_offset_copy(ctx, (const uint16_t*)in.ptr, I, n, d_off);
     }
+    mark(ctx, ST_SMOOTH_X);
     if (par.lambda > 0 && x > 2) launch_iir_outer<uint16_t>(ctx, I, y, x, z, 1, iir_coeffs(par.lambda, tol, x), d_err + 1);
+    mark(ctx, ST_SMOOTH_Z);
     if (par.lambda > 0 && z > 2) launch_iir_outer<uint16_t>(ctx, I, y, x, z, 2, iir_coeffs(par.lambda, tol, z), d_err + 2);
-    tm.mark(ST_GRADIENT);
+    mark(ctx, ST_GRADIENT);
     /* gradient magnitude + 2x2x2 max (ComputeGradient.hpp:92-93), mean downsample and inverse B-spline (:95-118) */
     Buf grad(ctx->pool, nd * 2);
     Buf mean(ctx->pool, nd * 4);
@@ -156,6 +141,7 @@ void get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x,
         keep_copy(ctx, a->grad_raw, grad.ptr(), nd * 2);
     }
     img.release();
+    mark(ctx, ST_INV_BSPLINE);
     if (par.lambda > 0) {
         Buf tmp(ctx->pool, nd * 4);
         launch_inv_bspline(ctx, mean.as<float>(), tmp.as<float>(), yd, xd, zd, y > 2, x > 2, z > 2);
@@ -167,7 +153,7 @@ void get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x,
             throw Error(APRH_ERR_OVERFLOW, std::string("integer under-/overflow encountered in ComputeGradient::bspline_filt_rec_") +
                                                axis_name[i] +
                                                " - try squashing the input image to a narrower range or use APRConverter<float>");
-    tm.mark(ST_LIS);
+    mark(ctx, ST_LIS);
     /* local intensity scale (LocalIntensityScale.hpp:26-150) */
     LisParams lp = lis_params(par, yd, xd, zd);
     const int nact = (int)lp.active[0] + (int)lp.active[1] + (int)lp.active[2];
@@ -187,10 +173,12 @@ void get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x,
         Ap = A.as<float>();
     }
     /* applyParameters + solveForAPR + generateDatastructures */
-    solve(ctx, *a, par, Ap, lpa, mean.as<float>(), grad.as<uint16_t>(), d_off, tm);
+    mark(ctx, ST_LEVELS);
+    solve(ctx, *a, par, Ap, lpa, mean.as<float>(), grad.as<uint16_t>(), d_off);
+    mark(ctx, ST_SAMPLE);
     HIP_CHECK(hipMemcpyAsync(&a->bspline_offset, d_off, 4, hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
-    tm.collect(ST_OFFSET, ST_STRUCTURE);
+    collect_times(ctx, ST_OFFSET, ST_SAMPLE);
 }
 
 }  // namespace
@@ -235,8 +223,7 @@ int aprh_apr_from_grad_lis(aprh_context* ctx, const uint16_t* grad_ds, const flo
         HIP_CHECK(hipMemcpyAsync(d_off, &bspline_offset, 4, hipMemcpyHostToDevice, ctx->stream));
         LisParams lp{};
         lp.var_rescale = 1.0f;
-        StageTimer tm(ctx);
-        solve(ctx, *a, *par, (const float*)l.ptr, lp, (const float*)it.ptr, gcopy.as<uint16_t>(), d_off, tm);
+        solve(ctx, *a, *par, (const float*)l.ptr, lp, (const float*)it.ptr, gcopy.as<uint16_t>(), d_off);
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
     } catch (...) {
         (void)hipStreamSynchronize(ctx->stream);
@@ -331,14 +318,14 @@ int aprh_sample_image_u16(aprh_context* ctx, const aprh_apr* a, const uint16_t* 
     check_ctx(ctx);
     if (!a || !image || !parts) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
     const size_t n = (size_t)(a->org[0] * a->org[1] * a->org[2]);
-    StageTimer tm(ctx);
-    tm.mark(ST_SAMPLE);
+    reset_times(ctx, ST_SAMPLE, ST_COUNT);
     DevIn in(ctx, image, n * 2);
     DevOut out(ctx, parts, std::max<uint64_t>(a->total, 1) * 2);
+    mark(ctx, ST_SAMPLE);
     launch_sample_u16(ctx, *a, (const uint16_t*)in.ptr, (uint16_t*)out.ptr);
-    tm.mark(ST_COUNT);
+    mark(ctx, ST_COUNT);
     out.finish(ctx);
-    tm.collect(ST_SAMPLE, ST_COUNT);
+    collect_times(ctx, ST_SAMPLE, ST_COUNT);
     APRH_CATCH
 }
 

@@ -251,6 +251,19 @@ void synth_device(aprh_context* ctx, uint16_t* out, int64_t y, int64_t x, int64_
 using namespace aprh;
 
 namespace aprh {
+void reset_times(aprh_context* ctx, int first, int last) {
+    for (int i = first; i <= last; ++i) ctx->ev_marked[i] = false;
+    for (int i = first; i < last; ++i) ctx->stage_ms[i] = 0.0f;
+}
+void collect_times(aprh_context* ctx, int first, int last) {
+    if (!ctx->timing) return;
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    for (int i = first; i < last; ++i) {
+        int j = i + 1;
+        while (j <= last && !ctx->ev_marked[j]) ++j;
+        if (ctx->ev_marked[i] && j <= last) HIP_CHECK(hipEventElapsedTime(&ctx->stage_ms[i], ctx->ev[i], ctx->ev[j]));
+    }
+}
 void set_last_error(const std::string& s) { g_last_error = s; }
 }
 
@@ -303,12 +316,21 @@ int aprh_context_set_timing(aprh_context* ctx, int on) {
     APRH_CATCH
 }
 
-int aprh_context_stage_times(aprh_context* ctx, float* ms) {
+int aprh_context_stage_times(aprh_context* ctx, float* ms, int n) {
     APRH_TRY
     if (!ctx || !ms) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
-    for (int i = 0; i < ST_COUNT; ++i) ms[i] = ctx->stage_ms[i];
+    for (int i = 0; i < ST_COUNT && i < n; ++i) ms[i] = ctx->stage_ms[i];
     APRH_CATCH
 }
+
+const char* aprh_stage_name(int i) {
+    static const char* names[ST_COUNT] = {"offset", "smooth_bspline_y", "smooth_bspline_x", "smooth_bspline_z",
+                                          "gradient_ds", "inv_bspline", "local_intensity_scale", "levels",
+                                          "pulling_scheme", "linear_structure", "sample_image"};
+    return (i >= 0 && i < ST_COUNT) ? names[i] : "";
+}
+
+int aprh_stage_count(void) { return ST_COUNT; }
 
 int aprh_context_release_cache(aprh_context* ctx) {
     APRH_TRY

@@ -357,6 +357,7 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
         HIP_CHECK(hipStreamSynchronize(st));
         return;
     }
+    mark(ctx, ST_PULLING);
     /* particle-cell tree levels lmin .. top (= lmax - 1), one contiguous u8 buffer */
     const int top = lmax - 1;
     std::vector<int64_t> off(top + 2, 0);
@@ -392,6 +393,7 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
         HIP_LAUNCH_CHECK();
     }
     const int min_type = par.neighborhood_optimization ? 1 : 2;
+    mark(ctx, ST_STRUCTURE);
     /* equivalence optimisation, levels upward in order */
     for (int l = lmin + 1; l <= top; ++l) {
         hipLaunchKernelGGL(k_upseed, grid, block, 0, st, P + off[l], D(l), P + off[l - 1], D(l - 1), min_type);
