@@ -171,8 +171,10 @@ template <typename T>
 void launch_iir_outer(aprh_context* ctx, T* img, int64_t y, int64_t x, int64_t z, int axis, const IIRCoeffs& c, int* d_err);
 
 /* ---- gradient / inverse B-spline / LIS (k_grad_lis.hip) ---- */
+/* accum: take the existing grad values into the maximum (calc_bspline_fd_ds_mag semantics);
+   the pipeline passes false on a buffer it would otherwise have zero-filled */
 void launch_grad_ds_u16(aprh_context* ctx, const uint16_t* img, uint16_t* grad, float* mean, int64_t y, int64_t x, int64_t z,
-                        float hx, float hy, float hz);
+                        float hx, float hy, float hz, bool accum);
 void launch_inv_bspline(aprh_context* ctx, float* m, float* tmp, int64_t y, int64_t x, int64_t z, bool ay, bool ax, bool az);
 struct LisParams {
     int win[6];

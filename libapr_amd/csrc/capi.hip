@@ -132,8 +132,7 @@ void get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x,
     /* gradient magnitude + 2x2x2 max (ComputeGradient.hpp:92-93), mean downsample and inverse B-spline (:95-118) */
     Buf grad(ctx->pool, nd * 2);
     Buf mean(ctx->pool, nd * 4);
-    HIP_CHECK(hipMemsetAsync(grad.ptr(), 0, nd * 2, st));
-    launch_grad_ds_u16(ctx, I, grad.as<uint16_t>(), mean.as<float>(), y, x, z, par.dx, par.dy, par.dz);
+    launch_grad_ds_u16(ctx, I, grad.as<uint16_t>(), mean.as<float>(), y, x, z, par.dx, par.dy, par.dz, false);
     int herr[3] = {0, 0, 0};
     HIP_CHECK(hipMemcpyAsync(herr, d_err, sizeof(herr), hipMemcpyDeviceToHost, st));
     if (ctx->keep_intermediates) {
@@ -401,10 +400,10 @@ int aprh_gradient_ds_u16(aprh_context* ctx, const uint16_t* image, uint16_t* gra
     if (g.host) HIP_CHECK(hipMemcpyAsync(g.ptr, grad_ds, nd * 2, hipMemcpyHostToDevice, ctx->stream));
     if (mean_ds) {
         DevOut m(ctx, mean_ds, nd * 4);
-        launch_grad_ds_u16(ctx, (const uint16_t*)in.ptr, (uint16_t*)g.ptr, (float*)m.ptr, y, x, z, hx, hy, hz);
+        launch_grad_ds_u16(ctx, (const uint16_t*)in.ptr, (uint16_t*)g.ptr, (float*)m.ptr, y, x, z, hx, hy, hz, true);
         m.finish(ctx);
     } else {
-        launch_grad_ds_u16(ctx, (const uint16_t*)in.ptr, (uint16_t*)g.ptr, nullptr, y, x, z, hx, hy, hz);
+        launch_grad_ds_u16(ctx, (const uint16_t*)in.ptr, (uint16_t*)g.ptr, nullptr, y, x, z, hx, hy, hz, true);
     }
     g.finish(ctx);
     APRH_CATCH

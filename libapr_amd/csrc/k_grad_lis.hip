@@ -41,13 +41,14 @@ __device__ __forceinline__ double sq_term(int d, float h2) {
 }
 
 __global__ __launch_bounds__(256) void k_grad_ds(const uint16_t* __restrict__ img, uint16_t* grad, float* mean, int64_t y_num,
-                                                 int64_t x_num, int64_t z_num, int64_t yd_num, float h2x, float h2y, float h2z) {
+                                                 int64_t x_num, int64_t z_num, int64_t yd_num, float h2x, float h2y, float h2z,
+                                                 int accum) {
     const int64_t yd = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (yd >= yd_num) return;
     const int64_t xd = blockIdx.y, zd = blockIdx.z;
     const int64_t xd_num = gridDim.y;
     const int64_t xy = x_num * y_num;
-    uint16_t g = grad[(zd * xd_num + xd) * yd_num + yd];
+    uint16_t g = accum ? grad[(zd * xd_num + xd) * yd_num + yd] : (uint16_t)0;
     const int64_t y0 = 2 * yd, y1 = min(2 * yd + 1, y_num - 1);
     for (int64_t z = 2 * zd; z <= min(2 * zd + 1, z_num - 1); ++z) {
         const int64_t zm = z > 0 ? z - 1 : 0, zp = min(z + 1, z_num - 1);
@@ -90,6 +91,107 @@ __global__ __launch_bounds__(256) void k_grad_ds(const uint16_t* __restrict__ im
         s = s + (float)img[shz * xy + shx * y_num + y0];
         s = s + (float)img[shz * xy + shx * y_num + shy];
         mean[(zd * xd_num + xd) * yd_num + yd] = (float)((double)s / 8.0);
+    }
+}
+
+
+/* Vector form of k_grad_ds for y_num % 8 == 0: a lane owns 8 consecutive y of
+ * one downsampled (x, z) column pair, i.e. 4 downsampled cells.  It loads the
+ * 12 full-resolution rows its 2x2 block and their x/z neighbours touch as
+ * 16-byte vectors (plus the y-1 / y+8 samples of the centre rows), computes
+ * the 32 gradient magnitudes, and writes 4 maxima and 4 means.
+ * Unit spacing (hx = hy = hz = 1) uses the exact integer form of the
+ * reference's double arithmetic: each squared term is d^2/4, the sum k/4 is
+ * exact, and trunc(sqrt(k/4)) == floor(sqrt(k)) >> 1 == (uint)sqrt((double)k) >> 1
+ * because a correctly rounded sqrt of an integer below 2^52 never rounds up
+ * to the next integer. */
+__device__ __forceinline__ int u16at(const uint4& v, int i) {
+    const uint32_t w = (i < 2) ? v.x : (i < 4) ? v.y : (i < 6) ? v.z : v.w;
+    return (int)((i & 1) ? (w >> 16) : (w & 0xFFFFu));
+}
+
+template <bool UNIT>
+__global__ __launch_bounds__(256) void k_grad_ds_vec(const uint16_t* __restrict__ img, uint16_t* grad, float* mean, int64_t y_num,
+                                                     int64_t x_num, int64_t z_num, int64_t yd_num, float h2x, float h2y, float h2z,
+                                                     int accum) {
+    const int64_t yg = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;   /* group of 8 y */
+    if (yg * 8 >= y_num) return;
+    const int64_t xd = blockIdx.y, zd = blockIdx.z;
+    const int64_t xd_num = gridDim.y;
+    const int64_t xy = x_num * y_num;
+    const int64_t y0 = yg * 8;
+    const int64_t x0 = 2 * xd, z0 = 2 * zd;
+    const bool hx1 = x0 + 1 < x_num, hz1 = z0 + 1 < z_num;
+    const int64_t x1 = hx1 ? x0 + 1 : x0, z1 = hz1 ? z0 + 1 : z0;
+    auto row = [&](int64_t x, int64_t z) -> const uint16_t* { return img + z * xy + x * y_num + y0; };
+    auto ld = [&](int64_t x, int64_t z) -> uint4 { return *(const uint4*)row(x, z); };
+    const int64_t xl0 = x0 > 0 ? x0 - 1 : 0, xr1 = min(x1 + 1, x_num - 1);
+    const int64_t zu0 = z0 > 0 ? z0 - 1 : 0, zd1 = min(z1 + 1, z_num - 1);
+    /* centre rows c[a][b]: x = a ? x1 : x0, z = b ? z1 : z0 */
+    const uint4 c00 = ld(x0, z0), c10 = ld(x1, z0), c01 = ld(x0, z1), c11 = ld(x1, z1);
+    const uint4 l0 = ld(xl0, z0), l1 = ld(xl0, z1), r0 = ld(xr1, z0), r1 = ld(xr1, z1);
+    const uint4 u0 = ld(x0, zu0), u1 = ld(x1, zu0), d0 = ld(x0, zd1), d1 = ld(x1, zd1);
+    /* y neighbours beyond the vector (replicate at the line ends) */
+    auto edge = [&](int64_t x, int64_t z, int which) -> int {
+        const uint16_t* p = img + z * xy + x * y_num;
+        if (which == 0) return y0 > 0 ? (int)p[y0 - 1] : -1;
+        return y0 + 8 < y_num ? (int)p[y0 + 8] : -1;
+    };
+    const int e00m = edge(x0, z0, 0), e00p = edge(x0, z0, 1), e10m = edge(x1, z0, 0), e10p = edge(x1, z0, 1);
+    const int e01m = edge(x0, z1, 0), e01p = edge(x0, z1, 1), e11m = edge(x1, z1, 0), e11p = edge(x1, z1, 1);
+    uint32_t gmax[4] = {0, 0, 0, 0};
+    auto one_row = [&](const uint4& c, const uint4& lft, const uint4& rgt, const uint4& up, const uint4& dn, int em, int ep) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int64_t y = y0 + i;
+            int dyv;
+            /* y == 0: c[1]-c[0]; y == Y-1: c[Y-1]-c[Y-2]; else c[y+1]-c[y-1] */
+            const int cm = (i > 0) ? u16at(c, i - 1) : em;
+            const int cp = (i < 7) ? u16at(c, i + 1) : ep;
+            if (y == 0) dyv = u16at(c, 1) - u16at(c, 0);
+            else if (y == y_num - 1) dyv = u16at(c, i) - cm;
+            else dyv = cp - cm;
+            const int dxv = u16at(rgt, i) - u16at(lft, i);
+            const int dzv = u16at(dn, i) - u16at(up, i);
+            uint32_t g;
+            if (UNIT) {
+                const int64_t k = (int64_t)dxv * dxv + (int64_t)dzv * dzv + (int64_t)dyv * dyv;
+                g = ((uint32_t)sqrt((double)k)) >> 1;
+            } else {
+                const double q = sq_term(dxv, h2x) + sq_term(dzv, h2z) + sq_term(dyv, h2y);
+                g = d2u16(sqrt(q));
+            }
+            gmax[i >> 1] = max(gmax[i >> 1], g);
+        }
+    };
+    /* rows (x0,z0), (x1,z0), (x0,z1), (x1,z1); left/right neighbours of x1 are x0 / xr1 etc. */
+    one_row(c00, l0, hx1 ? c10 : r0, u0, hz1 ? c01 : d0, e00m, e00p);
+    if (hx1) one_row(c10, c00, r0, u1, hz1 ? c11 : d1, e10m, e10p);
+    if (hz1) {
+        one_row(c01, l1, hx1 ? c11 : r1, c00, d0, e01m, e01p);
+        if (hx1) one_row(c11, c01, r1, c10, d1, e11m, e11p);
+    }
+    const int64_t o = (zd * xd_num + xd) * yd_num + yg * 4;
+    if (accum) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) gmax[j] = max(gmax[j], (uint32_t)grad[o + j]);
+    }
+    uint2 gw;
+    gw.x = gmax[0] | (gmax[1] << 16);
+    gw.y = gmax[2] | (gmax[3] << 16);
+    *(uint2*)(grad + o) = gw;
+    if (mean) {
+        /* downsample(sum, /8.0) with clamped x/z (PixelData.hpp:944-962); exact integer sums */
+        float4 m;
+        float* mp = &m.x;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int a = 2 * j, b = 2 * j + 1;
+            const int sum = u16at(c00, a) + u16at(c00, b) + u16at(c10, a) + u16at(c10, b) + u16at(c01, a) + u16at(c01, b) +
+                            u16at(c11, a) + u16at(c11, b);
+            mp[j] = (float)((double)(float)sum / 8.0);
+        }
+        *(float4*)(mean + o) = m;
     }
 }
 
@@ -386,11 +488,22 @@ PadGeom pad_geom(int64_t yd, int64_t xd, int64_t zd, const LisParams& lp) {
 }  // namespace
 
 void launch_grad_ds_u16(aprh_context* ctx, const uint16_t* img, uint16_t* grad, float* mean, int64_t y, int64_t x, int64_t z,
-                        float hx, float hy, float hz) {
+                        float hx, float hy, float hz, bool accum) {
     const int64_t yd = cdiv(y, 2), xd = cdiv(x, 2), zd = cdiv(z, 2);
     if (xd > 65535 || zd > 65535) throw Error(APRH_ERR_UNSUPPORTED, "x or z dimension too large");
-    hipLaunchKernelGGL(k_grad_ds, dim3((unsigned)cdiv(yd, 256), (unsigned)xd, (unsigned)zd), dim3(256), 0, ctx->stream, img, grad,
-                       mean, y, x, z, yd, 2.0f * hx, 2.0f * hy, 2.0f * hz);
+    const bool aligned = (((uintptr_t)img | (uintptr_t)grad | (uintptr_t)mean) & 15u) == 0;
+    if (y % 8 == 0 && aligned) {
+        const dim3 grid((unsigned)cdiv(y / 8, 256), (unsigned)xd, (unsigned)zd);
+        if (hx == 1.0f && hy == 1.0f && hz == 1.0f)
+            hipLaunchKernelGGL(k_grad_ds_vec<true>, grid, dim3(256), 0, ctx->stream, img, grad, mean, y, x, z, yd, 2.0f, 2.0f, 2.0f,
+                               accum ? 1 : 0);
+        else
+            hipLaunchKernelGGL(k_grad_ds_vec<false>, grid, dim3(256), 0, ctx->stream, img, grad, mean, y, x, z, yd, 2.0f * hx,
+                               2.0f * hy, 2.0f * hz, accum ? 1 : 0);
+    } else {
+        hipLaunchKernelGGL(k_grad_ds, dim3((unsigned)cdiv(yd, 256), (unsigned)xd, (unsigned)zd), dim3(256), 0, ctx->stream, img, grad,
+                           mean, y, x, z, yd, 2.0f * hx, 2.0f * hy, 2.0f * hz, accum ? 1 : 0);
+    }
     HIP_LAUNCH_CHECK();
 }
 

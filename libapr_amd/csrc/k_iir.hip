@@ -2,18 +2,21 @@
  * k_iir.hip -- B-spline smoothing of the conversion pipeline on gfx950.
  *
  *  - min/max reduction + bspline offset     (AutoParameters.hpp:165-196)
- *  - y pass: one lane per y-line; 64 lines per wave are staged through an
- *    LDS tile (64 lines x 64 samples, padded rows) so global loads/stores are
- *    coalesced along y while each lane walks its own line sequentially
- *    (ComputeGradient.hpp:248-413).  The offset copy of computeL
- *    (APRConverter.hpp:213-224) is fused into the tile load.
- *  - x / z passes: one lane per (y, outer) line; lanes run along y, so every
- *    step of the recursion is a coalesced row access
- *    (ComputeGradient.hpp:415-599, 601-779).
+ *  - y pass (ComputeGradient.hpp:248-413): one lane per y-line.  Lines whose
+ *    length is a multiple of the 16-byte vector are streamed with 16-byte
+ *    loads/stores, several vectors ahead of the recursion; other lengths go
+ *    through an LDS tile (64 lines x 64 samples) so global accesses stay
+ *    coalesced.  The offset copy of computeL (APRConverter.hpp:213-224) is
+ *    fused into the first read.
+ *  - x / z passes (ComputeGradient.hpp:415-599, 601-779): lanes run along y
+ *    (8 u16 / 4 f32 samples per lane), so each recursion step is one
+ *    coalesced 16-byte-per-lane row access; 8 steps are loaded ahead.
  *
  * Every lane keeps the reference's per-line FP32 operation order; the build
  * disables FP contraction, so results are bit-identical to the CPU path.
  */
+#include <type_traits>
+
 #include "aprh_internal.hpp"
 
 namespace aprh {
@@ -53,8 +56,38 @@ __device__ __forceinline__ float load_in(const T* p, float off, bool has_off) {
 
 __global__ void k_minmax(const uint16_t* img, size_t n, uint32_t* mm) {
     uint32_t lo = 65535u, hi = 0u;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        uint32_t v = img[i];
+    /* 16-byte loads over the aligned body, scalar tail */
+    const size_t nv = n / 8;
+    const uint4* v4 = (const uint4*)img;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < nv; i += 4 * stride) {
+        uint4 a[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) a[u] = v4[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t w[4] = {a[u].x, a[u].y, a[u].z, a[u].w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t l = w[q] & 0xFFFFu, h = w[q] >> 16;
+                lo = min(lo, min(l, h));
+                hi = max(hi, max(l, h));
+            }
+        }
+    }
+    for (; i < nv; i += stride) {
+        const uint4 a = v4[i];
+        const uint32_t w[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t l = w[q] & 0xFFFFu, h = w[q] >> 16;
+            lo = min(lo, min(l, h));
+            hi = max(hi, max(l, h));
+        }
+    }
+    for (size_t j = nv * 8 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        uint32_t v = img[j];
         lo = v < lo ? v : lo;
         hi = v > hi ? v : hi;
     }
@@ -237,6 +270,409 @@ __global__ __launch_bounds__(256) void k_iir_outer(T* img, int64_t y_num, int64_
     if (err) atomicAdd(d_err, err);
 }
 
+
+/* ---------------- vector paths ----------------
+ * y pass: lane = one y-line, streamed with 16-byte loads/stores (8 u16 or 4 f32
+ * samples); D vectors are loaded ahead of the recursion so several HBM
+ * requests per lane stay in flight.  Requires y_num % VEC == 0 and 16-byte
+ * aligned lines. */
+template <typename T> struct Vec16;
+template <> struct Vec16<uint16_t> {
+    static constexpr int N = 8;
+    static __device__ __forceinline__ float get(const uint4& v, int i) {
+        const uint32_t w = (i < 2) ? v.x : (i < 4) ? v.y : (i < 6) ? v.z : v.w;
+        return (float)((i & 1) ? (w >> 16) : (w & 0xFFFFu));
+    }
+    static __device__ __forceinline__ void set(uint4& v, int i, float f) {
+        const uint32_t h = (uint32_t)f2u16(f);
+        uint32_t* w = (i < 2) ? &v.x : (i < 4) ? &v.y : (i < 6) ? &v.z : &v.w;
+        *w = (i & 1) ? ((*w & 0xFFFFu) | (h << 16)) : ((*w & 0xFFFF0000u) | h);
+    }
+};
+template <> struct Vec16<float> {
+    static constexpr int N = 4;
+    static __device__ __forceinline__ float get(const uint4& v, int i) {
+        const uint32_t w = (i == 0) ? v.x : (i == 1) ? v.y : (i == 2) ? v.z : v.w;
+        return __uint_as_float(w);
+    }
+    static __device__ __forceinline__ void set(uint4& v, int i, float f) {
+        const uint32_t b = __float_as_uint(f);
+        if (i == 0) v.x = b; else if (i == 1) v.y = b; else if (i == 2) v.z = b; else v.w = b;
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ float vin(const uint4& v, int i, float off, bool has_off) {
+    const float x = Vec16<T>::get(v, i);
+    return has_off ? (float)f2u16(x + off) : x;
+}
+
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_iir_y_vec(const T* __restrict__ in, T* out, int64_t y_num, int64_t n_lines, IIRCoeffs c,
+                                                   const float* d_off, int* d_err) {
+    constexpr int V = Vec16<T>::N;
+    const int64_t me = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (me >= n_lines) return;
+    const bool has_off = d_off != nullptr;
+    const float off = has_off ? *d_off : 0.0f;
+    const float b1 = c.b1, b2 = c.b2, nf = c.nf;
+    const uint4* src = (const uint4*)(in + me * y_num);
+    uint4* dst = (uint4*)(out + me * y_num);
+    const int64_t nv = y_num / V;
+    int err = 0;
+    /* boundary sums over the first / last minLen samples (ComputeGradient.hpp:342-356) */
+    float t1 = 0, t2 = 0, t3 = 0, t4 = 0;
+    {
+        const int nb = (c.mL + V - 1) / V;
+        for (int b = 0; b < nb; ++b) {
+            const uint4 v = src[b];
+#pragma unroll
+            for (int i = 0; i < V; ++i) {
+                const int k = b * V + i;
+                if (k < c.mL) {
+                    const float x = vin<T>(v, i, off, has_off);
+                    t1 += c.bc1[k] * x;
+                    t2 += c.bc2[k] * x;
+                }
+            }
+        }
+        for (int b = 0; b < nb; ++b) {
+            const uint4 v = src[nv - 1 - b];
+#pragma unroll
+            for (int i = V - 1; i >= 0; --i) {
+                const int k = b * V + (V - 1 - i);   /* distance from the end */
+                if (k < c.mL) {
+                    const float x = vin<T>(v, i, off, has_off);
+                    t3 += c.bc3[k] * x;
+                    t4 += c.bc4[k] * x;
+                }
+            }
+        }
+    }
+    /* causal pass: state unrounded (ComputeGradient.hpp:359-373) */
+    float s1 = t1, s2 = t2;
+    for (int64_t b0 = 0; b0 < nv; b0 += D) {
+        uint4 buf[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            if (b0 + d < nv) buf[d] = src[b0 + d];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            if (b0 + d >= nv) break;
+            uint4 o = buf[d];
+#pragma unroll
+            for (int i = 0; i < V; ++i) {
+                const int64_t k = (b0 + d) * V + i;
+                float r;
+                if (k == 0) {
+                    r = t2;
+                } else if (k == 1) {
+                    r = t1;
+                } else {
+                    const float t = s1 * b1 + s2 * b2 + vin<T>(buf[d], i, off, has_off);
+                    r = Val<T>::rnd(t, err);
+                    s2 = s1;
+                    s1 = t;
+                }
+                Vec16<T>::set(o, i, r);
+            }
+            dst[b0 + d] = o;
+        }
+    }
+    /* end values (ComputeGradient.hpp:375-376) */
+    const float e2 = Val<T>::rnd(t3 * nf, err);
+    const float e1 = Val<T>::rnd(t4 * nf, err);
+    const float bv_m2 = Val<T>::store(e2), bv_m1 = Val<T>::store(e1);
+    /* anti-causal pass: state from stored/norm (ComputeGradient.hpp:382-407) */
+    float r2 = bv_m1 / nf;
+    float r1 = bv_m2 / nf;
+    for (int64_t b0 = nv - 1; b0 >= 0; b0 -= D) {
+        uint4 buf[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            if (b0 - d >= 0) buf[d] = dst[b0 - d];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            if (b0 - d < 0) break;
+            uint4 o = buf[d];
+#pragma unroll
+            for (int i = V - 1; i >= 0; --i) {
+                const int64_t k = (b0 - d) * V + i;
+                float r;
+                if (k == y_num - 1) {
+                    r = bv_m1;
+                } else if (k == y_num - 2) {
+                    r = bv_m2;
+                } else {
+                    const float t = r1 * b1 + r2 * b2 + Vec16<T>::get(buf[d], i);
+                    r = Val<T>::rnd(t * nf, err);
+                    r2 = r1;
+                    r1 = t;
+                }
+                Vec16<T>::set(o, i, r);
+            }
+            dst[b0 - d] = o;
+        }
+    }
+    if (err) atomicAdd(d_err, err);
+}
+
+/* x / z pass, 16-byte vectors along y, D steps of the recursion loaded ahead */
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_iir_outer_vec(T* img, int64_t y_num, int64_t n, int64_t s, int64_t outer_stride, IIRCoeffs c,
+                                                       int* d_err) {
+    constexpr int V = Vec16<T>::N;
+    const int64_t yv = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (yv * V >= y_num) return;
+    T* basep = img + (int64_t)blockIdx.y * outer_stride + yv * V;
+    const int64_t sv = s / V;   /* stride in vectors */
+    uint4* base = (uint4*)basep;
+    const float b1 = c.b1, b2 = c.b2, nf = c.nf;
+    int err = 0;
+    float v1[V], v2[V], v3[V], v4[V];
+#pragma unroll
+    for (int i = 0; i < V; ++i) v1[i] = v2[i] = v3[i] = v4[i] = 0.0f;
+    for (int j = 0; j < c.mL; ++j) {
+        const uint4 a = base[(int64_t)j * sv];
+        const uint4 b = base[(n - 1 - j) * sv];
+        const float w1 = c.bc1[j], w2 = c.bc2[j], w3 = c.bc3[j], w4 = c.bc4[j];
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            const float xa = Vec16<T>::get(a, i), xb = Vec16<T>::get(b, i);
+            v1[i] += w1 * xa;
+            v2[i] += w2 * xa;
+            v3[i] += w3 * xb;
+            v4[i] += w4 * xb;
+        }
+    }
+    {
+        uint4 o0 = make_uint4(0, 0, 0, 0), o1 = make_uint4(0, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            Vec16<T>::set(o0, i, Val<T>::rnd(v2[i], err));
+            Vec16<T>::set(o1, i, Val<T>::rnd(v1[i], err));
+        }
+        base[0] = o0;
+        base[sv] = o1;
+    }
+    /* causal: state becomes the rounded output after the first step */
+    float p1[V], p2[V];
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+        p1[i] = v1[i];
+        p2[i] = v2[i];
+    }
+    for (int64_t j0 = 2; j0 < n; j0 += D) {
+        uint4 buf[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            if (j0 + d < n) buf[d] = base[(j0 + d) * sv];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            if (j0 + d >= n) break;
+            uint4 o = make_uint4(0, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < V; ++i) {
+                const float t = Val<T>::rnd(Vec16<T>::get(buf[d], i) + b1 * p1[i] + b2 * p2[i], err);
+                Vec16<T>::set(o, i, t);
+                p2[i] = p1[i];
+                p1[i] = t;
+            }
+            base[(j0 + d) * sv] = o;
+        }
+    }
+    /* anti-causal: state from the unrounded boundary sums */
+    {
+        uint4 oa = make_uint4(0, 0, 0, 0), ob = make_uint4(0, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            Vec16<T>::set(oa, i, Val<T>::rnd(v4[i] * nf, err));
+            Vec16<T>::set(ob, i, Val<T>::rnd(v3[i] * nf, err));
+        }
+        base[(n - 1) * sv] = oa;
+        base[(n - 2) * sv] = ob;
+    }
+    for (int64_t j0 = n - 3; j0 >= 0; j0 -= D) {
+        uint4 buf[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            if (j0 - d >= 0) buf[d] = base[(j0 - d) * sv];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            if (j0 - d < 0) break;
+            uint4 o = make_uint4(0, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < V; ++i) {
+                const float t = (Vec16<T>::get(buf[d], i) + b1 * v3[i]) + b2 * v4[i];
+                Vec16<T>::set(o, i, Val<T>::rnd(t * nf, err));
+                v4[i] = v3[i];
+                v3[i] = t;
+            }
+            base[(j0 - d) * sv] = o;
+        }
+    }
+    if (err) atomicAdd(d_err, err);
+}
+
+/* y pass through an LDS tile with full-line coalescing: a wave owns 64 lines;
+ * each chunk of CH samples per line is loaded by 8 lanes per line (16 B each,
+ * CH*2 = 128 B per line), staged in LDS with a 144-byte row pitch (conflict
+ * free for 16-byte LDS accesses), and each lane then runs the recursion over
+ * its own row.  The next chunk's global loads are issued before the current
+ * chunk is processed.  u16 only; requires y_num % 64 == 0. */
+__global__ __launch_bounds__(64) void k_iir_y_lds(const uint16_t* __restrict__ in, uint16_t* out, int64_t y_num, int64_t n_lines,
+                                                  IIRCoeffs c, const float* d_off, int* d_err) {
+    constexpr int CH = 64;            /* samples per line per chunk */
+    constexpr int PITCH = 72;         /* u16 per LDS row (144 B)    */
+    __shared__ __attribute__((aligned(16))) uint16_t tile[64 * PITCH];
+    const int lane = threadIdx.x;
+    const int64_t line0 = (int64_t)blockIdx.x * 64;
+    const int64_t me = line0 + lane;
+    const bool valid = me < n_lines;
+    const bool has_off = d_off != nullptr;
+    const float off = has_off ? *d_off : 0.0f;
+    const float b1 = c.b1, b2 = c.b2, nf = c.nf;
+    const int64_t nch = y_num / CH;
+    /* cooperative layout: load instruction i covers lines 8i..8i+7, lane -> (line 8i + lane/8, segment lane%8) */
+    const int seg = lane & 7;
+    int err = 0;
+
+    auto gload = [&](const uint16_t* base, int64_t ch, uint4* r) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int64_t ln = line0 + 8 * i + (lane >> 3);
+            r[i] = (ln < n_lines) ? *(const uint4*)(base + ln * y_num + ch * CH + seg * 8) : make_uint4(0, 0, 0, 0);
+        }
+    };
+    auto lds_put = [&](const uint4* r) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) *(uint4*)(&tile[(8 * i + (lane >> 3)) * PITCH + seg * 8]) = r[i];
+    };
+    auto gstore = [&](uint16_t* base, int64_t ch) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int64_t ln = line0 + 8 * i + (lane >> 3);
+            if (ln < n_lines) *(uint4*)(base + ln * y_num + ch * CH + seg * 8) = *(const uint4*)(&tile[(8 * i + (lane >> 3)) * PITCH + seg * 8]);
+        }
+    };
+
+    /* boundary sums (ComputeGradient.hpp:342-356), direct 16-byte loads of the own line */
+    float t1 = 0, t2 = 0, t3 = 0, t4 = 0;
+    if (valid) {
+        const uint4* src = (const uint4*)(in + me * y_num);
+        const int64_t nv = y_num / 8;
+        const int nb = (c.mL + 7) / 8;
+        for (int b = 0; b < nb; ++b) {
+            const uint4 v = src[b];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int k = b * 8 + i;
+                if (k < c.mL) {
+                    const float x = vin<uint16_t>(v, i, off, has_off);
+                    t1 += c.bc1[k] * x;
+                    t2 += c.bc2[k] * x;
+                }
+            }
+        }
+        for (int b = 0; b < nb; ++b) {
+            const uint4 v = src[nv - 1 - b];
+#pragma unroll
+            for (int i = 7; i >= 0; --i) {
+                const int k = b * 8 + (7 - i);
+                if (k < c.mL) {
+                    const float x = vin<uint16_t>(v, i, off, has_off);
+                    t3 += c.bc3[k] * x;
+                    t4 += c.bc4[k] * x;
+                }
+            }
+        }
+    }
+
+    /* causal pass */
+    float s1 = t1, s2 = t2;
+    uint4 nxt[8];
+    gload(in, 0, nxt);
+    for (int64_t ch = 0; ch < nch; ++ch) {
+        __syncthreads();
+        lds_put(nxt);
+        if (ch + 1 < nch) gload(in, ch + 1, nxt);
+        __syncthreads();
+        if (valid) {
+            uint16_t* row = &tile[lane * PITCH];
+#pragma unroll
+            for (int q = 0; q < CH / 8; ++q) {
+                uint4 v = *(uint4*)(row + q * 8);
+                uint4 o = make_uint4(0, 0, 0, 0);
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int64_t k = ch * CH + q * 8 + i;
+                    float r;
+                    if (k == 0) {
+                        r = t2;
+                    } else if (k == 1) {
+                        r = t1;
+                    } else {
+                        const float t = s1 * b1 + s2 * b2 + vin<uint16_t>(v, i, off, has_off);
+                        r = Val<uint16_t>::rnd(t, err);
+                        s2 = s1;
+                        s1 = t;
+                    }
+                    Vec16<uint16_t>::set(o, i, r);
+                }
+                *(uint4*)(row + q * 8) = o;
+            }
+        }
+        __syncthreads();
+        gstore(out, ch);
+    }
+    const float e2 = Val<uint16_t>::rnd(t3 * nf, err);
+    const float e1 = Val<uint16_t>::rnd(t4 * nf, err);
+    const float bv_m2 = Val<uint16_t>::store(e2), bv_m1 = Val<uint16_t>::store(e1);
+    float r2 = bv_m1 / nf;
+    float r1 = bv_m2 / nf;
+    /* every lane's forward stores must be visible before the tile loads below */
+    __threadfence_block();
+    __syncthreads();
+    /* anti-causal pass over the forward output */
+    gload(out, nch - 1, nxt);
+    for (int64_t ch = nch - 1; ch >= 0; --ch) {
+        __syncthreads();
+        lds_put(nxt);
+        if (ch - 1 >= 0) gload(out, ch - 1, nxt);
+        __syncthreads();
+        if (valid) {
+            uint16_t* row = &tile[lane * PITCH];
+#pragma unroll
+            for (int q = CH / 8 - 1; q >= 0; --q) {
+                uint4 v = *(uint4*)(row + q * 8);
+                uint4 o = make_uint4(0, 0, 0, 0);
+#pragma unroll
+                for (int i = 7; i >= 0; --i) {
+                    const int64_t k = ch * CH + q * 8 + i;
+                    float r;
+                    if (k == y_num - 1) {
+                        r = bv_m1;
+                    } else if (k == y_num - 2) {
+                        r = bv_m2;
+                    } else {
+                        const float t = r1 * b1 + r2 * b2 + Vec16<uint16_t>::get(v, i);
+                        r = Val<uint16_t>::rnd(t * nf, err);
+                        r2 = r1;
+                        r1 = t;
+                    }
+                    Vec16<uint16_t>::set(o, i, r);
+                }
+                *(uint4*)(row + q * 8) = o;
+            }
+        }
+        __syncthreads();
+        gstore(out, ch);
+    }
+    if (err) atomicAdd(d_err, err);
+}
+
+inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
 }  // namespace
 
 void launch_minmax_offset(aprh_context* ctx, const uint16_t* img, size_t n, float lambda, float* d_offset, uint32_t* d_mm) {
@@ -257,12 +693,33 @@ template <typename T>
 void launch_iir_y(aprh_context* ctx, const T* in, T* out, int64_t y, int64_t x, int64_t z, const IIRCoeffs& c,
                   const float* d_offset, int* d_err) {
     const int64_t lines = x * z;
-    hipLaunchKernelGGL(k_iir_y<T>, dim3((unsigned)cdiv(lines, TL)), dim3(64), 0, ctx->stream, in, out, y, lines, c, d_offset, d_err);
+    constexpr int V = Vec16<T>::N;
+    if (std::is_same<T, uint16_t>::value && y % 64 == 0 && aligned16(in) && aligned16(out)) {
+        hipLaunchKernelGGL(k_iir_y_lds, dim3((unsigned)cdiv(lines, 64)), dim3(64), 0, ctx->stream, (const uint16_t*)in,
+                           (uint16_t*)out, y, lines, c, d_offset, d_err);
+    } else if (y % V == 0 && y >= 2 * V && aligned16(in) && aligned16(out)) {
+        hipLaunchKernelGGL((k_iir_y_vec<T, 4>), dim3((unsigned)cdiv(lines, 256)), dim3(256), 0, ctx->stream, in, out, y, lines, c,
+                           d_offset, d_err);
+    } else {
+        hipLaunchKernelGGL(k_iir_y<T>, dim3((unsigned)cdiv(lines, TL)), dim3(64), 0, ctx->stream, in, out, y, lines, c, d_offset,
+                           d_err);
+    }
     HIP_LAUNCH_CHECK();
 }
 
 template <typename T>
 void launch_iir_outer(aprh_context* ctx, T* img, int64_t y, int64_t x, int64_t z, int axis, const IIRCoeffs& c, int* d_err) {
+    constexpr int V = Vec16<T>::N;
+    if (y % V == 0 && aligned16(img)) {
+        const unsigned gx = (unsigned)cdiv(y / V, 256);
+        if (axis == 1) {
+            hipLaunchKernelGGL((k_iir_outer_vec<T, 8>), dim3(gx, (unsigned)z), dim3(256), 0, ctx->stream, img, y, x, y, x * y, c, d_err);
+        } else {
+            hipLaunchKernelGGL((k_iir_outer_vec<T, 8>), dim3(gx, (unsigned)x), dim3(256), 0, ctx->stream, img, y, z, x * y, y, c, d_err);
+        }
+        HIP_LAUNCH_CHECK();
+        return;
+    }
     if (axis == 1) {
         /* x pass: outer = z */
         hipLaunchKernelGGL(k_iir_outer<T>, dim3((unsigned)cdiv(y, 256), (unsigned)z), dim3(256), 0, ctx->stream, img, y, x, y, x * y,
