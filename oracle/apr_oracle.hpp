@@ -1122,6 +1122,335 @@ void sample_image(const GenInfo& g, const Linear& L, const Mesh<ImageType>& img,
     }
 }
 
+
+/* ======================================================================== */
+/* Particle operations on an APR: tree, tree means, stencils, convolution,   */
+/* Richardson-Lucy.                                                          */
+/* ======================================================================== */
+
+struct Tree {
+    GenInfo info;   /* tree geometry (GenInfo::init_tree)            */
+    Linear lin;     /* tree LinearAccess (interior cells)             */
+};
+
+/* APR::initialize_tree_dense (APR.hpp:633-689) +                            */
+/* LinearAccess::initialize_tree_access_dense (LinearAccess.hpp:67-122)      */
+static inline void build_tree(const GenInfo& g, const Linear& L, Tree& t) {
+    t.info.init_tree(g.org[0], g.org[1], g.org[2]);
+    const GenInfo& ti = t.info;
+    std::vector<Mesh<uint8_t>> m(ti.l_max + 1);
+    for (int l = ti.l_min; l <= ti.l_max; ++l) m[l].resize(ti.y_num[l], ti.x_num[l], ti.z_num[l], 0);
+    /* parents of APR particles */
+    for (int level = g.l_max; level >= g.l_min; --level) {
+        if (level - 1 < ti.l_min || level - 1 > ti.l_max) continue;
+        for (int64_t z = 0; z < g.z_num[level]; ++z)
+            for (int64_t x = 0; x < g.x_num[level]; ++x)
+                for (uint64_t i = L.row_begin(level, z, x, g); i < L.row_end(level, z, x, g); ++i)
+                    m[level - 1].at(L.y_vec[i] / 2, x / 2, z / 2) = 1;
+    }
+    /* the rest of the tree, level by level ("suffices to check one child") */
+    for (int level = ti.l_max - 1; level >= ti.l_min; --level)
+        for (int64_t z = 0; z < ti.z_num[level]; ++z)
+            for (int64_t x = 0; x < ti.x_num[level]; ++x)
+                for (int64_t y = 0; y < ti.y_num[level]; ++y)
+                    m[level].at(y, x, z) = m[level].at(y, x, z) | m[level + 1].at(2 * y, 2 * x, 2 * z);
+    /* linear access of the tree */
+    t.lin.init_xz(ti);
+    for (int level = ti.l_max; level >= ti.l_min; --level)
+        for (int64_t z = 0; z < ti.z_num[level]; ++z)
+            for (int64_t x = 0; x < ti.x_num[level]; ++x) {
+                uint64_t c = 0;
+                for (int64_t y = 0; y < ti.y_num[level]; ++y) c += m[level].at(y, x, z);
+                t.lin.xz_end_vec[t.lin.level_xz_vec[level] + z * ti.x_num[level] + x] = c;
+            }
+    std::partial_sum(t.lin.xz_end_vec.begin(), t.lin.xz_end_vec.end(), t.lin.xz_end_vec.begin());
+    t.lin.total = t.lin.xz_end_vec.back();
+    t.lin.y_vec.assign(t.lin.total, 0);
+    for (int level = ti.l_max; level >= ti.l_min; --level)
+        for (int64_t z = 0; z < ti.z_num[level]; ++z)
+            for (int64_t x = 0; x < ti.x_num[level]; ++x) {
+                uint64_t pi = t.lin.row_begin(level, z, x, ti);
+                for (int64_t y = 0; y < ti.y_num[level]; ++y)
+                    if (m[level].at(y, x, z)) t.lin.y_vec[pi++] = (uint16_t)y;
+            }
+}
+
+/* APRTreeNumerics::fill_tree_mean (APRTreeNumerics.hpp:63-202): tree value =
+   sum over children of scale*value/8, accumulated in particle order: all
+   particle children first (levels l_max..l_min), then tree children
+   (tree levels from the top down).                                          */
+template <typename PartT>
+void fill_tree_mean(const GenInfo& g, const Linear& L, const Tree& t, const PartT* parts, std::vector<float>& tree) {
+    const GenInfo& ti = t.info;
+    tree.assign(t.lin.total, 0.0f);
+    auto accumulate = [&](int level, const GenInfo& gc, const Linear& Lc, auto value_of) {
+        /* children at `level` (geometry gc / Lc), parents at tree level `level - 1` */
+        const int pl = level - 1;
+        const int64_t pxn = ti.x_num[pl], pzn = ti.z_num[pl], pyn = ti.y_num[pl];
+#pragma omp parallel for schedule(dynamic)
+        for (int64_t zd = 0; zd < pzn; ++zd) {
+            for (int64_t z = 2 * zd; z <= std::min(2 * zd + 1, gc.z_num[level] - 1); ++z) {
+                for (int64_t xd = 0; xd < pxn; ++xd) {
+                    for (int64_t x = 2 * xd; x <= std::min(2 * xd + 1, gc.x_num[level] - 1); ++x) {
+                        uint64_t pb = t.lin.row_begin(pl, z / 2, x / 2, ti), pe = t.lin.row_end(pl, z / 2, x / 2, ti);
+                        float sxz = (float)((((2 * pxn != gc.x_num[level]) && ((x / 2) == (pxn - 1))) +
+                                             ((2 * pzn != gc.z_num[level]) && ((z / 2) == (pzn - 1)))) * 2);
+                        if (sxz == 0) sxz = 1;
+                        float syxz = sxz;
+                        if (2 * pyn != gc.y_num[level]) syxz = sxz * 2;
+                        uint64_t pi = pb;
+                        for (uint64_t i = Lc.row_begin(level, z, x, gc); i < Lc.row_end(level, z, x, gc); ++i) {
+                            const int64_t y = Lc.y_vec[i];
+                            while (pi < pe && t.lin.y_vec[pi] != y / 2) ++pi;
+                            const float v = value_of(i);
+                            if ((int64_t)t.lin.y_vec[pi] == pyn - 1) tree[pi] = syxz * v / 8.0f + tree[pi];
+                            else tree[pi] = sxz * v / 8.0f + tree[pi];
+                        }
+                    }
+                }
+            }
+        }
+    };
+    for (int level = g.l_max; level >= g.l_min; --level) {
+        if (level - 1 < ti.l_min) continue;
+        accumulate(level, g, L, [&](uint64_t i) { return (float)parts[i]; });
+    }
+    for (int level = ti.l_max; level > ti.l_min; --level)
+        accumulate(level, ti, t.lin, [&](uint64_t i) { return tree[i]; });
+}
+
+/* APRStencil::downsample_stencil / get_downsampled_stencils (APRStencil.hpp:12-95) */
+static inline void downsample_stencil(const Mesh<float>& in, Mesh<float>& out, int level_delta, bool normalize) {
+    const int z_num = (int)in.z_num, x_num = (int)in.x_num, y_num = (int)in.y_num;
+    const int ndim = (z_num > 1) + (x_num > 1) + (y_num > 1);
+    const int step = (int)std::pow(2.0f, (float)level_delta);
+    const int factor = (int)std::pow((float)step, (float)ndim);
+    auto dsz = [&](int n) { return n > 1 ? std::max(2 * (((n - 1) / 2 + step - 1) / step) + 1, 3) : 1; };
+    const int zd = dsz(z_num), xd = dsz(x_num), yd = dsz(y_num);
+    out.resize(yd, xd, zd, 0.0f);
+    const int zo = (zd / 2) * step - z_num / 2, xo = (xd / 2) * step - x_num / 2, yo = (yd / 2) * step - y_num / 2;
+    for (int iz = 0; iz < z_num; ++iz)
+        for (int ix = 0; ix < x_num; ++ix)
+            for (int iy = 0; iy < y_num; ++iy)
+                for (int dz = 0; dz < zd; ++dz) {
+                    const float ovz = (float)std::max(std::min(dz * step + step, zo + iz + step) - std::max(dz * step, zo + iz), 0);
+                    for (int dx = 0; dx < xd; ++dx) {
+                        const float ovx = (float)std::max(std::min(dx * step + step, xo + ix + step) - std::max(dx * step, xo + ix), 0);
+                        for (int dy = 0; dy < yd; ++dy) {
+                            const float ovy = (float)std::max(std::min(dy * step + step, yo + iy + step) - std::max(dy * step, yo + iy), 0);
+                            const float ov = ovx * ovy * ovz;
+                            out.at(dy, dx, dz) += ov * in.at(iy, ix, iz);
+                        }
+                    }
+                }
+    float sum = 0;
+    for (auto& v : out.v) {
+        v /= (float)factor;
+        sum += v;
+    }
+    if (normalize) {
+        const float nf = 1.0f / sum;
+        for (auto& v : out.v) v *= nf;
+    }
+}
+
+/* PixelData::normalize (PixelData.hpp:771-781): divide by the float sum */
+static inline void normalize_mesh(Mesh<float>& m) {
+    float sum = 0;
+    for (float v : m.v) sum += v;
+    for (auto& v : m.v) v /= sum;
+}
+
+static inline void downsampled_stencils(const Mesh<float>& in, std::vector<Mesh<float>>& out, int nlevels, bool normalize) {
+    out.assign(std::max(nlevels, 1), Mesh<float>());
+    out[0] = in;
+    if (normalize) normalize_mesh(out[0]);
+    for (int d = 1; d < nlevels; ++d) downsample_stencil(in, out[d], d, normalize);
+}
+
+/* APRFilter::convolve_pencil (APRFilter.hpp:1094-1216) with its dense ring   */
+/* buffer (Sz x Sx x y) and boundary helpers (:384-504, :705-849).           */
+struct RingBuf {
+    int64_t y_num = 0, x_num = 0, z_num = 0;
+    std::vector<float> v;
+    float& at(int64_t y, int64_t x, int64_t z) { return v[(size_t)(z * x_num * y_num + x * y_num + y)]; }
+};
+
+template <typename PartT>
+void convolve_pencil(const GenInfo& g, const Linear& L, const Tree& t, const std::vector<Mesh<float>>& stencils, const PartT* in,
+                     const float* tree_data, float* out, bool reflect) {
+    const int nthreads =
+#ifdef _OPENMP
+        omp_get_max_threads();
+#else
+        1;
+#endif
+    int64_t ynm = (g.org[0] > 1) ? g.y_num[g.l_max] + stencils[0].y_num - 1 : 1;
+    std::vector<RingBuf> bufs(nthreads);
+    for (auto& b : bufs) b.v.assign((size_t)(ynm * stencils[0].x_num * stencils[0].z_num), 0.0f);
+    const GenInfo& ti = t.info;
+    for (int level = g.l_max; level >= g.l_min; --level) {
+        const int sn = std::min((int)stencils.size() - 1, g.l_max - level);
+        const Mesh<float>& st = stencils[sn];
+        const int sy = (int)st.y_num, sx = (int)st.x_num, sz = (int)st.z_num;
+        const int hy = (sy - 1) / 2, hx = (sx - 1) / 2, hz = (sz - 1) / 2;
+        const int mr = std::max(hy, std::max(hx, hz));
+        const int npl_max = (int)std::ceil(std::log2(mr + 2) - 1);
+        const int npl = std::min(npl_max, level - g.l_min);
+        const int64_t z_num = g.z_num[level], x_num = g.x_num[level];
+        ynm = (g.org[0] > 1) ? g.y_num[level] + sy - 1 : 1;
+        for (auto& b : bufs) { b.y_num = ynm; b.x_num = sx; b.z_num = sz; }
+
+        auto bc_y = [&](RingBuf& b, int64_t z, int64_t x) {
+            const int64_t base = (z % sz) * b.x_num * b.y_num + (x % sx) * b.y_num;
+            if (reflect) {
+                for (int y = 0; y < hy; ++y) b.v[base + hy - 1 - y] = b.v[base + hy + 1 + y];
+                for (int y = 0; y < hy; ++y) {
+                    const int64_t r = b.y_num - 1 - hy;
+                    b.v[base + r + 1 + y] = b.v[base + r - 1 - y];
+                }
+            } else {
+                for (int y = 0; y < hy; ++y) b.v[base + y] = 0;
+                for (int y = 0; y < hy; ++y) b.v[base + b.y_num - hy + y] = 0;
+            }
+        };
+        auto bc_z = [&](RingBuf& b, int64_t z, bool low) {
+            const int64_t slice = b.x_num * b.y_num;
+            const int64_t oo = (z % sz) * slice;
+            if (reflect) {
+                int64_t zin;
+                if (low) zin = (uint64_t)(hz + (hz - z)) % (uint64_t)sz;
+                else {
+                    const uint64_t r = (uint64_t)(z_num - 1 + hz);
+                    zin = (int64_t)((r - ((uint64_t)z - r)) % (uint64_t)sz);
+                }
+                std::copy(b.v.begin() + zin * slice, b.v.begin() + zin * slice + slice, b.v.begin() + oo);
+            } else {
+                std::fill(b.v.begin() + oo, b.v.begin() + oo + slice, 0.0f);
+            }
+        };
+        auto bc_x = [&](RingBuf& b, int64_t x, bool low) {
+            for (int zz = 0; zz < sz; ++zz) {
+                const int64_t oo = zz * b.x_num * b.y_num + (x % sx) * b.y_num;
+                if (reflect) {
+                    int64_t xin;
+                    if (low) xin = (uint64_t)(hx + (hx - x)) % (uint64_t)sx;
+                    else {
+                        const uint64_t r = (uint64_t)(x_num - 1 + hx);
+                        xin = (int64_t)((r - ((uint64_t)x - r)) % (uint64_t)sx);
+                    }
+                    const int64_t io = zz * b.x_num * b.y_num + xin * b.y_num;
+                    std::copy(b.v.begin() + io, b.v.begin() + io + b.y_num, b.v.begin() + oo);
+                } else {
+                    std::fill(b.v.begin() + oo, b.v.begin() + oo + b.y_num, 0.0f);
+                }
+            }
+        };
+        auto update = [&](RingBuf& b, int64_t z, int64_t x) {
+            const int64_t mo = ((z + hz) % sz) * b.x_num * b.y_num + ((x + hx) % sx) * b.y_num + hy;
+            for (uint64_t i = L.row_begin(level, z, x, g); i < L.row_end(level, z, x, g); ++i) b.v[mo + L.y_vec[i]] = (float)in[i];
+            if (level > g.l_min) {
+                for (int d = 1; d <= npl; ++d) {
+                    const int step = (int)std::pow(2, d);
+                    for (uint64_t i = L.row_begin(level - d, z / step, x / step, g); i < L.row_end(level - d, z / step, x / step, g); ++i) {
+                        const int64_t y0 = step * L.y_vec[i];
+                        const int64_t y1 = std::min<int64_t>(y0 + step, g.y_num[level]);
+                        for (int64_t y = y0; y < y1; ++y) b.v[mo + y] = (float)in[i];
+                    }
+                }
+            }
+            if (level < g.l_max) {
+                for (uint64_t i = t.lin.row_begin(level, z, x, ti); i < t.lin.row_end(level, z, x, ti); ++i)
+                    b.v[mo + t.lin.y_vec[i]] = tree_data[i];
+            }
+            bc_y(b, z + hz, x + hx);
+        };
+        auto run = [&](RingBuf& b, int64_t z, int64_t x) {
+            const int64_t xy = b.x_num * b.y_num;
+            for (uint64_t i = L.row_begin(level, z, x, g); i < L.row_end(level, z, x, g); ++i) {
+                float val = 0;
+                const int64_t y = L.y_vec[i];
+                size_t c = 0;
+                for (int iz = 0; iz < sz; ++iz) {
+                    const int64_t bo = ((z + iz) % sz) * xy + y;
+                    for (int ix = 0; ix < sx; ++ix) {
+                        const int64_t o = bo + ((x + ix) % sx) * b.y_num;
+                        for (int iy = 0; iy < sy; ++iy) val += b.v[o + iy] * st.v[c++];
+                    }
+                }
+                out[i] = val;
+            }
+        };
+#pragma omp parallel for schedule(dynamic)
+        for (int64_t z = 0; z < z_num; ++z) {
+            const int tn =
+#ifdef _OPENMP
+                omp_get_thread_num();
+#else
+                0;
+#endif
+            RingBuf& b = bufs[tn];
+            int64_t zs = std::max<int64_t>(z - hz, 0), ze = std::min<int64_t>(z + hz + 1, z_num);
+            for (int64_t iz = zs; iz < ze; ++iz)
+                for (int64_t ix = 0; ix <= hx; ++ix) update(b, iz, ix);
+            for (int64_t iz = z; iz < hz; ++iz) bc_z(b, iz, true);
+            for (int64_t iz = z_num - hz; iz <= z; ++iz) bc_z(b, iz + 2 * hz, false);
+            for (int64_t ix = 0; ix < hx; ++ix) bc_x(b, ix, true);
+            run(b, z, 0);
+            for (int64_t x = 1; x < x_num; ++x) {
+                if (x < x_num - hx) {
+                    for (int64_t iz = zs; iz < ze; ++iz) update(b, iz, x + hx);
+                    for (int64_t iz = z; iz < hz; ++iz) bc_z(b, iz, true);
+                    for (int64_t iz = z_num - hz; iz <= z; ++iz) bc_z(b, iz + 2 * hz, false);
+                } else {
+                    bc_x(b, x + 2 * hx, false);
+                }
+                run(b, z, x);
+            }
+        }
+    }
+}
+
+/* APRNumerics::richardson_lucy (APRNumerics.hpp:549-593).  Optional stopping
+   rule (not in the reference): stop when ||u_k+1 - u_k||_2 / ||u_k||_2 < tol. */
+template <typename InT>
+int richardson_lucy(const GenInfo& g, const Linear& L, const Tree& t, const InT* in, float* u, const Mesh<float>& psf, int niter,
+                    bool use_ds, bool normalize, bool resume, double tol, double* last_rel) {
+    Mesh<float> flipped = psf;
+    for (size_t i = 0; i < psf.v.size(); ++i) flipped.v[i] = psf.v[psf.v.size() - 1 - i];
+    const int nst = use_ds ? g.l_max - g.l_min : 1;
+    std::vector<Mesh<float>> sv, fv;
+    downsampled_stencils(psf, sv, nst, normalize);
+    downsampled_stencils(flipped, fv, nst, normalize);
+    const size_t P = L.total;
+    if (!resume) std::fill(u, u + P, 1.0f);
+    std::vector<float> blur(P), est(P), tr;
+    int it = 0;
+    double rel = 0;
+    for (it = 0; it < niter; ++it) {
+        fill_tree_mean(g, L, t, u, tr);
+        convolve_pencil(g, L, t, fv, u, tr.data(), blur.data(), true);
+        for (size_t i = 0; i < P; ++i) blur[i] = (float)in[i] / blur[i];
+        fill_tree_mean(g, L, t, blur.data(), tr);
+        convolve_pencil(g, L, t, sv, blur.data(), tr.data(), est.data(), true);
+        double num = 0, den = 0;
+        for (size_t i = 0; i < P; ++i) {
+            const float nu = u[i] * est[i];
+            const double d = (double)nu - (double)u[i];
+            num += d * d;
+            den += (double)u[i] * (double)u[i];
+            u[i] = nu;
+        }
+        rel = den > 0 ? std::sqrt(num / den) : 0.0;
+        if (tol > 0 && rel < tol) {
+            ++it;
+            break;
+        }
+    }
+    if (last_rel) *last_rel = rel;
+    return it;
+}
+
 }  // namespace oracle
 
 #endif

@@ -25,7 +25,27 @@ namespace {
 struct Handle {
     AprResult r;
     int64_t dims[3];
+    bool has_tree = false;
+    Tree tree;
+    const Tree& get_tree() {
+        if (!has_tree) {
+            build_tree(r.info, r.lin, tree);
+            has_tree = true;
+        }
+        return tree;
+    }
 };
+
+std::vector<Mesh<float>> unpack_stencils(const float* flat, const int* dims, int n) {
+    std::vector<Mesh<float>> v(n);
+    size_t off = 0;
+    for (int i = 0; i < n; ++i) {
+        v[i].resize(dims[3 * i], dims[3 * i + 1], dims[3 * i + 2]);
+        std::copy(flat + off, flat + off + v[i].size(), v[i].v.begin());
+        off += v[i].size();
+    }
+    return v;
+}
 
 template <typename T>
 Mesh<T> wrap(const T* p, int64_t y, int64_t x, int64_t z) {
@@ -226,6 +246,110 @@ void orc_sample_image_u16(void* hp, const uint16_t* img, uint16_t* parts) {
 }
 
 void orc_apr_free(void* hp) { delete (Handle*)hp; }
+
+/* APR from given GenInfo dims + LinearAccess arrays (host copies of another build) */
+void* orc_apr_from_structure(int64_t y, int64_t x, int64_t z, const uint64_t* xz_end, uint64_t n_xz, const uint16_t* y_vec,
+                             uint64_t total) {
+    Handle* h = new Handle();
+    h->dims[0] = y; h->dims[1] = x; h->dims[2] = z;
+    h->r.info.init(y, x, z);
+    h->r.lin.init_xz(h->r.info);
+    if (h->r.lin.xz_end_vec.size() != n_xz) { delete h; return nullptr; }
+    std::copy(xz_end, xz_end + n_xz, h->r.lin.xz_end_vec.begin());
+    h->r.lin.y_vec.assign(y_vec, y_vec + total);
+    h->r.lin.total = total;
+    return h;
+}
+
+/* tree (APR::initialize_tree_dense): out = [l_min, l_max, total, n_xz, n_level_xz] */
+void orc_tree_info(void* hp, int64_t* out) {
+    Handle* h = (Handle*)hp;
+    const Tree& t = h->get_tree();
+    out[0] = t.info.l_min; out[1] = t.info.l_max; out[2] = (int64_t)t.lin.total;
+    out[3] = (int64_t)t.lin.xz_end_vec.size(); out[4] = (int64_t)t.lin.level_xz_vec.size();
+}
+
+void orc_tree_get(void* hp, uint64_t* level_xz, uint64_t* xz_end, uint16_t* y_vec) {
+    Handle* h = (Handle*)hp;
+    const Tree& t = h->get_tree();
+    std::copy(t.lin.level_xz_vec.begin(), t.lin.level_xz_vec.end(), level_xz);
+    std::copy(t.lin.xz_end_vec.begin(), t.lin.xz_end_vec.end(), xz_end);
+    std::copy(t.lin.y_vec.begin(), t.lin.y_vec.end(), y_vec);
+}
+
+void orc_fill_tree_mean_f32(void* hp, const float* parts, float* tree) {
+    Handle* h = (Handle*)hp;
+    std::vector<float> tr;
+    fill_tree_mean(h->r.info, h->r.lin, h->get_tree(), parts, tr);
+    std::copy(tr.begin(), tr.end(), tree);
+}
+
+void orc_fill_tree_mean_u16(void* hp, const uint16_t* parts, float* tree) {
+    Handle* h = (Handle*)hp;
+    std::vector<float> tr;
+    fill_tree_mean(h->r.info, h->r.lin, h->get_tree(), parts, tr);
+    std::copy(tr.begin(), tr.end(), tree);
+}
+
+/* get_downsampled_stencils: writes the stencils back to back into out and their
+   (y, x, z) dims into dims_out; returns the total number of floats */
+int64_t orc_downsampled_stencils(const float* st, int sy, int sx, int sz, int nlevels, int normalize, float* out, int64_t cap,
+                                 int* dims_out) {
+    Mesh<float> m(sy, sx, sz);
+    std::copy(st, st + m.size(), m.v.begin());
+    std::vector<Mesh<float>> v;
+    downsampled_stencils(m, v, nlevels, normalize != 0);
+    int64_t tot = 0;
+    for (auto& e : v) tot += (int64_t)e.size();
+    if (!out || tot > cap) return tot;
+    int64_t off = 0;
+    for (size_t i = 0; i < v.size(); ++i) {
+        std::copy(v[i].v.begin(), v[i].v.end(), out + off);
+        off += (int64_t)v[i].size();
+        dims_out[3 * i] = (int)v[i].y_num; dims_out[3 * i + 1] = (int)v[i].x_num; dims_out[3 * i + 2] = (int)v[i].z_num;
+    }
+    return tot;
+}
+
+/* APRFilter::convolve_pencil with a stencil stack (stencils[0] at l_max, ...) */
+void orc_convolve_pencil_f32(void* hp, const float* stencils, const int* dims, int n, const float* in, float* out, int reflect) {
+    Handle* h = (Handle*)hp;
+    auto st = unpack_stencils(stencils, dims, n);
+    std::vector<float> tr;
+    fill_tree_mean(h->r.info, h->r.lin, h->get_tree(), in, tr);
+    convolve_pencil(h->r.info, h->r.lin, h->get_tree(), st, in, tr.data(), out, reflect != 0);
+}
+
+void orc_convolve_pencil_u16(void* hp, const float* stencils, const int* dims, int n, const uint16_t* in, float* out, int reflect) {
+    Handle* h = (Handle*)hp;
+    auto st = unpack_stencils(stencils, dims, n);
+    std::vector<float> tr;
+    fill_tree_mean(h->r.info, h->r.lin, h->get_tree(), in, tr);
+    convolve_pencil(h->r.info, h->r.lin, h->get_tree(), st, in, tr.data(), out, reflect != 0);
+}
+
+/* APRFilter::convolve computes the same values in the same order as
+   convolve_pencil (full-plane instead of pencil buffer, APRFilter.hpp:971-1077) */
+void orc_convolve_f32(void* hp, const float* stencils, const int* dims, int n, const float* in, float* out, int reflect) {
+    orc_convolve_pencil_f32(hp, stencils, dims, n, in, out, reflect);
+}
+
+/* APRNumerics::richardson_lucy; tol > 0 adds the relative-update stopping rule */
+int orc_richardson_lucy_u16(void* hp, const uint16_t* in, float* u, const float* psf, int sy, int sx, int sz, int niter, int use_ds,
+                            int normalize, int resume, double tol, double* rel) {
+    Handle* h = (Handle*)hp;
+    Mesh<float> m(sy, sx, sz);
+    std::copy(psf, psf + m.size(), m.v.begin());
+    return richardson_lucy(h->r.info, h->r.lin, h->get_tree(), in, u, m, niter, use_ds != 0, normalize != 0, resume != 0, tol, rel);
+}
+
+int orc_richardson_lucy_f32(void* hp, const float* in, float* u, const float* psf, int sy, int sx, int sz, int niter, int use_ds,
+                            int normalize, int resume, double tol, double* rel) {
+    Handle* h = (Handle*)hp;
+    Mesh<float> m(sy, sx, sz);
+    std::copy(psf, psf + m.size(), m.v.begin());
+    return richardson_lucy(h->r.info, h->r.lin, h->get_tree(), in, u, m, niter, use_ds != 0, normalize != 0, resume != 0, tol, rel);
+}
 
 int orc_num_threads(void) {
 #ifdef _OPENMP

@@ -26,6 +26,10 @@
 #include "data_structures/APR/APR.hpp"
 #include "data_structures/APR/access/LinearAccess.hpp"
 #include "data_structures/APR/particles/ParticleData.hpp"
+#include "numerics/APRStencil.hpp"
+#include "numerics/APRTreeNumerics.hpp"
+#include "numerics/APRFilter.hpp"
+#include "numerics/APRNumerics.hpp"
 
 #include "../include/aprh_params.h"
 
@@ -101,6 +105,23 @@ void finish(Handle* h, APRParameters& par, PixelData<uint16_t>& grad_temp, Pixel
 
 void set_err(char* buf, int len, const char* msg) {
     if (buf && len > 0) std::snprintf(buf, (size_t)len, "%s", msg);
+}
+
+template <typename T>
+void fill_parts(ParticleData<T>& p, const T* v, size_t n) {
+    p.init(n);
+    std::copy(v, v + n, p.data.begin());
+}
+
+static std::vector<PixelData<float>> unpack(const float* flat, const int* dims, int n) {
+    std::vector<PixelData<float>> v(n);
+    size_t off = 0;
+    for (int i = 0; i < n; ++i) {
+        v[i].init(dims[3 * i], dims[3 * i + 1], dims[3 * i + 2]);
+        std::copy(flat + off, flat + off + v[i].mesh.size(), v[i].mesh.begin());
+        off += v[i].mesh.size();
+    }
+    return v;
 }
 
 }  // namespace
@@ -335,6 +356,136 @@ void orc_sample_image_u16(void* hp, const uint16_t* img, uint16_t* parts) {
 }
 
 void orc_apr_free(void* hp) { delete (Handle*)hp; }
+
+void* orc_apr_from_structure(int64_t y, int64_t x, int64_t z, const uint64_t* xz_end, uint64_t n_xz, const uint16_t* y_vec,
+                             uint64_t total) {
+    Handle* h = new Handle();
+    h->dims[0] = y; h->dims[1] = x; h->dims[2] = z;
+    h->apr.aprInfo.init(y, x, z);
+    h->apr.linearAccess.genInfo = &h->apr.aprInfo;
+    h->apr.linearAccess.initialize_xz_linear();
+    if (h->apr.linearAccess.xz_end_vec.size() != n_xz) { delete h; return nullptr; }
+    std::copy(xz_end, xz_end + n_xz, h->apr.linearAccess.xz_end_vec.begin());
+    h->apr.linearAccess.y_vec.resize(total);
+    std::copy(y_vec, y_vec + total, h->apr.linearAccess.y_vec.begin());
+    h->apr.aprInfo.total_number_particles = total;
+    h->apr.apr_initialized = true;
+    return h;
+}
+
+void orc_tree_info(void* hp, int64_t* out) {
+    Handle* h = (Handle*)hp;
+    auto it = h->apr.tree_iterator();
+    out[0] = h->apr.treeInfo.l_min; out[1] = h->apr.treeInfo.l_max;
+    out[2] = (int64_t)h->apr.linearAccessTree.y_vec.size();
+    out[3] = (int64_t)h->apr.linearAccessTree.xz_end_vec.size();
+    out[4] = (int64_t)h->apr.linearAccessTree.level_xz_vec.size();
+    (void)it;
+}
+
+void orc_tree_get(void* hp, uint64_t* level_xz, uint64_t* xz_end, uint16_t* y_vec) {
+    Handle* h = (Handle*)hp;
+    auto it = h->apr.tree_iterator();
+    (void)it;
+    auto& la = h->apr.linearAccessTree;
+    std::copy(la.level_xz_vec.begin(), la.level_xz_vec.end(), level_xz);
+    std::copy(la.xz_end_vec.begin(), la.xz_end_vec.end(), xz_end);
+    std::copy(la.y_vec.begin(), la.y_vec.end(), y_vec);
+}
+
+void orc_fill_tree_mean_f32(void* hp, const float* parts, float* tree) {
+    Handle* h = (Handle*)hp;
+    ParticleData<float> p, t;
+    fill_parts(p, parts, h->apr.total_number_particles());
+    APRTreeNumerics::fill_tree_mean(h->apr, p, t);
+    std::copy(t.data.begin(), t.data.end(), tree);
+}
+
+void orc_fill_tree_mean_u16(void* hp, const uint16_t* parts, float* tree) {
+    Handle* h = (Handle*)hp;
+    ParticleData<uint16_t> p;
+    ParticleData<float> t;
+    fill_parts(p, parts, h->apr.total_number_particles());
+    APRTreeNumerics::fill_tree_mean(h->apr, p, t);
+    std::copy(t.data.begin(), t.data.end(), tree);
+}
+
+int64_t orc_downsampled_stencils(const float* st, int sy, int sx, int sz, int nlevels, int normalize, float* out, int64_t cap,
+                                 int* dims_out) {
+    PixelData<float> m(sy, sx, sz);
+    std::copy(st, st + (size_t)sy * sx * sz, m.mesh.begin());
+    std::vector<PixelData<float>> v;
+    APRStencil::get_downsampled_stencils(m, v, nlevels, normalize != 0);
+    int64_t tot = 0;
+    for (auto& e : v) tot += (int64_t)e.mesh.size();
+    if (!out || tot > cap) return tot;
+    int64_t off = 0;
+    for (size_t i = 0; i < v.size(); ++i) {
+        std::copy(v[i].mesh.begin(), v[i].mesh.begin() + v[i].mesh.size(), out + off);
+        off += (int64_t)v[i].mesh.size();
+        dims_out[3 * i] = v[i].y_num; dims_out[3 * i + 1] = v[i].x_num; dims_out[3 * i + 2] = v[i].z_num;
+    }
+    return tot;
+}
+
+void orc_convolve_pencil_f32(void* hp, const float* stencils, const int* dims, int n, const float* in, float* out, int reflect) {
+    Handle* h = (Handle*)hp;
+    auto st = unpack(stencils, dims, n);
+    ParticleData<float> pi, po;
+    fill_parts(pi, in, h->apr.total_number_particles());
+    APRFilter::convolve_pencil(h->apr, st, pi, po, reflect != 0);
+    std::copy(po.data.begin(), po.data.end(), out);
+}
+
+void orc_convolve_pencil_u16(void* hp, const float* stencils, const int* dims, int n, const uint16_t* in, float* out, int reflect) {
+    Handle* h = (Handle*)hp;
+    auto st = unpack(stencils, dims, n);
+    ParticleData<uint16_t> pi;
+    ParticleData<float> po;
+    fill_parts(pi, in, h->apr.total_number_particles());
+    APRFilter::convolve_pencil(h->apr, st, pi, po, reflect != 0);
+    std::copy(po.data.begin(), po.data.end(), out);
+}
+
+void orc_convolve_f32(void* hp, const float* stencils, const int* dims, int n, const float* in, float* out, int reflect) {
+    Handle* h = (Handle*)hp;
+    auto st = unpack(stencils, dims, n);
+    ParticleData<float> pi, po;
+    fill_parts(pi, in, h->apr.total_number_particles());
+    APRFilter::convolve(h->apr, st, pi, po, reflect != 0);
+    std::copy(po.data.begin(), po.data.end(), out);
+}
+
+int orc_richardson_lucy_u16(void* hp, const uint16_t* in, float* u, const float* psf, int sy, int sx, int sz, int niter, int use_ds,
+                            int normalize, int resume, double tol, double* rel) {
+    (void)tol;
+    Handle* h = (Handle*)hp;
+    PixelData<float> m(sy, sx, sz);
+    std::copy(psf, psf + (size_t)sy * sx * sz, m.mesh.begin());
+    ParticleData<uint16_t> pi;
+    ParticleData<float> po;
+    fill_parts(pi, in, h->apr.total_number_particles());
+    if (resume) fill_parts(po, u, h->apr.total_number_particles());
+    APRNumerics::richardson_lucy(h->apr, pi, po, m, niter, use_ds != 0, normalize != 0, resume != 0);
+    std::copy(po.data.begin(), po.data.end(), u);
+    if (rel) *rel = 0;
+    return niter;
+}
+
+int orc_richardson_lucy_f32(void* hp, const float* in, float* u, const float* psf, int sy, int sx, int sz, int niter, int use_ds,
+                            int normalize, int resume, double tol, double* rel) {
+    (void)tol;
+    Handle* h = (Handle*)hp;
+    PixelData<float> m(sy, sx, sz);
+    std::copy(psf, psf + (size_t)sy * sx * sz, m.mesh.begin());
+    ParticleData<float> pi, po;
+    fill_parts(pi, in, h->apr.total_number_particles());
+    if (resume) fill_parts(po, u, h->apr.total_number_particles());
+    APRNumerics::richardson_lucy(h->apr, pi, po, m, niter, use_ds != 0, normalize != 0, resume != 0);
+    std::copy(po.data.begin(), po.data.end(), u);
+    if (rel) *rel = 0;
+    return niter;
+}
 
 int orc_num_threads(void) {
 #ifdef _OPENMP

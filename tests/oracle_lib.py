@@ -67,6 +67,22 @@ def _bind(lib):
     lib.orc_sample_image_u16.argtypes = [C.c_void_p, u16p, u16p]
     lib.orc_apr_free.argtypes = [C.c_void_p]
     lib.orc_num_threads.restype = C.c_int
+    lib.orc_apr_from_structure.argtypes = [i64, i64, i64, u64p, C.c_uint64, u16p, C.c_uint64]
+    lib.orc_apr_from_structure.restype = C.c_void_p
+    lib.orc_tree_info.argtypes = [C.c_void_p, np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")]
+    lib.orc_tree_get.argtypes = [C.c_void_p, u64p, u64p, u16p]
+    lib.orc_fill_tree_mean_f32.argtypes = [C.c_void_p, f32p, f32p]
+    lib.orc_fill_tree_mean_u16.argtypes = [C.c_void_p, u16p, f32p]
+    i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+    lib.orc_downsampled_stencils.argtypes = [f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, i64, C.c_void_p]
+    lib.orc_downsampled_stencils.restype = i64
+    for n in ("orc_convolve_pencil_f32", "orc_convolve_f32"):
+        getattr(lib, n).argtypes = [C.c_void_p, f32p, i32p, C.c_int, f32p, f32p, C.c_int]
+    lib.orc_convolve_pencil_u16.argtypes = [C.c_void_p, f32p, i32p, C.c_int, u16p, f32p, C.c_int]
+    for n, t in (("orc_richardson_lucy_u16", u16p), ("orc_richardson_lucy_f32", f32p)):
+        getattr(lib, n).argtypes = [C.c_void_p, t, f32p, f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.c_double, C.POINTER(C.c_double)]
+        getattr(lib, n).restype = C.c_int
     return lib
 
 
@@ -123,6 +139,48 @@ class OracleAPR:
         self.lib.orc_sample_image_u16(self.h, np.ascontiguousarray(img).reshape(-1), parts)
         return parts[:self.total]
 
+    # ---- particle operations ----
+    def tree(self):
+        info = np.zeros(5, np.int64)
+        self.lib.orc_tree_info(self.h, info)
+        tl_min, tl_max, tot, nxz, nlxz = (int(v) for v in info)
+        lxz = np.zeros(nlxz, np.uint64)
+        xze = np.zeros(nxz, np.uint64)
+        yv = np.zeros(max(tot, 1), np.uint16)
+        self.lib.orc_tree_get(self.h, lxz, xze, yv)
+        return dict(l_min=tl_min, l_max=tl_max, total=tot, level_xz_vec=lxz, xz_end_vec=xze, y_vec=yv[:tot])
+
+    def fill_tree_mean(self, parts):
+        t = self.tree()
+        out = np.zeros(max(t["total"], 1), np.float32)
+        if parts.dtype == np.uint16:
+            self.lib.orc_fill_tree_mean_u16(self.h, np.ascontiguousarray(parts), out)
+        else:
+            self.lib.orc_fill_tree_mean_f32(self.h, np.ascontiguousarray(parts, np.float32), out)
+        return out[:t["total"]]
+
+    def convolve_pencil(self, stencils, parts, reflect=True, which="pencil"):
+        flat = np.ascontiguousarray(np.concatenate([s.reshape(-1) for s in stencils]).astype(np.float32))
+        dims = np.ascontiguousarray(np.array([[s.shape[2], s.shape[1], s.shape[0]] for s in stencils], np.int32).reshape(-1))
+        out = np.zeros(max(self.total, 1), np.float32)
+        if parts.dtype == np.uint16:
+            self.lib.orc_convolve_pencil_u16(self.h, flat, dims, len(stencils), np.ascontiguousarray(parts), out, int(reflect))
+        else:
+            f = self.lib.orc_convolve_pencil_f32 if which == "pencil" else self.lib.orc_convolve_f32
+            f(self.h, flat, dims, len(stencils), np.ascontiguousarray(parts, np.float32), out, int(reflect))
+        return out[:self.total]
+
+    def richardson_lucy(self, parts, psf, niter, use_ds=True, normalize=True, u=None, tol=0.0):
+        resume = u is not None
+        u = np.ones(max(self.total, 1), np.float32) if u is None else np.ascontiguousarray(u, np.float32).copy()
+        rel = C.c_double(0)
+        z, x, y = psf.shape
+        f = self.lib.orc_richardson_lucy_u16 if parts.dtype == np.uint16 else self.lib.orc_richardson_lucy_f32
+        p = np.ascontiguousarray(parts) if parts.dtype == np.uint16 else np.ascontiguousarray(parts, np.float32)
+        it = f(self.h, p, u, np.ascontiguousarray(psf, np.float32).reshape(-1), y, x, z, niter, int(use_ds), int(normalize),
+               int(resume), tol, C.byref(rel))
+        return u[:self.total], it, rel.value
+
     def __del__(self):
         try:
             self.lib.orc_apr_free(self.h)
@@ -148,3 +206,60 @@ def apr_from_grad_lis(grad, lis, intensity, dims, par, offset, which="oracle"):
                                   np.ascontiguousarray(lis, np.float32).reshape(-1),
                                   np.ascontiguousarray(intensity, np.float32).reshape(-1), y, x, z, C.byref(par), offset)
     return OracleAPR(lib, h, dims)
+
+
+def apr_from_structure(dims, xz_end_vec, y_vec, which="oracle"):
+    """oracle APR handle built from a LinearAccess (e.g. the GPU's)"""
+    lib = load(which)
+    y, x, z = dims
+    xze = np.ascontiguousarray(xz_end_vec, np.uint64)
+    yv = np.ascontiguousarray(y_vec, np.uint16)
+    h = lib.orc_apr_from_structure(y, x, z, xze, xze.size, yv if yv.size else np.zeros(1, np.uint16), yv.size)
+    if not h:
+        raise ValueError("structure does not match the dimensions")
+    return OracleAPR(lib, h, dims)
+
+
+def downsampled_stencils(stencil_zxy, nlevels, normalize, which="oracle"):
+    lib = load(which)
+    z, x, y = stencil_zxy.shape
+    st = np.ascontiguousarray(stencil_zxy, np.float32).reshape(-1)
+    tot = lib.orc_downsampled_stencils(st, y, x, z, nlevels, int(normalize), None, 0, None)
+    out = np.zeros(tot, np.float32)
+    dims = np.zeros(3 * max(nlevels, 1), np.int32)
+    lib.orc_downsampled_stencils(st, y, x, z, nlevels, int(normalize), out.ctypes.data, tot, dims.ctypes.data)
+    res, off = [], 0
+    for i in range(max(nlevels, 1)):
+        yy, xx, zz = dims[3 * i: 3 * i + 3]
+        n = int(yy) * int(xx) * int(zz)
+        res.append(out[off:off + n].reshape(int(zz), int(xx), int(yy)))
+        off += n
+    return res
+
+
+def gaussian_stencil(sigma, size, normalize=True):
+    """APRStencil::create_gaussian_filter (APRStencil.hpp:173-228): separable float
+    Gaussian, expf on the host, product gauss_y*gauss_x*gauss_z, sum in float."""
+    import math
+    sy, sx, sz = size
+    def g1(s, n):
+        if n <= 1:
+            return np.ones(1, np.float32)
+        v = []
+        for i in range(-(n // 2), n // 2 + 1):
+            num = np.float32(-i * i) / (np.float32(2) * np.float32(s) * np.float32(s))
+            v.append(np.float32(np.exp(np.float32(num), dtype=np.float32)) /
+                     (np.float32(s) * np.float32(math.sqrt(2 * math.pi))))
+        return np.array(v, np.float32)
+    gy, gx, gz = g1(sigma[0], sy), g1(sigma[1], sx), g1(sigma[2], sz)
+    st = np.zeros((sz, sx, sy), np.float32)
+    tot = np.float32(0)
+    for i in range(sy):
+        for j in range(sx):
+            for k in range(sz):
+                v = np.float32(np.float32(gy[i] * gx[j]) * gz[k])
+                st[k, j, i] = v
+                tot = np.float32(tot + v)
+    if normalize:
+        st = (st / tot).astype(np.float32)
+    return st

@@ -1,0 +1,212 @@
+"""Pins the oracle's particle operations (tree, fill_tree_mean, downsampled
+stencils, convolve_pencil / convolve, richardson_lucy) -- CPU only.
+
+* bit equality with the reference build (oracle/_ref) on synthetic APRs;
+* the reference test's ground truth for convolution (test/FilterTestHelpers.hpp:
+  98-192): reconstruct each level piecewise-constant (coarser particles fill their
+  blocks, finer regions take the tree means) and apply the stencil densely with
+  reflective or zero boundaries, in float64 -- tolerance 1e-4 relative like
+  compareParticles (TestTools.hpp:93-111);
+* closed forms: a constant input gives constant tree means; normalised stencils
+  sum to 1; Richardson-Lucy keeps a constant image constant.
+"""
+import numpy as np
+import pytest
+
+import libapr_amd as A
+import oracle_lib as O
+
+CASES = [((32, 64, 64), 20, 0), ((29, 45, 61), 12, 3), ((16, 16, 130), 8, 4)]
+
+
+def make_apr(shape, nb, seed, which="oracle"):
+    img = A.synth_blobs(shape, seed, nb)
+    par = O.params(rel_error=0.1, sigma_th=50, Ip_th=120, grad_th=10)
+    return img, O.get_apr(img, par, which)
+
+
+def need_ref():
+    if O.load("ref") is None:
+        pytest.skip("reference build (oracle/_ref) not available")
+
+
+def particle_coords(apr):
+    """(level, z, x, y) of every particle, in particle order"""
+    lv, zz, xx = np.zeros(apr.total, np.int64), np.zeros(apr.total, np.int64), np.zeros(apr.total, np.int64)
+    y, x, z = apr.dims
+    lxz = apr.level_xz_vec.astype(np.int64)
+    xze = apr.xz_end_vec.astype(np.int64)
+    for l in range(apr.l_min, apr.l_max + 1):
+        s = 2 ** (apr.l_max - l)
+        xn, zn = -(-x // s), -(-z // s)
+        rows = np.arange(xn * zn)
+        b = xze[lxz[l] + rows - 1]
+        e = xze[lxz[l] + rows]
+        for r in np.nonzero(e > b)[0]:
+            lv[b[r]:e[r]] = l
+            zz[b[r]:e[r]] = r // xn
+            xx[b[r]:e[r]] = r % xn
+    return lv, zz, xx, apr.y_vec.astype(np.int64)
+
+
+def tree_coords(apr, t):
+    y, x, z = apr.dims
+    n = t["total"]
+    lv, zz, xx = np.zeros(n, np.int64), np.zeros(n, np.int64), np.zeros(n, np.int64)
+    lxz, xze = t["level_xz_vec"].astype(np.int64), t["xz_end_vec"].astype(np.int64)
+    for l in range(t["l_min"], t["l_max"] + 1):
+        s = 2 ** (apr.l_max - l)
+        xn, zn = -(-x // s), -(-z // s)
+        rows = np.arange(xn * zn)
+        b, e = xze[lxz[l] + rows - 1], xze[lxz[l] + rows]
+        for r in np.nonzero(e > b)[0]:
+            lv[b[r]:e[r]] = l
+            zz[b[r]:e[r]] = r // xn
+            xx[b[r]:e[r]] = r % xn
+    return lv, zz, xx, t["y_vec"].astype(np.int64)
+
+
+def recon_level(apr, parts, tree, t, level):
+    """piecewise-constant reconstruction at `level` (APRReconstruction.hpp:297-394)"""
+    y, x, z = apr.dims
+    s = 2 ** (apr.l_max - level)
+    dims = (-(-z // s), -(-x // s), -(-y // s))
+    img = np.full(dims, np.nan)
+    lv, pz, px, py = particle_coords(apr)
+    for l in range(apr.l_min, level + 1):
+        m = lv == l
+        st = 2 ** (level - l)
+        for (a, b, c, v) in zip(pz[m], px[m], py[m], parts[m]):
+            img[a * st:(a + 1) * st, b * st:(b + 1) * st, c * st:(c + 1) * st] = v
+    if level < apr.l_max:
+        tl, tz, tx, ty = tree_coords(apr, t)
+        m = tl == level
+        img[tz[m], tx[m], ty[m]] = tree[m]
+    return img
+
+
+def conv_gt(apr, stencils, parts, tree, t, reflect):
+    """FilterTestHelpers::compute_convolution_gt in float64"""
+    out = np.zeros(apr.total)
+    lv, pz, px, py = particle_coords(apr)
+    for li, level in enumerate(range(apr.l_max, apr.l_min - 1, -1)):
+        st = stencils[min(li, len(stencils) - 1)].astype(np.float64)
+        hz, hx, hy = [(n - 1) // 2 for n in st.shape]
+        img = recon_level(apr, parts, tree, t, level)
+        Z, X, Y = img.shape
+        if reflect:
+            pad = np.pad(img, ((hz, hz), (hx, hx), (hy, hy)), mode="reflect")
+        else:
+            pad = np.pad(img, ((hz, hz), (hx, hx), (hy, hy)), mode="constant")
+        m = np.nonzero(lv == level)[0]
+        for i in m:
+            blk = pad[pz[i]:pz[i] + 2 * hz + 1, px[i]:px[i] + 2 * hx + 1, py[i]:py[i] + 2 * hy + 1]
+            out[i] = float((blk * st).sum())
+    return out
+
+
+def stencil_stack(kind, apr, normalize):
+    if kind == "mean3":
+        st = np.full((3, 3, 3), 1.0 / 27, np.float32)
+    elif kind == "gauss5":
+        st = O.gaussian_stencil((1.0, 1.0, 1.0), (5, 5, 5), True)
+    else:
+        st = O.gaussian_stencil((1.0, 1.5, 2.0), (3, 5, 7), False)
+    return O.downsampled_stencils(st, apr.l_max - apr.l_min, normalize, "oracle")
+
+
+@pytest.mark.parametrize("shape,nb,seed", CASES)
+def test_tree_and_tree_mean_equal_reference(shape, nb, seed):
+    need_ref()
+    img, o = make_apr(shape, nb, seed, "oracle")
+    _, r = make_apr(shape, nb, seed, "ref")
+    to, tr = o.tree(), r.tree()
+    assert to["total"] == tr["total"]
+    for k in ("level_xz_vec", "xz_end_vec", "y_vec"):
+        assert np.array_equal(to[k], tr[k])
+    parts = o.sample(img)
+    assert np.array_equal(o.fill_tree_mean(parts).view(np.uint32), r.fill_tree_mean(parts).view(np.uint32))
+    pf = (parts.astype(np.float32) * np.float32(1.37)).astype(np.float32)
+    assert np.array_equal(o.fill_tree_mean(pf).view(np.uint32), r.fill_tree_mean(pf).view(np.uint32))
+
+
+@pytest.mark.parametrize("shape,nb,seed", CASES[:2])
+def test_tree_mean_constant(shape, nb, seed):
+    """boundary scale factors make a constant input give a constant tree"""
+    img, o = make_apr(shape, nb, seed)
+    tm = o.fill_tree_mean(np.full(o.total, 7.0, np.float32))
+    np.testing.assert_allclose(tm, 7.0, rtol=1e-6)
+
+
+@pytest.mark.parametrize("kind", ["mean3", "gauss5", "aniso"])
+@pytest.mark.parametrize("normalize", [False, True])
+def test_stencils_equal_reference(kind, normalize):
+    need_ref()
+    img, o = make_apr((32, 64, 64), 20, 0)
+    if kind == "mean3":
+        st = np.full((3, 3, 3), 1.0 / 27, np.float32)
+    elif kind == "gauss5":
+        st = O.gaussian_stencil((1.0, 1.0, 1.0), (5, 5, 5), True)
+    else:
+        st = O.gaussian_stencil((1.0, 1.5, 2.0), (3, 5, 7), False)
+    so = O.downsampled_stencils(st, 6, normalize, "oracle")
+    sr = O.downsampled_stencils(st, 6, normalize, "ref")
+    for a, b in zip(so, sr):
+        assert a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        if normalize:
+            assert abs(float(a.astype(np.float64).sum()) - 1.0) < 1e-5
+
+
+@pytest.mark.parametrize("shape,nb,seed", CASES)
+@pytest.mark.parametrize("kind", ["mean3", "gauss5", "aniso"])
+@pytest.mark.parametrize("reflect", [True, False])
+def test_convolve_equal_reference(shape, nb, seed, kind, reflect):
+    need_ref()
+    img, o = make_apr(shape, nb, seed, "oracle")
+    _, r = make_apr(shape, nb, seed, "ref")
+    parts = o.sample(img)
+    st = stencil_stack(kind, o, True)
+    pf = parts.astype(np.float32)
+    co = o.convolve_pencil(st, pf, reflect)
+    assert np.array_equal(co.view(np.uint32), r.convolve_pencil(st, pf, reflect).view(np.uint32))
+    assert np.array_equal(co.view(np.uint32), r.convolve_pencil(st, pf, reflect, which="full").view(np.uint32))
+    assert np.array_equal(o.convolve_pencil(st, parts, reflect).view(np.uint32),
+                          r.convolve_pencil(st, parts, reflect).view(np.uint32))
+
+
+@pytest.mark.parametrize("kind", ["mean3", "gauss5", "aniso"])
+@pytest.mark.parametrize("reflect", [True, False])
+def test_convolve_ground_truth(kind, reflect):
+    img, o = make_apr((29, 45, 61), 12, 3)
+    parts = o.sample(img).astype(np.float32)
+    st = stencil_stack(kind, o, True)
+    out = o.convolve_pencil(st, parts, reflect)
+    t = o.tree()
+    gt = conv_gt(o, st, parts.astype(np.float64), o.fill_tree_mean(parts).astype(np.float64), t, reflect)
+    rel = np.abs(out - gt) / np.maximum(np.abs(gt), 1e-3)
+    assert rel.max() < 1e-4, rel.max()
+
+
+@pytest.mark.parametrize("shape,nb,seed", CASES[:2])
+def test_richardson_lucy_equal_reference(shape, nb, seed):
+    need_ref()
+    img, o = make_apr(shape, nb, seed, "oracle")
+    _, r = make_apr(shape, nb, seed, "ref")
+    parts = o.sample(img)
+    psf = O.gaussian_stencil((1.0, 1.0, 2.0), (9, 9, 9), True)
+    uo, it, _ = o.richardson_lucy(parts, psf, 3)
+    ur, _, _ = r.richardson_lucy(parts, psf, 3)
+    assert it == 3 and np.array_equal(uo.view(np.uint32), ur.view(np.uint32))
+    # resume continues from the given estimate (APRNumerics.hpp:559-562)
+    uo2, _, _ = o.richardson_lucy(parts, psf, 2, u=uo)
+    ur2, _, _ = r.richardson_lucy(parts, psf, 2, u=ur)
+    assert np.array_equal(uo2.view(np.uint32), ur2.view(np.uint32))
+
+
+def test_richardson_lucy_constant():
+    img, o = make_apr((32, 64, 64), 20, 0)
+    psf = O.gaussian_stencil((1.0, 1.0, 2.0), (9, 9, 9), True)
+    u, it, rel = o.richardson_lucy(np.full(o.total, 5.0, np.float32), psf, 4)
+    np.testing.assert_allclose(u, 5.0, rtol=2e-5)
+    u, it, rel = o.richardson_lucy(o.sample(img), psf, 200, tol=1e-3)
+    assert it < 200 and rel < 1e-3 and (u > 0).all()
