@@ -93,6 +93,70 @@ int aprh_apr_destroy(aprh_apr* apr);
 /* ParticleData<uint16_t>::sample_image (ParticleData.hpp:122-127, :221-261) */
 int aprh_sample_image_u16(aprh_context* ctx, const aprh_apr* apr, const uint16_t* image, uint16_t* parts);
 
+/* ---- particle operations ---- */
+/* element type of particle buffers (ParticleData<uint16_t> / ParticleData<float>) */
+typedef enum aprh_dtype { APRH_DTYPE_U16 = 0, APRH_DTYPE_F32 = 1 } aprh_dtype;
+
+/* tree (interior cell) summary: GenInfo::init_tree (GenInfo.hpp:68-99) and the tree
+   LinearAccess (APR::initialize_tree_dense, APR.hpp:633-689) */
+typedef struct aprh_tree_info {
+    int32_t l_min, l_max;
+    uint64_t total_number_tree_particles;
+    uint64_t xz_end_size;
+    uint64_t level_xz_size;
+} aprh_tree_info;
+
+/* outcome of aprh_richardson_lucy */
+typedef struct aprh_rl_stats {
+    int32_t iterations;             /* iterations run                                       */
+    double last_rel_update;         /* ||u_k+1 - u_k||_2 / ||u_k||_2 of the last iteration   */
+} aprh_rl_stats;
+
+/* APR from a stored structure, as APRFile::read_apr restores LinearAccess
+   (LinearAccess.hpp:33-35): xz_end_vec[xz_end_size] for the GenInfo of (y, x, z) and
+   y_vec[total].  Host or device pointers; the structure is validated. */
+int aprh_apr_from_structure(aprh_context* ctx, int64_t y_num, int64_t x_num, int64_t z_num, const uint64_t* xz_end_vec,
+                            uint64_t xz_end_size, const uint16_t* y_vec, uint64_t total, aprh_apr** out);
+/* APR::initialize_tree_dense (APR.hpp:633-689) on first use; tree GenInfo + sizes */
+int aprh_apr_tree_info(aprh_context* ctx, aprh_apr* apr, aprh_tree_info* info);
+/* tree LinearAccess (LinearAccess.hpp:67-122) out; any pointer may be NULL to skip */
+int aprh_apr_copy_tree(aprh_context* ctx, aprh_apr* apr, uint64_t* level_xz_vec, uint64_t* xz_end_vec, uint16_t* y_vec);
+/* APRTreeNumerics::fill_tree_mean (APRTreeNumerics.hpp:63-202): parts[total] -> tree[tree total] */
+int aprh_fill_tree_mean(aprh_context* ctx, aprh_apr* apr, const void* parts, int dtype, float* tree_out);
+/* APRStencil::get_downsampled_stencils (APRStencil.hpp:81-97, downsample_stencil :12-78), host
+   only.  Stencil layout: y fastest, then x, then z (PixelData mesh order); sizes odd.
+   With out == NULL only *needed (total floats) is written; dims_out gets (y, x, z) per level. */
+int aprh_downsampled_stencils(const float* stencil, int32_t sy, int32_t sx, int32_t sz, int32_t nlevels, int normalize, float* out,
+                              int64_t out_capacity, int32_t* dims_out, int64_t* needed);
+/* APRStencil::create_gaussian_filter (APRStencil.hpp:173-228), host only: sigma and size
+   per axis in (y, x, z) order; out[size_y*size_x*size_z], y fastest */
+int aprh_gaussian_stencil(const float* sigma3, const int32_t* size3, int normalize, float* out);
+/* APRStencil::create_mean_filter (APRStencil.hpp:236-244), host only */
+int aprh_mean_stencil(const int32_t* size3, float* out);
+/* APRFilter::convolve_pencil, vector-of-stencils overload (APRFilter.hpp:89-91, :1094-1216):
+   stencils[0] applies to level l_max, stencils[k] to level l_max - k, the last one to every
+   coarser level; dims holds (y, x, z) per stencil, all odd. */
+int aprh_convolve_pencil(aprh_context* ctx, aprh_apr* apr, const float* stencils, const int32_t* dims, int32_t n_stencils,
+                         const void* parts_in, int dtype, float* parts_out, int reflect);
+/* APRFilter::convolve (APRFilter.hpp:67-69, :971-1077): same result as convolve_pencil */
+int aprh_convolve(aprh_context* ctx, aprh_apr* apr, const float* stencils, const int32_t* dims, int32_t n_stencils,
+                  const void* parts_in, int dtype, float* parts_out, int reflect);
+/* APRFilter::convolve_pencil / convolve with one stencil (APRFilter.hpp:110-120, :139-149):
+   get_downsampled_stencils(stencil, use_stencil_downsample ? l_max - l_min : 1, normalize) first */
+int aprh_convolve_stencil(aprh_context* ctx, aprh_apr* apr, const float* stencil, int32_t sy, int32_t sx, int32_t sz,
+                          const void* parts_in, int dtype, float* parts_out, int reflect, int use_stencil_downsample, int normalize);
+/* APRNumerics::richardson_lucy (APRNumerics.hpp:130-134, :549-593): u (f32[total]) is set to 1
+   unless resume; at most max_iter iterations; with tol > 0 it stops after the first iteration
+   whose relative update ||u_k+1 - u_k|| / ||u_k|| is below tol (no reference counterpart). */
+int aprh_richardson_lucy(aprh_context* ctx, aprh_apr* apr, const void* parts_in, int dtype, float* u, const float* psf, int32_t sy,
+                         int32_t sx, int32_t sz, int32_t max_iter, int use_stencil_downsample, int normalize, int resume, double tol,
+                         aprh_rl_stats* stats);
+/* APRReconstruction::reconstruct_constant (APRReconstruction.hpp:80-120; with level_delta < 0
+   the full-volume patch form :297-394, tree means from fill_tree_mean).  image has the
+   dimensions of level l_max + level_delta, (y, x, z) with y fastest. */
+int aprh_reconstruct_constant(aprh_context* ctx, aprh_apr* apr, const void* parts, int parts_dtype, int level_delta, void* image,
+                              int image_dtype);
+
 /* ---- stage-level entry points (public methods of the reference's stage classes) ---- */
 /* ComputeGradient::bspline_filt_rec_y/_x/_z (ComputeGradient.hpp:248-413, 415-599, 601-779); axis 0=y 1=x 2=z.
    In place.  Returns APRH_ERR_OVERFLOW when a rounded u16 value leaves [0,65535]. */

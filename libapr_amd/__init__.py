@@ -57,6 +57,18 @@ class AprhAprInfo(C.Structure):
                 ("bspline_offset", C.c_float)]
 
 
+class AprhTreeInfo(C.Structure):
+    _fields_ = [("l_min", C.c_int32), ("l_max", C.c_int32), ("total_number_tree_particles", C.c_uint64),
+                ("xz_end_size", C.c_uint64), ("level_xz_size", C.c_uint64)]
+
+
+class AprhRLStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("last_rel_update", C.c_double)]
+
+
+APRH_DTYPE_U16 = 0
+APRH_DTYPE_F32 = 1
+
 _lib = None
 vp = C.c_void_p
 i64 = C.c_int64
@@ -98,6 +110,22 @@ def lib():
                                         C.c_float, C.c_int]
     L.aprh_synth_blobs_device.argtypes = [vp, vp, i64, i64, i64, C.c_uint64, i64, C.c_float, C.c_float, C.c_float,
                                           C.c_float, C.c_float, C.c_int]
+    i32p = C.POINTER(C.c_int32)
+    L.aprh_apr_from_structure.argtypes = [vp, i64, i64, i64, vp, C.c_uint64, vp, C.c_uint64, C.POINTER(vp)]
+    L.aprh_apr_tree_info.argtypes = [vp, vp, C.POINTER(AprhTreeInfo)]
+    L.aprh_apr_copy_tree.argtypes = [vp, vp, vp, vp, vp]
+    L.aprh_fill_tree_mean.argtypes = [vp, vp, vp, C.c_int, vp]
+    L.aprh_downsampled_stencils.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int, vp, i64, vp,
+                                            C.POINTER(C.c_int64)]
+    L.aprh_gaussian_stencil.argtypes = [vp, vp, C.c_int, vp]
+    L.aprh_mean_stencil.argtypes = [vp, vp]
+    for f in (L.aprh_convolve_pencil, L.aprh_convolve):
+        f.argtypes = [vp, vp, vp, vp, C.c_int32, vp, C.c_int, vp, C.c_int]
+    L.aprh_convolve_stencil.argtypes = [vp, vp, vp, C.c_int32, C.c_int32, C.c_int32, vp, C.c_int, vp, C.c_int, C.c_int,
+                                        C.c_int]
+    L.aprh_richardson_lucy.argtypes = [vp, vp, vp, C.c_int, vp, vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int,
+                                       C.c_int, C.c_int, C.c_double, C.POINTER(AprhRLStats)]
+    L.aprh_reconstruct_constant.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp, C.c_int]
     L.aprh_malloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
     L.aprh_free.argtypes = [vp, vp]
     L.aprh_memcpy.argtypes = [vp, vp, vp, C.c_size_t]
@@ -255,6 +283,38 @@ class APR:
         _check(lib().aprh_apr_particle_cell_tree(self.h, level, _ptr(out), dims))
         return out
 
+    @classmethod
+    def from_structure(cls, dims, xz_end_vec, y_vec, ctx=None):
+        """APR from a stored LinearAccess (xz_end_vec u64, y_vec u16) of an image with dims (y, x, z),
+        as APRFile::read_apr restores it; host arrays or device addresses (then pass sizes via
+        numpy-like objects).  The structure is validated."""
+        ctx = ctx or default_context()
+        y, x, z = dims
+        xze = np.ascontiguousarray(xz_end_vec, np.uint64)
+        yv = np.ascontiguousarray(y_vec, np.uint16)
+        h = vp()
+        _check(lib().aprh_apr_from_structure(ctx.h, y, x, z, _ptr(xze), xze.size, _ptr(yv) if yv.size else None, yv.size,
+                                             C.byref(h)))
+        return cls(ctx, h)
+
+    def tree_info(self):
+        info = AprhTreeInfo()
+        _check(lib().aprh_apr_tree_info(self.ctx.h, self.h, C.byref(info)))
+        return info
+
+    def total_number_tree_particles(self):
+        return self.tree_info().total_number_tree_particles
+
+    def tree(self):
+        """tree (interior cell) access: dict with l_min, l_max, total, level_xz_vec, xz_end_vec, y_vec"""
+        info = self.tree_info()
+        lxz = np.zeros(info.level_xz_size, np.uint64)
+        xze = np.zeros(info.xz_end_size, np.uint64)
+        yv = np.zeros(max(info.total_number_tree_particles, 1), np.uint16)
+        _check(lib().aprh_apr_copy_tree(self.ctx.h, self.h, _ptr(lxz), _ptr(xze), _ptr(yv)))
+        return {"l_min": info.l_min, "l_max": info.l_max, "total": info.total_number_tree_particles,
+                "level_xz_vec": lxz, "xz_end_vec": xze, "y_vec": yv[:info.total_number_tree_particles]}
+
     def close(self):
         if getattr(self, "h", None):
             lib().aprh_apr_destroy(self.h)
@@ -372,3 +432,167 @@ def synth_blobs(shape_zxy, seed, n_blobs, sigma=(2.0, 6.0), amp=(500.0, 3000.0),
     _check(lib().aprh_synth_blobs_device(ctx.h, _ptr(out), y, x, z, seed, n_blobs, sigma[0], sigma[1], amp[0], amp[1],
                                          background, int(noise)))
     return out
+
+
+# ---------------- particle operations ----------------
+
+def _dtype_of(parts, dtype):
+    """aprh dtype code of a particle buffer (numpy array, torch tensor or device address + dtype)"""
+    if dtype is None:
+        dt = getattr(parts, "dtype", None)
+        if dt is None:
+            raise TypeError("pass dtype= for a raw device address")
+        dtype = str(dt).replace("torch.", "")
+    dtype = str(np.dtype(dtype)) if not isinstance(dtype, str) else dtype
+    if dtype in ("uint16", "u16"):
+        return APRH_DTYPE_U16
+    if dtype in ("float32", "f32", "float"):
+        return APRH_DTYPE_F32
+    raise TypeError(f"particles must be uint16 or float32, not {dtype}")
+
+
+def _host_parts(parts):
+    if isinstance(parts, np.ndarray):
+        if parts.dtype not in (np.uint16, np.float32):
+            parts = parts.astype(np.float32)
+        return np.ascontiguousarray(parts)
+    return parts
+
+
+def _stencil_list(stencils):
+    if isinstance(stencils, np.ndarray):
+        stencils = [stencils]
+    flat, dims = [], []
+    for st in stencils:
+        st = np.ascontiguousarray(st, np.float32)
+        if st.ndim != 3:
+            raise ValueError("stencils are 3-D arrays indexed [z, x, y]")
+        z, x, y = st.shape
+        flat.append(st.reshape(-1))
+        dims += [y, x, z]
+    return np.concatenate(flat), np.array(dims, np.int32), len(stencils)
+
+
+class APRTreeNumerics:
+    """APRTreeNumerics (APRTreeNumerics.hpp:19, :63-202)."""
+
+    @staticmethod
+    def fill_tree_mean(apr, particle_data, tree_data=None, dtype=None):
+        parts = _host_parts(particle_data)
+        code = _dtype_of(parts, dtype)
+        if tree_data is None:
+            tree_data = np.zeros(max(apr.total_number_tree_particles(), 1), np.float32)
+        _check(lib().aprh_fill_tree_mean(apr.ctx.h, apr.h, _ptr(parts), code, _ptr(tree_data)))
+        if isinstance(tree_data, np.ndarray):
+            return tree_data[:apr.total_number_tree_particles()]
+        return tree_data
+
+
+class APRStencil:
+    """APRStencil (APRStencil.hpp:12-244).  Stencils are float32 arrays indexed [z, x, y]."""
+
+    @staticmethod
+    def get_downsampled_stencils(stencil, nlevels, normalize=False):
+        st = np.ascontiguousarray(stencil, np.float32)
+        z, x, y = st.shape
+        need = C.c_int64()
+        _check(lib().aprh_downsampled_stencils(_ptr(st), y, x, z, nlevels, int(normalize), None, 0, None, C.byref(need)))
+        out = np.zeros(need.value, np.float32)
+        dims = np.zeros(3 * max(nlevels, 1), np.int32)
+        _check(lib().aprh_downsampled_stencils(_ptr(st), y, x, z, nlevels, int(normalize), _ptr(out), out.size, _ptr(dims),
+                                               C.byref(need)))
+        res, off = [], 0
+        for i in range(max(nlevels, 1)):
+            yy, xx, zz = dims[3 * i:3 * i + 3]
+            n = int(yy) * int(xx) * int(zz)
+            res.append(out[off:off + n].reshape(zz, xx, yy).copy())
+            off += n
+        return res
+
+    @staticmethod
+    def create_gaussian_filter(sigma=(1.0, 1.0, 1.0), size=(5, 5, 5), normalize=True):
+        """sigma and size in the reference's (y, x, z) order; result indexed [z, x, y]"""
+        sg = np.array(sigma, np.float32)
+        sz = np.array(size, np.int32)
+        out = np.zeros((int(sz[2]), int(sz[1]), int(sz[0])), np.float32)
+        _check(lib().aprh_gaussian_stencil(_ptr(sg), _ptr(sz), int(normalize), _ptr(out)))
+        return out
+
+    @staticmethod
+    def create_mean_filter(size=(5, 5, 5)):
+        sz = np.array(size, np.int32)
+        out = np.zeros((int(sz[2]), int(sz[1]), int(sz[0])), np.float32)
+        _check(lib().aprh_mean_stencil(_ptr(sz), _ptr(out)))
+        return out
+
+
+class APRFilter:
+    """APRFilter::convolve / convolve_pencil (APRFilter.hpp:67-149).
+
+    `stencils` is one [z, x, y] array (then prepared with get_downsampled_stencils unless
+    use_stencil_downsample is False) or a list of arrays (stencils[0] at level l_max)."""
+
+    @staticmethod
+    def convolve_pencil(apr, stencils, particle_input, particle_output=None, reflect_boundary=True,
+                        use_stencil_downsample=True, normalize=False, dtype=None):
+        parts = _host_parts(particle_input)
+        code = _dtype_of(parts, dtype)
+        if particle_output is None:
+            particle_output = np.zeros(max(apr.total, 1), np.float32)
+        if isinstance(stencils, np.ndarray):
+            st = np.ascontiguousarray(stencils, np.float32)
+            z, x, y = st.shape
+            _check(lib().aprh_convolve_stencil(apr.ctx.h, apr.h, _ptr(st), y, x, z, _ptr(parts), code,
+                                               _ptr(particle_output), int(reflect_boundary), int(use_stencil_downsample),
+                                               int(normalize)))
+        else:
+            flat, dims, n = _stencil_list(stencils)
+            _check(lib().aprh_convolve_pencil(apr.ctx.h, apr.h, _ptr(flat), _ptr(dims), n, _ptr(parts), code,
+                                              _ptr(particle_output), int(reflect_boundary)))
+        if isinstance(particle_output, np.ndarray):
+            return particle_output[:apr.total]
+        return particle_output
+
+    convolve = convolve_pencil
+
+
+class APRNumerics:
+    """APRNumerics::richardson_lucy (APRNumerics.hpp:130-134, :549-593)."""
+
+    @staticmethod
+    def richardson_lucy(apr, particle_input, particle_output, psf, number_iterations, use_stencil_downsample=True,
+                        normalize=False, resume=False, tol=0.0, dtype=None):
+        """particle_output: float32[total] (host or device) or None; returns (output, stats).
+        tol > 0 stops after the first iteration whose relative update is below tol."""
+        parts = _host_parts(particle_input)
+        code = _dtype_of(parts, dtype)
+        if particle_output is None:
+            if resume:
+                raise ValueError("resume needs the previous estimate")
+            particle_output = np.zeros(max(apr.total, 1), np.float32)
+        st = np.ascontiguousarray(psf, np.float32)
+        z, x, y = st.shape
+        stats = AprhRLStats()
+        _check(lib().aprh_richardson_lucy(apr.ctx.h, apr.h, _ptr(parts), code, _ptr(particle_output), _ptr(st), y, x, z,
+                                          int(number_iterations), int(use_stencil_downsample), int(normalize),
+                                          int(resume), float(tol), C.byref(stats)))
+        out = particle_output[:apr.total] if isinstance(particle_output, np.ndarray) else particle_output
+        return out, {"iterations": stats.iterations, "last_rel_update": stats.last_rel_update}
+
+
+class APRReconstruction:
+    """APRReconstruction::reconstruct_constant (APRReconstruction.hpp:80-120, :297-394)."""
+
+    @staticmethod
+    def reconstruct_constant(apr, parts, level_delta=0, out_dtype=None, out=None, dtype=None):
+        parts = _host_parts(parts)
+        code = _dtype_of(parts, dtype)
+        odt = np.dtype(out_dtype) if out_dtype is not None else (np.dtype(np.uint16) if code == APRH_DTYPE_U16
+                                                                  else np.dtype(np.float32))
+        ocode = _dtype_of(None, str(odt))
+        if out is None:
+            y, x, z = apr.org_dims
+            s = 2 ** (-level_delta)
+            out = np.zeros((-(-z // s), -(-x // s), -(-y // s)), odt)
+        _check(lib().aprh_reconstruct_constant(apr.ctx.h, apr.h, _ptr(parts), code, int(level_delta), _ptr(out), ocode))
+        return out

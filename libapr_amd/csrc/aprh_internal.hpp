@@ -140,6 +140,15 @@ struct aprh_apr {
     aprh::Buf smoothed, grad_raw, lis_raw, grad, lis, levels;
     std::vector<aprh::Buf> pct;                    /* final particle-cell tree  */
     std::vector<int64_t> pct_dims;                 /* 3 per pct level           */
+    /* tree (interior cells): GenInfo::init_tree levels 0..t_l_max + LinearAccess,
+       built on first use */
+    bool tree_built = false;
+    int t_l_max = 0;
+    std::vector<int64_t> t_y, t_x, t_z;
+    std::vector<uint64_t> t_level_xz;
+    uint64_t t_n_xz = 0, t_total = 0;
+    aprh::Buf t_xz_end;                            /* u64[t_n_xz]               */
+    aprh::Buf t_y_vec;                             /* u16[t_total]              */
 };
 
 namespace aprh {
@@ -202,6 +211,30 @@ void launch_levels_padded(aprh_context* ctx, const float* A, const LisParams& lp
 void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, const aprh_params& par);
 void launch_sample_u16(aprh_context* ctx, const aprh_apr& a, const uint16_t* img, uint16_t* parts);
 
+/* exclusive scan of n u32 counts into u64 offsets; returns the total (synchronises the stream) */
+uint64_t exclusive_scan_u32(aprh_context* ctx, const uint32_t* in, uint64_t* out, int64_t n);
+
+/* ---- particle operations (k_particles.hip) ---- */
+struct HostStencil {
+    int sy = 1, sx = 1, sz = 1;
+    std::vector<float> w;   /* (z, x, y) order, y fastest */
+};
+/* APRStencil::get_downsampled_stencils (APRStencil.hpp:81-97) on the host */
+std::vector<HostStencil> downsampled_stencils(const HostStencil& in, int nlevels, bool normalize);
+/* throws APRH_ERR_INVALID_ARGUMENT unless xz_end / y_vec describe a.org's level geometry */
+void validate_structure(aprh_context* ctx, const aprh_apr& a);
+void ensure_tree(aprh_context* ctx, aprh_apr& a);
+template <typename T>
+void tree_mean(aprh_context* ctx, aprh_apr& a, const T* parts, float* tree);
+template <typename T>
+void convolve(aprh_context* ctx, aprh_apr& a, const std::vector<HostStencil>& stencils, const T* in, float* out, bool reflect);
+template <typename T>
+int richardson_lucy(aprh_context* ctx, aprh_apr& a, const T* in, float* u, const std::vector<HostStencil>& psf,
+                    const std::vector<HostStencil>& psf_flipped, int max_iter, bool resume, double tol, double* last_rel);
+/* piecewise-constant image of level `level` (l_max: APRReconstruction::reconstruct_constant);
+   dtypes 0 = u16, 1 = f32 for particles (pdt) and image (idt) */
+void reconstruct_any(aprh_context* ctx, aprh_apr& a, const void* parts, int pdt, int level, void* img, int idt);
+
 /* ---- synthetic data (k_synth.hip) ---- */
 void synth_device(aprh_context* ctx, uint16_t* out, int64_t y, int64_t x, int64_t z, uint64_t seed, int64_t n_blobs, float smin,
                   float smax, float amin, float amax, float background, int noise);
@@ -220,6 +253,48 @@ void collect_times(aprh_context* ctx, int first, int last);
 void reset_times(aprh_context* ctx, int first, int last);
 
 void set_last_error(const std::string& s);
+
+/* device copy of a caller buffer (no copy when it already is device memory) */
+struct DevIn {
+    const void* ptr = nullptr;
+    Buf own;
+    DevIn(aprh_context* ctx, const void* p, size_t bytes) {
+        if (is_device_ptr(p)) {
+            ptr = p;
+        } else {
+            own.reset(ctx->pool, bytes);
+            HIP_CHECK(hipMemcpyAsync(own.ptr(), p, bytes, hipMemcpyHostToDevice, ctx->stream));
+            ptr = own.ptr();
+        }
+    }
+};
+
+/* device output that is copied to the caller's host buffer on finish() */
+struct DevOut {
+    void* ptr = nullptr;
+    void* host = nullptr;
+    size_t bytes = 0;
+    Buf own;
+    DevOut(aprh_context* ctx, void* p, size_t b) : bytes(b) {
+        if (is_device_ptr(p)) {
+            ptr = p;
+        } else {
+            own.reset(ctx->pool, b);
+            ptr = own.ptr();
+            host = p;
+        }
+    }
+    void finish(aprh_context* ctx) {
+        if (host) HIP_CHECK(hipMemcpyAsync(host, ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    }
+};
+
+inline void check_ctx(aprh_context* ctx) {
+    if (!ctx) throw Error(APRH_ERR_INVALID_ARGUMENT, "context is NULL");
+    HIP_CHECK(hipSetDevice(ctx->device));
+}
+
 
 }  // namespace aprh
 

@@ -12,47 +12,6 @@ using namespace aprh;
 
 namespace {
 
-/* device copy of a caller buffer (no copy when it already is device memory) */
-struct DevIn {
-    const void* ptr = nullptr;
-    Buf own;
-    DevIn(aprh_context* ctx, const void* p, size_t bytes) {
-        if (is_device_ptr(p)) {
-            ptr = p;
-        } else {
-            own.reset(ctx->pool, bytes);
-            HIP_CHECK(hipMemcpyAsync(own.ptr(), p, bytes, hipMemcpyHostToDevice, ctx->stream));
-            ptr = own.ptr();
-        }
-    }
-};
-
-/* device output that is copied to the caller's host buffer on finish() */
-struct DevOut {
-    void* ptr = nullptr;
-    void* host = nullptr;
-    size_t bytes = 0;
-    Buf own;
-    DevOut(aprh_context* ctx, void* p, size_t b) : bytes(b) {
-        if (is_device_ptr(p)) {
-            ptr = p;
-        } else {
-            own.reset(ctx->pool, b);
-            ptr = own.ptr();
-            host = p;
-        }
-    }
-    void finish(aprh_context* ctx) {
-        if (host) HIP_CHECK(hipMemcpyAsync(host, ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
-        HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    }
-};
-
-void check_ctx(aprh_context* ctx) {
-    if (!ctx) throw Error(APRH_ERR_INVALID_ARGUMENT, "context is NULL");
-    HIP_CHECK(hipSetDevice(ctx->device));
-}
-
 void check_dims(int64_t y, int64_t x, int64_t z) {
     if (y <= 0 || x <= 0 || z <= 0) throw Error(APRH_ERR_BAD_DIMS, "dimensions must be positive");
     if (y > 65535 || x > 65535 || z > 65535) throw Error(APRH_ERR_UNSUPPORTED, "dimensions above 65535 (u16 y_vec / iterator limits)");

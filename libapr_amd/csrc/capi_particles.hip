@@ -1,0 +1,284 @@
+/*
+ * capi_particles.hip -- extern "C" entry points for operations on the
+ * particles of an APR: structure import, tree, fill_tree_mean, stencils,
+ * convolution, Richardson-Lucy and piecewise-constant reconstruction.
+ */
+#include <algorithm>
+#include <cmath>
+
+#include "aprh_internal.hpp"
+
+using namespace aprh;
+
+namespace {
+
+size_t dtype_size(int dt) {
+    if (dt == APRH_DTYPE_U16) return 2;
+    if (dt == APRH_DTYPE_F32) return 4;
+    throw Error(APRH_ERR_INVALID_ARGUMENT, "dtype must be APRH_DTYPE_U16 or APRH_DTYPE_F32");
+}
+
+HostStencil host_stencil(const float* w, int sy, int sx, int sz) {
+    if (!w) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL stencil");
+    if (sy < 1 || sx < 1 || sz < 1) throw Error(APRH_ERR_INVALID_ARGUMENT, "stencil sizes must be positive");
+    if (sy % 2 == 0 || sx % 2 == 0 || sz % 2 == 0) throw Error(APRH_ERR_INVALID_ARGUMENT, "stencil sizes must be odd");
+    if ((int64_t)sy * sx * sz > (1 << 20)) throw Error(APRH_ERR_INVALID_ARGUMENT, "stencil too large");
+    HostStencil s;
+    s.sy = sy;
+    s.sx = sx;
+    s.sz = sz;
+    s.w.assign(w, w + (size_t)sy * sx * sz);
+    return s;
+}
+
+std::vector<HostStencil> stencil_list(const float* stencils, const int32_t* dims, int n) {
+    if (!stencils || !dims || n < 1) throw Error(APRH_ERR_INVALID_ARGUMENT, "need at least one stencil");
+    std::vector<HostStencil> v;
+    size_t off = 0;
+    for (int i = 0; i < n; ++i) {
+        v.push_back(host_stencil(stencils + off, dims[3 * i], dims[3 * i + 1], dims[3 * i + 2]));
+        off += v.back().w.size();
+    }
+    return v;
+}
+
+int n_stencil_levels(const aprh_apr& a, int use_ds) { return use_ds ? a.l_max - a.l_min : 1; }
+
+void check_apr(aprh_context* ctx, aprh_apr* a) {
+    check_ctx(ctx);
+    if (!a) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL apr");
+    if (a->ctx != ctx) throw Error(APRH_ERR_INVALID_ARGUMENT, "apr belongs to another context");
+}
+
+}  // namespace
+
+extern "C" {
+
+int aprh_apr_from_structure(aprh_context* ctx, int64_t y, int64_t x, int64_t z, const uint64_t* xz_end_vec, uint64_t xz_end_size,
+                            const uint16_t* y_vec, uint64_t total, aprh_apr** out) {
+    APRH_TRY
+    check_ctx(ctx);
+    if (!xz_end_vec || !out || (total > 0 && !y_vec)) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (y <= 0 || x <= 0 || z <= 0) throw Error(APRH_ERR_BAD_DIMS, "dimensions must be positive");
+    if (y > 65535 || x > 65535 || z > 65535) throw Error(APRH_ERR_UNSUPPORTED, "dimensions above 65535");
+    aprh_apr* a = new aprh_apr();
+    a->ctx = ctx;
+    try {
+        geninfo_init(*a, y, x, z);
+        if (xz_end_size != a->n_xz) throw Error(APRH_ERR_INVALID_ARGUMENT, "xz_end_size does not match the dimensions");
+        a->total = total;
+        a->xz_end.reset(ctx->pool, a->n_xz * 8);
+        a->y_vec.reset(ctx->pool, std::max<uint64_t>(total, 1) * 2);
+        copy_any(ctx, a->xz_end.ptr(), xz_end_vec, a->n_xz * 8);
+        if (total) copy_any(ctx, a->y_vec.ptr(), y_vec, total * 2);
+        validate_structure(ctx, *a);
+    } catch (...) {
+        (void)hipStreamSynchronize(ctx->stream);
+        delete a;
+        throw;
+    }
+    *out = a;
+    APRH_CATCH
+}
+
+int aprh_apr_tree_info(aprh_context* ctx, aprh_apr* a, aprh_tree_info* info) {
+    APRH_TRY
+    check_apr(ctx, a);
+    if (!info) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL info");
+    ensure_tree(ctx, *a);
+    info->l_min = 0;
+    info->l_max = a->t_l_max;
+    info->total_number_tree_particles = a->t_total;
+    info->xz_end_size = a->t_n_xz;
+    info->level_xz_size = a->t_level_xz.size();
+    APRH_CATCH
+}
+
+int aprh_apr_copy_tree(aprh_context* ctx, aprh_apr* a, uint64_t* level_xz_vec, uint64_t* xz_end_vec, uint16_t* y_vec) {
+    APRH_TRY
+    check_apr(ctx, a);
+    ensure_tree(ctx, *a);
+    if (level_xz_vec) std::copy(a->t_level_xz.begin(), a->t_level_xz.end(), level_xz_vec);
+    if (xz_end_vec) copy_any(ctx, xz_end_vec, a->t_xz_end.ptr(), a->t_n_xz * 8);
+    if (y_vec && a->t_total) copy_any(ctx, y_vec, a->t_y_vec.ptr(), a->t_total * 2);
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    APRH_CATCH
+}
+
+int aprh_fill_tree_mean(aprh_context* ctx, aprh_apr* a, const void* parts, int dtype, float* tree_out) {
+    APRH_TRY
+    check_apr(ctx, a);
+    if (!parts || !tree_out) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    const size_t es = dtype_size(dtype);
+    ensure_tree(ctx, *a);
+    DevIn in(ctx, parts, std::max<uint64_t>(a->total, 1) * es);
+    DevOut out(ctx, tree_out, std::max<uint64_t>(a->t_total, 1) * 4);
+    if (dtype == APRH_DTYPE_U16) tree_mean<uint16_t>(ctx, *a, (const uint16_t*)in.ptr, (float*)out.ptr);
+    else tree_mean<float>(ctx, *a, (const float*)in.ptr, (float*)out.ptr);
+    out.finish(ctx);
+    APRH_CATCH
+}
+
+int aprh_downsampled_stencils(const float* stencil, int32_t sy, int32_t sx, int32_t sz, int32_t nlevels, int normalize, float* out,
+                              int64_t out_capacity, int32_t* dims_out, int64_t* needed) {
+    APRH_TRY
+    const HostStencil s = host_stencil(stencil, sy, sx, sz);
+    if (nlevels < 1) throw Error(APRH_ERR_INVALID_ARGUMENT, "nlevels must be >= 1");
+    const std::vector<HostStencil> v = downsampled_stencils(s, nlevels, normalize != 0);
+    int64_t tot = 0;
+    for (const auto& h : v) tot += (int64_t)h.w.size();
+    if (needed) *needed = tot;
+    if (out) {
+        if (out_capacity < tot) throw Error(APRH_ERR_INVALID_ARGUMENT, "output capacity too small");
+        int64_t off = 0;
+        for (size_t i = 0; i < v.size(); ++i) {
+            std::copy(v[i].w.begin(), v[i].w.end(), out + off);
+            off += (int64_t)v[i].w.size();
+            if (dims_out) {
+                dims_out[3 * i] = v[i].sy;
+                dims_out[3 * i + 1] = v[i].sx;
+                dims_out[3 * i + 2] = v[i].sz;
+            }
+        }
+    }
+    APRH_CATCH
+}
+
+int aprh_gaussian_stencil(const float* sigma3, const int32_t* size3, int normalize, float* out) {
+    APRH_TRY
+    if (!sigma3 || !size3 || !out) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    for (int d = 0; d < 3; ++d)
+        if (size3[d] < 1 || size3[d] > 4096) throw Error(APRH_ERR_INVALID_ARGUMENT, "stencil size out of range");
+    /* APRStencil::create_gaussian_filter (APRStencil.hpp:173-228): separable factors
+       expf(-i^2 / (2 s^2)) / (s sqrt(2 pi)), product in (y, x, z) order, float sum over
+       y-major loops */
+    std::vector<float> g[3];
+    for (int d = 0; d < 3; ++d) {
+        const int n = size3[d];
+        const float s = sigma3[d];
+        g[d].assign((size_t)n, 1.0f);
+        if (n > 1) {
+            int c = 0;
+            for (int i = -n / 2; i <= n / 2; ++i) g[d][(size_t)c++] = expf(-i * i / (2 * s * s)) / (s * sqrtf(2 * M_PI));
+        }
+    }
+    const int sy = size3[0], sx = size3[1], sz = size3[2];
+    float sum = 0;
+    for (int i = 0; i < sy; ++i)
+        for (int j = 0; j < sx; ++j)
+            for (int k = 0; k < sz; ++k) {
+                const float v = g[0][(size_t)i] * g[1][(size_t)j] * g[2][(size_t)k];
+                out[((size_t)k * sx + j) * sy + i] = v;
+                sum += v;
+            }
+    if (normalize)
+        for (size_t i = 0; i < (size_t)sy * sx * sz; ++i) out[i] /= sum;
+    APRH_CATCH
+}
+
+int aprh_mean_stencil(const int32_t* size3, float* out) {
+    APRH_TRY
+    if (!size3 || !out) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    for (int d = 0; d < 3; ++d)
+        if (size3[d] < 1 || size3[d] > 4096) throw Error(APRH_ERR_INVALID_ARGUMENT, "stencil size out of range");
+    /* APRStencil::create_mean_filter (APRStencil.hpp:236-244) */
+    const size_t n = (size_t)size3[0] * size3[1] * size3[2];
+    const float cnt = (float)n;
+    const float v = (float)(1.0 / cnt);
+    std::fill(out, out + n, v);
+    APRH_CATCH
+}
+
+int aprh_convolve_pencil(aprh_context* ctx, aprh_apr* a, const float* stencils, const int32_t* dims, int32_t n_stencils,
+                         const void* parts_in, int dtype, float* parts_out, int reflect) {
+    APRH_TRY
+    check_apr(ctx, a);
+    if (!parts_in || !parts_out) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    const size_t es = dtype_size(dtype);
+    const std::vector<HostStencil> v = stencil_list(stencils, dims, n_stencils);
+    DevIn in(ctx, parts_in, std::max<uint64_t>(a->total, 1) * es);
+    DevOut out(ctx, parts_out, std::max<uint64_t>(a->total, 1) * 4);
+    if (dtype == APRH_DTYPE_U16) convolve<uint16_t>(ctx, *a, v, (const uint16_t*)in.ptr, (float*)out.ptr, reflect != 0);
+    else convolve<float>(ctx, *a, v, (const float*)in.ptr, (float*)out.ptr, reflect != 0);
+    out.finish(ctx);
+    APRH_CATCH
+}
+
+int aprh_convolve(aprh_context* ctx, aprh_apr* a, const float* stencils, const int32_t* dims, int32_t n_stencils, const void* parts_in,
+                  int dtype, float* parts_out, int reflect) {
+    return aprh_convolve_pencil(ctx, a, stencils, dims, n_stencils, parts_in, dtype, parts_out, reflect);
+}
+
+int aprh_convolve_stencil(aprh_context* ctx, aprh_apr* a, const float* stencil, int32_t sy, int32_t sx, int32_t sz, const void* parts_in,
+                          int dtype, float* parts_out, int reflect, int use_stencil_downsample, int normalize) {
+    APRH_TRY
+    check_apr(ctx, a);
+    if (!parts_in || !parts_out) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    const size_t es = dtype_size(dtype);
+    const std::vector<HostStencil> v =
+        downsampled_stencils(host_stencil(stencil, sy, sx, sz), n_stencil_levels(*a, use_stencil_downsample), normalize != 0);
+    DevIn in(ctx, parts_in, std::max<uint64_t>(a->total, 1) * es);
+    DevOut out(ctx, parts_out, std::max<uint64_t>(a->total, 1) * 4);
+    if (dtype == APRH_DTYPE_U16) convolve<uint16_t>(ctx, *a, v, (const uint16_t*)in.ptr, (float*)out.ptr, reflect != 0);
+    else convolve<float>(ctx, *a, v, (const float*)in.ptr, (float*)out.ptr, reflect != 0);
+    out.finish(ctx);
+    APRH_CATCH
+}
+
+int aprh_richardson_lucy(aprh_context* ctx, aprh_apr* a, const void* parts_in, int dtype, float* u, const float* psf, int32_t sy,
+                         int32_t sx, int32_t sz, int32_t max_iter, int use_stencil_downsample, int normalize, int resume, double tol,
+                         aprh_rl_stats* stats) {
+    APRH_TRY
+    check_apr(ctx, a);
+    if (!parts_in || !u) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (max_iter < 0) throw Error(APRH_ERR_INVALID_ARGUMENT, "negative iteration count");
+    const size_t es = dtype_size(dtype);
+    /* APRNumerics.hpp:575-593: flipped PSF and the two stencil stacks */
+    const HostStencil h = host_stencil(psf, sy, sx, sz);
+    HostStencil hf = h;
+    std::reverse(hf.w.begin(), hf.w.end());
+    const int nst = n_stencil_levels(*a, use_stencil_downsample);
+    const std::vector<HostStencil> sv = downsampled_stencils(h, nst, normalize != 0);
+    const std::vector<HostStencil> fv = downsampled_stencils(hf, nst, normalize != 0);
+    DevIn in(ctx, parts_in, std::max<uint64_t>(a->total, 1) * es);
+    const bool u_dev = is_device_ptr(u);
+    Buf ubuf;
+    float* du = u;
+    if (!u_dev) {
+        ubuf.reset(ctx->pool, std::max<uint64_t>(a->total, 1) * 4);
+        du = ubuf.as<float>();
+        if (resume && a->total) HIP_CHECK(hipMemcpyAsync(du, u, a->total * 4, hipMemcpyHostToDevice, ctx->stream));
+    }
+    double rel = 0;
+    int it;
+    if (dtype == APRH_DTYPE_U16)
+        it = richardson_lucy<uint16_t>(ctx, *a, (const uint16_t*)in.ptr, du, sv, fv, max_iter, resume != 0, tol, &rel);
+    else
+        it = richardson_lucy<float>(ctx, *a, (const float*)in.ptr, du, sv, fv, max_iter, resume != 0, tol, &rel);
+    if (!u_dev && a->total) HIP_CHECK(hipMemcpyAsync(u, du, a->total * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (stats) {
+        stats->iterations = it;
+        stats->last_rel_update = rel;
+    }
+    APRH_CATCH
+}
+
+int aprh_reconstruct_constant(aprh_context* ctx, aprh_apr* a, const void* parts, int parts_dtype, int level_delta, void* image,
+                              int image_dtype) {
+    APRH_TRY
+    check_apr(ctx, a);
+    if (!parts || !image) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    const size_t pes = dtype_size(parts_dtype), ies = dtype_size(image_dtype);
+    if (level_delta > 0) throw Error(APRH_ERR_INVALID_ARGUMENT, "level_delta must be <= 0");
+    const int level = a->l_max + level_delta;
+    if (level < a->l_min) throw Error(APRH_ERR_INVALID_ARGUMENT, "level_delta below the APR's minimum level");
+    const size_t n = (size_t)(a->y_num[level] * a->x_num[level] * a->z_num[level]);
+    DevIn in(ctx, parts, std::max<uint64_t>(a->total, 1) * pes);
+    DevOut out(ctx, image, n * ies);
+    reconstruct_any(ctx, *a, in.ptr, parts_dtype, level, out.ptr, image_dtype);
+    out.finish(ctx);
+    APRH_CATCH
+}
+
+}  // extern "C"

@@ -1,0 +1,689 @@
+/*
+ * k_particles.hip -- operations on the particles of a device-resident APR:
+ * tree (interior cells), fill_tree_mean, piecewise-constant reconstruction,
+ * APR convolution and Richardson-Lucy.
+ *
+ *  - tree (APR::initialize_tree_dense, APR.hpp:633-689): per-level dense u8
+ *    maps; one wave per particle row marks the parents of its particles, the
+ *    maps are OR-ed upward through the (0,0,0) child, and the linear tree
+ *    access is built by row counts + scan + fill.
+ *  - fill_tree_mean (APRTreeNumerics.hpp:63-202): tree levels from the top
+ *    down; the children level is scattered into a dense value/type scratch and
+ *    each tree node accumulates scale*value/8 over its particle children first
+ *    and its tree children second, both in (z, x, y) order -- the order in which
+ *    the reference's two passes reach every node -- so the sums are bit-equal.
+ *  - level images: the piecewise-constant image at level l is built coarse to
+ *    fine (upsample level l-1, then write the particles of level l and, where
+ *    finer particles live, the tree means of level l).  It is the isotropic
+ *    neighbourhood that APRFilter::convolve_pencil (APRFilter.hpp:1094-1216)
+ *    assembles in its ring buffer, and at level l_max + level_delta it is
+ *    APRReconstruction::reconstruct_constant (APRReconstruction.hpp:80-120,
+ *    :297-394).
+ *  - convolution: every particle sums stencil x level image over its reflected
+ *    or zero-padded window in the reference's (z, x, y) tap order.
+ *  - Richardson-Lucy (APRNumerics.hpp:549-593) with an optional relative-update
+ *    stopping rule; the update norms are reduced in a fixed order.
+ */
+#include <algorithm>
+#include <cmath>
+
+#include "aprh_internal.hpp"
+
+namespace aprh {
+
+namespace {
+
+constexpr int MAXL = 24;
+
+/* rows of a linear access (APR or tree) over levels lo..hi */
+struct Rows {
+    int n_levels, l_lo;
+    int64_t prefix[MAXL + 1];     /* first global row id of each level */
+    int64_t row_begin[MAXL];      /* index in xz_end of row 0          */
+    int64_t x_num[MAXL], y_num[MAXL], z_num[MAXL];
+};
+
+Rows make_rows(const std::vector<int64_t>& yn, const std::vector<int64_t>& xn, const std::vector<int64_t>& zn,
+               const std::vector<uint64_t>& level_xz, int lo, int hi) {
+    Rows r;
+    std::memset(&r, 0, sizeof(r));
+    r.n_levels = std::max(hi - lo + 1, 0);
+    r.l_lo = lo;
+    if (r.n_levels > MAXL) throw Error(APRH_ERR_UNSUPPORTED, "too many levels");
+    int64_t p = 0;
+    for (int l = lo; l <= hi; ++l) {
+        const int i = l - lo;
+        r.prefix[i] = p;
+        r.row_begin[i] = (int64_t)level_xz[l];
+        r.x_num[i] = xn[l];
+        r.y_num[i] = yn[l];
+        r.z_num[i] = zn[l];
+        p += xn[l] * zn[l];
+    }
+    r.prefix[r.n_levels] = p;
+    return r;
+}
+
+__device__ __forceinline__ int row_level(const Rows& R, int64_t r) {
+    int li = 0;
+    while (li + 1 < R.n_levels && r >= R.prefix[li + 1]) ++li;
+    return li;
+}
+
+__device__ __forceinline__ uint16_t f2u16(float v) {
+    /* g++/x86-64 float->uint16 conversion: truncation through int32 */
+    if (!(v > -2147483648.0f && v < 2147483648.0f)) return 0;
+    return (uint16_t)(int32_t)v;
+}
+
+template <typename D> struct Cvt;
+template <> struct Cvt<float> {
+    static __device__ __forceinline__ float from(float v) { return v; }
+    static __device__ __forceinline__ float from(uint16_t v) { return (float)v; }
+};
+template <> struct Cvt<uint16_t> {
+    static __device__ __forceinline__ uint16_t from(float v) { return f2u16(v); }
+    static __device__ __forceinline__ uint16_t from(uint16_t v) { return v; }
+};
+
+/* ---------------- tree build ---------------- */
+/* APR particle (level l) marks tree cell (l-1, y/2, x/2, z/2) */
+__global__ __launch_bounds__(256) void k_mark_parents(Rows R, const uint64_t* __restrict__ xz_end, const uint16_t* __restrict__ y_vec,
+                                                      uint8_t* const* maps, int64_t total_rows) {
+    const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / 64;
+    for (int64_t r = wid; r < total_rows; r += nw) {
+        const int li = row_level(R, r);
+        const int level = R.l_lo + li;
+        const int64_t row = r - R.prefix[li];
+        const int64_t z = row / R.x_num[li], x = row % R.x_num[li];
+        const uint64_t b = xz_end[R.row_begin[li] + row - 1], e = xz_end[R.row_begin[li] + row];
+        uint8_t* m = maps[level - 1];
+        const int64_t pyn = (R.y_num[li] + 1) / 2, pxn = (R.x_num[li] + 1) / 2;
+        uint8_t* prow = m + ((z / 2) * pxn + (x / 2)) * pyn;
+        for (uint64_t i = b + lane; i < e; i += 64) prow[y_vec[i] / 2] = 1;
+    }
+}
+
+/* tree[l](y,x,z) |= tree[l+1](2y,2x,2z) */
+__global__ void k_or_up(uint8_t* m, int64_t yn, int64_t xn, int64_t zn, const uint8_t* __restrict__ c, int64_t cyn, int64_t cxn) {
+    const size_t n = (size_t)(yn * xn * zn);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int64_t y = (int64_t)(i % (size_t)yn);
+        const int64_t r = (int64_t)(i / (size_t)yn);
+        const int64_t x = r % xn, z = r / xn;
+        m[i] = m[i] | c[((2 * z) * cxn + 2 * x) * cyn + 2 * y];
+    }
+}
+
+/* per-row count of a 0/1 map row (LinearAccess::initialize_tree_access_dense, LinearAccess.hpp:75-90):
+   one wave per row */
+__global__ __launch_bounds__(256) void k_map_row_count(Rows R, const uint8_t* const* maps, uint32_t* cnt, int64_t total_rows) {
+    const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / 64;
+    for (int64_t r = wid; r < total_rows; r += nw) {
+        const int li = row_level(R, r);
+        const int64_t row = r - R.prefix[li];
+        const int64_t yn = R.y_num[li];
+        const uint8_t* p = maps[R.l_lo + li] + row * yn;
+        uint32_t c = 0;
+        for (int64_t y = lane; y < yn; y += 64) c += p[y];
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+        if (lane == 0) cnt[r] = c;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_map_row_fill(Rows R, const uint8_t* const* maps, const uint64_t* __restrict__ xz_end,
+                                                      uint16_t* y_vec, int64_t total_rows) {
+    const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / 64;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int64_t r = wid; r < total_rows; r += nw) {
+        const int li = row_level(R, r);
+        const int64_t row = r - R.prefix[li];
+        const int64_t yn = R.y_num[li];
+        const uint8_t* p = maps[R.l_lo + li] + row * yn;
+        uint64_t o = xz_end[R.row_begin[li] + row - 1];
+        for (int64_t y0 = 0; y0 < yn; y0 += 64) {
+            const int64_t y = y0 + lane;
+            const bool on = y < yn && p[y];
+            const uint64_t b = __ballot(on);
+            if (on) y_vec[o + __popcll(b & lt)] = (uint16_t)y;
+            o += __popcll(b);
+        }
+    }
+}
+
+/* xz_end[row_begin + row] = inclusive scan value of global row r */
+__global__ void k_rows_to_xz(Rows R, const uint64_t* __restrict__ excl, const uint32_t* __restrict__ cnt, uint64_t* xz_end,
+                             int64_t total_rows) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= total_rows) return;
+    const int li = row_level(R, r);
+    xz_end[R.row_begin[li] + (r - R.prefix[li])] = excl[r] + cnt[r];
+}
+
+/* ---------------- dense level images ---------------- */
+/* values (and optionally a type tag) of every particle / tree node of one level into a dense image */
+template <typename T, typename D>
+__global__ __launch_bounds__(256) void k_scatter_rows(Rows R, int li, const uint64_t* __restrict__ xz_end,
+                                                      const uint16_t* __restrict__ y_vec, const T* __restrict__ vals, D* V,
+                                                      uint8_t* Ty, uint8_t type) {
+    const int64_t nrows = R.x_num[li] * R.z_num[li];
+    const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / 64;
+    for (int64_t row = wid; row < nrows; row += nw) {
+        const uint64_t b = xz_end[R.row_begin[li] + row - 1], e = xz_end[R.row_begin[li] + row];
+        D* vr = V + row * R.y_num[li];
+        uint8_t* tr = Ty ? Ty + row * R.y_num[li] : nullptr;
+        for (uint64_t i = b + lane; i < e; i += 64) {
+            const uint16_t y = y_vec[i];
+            vr[y] = Cvt<D>::from(vals[i]);
+            if (tr) tr[y] = type;
+        }
+    }
+}
+
+/* one tree level of fill_tree_mean: children (level t+1) are in the dense V/Ty scratch */
+__global__ __launch_bounds__(256) void k_tree_mean_level(Rows TR, int ti, const uint64_t* __restrict__ txz, const uint16_t* __restrict__ ty,
+                                                         const float* __restrict__ V, const uint8_t* __restrict__ Ty, int64_t cyn,
+                                                         int64_t cxn, int64_t czn, float* tree) {
+    const int64_t pxn = TR.x_num[ti], pzn = TR.z_num[ti], pyn = TR.y_num[ti];
+    const int64_t nrows = pxn * pzn;
+    const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / 64;
+    for (int64_t row = wid; row < nrows; row += nw) {
+        const int64_t zp = row / pxn, xp = row % pxn;
+        const uint64_t b = txz[TR.row_begin[ti] + row - 1], e = txz[TR.row_begin[ti] + row];
+        /* boundary scale factors (APRTreeNumerics.hpp:99-113) */
+        float sxz = (float)((((2 * pxn != cxn) && (xp == pxn - 1)) + ((2 * pzn != czn) && (zp == pzn - 1))) * 2);
+        if (sxz == 0) sxz = 1;
+        const float syxz = (2 * pyn != cyn) ? sxz * 2 : sxz;
+        const int nz = (int)min((int64_t)2, czn - 2 * zp), nx = (int)min((int64_t)2, cxn - 2 * xp);
+        for (uint64_t i = b + lane; i < e; i += 64) {
+            const int64_t yp = ty[i];
+            const float s = (yp == pyn - 1) ? syxz : sxz;
+            const int ny = (int)min((int64_t)2, cyn - 2 * yp);
+            float acc = 0.0f;
+            for (int pass = 1; pass <= 2; ++pass) {
+                for (int a = 0; a < nz; ++a)
+                    for (int bb = 0; bb < nx; ++bb)
+                        for (int c = 0; c < ny; ++c) {
+                            const int64_t ci = ((2 * zp + a) * cxn + (2 * xp + bb)) * cyn + (2 * yp + c);
+                            if (Ty[ci] == pass) acc = s * V[ci] / 8.0f + acc;
+                        }
+            }
+            tree[i] = acc;
+        }
+    }
+}
+
+/* L(z, x, y) = C(z/2, x/2, y/2) */
+template <typename D>
+__global__ void k_upsample(const D* __restrict__ C, int64_t cyn, int64_t cxn, D* L, int64_t yn, int64_t xn, int64_t zn) {
+    const size_t n = (size_t)(yn * xn * zn);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int64_t y = (int64_t)(i % (size_t)yn);
+        const int64_t r = (int64_t)(i / (size_t)yn);
+        const int64_t x = r % xn, z = r / xn;
+        L[i] = C[((z / 2) * cxn + (x / 2)) * cyn + (y / 2)];
+    }
+}
+
+struct StencilDev {
+    int sy, sx, sz;
+    int64_t off;   /* float offset in the stencil buffer */
+};
+
+__device__ __forceinline__ int64_t refl_idx(int64_t q, int64_t n) {
+    /* reflection about the edge sample (no repeat) */
+    if (q < 0) q = -q;
+    if (q >= n) q = 2 * (n - 1) - q;
+    return q < 0 ? 0 : (q >= n ? n - 1 : q);
+}
+
+/* stencil x level image at every particle of level li (one wave per particle row) */
+__global__ __launch_bounds__(256) void k_conv_level(Rows R, int li, const uint64_t* __restrict__ xz_end, const uint16_t* __restrict__ y_vec,
+                                                    const float* __restrict__ D, const float* __restrict__ W, StencilDev sd, int reflect,
+                                                    float* out) {
+    const int64_t xn = R.x_num[li], zn = R.z_num[li], yn = R.y_num[li];
+    const int64_t nrows = xn * zn;
+    const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / 64;
+    const int hy = (sd.sy - 1) / 2, hx = (sd.sx - 1) / 2, hz = (sd.sz - 1) / 2;
+    const float* w = W + sd.off;
+    for (int64_t row = wid; row < nrows; row += nw) {
+        const int64_t z = row / xn, x = row % xn;
+        const uint64_t b = xz_end[R.row_begin[li] + row - 1], e = xz_end[R.row_begin[li] + row];
+        for (uint64_t i = b + lane; i < e; i += 64) {
+            const int64_t y = y_vec[i];
+            float val = 0.0f;
+            int c = 0;
+            for (int iz = 0; iz < sd.sz; ++iz) {
+                int64_t zq = z + iz - hz;
+                const bool zin = zq >= 0 && zq < zn;
+                if (reflect) zq = refl_idx(zq, zn);
+                for (int ix = 0; ix < sd.sx; ++ix) {
+                    int64_t xq = x + ix - hx;
+                    const bool xin = xq >= 0 && xq < xn;
+                    if (reflect) xq = refl_idx(xq, xn);
+                    if (reflect) {
+                        const float* drow = D + (zq * xn + xq) * yn;
+                        for (int iy = 0; iy < sd.sy; ++iy) val += drow[refl_idx(y + iy - hy, yn)] * w[c++];
+                    } else if (zin && xin) {
+                        const float* drow = D + (zq * xn + xq) * yn;
+                        for (int iy = 0; iy < sd.sy; ++iy) {
+                            const int64_t yq = y + iy - hy;
+                            const float dv = (yq >= 0 && yq < yn) ? drow[yq] : 0.0f;
+                            val += dv * w[c++];
+                        }
+                    } else {
+                        for (int iy = 0; iy < sd.sy; ++iy) val += 0.0f * w[c++];
+                    }
+                }
+            }
+            out[i] = val;
+        }
+    }
+}
+
+/* relative_blur = input / blur (APRNumerics.hpp:553, 569) */
+template <typename T>
+__global__ void k_divide(const T* __restrict__ in, float* blur, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        blur[i] = (float)in[i] / blur[i];
+}
+
+/* u *= est, with per-block partial sums of (u_new - u)^2 and u^2 */
+__global__ __launch_bounds__(256) void k_multiply_norms(float* u, const float* __restrict__ est, size_t n, double* part) {
+    __shared__ double s1[256], s2[256];
+    double a = 0, b = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float uo = u[i];
+        const float nu = uo * est[i];
+        const double d = (double)nu - (double)uo;
+        a += d * d;
+        b += (double)uo * (double)uo;
+        u[i] = nu;
+    }
+    s1[threadIdx.x] = a;
+    s2[threadIdx.x] = b;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            s1[threadIdx.x] += s1[threadIdx.x + o];
+            s2[threadIdx.x] += s2[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s1[0];
+        part[2 * blockIdx.x + 1] = s2[0];
+    }
+}
+
+__global__ void k_sum_parts(const double* __restrict__ part, int nb, double* out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double a = 0, b = 0;
+    for (int i = 0; i < nb; ++i) {
+        a += part[2 * i];
+        b += part[2 * i + 1];
+    }
+    out[0] = a;
+    out[1] = b;
+}
+
+__global__ void k_fill_f32(float* p, size_t n, float v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+/* a stored structure is well formed: rows non-decreasing within [0, total], y strictly
+   increasing inside a row and below the level's y_num */
+__global__ void k_validate_rows(Rows R, const uint64_t* __restrict__ xz_end, const uint16_t* __restrict__ y_vec, int64_t total_rows,
+                                uint64_t total, int* err) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= total_rows) return;
+    const int li = row_level(R, r);
+    const int64_t row = r - R.prefix[li];
+    const uint64_t b = xz_end[R.row_begin[li] + row - 1], e = xz_end[R.row_begin[li] + row];
+    if (e < b || e > total) {
+        atomicAdd(err, 1);
+        return;
+    }
+    int bad = 0;
+    for (uint64_t i = b; i < e; ++i) {
+        if ((int64_t)y_vec[i] >= R.y_num[li]) bad = 1;
+        if (i > b && y_vec[i] <= y_vec[i - 1]) bad = 1;
+    }
+    if (bad) atomicAdd(err, 1);
+}
+
+const dim3 kGrid(4096), kBlock(256);
+
+size_t level_elems(const aprh_apr& a, int l) { return (size_t)(a.y_num[l] * a.x_num[l] * a.z_num[l]); }
+
+/* build the level image of level `lv` into `cur` from the level image of lv-1 in `prev` (or nothing) */
+template <typename T, typename D>
+void build_level(aprh_context* ctx, const aprh_apr& a, const Rows& AR, const Rows& TR, int lv, const D* prev, D* cur, const T* parts,
+                 const float* tree, bool with_tree) {
+    hipStream_t st = ctx->stream;
+    if (prev) {
+        hipLaunchKernelGGL(k_upsample<D>, kGrid, kBlock, 0, st, prev, a.y_num[lv - 1], a.x_num[lv - 1], cur, a.y_num[lv], a.x_num[lv],
+                           a.z_num[lv]);
+        HIP_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL((k_scatter_rows<T, D>), kGrid, kBlock, 0, st, AR, lv - a.l_min, a.xz_end.as<uint64_t>(), a.y_vec.as<uint16_t>(), parts,
+                       cur, (uint8_t*)nullptr, (uint8_t)0);
+    HIP_LAUNCH_CHECK();
+    if (with_tree && lv <= a.t_l_max && a.t_total > 0) {
+        hipLaunchKernelGGL((k_scatter_rows<float, D>), kGrid, kBlock, 0, st, TR, lv, a.t_xz_end.as<uint64_t>(), a.t_y_vec.as<uint16_t>(),
+                           tree, cur, (uint8_t*)nullptr, (uint8_t)0);
+        HIP_LAUNCH_CHECK();
+    }
+}
+
+}  // namespace
+
+/* ======================================================================== */
+
+void validate_structure(aprh_context* ctx, const aprh_apr& a) {
+    hipStream_t st = ctx->stream;
+    Rows R = make_rows(a.y_num, a.x_num, a.z_num, a.level_xz_vec, a.l_min, a.l_max);
+    const int64_t nrows = R.prefix[R.n_levels];
+    Buf err(ctx->pool, 4);
+    HIP_CHECK(hipMemsetAsync(err.ptr(), 0, 4, st));
+    if (nrows > 0) {
+        hipLaunchKernelGGL(k_validate_rows, dim3((unsigned)cdiv(nrows, 256)), kBlock, 0, st, R, a.xz_end.as<uint64_t>(),
+                           a.y_vec.as<uint16_t>(), nrows, a.total, err.as<int>());
+        HIP_LAUNCH_CHECK();
+    }
+    int h_err = 0;
+    uint64_t first = 0, last = 0;
+    HIP_CHECK(hipMemcpyAsync(&h_err, err.ptr(), 4, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipMemcpyAsync(&first, a.xz_end.ptr(), 8, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipMemcpyAsync(&last, a.xz_end.as<uint64_t>() + a.n_xz - 1, 8, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    if (h_err || first != 0 || last != a.total)
+        throw Error(APRH_ERR_INVALID_ARGUMENT, "malformed APR structure (xz_end_vec / y_vec do not describe the given dimensions)");
+}
+
+void ensure_tree(aprh_context* ctx, aprh_apr& a) {
+    if (a.tree_built) return;
+    hipStream_t st = ctx->stream;
+    /* tree geometry (GenInfo::init_tree, GenInfo.hpp:68-99) */
+    const int64_t y = a.org[0], x = a.org[1], z = a.org[2];
+    const int64_t md = std::max(std::max(y, x), z);
+    const int lm = (int)std::ceil(std::log2((double)md));
+    a.t_l_max = std::max(lm - 1, 0);
+    if (a.t_l_max + 1 > a.l_max) throw Error(APRH_ERR_UNSUPPORTED, "tree of a degenerate APR");
+    a.t_y.assign(a.t_l_max + 1, 0);
+    a.t_x.assign(a.t_l_max + 1, 0);
+    a.t_z.assign(a.t_l_max + 1, 0);
+    for (int l = 0; l <= a.t_l_max; ++l) {
+        const double cs = std::round(std::pow(2.0, (double)(a.t_l_max - l + 1)));
+        a.t_y[l] = (int64_t)std::ceil((double)y / cs);
+        a.t_x[l] = (int64_t)std::ceil((double)x / cs);
+        a.t_z[l] = (int64_t)std::ceil((double)z / cs);
+    }
+    a.t_level_xz.assign(a.t_l_max + 2, 0);
+    uint64_t ct = 1;
+    a.t_level_xz[0] = 1;
+    for (int i = 0; i <= a.t_l_max; ++i) {
+        ct += (uint64_t)(a.t_x[i] * a.t_z[i]);
+        a.t_level_xz[i + 1] = ct;
+    }
+    a.t_n_xz = ct;
+    /* dense maps, one byte per tree cell */
+    std::vector<int64_t> off(a.t_l_max + 2, 0);
+    for (int l = 0; l <= a.t_l_max; ++l) off[l + 1] = off[l] + a.t_y[l] * a.t_x[l] * a.t_z[l];
+    Buf maps(ctx->pool, (size_t)std::max<int64_t>(off[a.t_l_max + 1], 1));
+    HIP_CHECK(hipMemsetAsync(maps.ptr(), 0, (size_t)off[a.t_l_max + 1], st));
+    std::vector<uint8_t*> mp(MAXL, nullptr);
+    for (int l = 0; l <= a.t_l_max; ++l) mp[l] = maps.as<uint8_t>() + off[l];
+    Buf d_mp(ctx->pool, MAXL * sizeof(uint8_t*));
+    HIP_CHECK(hipMemcpyAsync(d_mp.ptr(), mp.data(), MAXL * sizeof(uint8_t*), hipMemcpyHostToDevice, st));
+    if (a.total > 0) {
+        Rows R = make_rows(a.y_num, a.x_num, a.z_num, a.level_xz_vec, a.l_min, a.l_max);
+        hipLaunchKernelGGL(k_mark_parents, dim3(8192), kBlock, 0, st, R, a.xz_end.as<uint64_t>(), a.y_vec.as<uint16_t>(),
+                           (uint8_t* const*)d_mp.ptr(), R.prefix[R.n_levels]);
+        HIP_LAUNCH_CHECK();
+    }
+    for (int l = a.t_l_max - 1; l >= 0; --l) {
+        hipLaunchKernelGGL(k_or_up, kGrid, kBlock, 0, st, mp[l], a.t_y[l], a.t_x[l], a.t_z[l], mp[l + 1], a.t_y[l + 1], a.t_x[l + 1]);
+        HIP_LAUNCH_CHECK();
+    }
+    /* linear access of the tree */
+    Rows TR = make_rows(a.t_y, a.t_x, a.t_z, a.t_level_xz, 0, a.t_l_max);
+    const int64_t nrows = TR.prefix[TR.n_levels];
+    Buf cnt(ctx->pool, (size_t)std::max<int64_t>(nrows, 1) * 4);
+    Buf excl(ctx->pool, (size_t)std::max<int64_t>(nrows, 1) * 8);
+    a.t_xz_end.reset(ctx->pool, a.t_n_xz * 8);
+    HIP_CHECK(hipMemsetAsync(a.t_xz_end.ptr(), 0, a.t_n_xz * 8, st));
+    uint64_t total = 0;
+    if (nrows > 0) {
+        hipLaunchKernelGGL(k_map_row_count, dim3((unsigned)std::min<int64_t>(cdiv(nrows, 4), 16384)), kBlock, 0, st, TR,
+                           (const uint8_t* const*)d_mp.ptr(), cnt.as<uint32_t>(), nrows);
+        HIP_LAUNCH_CHECK();
+        total = exclusive_scan_u32(ctx, cnt.as<uint32_t>(), excl.as<uint64_t>(), nrows);
+        hipLaunchKernelGGL(k_rows_to_xz, dim3((unsigned)cdiv(nrows, 256)), kBlock, 0, st, TR, excl.as<uint64_t>(), cnt.as<uint32_t>(),
+                           a.t_xz_end.as<uint64_t>(), nrows);
+        HIP_LAUNCH_CHECK();
+    }
+    a.t_total = total;
+    a.t_y_vec.reset(ctx->pool, std::max<uint64_t>(total, 1) * 2);
+    if (nrows > 0 && total > 0) {
+        hipLaunchKernelGGL(k_map_row_fill, dim3((unsigned)std::min<int64_t>(cdiv(nrows, 4), 16384)), kBlock, 0, st, TR,
+                           (const uint8_t* const*)d_mp.ptr(), a.t_xz_end.as<uint64_t>(), a.t_y_vec.as<uint16_t>(), nrows);
+        HIP_LAUNCH_CHECK();
+    }
+    HIP_CHECK(hipStreamSynchronize(st));
+    a.tree_built = true;
+}
+
+template <typename T>
+void tree_mean(aprh_context* ctx, aprh_apr& a, const T* parts, float* tree) {
+    ensure_tree(ctx, a);
+    hipStream_t st = ctx->stream;
+    if (a.t_total == 0) return;
+    Rows AR = make_rows(a.y_num, a.x_num, a.z_num, a.level_xz_vec, a.l_min, a.l_max);
+    Rows TR = make_rows(a.t_y, a.t_x, a.t_z, a.t_level_xz, 0, a.t_l_max);
+    /* largest children level: APR level t_l_max + 1 */
+    const size_t maxn = level_elems(a, a.t_l_max + 1);
+    Buf V(ctx->pool, maxn * 4), Ty(ctx->pool, maxn);
+    for (int t = a.t_l_max; t >= 0; --t) {
+        const int c = t + 1;   /* children level; APR and tree levels share the level geometry */
+        const int64_t cyn = a.y_num[c], cxn = a.x_num[c], czn = a.z_num[c];
+        HIP_CHECK(hipMemsetAsync(Ty.ptr(), 0, level_elems(a, c), st));
+        if (c >= a.l_min) {
+            hipLaunchKernelGGL((k_scatter_rows<T, float>), kGrid, kBlock, 0, st, AR, c - a.l_min, a.xz_end.as<uint64_t>(),
+                               a.y_vec.as<uint16_t>(), parts, V.as<float>(), Ty.as<uint8_t>(), (uint8_t)1);
+            HIP_LAUNCH_CHECK();
+        }
+        if (c <= a.t_l_max) {
+            hipLaunchKernelGGL((k_scatter_rows<float, float>), kGrid, kBlock, 0, st, TR, c, a.t_xz_end.as<uint64_t>(),
+                               a.t_y_vec.as<uint16_t>(), (const float*)tree, V.as<float>(), Ty.as<uint8_t>(), (uint8_t)2);
+            HIP_LAUNCH_CHECK();
+        }
+        hipLaunchKernelGGL(k_tree_mean_level, kGrid, kBlock, 0, st, TR, t, a.t_xz_end.as<uint64_t>(), a.t_y_vec.as<uint16_t>(),
+                           V.as<float>(), Ty.as<uint8_t>(), cyn, cxn, czn, tree);
+        HIP_LAUNCH_CHECK();
+    }
+}
+
+std::vector<HostStencil> downsampled_stencils(const HostStencil& in, int nlevels, bool normalize) {
+    /* APRStencil::downsample_stencil (APRStencil.hpp:12-78): per output tap the area-overlap
+       weighted sum over input taps, visited in (z, x, y) input order */
+    std::vector<HostStencil> out((size_t)std::max(nlevels, 1));
+    out[0] = in;
+    if (normalize) {
+        /* PixelData::normalize (PixelData.hpp:771-781) */
+        float sum = 0;
+        for (float v : out[0].w) sum += v;
+        for (auto& v : out[0].w) v /= sum;
+    }
+    const int nd = (in.sz > 1) + (in.sx > 1) + (in.sy > 1);
+    for (int d = 1; d < nlevels; ++d) {
+        const int step = (int)std::pow(2.0f, (float)d);
+        const int factor = (int)std::pow((float)step, (float)nd);
+        auto ds = [&](int n) { return n > 1 ? std::max(2 * (((n - 1) / 2 + step - 1) / step) + 1, 3) : 1; };
+        HostStencil& o = out[(size_t)d];
+        o.sy = ds(in.sy);
+        o.sx = ds(in.sx);
+        o.sz = ds(in.sz);
+        o.w.assign((size_t)o.sy * o.sx * o.sz, 0.0f);
+        const int oz = (o.sz / 2) * step - in.sz / 2, ox = (o.sx / 2) * step - in.sx / 2, oy = (o.sy / 2) * step - in.sy / 2;
+        auto overlap = [&](int dcell, int off_i) {
+            return (float)std::max(std::min(dcell * step + step, off_i + step) - std::max(dcell * step, off_i), 0);
+        };
+        for (int dz = 0; dz < o.sz; ++dz)
+            for (int dx = 0; dx < o.sx; ++dx)
+                for (int dy = 0; dy < o.sy; ++dy) {
+                    float acc = 0.0f;
+                    for (int iz = 0; iz < in.sz; ++iz) {
+                        const float wz = overlap(dz, oz + iz);
+                        for (int ix = 0; ix < in.sx; ++ix) {
+                            const float wx = overlap(dx, ox + ix);
+                            for (int iy = 0; iy < in.sy; ++iy) {
+                                const float wy = overlap(dy, oy + iy);
+                                acc += (wx * wy * wz) * in.w[((size_t)iz * in.sx + ix) * in.sy + iy];
+                            }
+                        }
+                    }
+                    o.w[((size_t)dz * o.sx + dx) * o.sy + dy] = acc;
+                }
+        float sum = 0;
+        for (auto& v : o.w) {
+            v /= (float)factor;
+            sum += v;
+        }
+        if (normalize) {
+            const float nf = 1.0f / sum;
+            for (auto& v : o.w) v *= nf;
+        }
+    }
+    return out;
+}
+
+template <typename T>
+void convolve(aprh_context* ctx, aprh_apr& a, const std::vector<HostStencil>& stencils, const T* in, float* out, bool reflect) {
+    if (stencils.empty()) throw Error(APRH_ERR_INVALID_ARGUMENT, "no stencils");
+    if (a.total == 0) return;
+    ensure_tree(ctx, a);
+    hipStream_t st = ctx->stream;
+    Buf tree(ctx->pool, std::max<uint64_t>(a.t_total, 1) * 4);
+    /* APRFilter.hpp:1088: tree means of the input */
+    tree_mean(ctx, a, in, tree.as<float>());
+    std::vector<float> flat;
+    std::vector<StencilDev> sd;
+    for (const auto& s : stencils) {
+        sd.push_back(StencilDev{s.sy, s.sx, s.sz, (int64_t)flat.size()});
+        flat.insert(flat.end(), s.w.begin(), s.w.end());
+    }
+    Buf W(ctx->pool, flat.size() * 4);
+    HIP_CHECK(hipMemcpyAsync(W.ptr(), flat.data(), flat.size() * 4, hipMemcpyHostToDevice, st));
+    Rows AR = make_rows(a.y_num, a.x_num, a.z_num, a.level_xz_vec, a.l_min, a.l_max);
+    Rows TR = make_rows(a.t_y, a.t_x, a.t_z, a.t_level_xz, 0, a.t_l_max);
+    const size_t maxn = level_elems(a, a.l_max);
+    Buf D0(ctx->pool, maxn * 4), D1(ctx->pool, maxn * 4);
+    float* prev = nullptr;
+    float* cur = D0.as<float>();
+    float* other = D1.as<float>();
+    for (int l = a.l_min; l <= a.l_max; ++l) {
+        build_level<T, float>(ctx, a, AR, TR, l, prev, cur, in, tree.as<float>(), true);
+        const int sn = std::min((int)stencils.size() - 1, a.l_max - l);
+        hipLaunchKernelGGL(k_conv_level, dim3(8192), kBlock, 0, st, AR, l - a.l_min, a.xz_end.as<uint64_t>(), a.y_vec.as<uint16_t>(),
+                           (const float*)cur, W.as<float>(), sd[(size_t)sn], reflect ? 1 : 0, out);
+        HIP_LAUNCH_CHECK();
+        prev = cur;
+        std::swap(cur, other);
+    }
+    HIP_CHECK(hipStreamSynchronize(st));
+}
+
+template <typename T, typename U>
+void reconstruct(aprh_context* ctx, aprh_apr& a, const T* parts, int level, U* img) {
+    if (level < a.l_min || level > a.l_max) throw Error(APRH_ERR_INVALID_ARGUMENT, "reconstruction level out of range");
+    hipStream_t st = ctx->stream;
+    const bool with_tree = level < a.l_max;
+    Buf tree;
+    Rows TR;
+    std::memset(&TR, 0, sizeof(TR));
+    if (with_tree) {
+        ensure_tree(ctx, a);
+        tree.reset(ctx->pool, std::max<uint64_t>(a.t_total, 1) * 4);
+        tree_mean(ctx, a, parts, tree.as<float>());
+        TR = make_rows(a.t_y, a.t_x, a.t_z, a.t_level_xz, 0, a.t_l_max);
+    }
+    Rows AR = make_rows(a.y_num, a.x_num, a.z_num, a.level_xz_vec, a.l_min, a.l_max);
+    const size_t maxn = level > a.l_min ? level_elems(a, level - 1) : 1;
+    Buf S0(ctx->pool, maxn * sizeof(U)), S1(ctx->pool, maxn * sizeof(U));
+    U* prev = nullptr;
+    U* bufs[2] = {S0.as<U>(), S1.as<U>()};
+    for (int l = a.l_min; l <= level; ++l) {
+        U* cur = (l == level) ? img : bufs[l & 1];
+        build_level<T, U>(ctx, a, AR, TR, l, prev, cur, parts, tree.as<float>(), with_tree && l == level);
+        prev = cur;
+    }
+    HIP_CHECK(hipStreamSynchronize(st));
+}
+
+void reconstruct_any(aprh_context* ctx, aprh_apr& a, const void* parts, int pdt, int level, void* img, int idt) {
+    if (pdt == 0 && idt == 0) reconstruct<uint16_t, uint16_t>(ctx, a, (const uint16_t*)parts, level, (uint16_t*)img);
+    else if (pdt == 0 && idt == 1) reconstruct<uint16_t, float>(ctx, a, (const uint16_t*)parts, level, (float*)img);
+    else if (pdt == 1 && idt == 0) reconstruct<float, uint16_t>(ctx, a, (const float*)parts, level, (uint16_t*)img);
+    else reconstruct<float, float>(ctx, a, (const float*)parts, level, (float*)img);
+}
+
+template void tree_mean<uint16_t>(aprh_context*, aprh_apr&, const uint16_t*, float*);
+template void tree_mean<float>(aprh_context*, aprh_apr&, const float*, float*);
+template void convolve<uint16_t>(aprh_context*, aprh_apr&, const std::vector<HostStencil>&, const uint16_t*, float*, bool);
+template void convolve<float>(aprh_context*, aprh_apr&, const std::vector<HostStencil>&, const float*, float*, bool);
+
+template <typename T>
+int richardson_lucy(aprh_context* ctx, aprh_apr& a, const T* in, float* u, const std::vector<HostStencil>& psf,
+                    const std::vector<HostStencil>& psf_flipped, int max_iter, bool resume, double tol, double* last_rel) {
+    hipStream_t st = ctx->stream;
+    const size_t P = a.total;
+    if (!resume && P > 0) {
+        hipLaunchKernelGGL(k_fill_f32, kGrid, kBlock, 0, st, u, P, 1.0f);
+        HIP_LAUNCH_CHECK();
+    }
+    Buf blur(ctx->pool, std::max<size_t>(P, 1) * 4), est(ctx->pool, std::max<size_t>(P, 1) * 4);
+    const int nb = 1024;
+    Buf part(ctx->pool, nb * 2 * sizeof(double)), sums(ctx->pool, 2 * sizeof(double));
+    int it = 0;
+    double rel = 0;
+    for (it = 0; it < max_iter && P > 0; ++it) {
+        convolve<float>(ctx, a, psf_flipped, u, blur.as<float>(), true);
+        hipLaunchKernelGGL(k_divide<T>, kGrid, kBlock, 0, st, in, blur.as<float>(), P);
+        HIP_LAUNCH_CHECK();
+        convolve<float>(ctx, a, psf, blur.as<float>(), est.as<float>(), true);
+        hipLaunchKernelGGL(k_multiply_norms, dim3(nb), kBlock, 0, st, u, est.as<float>(), P, part.as<double>());
+        HIP_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, part.as<double>(), nb, sums.as<double>());
+        HIP_LAUNCH_CHECK();
+        double h[2];
+        HIP_CHECK(hipMemcpyAsync(h, sums.ptr(), 16, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        rel = h[1] > 0 ? std::sqrt(h[0] / h[1]) : 0.0;
+        if (tol > 0 && rel < tol) {
+            ++it;
+            break;
+        }
+    }
+    if (last_rel) *last_rel = rel;
+    return it;
+}
+
+template int richardson_lucy<uint16_t>(aprh_context*, aprh_apr&, const uint16_t*, float*, const std::vector<HostStencil>&,
+                                       const std::vector<HostStencil>&, int, bool, double, double*);
+template int richardson_lucy<float>(aprh_context*, aprh_apr&, const float*, float*, const std::vector<HostStencil>&,
+                                    const std::vector<HostStencil>&, int, bool, double, double*);
+
+}  // namespace aprh

@@ -329,6 +329,24 @@ __global__ __launch_bounds__(256) void k_gather(LevelPtrs L, const uint64_t* __r
 
 }  // namespace
 
+uint64_t exclusive_scan_u32(aprh_context* ctx, const uint32_t* in, uint64_t* out, int64_t n) {
+    if (n <= 0) return 0;
+    hipStream_t st = ctx->stream;
+    const int64_t nb = cdiv(n, 1024);
+    Buf bsum(ctx->pool, (size_t)(nb + 1) * 8);
+    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(1024), 0, st, in, out, bsum.as<uint64_t>(), n);
+    HIP_LAUNCH_CHECK();
+    uint64_t* d_total = (uint64_t*)ctx->d_flags + 8;
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, bsum.as<uint64_t>(), nb, d_total);
+    HIP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st, out, bsum.as<uint64_t>(), n);
+    HIP_LAUNCH_CHECK();
+    uint64_t total = 0;
+    HIP_CHECK(hipMemcpyAsync(&total, d_total, 8, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    return total;
+}
+
 void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, const aprh_params& par) {
     hipStream_t st = ctx->stream;
     const int lmax = a.l_max, lmin = a.l_min;

@@ -1451,6 +1451,63 @@ int richardson_lucy(const GenInfo& g, const Linear& L, const Tree& t, const InT*
     return it;
 }
 
+/* APRReconstruction::reconstruct_constant (APRReconstruction.hpp:80-120) and the
+   full-volume patch form with level_delta <= 0 (:297-394): the image at level
+   L = l_max + level_delta; particles of level l <= L fill blocks of 2^(L-l) (clipped
+   at the level's edge), and for L < l_max the tree means of level L overwrite the
+   cells where finer particles live.  Values convert to the image type like a C++
+   assignment on x86-64 (float -> u16 through int32).                          */
+template <typename ImgT> struct ImgCvt { static ImgT of(float v) { return (ImgT)v; } };
+template <> struct ImgCvt<uint16_t> { static uint16_t of(float v) { return f2u16(v); } };
+
+template <typename PartT, typename ImgT>
+void reconstruct_constant(const GenInfo& g, const Linear& L, const Tree* t, const PartT* parts, const float* tree, int level_delta,
+                          Mesh<ImgT>& img) {
+    const int lv = g.l_max + level_delta;
+    const int64_t yn = g.y_num[lv], xn = g.x_num[lv], zn = g.z_num[lv];
+    img.resize(yn, xn, zn, ImgT(0));
+    for (int64_t z = 0; z < zn; ++z)
+        for (int64_t x = 0; x < xn; ++x)
+            for (int level = lv; level >= g.l_min; --level) {
+                const int64_t step = (int64_t)1 << (lv - level);
+                const int64_t xl = x / step, zl = z / step;
+                for (uint64_t i = L.row_begin(level, zl, xl, g); i < L.row_end(level, zl, xl, g); ++i) {
+                    const int64_t y0 = (int64_t)L.y_vec[i] * step;
+                    const int64_t y1 = std::min(y0 + step, yn);
+                    const ImgT v = ImgCvt<ImgT>::of((float)parts[i]);
+                    for (int64_t y = y0; y < y1; ++y) img.at(y, x, z) = v;
+                }
+            }
+    if (lv < g.l_max) {
+        const GenInfo& ti = t->info;
+        for (int64_t z = 0; z < zn; ++z)
+            for (int64_t x = 0; x < xn; ++x)
+                for (uint64_t i = t->lin.row_begin(lv, z, x, ti); i < t->lin.row_end(lv, z, x, ti); ++i)
+                    img.at(t->lin.y_vec[i], x, z) = ImgCvt<ImgT>::of(tree[i]);
+    }
+}
+
+/* APRStencil::create_gaussian_filter (APRStencil.hpp:173-228) */
+static inline void gaussian_stencil(const float* sigma, const int* size, bool normalize, Mesh<float>& st) {
+    st.resize(size[0], size[1], size[2], 0.0f);
+    std::vector<float> g[3];
+    for (int d = 0; d < 3; ++d) {
+        g[d].assign((size_t)size[d], 1.0f);
+        if (size[d] > 1)
+            for (int i = -size[d] / 2, c = 0; i <= size[d] / 2; ++i, ++c)
+                g[d][(size_t)c] = expf(-i * i / (2 * sigma[d] * sigma[d])) / (sigma[d] * sqrtf(2 * M_PI));
+    }
+    float sum = 0;
+    for (int i = 0; i < size[0]; ++i)
+        for (int j = 0; j < size[1]; ++j)
+            for (int k = 0; k < size[2]; ++k) {
+                st.at(i, j, k) = g[0][(size_t)i] * g[1][(size_t)j] * g[2][(size_t)k];
+                sum += st.at(i, j, k);
+            }
+    if (normalize)
+        for (auto& v : st.v) v /= sum;
+}
+
 }  // namespace oracle
 
 #endif

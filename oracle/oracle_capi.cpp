@@ -13,6 +13,7 @@
  *   full get_apr structure, sampling sphere_120 / sphere_diff_dims fixtures
  *                                    (level/x/y/z/pc images) + _ref equality
  */
+#include <type_traits>
 #include <cstdio>
 #include <cstring>
 #include <exception>
@@ -349,6 +350,34 @@ int orc_richardson_lucy_f32(void* hp, const float* in, float* u, const float* ps
     Mesh<float> m(sy, sx, sz);
     std::copy(psf, psf + m.size(), m.v.begin());
     return richardson_lucy(h->r.info, h->r.lin, h->get_tree(), in, u, m, niter, use_ds != 0, normalize != 0, resume != 0, tol, rel);
+}
+
+/* piecewise-constant reconstruction at level l_max + level_delta; dtypes 0 u16, 1 f32 */
+void orc_reconstruct(void* hp, const void* parts, int pdt, int level_delta, void* img, int idt) {
+    Handle* h = (Handle*)hp;
+    const Tree* t = nullptr;
+    std::vector<float> tr;
+    if (level_delta < 0) {
+        t = &h->get_tree();
+        if (pdt == 0) fill_tree_mean(h->r.info, h->r.lin, *t, (const uint16_t*)parts, tr);
+        else fill_tree_mean(h->r.info, h->r.lin, *t, (const float*)parts, tr);
+    }
+    auto run = [&](auto pp, auto* out) {
+        using ImgT = std::remove_pointer_t<decltype(out)>;
+        Mesh<ImgT> m;
+        reconstruct_constant(h->r.info, h->r.lin, t, pp, tr.data(), level_delta, m);
+        std::copy(m.v.begin(), m.v.end(), out);
+    };
+    if (pdt == 0 && idt == 0) run((const uint16_t*)parts, (uint16_t*)img);
+    else if (pdt == 0) run((const uint16_t*)parts, (float*)img);
+    else if (idt == 0) run((const float*)parts, (uint16_t*)img);
+    else run((const float*)parts, (float*)img);
+}
+
+void orc_gaussian_stencil(const float* sigma, const int* size, int normalize, float* out) {
+    Mesh<float> m;
+    gaussian_stencil(sigma, size, normalize != 0, m);
+    std::copy(m.v.begin(), m.v.end(), out);
 }
 
 int orc_num_threads(void) {

@@ -487,6 +487,12 @@ int orc_richardson_lucy_f32(void* hp, const float* in, float* u, const float* ps
     return niter;
 }
 
+void orc_gaussian_stencil(const float* sigma, const int* size, int normalize, float* out) {
+    PixelData<float> m = APRStencil::create_gaussian_filter<float>({sigma[0], sigma[1], sigma[2]}, {size[0], size[1], size[2]},
+                                                                   normalize != 0);
+    std::copy(m.mesh.begin(), m.mesh.begin() + m.mesh.size(), out);
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

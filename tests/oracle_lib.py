@@ -83,6 +83,9 @@ def _bind(lib):
         getattr(lib, n).argtypes = [C.c_void_p, t, f32p, f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_double, C.POINTER(C.c_double)]
         getattr(lib, n).restype = C.c_int
+    lib.orc_gaussian_stencil.argtypes = [f32p, i32p, C.c_int, f32p]
+    if hasattr(lib, "orc_reconstruct"):
+        lib.orc_reconstruct.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int]
     return lib
 
 
@@ -181,6 +184,19 @@ class OracleAPR:
                int(resume), tol, C.byref(rel))
         return u[:self.total], it, rel.value
 
+    def reconstruct(self, parts, level_delta=0, out_dtype=None):
+        """piecewise-constant image [z, x, y] at level l_max + level_delta"""
+        parts = np.ascontiguousarray(parts)
+        pdt = 0 if parts.dtype == np.uint16 else 1
+        if pdt == 1:
+            parts = np.ascontiguousarray(parts, np.float32)
+        odt = np.dtype(out_dtype) if out_dtype is not None else parts.dtype
+        y, x, z = self.dims
+        s = 2 ** (-level_delta)
+        out = np.zeros((-(-z // s), -(-x // s), -(-y // s)), odt)
+        self.lib.orc_reconstruct(self.h, parts.ctypes.data, pdt, level_delta, out.ctypes.data, 0 if odt == np.uint16 else 1)
+        return out
+
     def __del__(self):
         try:
             self.lib.orc_apr_free(self.h)
@@ -235,6 +251,14 @@ def downsampled_stencils(stencil_zxy, nlevels, normalize, which="oracle"):
         res.append(out[off:off + n].reshape(int(zz), int(xx), int(yy)))
         off += n
     return res
+
+
+def gaussian_stencil_lib(sigma, size, normalize=True, which="oracle"):
+    """APRStencil::create_gaussian_filter through an oracle library; sigma/size in (y, x, z)"""
+    lib = load(which)
+    out = np.zeros((size[2], size[1], size[0]), np.float32)
+    lib.orc_gaussian_stencil(np.array(sigma, np.float32), np.array(size, np.int32), int(normalize), out.reshape(-1))
+    return out
 
 
 def gaussian_stencil(sigma, size, normalize=True):

@@ -1,0 +1,193 @@
+"""GPU parity of the particle operations against the CPU oracle, through the C-ABI:
+tree structure, fill_tree_mean, convolve_pencil / convolve, Richardson-Lucy and
+reconstruct_constant.
+
+The GPU sums in the reference's order (tree children: particles then tree nodes,
+(z, x, y); convolution taps: (z, x, y)), so every float result is bit-equal to the
+oracle, which is itself bit-equal to the reference build (test_oracle_numerics.py).
+Larger volumes are checked through size-independent properties: a normalised
+stencil maps a constant field to itself, tree means of a constant are constant,
+Richardson-Lucy keeps a constant image, and sampling a reconstruction gives the
+particles back.
+"""
+import numpy as np
+import pytest
+
+import libapr_amd as A
+
+pytestmark = pytest.mark.gpu
+
+CASES = [((32, 64, 64), 20, 0), ((29, 45, 61), 12, 3), ((16, 16, 130), 8, 4), ((64, 96, 128), 60, 5)]
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def _pair(O, ctx, shape, nb, seed):
+    """oracle APR and the GPU APR built from the oracle's structure"""
+    img = A.synth_blobs(shape, seed, nb)
+    o = O.get_apr(img, O.params(rel_error=0.1, sigma_th=50, Ip_th=120, grad_th=10), "oracle")
+    g = A.APR.from_structure(o.dims, o.xz_end_vec, o.y_vec, ctx)
+    print(f"case {shape} seed {seed} P={o.total}", flush=True)
+    return img, o, g
+
+
+def _stencils(O, kind, o):
+    if kind == "mean3":
+        st = np.full((3, 3, 3), 1.0 / 27, np.float32)
+    elif kind == "gauss5":
+        st = O.gaussian_stencil((1.0, 1.0, 1.0), (5, 5, 5), True)
+    else:
+        st = O.gaussian_stencil((1.0, 1.5, 2.0), (3, 5, 7), False)
+    return st, O.downsampled_stencils(st, o.l_max - o.l_min, True, "oracle")
+
+
+@pytest.mark.parametrize("shape,nb,seed", CASES)
+def test_tree_and_tree_mean(oracle, ctx, shape, nb, seed):
+    img, o, g = _pair(oracle, ctx, shape, nb, seed)
+    to, tg = o.tree(), g.tree()
+    assert tg["total"] == to["total"] and tg["l_max"] == to["l_max"]
+    for k in ("level_xz_vec", "xz_end_vec", "y_vec"):
+        assert np.array_equal(tg[k], to[k]), k
+    parts = o.sample(img)
+    assert np.array_equal(_bits(A.APRTreeNumerics.fill_tree_mean(g, parts)), _bits(o.fill_tree_mean(parts)))
+    pf = (parts.astype(np.float32) * np.float32(1.37)).astype(np.float32)
+    assert np.array_equal(_bits(A.APRTreeNumerics.fill_tree_mean(g, pf)), _bits(o.fill_tree_mean(pf)))
+
+
+@pytest.mark.parametrize("shape,nb,seed", CASES)
+@pytest.mark.parametrize("kind", ["mean3", "gauss5", "aniso"])
+@pytest.mark.parametrize("reflect", [True, False])
+def test_convolve_pencil(oracle, ctx, shape, nb, seed, kind, reflect):
+    img, o, g = _pair(oracle, ctx, shape, nb, seed)
+    parts = o.sample(img)
+    st, stack = _stencils(oracle, kind, o)
+    pf = parts.astype(np.float32)
+    exp = o.convolve_pencil(stack, pf, reflect)
+    got = A.APRFilter.convolve_pencil(g, stack, pf, reflect_boundary=reflect)
+    nbad = int((_bits(got) != _bits(exp)).sum())
+    assert nbad == 0, f"{nbad} of {got.size} differ; max abs {np.abs(got - exp).max()}"
+    # u16 input, and the single-stencil form that prepares the stack itself
+    assert np.array_equal(_bits(A.APRFilter.convolve(g, stack, parts, reflect_boundary=reflect)),
+                          _bits(o.convolve_pencil(stack, parts, reflect)))
+    assert np.array_equal(_bits(A.APRFilter.convolve_pencil(g, st, pf, reflect_boundary=reflect, normalize=True)), _bits(exp))
+
+
+def test_convolve_unit_stencil_is_identity(oracle, ctx):
+    img, o, g = _pair(oracle, ctx, *CASES[0])
+    parts = o.sample(img).astype(np.float32)
+    one = np.zeros((3, 3, 3), np.float32)
+    one[1, 1, 1] = 1
+    out = A.APRFilter.convolve_pencil(g, [one], parts)
+    assert np.array_equal(out, parts)
+
+
+@pytest.mark.parametrize("shape,nb,seed", CASES[:2])
+def test_richardson_lucy(oracle, ctx, shape, nb, seed):
+    img, o, g = _pair(oracle, ctx, shape, nb, seed)
+    parts = o.sample(img)
+    psf = oracle.gaussian_stencil((1.0, 1.0, 2.0), (9, 9, 9), True)
+    uo, it, rel_o = o.richardson_lucy(parts, psf, 3)
+    ug, st = A.APRNumerics.richardson_lucy(g, parts, None, psf, 3, normalize=True)
+    assert st["iterations"] == it == 3
+    assert np.array_equal(_bits(ug), _bits(uo))
+    assert abs(st["last_rel_update"] - rel_o) <= 1e-9 * max(rel_o, 1e-30)
+    # resume continues from the estimate (APRNumerics.hpp:559-562)
+    uo2, _, _ = o.richardson_lucy(parts, psf, 2, u=uo)
+    ug2, _ = A.APRNumerics.richardson_lucy(g, parts, ug.copy(), psf, 2, normalize=True, resume=True)
+    assert np.array_equal(_bits(ug2), _bits(uo2))
+    # float input
+    pf = parts.astype(np.float32)
+    uo3, _, _ = o.richardson_lucy(pf, psf, 2)
+    ug3, _ = A.APRNumerics.richardson_lucy(g, pf, None, psf, 2, normalize=True)
+    assert np.array_equal(_bits(ug3), _bits(uo3))
+
+
+def test_richardson_lucy_stopping_rule(oracle, ctx):
+    img, o, g = _pair(oracle, ctx, *CASES[0])
+    parts = o.sample(img)
+    psf = oracle.gaussian_stencil((1.0, 1.0, 2.0), (9, 9, 9), True)
+    uo, it, rel = o.richardson_lucy(parts, psf, 200, tol=1e-3)
+    ug, st = A.APRNumerics.richardson_lucy(g, parts, None, psf, 200, normalize=True, tol=1e-3)
+    assert st["iterations"] == it < 200 and st["last_rel_update"] < 1e-3
+    assert np.array_equal(_bits(ug), _bits(uo))
+
+
+@pytest.mark.parametrize("shape,nb,seed", CASES[:3])
+@pytest.mark.parametrize("delta", [0, -1, -2])
+def test_reconstruct_constant(oracle, ctx, shape, nb, seed, delta):
+    img, o, g = _pair(oracle, ctx, shape, nb, seed)
+    parts = o.sample(img)
+    for out_dtype in (np.uint16, np.float32):
+        assert np.array_equal(A.APRReconstruction.reconstruct_constant(g, parts, delta, out_dtype),
+                              o.reconstruct(parts, delta, out_dtype))
+    pf = (parts * np.float32(0.73)).astype(np.float32)
+    for out_dtype in (np.uint16, np.float32):
+        got = A.APRReconstruction.reconstruct_constant(g, pf, delta, out_dtype)
+        exp = o.reconstruct(pf, delta, out_dtype)
+        assert np.array_equal(got.view(np.uint8), exp.view(np.uint8))
+
+
+def test_gpu_apr_particle_ops(oracle, ctx):
+    """the same operations on an APR made by the GPU converter"""
+    img = A.synth_blobs((61, 63, 65), 1, 30)
+    conv = A.APRConverter(ctx)
+    conv.par = A.APRParameters(rel_error=0.1, sigma_th=50, Ip_th=120, grad_th=10)
+    g = conv.get_apr(img)
+    o = oracle.get_apr(img, oracle.params(rel_error=0.1, sigma_th=50, Ip_th=120, grad_th=10), "oracle")
+    parts = A.ParticleData().sample_image(g, img)
+    _, stack = _stencils(oracle, "gauss5", o)
+    assert np.array_equal(_bits(A.APRFilter.convolve_pencil(g, stack, parts)), _bits(o.convolve_pencil(stack, parts, True)))
+    assert np.array_equal(A.APRReconstruction.reconstruct_constant(g, parts), o.reconstruct(parts))
+
+
+def test_malformed_structure_rejected(oracle, ctx):
+    img = A.synth_blobs((32, 64, 64), 0, 20)
+    o = oracle.get_apr(img, oracle.params(rel_error=0.1, sigma_th=50, Ip_th=120, grad_th=10), "oracle")
+    y = o.y_vec.copy()
+    y[len(y) // 2] = 65000            # beyond y_num of any level
+    with pytest.raises(A.AprhError):
+        A.APR.from_structure(o.dims, o.xz_end_vec, y, ctx)
+    x = o.xz_end_vec.copy()
+    x[-1] += 1                        # total mismatch
+    with pytest.raises(A.AprhError):
+        A.APR.from_structure(o.dims, x, o.y_vec, ctx)
+    with pytest.raises(A.AprhError):
+        A.APR.from_structure(o.dims, o.xz_end_vec[:-1], o.y_vec, ctx)
+
+
+def test_argument_errors(oracle, ctx):
+    img, o, g = _pair(oracle, ctx, *CASES[0])
+    parts = o.sample(img)
+    with pytest.raises(A.AprhError):
+        A.APRFilter.convolve_pencil(g, [np.ones((2, 3, 3), np.float32)], parts)          # even stencil
+    with pytest.raises(A.AprhError):
+        A.APRReconstruction.reconstruct_constant(g, parts, -(o.l_max))                # below l_min
+    with pytest.raises(A.AprhError):
+        A.APRNumerics.richardson_lucy(g, parts, None, np.ones((3, 3, 3), np.float32), -1)
+
+
+def test_properties_at_size(ctx):
+    """size-independent checks on a 512x512x256 volume (GPU only)"""
+    shape = (256, 512, 512)
+    img = A.synth_blobs(shape, 7, 3000, sigma=(1.5, 6.0), amp=(300.0, 4000.0))
+    conv = A.APRConverter(ctx)
+    conv.par = A.APRParameters(rel_error=0.1, sigma_th=50, Ip_th=120, grad_th=10)
+    g = conv.get_apr(img)
+    parts = A.ParticleData().sample_image(g, img)
+    print(f"P={g.total}", flush=True)
+    # reconstruction then sampling returns the particles
+    rec = A.APRReconstruction.reconstruct_constant(g, parts)
+    assert np.array_equal(A.ParticleData().sample_image(g, rec), parts)
+    # constant field: tree means, normalised convolution and RL keep it
+    c = np.full(g.total, 7.0, np.float32)
+    tm = A.APRTreeNumerics.fill_tree_mean(g, c)
+    np.testing.assert_allclose(tm, 7.0, rtol=1e-6)
+    st = A.APRStencil.create_gaussian_filter((1.0, 1.0, 1.0), (5, 5, 5), True)
+    out = A.APRFilter.convolve_pencil(g, st, c, normalize=True)
+    np.testing.assert_allclose(out, 7.0, rtol=1e-5)
+    # u = 1 -> blur 1 -> ratio 7 -> u = 7, then ratio 1 keeps it
+    u, s = A.APRNumerics.richardson_lucy(g, c, None, st, 3, normalize=True)
+    np.testing.assert_allclose(u, 7.0, rtol=1e-4)
+    assert s["iterations"] == 3

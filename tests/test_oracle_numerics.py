@@ -210,3 +210,33 @@ def test_richardson_lucy_constant():
     np.testing.assert_allclose(u, 5.0, rtol=2e-5)
     u, it, rel = o.richardson_lucy(o.sample(img), psf, 200, tol=1e-3)
     assert it < 200 and rel < 1e-3 and (u > 0).all()
+
+
+@pytest.mark.parametrize("shape,nb,seed", CASES[:2])
+@pytest.mark.parametrize("delta", [0, -1, -2])
+def test_reconstruct_matches_numpy_restatement(shape, nb, seed, delta):
+    """oracle reconstruct_constant (APRReconstruction.hpp:80-120, :297-394) against the
+    independent numpy piecewise-constant restatement used by the convolution ground truth"""
+    img, o = make_apr(shape, nb, seed)
+    parts = o.sample(img)
+    t = o.tree()
+    tree = o.fill_tree_mean(parts)
+    level = o.l_max + delta
+    ref = recon_level(o, parts.astype(np.float64), tree.astype(np.float64), t, level)
+    assert not np.isnan(ref).any()
+    got = o.reconstruct(parts, delta, np.float32)
+    assert got.shape == ref.shape
+    assert np.array_equal(got.astype(np.float64), ref.astype(np.float32).astype(np.float64))
+    # u16 image: particles exact, tree means truncated like a float -> uint16 assignment
+    got16 = o.reconstruct(parts, delta)
+    assert np.array_equal(got16, ref.astype(np.float32).astype(np.int32).astype(np.uint16))
+
+
+def test_reconstruct_then_sample_is_identity():
+    """sample_image of the level-l_max reconstruction gives the particles back (every
+    particle's pyramid block is constant)"""
+    img, o = make_apr((29, 45, 61), 12, 3)
+    parts = o.sample(img)
+    rec = o.reconstruct(parts)
+    assert rec.shape == img.shape
+    assert np.array_equal(o.sample(rec), parts)
