@@ -1,14 +1,23 @@
 #!/usr/bin/env python3
-"""Benchmark: APR conversion (get_apr + sample_image) of a 2048x2048x512 uint16
-volume on one MI355X, through libaprh's C-ABI.
+"""Benchmark of the APR hot path on one MI355X, through libaprh's C-ABI.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-rl]
 
-One step = APRConverter<uint16_t>::get_apr on the device-resident volume
-followed by ParticleData<uint16_t>::sample_image of the same volume.  The input
+Headline (value): APR conversion of a 2048x2048x512 uint16 volume.  One step =
+APRConverter<uint16_t>::get_apr on the device-resident volume followed by
+ParticleData<uint16_t>::sample_image of the same volume.  The input
 (background 100, 200k Gaussian blobs with sigma U(1.5,8) px and amplitude
 U(300,4000), shot noise, seed 1) is synthesised on the device before timing.
 APR parameters: rel_error 0.1, lambda 3, sigma_th 50, Ip_th 120, grad_th 10.
+
+Also measured (reported under "config"):
+  * APR convolution: 3x3x3 mean stencil on the 256x256x128 volume (500 blobs,
+    sigma 2-6, amplitude 500-3000, seed 0) and the 5x5x5 Gaussian (sigma 1)
+    on the particles of the 2048x2048x512 volume;
+  * Richardson-Lucy time to tolerance: ground truth 1024x1024x256 (background
+    100 + 50k blobs, sigma 1-4, amplitude 200-2000), blurred by the 9x9x9
+    Gaussian PSF (sigma 1, 1, 2), shot noise, seed 3, converted to an APR; plain
+    RL from u = 1 until the relative update is below 1e-3 (cap 200).
 
 Prints one JSON line (stdout).  Progress goes to stderr.
 """
@@ -73,8 +82,12 @@ def run_gpu(args):
     d_parts = None
     P = None
 
+    last = {}
+
     def step():
         nonlocal d_parts, P
+        if "apr" in last:
+            last.pop("apr").close()
         apr = conv.get_apr(d_img.value, dims=(y, x, z))
         st = ctx.stage_times()
         if d_parts is None:
@@ -84,7 +97,7 @@ def run_gpu(args):
         pd.sample_image(apr, d_img.value, out=d_parts.value)
         st2 = ctx.stage_times()
         st["sample_image"] = st2["sample_image"]
-        apr.close()
+        last["apr"] = apr
         return st
 
     for i in range(args.warmup):
@@ -120,8 +133,197 @@ def run_gpu(args):
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "algorithmic_bytes": ab,
                 "avg_ms": round(cands[dom], 4)}
     total_ab = 23 * N + 6 * P
+    # 5x5x5 Gaussian (sigma 1) APR convolution on these particles (u16 in, f32 out)
+    apr = last.pop("apr")
+    A._check(L.aprh_free(ctx.h, d_img))
+    d_out = C.c_void_p()
+    A._check(L.aprh_malloc(ctx.h, max(P, 1) * 4, C.byref(d_out)))
+    g5 = A.APRStencil.create_gaussian_filter((1.0, 1.0, 1.0), (5, 5, 5), True)
+    c2 = conv_timing(A, apr, g5, d_parts.value, d_out.value, "uint16", 2, args.conv_reps)
+    log(f"[bench] 5x5x5 convolution on the 2048x2048x512 APR (reference order): {c2['ms']:.2f} ms")
+    ctx.set_conv_mode(A.CONV_SEPARABLE)
+    c2s = conv_timing(A, apr, g5, d_parts.value, d_out.value, "uint16", 2, args.conv_reps)
+    ctx.set_conv_mode(A.CONV_EXACT)
+    log(f"[bench] 5x5x5 convolution on the 2048x2048x512 APR (separable): {c2s['ms']:.2f} ms")
+    c2 = {"exact": c2, "separable": c2s}
+    apr.close()
+    A._check(L.aprh_free(ctx.h, d_out))
+    A._check(L.aprh_free(ctx.h, d_parts))
     ctx.close()
-    return ms_per_step, P, stage_avg, roofline, total_ab
+    return ms_per_step, P, stage_avg, roofline, total_ab, c2
+
+
+def conv_algorithmic_bytes(P, T, R, s_in):
+    """APR convolution work model: read the input particles, write f32 outputs, read the
+    y_vec (2 B), tree means written and read (8 B), row offsets (16 B per row)"""
+    return P * (s_in + 6) + 8 * T + 16 * R
+
+
+def conv_timing(A, apr, stencil, d_in, d_out, dtype, s_in, reps):
+    apr.tree_info()   # build the tree once (cached per APR, like APR::initialize_tree_dense)
+    A.APRFilter.convolve_pencil(apr, stencil, d_in, d_out, dtype=dtype)
+    t = time.perf_counter()
+    for _ in range(reps):
+        A.APRFilter.convolve_pencil(apr, stencil, d_in, d_out, dtype=dtype)
+    ms = (time.perf_counter() - t) * 1000.0 / reps
+    P, T = apr.total, apr.total_number_tree_particles()
+    R = int(apr.xz_end_size) + int(apr.tree_info().xz_end_size)
+    ab = conv_algorithmic_bytes(P, T, R, s_in)
+    return {"ms": round(ms, 3), "particles": P, "tree_particles": T, "algorithmic_bytes": ab,
+            "algorithmic_GBps": round(ab / (ms / 1000.0) / 1e9, 1), "stencil": list(stencil.shape[::-1])}
+
+
+def run_conv_small(args):
+    """3x3x3 mean convolution of the 256x256x128 volume's particles"""
+    import libapr_amd as A
+    ctx = A.Context(0)
+    L = A.lib()
+    shape = (128, 256, 256)
+    z, x, y = shape
+    n = x * y * z
+    d = C.c_void_p()
+    A._check(L.aprh_malloc(ctx.h, n * 2, C.byref(d)))
+    A.synth_blobs(shape, 0, 500, sigma=(2.0, 6.0), amp=(500.0, 3000.0), background=100.0, noise=True, out=d.value, ctx=ctx)
+    conv = A.APRConverter(ctx)
+    conv.par = A.APRParameters(**APR_PARAMS)
+    apr = conv.get_apr(d.value, dims=(y, x, z))
+    P = apr.total
+    dp, do = C.c_void_p(), C.c_void_p()
+    A._check(L.aprh_malloc(ctx.h, P * 2, C.byref(dp)))
+    A._check(L.aprh_malloc(ctx.h, P * 4, C.byref(do)))
+    A.ParticleData().sample_image(apr, d.value, out=dp.value)
+    st = A.APRStencil.create_mean_filter((3, 3, 3))
+    res = conv_timing(A, apr, st, dp.value, do.value, "uint16", 2, max(args.conv_reps, 5))
+    log(f"[bench] 3x3x3 convolution on the 256x256x128 APR: {res['ms']:.3f} ms (P={P})")
+    apr.close()
+    for p in (d, dp, do):
+        A._check(L.aprh_free(ctx.h, p))
+    ctx.close()
+    return res
+
+
+RL_SHAPE_ZXY = (256, 1024, 1024)
+RL_GEN = dict(n_blobs=50_000, sigma=(1.0, 4.0), amp=(200.0, 2000.0), background=100.0, psf_sigma=(1.0, 1.0, 2.0),
+              psf_taps=9, seed=3)
+RL_TOL = 1e-3
+RL_CAP = 200
+
+
+def _dev(A, ctx, nbytes):
+    p = C.c_void_p()
+    A._check(A.lib().aprh_malloc(ctx.h, max(nbytes, 1), C.byref(p)))
+    return p
+
+
+def _to_host(A, ctx, d, n, dtype):
+    h = np.empty(n, dtype)
+    A._check(A.lib().aprh_memcpy(ctx.h, C.c_void_p(h.ctypes.data), d, h.nbytes))
+    return h
+
+
+def _rel_l2(a, b):
+    a = a.astype(np.float64)
+    b = b.astype(np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def run_rl(args):
+    """plain APR Richardson-Lucy to tolerance on the 1024x1024x256 deconvolution workload"""
+    import libapr_amd as A
+    ctx = A.Context(0)
+    L = A.lib()
+    z, x, y = RL_SHAPE_ZXY
+    n = x * y * z
+    gt, img = _dev(A, ctx, n * 4), _dev(A, ctx, n * 2)
+    g = RL_GEN
+    A.synth_deconv(RL_SHAPE_ZXY, g["seed"], g["n_blobs"], gt.value, img.value, sigma=g["sigma"], amp=g["amp"],
+                   background=g["background"], psf_sigma=g["psf_sigma"], psf_taps=g["psf_taps"], ctx=ctx)
+    conv = A.APRConverter(ctx)
+    conv.par = A.APRParameters(**APR_PARAMS)
+    apr = conv.get_apr(img.value, dims=(y, x, z))
+    P = apr.total
+    parts, u = _dev(A, ctx, P * 2), _dev(A, ctx, P * 4)
+    A.ParticleData().sample_image(apr, img.value, out=parts.value)
+    psf = A.APRStencil.create_gaussian_filter(g["psf_sigma"], (g["psf_taps"],) * 3, True)
+    apr.tree_info()
+    # per-iteration time with every tap in the reference's order
+    A.APRNumerics.richardson_lucy(apr, parts.value, u.value, psf, 1, normalize=True, dtype="uint16")
+    t = time.perf_counter()
+    A.APRNumerics.richardson_lucy(apr, parts.value, u.value, psf, 2, normalize=True, dtype="uint16")
+    exact_ms = (time.perf_counter() - t) * 500.0
+    log(f"[bench] RL iteration, reference tap order: {exact_ms:.1f} ms")
+    # time to tolerance with the PSF evaluated separably (one untimed iteration first)
+    ctx.set_conv_mode(A.CONV_SEPARABLE)
+    A.APRNumerics.richardson_lucy(apr, parts.value, u.value, psf, 1, normalize=True, dtype="uint16")
+    t = time.perf_counter()
+    _, st = A.APRNumerics.richardson_lucy(apr, parts.value, u.value, psf, RL_CAP, normalize=True, tol=RL_TOL,
+                                          dtype="uint16")
+    wall = time.perf_counter() - t
+    log(f"[bench] plain RL: {st['iterations']} iterations to rel update {st['last_rel_update']:.2e} in {wall:.2f} s "
+        f"(P={P})")
+    # error against the ground truth (piecewise-constant reconstruction of the estimate), on the host in f64
+    rec = _dev(A, ctx, n * 4)
+    gt_h = _to_host(A, ctx, gt, n, np.float32)
+    A.APRReconstruction.reconstruct_constant(apr, u.value, 0, np.float32, out=rec.value, dtype="float32")
+    err = _rel_l2(_to_host(A, ctx, rec, n, np.float32), gt_h)
+    A.APRReconstruction.reconstruct_constant(apr, parts.value, 0, np.float32, out=rec.value, dtype="uint16")
+    err_in = _rel_l2(_to_host(A, ctx, rec, n, np.float32), gt_h)
+    del gt_h
+    # data residual ||psf * u - input|| / ||input|| on the particles
+    blur = _dev(A, ctx, P * 4)
+    A.APRFilter.convolve_pencil(apr, psf, u.value, blur.value, normalize=True, dtype="float32")
+    resid = _rel_l2(_to_host(A, ctx, blur, P, np.float32), _to_host(A, ctx, parts, P, np.uint16))
+    log(f"[bench] RL error vs ground truth {err:.4f} (input {err_in:.4f}), data residual {resid:.4f}")
+    res = {"workload": "1024x1024x256 GT (bg 100 + 50k blobs sigma 1-4 amp 200-2000) * 9^3 Gaussian PSF "
+                       "(sigma 1,1,2) + shot noise, seed 3; plain RL from 1 to rel update < 1e-3 (cap 200)",
+           "particles": P, "iterations": st["iterations"], "time_to_tol_s": round(wall, 3),
+           "ms_per_iteration": round(wall * 1000.0 / max(st["iterations"], 1), 2),
+           "final_rel_update": st["last_rel_update"], "data_residual": round(resid, 5),
+           "rel_l2_error_vs_gt": round(err, 5), "input_rel_l2_error_vs_gt": round(err_in, 5),
+           "conv_mode": "separable", "exact_order_ms_per_iteration": round(exact_ms, 2),
+           "warm_start": "not built yet"}
+    apr.close()
+    for p in (gt, img, parts, u, rec, blur):
+        A._check(L.aprh_free(ctx.h, p))
+    ctx.close()
+    return res
+
+
+def run_cpu_rl():
+    """Reference CPU Richardson-Lucy (oracle/_ref Release build, else the oracle) timed for 3
+    iterations on a 256x256x128 volume of the deconvolution generator (same blob density)"""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    import libapr_amd as A
+    kind, which = "reference", "ref_rel"
+    if oracle_lib.load(which) is None:
+        kind, which = "port", "oracle"
+    if oracle_lib.load(which) is None:
+        return None
+    shape = (128, 256, 256)
+    n = shape[0] * shape[1] * shape[2]
+    g = RL_GEN
+    nb = g["n_blobs"] * n // (RL_SHAPE_ZXY[0] * RL_SHAPE_ZXY[1] * RL_SHAPE_ZXY[2])
+    ctx = A.Context(0)
+    gt, img = _dev(A, ctx, n * 4), _dev(A, ctx, n * 2)
+    A.synth_deconv(shape, g["seed"], nb, gt.value, img.value, sigma=g["sigma"], amp=g["amp"],
+                   background=g["background"], psf_sigma=g["psf_sigma"], psf_taps=g["psf_taps"], ctx=ctx)
+    host = _to_host(A, ctx, img, n, np.uint16).reshape(shape)
+    for p in (gt, img):
+        A._check(A.lib().aprh_free(ctx.h, p))
+    ctx.close()
+    par = oracle_lib.params(rel_error=0.1, sigma_th=50, Ip_th=120, grad_th=10)
+    apr = oracle_lib.get_apr(host, par, which, keep=False)
+    parts = apr.sample(host)
+    psf = A.APRStencil.create_gaussian_filter(g["psf_sigma"], (g["psf_taps"],) * 3, True)
+    threads = oracle_lib.load(which).orc_num_threads()
+    t = time.perf_counter()
+    apr.richardson_lucy(parts, psf, 3)
+    dt = (time.perf_counter() - t) / 3
+    log(f"[bench] cpu RL ({kind}, {threads} threads): {dt * 1000:.1f} ms/iteration at P={apr.total}")
+    return {"ms_per_iteration": round(dt * 1000.0, 2), "particles": apr.total, "cores": threads, "kind": kind,
+            "ns_per_particle_iteration": round(dt * 1e9 / apr.total, 2),
+            "sample": f"3 RL iterations (9^3 PSF) on the APR of a 256x256x128 volume of the same generator ({nb} blobs)"}
 
 
 def run_cpu_baseline():
@@ -169,15 +371,21 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-rl", action="store_true")
+    ap.add_argument("--conv-reps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
               file=sys.stderr)
         sys.exit(2)
-    ms, P, stages, roofline, total_ab = run_gpu(args)
+    ms, P, stages, roofline, total_ab, conv5 = run_gpu(args)
     z, x, y = SHAPE_ZXY
     N = x * y * z
+    conv3 = run_conv_small(args)
+    rl = None if args.no_rl else run_rl(args)
     cpu = None if args.no_cpu else run_cpu_baseline()
+    if rl is not None and not args.no_cpu:
+        rl["cpu_baseline"] = run_cpu_rl()
     out = {
         "metric": METRIC,
         "value": round(N / (ms / 1000.0) / 1e9, 3),
@@ -196,7 +404,8 @@ def main():
                    "particles": P, "pipeline_algorithmic_bytes": total_ab,
                    "pipeline_GBps": round(total_ab / (ms / 1000.0) / 1e9, 1),
                    "stage_ms": {k: round(v, 3) for k, v in stages.items()},
-                   "rl_time_to_tol": "not measured yet"},
+                   "conv_3x3x3_256x256x128": conv3, "conv_5x5x5_2048x2048x512": conv5,
+                   "rl_time_to_tol": rl},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

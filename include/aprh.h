@@ -63,6 +63,15 @@ int aprh_context_set_keep_intermediates(aprh_context* ctx, int keep);
    named by aprh_stage_name(i), i < aprh_stage_count(). */
 int aprh_context_set_timing(aprh_context* ctx, int on);
 int aprh_context_stage_times(aprh_context* ctx, float* ms, int n);
+/* how APR convolutions on this context sum a stencil:
+   APRH_CONV_EXACT      every tap in the reference's (z, x, y) order (APRFilter.hpp:721-733):
+                        results bit-equal to the reference;
+   APRH_CONV_SEPARABLE  stencils that are outer products of 1-D factors (Gaussian PSFs and
+                        their downsampled stacks) run as three 1-D passes over each level's
+                        piecewise-constant image; others as EXACT.  Results within float
+                        rounding of EXACT (no reference counterpart; faster for large stencils). */
+enum { APRH_CONV_EXACT = 0, APRH_CONV_SEPARABLE = 1 };
+int aprh_context_set_conv_mode(aprh_context* ctx, int mode);
 int aprh_stage_count(void);
 const char* aprh_stage_name(int i);
 int aprh_context_release_cache(aprh_context* ctx);
@@ -186,6 +195,13 @@ int aprh_synth_blobs_host(uint16_t* out, int64_t y_num, int64_t x_num, int64_t z
 int aprh_synth_blobs_device(aprh_context* ctx, uint16_t* out, int64_t y_num, int64_t x_num, int64_t z_num, uint64_t seed,
                             int64_t n_blobs, float sigma_min, float sigma_max, float amp_min, float amp_max,
                             float background, int noise);
+
+/* deconvolution workload: gt = background + blobs (f32, device), blurred by a separable
+   Gaussian PSF (sigma per axis y, x, z; psf_taps odd, normalised factors, reflective ends),
+   then shot noise into noisy (u16, device) */
+int aprh_synth_deconv_device(aprh_context* ctx, float* gt, uint16_t* noisy, int64_t y_num, int64_t x_num, int64_t z_num,
+                             uint64_t seed, int64_t n_blobs, float sigma_min, float sigma_max, float amp_min, float amp_max,
+                             float background, const float* psf_sigma3, int32_t psf_taps);
 
 /* ---- device memory helpers for callers without their own allocator ---- */
 int aprh_malloc(aprh_context* ctx, size_t bytes, void** out);

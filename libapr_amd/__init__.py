@@ -68,6 +68,8 @@ class AprhRLStats(C.Structure):
 
 APRH_DTYPE_U16 = 0
 APRH_DTYPE_F32 = 1
+CONV_EXACT = 0
+CONV_SEPARABLE = 1
 
 _lib = None
 vp = C.c_void_p
@@ -87,6 +89,7 @@ def lib():
     L.aprh_context_destroy.argtypes = [vp]
     L.aprh_context_set_keep_intermediates.argtypes = [vp, C.c_int]
     L.aprh_context_set_timing.argtypes = [vp, C.c_int]
+    L.aprh_context_set_conv_mode.argtypes = [vp, C.c_int]
     L.aprh_context_stage_times.argtypes = [vp, C.POINTER(C.c_float), C.c_int]
     L.aprh_stage_name.restype = C.c_char_p
     L.aprh_stage_name.argtypes = [C.c_int]
@@ -126,6 +129,8 @@ def lib():
     L.aprh_richardson_lucy.argtypes = [vp, vp, vp, C.c_int, vp, vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int,
                                        C.c_int, C.c_int, C.c_double, C.POINTER(AprhRLStats)]
     L.aprh_reconstruct_constant.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp, C.c_int]
+    L.aprh_synth_deconv_device.argtypes = [vp, vp, vp, i64, i64, i64, C.c_uint64, i64, C.c_float, C.c_float, C.c_float,
+                                           C.c_float, C.c_float, vp, C.c_int32]
     L.aprh_malloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
     L.aprh_free.argtypes = [vp, vp]
     L.aprh_memcpy.argtypes = [vp, vp, vp, C.c_size_t]
@@ -194,6 +199,11 @@ class Context:
 
     def set_timing(self, on):
         _check(lib().aprh_context_set_timing(self.h, int(on)))
+
+    def set_conv_mode(self, mode):
+        """CONV_EXACT (reference summation order, bit-equal) or CONV_SEPARABLE (rank-1 stencils
+        as three 1-D passes, within float rounding of exact)"""
+        _check(lib().aprh_context_set_conv_mode(self.h, int(mode)))
 
     def stage_times(self):
         n = lib().aprh_stage_count()
@@ -596,3 +606,15 @@ class APRReconstruction:
             out = np.zeros((-(-z // s), -(-x // s), -(-y // s)), odt)
         _check(lib().aprh_reconstruct_constant(apr.ctx.h, apr.h, _ptr(parts), code, int(level_delta), _ptr(out), ocode))
         return out
+
+
+def synth_deconv(shape_zxy, seed, n_blobs, gt_out, noisy_out, sigma=(1.0, 4.0), amp=(200.0, 2000.0), background=100.0,
+                 psf_sigma=(1.0, 1.0, 2.0), psf_taps=9, ctx=None):
+    """Deconvolution workload on the device: gt_out (f32) = background + blobs, noisy_out (u16) =
+    shot noise of gt blurred by a separable Gaussian PSF (psf_sigma in (y, x, z) order).
+    gt_out / noisy_out are device addresses (or torch tensors)."""
+    ctx = ctx or default_context()
+    z, x, y = shape_zxy
+    ps = np.array(psf_sigma, np.float32)
+    _check(lib().aprh_synth_deconv_device(ctx.h, _ptr(gt_out), _ptr(noisy_out), y, x, z, seed, n_blobs, sigma[0], sigma[1],
+                                          amp[0], amp[1], background, _ptr(ps), psf_taps))

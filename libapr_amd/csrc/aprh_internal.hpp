@@ -122,6 +122,7 @@ struct aprh_context {
     bool ev_marked[aprh::ST_COUNT + 1] = {};
     float stage_ms[aprh::ST_COUNT] = {};
     int* d_flags = nullptr;   /* small device scratch: overflow counters etc. */
+    int conv_mode = 0;        /* APRH_CONV_EXACT / APRH_CONV_SEPARABLE */
 };
 
 /* device-resident APR: GenInfo + LinearAccess (+ optional intermediates) */
@@ -236,6 +237,8 @@ int richardson_lucy(aprh_context* ctx, aprh_apr& a, const T* in, float* u, const
 void reconstruct_any(aprh_context* ctx, aprh_apr& a, const void* parts, int pdt, int level, void* img, int idt);
 
 /* ---- synthetic data (k_synth.hip) ---- */
+void synth_deconv_device(aprh_context* ctx, float* gt, uint16_t* noisy, int64_t y, int64_t x, int64_t z, uint64_t seed, int64_t n_blobs,
+                         float smin, float smax, float amin, float amax, float background, const float* psf_sigma3, int taps);
 void synth_device(aprh_context* ctx, uint16_t* out, int64_t y, int64_t x, int64_t z, uint64_t seed, int64_t n_blobs, float smin,
                   float smax, float amin, float amax, float background, int noise);
 

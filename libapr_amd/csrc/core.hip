@@ -316,6 +316,14 @@ int aprh_context_set_timing(aprh_context* ctx, int on) {
     APRH_CATCH
 }
 
+int aprh_context_set_conv_mode(aprh_context* ctx, int mode) {
+    APRH_TRY
+    if (!ctx) throw Error(APRH_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (mode != APRH_CONV_EXACT && mode != APRH_CONV_SEPARABLE) throw Error(APRH_ERR_INVALID_ARGUMENT, "unknown convolution mode");
+    ctx->conv_mode = mode;
+    APRH_CATCH
+}
+
 int aprh_context_stage_times(aprh_context* ctx, float* ms, int n) {
     APRH_TRY
     if (!ctx || !ms) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");

@@ -238,6 +238,8 @@ __global__ void k_upsample(const D* __restrict__ C, int64_t cyn, int64_t cxn, D*
 struct StencilDev {
     int sy, sx, sz;
     int64_t off;   /* float offset in the stencil buffer */
+    int sep;       /* evaluated as 1-D factors (fy, fx, fz at fy_off, fx_off, fz_off) */
+    int64_t fy_off, fx_off, fz_off;
 };
 
 __device__ __forceinline__ int64_t refl_idx(int64_t q, int64_t n) {
@@ -289,6 +291,53 @@ __global__ __launch_bounds__(256) void k_conv_level(Rows R, int li, const uint64
                 }
             }
             out[i] = val;
+        }
+    }
+}
+
+/* one 1-D pass of a separable stencil along y (AXIS 0) or x (AXIS 1) over a level image */
+template <int AXIS>
+__global__ __launch_bounds__(256) void k_sep_pass(const float* __restrict__ in, float* out, int64_t yn, int64_t xn, int64_t zn,
+                                                  const float* __restrict__ w, int taps, int reflect) {
+    const size_t n = (size_t)(yn * xn * zn);
+    const int h = (taps - 1) / 2;
+    const int64_t stride = AXIS == 0 ? 1 : yn;
+    const int64_t len = AXIS == 0 ? yn : xn;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int64_t c = AXIS == 0 ? (int64_t)(i % (size_t)yn) : (int64_t)((i / (size_t)yn) % (size_t)xn);
+        const float* base = in + ((int64_t)i - c * stride);
+        float acc = 0.0f;
+        for (int k = 0; k < taps; ++k) {
+            const int64_t q = c + k - h;
+            if (reflect) acc += w[k] * base[refl_idx(q, len) * stride];
+            else if (q >= 0 && q < len) acc += w[k] * base[q * stride];
+        }
+        out[i] = acc;
+    }
+}
+
+/* last (z) pass of a separable stencil, evaluated only at the particles of level li */
+__global__ __launch_bounds__(256) void k_sep_z_gather(Rows R, int li, const uint64_t* __restrict__ xz_end,
+                                                      const uint16_t* __restrict__ y_vec, const float* __restrict__ T,
+                                                      const float* __restrict__ w, int taps, int reflect, float* out) {
+    const int64_t xn = R.x_num[li], zn = R.z_num[li], yn = R.y_num[li];
+    const int64_t nrows = xn * zn;
+    const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / 64;
+    const int h = (taps - 1) / 2;
+    for (int64_t row = wid; row < nrows; row += nw) {
+        const int64_t z = row / xn, x = row % xn;
+        const uint64_t b = xz_end[R.row_begin[li] + row - 1], e = xz_end[R.row_begin[li] + row];
+        for (uint64_t i = b + lane; i < e; i += 64) {
+            const int64_t y = y_vec[i];
+            float acc = 0.0f;
+            for (int k = 0; k < taps; ++k) {
+                const int64_t q = z + k - h;
+                if (reflect) acc += w[k] * T[(refl_idx(q, zn) * xn + x) * yn + y];
+                else if (q >= 0 && q < zn) acc += w[k] * T[(q * xn + x) * yn + y];
+            }
+            out[i] = acc;
         }
     }
 }
@@ -570,6 +619,33 @@ std::vector<HostStencil> downsampled_stencils(const HostStencil& in, int nlevels
     return out;
 }
 
+/* 1-D factors (fy, fx, fz) with fz (x) fx (x) fy == w to within 1e-6 of max|w|, if they exist */
+bool rank1_factors(const HostStencil& s, std::vector<float>& fy, std::vector<float>& fx, std::vector<float>& fz) {
+    size_t c = 0;
+    double mx = 0;
+    for (size_t i = 0; i < s.w.size(); ++i)
+        if (std::fabs((double)s.w[i]) > mx) {
+            mx = std::fabs((double)s.w[i]);
+            c = i;
+        }
+    if (mx == 0) return false;
+    const int cy = (int)(c % s.sy), cx = (int)((c / s.sy) % s.sx), cz = (int)(c / ((size_t)s.sy * s.sx));
+    auto W = [&](int z, int x, int y) { return (double)s.w[((size_t)z * s.sx + x) * s.sy + y]; };
+    const double wc = W(cz, cx, cy);
+    std::vector<double> dy(s.sy), dx(s.sx), dz(s.sz);
+    for (int y = 0; y < s.sy; ++y) dy[y] = W(cz, cx, y) / wc;
+    for (int x = 0; x < s.sx; ++x) dx[x] = W(cz, x, cy) / wc;
+    for (int z = 0; z < s.sz; ++z) dz[z] = W(z, cx, cy);
+    for (int z = 0; z < s.sz; ++z)
+        for (int x = 0; x < s.sx; ++x)
+            for (int y = 0; y < s.sy; ++y)
+                if (std::fabs(dz[z] * dx[x] * dy[y] - W(z, x, y)) > 1e-6 * mx) return false;
+    fy.assign(dy.begin(), dy.end());
+    fx.assign(dx.begin(), dx.end());
+    fz.assign(dz.begin(), dz.end());
+    return true;
+}
+
 template <typename T>
 void convolve(aprh_context* ctx, aprh_apr& a, const std::vector<HostStencil>& stencils, const T* in, float* out, bool reflect) {
     if (stencils.empty()) throw Error(APRH_ERR_INVALID_ARGUMENT, "no stencils");
@@ -581,9 +657,22 @@ void convolve(aprh_context* ctx, aprh_apr& a, const std::vector<HostStencil>& st
     tree_mean(ctx, a, in, tree.as<float>());
     std::vector<float> flat;
     std::vector<StencilDev> sd;
+    bool any_sep = false;
     for (const auto& s : stencils) {
-        sd.push_back(StencilDev{s.sy, s.sx, s.sz, (int64_t)flat.size()});
+        StencilDev d{s.sy, s.sx, s.sz, (int64_t)flat.size(), 0, 0, 0, 0};
         flat.insert(flat.end(), s.w.begin(), s.w.end());
+        std::vector<float> fy, fx, fz;
+        if (ctx->conv_mode == APRH_CONV_SEPARABLE && rank1_factors(s, fy, fx, fz)) {
+            d.sep = 1;
+            d.fy_off = (int64_t)flat.size();
+            flat.insert(flat.end(), fy.begin(), fy.end());
+            d.fx_off = (int64_t)flat.size();
+            flat.insert(flat.end(), fx.begin(), fx.end());
+            d.fz_off = (int64_t)flat.size();
+            flat.insert(flat.end(), fz.begin(), fz.end());
+            any_sep = true;
+        }
+        sd.push_back(d);
     }
     Buf W(ctx->pool, flat.size() * 4);
     HIP_CHECK(hipMemcpyAsync(W.ptr(), flat.data(), flat.size() * 4, hipMemcpyHostToDevice, st));
@@ -591,15 +680,35 @@ void convolve(aprh_context* ctx, aprh_apr& a, const std::vector<HostStencil>& st
     Rows TR = make_rows(a.t_y, a.t_x, a.t_z, a.t_level_xz, 0, a.t_l_max);
     const size_t maxn = level_elems(a, a.l_max);
     Buf D0(ctx->pool, maxn * 4), D1(ctx->pool, maxn * 4);
+    Buf T1, T2;
+    if (any_sep) {
+        T1.reset(ctx->pool, maxn * 4);
+        T2.reset(ctx->pool, maxn * 4);
+    }
     float* prev = nullptr;
     float* cur = D0.as<float>();
     float* other = D1.as<float>();
     for (int l = a.l_min; l <= a.l_max; ++l) {
         build_level<T, float>(ctx, a, AR, TR, l, prev, cur, in, tree.as<float>(), true);
         const int sn = std::min((int)stencils.size() - 1, a.l_max - l);
-        hipLaunchKernelGGL(k_conv_level, dim3(8192), kBlock, 0, st, AR, l - a.l_min, a.xz_end.as<uint64_t>(), a.y_vec.as<uint16_t>(),
-                           (const float*)cur, W.as<float>(), sd[(size_t)sn], reflect ? 1 : 0, out);
-        HIP_LAUNCH_CHECK();
+        const StencilDev& d = sd[(size_t)sn];
+        if (d.sep) {
+            const int64_t yn = a.y_num[l], xn = a.x_num[l], zn = a.z_num[l];
+            hipLaunchKernelGGL(k_sep_pass<0>, dim3(8192), kBlock, 0, st, (const float*)cur, T1.as<float>(), yn, xn, zn,
+                               (const float*)(W.as<float>() + d.fy_off), d.sy, reflect ? 1 : 0);
+            HIP_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_sep_pass<1>, dim3(8192), kBlock, 0, st, (const float*)T1.as<float>(), T2.as<float>(), yn, xn, zn,
+                               (const float*)(W.as<float>() + d.fx_off), d.sx, reflect ? 1 : 0);
+            HIP_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_sep_z_gather, dim3(8192), kBlock, 0, st, AR, l - a.l_min, a.xz_end.as<uint64_t>(),
+                               a.y_vec.as<uint16_t>(), (const float*)T2.as<float>(), (const float*)(W.as<float>() + d.fz_off), d.sz,
+                               reflect ? 1 : 0, out);
+            HIP_LAUNCH_CHECK();
+        } else {
+            hipLaunchKernelGGL(k_conv_level, dim3(8192), kBlock, 0, st, AR, l - a.l_min, a.xz_end.as<uint64_t>(),
+                               a.y_vec.as<uint16_t>(), (const float*)cur, W.as<float>(), d, reflect ? 1 : 0, out);
+            HIP_LAUNCH_CHECK();
+        }
         prev = cur;
         std::swap(cur, other);
     }

@@ -191,3 +191,50 @@ def test_properties_at_size(ctx):
     u, s = A.APRNumerics.richardson_lucy(g, c, None, st, 3, normalize=True)
     np.testing.assert_allclose(u, 7.0, rtol=1e-4)
     assert s["iterations"] == 3
+
+
+@pytest.fixture(scope="module")
+def sep_ctx():
+    c = A.Context(0)
+    c.set_conv_mode(A.CONV_SEPARABLE)
+    yield c
+    c.close()
+
+
+def _rel(got, exp):
+    exp = exp.astype(np.float64)
+    return np.abs(got.astype(np.float64) - exp) / np.maximum(np.abs(exp), 1e-3 * np.abs(exp).max())
+
+
+@pytest.mark.parametrize("shape,nb,seed", CASES)
+@pytest.mark.parametrize("kind", ["mean3", "gauss5", "aniso"])
+@pytest.mark.parametrize("reflect", [True, False])
+def test_convolve_separable_mode(oracle, sep_ctx, shape, nb, seed, kind, reflect):
+    """CONV_SEPARABLE: rank-1 stencils as three 1-D passes; within 1e-5 relative of the
+    reference-order result, and as close as it to the float64 ground truth"""
+    img, o, g = _pair(oracle, sep_ctx, shape, nb, seed)
+    parts = o.sample(img).astype(np.float32)
+    st, stack = _stencils(oracle, kind, o)
+    exp = o.convolve_pencil(stack, parts, reflect)
+    got = A.APRFilter.convolve_pencil(g, stack, parts, reflect_boundary=reflect)
+    r = _rel(got, exp).max()
+    print(f"separable vs reference order: max rel {r:.2e}")
+    assert r < 1e-5
+    if shape == CASES[1][0]:
+        from test_oracle_numerics import conv_gt
+        t = o.tree()
+        gt = conv_gt(o, stack, parts.astype(np.float64), o.fill_tree_mean(parts).astype(np.float64), t, reflect)
+        e_exact, e_sep = _rel(exp, gt).max(), _rel(got, gt).max()
+        print(f"vs float64 ground truth: reference order {e_exact:.2e}, separable {e_sep:.2e}")
+        assert e_sep < 1e-5 and e_exact < 1e-5
+
+
+def test_richardson_lucy_separable_mode(oracle, sep_ctx):
+    img, o, g = _pair(oracle, sep_ctx, *CASES[0])
+    parts = o.sample(img)
+    psf = oracle.gaussian_stencil((1.0, 1.0, 2.0), (9, 9, 9), True)
+    uo, _, _ = o.richardson_lucy(parts, psf, 3)
+    ug, st = A.APRNumerics.richardson_lucy(g, parts, None, psf, 3, normalize=True)
+    r = _rel(ug, uo).max()
+    print(f"RL 3 iterations, separable vs reference order: max rel {r:.2e}")
+    assert st["iterations"] == 3 and r < 1e-5

@@ -5,8 +5,8 @@ stencils, convolve_pencil / convolve, richardson_lucy) -- CPU only.
 * the reference test's ground truth for convolution (test/FilterTestHelpers.hpp:
   98-192): reconstruct each level piecewise-constant (coarser particles fill their
   blocks, finer regions take the tree means) and apply the stencil densely with
-  reflective or zero boundaries, in float64 -- tolerance 1e-4 relative like
-  compareParticles (TestTools.hpp:93-111);
+  reflective or zero boundaries, in float64 -- tolerance 1e-5 relative (the
+  reference's compareParticles allows 1e-4, TestTools.hpp:93-111);
 * closed forms: a constant input gives constant tree means; normalised stencils
   sum to 1; Richardson-Lucy keeps a constant image constant.
 """
@@ -184,7 +184,7 @@ def test_convolve_ground_truth(kind, reflect):
     t = o.tree()
     gt = conv_gt(o, st, parts.astype(np.float64), o.fill_tree_mean(parts).astype(np.float64), t, reflect)
     rel = np.abs(out - gt) / np.maximum(np.abs(gt), 1e-3)
-    assert rel.max() < 1e-4, rel.max()
+    assert rel.max() < 1e-5, rel.max()
 
 
 @pytest.mark.parametrize("shape,nb,seed", CASES[:2])
