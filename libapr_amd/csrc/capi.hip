@@ -147,8 +147,8 @@ int aprh_get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_
                      aprh_apr** out) {
     APRH_TRY
     check_ctx(ctx);
-    if (!image || !par || !out) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
     check_dims(y, x, z);
+    if (!image || !par || !out) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
     aprh_apr* a = new aprh_apr();
     a->ctx = ctx;
     try {

@@ -1,0 +1,148 @@
+// C++ drop-in check: the apr_amd.hpp facade (reference-shaped API) against the
+// CPU oracle (liboracle.so), on a small synthetic volume.  Exit status 0 and
+// "facade ok" on success.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+#include "../../include/apr_amd.hpp"
+
+extern "C" {
+void* orc_get_apr_u16(const uint16_t* img, int64_t y, int64_t x, int64_t z, const aprh_params* par, int keep, char* errbuf, int errlen);
+void orc_apr_info(void* h, int64_t* out);
+void orc_apr_get(void* h, uint64_t* level_xz, uint64_t* xz_end, uint16_t* y_vec);
+void orc_sample_image_u16(void* h, const uint16_t* img, uint16_t* parts);
+void orc_fill_tree_mean_u16(void* h, const uint16_t* parts, float* tree);
+void orc_tree_info(void* h, int64_t* out);
+int64_t orc_downsampled_stencils(const float* st, int sy, int sx, int sz, int nlevels, int normalize, float* out, int64_t cap,
+                                 int* dims_out);
+void orc_convolve_pencil_u16(void* h, const float* stencils, const int* dims, int n, const uint16_t* in, float* out, int reflect);
+int orc_richardson_lucy_u16(void* h, const uint16_t* in, float* u, const float* psf, int sy, int sx, int sz, int niter, int use_ds,
+                            int normalize, int resume, double tol, double* rel);
+void orc_reconstruct(void* h, const void* parts, int pdt, int level_delta, void* img, int idt);
+void orc_apr_free(void* h);
+}
+
+using namespace apr_amd;
+
+static int failures = 0;
+#define CHECK(c)                                                         \
+    do {                                                                 \
+        if (!(c)) {                                                      \
+            std::printf("FAIL %s:%d: %s\n", __FILE__, __LINE__, #c);     \
+            ++failures;                                                  \
+        }                                                                \
+    } while (0)
+
+template <typename T>
+static bool same_bytes(const std::vector<T>& a, const T* b, size_t n) {
+    return a.size() == n && (n == 0 || std::memcmp(a.data(), b, n * sizeof(T)) == 0);
+}
+
+int main() {
+    const int64_t y = 96, x = 72, z = 40;
+    PixelData<uint16_t> img(y, x, z);
+    if (aprh_synth_blobs_host(img.mesh.data(), y, x, z, 0, 24, 2.0f, 6.0f, 500.0f, 3000.0f, 100.0f, 1) != APRH_OK) return 2;
+
+    Context ctx(0);
+    APRConverter<uint16_t> conv(ctx);
+    conv.par.rel_error = 0.1f;
+    conv.par.sigma_th = 50.0f;
+    conv.par.Ip_th = 120.0f;
+    conv.par.grad_th = 10.0f;
+    APR apr;
+    CHECK(conv.get_apr(apr, img));
+
+    char err[256];
+    void* o = orc_get_apr_u16(img.mesh.data(), y, x, z, &conv.par, 0, err, sizeof(err));
+    CHECK(o != nullptr);
+    int64_t info[8];
+    orc_apr_info(o, info);
+    const uint64_t P = apr.total_number_particles();
+    std::vector<uint64_t> lxz, xze;
+    std::vector<uint16_t> yv;
+    apr.linear_access(lxz, xze, yv);
+    std::vector<uint64_t> olxz(lxz.size()), oxze(xze.size());
+    std::vector<uint16_t> oyv(P);
+    orc_apr_get(o, olxz.data(), oxze.data(), oyv.data());
+    CHECK(same_bytes(xze, oxze.data(), oxze.size()));
+    CHECK(same_bytes(yv, oyv.data(), oyv.size()));
+    CHECK(same_bytes(lxz, olxz.data(), olxz.size()));
+
+    ParticleData<uint16_t> parts;
+    parts.sample_image(apr, img);
+    std::vector<uint16_t> oparts(P);
+    orc_sample_image_u16(o, img.mesh.data(), oparts.data());
+    CHECK(same_bytes(parts.data, oparts.data(), P));
+
+    // tree means
+    ParticleData<float> tree;
+    APRTreeNumerics::fill_tree_mean(apr, parts, tree);
+    int64_t tinfo[5];
+    orc_tree_info(o, tinfo);
+    std::vector<float> otree((size_t)tinfo[2]);
+    orc_fill_tree_mean_u16(o, oparts.data(), otree.data());
+    CHECK(same_bytes(tree.data, otree.data(), otree.size()));
+
+    // 5x5x5 Gaussian convolution, single-stencil overload (downsampled + normalised stack)
+    PixelData<float> g5 = APRStencil::create_gaussian_filter({1.0f, 1.0f, 1.0f}, {5, 5, 5}, true);
+    ParticleData<float> out;
+    APRFilter::convolve_pencil(apr, g5, parts, out, true, true, true);
+    const int nl = apr.level_max() - apr.level_min();
+    const int64_t tot = orc_downsampled_stencils(g5.mesh.data(), 5, 5, 5, nl, 1, nullptr, 0, nullptr);
+    std::vector<float> flat((size_t)tot);
+    std::vector<int> dims(3 * (size_t)nl);
+    orc_downsampled_stencils(g5.mesh.data(), 5, 5, 5, nl, 1, flat.data(), tot, dims.data());
+    std::vector<float> oout(P);
+    orc_convolve_pencil_u16(o, flat.data(), dims.data(), nl, oparts.data(), oout.data(), 1);
+    CHECK(same_bytes(out.data, oout.data(), P));
+    // the vector overload and APRFilter::convolve agree
+    std::vector<PixelData<float>> stack;
+    APRStencil::get_downsampled_stencils(g5, stack, nl, true);
+    ParticleData<float> out2;
+    APRFilter::convolve(apr, stack, parts, out2, true);
+    CHECK(same_bytes(out2.data, oout.data(), P));
+
+    // Richardson-Lucy, 2 iterations, 9x9x9 PSF
+    PixelData<float> psf = APRStencil::create_gaussian_filter({1.0f, 1.0f, 2.0f}, {9, 9, 9}, true);
+    ParticleData<float> u;
+    aprh_rl_stats st = APRNumerics::richardson_lucy(apr, parts, u, psf, 2, true, true);
+    std::vector<float> ou(P, 1.0f);
+    double rel = 0;
+    orc_richardson_lucy_u16(o, oparts.data(), ou.data(), psf.mesh.data(), 9, 9, 9, 2, 1, 1, 0, 0.0, &rel);
+    CHECK(st.iterations == 2);
+    CHECK(same_bytes(u.data, ou.data(), P));
+
+    // reconstruction at full resolution and one level down
+    for (int d = 0; d >= -1; --d) {
+        PixelData<uint16_t> rec;
+        APRReconstruction::reconstruct_constant(apr, rec, parts, d);
+        std::vector<uint16_t> orec(rec.mesh.size());
+        orc_reconstruct(o, oparts.data(), 0, d, orec.data(), 0);
+        CHECK(same_bytes(rec.mesh, orec.data(), orec.size()));
+    }
+
+    // error conventions
+    PixelData<uint16_t> empty;
+    APR a2;
+    CHECK(!conv.get_apr(a2, empty));
+    bool threw = false;
+    try {
+        PixelData<float> even(4, 3, 3);
+        APRFilter::convolve_pencil(apr, even, parts, out);
+    } catch (std::runtime_error&) {
+        threw = true;
+    }
+    CHECK(threw);
+
+    orc_apr_free(o);
+    if (failures) {
+        std::printf("%d failure(s)\n", failures);
+        return 1;
+    }
+    std::printf("facade ok: %lldx%lldx%lld -> %llu particles, bit-exact vs oracle\n", (long long)y, (long long)x, (long long)z,
+                (unsigned long long)P);
+    return 0;
+}
