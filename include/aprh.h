@@ -101,6 +101,8 @@ int aprh_apr_destroy(aprh_apr* apr);
 
 /* ParticleData<uint16_t>::sample_image (ParticleData.hpp:122-127, :221-261) */
 int aprh_sample_image_u16(aprh_context* ctx, const aprh_apr* apr, const uint16_t* image, uint16_t* parts);
+/* ParticleData<float>::sample_image of a float image (same mean pyramid, float means) */
+int aprh_sample_image_f32(aprh_context* ctx, const aprh_apr* apr, const float* image, float* parts);
 
 /* ---- particle operations ---- */
 /* element type of particle buffers (ParticleData<uint16_t> / ParticleData<float>) */
@@ -165,6 +167,41 @@ int aprh_richardson_lucy(aprh_context* ctx, aprh_apr* apr, const void* parts_in,
    dimensions of level l_max + level_delta, (y, x, z) with y fastest. */
 int aprh_reconstruct_constant(aprh_context* ctx, aprh_apr* apr, const void* parts, int parts_dtype, int level_delta, void* image,
                               int image_dtype);
+
+/* ---- learned Richardson-Lucy warm start (no reference counterpart) ----
+   A 3-D CNN: four valid 3x3x3 convolutions 1 -> 16 -> 16 -> 16 -> 1 channels, ReLU after
+   the first three, residual output y = x + conv4.  A tile of side S + 8 gives an output of
+   side S.  Layers 1-3 run on bf16 MFMA with f32 accumulation, layer 4 in f32.  Parameters
+   (f32, aprh_cnn_param_count() values): W1[27][16] b1[16] W2[27][16][16] b2[16]
+   W3[27][16][16] b3[16] W4[27][16] b4, tap = (dz*3 + dx)*3 + dy. */
+typedef struct aprh_cnn aprh_cnn;
+int aprh_cnn_param_count(void);
+/* He-initialised network (seeded); Adam state zero */
+int aprh_cnn_create(aprh_context* ctx, uint64_t seed, aprh_cnn** out);
+int aprh_cnn_destroy(aprh_cnn* net);
+int aprh_cnn_get_params(aprh_context* ctx, aprh_cnn* net, float* out);
+/* replaces the parameters and resets the Adam state */
+int aprh_cnn_set_params(aprh_context* ctx, aprh_cnn* net, const float* in);
+/* in[T][(S+8)^3] -> out[T][S^3] (f32, y fastest within a tile) */
+int aprh_cnn_forward_tiles(aprh_context* ctx, aprh_cnn* net, const float* in, int32_t T, int32_t S, float* out);
+/* one Adam step (lr 0: gradient only) on the mean squared error against target[T][S^3];
+   loss (before the step) and, optionally, the gradient (param order) are returned */
+int aprh_cnn_train_step(aprh_context* ctx, aprh_cnn* net, const float* in, const float* target, int32_t T, int32_t S, float lr,
+                        float* loss, float* grads_out);
+/* tiles of side size + 2*halo at origins (y, x, z per tile) minus halo, reflect-padded at the
+   volume border, divided by scale */
+int aprh_cnn_extract_tiles(aprh_context* ctx, const void* image, int dtype, int64_t y_num, int64_t x_num, int64_t z_num,
+                           const int32_t* origins, int32_t T, int32_t size, int32_t halo, float scale, float* out);
+/* the network over a whole image in 64^3 tiles (4-voxel reflect halo at the border);
+   the image is divided by scale (<= 0: its mean) and the result multiplied back */
+int aprh_cnn_apply(aprh_context* ctx, aprh_cnn* net, const void* image, int dtype, int64_t y_num, int64_t x_num, int64_t z_num,
+                   float scale, float* out);
+/* warm start for aprh_richardson_lucy(..., resume = 1): the network applied to the
+   piecewise-constant reconstruction of the particles (normalised by its mean), floored at
+   1e-3 of the mean, sampled back onto the particles (ParticleData::sample_image) */
+int aprh_cnn_warm_start(aprh_context* ctx, aprh_cnn* net, aprh_apr* apr, const void* parts, int dtype, float* u0);
+/* mean of n u16 / f32 values (host or device), accumulated in double */
+int aprh_mean(aprh_context* ctx, const void* data, int dtype, uint64_t n, double* mean);
 
 /* ---- stage-level entry points (public methods of the reference's stage classes) ---- */
 /* ComputeGradient::bspline_filt_rec_y/_x/_z (ComputeGradient.hpp:248-413, 415-599, 601-779); axis 0=y 1=x 2=z.

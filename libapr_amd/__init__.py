@@ -131,6 +131,17 @@ def lib():
     L.aprh_reconstruct_constant.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp, C.c_int]
     L.aprh_synth_deconv_device.argtypes = [vp, vp, vp, i64, i64, i64, C.c_uint64, i64, C.c_float, C.c_float, C.c_float,
                                            C.c_float, C.c_float, vp, C.c_int32]
+    L.aprh_sample_image_f32.argtypes = [vp, vp, vp, vp]
+    L.aprh_cnn_create.argtypes = [vp, C.c_uint64, C.POINTER(vp)]
+    L.aprh_cnn_destroy.argtypes = [vp]
+    L.aprh_cnn_get_params.argtypes = [vp, vp, vp]
+    L.aprh_cnn_set_params.argtypes = [vp, vp, vp]
+    L.aprh_cnn_forward_tiles.argtypes = [vp, vp, vp, C.c_int32, C.c_int32, vp]
+    L.aprh_cnn_train_step.argtypes = [vp, vp, vp, vp, C.c_int32, C.c_int32, C.c_float, C.POINTER(C.c_float), vp]
+    L.aprh_cnn_extract_tiles.argtypes = [vp, vp, C.c_int, i64, i64, i64, vp, C.c_int32, C.c_int32, C.c_int32, C.c_float, vp]
+    L.aprh_cnn_apply.argtypes = [vp, vp, vp, C.c_int, i64, i64, i64, C.c_float, vp]
+    L.aprh_cnn_warm_start.argtypes = [vp, vp, vp, vp, C.c_int, vp]
+    L.aprh_mean.argtypes = [vp, vp, C.c_int, C.c_uint64, C.POINTER(C.c_double)]
     L.aprh_malloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
     L.aprh_free.argtypes = [vp, vp]
     L.aprh_memcpy.argtypes = [vp, vp, vp, C.c_size_t]
@@ -369,15 +380,17 @@ class APRConverter:
 
 
 class ParticleData:
-    """ParticleData<uint16_t>::sample_image (ParticleData.hpp:122-127)."""
+    """ParticleData<T>::sample_image (ParticleData.hpp:122-127) for uint16 and float images."""
 
     def __init__(self):
         self.data = None
 
-    def sample_image(self, apr, image, out=None):
+    def sample_image(self, apr, image, out=None, dtype=None):
+        code = _dtype_of(image, dtype)
         if out is None:
-            out = np.zeros(max(apr.total, 1), np.uint16)
-        _check(lib().aprh_sample_image_u16(apr.ctx.h, apr.h, _ptr(image), _ptr(out)))
+            out = np.zeros(max(apr.total, 1), np.uint16 if code == APRH_DTYPE_U16 else np.float32)
+        f = lib().aprh_sample_image_u16 if code == APRH_DTYPE_U16 else lib().aprh_sample_image_f32
+        _check(f(apr.ctx.h, apr.h, _ptr(image), _ptr(out)))
         self.data = out[:apr.total] if isinstance(out, np.ndarray) else out
         return self.data
 
@@ -618,3 +631,117 @@ def synth_deconv(shape_zxy, seed, n_blobs, gt_out, noisy_out, sigma=(1.0, 4.0), 
     ps = np.array(psf_sigma, np.float32)
     _check(lib().aprh_synth_deconv_device(ctx.h, _ptr(gt_out), _ptr(noisy_out), y, x, z, seed, n_blobs, sigma[0], sigma[1],
                                           amp[0], amp[1], background, _ptr(ps), psf_taps))
+
+
+# ---------------- learned Richardson-Lucy warm start ----------------
+
+CNN_PARAMS = 14737
+CNN_TILE = 64
+CNN_HALO = 4
+
+
+def image_mean(data, n=None, dtype=None, ctx=None):
+    """mean of a u16 / f32 array (host or device), accumulated in double on the device"""
+    ctx = ctx or default_context()
+    code = _dtype_of(data, dtype)
+    n = data.size if n is None else n
+    m = C.c_double()
+    _check(lib().aprh_mean(ctx.h, _ptr(data), code, n, C.byref(m)))
+    return m.value
+
+
+def extract_tiles(image, dims, origins, size, halo, scale, out=None, dtype=None, ctx=None):
+    """tiles of side size + 2*halo at origins ((y, x, z) per tile) minus halo, reflect-padded,
+    divided by scale; returns float32 [T, s, s, s]"""
+    ctx = ctx or default_context()
+    code = _dtype_of(image, dtype)
+    y, x, z = dims
+    org = np.ascontiguousarray(np.asarray(origins, np.int32).reshape(-1, 3))
+    T = org.shape[0]
+    s = size + 2 * halo
+    if out is None:
+        out = np.zeros((T, s, s, s), np.float32)
+    _check(lib().aprh_cnn_extract_tiles(ctx.h, _ptr(image), code, y, x, z, _ptr(org), T, size, halo, scale, _ptr(out)))
+    return out
+
+
+class WarmStartCNN:
+    """The learned warm start for APRNumerics.richardson_lucy: four valid 3x3x3 convolutions
+    (1 -> 16 -> 16 -> 16 -> 1, ReLU) with a residual output, bf16 MFMA on the device."""
+
+    def __init__(self, ctx=None, seed=0, params=None):
+        self.ctx = ctx or default_context()
+        h = vp()
+        _check(lib().aprh_cnn_create(self.ctx.h, seed, C.byref(h)))
+        self.h = h
+        if params is not None:
+            self.set_params(params)
+
+    def get_params(self):
+        p = np.zeros(CNN_PARAMS, np.float32)
+        _check(lib().aprh_cnn_get_params(self.ctx.h, self.h, _ptr(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != CNN_PARAMS:
+            raise ValueError(f"expected {CNN_PARAMS} parameters")
+        _check(lib().aprh_cnn_set_params(self.ctx.h, self.h, _ptr(p)))
+
+    def save(self, path):
+        np.save(path, self.get_params(), allow_pickle=False)
+
+    @classmethod
+    def load(cls, path, ctx=None):
+        return cls(ctx, params=np.load(path, allow_pickle=False))
+
+    def forward_tiles(self, x, S):
+        x = np.ascontiguousarray(x, np.float32)
+        T = x.size // (S + 8) ** 3
+        y = np.zeros((T, S, S, S), np.float32)
+        _check(lib().aprh_cnn_forward_tiles(self.ctx.h, self.h, _ptr(x), T, S, _ptr(y)))
+        return y
+
+    def train_step(self, x, target, S, lr, grads=False, T=None):
+        """x: [T, S+8, S+8, S+8], target: [T, S, S, S] (host arrays or device addresses with T)"""
+        if isinstance(x, np.ndarray):
+            x = np.ascontiguousarray(x, np.float32)
+            target = np.ascontiguousarray(target, np.float32)
+            T = x.size // (S + 8) ** 3
+        loss = C.c_float()
+        g = np.zeros(CNN_PARAMS, np.float32) if grads else None
+        _check(lib().aprh_cnn_train_step(self.ctx.h, self.h, _ptr(x), _ptr(target), T, S, lr, C.byref(loss),
+                                         _ptr(g) if grads else None))
+        return (loss.value, g) if grads else loss.value
+
+    def apply(self, image, dims=None, scale=0.0, out=None, dtype=None):
+        """the network over a whole image [z, x, y] (or a device address with dims=(y, x, z))"""
+        code = _dtype_of(image, dtype)
+        if dims is None:
+            z, x, y = image.shape
+        else:
+            y, x, z = dims
+        if out is None:
+            out = np.zeros((z, x, y), np.float32)
+        _check(lib().aprh_cnn_apply(self.ctx.h, self.h, _ptr(image), code, y, x, z, scale, _ptr(out)))
+        return out
+
+    def warm_start(self, apr, parts, out=None, dtype=None):
+        """initial Richardson-Lucy estimate for `apr`'s particles (use with resume=True)"""
+        parts = _host_parts(parts)
+        code = _dtype_of(parts, dtype)
+        if out is None:
+            out = np.zeros(max(apr.total, 1), np.float32)
+        _check(lib().aprh_cnn_warm_start(self.ctx.h, self.h, apr.h, _ptr(parts), code, _ptr(out)))
+        return out[:apr.total] if isinstance(out, np.ndarray) else out
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().aprh_cnn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -211,6 +211,8 @@ void launch_levels_padded(aprh_context* ctx, const float* A, const LisParams& lp
 /* ---- pulling scheme / structure / sampling (k_structure.hip) ---- */
 void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, const aprh_params& par);
 void launch_sample_u16(aprh_context* ctx, const aprh_apr& a, const uint16_t* img, uint16_t* parts);
+/* ParticleData<float>::sample_image of a float image (same pyramid, float means) */
+void launch_sample_f32(aprh_context* ctx, const aprh_apr& a, const float* img, float* parts);
 
 /* exclusive scan of n u32 counts into u64 offsets; returns the total (synchronises the stream) */
 uint64_t exclusive_scan_u32(aprh_context* ctx, const uint32_t* in, uint64_t* out, int64_t n);

@@ -287,6 +287,18 @@ int aprh_sample_image_u16(aprh_context* ctx, const aprh_apr* a, const uint16_t* 
     APRH_CATCH
 }
 
+int aprh_sample_image_f32(aprh_context* ctx, const aprh_apr* a, const float* image, float* parts) {
+    APRH_TRY
+    check_ctx(ctx);
+    if (!a || !image || !parts) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    const size_t n = (size_t)(a->org[0] * a->org[1] * a->org[2]);
+    DevIn in(ctx, image, n * 4);
+    DevOut out(ctx, parts, std::max<uint64_t>(a->total, 1) * 4);
+    launch_sample_f32(ctx, *a, (const float*)in.ptr, (float*)out.ptr);
+    out.finish(ctx);
+    APRH_CATCH
+}
+
 /* ---------------- stage-level entry points ---------------- */
 
 int aprh_bspline_filt_rec_u16(aprh_context* ctx, uint16_t* image, int64_t y, int64_t x, int64_t z, int axis, float lambda,

@@ -279,7 +279,14 @@ __global__ __launch_bounds__(256) void k_seg_fill(const uint8_t* __restrict__ ma
 }
 
 /* mean pyramid level (downsamplePyramid with sum then x/8.0f, u16 store) */
-__global__ void k_pyr_down(const uint16_t* __restrict__ fine, Dims fd, uint16_t* coarse, Dims cd) {
+/* mean of the 2x2x2 block in the reference's summation order (PixelData.hpp:952-962), stored
+   as the pyramid's element type (u16: truncation like a float -> uint16 assignment) */
+template <typename T> __device__ __forceinline__ T pyr_store(float m);
+template <> __device__ __forceinline__ uint16_t pyr_store<uint16_t>(float m) { return (uint16_t)(int32_t)m; }
+template <> __device__ __forceinline__ float pyr_store<float>(float m) { return m; }
+
+template <typename T>
+__global__ void k_pyr_down(const T* __restrict__ fine, Dims fd, T* coarse, Dims cd) {
     const size_t n = (size_t)(cd.y * cd.x * cd.z);
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const int64_t y = (int64_t)(i % (size_t)cd.y);
@@ -296,12 +303,13 @@ __global__ void k_pyr_down(const uint16_t* __restrict__ fine, Dims fd, uint16_t*
         s = s + (float)fine[(z1 * fd.x + x1) * fd.y + y0];
         s = s + (float)fine[(z1 * fd.x + x1) * fd.y + y1];
         const float m = s / 8.0f;
-        coarse[i] = (uint16_t)(int32_t)m;
+        coarse[i] = pyr_store<T>(m);
     }
 }
 
+template <typename T>
 struct LevelPtrs {
-    const uint16_t* img[MAXL];
+    const T* img[MAXL];
     int64_t y[MAXL], x[MAXL];
     int64_t row_begin[MAXL];
     int64_t nrows[MAXL];
@@ -310,8 +318,9 @@ struct LevelPtrs {
 };
 
 /* one wave per row; lanes stride the row's particles */
-__global__ __launch_bounds__(256) void k_gather(LevelPtrs L, const uint64_t* __restrict__ xz_end, const uint16_t* __restrict__ y_vec,
-                                                uint16_t* parts, int64_t total_rows) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather(LevelPtrs<T> L, const uint64_t* __restrict__ xz_end, const uint16_t* __restrict__ y_vec,
+                                                T* parts, int64_t total_rows) {
     const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
     const int lane = threadIdx.x & 63;
     const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / 64;
@@ -322,7 +331,7 @@ __global__ __launch_bounds__(256) void k_gather(LevelPtrs L, const uint64_t* __r
         const int64_t xzi = L.row_begin[li] + row;
         const uint64_t b = xz_end[xzi - 1], e = xz_end[xzi];
         const int64_t z = row / L.x[li], x = row % L.x[li];
-        const uint16_t* plane = L.img[li] + (z * L.x[li] + x) * L.y[li];
+        const T* plane = L.img[li] + (z * L.x[li] + x) * L.y[li];
         for (uint64_t i = b + lane; i < e; i += 64) parts[i] = plane[y_vec[i]];
     }
 }
@@ -487,7 +496,8 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
     HIP_CHECK(hipStreamSynchronize(st));
 }
 
-void launch_sample_u16(aprh_context* ctx, const aprh_apr& a, const uint16_t* img, uint16_t* parts) {
+template <typename T>
+void launch_sample(aprh_context* ctx, const aprh_apr& a, const T* img, T* parts) {
     hipStream_t st = ctx->stream;
     const int lmax = a.l_max, lmin = a.l_min;
     /* pyramid levels lmin..lmax-1 in one buffer */
@@ -497,16 +507,16 @@ void launch_sample_u16(aprh_context* ctx, const aprh_apr& a, const uint16_t* img
         off[l] = tot;
         tot += a.y_num[l] * a.x_num[l] * a.z_num[l];
     }
-    Buf pyr(ctx->pool, (size_t)std::max<int64_t>(tot, 1) * 2);
-    uint16_t* Pyr = pyr.as<uint16_t>();
-    const uint16_t* fine = img;
+    Buf pyr(ctx->pool, (size_t)std::max<int64_t>(tot, 1) * sizeof(T));
+    T* Pyr = pyr.as<T>();
+    const T* fine = img;
     for (int l = lmax - 1; l >= lmin; --l) {
-        hipLaunchKernelGGL(k_pyr_down, dim3(4096), dim3(256), 0, st, fine, Dims{a.y_num[l + 1], a.x_num[l + 1], a.z_num[l + 1]},
+        hipLaunchKernelGGL(k_pyr_down<T>, dim3(4096), dim3(256), 0, st, fine, Dims{a.y_num[l + 1], a.x_num[l + 1], a.z_num[l + 1]},
                            Pyr + off[l], Dims{a.y_num[l], a.x_num[l], a.z_num[l]});
         HIP_LAUNCH_CHECK();
         fine = Pyr + off[l];
     }
-    LevelPtrs L;
+    LevelPtrs<T> L;
     std::memset(&L, 0, sizeof(L));
     L.n_levels = lmax - lmin + 1;
     int64_t rp = 0;
@@ -522,10 +532,13 @@ void launch_sample_u16(aprh_context* ctx, const aprh_apr& a, const uint16_t* img
     }
     L.rows_prefix[L.n_levels] = rp;
     if (a.total > 0) {
-        hipLaunchKernelGGL(k_gather, dim3(8192), dim3(256), 0, st, L, a.xz_end.as<uint64_t>(), a.y_vec.as<uint16_t>(), parts, rp);
+        hipLaunchKernelGGL(k_gather<T>, dim3(8192), dim3(256), 0, st, L, a.xz_end.as<uint64_t>(), a.y_vec.as<uint16_t>(), parts, rp);
         HIP_LAUNCH_CHECK();
     }
     HIP_CHECK(hipStreamSynchronize(st));
 }
+
+void launch_sample_u16(aprh_context* ctx, const aprh_apr& a, const uint16_t* img, uint16_t* parts) { launch_sample<uint16_t>(ctx, a, img, parts); }
+void launch_sample_f32(aprh_context* ctx, const aprh_apr& a, const float* img, float* parts) { launch_sample<float>(ctx, a, img, parts); }
 
 }  // namespace aprh

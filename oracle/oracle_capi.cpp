@@ -19,6 +19,7 @@
 #include <exception>
 
 #include "apr_oracle.hpp"
+#include "cnn_oracle.hpp"
 
 using namespace oracle;
 
@@ -246,6 +247,14 @@ void orc_sample_image_u16(void* hp, const uint16_t* img, uint16_t* parts) {
     std::memcpy(parts, p.data(), p.size() * 2);
 }
 
+void orc_sample_image_f32(void* hp, const float* img, float* parts) {
+    Handle* h = (Handle*)hp;
+    Mesh<float> m = wrap(img, h->dims[0], h->dims[1], h->dims[2]);
+    std::vector<float> p;
+    sample_image(h->r.info, h->r.lin, m, p);
+    std::memcpy(parts, p.data(), p.size() * 4);
+}
+
 void orc_apr_free(void* hp) { delete (Handle*)hp; }
 
 /* APR from given GenInfo dims + LinearAccess arrays (host copies of another build) */
@@ -379,6 +388,36 @@ void orc_gaussian_stencil(const float* sigma, const int* size, int normalize, fl
     gaussian_stencil(sigma, size, normalize != 0, m);
     std::copy(m.v.begin(), m.v.end(), out);
 }
+
+/* warm-start network: forward of T tiles x[T][(S+8)^3] -> y[T][S^3] */
+void orc_cnn_forward(const float* params, const float* x, int T, int S, int emulate_bf16, float* y) {
+    cnn_oracle::Net n(params, emulate_bf16 != 0);
+    const int64_t in = (int64_t)(S + 8) * (S + 8) * (S + 8), out = (int64_t)S * S * S;
+#pragma omp parallel for schedule(dynamic)
+    for (int t = 0; t < T; ++t) {
+        cnn_oracle::Acts A;
+        cnn_oracle::forward(n, x + t * in, S, A);
+        for (int64_t i = 0; i < out; ++i) y[t * out + i] = (float)A.y[(size_t)i];
+    }
+}
+
+/* mean squared error over the batch and its gradient (N_PARAMS floats) */
+double orc_cnn_grad(const float* params, const float* x, const float* tgt, int T, int S, int emulate_bf16, double* grad) {
+    cnn_oracle::Net n(params, emulate_bf16 != 0);
+    const int64_t in = (int64_t)(S + 8) * (S + 8) * (S + 8), out = (int64_t)S * S * S;
+    const double inv_n = 1.0 / (double)(out * T);
+    std::vector<double> g(cnn_oracle::N_PARAMS, 0.0);
+    double loss = 0;
+    for (int t = 0; t < T; ++t) {
+        cnn_oracle::Acts A;
+        cnn_oracle::forward(n, x + t * in, S, A);
+        loss += cnn_oracle::backward(n, x + t * in, tgt + t * out, A, inv_n, g);
+    }
+    std::copy(g.begin(), g.end(), grad);
+    return loss;
+}
+
+int orc_cnn_param_count(void) { return cnn_oracle::N_PARAMS; }
 
 int orc_num_threads(void) {
 #ifdef _OPENMP

@@ -84,6 +84,13 @@ def _bind(lib):
                                     C.c_double, C.POINTER(C.c_double)]
         getattr(lib, n).restype = C.c_int
     lib.orc_gaussian_stencil.argtypes = [f32p, i32p, C.c_int, f32p]
+    if hasattr(lib, "orc_sample_image_f32"):
+        lib.orc_sample_image_f32.argtypes = [C.c_void_p, f32p, f32p]
+    if hasattr(lib, "orc_cnn_forward"):
+        f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        lib.orc_cnn_forward.argtypes = [f32p, f32p, C.c_int, C.c_int, C.c_int, f32p]
+        lib.orc_cnn_grad.argtypes = [f32p, f32p, f32p, C.c_int, C.c_int, C.c_int, f64p]
+        lib.orc_cnn_grad.restype = C.c_double
     if hasattr(lib, "orc_reconstruct"):
         lib.orc_reconstruct.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int]
     return lib
@@ -183,6 +190,11 @@ class OracleAPR:
         it = f(self.h, p, u, np.ascontiguousarray(psf, np.float32).reshape(-1), y, x, z, niter, int(use_ds), int(normalize),
                int(resume), tol, C.byref(rel))
         return u[:self.total], it, rel.value
+
+    def sample_f32(self, img):
+        parts = np.zeros(max(self.total, 1), np.float32)
+        self.lib.orc_sample_image_f32(self.h, np.ascontiguousarray(img, np.float32).reshape(-1), parts)
+        return parts[:self.total]
 
     def reconstruct(self, parts, level_delta=0, out_dtype=None):
         """piecewise-constant image [z, x, y] at level l_max + level_delta"""
