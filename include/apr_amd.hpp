@@ -154,17 +154,22 @@ struct ParticleData {
     T& operator[](size_t i) { return data[i]; }
     const T& operator[](size_t i) const { return data[i]; }
     void fill(T v) { std::fill(data.begin(), data.end(), v); }
-    /* ParticleData::sample_image (ParticleData.hpp:122-127) */
-    void sample_image(APR& apr, const PixelData<uint16_t>& img) {
-        static_assert(std::is_same<T, uint16_t>::value, "sample_image of a uint16 image into uint16 particles");
+    /* ParticleData::sample_image (ParticleData.hpp:122-127): uint16 -> uint16, float -> float */
+    void sample_image(APR& apr, const PixelData<T>& img) {
         init(apr.total_number_particles());
-        detail::check(aprh_sample_image_u16(apr.context().get(), apr.get(), img.mesh.data(), data.empty() ? nullptr : data.data()));
+        if (std::is_same<T, float>::value)
+            detail::check(aprh_sample_image_f32(apr.context().get(), apr.get(), (const float*)img.mesh.data(),
+                                                data.empty() ? nullptr : (float*)data.data()));
+        else
+            detail::check(aprh_sample_image_u16(apr.context().get(), apr.get(), (const uint16_t*)img.mesh.data(),
+                                                data.empty() ? nullptr : (uint16_t*)data.data()));
     }
 };
 
 template <typename ImageType>
 class APRConverter {
-    static_assert(std::is_same<ImageType, uint16_t>::value, "this build converts uint16 images");
+    static_assert(std::is_same<ImageType, uint16_t>::value || std::is_same<ImageType, float>::value,
+                  "this build converts uint16 and float images");
 
 public:
     APRParameters par;
@@ -172,7 +177,11 @@ public:
     /* APRConverter::get_apr (APRConverter.hpp:591-600): false on bad dimensions */
     bool get_apr(APR& apr, const PixelData<ImageType>& img) {
         aprh_apr* h = nullptr;
-        const int rc = aprh_get_apr_u16(ctx_->get(), img.mesh.data(), img.y_num, img.x_num, img.z_num, &par, &h);
+        int rc;
+        if (std::is_same<ImageType, float>::value)
+            rc = aprh_get_apr_f32(ctx_->get(), (const float*)img.mesh.data(), img.y_num, img.x_num, img.z_num, &par, &h);
+        else
+            rc = aprh_get_apr_u16(ctx_->get(), (const uint16_t*)img.mesh.data(), img.y_num, img.x_num, img.z_num, &par, &h);
         if (rc == APRH_ERR_BAD_DIMS) return false;
         detail::check(rc);
         apr.adopt(h, ctx_);

@@ -46,6 +46,7 @@ typedef struct aprh_apr_info {
     uint64_t xz_end_size;           /* elements of xz_end_vec                        */
     uint64_t level_xz_size;         /* elements of level_xz_vec (= l_max + 2)        */
     float bspline_offset;
+    int32_t image_dtype;            /* APRH_DTYPE_U16 or APRH_DTYPE_F32: the converter's ImageType */
 } aprh_apr_info;
 
 const char* aprh_last_error(void);
@@ -80,6 +81,10 @@ int aprh_context_release_cache(aprh_context* ctx);
 /* APRConverter<uint16_t>::get_apr (src/algorithm/APRConverter.hpp:591-600, get_apr_cpu :534-582) */
 int aprh_get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y_num, int64_t x_num, int64_t z_num,
                      const aprh_params* par, aprh_apr** out);
+/* APRConverter<float>::get_apr (APRConverter.hpp:534-600 with ImageType = float): no B-spline
+   offset (:218-220), float smoothing and gradient; same structure semantics */
+int aprh_get_apr_f32(aprh_context* ctx, const float* image, int64_t y_num, int64_t x_num, int64_t z_num, const aprh_params* par,
+                     aprh_apr** out);
 /* APRConverter::get_apr_custom_grad_scale (APRConverter.hpp:169-191): structure from a given
    downsampled gradient (u16) and local intensity scale (f32); intensity (f32, downsampled) is
    the Ip_th reference that get_apr would take from local_scale_temp2 */
@@ -93,7 +98,8 @@ int aprh_apr_copy_structure(const aprh_apr* apr, uint64_t* level_xz_vec, uint64_
 int aprh_apr_device_structure(const aprh_apr* apr, const uint64_t** xz_end_vec, const uint16_t** y_vec);
 /* kept stage outputs, see aprh_context_set_keep_intermediates.  which:
    0 smoothed u16[N], 1 gradient_step u16[Nd], 2 local_intensity_scale_step f32[Nd],
-   3 thresholded gradient u16[Nd], 4 rescaled scale f32[Nd], 5 levels f32[Nd]  (Nd = ds volume) */
+   3 thresholded gradient u16[Nd], 4 rescaled scale f32[Nd], 5 levels f32[Nd]  (Nd = ds volume);
+   0, 1 and 3 are f32 for an APR from aprh_get_apr_f32 */
 int aprh_apr_intermediate(const aprh_apr* apr, int which, void* dst);
 /* final particle-cell tree of pulling-scheme level l (PullingScheme.hpp:83), u8 */
 int aprh_apr_particle_cell_tree(const aprh_apr* apr, int level, uint8_t* dst, int64_t* dims3);

@@ -54,7 +54,7 @@ class AprhParams(C.Structure):
 class AprhAprInfo(C.Structure):
     _fields_ = [("l_min", C.c_int32), ("l_max", C.c_int32), ("org_dims", C.c_int64 * 3),
                 ("total_number_particles", C.c_uint64), ("xz_end_size", C.c_uint64), ("level_xz_size", C.c_uint64),
-                ("bspline_offset", C.c_float)]
+                ("bspline_offset", C.c_float), ("image_dtype", C.c_int32)]
 
 
 class AprhTreeInfo(C.Structure):
@@ -95,6 +95,7 @@ def lib():
     L.aprh_stage_name.argtypes = [C.c_int]
     L.aprh_context_release_cache.argtypes = [vp]
     L.aprh_get_apr_u16.argtypes = [vp, vp, i64, i64, i64, C.POINTER(AprhParams), C.POINTER(vp)]
+    L.aprh_get_apr_f32.argtypes = [vp, vp, i64, i64, i64, C.POINTER(AprhParams), C.POINTER(vp)]
     L.aprh_apr_from_grad_lis.argtypes = [vp, vp, vp, vp, i64, i64, i64, C.POINTER(AprhParams), C.c_float, C.POINTER(vp)]
     L.aprh_apr_info_get.argtypes = [vp, C.POINTER(AprhAprInfo)]
     L.aprh_apr_copy_structure.argtypes = [vp, vp, vp, vp]
@@ -260,6 +261,7 @@ class APR:
         self.xz_end_size = info.xz_end_size
         self.level_xz_size = info.level_xz_size
         self.bspline_offset = info.bspline_offset
+        self.image_dtype = np.float32 if info.image_dtype == 1 else np.uint16
         self._struct = None
 
     def total_number_particles(self):
@@ -292,6 +294,8 @@ class APR:
         shapes = {0: ((z, x, y), np.uint16), 1: ((zd, xd, yd), np.uint16), 2: ((zd, xd, yd), np.float32),
                   3: ((zd, xd, yd), np.uint16), 4: ((zd, xd, yd), np.float32), 5: ((zd, xd, yd), np.float32)}
         shp, dt = shapes[which]
+        if which in (0, 1, 3):
+            dt = self.image_dtype
         out = np.zeros(shp, dt)
         _check(lib().aprh_apr_intermediate(self.h, which, _ptr(out)))
         return out
@@ -349,23 +353,27 @@ class APR:
 
 
 class APRConverter:
-    """APRConverter<uint16_t> (APRConverter.hpp:34-134)."""
+    """APRConverter<ImageType> (APRConverter.hpp:34-134) for ImageType uint16 (default) or float32."""
 
-    def __init__(self, ctx=None):
+    def __init__(self, ctx=None, image_type=np.uint16):
         self.par = APRParameters()
         self.ctx = ctx or default_context()
+        self.image_type = np.dtype(image_type)
+        if self.image_type not in (np.dtype(np.uint16), np.dtype(np.float32)):
+            raise TypeError("APRConverter supports uint16 and float32 images")
 
     def get_apr(self, image, dims=None):
-        """image: numpy uint16 [z, x, y] or a device address with dims=(y, x, z)"""
+        """image: numpy [z, x, y] of the converter's image type, or a device address with dims=(y, x, z)"""
         if dims is None:
             z, x, y = image.shape
-            if image.dtype != np.uint16:
-                raise TypeError("APRConverter<uint16_t> needs a uint16 image")
+            if image.dtype != self.image_type:
+                raise TypeError(f"APRConverter<{self.image_type}> needs a {self.image_type} image")
         else:
             y, x, z = dims
         h = vp()
         p = self.par.c()
-        _check(lib().aprh_get_apr_u16(self.ctx.h, _ptr(image), y, x, z, C.byref(p), C.byref(h)))
+        f = lib().aprh_get_apr_f32 if self.image_type == np.float32 else lib().aprh_get_apr_u16
+        _check(f(self.ctx.h, _ptr(image), y, x, z, C.byref(p), C.byref(h)))
         return APR(self.ctx, h)
 
     def get_apr_custom_grad_scale(self, grad_ds, lis_ds, intensity_ds, dims, bspline_offset=0.0):
@@ -386,6 +394,8 @@ class ParticleData:
         self.data = None
 
     def sample_image(self, apr, image, out=None, dtype=None):
+        if dtype is None and isinstance(image, (int, np.integer)):
+            dtype = "uint16"           # a raw device address: the uint16 image of APRConverter<uint16_t>
         code = _dtype_of(image, dtype)
         if out is None:
             out = np.zeros(max(apr.total, 1), np.uint16 if code == APRH_DTYPE_U16 else np.float32)
@@ -743,5 +753,44 @@ class WarmStartCNN:
     def __del__(self):
         try:
             self.close()
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """device memory from the context's allocator (aprh_malloc), freed with the object"""
+
+    def __init__(self, nbytes, ctx=None):
+        self.ctx = ctx or default_context()
+        p = vp()
+        _check(lib().aprh_malloc(self.ctx.h, max(int(nbytes), 1), C.byref(p)))
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+
+    def __int__(self):
+        return self.ptr
+
+    def data_ptr(self):
+        return self.ptr
+
+    def to_host(self, dtype, count=None):
+        dt = np.dtype(dtype)
+        count = self.nbytes // dt.itemsize if count is None else count
+        h = np.empty(count, dt)
+        _check(lib().aprh_memcpy(self.ctx.h, C.c_void_p(h.ctypes.data), C.c_void_p(self.ptr), h.nbytes))
+        return h
+
+    def from_host(self, arr):
+        arr = np.ascontiguousarray(arr)
+        _check(lib().aprh_memcpy(self.ctx.h, C.c_void_p(self.ptr), C.c_void_p(arr.ctypes.data), arr.nbytes))
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().aprh_free(self.ctx.h, C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
         except Exception:
             pass

@@ -135,6 +135,7 @@ struct aprh_apr {
     uint64_t total = 0;
     uint64_t n_xz = 0;                             /* size of xz_end_vec        */
     float bspline_offset = 0;
+    int image_dtype = 0;                           /* APRH_DTYPE_U16 / _F32 of the converted image */
     aprh::Buf xz_end;                              /* u64[n_xz]                 */
     aprh::Buf y_vec;                               /* u16[total]                */
     /* optional stage outputs (context keep_intermediates) */
@@ -204,9 +205,13 @@ void launch_u8_to_f32(aprh_context* ctx, const uint8_t* in, float* out, size_t n
 void launch_sat_mean(aprh_context* ctx, float* m, int64_t y, int64_t x, int64_t z, int axis, int off);
 /* rescale + unpad of A, applyParameters and computeLevels (level map u8);
    optional outputs: sigma_raw (local_intensity_scale_step), sigma_out (after sigma_th), grad_keep */
+template <typename G>
 void launch_levels_padded(aprh_context* ctx, const float* A, const LisParams& lp, int64_t yd, int64_t xd, int64_t zd,
-                          const float* inten, uint16_t* grad, float* sigma_raw, float* sigma_out, uint8_t* level,
-                          const aprh_params& par, const float* d_offset, float mult, uint16_t* grad_keep);
+                          const float* inten, G* grad, float* sigma_raw, float* sigma_out, uint8_t* level,
+                          const aprh_params& par, const float* d_offset, float mult, G* grad_keep);
+/* gradient magnitude + 2x2x2 max + mean downsample of a float image (APRConverter<float>) */
+void launch_grad_ds_f32(aprh_context* ctx, const float* img, float* grad, float* mean, int64_t y, int64_t x, int64_t z, float hx,
+                        float hy, float hz, bool accum);
 
 /* ---- pulling scheme / structure / sampling (k_structure.hip) ---- */
 void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, const aprh_params& par);

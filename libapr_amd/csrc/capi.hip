@@ -31,25 +31,26 @@ void keep_copy(aprh_context* ctx, Buf& dst, const void* src, size_t bytes) {
 
 /* levels + pulling scheme + structure from a downsampled gradient, an unpadded
    scale (A with zero pads, var_rescale 1) or the padded LIS array */
+template <typename G>
 void solve(aprh_context* ctx, aprh_apr& a, const aprh_params& par, const float* A, const LisParams& lp, const float* inten,
-           uint16_t* grad, const float* d_off) {
+           G* grad, const float* d_off) {
     const int64_t yd = cdiv(a.org[0], 2), xd = cdiv(a.org[1], 2), zd = cdiv(a.org[2], 2);
     const size_t nd = (size_t)(yd * xd * zd);
     Buf level(ctx->pool, nd);
     const bool keep = ctx->keep_intermediates != 0;
     float* sig_raw = nullptr;
     float* sig = nullptr;
-    uint16_t* gk = nullptr;
+    G* gk = nullptr;
     if (keep) {
         a.lis_raw.reset(ctx->pool, nd * 4);
         a.lis.reset(ctx->pool, nd * 4);
-        a.grad.reset(ctx->pool, nd * 2);
+        a.grad.reset(ctx->pool, nd * sizeof(G));
         sig_raw = a.lis_raw.as<float>();
         sig = a.lis.as<float>();
-        gk = a.grad.as<uint16_t>();
+        gk = a.grad.as<G>();
     }
-    launch_levels_padded(ctx, A, lp, yd, xd, zd, inten, grad, sig_raw, sig, level.as<uint8_t>(), par, d_off, level_mult(par, a.l_max),
-                         gk);
+    launch_levels_padded<G>(ctx, A, lp, yd, xd, zd, inten, grad, sig_raw, sig, level.as<uint8_t>(), par, d_off,
+                            level_mult(par, a.l_max), gk);
     if (keep) {
         /* levels as float, like local_scale_temp after computeLevels */
         a.levels.reset(ctx->pool, nd * 4);
@@ -57,6 +58,9 @@ void solve(aprh_context* ctx, aprh_apr& a, const aprh_params& par, const float* 
     }
     build_structure(ctx, a, level.as<uint8_t>(), par);
 }
+
+template <typename G>
+void lis_and_solve(aprh_context* ctx, aprh_apr* a, const aprh_params& par, Buf& mean, G* grad, float* d_off);
 
 void get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x, int64_t z, const aprh_params& par, aprh_apr* a) {
     reset_times(ctx, ST_OFFSET, ST_SAMPLE);
@@ -111,6 +115,15 @@ void get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x,
             throw Error(APRH_ERR_OVERFLOW, std::string("integer under-/overflow encountered in ComputeGradient::bspline_filt_rec_") +
                                                axis_name[i] +
                                                " - try squashing the input image to a narrower range or use APRConverter<float>");
+    lis_and_solve<uint16_t>(ctx, a, par, mean, grad.as<uint16_t>(), d_off);
+}
+
+/* local intensity scale, applyParameters, solveForAPR, generateDatastructures */
+template <typename G>
+void lis_and_solve(aprh_context* ctx, aprh_apr* a, const aprh_params& par, Buf& mean, G* grad, float* d_off) {
+    hipStream_t st = ctx->stream;
+    const int64_t yd = cdiv(a->org[0], 2), xd = cdiv(a->org[1], 2), zd = cdiv(a->org[2], 2);
+    const size_t nd = (size_t)(yd * xd * zd);
     mark(ctx, ST_LIS);
     /* local intensity scale (LocalIntensityScale.hpp:26-150) */
     LisParams lp = lis_params(par, yd, xd, zd);
@@ -132,11 +145,56 @@ void get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x,
     }
     /* applyParameters + solveForAPR + generateDatastructures */
     mark(ctx, ST_LEVELS);
-    solve(ctx, *a, par, Ap, lpa, mean.as<float>(), grad.as<uint16_t>(), d_off);
+    solve<G>(ctx, *a, par, Ap, lpa, mean.as<float>(), grad, d_off);
     mark(ctx, ST_SAMPLE);
     HIP_CHECK(hipMemcpyAsync(&a->bspline_offset, d_off, 4, hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
     collect_times(ctx, ST_OFFSET, ST_SAMPLE);
+}
+
+/* APRConverter<float>::get_apr: no B-spline offset (APRConverter.hpp:218-220), float
+   smoothing without rounding, float gradient (grad_temp is PixelData<float>) */
+void get_apr_f32(aprh_context* ctx, const float* image, int64_t y, int64_t x, int64_t z, const aprh_params& par, aprh_apr* a) {
+    reset_times(ctx, ST_OFFSET, ST_SAMPLE);
+    const size_t n = (size_t)(y * x * z);
+    const int64_t yd = cdiv(y, 2), xd = cdiv(x, 2), zd = cdiv(z, 2);
+    const size_t nd = (size_t)(yd * xd * zd);
+    hipStream_t st = ctx->stream;
+    DevIn in(ctx, image, n * 4);
+    geninfo_init(*a, y, x, z);
+    a->image_dtype = APRH_DTYPE_F32;
+    a->bspline_offset = 0;
+    mark(ctx, ST_OFFSET);
+    float* d_off = (float*)ctx->d_flags;
+    int* d_err = ctx->d_flags + 4;
+    HIP_CHECK(hipMemsetAsync(ctx->d_flags, 0, 8 * sizeof(int), st));   /* offset 0.0f, error counters 0 */
+    mark(ctx, ST_SMOOTH_Y);
+    Buf img(ctx->pool, n * 4);
+    float* I = img.as<float>();
+    const float tol = 0.0001f;
+    if (par.lambda > 0 && y > 2)
+        launch_iir_y<float>(ctx, (const float*)in.ptr, I, y, x, z, iir_coeffs(par.lambda, tol, y), nullptr, d_err + 0);
+    else
+        HIP_CHECK(hipMemcpyAsync(I, in.ptr, n * 4, hipMemcpyDeviceToDevice, st));
+    mark(ctx, ST_SMOOTH_X);
+    if (par.lambda > 0 && x > 2) launch_iir_outer<float>(ctx, I, y, x, z, 1, iir_coeffs(par.lambda, tol, x), d_err + 1);
+    mark(ctx, ST_SMOOTH_Z);
+    if (par.lambda > 0 && z > 2) launch_iir_outer<float>(ctx, I, y, x, z, 2, iir_coeffs(par.lambda, tol, z), d_err + 2);
+    mark(ctx, ST_GRADIENT);
+    Buf grad(ctx->pool, nd * 4);
+    Buf mean(ctx->pool, nd * 4);
+    launch_grad_ds_f32(ctx, I, grad.as<float>(), mean.as<float>(), y, x, z, par.dx, par.dy, par.dz, false);
+    if (ctx->keep_intermediates) {
+        keep_copy(ctx, a->smoothed, I, n * 4);
+        keep_copy(ctx, a->grad_raw, grad.ptr(), nd * 4);
+    }
+    img.release();
+    mark(ctx, ST_INV_BSPLINE);
+    if (par.lambda > 0) {
+        Buf tmp(ctx->pool, nd * 4);
+        launch_inv_bspline(ctx, mean.as<float>(), tmp.as<float>(), yd, xd, zd, y > 2, x > 2, z > 2);
+    }
+    lis_and_solve<float>(ctx, a, par, mean, grad.as<float>(), d_off);
 }
 
 }  // namespace
@@ -153,6 +211,24 @@ int aprh_get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_
     a->ctx = ctx;
     try {
         get_apr_u16(ctx, image, y, x, z, *par, a);
+    } catch (...) {
+        (void)hipStreamSynchronize(ctx->stream);
+        delete a;
+        throw;
+    }
+    *out = a;
+    APRH_CATCH
+}
+
+int aprh_get_apr_f32(aprh_context* ctx, const float* image, int64_t y, int64_t x, int64_t z, const aprh_params* par, aprh_apr** out) {
+    APRH_TRY
+    check_ctx(ctx);
+    check_dims(y, x, z);
+    if (!image || !par || !out) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    aprh_apr* a = new aprh_apr();
+    a->ctx = ctx;
+    try {
+        get_apr_f32(ctx, image, y, x, z, *par, a);
     } catch (...) {
         (void)hipStreamSynchronize(ctx->stream);
         delete a;
@@ -202,6 +278,7 @@ int aprh_apr_info_get(const aprh_apr* a, aprh_apr_info* info) {
     info->xz_end_size = a->n_xz;
     info->level_xz_size = a->level_xz_vec.size();
     info->bspline_offset = a->bspline_offset;
+    info->image_dtype = a->image_dtype;
     APRH_CATCH
 }
 

@@ -40,7 +40,19 @@ __device__ __forceinline__ double sq_term(int d, float h2) {
     return q * q;
 }
 
-__global__ __launch_bounds__(256) void k_grad_ds(const uint16_t* __restrict__ img, uint16_t* grad, float* mean, int64_t y_num,
+/* difference of two samples as the reference computes it: int for uint16 (promoted, then
+   converted to float for the division), float for float images */
+__device__ __forceinline__ float gdiff(uint16_t a, uint16_t b) { return (float)((int)a - (int)b); }
+__device__ __forceinline__ float gdiff(float a, float b) { return a - b; }
+__device__ __forceinline__ double gterm(float d, float h2) {
+    const double q = (double)(d / h2);
+    return q * q;
+}
+__device__ __forceinline__ void gstore(double v, uint16_t& out) { out = d2u16(v); }
+__device__ __forceinline__ void gstore(double v, float& out) { out = (float)v; }
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_grad_ds(const T* __restrict__ img, T* grad, float* mean, int64_t y_num,
                                                  int64_t x_num, int64_t z_num, int64_t yd_num, float h2x, float h2y, float h2z,
                                                  int accum) {
     const int64_t yd = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -48,30 +60,29 @@ __global__ __launch_bounds__(256) void k_grad_ds(const uint16_t* __restrict__ im
     const int64_t xd = blockIdx.y, zd = blockIdx.z;
     const int64_t xd_num = gridDim.y;
     const int64_t xy = x_num * y_num;
-    uint16_t g = accum ? grad[(zd * xd_num + xd) * yd_num + yd] : (uint16_t)0;
+    T g = accum ? grad[(zd * xd_num + xd) * yd_num + yd] : (T)0;
     const int64_t y0 = 2 * yd, y1 = min(2 * yd + 1, y_num - 1);
     for (int64_t z = 2 * zd; z <= min(2 * zd + 1, z_num - 1); ++z) {
         const int64_t zm = z > 0 ? z - 1 : 0, zp = min(z + 1, z_num - 1);
         for (int64_t x = 2 * xd; x <= min(2 * xd + 1, x_num - 1); ++x) {
             const int64_t xm = x > 0 ? x - 1 : 0, xp = min(x + 1, x_num - 1);
-            const uint16_t* cen = img + z * xy + x * y_num;
-            const uint16_t* left = img + z * xy + xm * y_num;
-            const uint16_t* right = img + z * xy + xp * y_num;
-            const uint16_t* up = img + zm * xy + x * y_num;
-            const uint16_t* down = img + zp * xy + x * y_num;
+            const T* cen = img + z * xy + x * y_num;
+            const T* left = img + z * xy + xm * y_num;
+            const T* right = img + z * xy + xp * y_num;
+            const T* up = img + zm * xy + x * y_num;
+            const T* down = img + zp * xy + x * y_num;
             for (int w = 0; w < 2; ++w) {
                 const int64_t y = w ? y1 : y0;
-                uint16_t t;
+                T t;
                 if (y_num >= 2) {
-                    int dyv;
-                    if (y == 0) dyv = (int)cen[1] - (int)cen[0];
-                    else if (y == y_num - 1) dyv = (int)cen[y_num - 1] - (int)cen[y_num - 2];
-                    else dyv = (int)cen[y + 1] - (int)cen[y - 1];
-                    const double s = sq_term((int)right[y] - (int)left[y], h2x) + sq_term((int)down[y] - (int)up[y], h2z) +
-                                     sq_term(dyv, h2y);
-                    t = d2u16(sqrt(s));
+                    float dyv;
+                    if (y == 0) dyv = gdiff(cen[1], cen[0]);
+                    else if (y == y_num - 1) dyv = gdiff(cen[y_num - 1], cen[y_num - 2]);
+                    else dyv = gdiff(cen[y + 1], cen[y - 1]);
+                    const double s = gterm(gdiff(right[y], left[y]), h2x) + gterm(gdiff(down[y], up[y]), h2z) + gterm(dyv, h2y);
+                    gstore(sqrt(s), t);
                 } else {
-                    t = 0;
+                    t = (T)0;
                 }
                 g = t > g ? t : g;
             }
@@ -422,9 +433,10 @@ void mean_z(aprh_context* ctx, int O, float* A, int64_t y, int64_t x, int64_t z)
 }
 
 /* applyParameters + computeLevels, one lane per downsampled cell */
+template <typename G>
 __global__ void k_levels(const float* __restrict__ A, PadGeom g, float var_rescale, const float* __restrict__ inten,
-                         uint16_t* grad, float* sigma_raw, float* sigma_out, uint8_t* level, const float* d_off, float Ip_th,
-                         float sigma_th, float sigma_th_max, float grad_th, float mult, uint16_t* grad_keep) {
+                         G* grad, float* sigma_raw, float* sigma_out, uint8_t* level, const float* d_off, float Ip_th,
+                         float sigma_th, float sigma_th_max, float grad_th, float mult, G* grad_keep) {
     const float ipth = Ip_th + *d_off;
     const size_t n = (size_t)(g.yd * g.xd * g.zd);
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
@@ -434,11 +446,11 @@ __global__ void k_levels(const float* __restrict__ A, PadGeom g, float var_resca
         /* rescale_var + unpad (LocalIntensityScale.hpp:173-184, PixelData.hpp:1189-1210) */
         float s = A[((z + g.pz) * g.xp + (x + g.px)) * g.yp + (y + g.py)] * var_rescale;
         if (sigma_raw) sigma_raw[i] = s;
-        uint16_t gv = grad[i];
+        G gv = grad[i];
         /* threshold_gradient (ComputeGradient.hpp:159-171) */
-        if (inten[i] <= ipth) gv = 0;
+        if (inten[i] <= ipth) gv = (G)0;
         if (s < sigma_th) s = (s < sigma_th_max) ? 60000.0f : sigma_th;
-        if ((float)gv < grad_th) gv = 0;
+        if ((float)gv < grad_th) gv = (G)0;
         if (sigma_out) sigma_out[i] = s;
         if (grad_keep) grad_keep[i] = gv;
         const float L = (float)gv / s;
@@ -501,7 +513,7 @@ void launch_grad_ds_u16(aprh_context* ctx, const uint16_t* img, uint16_t* grad, 
             hipLaunchKernelGGL(k_grad_ds_vec<false>, grid, dim3(256), 0, ctx->stream, img, grad, mean, y, x, z, yd, 2.0f * hx,
                                2.0f * hy, 2.0f * hz, accum ? 1 : 0);
     } else {
-        hipLaunchKernelGGL(k_grad_ds, dim3((unsigned)cdiv(yd, 256), (unsigned)xd, (unsigned)zd), dim3(256), 0, ctx->stream, img, grad,
+        hipLaunchKernelGGL(k_grad_ds<uint16_t>, dim3((unsigned)cdiv(yd, 256), (unsigned)xd, (unsigned)zd), dim3(256), 0, ctx->stream, img, grad,
                            mean, y, x, z, yd, 2.0f * hx, 2.0f * hy, 2.0f * hz, accum ? 1 : 0);
     }
     HIP_LAUNCH_CHECK();
@@ -587,12 +599,26 @@ void launch_lis(aprh_context* ctx, const float* intensity, float* sigma_padded, 
     if (lp.active[2]) mean_z(ctx, lp.win[5], A, g.yp, g.xp, g.zp);
 }
 
+template <typename G>
 void launch_levels_padded(aprh_context* ctx, const float* A, const LisParams& lp, int64_t yd, int64_t xd, int64_t zd,
-                          const float* inten, uint16_t* grad, float* sigma_raw, float* sigma_out, uint8_t* level,
-                          const aprh_params& par, const float* d_offset, float mult, uint16_t* grad_keep) {
+                          const float* inten, G* grad, float* sigma_raw, float* sigma_out, uint8_t* level,
+                          const aprh_params& par, const float* d_offset, float mult, G* grad_keep) {
     PadGeom g = pad_geom(yd, xd, zd, lp);
-    hipLaunchKernelGGL(k_levels, dim3(4096), dim3(256), 0, ctx->stream, A, g, lp.var_rescale, inten, grad, sigma_raw, sigma_out, level,
+    hipLaunchKernelGGL(k_levels<G>, dim3(4096), dim3(256), 0, ctx->stream, A, g, lp.var_rescale, inten, grad, sigma_raw, sigma_out, level,
                        d_offset, par.Ip_th, par.sigma_th, par.sigma_th_max, par.grad_th, mult, grad_keep);
+    HIP_LAUNCH_CHECK();
+}
+
+template void launch_levels_padded<uint16_t>(aprh_context*, const float*, const LisParams&, int64_t, int64_t, int64_t, const float*,
+                                             uint16_t*, float*, float*, uint8_t*, const aprh_params&, const float*, float, uint16_t*);
+template void launch_levels_padded<float>(aprh_context*, const float*, const LisParams&, int64_t, int64_t, int64_t, const float*, float*,
+                                          float*, float*, uint8_t*, const aprh_params&, const float*, float, float*);
+
+void launch_grad_ds_f32(aprh_context* ctx, const float* img, float* grad, float* mean, int64_t y, int64_t x, int64_t z, float hx,
+                        float hy, float hz, bool accum) {
+    const int64_t yd = cdiv(y, 2), xd = cdiv(x, 2), zd = cdiv(z, 2);
+    hipLaunchKernelGGL(k_grad_ds<float>, dim3((unsigned)cdiv(yd, 256), (unsigned)xd, (unsigned)zd), dim3(256), 0, ctx->stream, img, grad,
+                       mean, y, x, z, yd, 2.0f * hx, 2.0f * hy, 2.0f * hz, accum ? 1 : 0);
     HIP_LAUNCH_CHECK();
 }
 

@@ -878,7 +878,8 @@ struct PullingScheme {
 
 /* computeLevels + compute_level_for_array + asmlog_2                        */
 /* (LocalParticleCellSet.hpp:128-171). lis becomes the level map (as float). */
-static inline void compute_levels(const Mesh<uint16_t>& grad, Mesh<float>& lis, int max_level, float rel_error, float dx, float dy,
+template <typename G>
+static inline void compute_levels(const Mesh<G>& grad, Mesh<float>& lis, int max_level, float rel_error, float dx, float dy,
                                   float dz) {
     for (size_t i = 0; i < grad.size(); ++i) lis.v[i] = (float)grad.v[i] / lis.v[i];
     float min_dim = std::min(dy, std::min(dx, dz));
@@ -1029,6 +1030,8 @@ struct AprResult {
     Mesh<uint16_t> smoothed;   /* after the three IIR passes            */
     Mesh<uint16_t> grad;       /* after thresholds (input to levels)     */
     Mesh<uint16_t> grad_raw;   /* gradient_step.tif equivalent           */
+    bool is_float = false;     /* APRConverter<float>: the three above as float */
+    Mesh<float> smoothed_f, grad_f, grad_raw_f;
     Mesh<float> lis_raw;       /* local_intensity_scale_step.tif equiv.  */
     Mesh<float> lis;           /* after sigma_th rescale                 */
     Mesh<float> levels;        /* computeLevels output                   */
@@ -1049,7 +1052,15 @@ static inline float bspline_offset_u16(const uint16_t* img, size_t n, float lamb
 
 /* The stages after computeL: applyParameters, solveForAPR,                  */
 /* generateDatastructures (APRConverter.hpp:249-363).                        */
-static inline void apr_from_grad_lis(AprResult& r, Mesh<uint16_t>& grad, Mesh<float>& lis, Mesh<float>& intensity,
+template <typename G>
+static inline void keep_grad(AprResult& r, const Mesh<G>& g);
+template <>
+inline void keep_grad<uint16_t>(AprResult& r, const Mesh<uint16_t>& g) { r.grad = g; }
+template <>
+inline void keep_grad<float>(AprResult& r, const Mesh<float>& g) { r.grad_f = g; }
+
+template <typename G>
+static inline void apr_from_grad_lis(AprResult& r, Mesh<G>& grad, Mesh<float>& lis, Mesh<float>& intensity,
                                      const aprh_params& par, bool keep) {
     /* applyParameters (APRConverter.hpp:249-297) */
     const float ipth = par.Ip_th + r.bspline_offset;
@@ -1062,7 +1073,7 @@ static inline void apr_from_grad_lis(AprResult& r, Mesh<uint16_t>& grad, Mesh<fl
         }
     }
     for (size_t i = 0; i < grad.size(); ++i) if ((float)grad.v[i] < par.grad_th) grad.v[i] = 0;
-    if (keep) { r.grad = grad; r.lis = lis; }
+    if (keep) { keep_grad(r, grad); r.lis = lis; }
     /* solveForAPR (APRConverter.hpp:300-336) */
     compute_levels(grad, lis, r.info.l_max, par.rel_error, par.dx, par.dy, par.dz);
     if (keep) r.levels = lis;
@@ -1097,6 +1108,34 @@ static inline void get_apr_u16(AprResult& r, const Mesh<uint16_t>& input, const 
         if (img.z_num > 2) inv_bspline_z(lis);
     }
     if (keep) r.grad_raw = grad;
+    Mesh<float> intensity;
+    local_intensity_scale(lis, intensity, par);
+    if (keep) r.lis_raw = lis;
+    apr_from_grad_lis(r, grad, lis, intensity, par, keep);
+}
+
+/* get_apr for ImageType = float (APRConverter<float>): no B-spline offset   */
+/* (APRConverter.hpp:218-220), float smoothing, float gradient (grad_temp is */
+/* PixelData<float>), then the same stages.                                  */
+static inline void get_apr_f32(AprResult& r, const Mesh<float>& input, const aprh_params& par, bool keep) {
+    r.info.init(input.y_num, input.x_num, input.z_num);
+    r.is_float = true;
+    const int64_t zd = (int64_t)std::ceil(1.0 * input.z_num / 2.0), xd = (int64_t)std::ceil(1.0 * input.x_num / 2.0),
+                  yd = (int64_t)std::ceil(1.0 * input.y_num / 2.0);
+    Mesh<float> grad(yd, xd, zd, 0.0f);
+    Mesh<float> img = input;
+    r.bspline_offset = 0;
+    if (par.lambda > 0) smooth_bspline_3d(img, par.lambda);
+    if (keep) r.smoothed_f = img;
+    grad_mag_ds(img, grad, par.dx, par.dy, par.dz);
+    Mesh<float> lis;
+    downsample(img, lis, [](float a, float b) { return a + b; }, [](float a) { return (float)((double)a / 8.0); });
+    if (par.lambda > 0) {
+        if (img.y_num > 2) inv_bspline_y(lis);
+        if (img.x_num > 2) inv_bspline_x(lis);
+        if (img.z_num > 2) inv_bspline_z(lis);
+    }
+    if (keep) r.grad_raw_f = grad;
     Mesh<float> intensity;
     local_intensity_scale(lis, intensity, par);
     if (keep) r.lis_raw = lis;

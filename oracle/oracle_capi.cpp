@@ -184,6 +184,20 @@ void* orc_get_apr_u16(const uint16_t* img, int64_t y, int64_t x, int64_t z, cons
     }
 }
 
+void* orc_get_apr_f32(const float* img, int64_t y, int64_t x, int64_t z, const aprh_params* par, int keep, char* errbuf,
+                      int errlen) {
+    try {
+        Handle* h = new Handle();
+        h->dims[0] = y; h->dims[1] = x; h->dims[2] = z;
+        Mesh<float> m = wrap(img, y, x, z);
+        get_apr_f32(h->r, m, *par, keep != 0);
+        return h;
+    } catch (std::exception& e) {
+        set_err(errbuf, errlen, e.what());
+        return nullptr;
+    }
+}
+
 /* get_apr_custom_grad_scale (APRConverter.hpp:169-191): structure from given   */
 /* downsampled gradient and local intensity scale; intensity = Ip_th reference  */
 void* orc_apr_from_grad_lis(const uint16_t* grad, const float* lis, const float* intensity, int64_t y, int64_t x, int64_t z,
@@ -221,10 +235,13 @@ void orc_apr_get(void* hp, uint64_t* level_xz, uint64_t* xz_end, uint16_t* y_vec
 int orc_apr_intermediate(void* hp, int which, void* dst) {
     Handle* h = (Handle*)hp;
     switch (which) {
-        case 0: if (h->r.smoothed.size() == 0) return -1; unwrap(h->r.smoothed, (uint16_t*)dst); return 0;
-        case 1: if (h->r.grad_raw.size() == 0) return -1; unwrap(h->r.grad_raw, (uint16_t*)dst); return 0;
+        case 0: if (h->r.is_float) { if (h->r.smoothed_f.size() == 0) return -1; unwrap(h->r.smoothed_f, (float*)dst); return 0; }
+                if (h->r.smoothed.size() == 0) return -1; unwrap(h->r.smoothed, (uint16_t*)dst); return 0;
+        case 1: if (h->r.is_float) { if (h->r.grad_raw_f.size() == 0) return -1; unwrap(h->r.grad_raw_f, (float*)dst); return 0; }
+                if (h->r.grad_raw.size() == 0) return -1; unwrap(h->r.grad_raw, (uint16_t*)dst); return 0;
         case 2: if (h->r.lis_raw.size() == 0) return -1; unwrap(h->r.lis_raw, (float*)dst); return 0;
-        case 3: if (h->r.grad.size() == 0) return -1; unwrap(h->r.grad, (uint16_t*)dst); return 0;
+        case 3: if (h->r.is_float) { if (h->r.grad_f.size() == 0) return -1; unwrap(h->r.grad_f, (float*)dst); return 0; }
+                if (h->r.grad.size() == 0) return -1; unwrap(h->r.grad, (uint16_t*)dst); return 0;
         case 4: if (h->r.lis.size() == 0) return -1; unwrap(h->r.lis, (float*)dst); return 0;
         case 5: if (h->r.levels.size() == 0) return -1; unwrap(h->r.levels, (float*)dst); return 0;
         default: return -1;

@@ -64,12 +64,18 @@ struct Handle {
     int64_t dims[3];
     PixelData<uint16_t> smoothed, grad_raw, grad;
     PixelData<float> lis_raw, lis, levels;
+    bool is_float = false;                      /* APRConverter<float>: these three instead */
+    PixelData<float> smoothed_f, grad_raw_f, grad_f;
     std::vector<PixelData<uint8_t>> pct;
 };
 
 /* applyParameters + solveForAPR + generateDatastructures, with the stage    */
 /* classes of the reference (APRConverter.hpp:249-363).                      */
-void finish(Handle* h, APRParameters& par, PixelData<uint16_t>& grad_temp, PixelData<float>& local_scale_temp,
+void keep(PixelData<uint16_t>& dst16, PixelData<float>&, PixelData<uint16_t>& src) { dst16.init(src); dst16.copyFromMesh(src); }
+void keep(PixelData<uint16_t>&, PixelData<float>& dstf, PixelData<float>& src) { dstf.init(src); dstf.copyFromMesh(src); }
+
+template <typename G>
+void finish(Handle* h, APRParameters& par, PixelData<G>& grad_temp, PixelData<float>& local_scale_temp,
             PixelData<float>& local_scale_temp2) {
     ComputeGradient cg;
     LocalParticleCellSet lpcs;
@@ -87,7 +93,7 @@ void finish(Handle* h, APRParameters& par, PixelData<uint16_t>& grad_temp, Pixel
     for (size_t i = 0; i < grad_temp.mesh.size(); ++i) {
         if (grad_temp.mesh[i] < par.grad_th) grad_temp.mesh[i] = 0;
     }
-    h->grad.init(grad_temp); h->grad.copyFromMesh(grad_temp);
+    keep(h->grad, h->grad_f, grad_temp);
     h->lis.init(local_scale_temp); h->lis.copyFromMesh(local_scale_temp);
     lpcs.computeLevels(grad_temp, local_scale_temp, h->apr.level_max(), par.rel_error, par.dx, par.dy, par.dz);
     h->levels.init(local_scale_temp); h->levels.copyFromMesh(local_scale_temp);
@@ -122,6 +128,63 @@ static std::vector<PixelData<float>> unpack(const float* flat, const int* dims, 
         off += v[i].mesh.size();
     }
     return v;
+}
+
+/* initPipelineAPR + computeL + the finishing stages for either image type */
+template <typename T>
+void* get_apr_ref(const T* img, int64_t y, int64_t x, int64_t z, const aprh_params* p, char* errbuf, int errlen) {
+    Handle* h = new Handle();
+    try {
+        APRParameters par = to_par(p);
+        h->dims[0] = y; h->dims[1] = x; h->dims[2] = z;
+        h->is_float = std::is_floating_point<T>::value;
+        PixelData<T> input;
+        load(input, img, y, x, z);
+        /* initPipelineAPR (APRConverter.hpp:84-100) */
+        h->apr.parameters = par;
+        h->apr.aprInfo.init(input.y_num, input.x_num, input.z_num);
+        h->apr.linearAccess.genInfo = &h->apr.aprInfo;
+        /* initPipelineMemory (APRConverter.hpp:137-167): grad_temp has the image type */
+        const int zd = ceil(1.0 * z / 2.0), xd = ceil(1.0 * x / 2.0), yd = ceil(1.0 * y / 2.0);
+        PixelData<T> grad_temp;
+        PixelData<float> local_scale_temp, local_scale_temp2;
+        grad_temp.initWithResize(yd, xd, zd);
+        grad_temp.fill(0);
+        local_scale_temp.initWithResize(yd, xd, zd);
+        local_scale_temp2.initWithResize(yd, xd, zd);
+        /* computeL (APRConverter.hpp:194-247) */
+        PixelData<T> image_temp(input, false, false);
+        if (std::is_floating_point<T>::value) {
+            h->bspline_offset = 0;
+            image_temp.copyFromMesh(input);
+        } else {
+            h->bspline_offset = compute_bspline_offset<T>(input, par.lambda);
+            const float off = h->bspline_offset;
+            image_temp.copyFromMeshWithUnaryOp(input, [=](const auto& a) { return (a + off); });
+        }
+        ComputeGradient cg;
+        /* get_gradient split open to keep the smoothed image (ComputeGradient.hpp:70-119) */
+        if (par.lambda > 0) cg.get_smooth_bspline_3D(image_temp, par.lambda);
+        keep(h->smoothed, h->smoothed_f, image_temp);
+        cg.calc_bspline_fd_ds_mag(image_temp, grad_temp, par.dx, par.dy, par.dz);
+        downsample(image_temp, local_scale_temp, [](const float& a, const float& b) -> float { return a + b; },
+                   [](const float& a) -> float { return a / 8.0; });
+        if (par.lambda > 0) {
+            if (image_temp.y_num > 2) cg.calc_inv_bspline_y(local_scale_temp);
+            if (image_temp.x_num > 2) cg.calc_inv_bspline_x(local_scale_temp);
+            if (image_temp.z_num > 2) cg.calc_inv_bspline_z(local_scale_temp);
+        }
+        keep(h->grad_raw, h->grad_raw_f, grad_temp);
+        LocalIntensityScale lis;
+        lis.get_local_intensity_scale(local_scale_temp, local_scale_temp2, par);
+        h->lis_raw.init(local_scale_temp); h->lis_raw.copyFromMesh(local_scale_temp);
+        finish(h, par, grad_temp, local_scale_temp, local_scale_temp2);
+        return h;
+    } catch (std::exception& e) {
+        set_err(errbuf, errlen, e.what());
+        delete h;
+        return nullptr;
+    }
 }
 
 }  // namespace
@@ -239,55 +302,16 @@ void orc_local_intensity_scale(float* lis, float* intensity, int64_t y, int64_t 
     if (intensity) store(b, intensity);
 }
 
-void* orc_get_apr_u16(const uint16_t* img, int64_t y, int64_t x, int64_t z, const aprh_params* p, int keep, char* errbuf,
+void* orc_get_apr_u16(const uint16_t* img, int64_t y, int64_t x, int64_t z, const aprh_params* p, int keep_, char* errbuf,
                       int errlen) {
-    (void)keep;
-    Handle* h = new Handle();
-    try {
-        APRParameters par = to_par(p);
-        h->dims[0] = y; h->dims[1] = x; h->dims[2] = z;
-        PixelData<uint16_t> input;
-        load(input, img, y, x, z);
-        /* initPipelineAPR (APRConverter.hpp:84-100) */
-        h->apr.parameters = par;
-        h->apr.aprInfo.init(input.y_num, input.x_num, input.z_num);
-        h->apr.linearAccess.genInfo = &h->apr.aprInfo;
-        /* initPipelineMemory (APRConverter.hpp:137-167) */
-        const int zd = ceil(1.0 * z / 2.0), xd = ceil(1.0 * x / 2.0), yd = ceil(1.0 * y / 2.0);
-        PixelData<uint16_t> grad_temp;
-        PixelData<float> local_scale_temp, local_scale_temp2;
-        grad_temp.initWithResize(yd, xd, zd);
-        grad_temp.fill(0);
-        local_scale_temp.initWithResize(yd, xd, zd);
-        local_scale_temp2.initWithResize(yd, xd, zd);
-        /* computeL (APRConverter.hpp:194-247) */
-        PixelData<uint16_t> image_temp(input, false, false);
-        h->bspline_offset = compute_bspline_offset<uint16_t>(input, par.lambda);
-        const float off = h->bspline_offset;
-        image_temp.copyFromMeshWithUnaryOp(input, [=](const auto& a) { return (a + off); });
-        ComputeGradient cg;
-        /* get_gradient split open to keep the smoothed image (ComputeGradient.hpp:70-119) */
-        if (par.lambda > 0) cg.get_smooth_bspline_3D(image_temp, par.lambda);
-        h->smoothed.init(image_temp); h->smoothed.copyFromMesh(image_temp);
-        cg.calc_bspline_fd_ds_mag(image_temp, grad_temp, par.dx, par.dy, par.dz);
-        downsample(image_temp, local_scale_temp, [](const float& a, const float& b) -> float { return a + b; },
-                   [](const float& a) -> float { return a / 8.0; });
-        if (par.lambda > 0) {
-            if (image_temp.y_num > 2) cg.calc_inv_bspline_y(local_scale_temp);
-            if (image_temp.x_num > 2) cg.calc_inv_bspline_x(local_scale_temp);
-            if (image_temp.z_num > 2) cg.calc_inv_bspline_z(local_scale_temp);
-        }
-        h->grad_raw.init(grad_temp); h->grad_raw.copyFromMesh(grad_temp);
-        LocalIntensityScale lis;
-        lis.get_local_intensity_scale(local_scale_temp, local_scale_temp2, par);
-        h->lis_raw.init(local_scale_temp); h->lis_raw.copyFromMesh(local_scale_temp);
-        finish(h, par, grad_temp, local_scale_temp, local_scale_temp2);
-        return h;
-    } catch (std::exception& e) {
-        set_err(errbuf, errlen, e.what());
-        delete h;
-        return nullptr;
-    }
+    (void)keep_;
+    return get_apr_ref(img, y, x, z, p, errbuf, errlen);
+}
+
+void* orc_get_apr_f32(const float* img, int64_t y, int64_t x, int64_t z, const aprh_params* p, int keep_, char* errbuf,
+                      int errlen) {
+    (void)keep_;
+    return get_apr_ref(img, y, x, z, p, errbuf, errlen);
 }
 
 void* orc_apr_from_grad_lis(const uint16_t* grad, const float* lisp, const float* intensity, int64_t y, int64_t x, int64_t z,
@@ -329,10 +353,13 @@ void orc_apr_get(void* hp, uint64_t* level_xz, uint64_t* xz_end, uint16_t* y_vec
 int orc_apr_intermediate(void* hp, int which, void* dst) {
     Handle* h = (Handle*)hp;
     switch (which) {
-        case 0: if (!h->smoothed.mesh.size()) return -1; store(h->smoothed, (uint16_t*)dst); return 0;
-        case 1: if (!h->grad_raw.mesh.size()) return -1; store(h->grad_raw, (uint16_t*)dst); return 0;
+        case 0: if (h->is_float) { store(h->smoothed_f, (float*)dst); return 0; }
+                if (!h->smoothed.mesh.size()) return -1; store(h->smoothed, (uint16_t*)dst); return 0;
+        case 1: if (h->is_float) { store(h->grad_raw_f, (float*)dst); return 0; }
+                if (!h->grad_raw.mesh.size()) return -1; store(h->grad_raw, (uint16_t*)dst); return 0;
         case 2: if (!h->lis_raw.mesh.size()) return -1; store(h->lis_raw, (float*)dst); return 0;
-        case 3: store(h->grad, (uint16_t*)dst); return 0;
+        case 3: if (h->is_float) { store(h->grad_f, (float*)dst); return 0; }
+                store(h->grad, (uint16_t*)dst); return 0;
         case 4: store(h->lis, (float*)dst); return 0;
         case 5: store(h->levels, (float*)dst); return 0;
         default: return -1;

@@ -114,8 +114,8 @@ def load(which="oracle"):
 class OracleAPR:
     """Result of get_apr on one of the CPU libraries."""
 
-    def __init__(self, lib, handle, dims):
-        self.lib, self.h, self.dims = lib, handle, dims
+    def __init__(self, lib, handle, dims, is_float=False):
+        self.lib, self.h, self.dims, self.is_float = lib, handle, dims, is_float
         info = np.zeros(6, np.int64)
         lib.orc_apr_info(handle, info)
         self.l_min, self.l_max, self.total = int(info[0]), int(info[1]), int(info[2])
@@ -133,6 +133,8 @@ class OracleAPR:
         shapes = {0: ((z, x, y), np.uint16), 1: ((zd, xd, yd), np.uint16), 2: ((zd, xd, yd), np.float32),
                   3: ((zd, xd, yd), np.uint16), 4: ((zd, xd, yd), np.float32), 5: ((zd, xd, yd), np.float32)}
         shp, dt = shapes[which]
+        if self.is_float and which in (0, 1, 3):
+            dt = np.float32
         out = np.zeros(shp, dt)
         if self.lib.orc_apr_intermediate(self.h, which, out.ctypes.data) != 0:
             return None
@@ -225,6 +227,21 @@ def get_apr(img_zxy, par, which="oracle", keep=True):
     if not h:
         raise ValueError(err.value.decode())
     return OracleAPR(lib, h, (y, x, z))
+
+
+def get_apr_f32(img_zxy, par, which="oracle", keep=True):
+    """APRConverter<float>::get_apr; img_zxy float32 (z, x, y)"""
+    lib = load(which)
+    f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+    lib.orc_get_apr_f32.argtypes = [f32p, C.c_int64, C.c_int64, C.c_int64, C.POINTER(AprhParams), C.c_int, C.c_char_p,
+                                    C.c_int]
+    lib.orc_get_apr_f32.restype = C.c_void_p
+    z, x, y = img_zxy.shape
+    err = C.create_string_buffer(512)
+    h = lib.orc_get_apr_f32(np.ascontiguousarray(img_zxy, np.float32).reshape(-1), y, x, z, C.byref(par), int(keep), err, 512)
+    if not h:
+        raise ValueError(err.value.decode())
+    return OracleAPR(lib, h, (y, x, z), is_float=True)
 
 
 def apr_from_grad_lis(grad, lis, intensity, dims, par, offset, which="oracle"):

@@ -91,3 +91,32 @@ def test_gradient_sqrt_exact(oracle, ctx):
     lib.orc_downsample_mean_u16_f32(img.reshape(-1), mo.reshape(-1), y, x, z)
     assert np.array_equal(g, go)
     assert np.array_equal(m.view(np.uint32), mo.view(np.uint32))
+
+
+F32_CASES = [((32, 64, 64), 20, 0, 0.731), ((29, 45, 61), 12, 3, 1.913), ((3, 37, 200), 10, 2, 0.05), ((64, 128, 192), 80, 6, 3.0)]
+
+
+@pytest.mark.parametrize("shape,nb,seed,scale", F32_CASES)
+def test_get_apr_f32_matches_oracle(oracle, ctx, shape, nb, seed, scale):
+    """APRConverter<float>: every stage output, the structure and float sampling bit-equal"""
+    img = (A.synth_blobs(shape, seed, nb).astype(np.float32) * np.float32(scale)).astype(np.float32)
+    kw = dict(rel_error=0.1, sigma_th=50 * scale, Ip_th=120 * scale, grad_th=10 * scale)
+    ref = oracle.get_apr_f32(img, oracle.params(**kw), "oracle")
+    conv = A.APRConverter(ctx, image_type=np.float32)
+    conv.par = A.APRParameters(**kw)
+    apr = conv.get_apr(img)
+    print(f"f32 case {shape} P={apr.total} oracle P={ref.total}", flush=True)
+    assert apr.bspline_offset == 0.0 and apr.image_dtype == np.float32
+    bad = []
+    for w in range(6):
+        g, o = apr.intermediate(w), ref.intermediate(w)
+        assert g.dtype == o.dtype
+        nmis = int((g.view(np.uint8) != o.view(np.uint8)).sum())
+        if nmis:
+            bad.append(f"intermediate {w}: {nmis} bytes differ")
+    assert not bad, "\n".join(bad)
+    lxz, xze, yv = apr.structure()
+    assert apr.total == ref.total
+    assert np.array_equal(xze, ref.xz_end_vec) and np.array_equal(yv, ref.y_vec)
+    parts = A.ParticleData().sample_image(apr, img)
+    assert np.array_equal(parts.view(np.uint32), ref.sample_f32(img).view(np.uint32))

@@ -250,3 +250,28 @@ def test_overflow_detection():
     img.reshape(-1)[img.size // 2] = 65535 - 100
     with pytest.raises(ValueError):
         O.get_apr(img, fixture_params(f), "oracle")
+
+
+F32_CASES = [((32, 64, 64), 20, 0, 0.731), ((29, 45, 61), 12, 3, 1.913), ((16, 16, 130), 8, 4, 0.05)]
+
+
+@pytest.mark.parametrize("shape,nb,seed,scale", F32_CASES)
+def test_oracle_get_apr_f32_equals_reference(shape, nb, seed, scale):
+    """APRConverter<float>: structure, stage outputs and float sampling bit-equal to the
+    reference build (no B-spline offset, float smoothing and gradient)"""
+    import oracle_lib as O
+    if O.load("ref") is None:
+        pytest.skip("reference build (oracle/_ref) not available")
+    import libapr_amd as A
+    img = (A.synth_blobs(shape, seed, nb).astype(np.float32) * np.float32(scale)).astype(np.float32)
+    par = O.params(rel_error=0.1, sigma_th=50 * scale, Ip_th=120 * scale, grad_th=10 * scale)
+    o = O.get_apr_f32(img, par, "oracle")
+    r = O.get_apr_f32(img, par, "ref")
+    assert o.total == r.total and o.bspline_offset == 0.0
+    for k in ("level_xz_vec", "xz_end_vec", "y_vec"):
+        assert np.array_equal(getattr(o, k), getattr(r, k)), k
+    for w in range(6):
+        a, b = o.intermediate(w), r.intermediate(w)
+        assert a is not None and b is not None and a.dtype == b.dtype
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), w
+    assert o.intermediate(0).dtype == np.float32
