@@ -258,61 +258,64 @@ struct PadGeom {
     int64_t yp, xp, zp;       /* padded dims                  */
 };
 
-/* source sample of the y-mean pass: mode 0 = padded copy of the intensity,
-   mode 1 = |A - padded intensity| (calc_abs_diff, LocalIntensityScale.hpp:188-196) */
-__device__ __forceinline__ float ysrc(int mode, const float* __restrict__ inten, const float* A, const PadGeom& g, int64_t line,
-                                      int64_t yy) {
-    const int64_t xp = line % g.xp, zp = line / g.xp;
-    const float b = inten[(refl(zp - g.pz, g.zd) * g.xd + refl(xp - g.px, g.xd)) * g.yd + refl(yy - g.py, g.yd)];
-    if (mode == 0) return b;
-    return fabsf(A[line * g.yp + yy] - b);
-}
-
-/* y running-sum mean with offset O over padded lines (LocalIntensityScale.hpp:311-363);
-   O == 0 means "axis inactive": copy only.  Output written to A (may alias the source). */
+/* y running-sum mean with offset O over padded lines (LocalIntensityScale.hpp:311-363).
+   Source sample: mode 0 = padded copy of the intensity, mode 1 = |A - padded intensity|
+   (calc_abs_diff, LocalIntensityScale.hpp:188-196).  O == 0 means "axis inactive": copy only.  Output written to A (may alias the source).
+   One lane per line (the recursion is sequential in y); 64-line x 64-sample tiles are
+   staged through LDS so global loads and stores stay coalesced along y.  The per-line
+   source offsets are computed once per block. */
 template <int O>
 __global__ __launch_bounds__(64) void k_mean_y(int mode, const float* __restrict__ inten, float* A, PadGeom g) {
     constexpr int TY = 64;
     constexpr int H = 2 * O + 2;
-    __shared__ float tin[64][TY + 1];
-    __shared__ float tout[64][TY + 1];
+    __shared__ float tile[64][TY + 1];
+    __shared__ int64_t src_row[64];   /* inten offset of the (reflected) source row of each line */
     const int lane = threadIdx.x;
     const int64_t n_lines = g.xp * g.zp;
     const int64_t line0 = (int64_t)blockIdx.x * 64;
     const int64_t me = line0 + lane;
     const bool valid = me < n_lines;
-    const int64_t yn = g.yp;
+    const int yn = (int)g.yp, ydn = (int)g.yd, py = (int)g.py;
     const float d = (float)(2 * O + 1);
+    if (valid) {
+        const int64_t xp = me % g.xp, zp = me / g.xp;
+        src_row[lane] = (refl(zp - g.pz, g.zd) * g.xd + refl(xp - g.px, g.xd)) * g.yd;
+    }
+    __syncthreads();
+    const int nl = (int)min((int64_t)64, n_lines - line0);
+    auto src = [&](int l, int yy) -> float {
+        int ys = yy - py;
+        if (ys < 0) ys = -ys;
+        if (ys >= ydn) ys = 2 * (ydn - 1) - ys;
+        const float b = inten[src_row[l] + ys];
+        if (mode == 0) return b;
+        return fabsf(A[(line0 + l) * yn + yy] - b);
+    };
     float h[H > 0 ? H : 1];
     for (int i = 0; i < H; ++i) h[i] = 0.0f;
     float t = 0.0f;
     /* pre-consume samples [0, O) */
     if (valid) {
         for (int j = 0; j < O; ++j) {
-            t += ysrc(mode, inten, A, g, me, j);
+            t += src(lane, j);
             for (int i = 0; i + 1 < H; ++i) h[i] = h[i + 1];
             h[H - 1] = t;
         }
     }
-    __syncthreads();
-    for (int64_t p0 = 0; p0 < yn; p0 += TY) {
+    for (int p0 = 0; p0 < yn; p0 += TY) {
         /* inputs for outputs [p0, p0+64) are samples [p0+O, p0+O+64) */
-        for (int l = 0; l < 64; ++l) {
-            const int64_t ln = line0 + l;
-            const int64_t yy = p0 + O + lane;
-            if (ln < n_lines && yy < yn) tin[l][lane] = ysrc(mode, inten, A, g, ln, yy);
-        }
+        const int yy = p0 + O + lane;
+        if (yy < yn)
+            for (int l = 0; l < nl; ++l) tile[l][lane] = src(l, yy);
         __syncthreads();
         if (valid) {
             for (int k = 0; k < TY; ++k) {
-                const int64_t q = p0 + k;
+                const int q = p0 + k;
                 if (q >= yn) break;
-                if (O == 0) {
-                    tout[lane][k] = tin[lane][k];
-                    continue;
-                }
+                if (O == 0) continue;   /* copy: the tile already holds the samples */
+                const float x = tile[lane][k];
                 if (q + O < yn) {
-                    t += tin[lane][k];
+                    t += x;
                     for (int i = 0; i + 1 < H; ++i) h[i] = h[i + 1];
                     h[H - 1] = t;
                 }
@@ -321,23 +324,21 @@ __global__ __launch_bounds__(64) void k_mean_y(int mode, const float* __restrict
                     v += h[H - 1] / d;
                     if (q >= 1 && q <= O) v = (float)((double)v * ((double)d / ((double)(q + O) + 1.0)));
                     if (q == 0) v = (float)((double)v * ((double)d / ((double)O + 1.0)));
-                    tout[lane][k] = v;
+                    tile[lane][k] = v;
                 }
             }
         }
         __syncthreads();
-        for (int l = 0; l < 64; ++l) {
-            const int64_t ln = line0 + l;
-            const int64_t q = p0 + lane;
-            if (ln < n_lines && q < yn - O) A[ln * yn + q] = tout[l][lane];
-        }
+        const int q = p0 + lane;
+        if (q < yn - O)
+            for (int l = 0; l < nl; ++l) A[(line0 + l) * yn + q] = tile[l][lane];
         __syncthreads();
     }
     /* right-hand end (LocalIntensityScale.hpp:346-352) */
     if (valid && O > 0) {
         const float tot = h[H - 1];
         for (int r = 0; r < O; ++r) {
-            const int64_t q = yn - O + r;
+            const int q = yn - O + r;
             if (q < 0) continue;
             float v = (q <= O) ? 0.0f : -h[1 + r] / d;
             const float cnt = (float)(r + 1);

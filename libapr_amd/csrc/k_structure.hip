@@ -64,50 +64,105 @@ __global__ void k_seed(const uint8_t* __restrict__ lev, uint8_t* pct, size_t n, 
     }
 }
 
-/* one pulling-scheme level (PullingScheme.hpp:111-132 loop body) */
-__global__ void k_pull_level(uint8_t* P, Dims d, uint8_t* Pc, Dims dc, uint8_t* Pp, Dims dp, int is_top, int has_parent) {
-    const size_t n = (size_t)(d.y * d.x * d.z);
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const int64_t y = (int64_t)(i % (size_t)d.y);
-        const int64_t r = (int64_t)(i / (size_t)d.y);
-        const int64_t x = r % d.x, z = r / d.x;
-        const uint8_t old = P[i];
-        bool any_asc = false, any_sp = false;
-        const int64_t za = z > 0 ? z - 1 : 0, zb = z < d.z - 1 ? z + 1 : z;
-        const int64_t xa = x > 0 ? x - 1 : 0, xb = x < d.x - 1 ? x + 1 : x;
-        const int64_t ya = y > 0 ? y - 1 : 0, yb = y < d.y - 1 ? y + 1 : y;
-        for (int64_t zz = za; zz <= zb; ++zz)
-            for (int64_t xx = xa; xx <= xb; ++xx)
-                for (int64_t yy = ya; yy <= yb; ++yy) {
-                    const uint8_t s = P[(zz * d.x + xx) * d.y + yy];
-                    any_asc |= (s == ASC);
-                    any_sp |= (s == SEED) | (s == PROP);
+/* one pulling-scheme level (PullingScheme.hpp:111-132 loop body): set_ascendant_neighbours
+   and fill_neighbours on this level, set_filler on the finer one, fill_parent on the coarser
+   one.  One lane per 8 consecutive y cells: the 3x3x3 neighbourhood
+   predicates are formed as bit masks over the 9 neighbouring rows (10 bits: y0-1 .. y0+8),
+   dilated along y by shifts, so a lane loads 9 row words instead of 27 bytes per cell. */
+__device__ __forceinline__ void load_row8(const uint8_t* row, int yn, int y0, bool aligned8, uint8_t b[10]) {
+    b[0] = (y0 > 0) ? row[y0 - 1] : (uint8_t)0;
+    if (aligned8 && y0 + 8 <= yn) {
+        const uint64_t w = *(const uint64_t*)(row + y0);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) b[1 + j] = (uint8_t)(w >> (8 * j));
+    } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) b[1 + j] = (y0 + j < yn) ? row[y0 + j] : (uint8_t)0;
+    }
+    b[9] = (y0 + 8 < yn) ? row[y0 + 8] : (uint8_t)0;
+}
+
+__global__ __launch_bounds__(256) void k_pull_level8(uint8_t* P, Dims d, uint8_t* Pc, Dims dc, uint8_t* Pp, Dims dp, int is_top,
+                                                     int has_parent) {
+    const int yn = (int)d.y, xn = (int)d.x, zn = (int)d.z;
+    const int nwr = (yn + 7) >> 3;
+    const int64_t total = (int64_t)nwr * xn * zn;
+    const bool aligned8 = (yn & 7) == 0 && ((uintptr_t)P & 7) == 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int w = (int)(i % nwr);
+        const int r = (int)(i / nwr);
+        const int x = r % xn, z = r / xn;
+        const int y0 = w * 8;
+        uint32_t am = 0, sm = 0;   /* bit k <-> y0 - 1 + k */
+        for (int dz = -1; dz <= 1; ++dz) {
+            const int zz = z + dz;
+            if (zz < 0 || zz >= zn) continue;
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int xx = x + dx;
+                if (xx < 0 || xx >= xn) continue;
+                uint8_t b[10];
+                load_row8(P + ((int64_t)zz * xn + xx) * yn, yn, y0, aligned8, b);
+#pragma unroll
+                for (int k = 0; k < 10; ++k) {
+                    am |= (uint32_t)(b[k] == ASC) << k;
+                    sm |= (uint32_t)((b[k] == SEED) | (b[k] == PROP)) << k;
                 }
-        uint8_t nw = old;
-        if (!is_top) {
-            /* set_ascendant_neighbours then fill_neighbours */
-            if (old == EMPTY) nw = any_asc ? ASCN : (any_sp ? BOUNDARY : EMPTY);
-            else if (old == SEED) nw = any_asc ? PROP : SEED;
-        } else {
-            if (old == EMPTY && any_sp) nw = BOUNDARY;
+            }
         }
-        if (nw != old) P[i] = nw;
-        /* set_filler: empty children of ASCENDANTNEIGHBOUR / PROPOGATE become FILLER */
-        if (!is_top && (nw == ASCN || nw == PROP)) {
-            const int64_t cz = (z == d.z - 1 && (dc.z & 1)) ? 1 : 2;
-            const int64_t cx = (x == d.x - 1 && (dc.x & 1)) ? 1 : 2;
-            const int64_t cy = (y == d.y - 1 && (dc.y & 1)) ? 1 : 2;
-            for (int64_t a = 0; a < cz; ++a)
-                for (int64_t b = 0; b < cx; ++b)
-                    for (int64_t c = 0; c < cy; ++c) {
-                        uint8_t* ch = &Pc[((2 * z + a) * dc.x + (2 * x + b)) * dc.y + (2 * y + c)];
-                        if (*ch == EMPTY) *ch = FILLER;
-                    }
+        const uint32_t ad = am | (am << 1) | (am >> 1), sd = sm | (sm << 1) | (sm >> 1);
+        uint8_t* row = P + ((int64_t)z * xn + x) * yn;
+        uint8_t ob[10];
+        load_row8(row, yn, y0, aligned8, ob);
+        uint8_t nb[8];
+        bool changed = false;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const uint8_t old = ob[1 + j];
+            const bool any_asc = (ad >> (j + 1)) & 1u, any_sp = (sd >> (j + 1)) & 1u;
+            uint8_t nw = old;
+            if (!is_top) {
+                if (old == EMPTY) nw = any_asc ? ASCN : (any_sp ? BOUNDARY : EMPTY);
+                else if (old == SEED) nw = any_asc ? PROP : SEED;
+            } else if (old == EMPTY && any_sp) {
+                nw = BOUNDARY;
+            }
+            nb[j] = nw;
+            changed |= (nw != old) && (y0 + j < yn);
         }
-        /* fill_parent */
-        if (has_parent && (nw == SEED || nw == PROP || nw == ASC)) {
-            uint8_t* p = &Pp[((z / 2) * dp.x + (x / 2)) * dp.y + (y / 2)];
-            if (*p != SEED) *p = ASC;
+        if (changed) {
+            if (aligned8) {
+                uint64_t v = 0;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) v |= (uint64_t)nb[j] << (8 * j);
+                *(uint64_t*)(row + y0) = v;
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (y0 + j < yn && nb[j] != ob[1 + j]) row[y0 + j] = nb[j];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int y = y0 + j;
+            if (y >= yn) break;
+            const uint8_t nw = nb[j];
+            /* set_filler: empty children of ASCENDANTNEIGHBOUR / PROPOGATE become FILLER */
+            if (!is_top && (nw == ASCN || nw == PROP)) {
+                const int cz = (z == zn - 1 && (dc.z & 1)) ? 1 : 2;
+                const int cx = (x == xn - 1 && (dc.x & 1)) ? 1 : 2;
+                const int cy = (y == yn - 1 && (dc.y & 1)) ? 1 : 2;
+                for (int a = 0; a < cz; ++a)
+                    for (int bb = 0; bb < cx; ++bb)
+                        for (int c = 0; c < cy; ++c) {
+                            uint8_t* ch = &Pc[((int64_t)(2 * z + a) * dc.x + (2 * x + bb)) * dc.y + (2 * y + c)];
+                            if (*ch == EMPTY) *ch = FILLER;
+                        }
+            }
+            /* fill_parent */
+            if (has_parent && (nw == SEED || nw == PROP || nw == ASC)) {
+                uint8_t* pp = &Pp[((int64_t)(z / 2) * dp.x + (x / 2)) * dp.y + (y / 2)];
+                if (*pp != SEED) *pp = ASC;
+            }
         }
     }
 }
@@ -137,14 +192,9 @@ struct SegTable {
     int64_t map_off[MAXL];        /* byte offset of the pct map used   */
     int64_t map_y[MAXL], map_x[MAXL]; /* dims of that pct map           */
     int64_t y_max;                /* y_num at l_max                    */
+    int64_t row_prefix[MAXL + 1]; /* first global row of each APR level */
     int min_type;
 };
-
-__device__ __forceinline__ int find_level(const SegTable& t, int64_t s) {
-    int li = 0;
-    while (li + 1 < t.n_levels && s >= t.seg_begin[li + 1]) ++li;
-    return li;
-}
 
 /* weight of one pct cell for APR level li (0 or 1; 0..2 at l_max) */
 __device__ __forceinline__ int cell_weight(const SegTable& t, int li, uint8_t st, int64_t yy) {
@@ -155,37 +205,48 @@ __device__ __forceinline__ int cell_weight(const SegTable& t, int li, uint8_t st
     return (st > t.min_type && st <= UPSEED) ? 1 : 0;
 }
 
-/* locate segment s: level index, pct row start pointer, y0 and row length */
-__device__ __forceinline__ const uint8_t* seg_row(const SegTable& t, const uint8_t* maps, int64_t s, int& li, int64_t& y0,
-                                                  int64_t& ylen, int64_t& xzrow) {
-    li = find_level(t, s);
-    const int64_t local = s - t.seg_begin[li];
-    const int64_t row = local / t.nseg_row[li];
-    const int64_t seg = local % t.nseg_row[li];
-    xzrow = row;
-    y0 = seg * SEG;
-    ylen = t.map_y[li];
-    int64_t prow;
-    if (li == t.n_levels - 1) {
-        /* max-level row (z, x) reads pct row (z/2, x/2) of level l_max - 1 */
-        const int64_t zm = row / t.rows_x[li], xm = row % t.rows_x[li];
-        prow = (zm / 2) * t.map_x[li] + xm / 2;
-    } else {
-        prow = row;
+/* One wave per particle row (grid-stride): the row's 64-cell segments in order, one cell
+   per lane.  fill == 0: per-segment particle counts; fill == 1: y values written from the
+   scanned segment offsets with ballot prefix counts. */
+__global__ __launch_bounds__(256) void k_row_segments(const uint8_t* __restrict__ maps, SegTable t, int64_t nrows, uint32_t* cnt,
+                                                      const uint64_t* __restrict__ seg_off, uint16_t* y_vec, int fill) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int64_t r = wid; r < nrows; r += nw) {
+        int li = 0;
+        while (li + 1 < t.n_levels && r >= t.row_prefix[li + 1]) ++li;
+        const int64_t row = r - t.row_prefix[li];
+        const int64_t ylen = t.map_y[li];
+        const bool top = (li == t.n_levels - 1);
+        int64_t prow = row;
+        if (top) {
+            const int64_t zm = row / t.rows_x[li], xm = row - zm * t.rows_x[li];
+            prow = (zm / 2) * t.map_x[li] + xm / 2;
+        }
+        const uint8_t* p = maps + t.map_off[li] + prow * ylen;
+        const int nseg = (int)t.nseg_row[li];
+        const int64_t s0 = t.seg_begin[li] + row * nseg;
+        for (int c = 0; c < nseg; ++c) {
+            const int yy = c * SEG + lane;
+            const int w = (yy < ylen) ? cell_weight(t, li, p[yy], yy) : 0;
+            const uint64_t b1 = __ballot(w >= 1);
+            const uint64_t b2 = __ballot(w >= 2);
+            if (!fill) {
+                if (lane == 0) cnt[s0 + c] = (uint32_t)(__popcll(b1) + __popcll(b2));
+                continue;
+            }
+            if (w == 0) continue;
+            const uint64_t pos = seg_off[s0 + c] + __popcll(b1 & lt) + __popcll(b2 & lt);
+            if (top) {
+                y_vec[pos] = (uint16_t)(2 * yy);
+                if (w == 2) y_vec[pos + 1] = (uint16_t)(2 * yy + 1);
+            } else {
+                y_vec[pos] = (uint16_t)yy;
+            }
+        }
     }
-    return maps + t.map_off[li] + prow * ylen;
-}
-
-__global__ void k_seg_count(const uint8_t* __restrict__ maps, SegTable t, int64_t nseg, uint32_t* cnt) {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nseg) return;
-    int li;
-    int64_t y0, ylen, row;
-    const uint8_t* p = seg_row(t, maps, s, li, y0, ylen, row);
-    const int64_t y1 = min(y0 + SEG, ylen);
-    uint32_t c = 0;
-    for (int64_t yy = y0; yy < y1; ++yy) c += cell_weight(t, li, p[yy], yy);
-    cnt[s] = c;
 }
 
 /* exclusive scan, three phases over blocks of 1024 */
@@ -253,32 +314,6 @@ __global__ void k_xz_end(const uint64_t* __restrict__ seg_off, const uint32_t* _
     xz_end[t.row_begin[li] + row] = seg_off[s_last] + cnt[s_last];
 }
 
-/* wave per segment: y values of the qualifying cells in order */
-__global__ __launch_bounds__(256) void k_seg_fill(const uint8_t* __restrict__ maps, SegTable t, int64_t nseg,
-                                                  const uint64_t* __restrict__ seg_off, uint16_t* y_vec) {
-    const int64_t s = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
-    const int lane = threadIdx.x & 63;
-    if (s >= nseg) return;
-    int li;
-    int64_t y0, ylen, row;
-    const uint8_t* p = seg_row(t, maps, s, li, y0, ylen, row);
-    const int64_t yy = y0 + lane;
-    int w = 0;
-    if (yy < ylen) w = cell_weight(t, li, p[yy], yy);
-    const uint64_t b1 = __ballot(w >= 1);
-    const uint64_t b2 = __ballot(w >= 2);
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const uint64_t pos = seg_off[s] + __popcll(b1 & lt) + __popcll(b2 & lt);
-    if (w == 0) return;
-    if (li == t.n_levels - 1) {
-        y_vec[pos] = (uint16_t)(2 * yy);
-        if (w == 2) y_vec[pos + 1] = (uint16_t)(2 * yy + 1);
-    } else {
-        y_vec[pos] = (uint16_t)yy;
-    }
-}
-
-/* mean pyramid level (downsamplePyramid with sum then x/8.0f, u16 store) */
 /* mean of the 2x2x2 block in the reference's summation order (PixelData.hpp:952-962), stored
    as the pyramid's element type (u16: truncation like a float -> uint16 assignment) */
 template <typename T> __device__ __forceinline__ T pyr_store(float m);
@@ -415,7 +450,7 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
         const bool is_top = (l == top);
         uint8_t* Pc = is_top ? nullptr : P + off[l + 1];
         uint8_t* Pp = has_parent ? P + off[l - 1] : nullptr;
-        hipLaunchKernelGGL(k_pull_level, grid, block, 0, st, P + off[l], D(l), Pc, is_top ? Dims{1, 1, 1} : D(l + 1), Pp,
+        hipLaunchKernelGGL(k_pull_level8, grid, block, 0, st, P + off[l], D(l), Pc, is_top ? Dims{1, 1, 1} : D(l + 1), Pp,
                            has_parent ? D(l - 1) : Dims{1, 1, 1}, is_top ? 1 : 0, has_parent ? 1 : 0);
         HIP_LAUNCH_CHECK();
     }
@@ -435,7 +470,7 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
     if (t.n_levels > MAXL) throw Error(APRH_ERR_UNSUPPORTED, "too many levels");
     t.min_type = min_type;
     t.y_max = a.y_num[lmax];
-    int64_t sb = 0;
+    int64_t sb = 0, rp = 0;
     for (int l = lmin; l <= lmax; ++l) {
         const int li = l - lmin;
         const int ml = (l == lmax) ? top : l;   /* pct map providing the cells */
@@ -446,15 +481,20 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
         t.map_off[li] = off[ml];
         t.map_y[li] = a.y_num[ml];
         t.map_x[li] = a.x_num[ml];
+        t.row_prefix[li] = rp;
+        rp += a.x_num[l] * a.z_num[l];
         sb += a.x_num[l] * a.z_num[l] * t.nseg_row[li];
     }
     t.seg_begin[t.n_levels] = sb;
+    t.row_prefix[t.n_levels] = rp;
     const int64_t nseg = sb;
     Buf cnt(ctx->pool, (size_t)nseg * 4);
     Buf segoff(ctx->pool, (size_t)nseg * 8);
     const int64_t nb = cdiv(nseg, 1024);
     Buf bsum(ctx->pool, (size_t)(nb + 1) * 8);
-    hipLaunchKernelGGL(k_seg_count, dim3((unsigned)cdiv(nseg, 256)), dim3(256), 0, st, P, t, nseg, cnt.as<uint32_t>());
+    const dim3 rgrid((unsigned)std::min<int64_t>(cdiv(rp, 4), 16384));
+    hipLaunchKernelGGL(k_row_segments, rgrid, dim3(256), 0, st, P, t, rp, cnt.as<uint32_t>(), (const uint64_t*)nullptr,
+                       (uint16_t*)nullptr, 0);
     HIP_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(1024), 0, st, cnt.as<uint32_t>(), segoff.as<uint64_t>(),
                        bsum.as<uint64_t>(), nseg);
@@ -477,8 +517,8 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
     HIP_CHECK(hipStreamSynchronize(st));
     a.total = total;
     a.y_vec.reset(ctx->pool, std::max<uint64_t>(total, 1) * 2);
-    hipLaunchKernelGGL(k_seg_fill, dim3((unsigned)cdiv(nseg * 64, 256)), dim3(256), 0, st, P, t, nseg, segoff.as<uint64_t>(),
-                       a.y_vec.as<uint16_t>());
+    hipLaunchKernelGGL(k_row_segments, rgrid, dim3(256), 0, st, P, t, rp, (uint32_t*)nullptr, (const uint64_t*)segoff.as<uint64_t>(),
+                       a.y_vec.as<uint16_t>(), 1);
     HIP_LAUNCH_CHECK();
     if (ctx->keep_intermediates) {
         a.pct.clear();
