@@ -228,63 +228,52 @@ def _rel_l2(a, b):
 
 
 def run_rl(args):
-    """plain APR Richardson-Lucy to tolerance on the 1024x1024x256 deconvolution workload"""
+    """plain and learned-warm-start APR Richardson-Lucy to tolerance on the 1024x1024x256
+    deconvolution workload (libapr_amd.warm_start), plus the reference-order iteration time"""
     import libapr_amd as A
+    from libapr_amd import warm_start as W
     ctx = A.Context(0)
-    L = A.lib()
     z, x, y = RL_SHAPE_ZXY
-    n = x * y * z
-    gt, img = _dev(A, ctx, n * 4), _dev(A, ctx, n * 2)
     g = RL_GEN
-    A.synth_deconv(RL_SHAPE_ZXY, g["seed"], g["n_blobs"], gt.value, img.value, sigma=g["sigma"], amp=g["amp"],
-                   background=g["background"], psf_sigma=g["psf_sigma"], psf_taps=g["psf_taps"], ctx=ctx)
-    conv = A.APRConverter(ctx)
-    conv.par = A.APRParameters(**APR_PARAMS)
-    apr = conv.get_apr(img.value, dims=(y, x, z))
-    P = apr.total
-    parts, u = _dev(A, ctx, P * 2), _dev(A, ctx, P * 4)
-    A.ParticleData().sample_image(apr, img.value, out=parts.value)
-    psf = A.APRStencil.create_gaussian_filter(g["psf_sigma"], (g["psf_taps"],) * 3, True)
-    apr.tree_info()
+    w = W.Workload(ctx, RL_SHAPE_ZXY, g["seed"], g["n_blobs"])
+    psf = W.psf_stencil()
+    u = A.DeviceArray(w.P * 4, ctx)
+    w.apr.tree_info()
     # per-iteration time with every tap in the reference's order
-    A.APRNumerics.richardson_lucy(apr, parts.value, u.value, psf, 1, normalize=True, dtype="uint16")
+    ctx.set_conv_mode(A.CONV_EXACT)
+    w.rl(u, iters=1, tol=0.0)
     t = time.perf_counter()
-    A.APRNumerics.richardson_lucy(apr, parts.value, u.value, psf, 2, normalize=True, dtype="uint16")
+    w.rl(u, iters=2, tol=0.0)
     exact_ms = (time.perf_counter() - t) * 500.0
     log(f"[bench] RL iteration, reference tap order: {exact_ms:.1f} ms")
-    # time to tolerance with the PSF evaluated separably (one untimed iteration first)
+    if os.path.exists(W.DEFAULT_WEIGHTS):
+        net = A.WarmStartCNN.load(W.DEFAULT_WEIGHTS, ctx)
+        trained = "libapr_amd/data/warm_start_cnn.npy"
+    else:
+        net = A.WarmStartCNN(ctx, params=None)
+        trained = "untrained (weights file missing)"
+    ev = W.evaluate(ctx, net, workload=w, log=log)
+    # data residual ||psf * u - input|| / ||input|| of the plain result
     ctx.set_conv_mode(A.CONV_SEPARABLE)
-    A.APRNumerics.richardson_lucy(apr, parts.value, u.value, psf, 1, normalize=True, dtype="uint16")
-    t = time.perf_counter()
-    _, st = A.APRNumerics.richardson_lucy(apr, parts.value, u.value, psf, RL_CAP, normalize=True, tol=RL_TOL,
-                                          dtype="uint16")
-    wall = time.perf_counter() - t
-    log(f"[bench] plain RL: {st['iterations']} iterations to rel update {st['last_rel_update']:.2e} in {wall:.2f} s "
-        f"(P={P})")
-    # error against the ground truth (piecewise-constant reconstruction of the estimate), on the host in f64
-    rec = _dev(A, ctx, n * 4)
-    gt_h = _to_host(A, ctx, gt, n, np.float32)
-    A.APRReconstruction.reconstruct_constant(apr, u.value, 0, np.float32, out=rec.value, dtype="float32")
-    err = _rel_l2(_to_host(A, ctx, rec, n, np.float32), gt_h)
-    A.APRReconstruction.reconstruct_constant(apr, parts.value, 0, np.float32, out=rec.value, dtype="uint16")
-    err_in = _rel_l2(_to_host(A, ctx, rec, n, np.float32), gt_h)
-    del gt_h
-    # data residual ||psf * u - input|| / ||input|| on the particles
-    blur = _dev(A, ctx, P * 4)
-    A.APRFilter.convolve_pencil(apr, psf, u.value, blur.value, normalize=True, dtype="float32")
-    resid = _rel_l2(_to_host(A, ctx, blur, P, np.float32), _to_host(A, ctx, parts, P, np.uint16))
-    log(f"[bench] RL error vs ground truth {err:.4f} (input {err_in:.4f}), data residual {resid:.4f}")
+    w.rl(u)
+    blur = A.DeviceArray(w.P * 4, ctx)
+    A.APRFilter.convolve_pencil(w.apr, psf, u.ptr, blur.ptr, normalize=True, dtype="float32")
+    resid = _rel_l2(blur.to_host(np.float32), w.parts.to_host(np.uint16))
     res = {"workload": "1024x1024x256 GT (bg 100 + 50k blobs sigma 1-4 amp 200-2000) * 9^3 Gaussian PSF "
-                       "(sigma 1,1,2) + shot noise, seed 3; plain RL from 1 to rel update < 1e-3 (cap 200)",
-           "particles": P, "iterations": st["iterations"], "time_to_tol_s": round(wall, 3),
-           "ms_per_iteration": round(wall * 1000.0 / max(st["iterations"], 1), 2),
-           "final_rel_update": st["last_rel_update"], "data_residual": round(resid, 5),
-           "rel_l2_error_vs_gt": round(err, 5), "input_rel_l2_error_vs_gt": round(err_in, 5),
-           "conv_mode": "separable", "exact_order_ms_per_iteration": round(exact_ms, 2),
-           "warm_start": "not built yet"}
-    apr.close()
-    for p in (gt, img, parts, u, rec, blur):
-        A._check(L.aprh_free(ctx.h, p))
+                       "(sigma 1,1,2) + shot noise, seed 3; RL from 1 (plain) or from the network (warm) "
+                       "to rel update < 1e-3 (cap 200)",
+           "particles": w.P, "iterations": ev["plain_iterations"], "time_to_tol_s": ev["plain_time_s"],
+           "ms_per_iteration": round(ev["plain_time_s"] * 1000.0 / max(ev["plain_iterations"], 1), 2),
+           "final_rel_update": ev["plain_final_rel_update"], "data_residual": round(resid, 5),
+           "rel_l2_error_vs_gt": ev["plain_rel_l2_error_vs_gt"], "conv_mode": "separable",
+           "exact_order_ms_per_iteration": round(exact_ms, 2),
+           "warm_start": {"weights": trained, "iterations": ev["warm_iterations"], "time_to_tol_s": ev["warm_time_s"],
+                          "network_s": ev["warm_network_time_s"], "final_rel_update": ev["warm_final_rel_update"],
+                          "rel_l2_error_vs_gt": ev["warm_rel_l2_error_vs_gt"],
+                          "time_ratio_warm_over_plain": ev["time_ratio_warm_over_plain"],
+                          "error_ratio_warm_over_plain": ev["final_error_ratio_warm_over_plain"],
+                          "rel_l2_diff_to_plain_result": ev["warm_vs_plain_result_rel_l2"]}}
+    net.close()
     ctx.close()
     return res
 

@@ -11,9 +11,9 @@
  *    that the pass cannot create or destroy (ASCENDANT; SEED-or-PROPOGATE),
  *    so concurrent in-place updates give the sequential result.
  *  - linear structure (LinearAccess.hpp:210-524): equivalence optimisation
- *    level by level, then per-64-cell-segment counts, one exclusive scan over
- *    all segments in particle order, and a wave-per-segment fill of y_vec
- *    using ballot prefix counts; xz_end_vec is read off the segment scan.
+ *    level by level, then per-row particle counts (a wave per row, 16 cells
+ *    per lane), one exclusive scan over all rows in particle order (which is
+ *    xz_end_vec), and a wave-per-row fill of y_vec with ballot prefix counts.
  *  - sampling (ParticleData.hpp:221-261): mean pyramid of the image, then a
  *    row-parallel gather.
  */
@@ -179,15 +179,12 @@ __global__ void k_upseed(uint8_t* P, Dims d, const uint8_t* __restrict__ Pp, Dim
     }
 }
 
-/* segment table: particle-order list of (level row, 64-cell segment) */
-constexpr int SEG = 64;
+/* row table of the APR levels l_min..l_max and the particle-cell maps that define them */
 constexpr int MAXL = 24;
-struct SegTable {
+struct RowTable {
     int n_levels;                 /* APR levels l_min..l_max           */
     int l_min, l_max;
-    int64_t seg_begin[MAXL + 1];  /* first segment id of each APR level (index l - l_min) */
     int64_t row_begin[MAXL];      /* xz_end index of row 0 of level    */
-    int64_t nseg_row[MAXL];       /* segments per row                  */
     int64_t rows_x[MAXL];         /* x_num of the APR level            */
     int64_t map_off[MAXL];        /* byte offset of the pct map used   */
     int64_t map_y[MAXL], map_x[MAXL]; /* dims of that pct map           */
@@ -197,7 +194,7 @@ struct SegTable {
 };
 
 /* weight of one pct cell for APR level li (0 or 1; 0..2 at l_max) */
-__device__ __forceinline__ int cell_weight(const SegTable& t, int li, uint8_t st, int64_t yy) {
+__device__ __forceinline__ int cell_weight(const RowTable& t, int li, uint8_t st, int64_t yy) {
     if (li == t.n_levels - 1) {
         if (st > 0 && st <= t.min_type) return 1 + ((2 * yy < t.y_max - 1) ? 1 : 0);
         return 0;
@@ -205,48 +202,100 @@ __device__ __forceinline__ int cell_weight(const SegTable& t, int li, uint8_t st
     return (st > t.min_type && st <= UPSEED) ? 1 : 0;
 }
 
-/* One wave per particle row (grid-stride): the row's 64-cell segments in order, one cell
-   per lane.  fill == 0: per-segment particle counts; fill == 1: y values written from the
-   scanned segment offsets with ballot prefix counts. */
-__global__ __launch_bounds__(256) void k_row_segments(const uint8_t* __restrict__ maps, SegTable t, int64_t nrows, uint32_t* cnt,
-                                                      const uint64_t* __restrict__ seg_off, uint16_t* y_vec, int fill) {
+/* pct row of APR row `row` of level index li (the max level reads row (z/2, x/2) of l_max - 1) */
+__device__ __forceinline__ const uint8_t* apr_row_cells(const RowTable& t, const uint8_t* maps, int li, int64_t row) {
+    int64_t prow = row;
+    if (li == t.n_levels - 1) {
+        const int64_t zm = row / t.rows_x[li], xm = row - zm * t.rows_x[li];
+        prow = (zm / 2) * t.map_x[li] + xm / 2;
+    }
+    return maps + t.map_off[li] + prow * t.map_y[li];
+}
+
+__device__ __forceinline__ int row_level_of(const RowTable& t, int64_t r) {
+    int li = 0;
+    while (li + 1 < t.n_levels && r >= t.row_prefix[li + 1]) ++li;
+    return li;
+}
+
+constexpr int CPL = 16;   /* cells per lane per chunk: a wave covers 1024 cells of a row */
+
+/* weights of the CPL cells [c0, c0 + CPL) of a row (0 beyond its end) */
+__device__ __forceinline__ void chunk_weights(const RowTable& t, int li, const uint8_t* p, int ylen, int c0, int w[CPL]) {
+    const bool top = (li == t.n_levels - 1);
+    if (c0 + CPL <= ylen && ((uintptr_t)(p + c0) & 15) == 0) {
+        const uint4 v = *(const uint4*)(p + c0);
+        const uint32_t words[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) w[j] = cell_weight(t, li, (uint8_t)(words[j >> 2] >> (8 * (j & 3))), c0 + j);
+    } else {
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) w[j] = (c0 + j < ylen) ? cell_weight(t, li, p[c0 + j], c0 + j) : 0;
+    }
+    (void)top;
+}
+
+/* particles of each APR row (one wave per row, 1024 cells per step) */
+__global__ __launch_bounds__(256) void k_row_count(const uint8_t* __restrict__ maps, RowTable t, int64_t nrows, uint32_t* cnt) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t r = wid; r < nrows; r += nw) {
+        const int li = row_level_of(t, r);
+        const uint8_t* p = apr_row_cells(t, maps, li, r - t.row_prefix[li]);
+        const int ylen = (int)t.map_y[li];
+        uint32_t c = 0;
+        for (int base = 0; base < ylen; base += 64 * CPL) {
+            int w[CPL];
+            chunk_weights(t, li, p, ylen, base + lane * CPL, w);
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) c += (uint32_t)w[j];
+        }
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+        if (lane == 0) cnt[r] = c;
+    }
+}
+
+/* y values of each APR row from its scanned offset: one cell per lane per step, positions
+   from ballot prefix counts, so consecutive lanes write consecutive y_vec entries */
+__global__ __launch_bounds__(256) void k_row_fill(const uint8_t* __restrict__ maps, RowTable t, int64_t nrows,
+                                                  const uint64_t* __restrict__ row_off, uint16_t* y_vec) {
     const int lane = threadIdx.x & 63;
     const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (int64_t r = wid; r < nrows; r += nw) {
-        int li = 0;
-        while (li + 1 < t.n_levels && r >= t.row_prefix[li + 1]) ++li;
-        const int64_t row = r - t.row_prefix[li];
-        const int64_t ylen = t.map_y[li];
+        const int li = row_level_of(t, r);
         const bool top = (li == t.n_levels - 1);
-        int64_t prow = row;
-        if (top) {
-            const int64_t zm = row / t.rows_x[li], xm = row - zm * t.rows_x[li];
-            prow = (zm / 2) * t.map_x[li] + xm / 2;
-        }
-        const uint8_t* p = maps + t.map_off[li] + prow * ylen;
-        const int nseg = (int)t.nseg_row[li];
-        const int64_t s0 = t.seg_begin[li] + row * nseg;
-        for (int c = 0; c < nseg; ++c) {
-            const int yy = c * SEG + lane;
+        const uint8_t* p = apr_row_cells(t, maps, li, r - t.row_prefix[li]);
+        const int ylen = (int)t.map_y[li];
+        uint64_t run = row_off[r];
+        for (int c0 = 0; c0 < ylen; c0 += 64) {
+            const int yy = c0 + lane;
             const int w = (yy < ylen) ? cell_weight(t, li, p[yy], yy) : 0;
             const uint64_t b1 = __ballot(w >= 1);
             const uint64_t b2 = __ballot(w >= 2);
-            if (!fill) {
-                if (lane == 0) cnt[s0 + c] = (uint32_t)(__popcll(b1) + __popcll(b2));
-                continue;
+            if (w) {
+                const uint64_t pos = run + __popcll(b1 & lt) + __popcll(b2 & lt);
+                if (top) {
+                    y_vec[pos] = (uint16_t)(2 * yy);
+                    if (w == 2) y_vec[pos + 1] = (uint16_t)(2 * yy + 1);
+                } else {
+                    y_vec[pos] = (uint16_t)yy;
+                }
             }
-            if (w == 0) continue;
-            const uint64_t pos = seg_off[s0 + c] + __popcll(b1 & lt) + __popcll(b2 & lt);
-            if (top) {
-                y_vec[pos] = (uint16_t)(2 * yy);
-                if (w == 2) y_vec[pos + 1] = (uint16_t)(2 * yy + 1);
-            } else {
-                y_vec[pos] = (uint16_t)yy;
-            }
+            run += (uint64_t)(__popcll(b1) + __popcll(b2));
         }
     }
+}
+
+/* xz_end_vec of each row: its offset + its count */
+__global__ void k_rows_xz(const uint64_t* __restrict__ row_off, const uint32_t* __restrict__ cnt, RowTable t, uint64_t* xz_end,
+                          int64_t nrows) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    const int li = row_level_of(t, r);
+    xz_end[t.row_begin[li] + (r - t.row_prefix[li])] = row_off[r] + cnt[r];
 }
 
 /* exclusive scan, three phases over blocks of 1024 */
@@ -293,25 +342,6 @@ __global__ __launch_bounds__(1024) void k_scan_sums(uint64_t* block_sums, int64_
 __global__ void k_scan_add(uint64_t* out, const uint64_t* __restrict__ block_sums, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] += block_sums[i / 1024];
-}
-
-/* xz_end_vec: inclusive count at the end of each row = offset of the row's last segment + its count */
-__global__ void k_xz_end(const uint64_t* __restrict__ seg_off, const uint32_t* __restrict__ cnt, SegTable t, uint64_t* xz_end,
-                         int64_t n_rows_total, const int64_t* __restrict__ dummy) {
-    (void)dummy;
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n_rows_total) return;
-    /* r enumerates rows of levels l_min..l_max in order */
-    int li = 0;
-    int64_t acc = 0;
-    for (; li < t.n_levels; ++li) {
-        const int64_t nrows = (t.seg_begin[li + 1] - t.seg_begin[li]) / t.nseg_row[li];
-        if (r < acc + nrows) break;
-        acc += nrows;
-    }
-    const int64_t row = r - acc;
-    const int64_t s_last = t.seg_begin[li] + row * t.nseg_row[li] + t.nseg_row[li] - 1;
-    xz_end[t.row_begin[li] + row] = seg_off[s_last] + cnt[s_last];
 }
 
 /* mean of the 2x2x2 block in the reference's summation order (PixelData.hpp:952-962), stored
@@ -367,7 +397,17 @@ __global__ __launch_bounds__(256) void k_gather(LevelPtrs<T> L, const uint64_t* 
         const uint64_t b = xz_end[xzi - 1], e = xz_end[xzi];
         const int64_t z = row / L.x[li], x = row % L.x[li];
         const T* plane = L.img[li] + (z * L.x[li] + x) * L.y[li];
-        for (uint64_t i = b + lane; i < e; i += 64) parts[i] = plane[y_vec[i]];
+        /* four independent particles per lane in flight */
+        uint64_t i = b + lane;
+        for (; i + 192 < e; i += 256) {
+            const uint16_t y0 = y_vec[i], y1 = y_vec[i + 64], y2 = y_vec[i + 128], y3 = y_vec[i + 192];
+            const T v0 = plane[y0], v1 = plane[y1], v2 = plane[y2], v3 = plane[y3];
+            parts[i] = v0;
+            parts[i + 64] = v1;
+            parts[i + 128] = v2;
+            parts[i + 192] = v3;
+        }
+        for (; i < e; i += 64) parts[i] = plane[y_vec[i]];
     }
 }
 
@@ -462,7 +502,7 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
         HIP_LAUNCH_CHECK();
     }
     /* segment table over APR levels lmin..lmax */
-    SegTable t;
+    RowTable t;
     std::memset(&t, 0, sizeof(t));
     t.l_min = lmin;
     t.l_max = lmax;
@@ -470,55 +510,34 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
     if (t.n_levels > MAXL) throw Error(APRH_ERR_UNSUPPORTED, "too many levels");
     t.min_type = min_type;
     t.y_max = a.y_num[lmax];
-    int64_t sb = 0, rp = 0;
+    int64_t rp = 0;
     for (int l = lmin; l <= lmax; ++l) {
         const int li = l - lmin;
         const int ml = (l == lmax) ? top : l;   /* pct map providing the cells */
-        t.seg_begin[li] = sb;
         t.row_begin[li] = (int64_t)a.level_xz_vec[l];
-        t.nseg_row[li] = cdiv(a.y_num[ml], SEG);
         t.rows_x[li] = a.x_num[l];
         t.map_off[li] = off[ml];
         t.map_y[li] = a.y_num[ml];
         t.map_x[li] = a.x_num[ml];
         t.row_prefix[li] = rp;
         rp += a.x_num[l] * a.z_num[l];
-        sb += a.x_num[l] * a.z_num[l] * t.nseg_row[li];
     }
-    t.seg_begin[t.n_levels] = sb;
     t.row_prefix[t.n_levels] = rp;
-    const int64_t nseg = sb;
-    Buf cnt(ctx->pool, (size_t)nseg * 4);
-    Buf segoff(ctx->pool, (size_t)nseg * 8);
-    const int64_t nb = cdiv(nseg, 1024);
-    Buf bsum(ctx->pool, (size_t)(nb + 1) * 8);
+    /* per-row counts, exclusive scan over rows in particle order, xz_end_vec, y_vec */
+    Buf cnt(ctx->pool, (size_t)rp * 4);
+    Buf rowoff(ctx->pool, (size_t)rp * 8);
     const dim3 rgrid((unsigned)std::min<int64_t>(cdiv(rp, 4), 16384));
-    hipLaunchKernelGGL(k_row_segments, rgrid, dim3(256), 0, st, P, t, rp, cnt.as<uint32_t>(), (const uint64_t*)nullptr,
-                       (uint16_t*)nullptr, 0);
+    hipLaunchKernelGGL(k_row_count, rgrid, dim3(256), 0, st, P, t, rp, cnt.as<uint32_t>());
     HIP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(1024), 0, st, cnt.as<uint32_t>(), segoff.as<uint64_t>(),
-                       bsum.as<uint64_t>(), nseg);
-    HIP_LAUNCH_CHECK();
-    uint64_t* d_total = (uint64_t*)ctx->d_flags + 8;
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, bsum.as<uint64_t>(), nb, d_total);
-    HIP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)cdiv(nseg, 256)), dim3(256), 0, st, segoff.as<uint64_t>(), bsum.as<uint64_t>(), nseg);
-    HIP_LAUNCH_CHECK();
-    /* xz_end_vec */
+    const uint64_t total = exclusive_scan_u32(ctx, cnt.as<uint32_t>(), rowoff.as<uint64_t>(), rp);
     a.xz_end.reset(ctx->pool, a.n_xz * 8);
     HIP_CHECK(hipMemsetAsync(a.xz_end.ptr(), 0, a.n_xz * 8, st));
-    int64_t nrows = 0;
-    for (int l = lmin; l <= lmax; ++l) nrows += a.x_num[l] * a.z_num[l];
-    hipLaunchKernelGGL(k_xz_end, dim3((unsigned)cdiv(nrows, 256)), dim3(256), 0, st, segoff.as<uint64_t>(), cnt.as<uint32_t>(), t,
-                       a.xz_end.as<uint64_t>(), nrows, (const int64_t*)nullptr);
+    hipLaunchKernelGGL(k_rows_xz, dim3((unsigned)cdiv(rp, 256)), dim3(256), 0, st, (const uint64_t*)rowoff.as<uint64_t>(),
+                       (const uint32_t*)cnt.as<uint32_t>(), t, a.xz_end.as<uint64_t>(), rp);
     HIP_LAUNCH_CHECK();
-    uint64_t total = 0;
-    HIP_CHECK(hipMemcpyAsync(&total, d_total, 8, hipMemcpyDeviceToHost, st));
-    HIP_CHECK(hipStreamSynchronize(st));
     a.total = total;
     a.y_vec.reset(ctx->pool, std::max<uint64_t>(total, 1) * 2);
-    hipLaunchKernelGGL(k_row_segments, rgrid, dim3(256), 0, st, P, t, rp, (uint32_t*)nullptr, (const uint64_t*)segoff.as<uint64_t>(),
-                       a.y_vec.as<uint16_t>(), 1);
+    hipLaunchKernelGGL(k_row_fill, rgrid, dim3(256), 0, st, P, t, rp, (const uint64_t*)rowoff.as<uint64_t>(), a.y_vec.as<uint16_t>());
     HIP_LAUNCH_CHECK();
     if (ctx->keep_intermediates) {
         a.pct.clear();

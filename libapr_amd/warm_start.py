@@ -117,8 +117,8 @@ def train(ctx, pairs, steps=2000, batch=8, lr=1e-3, seed=0, log=print):
     return net, losses
 
 
-def evaluate(ctx, net, shape_zxy=(256, 1024, 1024), seed=3, n_blobs=50_000, log=print):
-    w = Workload(ctx, shape_zxy, seed, n_blobs)
+def evaluate(ctx, net, shape_zxy=(256, 1024, 1024), seed=3, n_blobs=50_000, log=print, workload=None):
+    w = workload or Workload(ctx, shape_zxy, seed, n_blobs)
     ctx.set_conv_mode(A.CONV_SEPARABLE)
     u = A.DeviceArray(w.P * 4, ctx)
     w.rl(u, iters=1)                      # warm the pools and kernels
@@ -138,6 +138,7 @@ def evaluate(ctx, net, shape_zxy=(256, 1024, 1024), seed=3, n_blobs=50_000, log=
     uwh = uw.to_host(np.float32).astype(np.float64)
     diff = float(np.linalg.norm(uwh - up) / np.linalg.norm(up))
     res = {"particles": w.P, "plain_iterations": st_p["iterations"], "plain_time_s": round(t_plain, 4),
+           "plain_final_rel_update": st_p["last_rel_update"], "warm_final_rel_update": st_w["last_rel_update"],
            "plain_rel_l2_error_vs_gt": round(err_plain, 5), "warm_iterations": st_w["iterations"],
            "warm_time_s": round(t_warm, 4), "warm_network_time_s": round(t_cnn, 4),
            "warm_rel_l2_error_vs_gt": round(err_warm, 5), "time_ratio_warm_over_plain": round(t_warm / t_plain, 4),
