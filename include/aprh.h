@@ -105,6 +105,22 @@ int aprh_apr_intermediate(const aprh_apr* apr, int which, void* dst);
 int aprh_apr_particle_cell_tree(const aprh_apr* apr, int level, uint8_t* dst, int64_t* dims3);
 int aprh_apr_destroy(aprh_apr* apr);
 
+/* ---- batched conversion of many volumes of uneven sizes ----
+   APRConverter<uint16_t>::get_apr (APRConverter.hpp:534-600) over a fixed list of volumes,
+   recorded once into one HIP graph.  The plan owns each volume's u16 input buffer (write it
+   through the pointer from aprh_batch_input) and its output APR; aprh_batch_run replays the
+   graph and fills in the particle counts.  The APRs from aprh_batch_get are borrowed: valid
+   until the next aprh_batch_run or aprh_batch_destroy, never passed to aprh_apr_destroy.
+   Volumes need l_max >= 3 (a side above 4 pixels).  A batch has no reference counterpart:
+   the reference converts one image per call. */
+typedef struct aprh_batch aprh_batch;
+int aprh_batch_create(aprh_context* ctx, int32_t n_volumes, const int64_t* dims3 /* y, x, z per volume */,
+                      const aprh_params* par, aprh_batch** out);
+int aprh_batch_input(aprh_batch* batch, int32_t i, uint16_t** dev_image);
+int aprh_batch_run(aprh_context* ctx, aprh_batch* batch);
+int aprh_batch_get(aprh_batch* batch, int32_t i, aprh_apr** apr);
+int aprh_batch_destroy(aprh_batch* batch);
+
 /* ParticleData<uint16_t>::sample_image (ParticleData.hpp:122-127, :221-261) */
 int aprh_sample_image_u16(aprh_context* ctx, const aprh_apr* apr, const uint16_t* image, uint16_t* parts);
 /* ParticleData<float>::sample_image of a float image (same mean pyramid, float means) */

@@ -143,6 +143,11 @@ def lib():
     L.aprh_cnn_apply.argtypes = [vp, vp, vp, C.c_int, i64, i64, i64, C.c_float, vp]
     L.aprh_cnn_warm_start.argtypes = [vp, vp, vp, vp, C.c_int, vp]
     L.aprh_mean.argtypes = [vp, vp, C.c_int, C.c_uint64, C.POINTER(C.c_double)]
+    L.aprh_batch_create.argtypes = [vp, C.c_int32, vp, C.POINTER(AprhParams), C.POINTER(vp)]
+    L.aprh_batch_input.argtypes = [vp, C.c_int32, C.POINTER(vp)]
+    L.aprh_batch_run.argtypes = [vp, vp]
+    L.aprh_batch_get.argtypes = [vp, C.c_int32, C.POINTER(vp)]
+    L.aprh_batch_destroy.argtypes = [vp]
     L.aprh_malloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
     L.aprh_free.argtypes = [vp, vp]
     L.aprh_memcpy.argtypes = [vp, vp, vp, C.c_size_t]
@@ -251,8 +256,9 @@ def default_context():
 class APR:
     """Device-resident APR: GenInfo + LinearAccess (APR.hpp:18-61)."""
 
-    def __init__(self, ctx, handle):
+    def __init__(self, ctx, handle, owner=None):
         self.ctx, self.h = ctx, handle
+        self._owner = owner            # a batch plan that owns the handle (never destroyed here)
         info = AprhAprInfo()
         _check(lib().aprh_apr_info_get(handle, C.byref(info)))
         self.l_min, self.l_max = info.l_min, info.l_max
@@ -342,7 +348,8 @@ class APR:
 
     def close(self):
         if getattr(self, "h", None):
-            lib().aprh_apr_destroy(self.h)
+            if self._owner is None:
+                lib().aprh_apr_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -385,6 +392,58 @@ class APRConverter:
                                             _ptr(np.ascontiguousarray(intensity_ds, np.float32)), y, x, z, C.byref(p),
                                             bspline_offset, C.byref(h)))
         return APR(self.ctx, h)
+
+
+class APRBatch:
+    """Conversion of a fixed list of u16 volumes of uneven sizes, recorded once into one HIP graph
+    (APRConverter<uint16_t>::get_apr per volume, APRConverter.hpp:534-600).
+
+    dims: list of (y, x, z).  set_input(i, image) writes volume i ([z, x, y] numpy u16 or a device
+    address); run() replays the graph; get(i) returns volume i's APR (borrowed from the plan: valid
+    until the next run() or close())."""
+
+    def __init__(self, dims, par=None, ctx=None):
+        self.ctx = ctx or default_context()
+        self.dims = [tuple(int(v) for v in d) for d in dims]
+        self.par = par or APRParameters()
+        d = np.ascontiguousarray(np.array(self.dims, np.int64).reshape(-1))
+        p = self.par.c()
+        h = vp()
+        _check(lib().aprh_batch_create(self.ctx.h, len(self.dims), _ptr(d), C.byref(p), C.byref(h)))
+        self.h = h
+        self.voxels = int(sum(y * x * z for y, x, z in self.dims))
+
+    def input_address(self, i):
+        out = vp()
+        _check(lib().aprh_batch_input(self.h, i, C.byref(out)))
+        return out.value
+
+    def set_input(self, i, image):
+        y, x, z = self.dims[i]
+        if isinstance(image, np.ndarray):
+            if image.dtype != np.uint16 or image.shape != (z, x, y):
+                raise ValueError(f"volume {i} must be uint16 of shape {(z, x, y)}")
+            image = np.ascontiguousarray(image)
+        _check(lib().aprh_memcpy(self.ctx.h, self.input_address(i), _ptr(image), 2 * y * x * z))
+
+    def run(self):
+        _check(lib().aprh_batch_run(self.ctx.h, self.h))
+
+    def get(self, i):
+        out = vp()
+        _check(lib().aprh_batch_get(self.h, i, C.byref(out)))
+        return APR(self.ctx, out, owner=self)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().aprh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class ParticleData:

@@ -123,6 +123,12 @@ struct aprh_context {
     float stage_ms[aprh::ST_COUNT] = {};
     int* d_flags = nullptr;   /* small device scratch: overflow counters etc. */
     int conv_mode = 0;        /* APRH_CONV_EXACT / APRH_CONV_SEPARABLE */
+    /* asynchronous conversion (graph capture of a batch): no host synchronisation, no
+       memset nodes; per-volume results left in device slots; y_vec / xz_end preallocated */
+    int async_mode = 0;
+    uint64_t* slot_total = nullptr;   /* particle count                        */
+    float* slot_offset = nullptr;     /* B-spline offset                       */
+    int* slot_err = nullptr;          /* 3 IIR overflow counters (y, x, z)     */
 };
 
 /* device-resident APR: GenInfo + LinearAccess (+ optional intermediates) */
@@ -136,6 +142,7 @@ struct aprh_apr {
     uint64_t n_xz = 0;                             /* size of xz_end_vec        */
     float bspline_offset = 0;
     int image_dtype = 0;                           /* APRH_DTYPE_U16 / _F32 of the converted image */
+    bool borrowed = false;                         /* owned by a batch plan     */
     aprh::Buf xz_end;                              /* u64[n_xz]                 */
     aprh::Buf y_vec;                               /* u16[total]                */
     /* optional stage outputs (context keep_intermediates) */
@@ -221,6 +228,10 @@ void launch_sample_f32(aprh_context* ctx, const aprh_apr& a, const float* img, f
 
 /* exclusive scan of n u32 counts into u64 offsets; returns the total (synchronises the stream) */
 uint64_t exclusive_scan_u32(aprh_context* ctx, const uint32_t* in, uint64_t* out, int64_t n);
+/* the same, total left in device memory at d_total, no synchronisation */
+void exclusive_scan_u32_async(aprh_context* ctx, const uint32_t* in, uint64_t* out, int64_t n, uint64_t* d_total);
+/* zero device memory with a kernel (graph-capture safe, unlike a memset node) */
+void launch_zero(aprh_context* ctx, void* p, size_t bytes);
 
 /* ---- particle operations (k_particles.hip) ---- */
 struct HostStencil {

@@ -74,7 +74,7 @@ void get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x,
     float* d_off = (float*)ctx->d_flags;
     uint32_t* d_mm = (uint32_t*)ctx->d_flags + 2;
     int* d_err = ctx->d_flags + 4;
-    HIP_CHECK(hipMemsetAsync(d_err, 0, 3 * sizeof(int), st));
+    launch_zero(ctx, d_err, 3 * sizeof(int));
     /* computeL: bspline offset (APRConverter.hpp:213-224) */
     launch_minmax_offset(ctx, (const uint16_t*)in.ptr, n, par.lambda, d_off, d_mm);
     mark(ctx, ST_SMOOTH_Y);
@@ -97,7 +97,10 @@ void get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x,
     Buf mean(ctx->pool, nd * 4);
     launch_grad_ds_u16(ctx, I, grad.as<uint16_t>(), mean.as<float>(), y, x, z, par.dx, par.dy, par.dz, false);
     int herr[3] = {0, 0, 0};
-    HIP_CHECK(hipMemcpyAsync(herr, d_err, sizeof(herr), hipMemcpyDeviceToHost, st));
+    if (ctx->async_mode)
+        HIP_CHECK(hipMemcpyAsync(ctx->slot_err, d_err, sizeof(herr), hipMemcpyDeviceToDevice, st));
+    else
+        HIP_CHECK(hipMemcpyAsync(herr, d_err, sizeof(herr), hipMemcpyDeviceToHost, st));
     if (ctx->keep_intermediates) {
         keep_copy(ctx, a->smoothed, I, n * 2);
         keep_copy(ctx, a->grad_raw, grad.ptr(), nd * 2);
@@ -108,7 +111,7 @@ void get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x,
         Buf tmp(ctx->pool, nd * 4);
         launch_inv_bspline(ctx, mean.as<float>(), tmp.as<float>(), yd, xd, zd, y > 2, x > 2, z > 2);
     }
-    HIP_CHECK(hipStreamSynchronize(st));
+    if (!ctx->async_mode) HIP_CHECK(hipStreamSynchronize(st));
     static const char* axis_name[3] = {"y", "x", "z"};
     for (int i = 0; i < 3; ++i)
         if (herr[i])
@@ -147,6 +150,10 @@ void lis_and_solve(aprh_context* ctx, aprh_apr* a, const aprh_params& par, Buf& 
     mark(ctx, ST_LEVELS);
     solve<G>(ctx, *a, par, Ap, lpa, mean.as<float>(), grad, d_off);
     mark(ctx, ST_SAMPLE);
+    if (ctx->async_mode) {
+        HIP_CHECK(hipMemcpyAsync(ctx->slot_offset, d_off, 4, hipMemcpyDeviceToDevice, st));
+        return;
+    }
     HIP_CHECK(hipMemcpyAsync(&a->bspline_offset, d_off, 4, hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
     collect_times(ctx, ST_OFFSET, ST_SAMPLE);
@@ -167,7 +174,7 @@ void get_apr_f32(aprh_context* ctx, const float* image, int64_t y, int64_t x, in
     mark(ctx, ST_OFFSET);
     float* d_off = (float*)ctx->d_flags;
     int* d_err = ctx->d_flags + 4;
-    HIP_CHECK(hipMemsetAsync(ctx->d_flags, 0, 8 * sizeof(int), st));   /* offset 0.0f, error counters 0 */
+    launch_zero(ctx, ctx->d_flags, 8 * sizeof(int));   /* offset 0.0f, error counters 0 */
     mark(ctx, ST_SMOOTH_Y);
     Buf img(ctx->pool, n * 4);
     float* I = img.as<float>();
@@ -198,6 +205,14 @@ void get_apr_f32(aprh_context* ctx, const float* image, int64_t y, int64_t x, in
 }
 
 }  // namespace
+
+namespace aprh {
+/* the u16 pipeline for callers in other translation units (the batched plan) */
+void get_apr_u16_dispatch(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x, int64_t z, const aprh_params& par,
+                          aprh_apr* a) {
+    get_apr_u16(ctx, image, y, x, z, par, a);
+}
+}  // namespace aprh
 
 extern "C" {
 
@@ -340,6 +355,7 @@ int aprh_apr_particle_cell_tree(const aprh_apr* a, int level, uint8_t* dst, int6
 int aprh_apr_destroy(aprh_apr* a) {
     APRH_TRY
     if (!a) return APRH_OK;
+    if (a->borrowed) throw Error(APRH_ERR_INVALID_ARGUMENT, "this APR belongs to a batch plan (aprh_batch_destroy frees it)");
     if (a->ctx) {
         (void)hipSetDevice(a->ctx->device);
         (void)hipStreamSynchronize(a->ctx->stream);

@@ -194,6 +194,31 @@ IIRCoeffs iir_coeffs(float lambda, float tol, int64_t len) {
 
 }  // namespace aprh
 
+/* ---------------- zero fill ---------------- */
+namespace aprh {
+namespace {
+__global__ void k_zero_words(uint32_t* p, size_t nw) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+__global__ void k_zero_bytes(uint8_t* p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
+}
+}  // namespace
+
+void launch_zero(aprh_context* ctx, void* p, size_t bytes) {
+    if (!bytes) return;
+    if (((uintptr_t)p & 3) == 0 && (bytes & 3) == 0) {
+        const size_t nw = bytes / 4;
+        hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<size_t>((nw + 255) / 256, 4096)), dim3(256), 0, ctx->stream,
+                           (uint32_t*)p, nw);
+    } else {
+        hipLaunchKernelGGL(k_zero_bytes, dim3((unsigned)std::min<size_t>((bytes + 255) / 256, 4096)), dim3(256), 0, ctx->stream,
+                           (uint8_t*)p, bytes);
+    }
+    HIP_LAUNCH_CHECK();
+}
+}  // namespace aprh
+
 /* ---------------- synthetic volumes ---------------- */
 namespace aprh {
 

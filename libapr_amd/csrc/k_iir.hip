@@ -54,6 +54,13 @@ __device__ __forceinline__ float load_in(const T* p, float off, bool has_off) {
     return (float)(*p);
 }
 
+__global__ void k_minmax_init(uint32_t* mm) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        mm[0] = 65535u;
+        mm[1] = 0u;
+    }
+}
+
 __global__ void k_minmax(const uint16_t* img, size_t n, uint32_t* mm) {
     uint32_t lo = 65535u, hi = 0u;
     /* 16-byte loads over the aligned body, scalar tail */
@@ -676,8 +683,9 @@ inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 }  // namespace
 
 void launch_minmax_offset(aprh_context* ctx, const uint16_t* img, size_t n, float lambda, float* d_offset, uint32_t* d_mm) {
-    const uint32_t init[2] = {65535u, 0u};
-    HIP_CHECK(hipMemcpyAsync(d_mm, init, 8, hipMemcpyHostToDevice, ctx->stream));
+    /* initial (min, max) set by a kernel, so the sequence can be captured in a graph */
+    hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(1), 0, ctx->stream, d_mm);
+    HIP_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_minmax, dim3(2048), dim3(256), 0, ctx->stream, img, n, d_mm);
     HIP_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_offset, dim3(1), dim3(1), 0, ctx->stream, d_mm, lambda, d_offset);

@@ -413,18 +413,27 @@ __global__ __launch_bounds__(256) void k_gather(LevelPtrs<T> L, const uint64_t* 
 
 }  // namespace
 
-uint64_t exclusive_scan_u32(aprh_context* ctx, const uint32_t* in, uint64_t* out, int64_t n) {
-    if (n <= 0) return 0;
+void exclusive_scan_u32_async(aprh_context* ctx, const uint32_t* in, uint64_t* out, int64_t n, uint64_t* d_total) {
     hipStream_t st = ctx->stream;
+    if (n <= 0) {
+        launch_zero(ctx, d_total, 8);
+        return;
+    }
     const int64_t nb = cdiv(n, 1024);
     Buf bsum(ctx->pool, (size_t)(nb + 1) * 8);
     hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(1024), 0, st, in, out, bsum.as<uint64_t>(), n);
     HIP_LAUNCH_CHECK();
-    uint64_t* d_total = (uint64_t*)ctx->d_flags + 8;
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, bsum.as<uint64_t>(), nb, d_total);
     HIP_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_scan_add, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st, out, bsum.as<uint64_t>(), n);
     HIP_LAUNCH_CHECK();
+}
+
+uint64_t exclusive_scan_u32(aprh_context* ctx, const uint32_t* in, uint64_t* out, int64_t n) {
+    if (n <= 0) return 0;
+    hipStream_t st = ctx->stream;
+    uint64_t* d_total = (uint64_t*)ctx->d_flags + 8;
+    exclusive_scan_u32_async(ctx, in, out, n, d_total);
     uint64_t total = 0;
     HIP_CHECK(hipMemcpyAsync(&total, d_total, 8, hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
@@ -436,6 +445,7 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
     const int lmax = a.l_max, lmin = a.l_min;
     const dim3 grid(4096), block(256);
     if (lmax <= 2) {
+        if (ctx->async_mode) throw Error(APRH_ERR_UNSUPPORTED, "batched conversion of volumes with l_max <= 2");
         /* LinearAccess.hpp:229-260: every pixel is a particle at l_max */
         const int64_t yL = a.y_num[lmax], xL = a.x_num[lmax], zL = a.z_num[lmax];
         std::vector<uint64_t> xz(a.n_xz, 0);
@@ -529,14 +539,19 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
     const dim3 rgrid((unsigned)std::min<int64_t>(cdiv(rp, 4), 16384));
     hipLaunchKernelGGL(k_row_count, rgrid, dim3(256), 0, st, P, t, rp, cnt.as<uint32_t>());
     HIP_LAUNCH_CHECK();
-    const uint64_t total = exclusive_scan_u32(ctx, cnt.as<uint32_t>(), rowoff.as<uint64_t>(), rp);
-    a.xz_end.reset(ctx->pool, a.n_xz * 8);
-    HIP_CHECK(hipMemsetAsync(a.xz_end.ptr(), 0, a.n_xz * 8, st));
+    if (ctx->async_mode) {
+        /* y_vec and xz_end_vec are preallocated (y_vec for the upper bound P <= N); the count
+           stays on the device */
+        exclusive_scan_u32_async(ctx, cnt.as<uint32_t>(), rowoff.as<uint64_t>(), rp, ctx->slot_total);
+    } else {
+        a.total = exclusive_scan_u32(ctx, cnt.as<uint32_t>(), rowoff.as<uint64_t>(), rp);
+        a.xz_end.reset(ctx->pool, a.n_xz * 8);
+        a.y_vec.reset(ctx->pool, std::max<uint64_t>(a.total, 1) * 2);
+    }
+    launch_zero(ctx, a.xz_end.ptr(), a.n_xz * 8);
     hipLaunchKernelGGL(k_rows_xz, dim3((unsigned)cdiv(rp, 256)), dim3(256), 0, st, (const uint64_t*)rowoff.as<uint64_t>(),
                        (const uint32_t*)cnt.as<uint32_t>(), t, a.xz_end.as<uint64_t>(), rp);
     HIP_LAUNCH_CHECK();
-    a.total = total;
-    a.y_vec.reset(ctx->pool, std::max<uint64_t>(total, 1) * 2);
     hipLaunchKernelGGL(k_row_fill, rgrid, dim3(256), 0, st, P, t, rp, (const uint64_t*)rowoff.as<uint64_t>(), a.y_vec.as<uint16_t>());
     HIP_LAUNCH_CHECK();
     if (ctx->keep_intermediates) {
@@ -552,7 +567,7 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
             a.pct_dims[3 * l + 2] = a.z_num[l];
         }
     }
-    HIP_CHECK(hipStreamSynchronize(st));
+    if (!ctx->async_mode) HIP_CHECK(hipStreamSynchronize(st));
 }
 
 template <typename T>
