@@ -65,7 +65,26 @@ def algorithmic_bytes(stage, N, Nd, P, Npad):
     }.get(stage)
 
 
-def run_gpu(args):
+def run_copy(args):
+    """device-to-device copy of 4 GiB (steps times): the known-size kernel that calibrates FETCH_SIZE /
+    WRITE_SIZE (each copy reads and writes 4 GiB)"""
+    import libapr_amd as A
+    ctx = A.Context(0)
+    nbytes = 4 << 30
+    a, b = _dev(A, ctx, nbytes), _dev(A, ctx, nbytes)
+    A._check(A.lib().aprh_memcpy(ctx.h, b, a, nbytes))
+    t = time.perf_counter()
+    for _ in range(max(args.steps, 1)):
+        A._check(A.lib().aprh_memcpy(ctx.h, b, a, nbytes))
+    ms = (time.perf_counter() - t) * 1000.0 / max(args.steps, 1)
+    for p in (a, b):
+        A._check(A.lib().aprh_free(ctx.h, p))
+    ctx.close()
+    log(f"[bench] copy of 4 GiB: {ms:.3f} ms, {2 * nbytes / (ms / 1000.0) / 1e9:.0f} GB/s read+write")
+    return {"bytes": nbytes, "ms": round(ms, 3), "GBps_read_plus_write": round(2 * nbytes / (ms / 1000.0) / 1e9, 1)}
+
+
+def run_gpu(args, conv=True):
     import libapr_amd as A
     ctx = A.Context(0, timing=True)
     L = A.lib()
@@ -135,6 +154,12 @@ def run_gpu(args):
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "algorithmic_bytes": ab,
                 "avg_ms": round(cands[dom], 4)}
     total_ab = 23 * N + 6 * P
+    if not conv:
+        last.pop("apr").close()
+        A._check(L.aprh_free(ctx.h, d_img))
+        A._check(L.aprh_free(ctx.h, d_parts))
+        ctx.close()
+        return ms_per_step, P, stage_avg, roofline, total_ab, None
     # 5x5x5 Gaussian (sigma 1) APR convolution on these particles (u16 in, f32 out)
     apr = last.pop("apr")
     A._check(L.aprh_free(ctx.h, d_img))
@@ -218,7 +243,7 @@ def batch_dims():
     return [tuple(int(v) for v in r) for r in d]
 
 
-def run_batch(args):
+def run_batch(args, check=True):
     """Batched uneven volumes: every volume's get_apr recorded into one HIP graph (libapr_amd.APRBatch),
     timed over replays; each volume's structure is compared with a single-volume get_apr of the same
     device image (bit-exact), and the single-volume loop is timed for comparison."""
@@ -243,6 +268,15 @@ def run_batch(args):
         batch.run()
     ms = (time.perf_counter() - t) * 1000.0 / reps
     log(f"[bench] batch graph: {ms:.2f} ms per replay, {nvox / (ms / 1000.0) / 1e9:.2f} Gvoxels/s")
+    stages = batch.stages()
+    log(f"[bench] batch graph holds {len(stages)} stage launches over {len(dims)} volumes")
+    for st in sorted(stages, key=lambda r: -r["tuned_ms"])[:8]:
+        log(f"[bench]   {st['name']:22s} vols={st['volumes']:3d} tiles={st['tiles']:9d} grid={st['grid']:8d} {st['tuned_ms']:.3f} ms")
+    if not check:
+        batch.close()
+        ctx.close()
+        return {"voxels": nvox, "ms_per_batch": round(ms, 3), "Gvoxels_per_s": round(nvox / (ms / 1000.0) / 1e9, 3),
+                "stage_launches": len(stages)}
     conv = A.APRConverter(ctx)
     conv.par = par
     exact = 0
@@ -271,7 +305,9 @@ def run_batch(args):
                         "1 blob per 32^3 voxels (sigma 1.5-8, amp 300-4000), bg 100 + shot noise, seed 2",
             "voxels": nvox, "particles": total_p, "ms_per_batch": round(ms, 3),
             "Gvoxels_per_s": round(nvox / (ms / 1000.0) / 1e9, 3),
-            "single_volume_loop_ms": round(seq_ms, 3), "bit_exact_volumes": exact, "volumes": len(dims)}
+            "single_volume_loop_ms": round(seq_ms, 3), "bit_exact_volumes": exact, "volumes": len(dims),
+            "stage_launches": len(stages),
+            "autotuned_stages": [{k: st[k] for k in ("name", "volumes", "tiles", "grid", "tuned_ms")} for st in stages]}
 
 
 RL_SHAPE_ZXY = (256, 1024, 1024)
@@ -435,11 +471,19 @@ def main():
     ap.add_argument("--no-rl", action="store_true")
     ap.add_argument("--conv-reps", type=int, default=3)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--only", choices=["get_apr", "batch", "copy"],
+                    help="run one part (for profiling): the headline conversion steps, the batched volumes, "
+                         "or the device copy of 4 GiB that calibrates the HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
               file=sys.stderr)
         sys.exit(2)
+    if args.only:
+        part = {"get_apr": lambda: run_gpu(args, conv=False)[:4], "batch": lambda: run_batch(args, check=False),
+                "copy": lambda: run_copy(args)}[args.only]()
+        print(json.dumps({"only": args.only, "result": part}), flush=True)
+        return
     ms, P, stages, roofline, total_ab, conv5 = run_gpu(args)
     z, x, y = SHAPE_ZXY
     N = x * y * z

@@ -120,6 +120,19 @@ int aprh_batch_input(aprh_batch* batch, int32_t i, uint16_t** dev_image);
 int aprh_batch_run(aprh_context* ctx, aprh_batch* batch);
 int aprh_batch_get(aprh_batch* batch, int32_t i, aprh_apr** apr);
 int aprh_batch_destroy(aprh_batch* batch);
+/* the plan's merged stage launches (one per stage over all volumes that take part) and the
+   grid the autotuner chose for each: one block per tile, or 1-16 blocks per CU walking the
+   tiles persistently, whichever ran fastest at plan creation */
+typedef struct aprh_batch_stage {
+    const char* name;     /* stage body (static string) */
+    int32_t volumes;      /* volumes in this launch     */
+    int64_t tiles;        /* work tiles over them       */
+    uint32_t grid;        /* blocks launched            */
+    uint32_t threads;     /* threads per block          */
+    float tuned_ms;       /* time at the chosen grid    */
+} aprh_batch_stage;
+int aprh_batch_stage_count(const aprh_batch* batch);
+int aprh_batch_stage_info(const aprh_batch* batch, int32_t i, aprh_batch_stage* info);
 
 /* ParticleData<uint16_t>::sample_image (ParticleData.hpp:122-127, :221-261) */
 int aprh_sample_image_u16(aprh_context* ctx, const aprh_apr* apr, const uint16_t* image, uint16_t* parts);

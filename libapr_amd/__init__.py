@@ -45,6 +45,11 @@ class AprhOverflowError(AprhError, ValueError):
     (ComputeGradient.hpp:409-412)."""
 
 
+class AprhBatchStage(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("volumes", C.c_int32), ("tiles", C.c_int64), ("grid", C.c_uint32),
+                ("threads", C.c_uint32), ("tuned_ms", C.c_float)]
+
+
 class AprhParams(C.Structure):
     _fields_ = [(n, C.c_float) for n in ("dx", "dy", "dz", "psfx", "psfy", "psfz", "Ip_th", "lambda_", "rel_error",
                                          "sigma_th", "sigma_th_max", "grad_th")] + \
@@ -148,6 +153,8 @@ def lib():
     L.aprh_batch_run.argtypes = [vp, vp]
     L.aprh_batch_get.argtypes = [vp, C.c_int32, C.POINTER(vp)]
     L.aprh_batch_destroy.argtypes = [vp]
+    L.aprh_batch_stage_count.argtypes = [vp]
+    L.aprh_batch_stage_info.argtypes = [vp, C.c_int32, C.POINTER(AprhBatchStage)]
     L.aprh_malloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
     L.aprh_free.argtypes = [vp, vp]
     L.aprh_memcpy.argtypes = [vp, vp, vp, C.c_size_t]
@@ -428,6 +435,16 @@ class APRBatch:
 
     def run(self):
         _check(lib().aprh_batch_run(self.ctx.h, self.h))
+
+    def stages(self):
+        """the merged stage launches with the autotuned grid of each"""
+        out = []
+        for i in range(lib().aprh_batch_stage_count(self.h)):
+            st = AprhBatchStage()
+            _check(lib().aprh_batch_stage_info(self.h, i, C.byref(st)))
+            out.append({"name": st.name.decode(), "volumes": st.volumes, "tiles": st.tiles, "grid": st.grid,
+                        "threads": st.threads, "tuned_ms": round(st.tuned_ms, 4)})
+        return out
 
     def get(self, i):
         out = vp()

@@ -48,6 +48,9 @@ public:
     void put(void* p);
     void release_all();
     size_t bytes_cached() const { return cached_; }
+    /* keep mode: returned blocks stay allocated and are never handed out again (a batch
+       plan records every volume's stages before any of them runs) */
+    void set_keep(bool k) { keep_ = k; }
 
 private:
     static size_t bucket(size_t b);
@@ -55,6 +58,7 @@ private:
     std::multimap<size_t, void*> free_;
     std::map<void*, size_t> live_;
     size_t cached_ = 0;
+    bool keep_ = false;
 };
 
 /* device buffer leased from a Pool; returned on destruction */
@@ -112,6 +116,8 @@ enum Stage {
 
 }  // namespace aprh
 
+namespace aprh { struct TileRecorder; }
+
 struct aprh_context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -129,6 +135,8 @@ struct aprh_context {
     uint64_t* slot_total = nullptr;   /* particle count                        */
     float* slot_offset = nullptr;     /* B-spline offset                       */
     int* slot_err = nullptr;          /* 3 IIR overflow counters (y, x, z)     */
+    /* recording mode of a batch plan: stage launches are appended here instead of run */
+    aprh::TileRecorder* recorder = nullptr;
 };
 
 /* device-resident APR: GenInfo + LinearAccess (+ optional intermediates) */
@@ -232,6 +240,8 @@ uint64_t exclusive_scan_u32(aprh_context* ctx, const uint32_t* in, uint64_t* out
 void exclusive_scan_u32_async(aprh_context* ctx, const uint32_t* in, uint64_t* out, int64_t n, uint64_t* d_total);
 /* zero device memory with a kernel (graph-capture safe, unlike a memset node) */
 void launch_zero(aprh_context* ctx, void* p, size_t bytes);
+/* device-to-device copy with a kernel (recordable by a batch plan) */
+void launch_copy(aprh_context* ctx, void* dst, const void* src, size_t bytes);
 
 /* ---- particle operations (k_particles.hip) ---- */
 struct HostStencil {

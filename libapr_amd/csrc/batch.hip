@@ -1,19 +1,29 @@
 /*
- * batch.hip -- conversion of many volumes of uneven sizes as one HIP graph.
+ * batch.hip -- conversion of many volumes of uneven sizes: one persistent
+ * launch per stage over all volumes, the whole sequence in one HIP graph.
  *
- * A plan owns, for every volume, the input image buffer and the output
- * structure (xz_end_vec, and y_vec sized for the bound P <= N), plus a private
- * caching pool for all pipeline scratch.  The whole sequence of conversions is
- * run once eagerly (which fills the pool with every scratch block the graph
- * will address), then captured into a single hipGraph: the pipeline runs in its
- * asynchronous mode (no host synchronisation, no memset nodes, per-volume
- * particle counts, B-spline offsets and overflow counters left in device
- * slots).  aprh_batch_run replays the graph and reads the slots back.
- *
- * Volumes run one after the other inside the graph (they share the context's
- * flag words and, through the pool, scratch blocks), so the graph is a chain.
+ * Plan creation
+ *  1. records the u16 pipeline of every volume (the same code as a single
+ *     get_apr, with the context in recording mode): each stage launch becomes
+ *     (body, arguments, tiles) in the volume's list; buffers come from the
+ *     plan's pool in keep mode, so every volume owns all of its scratch;
+ *  2. merges the lists stage by stage: the next merged launch takes the body
+ *     that heads the most lists and the volumes whose head it is, so every
+ *     volume's order is kept and volumes with fewer levels simply sit out the
+ *     extra pulling-scheme launches;
+ *  3. autotunes each merged launch: the whole sequence runs once per
+ *     candidate grid (one block per tile, or 1, 2, 4, 8, 16 blocks per CU
+ *     walking the tiles persistently), each launch timed with events, and
+ *     the fastest grid of each launch kept;
+ *  4. captures the tuned sequence into one graph.
+ * aprh_batch_run replays the graph and reads each volume's particle count,
+ * B-spline offset and overflow flags from device slots.
  */
+#include <algorithm>
+#include <map>
+
 #include "aprh_internal.hpp"
+#include "tiles.hpp"
 
 namespace aprh {
 void get_apr_u16_dispatch(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x, int64_t z, const aprh_params& par,
@@ -22,21 +32,40 @@ void get_apr_u16_dispatch(aprh_context* ctx, const uint16_t* image, int64_t y, i
 
 using namespace aprh;
 
+namespace {
+
+struct Group {
+    const TileRec* proto = nullptr;           /* body, block size, launcher, name */
+    std::vector<const TileRec*> recs;         /* one per volume, in launch order  */
+    int64_t tiles = 0;
+    Buf d_args, d_prefix;
+    unsigned grid = 0;
+    float ms = 0.0f;
+};
+
+constexpr int N_CAND = 6;   /* one block per tile; 1, 2, 4, 8, 16 blocks per CU */
+
+}  // namespace
+
 struct aprh_batch {
     aprh_context* ctx = nullptr;
     int n = 0;
     std::vector<int64_t> dims;                 /* y, x, z per volume        */
     aprh_params par{};
-    Pool* pool = nullptr;                      /* scratch + outputs         */
+    Pool* pool = nullptr;                      /* outputs + all scratch (keep mode) */
     std::vector<Buf*> inputs;                  /* u16 images                */
     std::vector<aprh_apr*> aprs;               /* results (owned)           */
     Buf slots;                                 /* u64 totals | f32 offsets | int errs[3] */
+    std::vector<TileRecorder> recs;
+    std::vector<Group> groups;
+    int cus = 256;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint64_t voxels = 0;
     ~aprh_batch() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
+        groups.clear();
         for (auto* a : aprs) delete a;
         for (auto* b : inputs) delete b;
         slots.release();
@@ -50,47 +79,131 @@ uint64_t* slot_totals(aprh_batch* b) { return b->slots.as<uint64_t>(); }
 float* slot_offsets(aprh_batch* b) { return (float*)(b->slots.as<uint64_t>() + b->n); }
 int* slot_errs(aprh_batch* b) { return (int*)(slot_offsets(b) + b->n); }
 
-/* every conversion of the plan, in order, on the context's stream */
+/* step 1: every volume's stage launches, recorded with the plan's pool in keep mode */
 void record(aprh_batch* b) {
-    aprh_context* ctx = b->ctx;
-    for (int i = 0; i < b->n; ++i) {
-        ctx->slot_total = slot_totals(b) + i;
-        ctx->slot_offset = slot_offsets(b) + i;
-        ctx->slot_err = slot_errs(b) + 3 * i;
-        get_apr_u16_dispatch(ctx, b->inputs[i]->as<uint16_t>(), b->dims[3 * i], b->dims[3 * i + 1], b->dims[3 * i + 2], b->par,
-                             b->aprs[i]);
-    }
-}
-
-/* run `f` with the context switched to the plan's pool and asynchronous mode */
-template <typename F>
-void in_plan(aprh_batch* b, F f) {
     aprh_context* ctx = b->ctx;
     Pool* saved = ctx->pool;
     const int saved_timing = ctx->timing, saved_keep = ctx->keep_intermediates;
-    ctx->pool = b->pool;
-    ctx->async_mode = 1;
-    ctx->timing = 0;
-    ctx->keep_intermediates = 0;
-    try {
-        f();
-    } catch (...) {
+    auto restore = [&] {
         ctx->pool = saved;
         ctx->async_mode = 0;
         ctx->timing = saved_timing;
         ctx->keep_intermediates = saved_keep;
+        ctx->recorder = nullptr;
         ctx->slot_total = nullptr;
         ctx->slot_offset = nullptr;
         ctx->slot_err = nullptr;
+    };
+    ctx->pool = b->pool;
+    ctx->async_mode = 1;
+    ctx->timing = 0;
+    ctx->keep_intermediates = 0;
+    b->recs.assign(b->n, TileRecorder());
+    try {
+        for (int i = 0; i < b->n; ++i) {
+            ctx->recorder = &b->recs[i];
+            ctx->slot_total = slot_totals(b) + i;
+            ctx->slot_offset = slot_offsets(b) + i;
+            ctx->slot_err = slot_errs(b) + 3 * i;
+            get_apr_u16_dispatch(ctx, b->inputs[i]->as<uint16_t>(), b->dims[3 * i], b->dims[3 * i + 1], b->dims[3 * i + 2], b->par,
+                                 b->aprs[i]);
+        }
+    } catch (...) {
+        restore();
         throw;
     }
-    ctx->pool = saved;
-    ctx->async_mode = 0;
-    ctx->timing = saved_timing;
-    ctx->keep_intermediates = saved_keep;
-    ctx->slot_total = nullptr;
-    ctx->slot_offset = nullptr;
-    ctx->slot_err = nullptr;
+    restore();
+}
+
+/* step 2: merge the per-volume lists into launches */
+void merge(aprh_batch* b) {
+    std::vector<size_t> head(b->n, 0);
+    for (;;) {
+        std::map<const std::type_info*, int> count;
+        const std::type_info* best = nullptr;
+        int best_n = 0;
+        for (int v = 0; v < b->n; ++v) {
+            if (head[v] >= b->recs[v].recs.size()) continue;
+            const std::type_info* t = b->recs[v].recs[head[v]].body;
+            const int c = ++count[t];
+            if (c > best_n) {
+                best_n = c;
+                best = t;
+            }
+        }
+        if (!best) break;
+        Group g;
+        for (int v = 0; v < b->n; ++v) {
+            if (head[v] >= b->recs[v].recs.size()) continue;
+            const TileRec& r = b->recs[v].recs[head[v]];
+            if (r.body != best) continue;
+            g.recs.push_back(&r);
+            g.tiles += r.tiles;
+            ++head[v];
+        }
+        g.proto = g.recs.front();
+        b->groups.push_back(std::move(g));
+    }
+    /* arguments and tile prefixes of each launch, uploaded once */
+    for (auto& g : b->groups) {
+        const size_t as = g.proto->args.size();
+        std::vector<uint8_t> args(as * g.recs.size());
+        std::vector<int64_t> prefix(g.recs.size() + 1, 0);
+        for (size_t k = 0; k < g.recs.size(); ++k) {
+            std::memcpy(args.data() + k * as, g.recs[k]->args.data(), as);
+            prefix[k + 1] = prefix[k] + g.recs[k]->tiles;
+        }
+        g.d_args.reset(b->pool, args.size());
+        g.d_prefix.reset(b->pool, prefix.size() * 8);
+        HIP_CHECK(hipMemcpy(g.d_args.ptr(), args.data(), args.size(), hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(g.d_prefix.ptr(), prefix.data(), prefix.size() * 8, hipMemcpyHostToDevice));
+    }
+}
+
+unsigned candidate_grid(const aprh_batch* b, const Group& g, int c) {
+    const int64_t cap = (int64_t)(0xFFFFFFFFu / (unsigned)g.proto->threads);
+    const int64_t want = (c == 0) ? g.tiles : (int64_t)b->cus * (1 << (c - 1));
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>({want, g.tiles, cap}));
+}
+
+void launch_all(aprh_batch* b, const std::vector<hipEvent_t>* ev) {
+    hipStream_t st = b->ctx->stream;
+    for (size_t k = 0; k < b->groups.size(); ++k) {
+        const Group& g = b->groups[k];
+        if (ev) HIP_CHECK(hipEventRecord((*ev)[k], st));
+        g.proto->multi(st, g.d_args.ptr(), g.d_prefix.as<int64_t>(), (int)g.recs.size(), g.grid);
+    }
+    if (ev) HIP_CHECK(hipEventRecord(ev->back(), st));
+}
+
+/* step 3: per-launch grid from timed runs of the whole sequence (each run recomputes every
+   stage from the inputs, so the runs leave valid results behind) */
+void autotune(aprh_batch* b) {
+    hipStream_t st = b->ctx->stream;
+    std::vector<hipEvent_t> ev(b->groups.size() + 1);
+    for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
+    std::vector<std::vector<float>> t(b->groups.size(), std::vector<float>(N_CAND, 1e30f));
+    try {
+        for (int c = -1; c < N_CAND; ++c) {   /* c = -1: warm-up */
+            for (auto& g : b->groups) g.grid = candidate_grid(b, g, std::max(c, 0));
+            launch_all(b, &ev);
+            HIP_CHECK(hipStreamSynchronize(st));
+            if (c < 0) continue;
+            for (size_t k = 0; k < b->groups.size(); ++k) HIP_CHECK(hipEventElapsedTime(&t[k][c], ev[k], ev[k + 1]));
+        }
+    } catch (...) {
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        throw;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (size_t k = 0; k < b->groups.size(); ++k) {
+        Group& g = b->groups[k];
+        int bc = 0;
+        for (int c = 1; c < N_CAND; ++c)
+            if (t[k][c] < t[k][bc]) bc = c;
+        g.grid = candidate_grid(b, g, bc);
+        g.ms = t[k][bc];
+    }
 }
 
 void read_slots(aprh_batch* b) {
@@ -126,14 +239,14 @@ int aprh_batch_create(aprh_context* ctx, int32_t n, const int64_t* dims3, const 
         b->dims.assign(dims3, dims3 + 3 * (size_t)n);
         b->par = *par;
         b->pool = new Pool(ctx->device);
+        b->pool->set_keep(true);
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess && cus > 0) b->cus = cus;
         b->slots.reset(b->pool, (size_t)n * (8 + 4 + 12));
         for (int i = 0; i < n; ++i) {
             const int64_t y = dims3[3 * i], x = dims3[3 * i + 1], z = dims3[3 * i + 2];
             if (y < 1 || x < 1 || z < 1) throw Error(APRH_ERR_BAD_DIMS, "dimensions must be positive");
             if (y > 65535 || x > 65535 || z > 65535) throw Error(APRH_ERR_UNSUPPORTED, "dimensions above 65535");
-            const size_t nv = (size_t)(y * x * z);
-            b->voxels += nv;
-            b->inputs.push_back(new Buf(b->pool, nv * 2));
             aprh_apr* a = new aprh_apr();
             a->ctx = ctx;
             a->borrowed = true;
@@ -142,28 +255,29 @@ int aprh_batch_create(aprh_context* ctx, int32_t n, const int64_t* dims3, const 
                 delete a;
                 throw Error(APRH_ERR_UNSUPPORTED, "batched conversion needs l_max >= 3 (a side above 4 pixels)");
             }
+            const size_t nv = (size_t)(y * x * z);
+            b->voxels += nv;
+            b->inputs.push_back(new Buf(b->pool, nv * 2));
             a->xz_end.reset(b->pool, a->n_xz * 8);
             a->y_vec.reset(b->pool, nv * 2);   /* P <= N */
             b->aprs.push_back(a);
         }
         launch_zero(ctx, b->slots.ptr(), b->slots.bytes());
         for (auto* in : b->inputs) launch_zero(ctx, in->ptr(), in->bytes());
-        /* eager pass: fills the pool with every scratch block the capture will reuse */
-        in_plan(b, [&] { record(b); });
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
-        /* capture */
-        in_plan(b, [&] {
-            HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
-            try {
-                record(b);
-            } catch (...) {
-                hipGraph_t g = nullptr;
-                (void)hipStreamEndCapture(ctx->stream, &g);
-                if (g) (void)hipGraphDestroy(g);
-                throw;
-            }
-            HIP_CHECK(hipStreamEndCapture(ctx->stream, &b->graph));
-        });
+        record(b);
+        merge(b);
+        autotune(b);
+        HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+        try {
+            launch_all(b, nullptr);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(ctx->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HIP_CHECK(hipStreamEndCapture(ctx->stream, &b->graph));
         HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     } catch (...) {
         (void)hipStreamSynchronize(ctx->stream);
@@ -194,6 +308,21 @@ int aprh_batch_get(aprh_batch* b, int32_t i, aprh_apr** apr) {
     APRH_TRY
     if (!b || !apr || i < 0 || i >= b->n) throw Error(APRH_ERR_INVALID_ARGUMENT, "bad batch index");
     *apr = b->aprs[(size_t)i];
+    APRH_CATCH
+}
+
+int aprh_batch_stage_count(const aprh_batch* b) { return b ? (int)b->groups.size() : 0; }
+
+int aprh_batch_stage_info(const aprh_batch* b, int32_t i, aprh_batch_stage* info) {
+    APRH_TRY
+    if (!b || !info || i < 0 || i >= (int)b->groups.size()) throw Error(APRH_ERR_INVALID_ARGUMENT, "bad stage index");
+    const Group& g = b->groups[(size_t)i];
+    info->name = g.proto->name;
+    info->volumes = (int32_t)g.recs.size();
+    info->tiles = g.tiles;
+    info->grid = g.grid;
+    info->threads = (uint32_t)g.proto->threads;
+    info->tuned_ms = g.ms;
     APRH_CATCH
 }
 

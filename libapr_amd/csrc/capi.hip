@@ -71,9 +71,13 @@ void get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x,
     DevIn in(ctx, image, n * 2);
     geninfo_init(*a, y, x, z);
     mark(ctx, ST_OFFSET);
-    float* d_off = (float*)ctx->d_flags;
-    uint32_t* d_mm = (uint32_t*)ctx->d_flags + 2;
-    int* d_err = ctx->d_flags + 4;
+    /* offset, min/max and overflow words: the context's, or a recorded volume's own */
+    Buf own_flags;
+    if (ctx->recorder) own_flags.reset(ctx->pool, 256);
+    int* flags = ctx->recorder ? own_flags.as<int>() : ctx->d_flags;
+    float* d_off = (float*)flags;
+    uint32_t* d_mm = (uint32_t*)flags + 2;
+    int* d_err = flags + 4;
     launch_zero(ctx, d_err, 3 * sizeof(int));
     /* computeL: bspline offset (APRConverter.hpp:213-224) */
     launch_minmax_offset(ctx, (const uint16_t*)in.ptr, n, par.lambda, d_off, d_mm);
@@ -98,7 +102,7 @@ void get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x,
     launch_grad_ds_u16(ctx, I, grad.as<uint16_t>(), mean.as<float>(), y, x, z, par.dx, par.dy, par.dz, false);
     int herr[3] = {0, 0, 0};
     if (ctx->async_mode)
-        HIP_CHECK(hipMemcpyAsync(ctx->slot_err, d_err, sizeof(herr), hipMemcpyDeviceToDevice, st));
+        launch_copy(ctx, ctx->slot_err, d_err, sizeof(herr));
     else
         HIP_CHECK(hipMemcpyAsync(herr, d_err, sizeof(herr), hipMemcpyDeviceToHost, st));
     if (ctx->keep_intermediates) {
@@ -136,7 +140,7 @@ void lis_and_solve(aprh_context* ctx, aprh_apr* a, const aprh_params& par, Buf& 
     LisParams lpa = lp;
     if (par.constant_intensity_scale || nact == 0) {
         A.reset(ctx->pool, nd * 4);
-        HIP_CHECK(hipMemcpyAsync(A.ptr(), mean.ptr(), nd * 4, hipMemcpyDeviceToDevice, st));
+        launch_copy(ctx, A.ptr(), mean.ptr(), nd * 4);
         launch_constant_scale(ctx, A.as<float>(), nd);
         lpa.pad[0] = lpa.pad[1] = lpa.pad[2] = 0;
         lpa.var_rescale = 1.0f;
@@ -151,7 +155,7 @@ void lis_and_solve(aprh_context* ctx, aprh_apr* a, const aprh_params& par, Buf& 
     solve<G>(ctx, *a, par, Ap, lpa, mean.as<float>(), grad, d_off);
     mark(ctx, ST_SAMPLE);
     if (ctx->async_mode) {
-        HIP_CHECK(hipMemcpyAsync(ctx->slot_offset, d_off, 4, hipMemcpyDeviceToDevice, st));
+        launch_copy(ctx, ctx->slot_offset, d_off, 4);
         return;
     }
     HIP_CHECK(hipMemcpyAsync(&a->bspline_offset, d_off, 4, hipMemcpyDeviceToHost, st));

@@ -8,6 +8,7 @@
 #include <mutex>
 
 #include "aprh_internal.hpp"
+#include "tiles.hpp"
 #include "synth.hpp"
 
 namespace {
@@ -60,6 +61,7 @@ void* Pool::get(size_t bytes) {
 }
 
 void Pool::put(void* p) {
+    if (keep_) return;
     auto it = live_.find(p);
     if (it == live_.end()) return;
     free_.insert({it->second, p});
@@ -197,25 +199,73 @@ IIRCoeffs iir_coeffs(float lambda, float tol, int64_t len) {
 /* ---------------- zero fill ---------------- */
 namespace aprh {
 namespace {
-__global__ void k_zero_words(uint32_t* p, size_t nw) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
-}
-__global__ void k_zero_bytes(uint8_t* p, size_t n) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
-}
+struct ZeroWords {
+    struct Args {
+        uint32_t* p;
+        size_t nw;
+        int64_t G;
+    };
+    static constexpr int THREADS = 256;
+    static const char* name() { return "zero"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) {
+        for (size_t i = (size_t)t * blockDim.x + threadIdx.x; i < a.nw; i += (size_t)a.G * blockDim.x) a.p[i] = 0u;
+    }
+};
+struct ZeroBytes {
+    struct Args {
+        uint8_t* p;
+        size_t n;
+        int64_t G;
+    };
+    static constexpr int THREADS = 256;
+    static const char* name() { return "zero_bytes"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) {
+        for (size_t i = (size_t)t * blockDim.x + threadIdx.x; i < a.n; i += (size_t)a.G * blockDim.x) a.p[i] = 0;
+    }
+};
+/* device-to-device copy: 16-byte words when both ends and the size allow, else bytes */
+struct CopyBytes {
+    struct Args {
+        void* dst;
+        const void* src;
+        size_t n;
+        int wide;
+        int64_t G;
+    };
+    static constexpr int THREADS = 256;
+    static const char* name() { return "copy"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) {
+        const size_t stride = (size_t)a.G * blockDim.x;
+        if (a.wide) {
+            uint4* d = (uint4*)a.dst;
+            const uint4* s = (const uint4*)a.src;
+            for (size_t i = (size_t)t * blockDim.x + threadIdx.x; i < a.n / 16; i += stride) d[i] = s[i];
+        } else {
+            uint8_t* d = (uint8_t*)a.dst;
+            const uint8_t* s = (const uint8_t*)a.src;
+            for (size_t i = (size_t)t * blockDim.x + threadIdx.x; i < a.n; i += stride) d[i] = s[i];
+        }
+    }
+};
 }  // namespace
 
 void launch_zero(aprh_context* ctx, void* p, size_t bytes) {
     if (!bytes) return;
     if (((uintptr_t)p & 3) == 0 && (bytes & 3) == 0) {
         const size_t nw = bytes / 4;
-        hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<size_t>((nw + 255) / 256, 4096)), dim3(256), 0, ctx->stream,
-                           (uint32_t*)p, nw);
+        const int64_t G = stride_tiles((int64_t)nw, 256);
+        launch_tiles<ZeroWords>(ctx, {(uint32_t*)p, nw, G}, G);
     } else {
-        hipLaunchKernelGGL(k_zero_bytes, dim3((unsigned)std::min<size_t>((bytes + 255) / 256, 4096)), dim3(256), 0, ctx->stream,
-                           (uint8_t*)p, bytes);
+        const int64_t G = stride_tiles((int64_t)bytes, 256);
+        launch_tiles<ZeroBytes>(ctx, {(uint8_t*)p, bytes, G}, G);
     }
-    HIP_LAUNCH_CHECK();
+}
+
+void launch_copy(aprh_context* ctx, void* dst, const void* src, size_t bytes) {
+    if (!bytes) return;
+    const int wide = (((uintptr_t)dst | (uintptr_t)src | bytes) & 15u) == 0;
+    const int64_t G = stride_tiles((int64_t)(wide ? bytes / 16 : bytes), 256);
+    launch_tiles<CopyBytes>(ctx, {dst, src, bytes, wide, G}, G);
 }
 }  // namespace aprh
 

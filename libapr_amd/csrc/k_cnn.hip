@@ -175,6 +175,99 @@ __global__ __launch_bounds__(256) void k_conv16(const uint16_t* __restrict__ in_
     }
 }
 
+/* the same convolution with the input staged in LDS: a block owns an output brick of up to
+   16 (y) x 8 (x) x 4 (z) voxels, loads the (brick + 2)^3 input voxels (zero outside the input)
+   once, and its four waves take 16-voxel groups of the brick's voxels in turn, reading every
+   tap's A operand from LDS (the brick's linear voxel order, so edge bricks waste no MFMA work) */
+constexpr int BY = 16, BX = 8, BZ = 4, IY = BY + 2, IX = BX + 2, IZ = BZ + 2;
+
+__global__ __launch_bounds__(256) void k_conv16_lds(const uint16_t* __restrict__ in_cl, int sin, int sout, int off,
+                                                    const u32x4* __restrict__ Wp, const float* __restrict__ bias,
+                                                    const uint16_t* __restrict__ mask_cl, uint16_t* out_cl, uint16_t* out_pl, int relu,
+                                                    int T, int nby, int nbx, int nbz) {
+    __shared__ __attribute__((aligned(16))) uint16_t tile[IZ * IX * IY * CH];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, m = lane & 15, g = lane >> 4;
+    u32x4 w[NPAIR];
+#pragma unroll
+    for (int p = 0; p < NPAIR; ++p) w[p] = Wp[p * 64 + lane];
+    const float bv = bias ? bias[m] : 0.0f;
+    const int c8 = (g & 1) * 8, th = g >> 1;
+    /* LDS offset (in u16) of each pair's tap for this lane's half of the pair */
+    int toff[NPAIR];
+#pragma unroll
+    for (int p = 0; p < NPAIR; ++p) {
+        const int tap = min(2 * p + th, NT - 1);
+        int dz, dx, dy;
+        tap_offsets(tap, dz, dx, dy);
+        toff[p] = ((dz * IX + dx) * IY + dy) * CH;
+    }
+    const bool last_pair_live = th == 0;   /* tap 27 (pair 13, odd half) does not exist */
+    const int nv = sout * sout * sout;
+    const int64_t nbricks = (int64_t)T * nby * nbx * nbz;
+    for (int64_t br = blockIdx.x; br < nbricks; br += gridDim.x) {
+        int64_t r = br;
+        const int by = (int)(r % nby);
+        r /= nby;
+        const int bx = (int)(r % nbx);
+        r /= nbx;
+        const int bz = (int)(r % nbz);
+        const int t = (int)(r / nbz);
+        const int y0 = by * BY, x0 = bx * BX, z0 = bz * BZ;
+        const int vy = min(BY, sout - y0), vx = min(BX, sout - x0), vz = min(BZ, sout - z0);
+        const int iy = vy + 2, ix = vx + 2, iz = vz + 2;
+        const uint16_t* it = in_cl + (int64_t)t * sin * sin * sin * CH;
+        __syncthreads();   /* the previous brick's waves are done with the tile */
+        const int nchunk = iz * ix * iy * 2;
+        for (int c = threadIdx.x; c < nchunk; c += 256) {
+            const int h = c & 1;
+            int q = c >> 1;
+            const int ly = q % iy;
+            q /= iy;
+            const int lx = q % ix, lz = q / ix;
+            const int zi = z0 + off + lz, xi = x0 + off + lx, yi = y0 + off + ly;
+            u32x4 v = {0u, 0u, 0u, 0u};
+            if (zi >= 0 && zi < sin && xi >= 0 && xi < sin && yi >= 0 && yi < sin)
+                v = *(const u32x4*)(it + (int64_t)((zi * sin + xi) * sin + yi) * CH + h * 8);
+            *(u32x4*)&tile[((lz * IX + lx) * IY + ly) * CH + h * 8] = v;
+        }
+        __syncthreads();
+        const int nvox = vy * vx * vz, ngroups = (nvox + 15) / 16;
+        for (int gi = wave; gi < ngroups; gi += 4) {
+            const int k = gi * 16 + m;
+            const bool valid = k < nvox;
+            int ky = 0, kx = 0, kz = 0;
+            if (valid) {
+                ky = k % vy;
+                const int q = k / vy;
+                kx = q % vx;
+                kz = q / vx;
+            }
+            const uint16_t* base = &tile[((kz * IX + kx) * IY + ky) * CH + c8];
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int p = 0; p < NPAIR; ++p) {
+                u32x4 av = {0u, 0u, 0u, 0u};
+                if (valid && (p < NPAIR - 1 || last_pair_live)) av = *(const u32x4*)(base + toff[p]);
+                acc = mfma(av, w[p], acc);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int kk = gi * 16 + g * 4 + i;
+                if (kk < nvox) {
+                    const int oy = kk % vy, q = kk / vy, ox = q % vx, oz = q / vx;
+                    const int vo = ((z0 + oz) * sout + (x0 + ox)) * sout + (y0 + oy);
+                    float val = acc[i] + bv;
+                    if (relu) val = fmaxf(val, 0.0f);
+                    if (mask_cl && !(bf2f(mask_cl[((int64_t)t * nv + vo) * CH + m]) > 0.0f)) val = 0.0f;
+                    const uint16_t hv = f2bf(val);
+                    if (out_cl) out_cl[((int64_t)t * nv + vo) * CH + m] = hv;
+                    if (out_pl) out_pl[((int64_t)t * CH + m) * nv + vo] = hv;
+                }
+            }
+        }
+    }
+}
+
 /* layer 4 + residual (+ loss gradient): y = x(centre) + b4 + sum W4 a3; one thread per output voxel */
 __global__ __launch_bounds__(256) void k_l4_fwd(const uint16_t* __restrict__ a3_cl, int S, const float* __restrict__ x,
                                                 const float* __restrict__ W4, const float* __restrict__ b4, const float* __restrict__ tgt,
@@ -557,6 +650,17 @@ namespace {
 
 constexpr int WAVES = 2048;   /* waves of the weight-gradient kernels (one partial each) */
 
+/* 16 -> 16 convolution of T tiles (input side sin, output side sout, input offset off) */
+void conv16(hipStream_t st, const uint16_t* in_cl, int sin, int sout, int off, const u32x4* Wp, const float* bias,
+            const uint16_t* mask_cl, uint16_t* out_cl, uint16_t* out_pl, int relu, int T) {
+    const int nby = (int)cdiv(sout, BY), nbx = (int)cdiv(sout, BX), nbz = (int)cdiv(sout, BZ);
+    const int64_t nbricks = (int64_t)T * nby * nbx * nbz;
+    const unsigned grid = (unsigned)std::min<int64_t>(nbricks, 65536);
+    hipLaunchKernelGGL(k_conv16_lds, dim3(grid), kBlk, 0, st, in_cl, sin, sout, off, Wp, bias, mask_cl, out_cl, out_pl, relu, T, nby, nbx,
+                       nbz);
+    HIP_LAUNCH_CHECK();
+}
+
 void pack(aprh_cnn* n) {
     hipLaunchKernelGGL(k_pack, dim3((64 + 4 * NPAIR * 64 + 255) / 256), kBlk, 0, n->ctx->stream, n->P.as<float>(), n->W1p.as<u32x4>(),
                        n->W2p.as<u32x4>(), n->W3p.as<u32x4>(), n->W2t.as<u32x4>(), n->W3t.as<u32x4>());
@@ -586,14 +690,10 @@ void forward(aprh_cnn* n, const float* x, int T, int S, float* y, Acts& A, bool 
     hipLaunchKernelGGL(k_l1_fwd, grid(vox(s1)), kBlk, 0, st, x, s0, s1, (const u32x4*)n->W1p.as<u32x4>(), P + OFF_B1, A.a1c.as<uint16_t>(),
                        train ? A.a1p.as<uint16_t>() : (uint16_t*)nullptr, T);
     HIP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_conv16, grid(vox(s2)), kBlk, 0, st, (const uint16_t*)A.a1c.as<uint16_t>(), s1, s2, 0,
-                       (const u32x4*)n->W2p.as<u32x4>(), P + OFF_B2, (const uint16_t*)nullptr, A.a2c.as<uint16_t>(),
-                       train ? A.a2p.as<uint16_t>() : (uint16_t*)nullptr, 1, T);
-    HIP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_conv16, grid(vox(s3)), kBlk, 0, st, (const uint16_t*)A.a2c.as<uint16_t>(), s2, s3, 0,
-                       (const u32x4*)n->W3p.as<u32x4>(), P + OFF_B3, (const uint16_t*)nullptr, A.a3c.as<uint16_t>(),
-                       train ? A.a3p.as<uint16_t>() : (uint16_t*)nullptr, 1, T);
-    HIP_LAUNCH_CHECK();
+    conv16(st, A.a1c.as<uint16_t>(), s1, s2, 0, n->W2p.as<u32x4>(), P + OFF_B2, nullptr, A.a2c.as<uint16_t>(),
+           train ? A.a2p.as<uint16_t>() : nullptr, 1, T);
+    conv16(st, A.a2c.as<uint16_t>(), s2, s3, 0, n->W3p.as<u32x4>(), P + OFF_B3, nullptr, A.a3c.as<uint16_t>(),
+           train ? A.a3p.as<uint16_t>() : nullptr, 1, T);
     hipLaunchKernelGGL(k_l4_fwd, dim3(nblk_l4), kBlk, 0, st, (const uint16_t*)A.a3c.as<uint16_t>(), S, x, P + OFF_W4, P + OFF_B4, tgt, y, g4,
                        1.0f / (float)vox(S), part, T);
     HIP_LAUNCH_CHECK();
@@ -638,9 +738,8 @@ void train_step(aprh_cnn* n, const float* x, const float* tgt, int T, int S, flo
     HIP_LAUNCH_CHECK();
     auto grid = [&](size_t v) { return dim3((unsigned)std::min<size_t>((v / 16 + 3) / 4 + 1, 16384)); };
     Buf g2c(pool, vox(s2) * CH * 2), g2p(pool, vox(s2) * CH * 2);
-    hipLaunchKernelGGL(k_conv16, grid(vox(s2)), kBlk, 0, st, (const uint16_t*)g3c.as<uint16_t>(), s3, s2, -2, (const u32x4*)n->W3t.as<u32x4>(),
-                       (const float*)nullptr, (const uint16_t*)A.a2c.as<uint16_t>(), g2c.as<uint16_t>(), g2p.as<uint16_t>(), 0, T);
-    HIP_LAUNCH_CHECK();
+    conv16(st, g3c.as<uint16_t>(), s3, s2, -2, n->W3t.as<u32x4>(), nullptr, A.a2c.as<uint16_t>(), g2c.as<uint16_t>(), g2p.as<uint16_t>(),
+           0, T);
     /* layer 2 weights, then the gradient of a1 */
     hipLaunchKernelGGL(k_dw16, wgrid, kBlk, 0, st, (const uint16_t*)A.a1p.as<uint16_t>(), (const uint16_t*)g2p.as<uint16_t>(), s2,
                        pbuf.as<float>(), T);
@@ -649,9 +748,7 @@ void train_step(aprh_cnn* n, const float* x, const float* tgt, int T, int S, flo
                        NT * 256 + CH, NT * 256 + CH, G, OFF_W2, 0);
     HIP_LAUNCH_CHECK();
     Buf g1p(pool, vox(s1) * CH * 2);
-    hipLaunchKernelGGL(k_conv16, grid(vox(s1)), kBlk, 0, st, (const uint16_t*)g2c.as<uint16_t>(), s2, s1, -2, (const u32x4*)n->W2t.as<u32x4>(),
-                       (const float*)nullptr, (const uint16_t*)A.a1c.as<uint16_t>(), (uint16_t*)nullptr, g1p.as<uint16_t>(), 0, T);
-    HIP_LAUNCH_CHECK();
+    conv16(st, g2c.as<uint16_t>(), s2, s1, -2, n->W2t.as<u32x4>(), nullptr, A.a1c.as<uint16_t>(), nullptr, g1p.as<uint16_t>(), 0, T);
     /* layer 1 */
     hipLaunchKernelGGL(k_dw_l1, wgrid, kBlk, 0, st, x, (const uint16_t*)g1p.as<uint16_t>(), S, pbuf.as<float>(), T);
     HIP_LAUNCH_CHECK();

@@ -20,6 +20,7 @@
 #include <cmath>
 
 #include "aprh_internal.hpp"
+#include "tiles.hpp"
 
 namespace aprh {
 
@@ -52,13 +53,11 @@ __device__ __forceinline__ void gstore(double v, uint16_t& out) { out = d2u16(v)
 __device__ __forceinline__ void gstore(double v, float& out) { out = (float)v; }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_grad_ds(const T* __restrict__ img, T* grad, float* mean, int64_t y_num,
-                                                 int64_t x_num, int64_t z_num, int64_t yd_num, float h2x, float h2y, float h2z,
-                                                 int accum) {
-    const int64_t yd = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void grad_ds_body(const T* __restrict__ img, T* grad, float* mean, int64_t y_num, int64_t x_num,
+                                             int64_t z_num, int64_t yd_num, float h2x, float h2y, float h2z, int accum, int64_t bx,
+                                             int64_t xd, int64_t zd, int64_t xd_num) {
+    const int64_t yd = bx * blockDim.x + threadIdx.x;
     if (yd >= yd_num) return;
-    const int64_t xd = blockIdx.y, zd = blockIdx.z;
-    const int64_t xd_num = gridDim.y;
     const int64_t xy = x_num * y_num;
     T g = accum ? grad[(zd * xd_num + xd) * yd_num + yd] : (T)0;
     const int64_t y0 = 2 * yd, y1 = min(2 * yd + 1, y_num - 1);
@@ -122,13 +121,11 @@ __device__ __forceinline__ int u16at(const uint4& v, int i) {
 }
 
 template <bool UNIT>
-__global__ __launch_bounds__(256) void k_grad_ds_vec(const uint16_t* __restrict__ img, uint16_t* grad, float* mean, int64_t y_num,
-                                                     int64_t x_num, int64_t z_num, int64_t yd_num, float h2x, float h2y, float h2z,
-                                                     int accum) {
-    const int64_t yg = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;   /* group of 8 y */
+__device__ __forceinline__ void grad_ds_vec_body(const uint16_t* __restrict__ img, uint16_t* grad, float* mean, int64_t y_num,
+                                                 int64_t x_num, int64_t z_num, int64_t yd_num, float h2x, float h2y, float h2z,
+                                                 int accum, int64_t bx, int64_t xd, int64_t zd, int64_t xd_num) {
+    const int64_t yg = bx * blockDim.x + threadIdx.x;   /* group of 8 y */
     if (yg * 8 >= y_num) return;
-    const int64_t xd = blockIdx.y, zd = blockIdx.z;
-    const int64_t xd_num = gridDim.y;
     const int64_t xy = x_num * y_num;
     const int64_t y0 = yg * 8;
     const int64_t x0 = 2 * xd, z0 = 2 * zd;
@@ -207,9 +204,9 @@ __global__ __launch_bounds__(256) void k_grad_ds_vec(const uint16_t* __restrict_
 }
 
 /* ---------------- inverse cubic B-spline, [1 4 1]/6 ---------------- */
-__global__ void k_inv_y(const float* __restrict__ in, float* out, int64_t y_num, size_t n) {
+__device__ __forceinline__ void inv_y_body(const float* __restrict__ in, float* out, int64_t y_num, size_t n, int64_t G, int64_t t) {
     const float a1 = (float)(1.0 / 6.0), a2 = (float)(4.0 / 6.0), a3 = (float)(1.0 / 6.0);
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    for (size_t i = (size_t)t * blockDim.x + threadIdx.x; i < n; i += (size_t)G * blockDim.x) {
         const int64_t k = (int64_t)(i % (size_t)y_num);
         const float* l = in + (i - (size_t)k);
         float v;
@@ -227,9 +224,10 @@ __global__ void k_inv_y(const float* __restrict__ in, float* out, int64_t y_num,
 }
 
 /* x / z: out[0] = a1*in[1] + a2*in[0] + a3*in[1]; last = (a1+a3)*in[n-2] + a2*in[n-1] */
-__global__ void k_inv_outer(const float* __restrict__ in, float* out, int64_t n_axis, int64_t stride, size_t n) {
+__device__ __forceinline__ void inv_outer_body(const float* __restrict__ in, float* out, int64_t n_axis, int64_t stride, size_t n,
+                                               int64_t G, int64_t t) {
     const float a1 = (float)(1.0 / 6.0), a2 = (float)(4.0 / 6.0), a3 = (float)(1.0 / 6.0);
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    for (size_t i = (size_t)t * blockDim.x + threadIdx.x; i < n; i += (size_t)G * blockDim.x) {
         const int64_t j = (int64_t)((i / (size_t)stride) % (size_t)n_axis);
         float v;
         if (j == n_axis - 1) {
@@ -265,14 +263,14 @@ struct PadGeom {
    staged through LDS so global loads and stores stay coalesced along y.  The per-line
    source offsets are computed once per block. */
 template <int O>
-__global__ __launch_bounds__(64) void k_mean_y(int mode, const float* __restrict__ inten, float* A, PadGeom g) {
+__device__ __forceinline__ void mean_y_body(int mode, const float* __restrict__ inten, float* A, const PadGeom& g, int64_t blk) {
     constexpr int TY = 64;
     constexpr int H = 2 * O + 2;
     __shared__ float tile[64][TY + 1];
     __shared__ int64_t src_row[64];   /* inten offset of the (reflected) source row of each line */
     const int lane = threadIdx.x;
     const int64_t n_lines = g.xp * g.zp;
-    const int64_t line0 = (int64_t)blockIdx.x * 64;
+    const int64_t line0 = blk * 64;
     const int64_t me = line0 + lane;
     const bool valid = me < n_lines;
     const int yn = (int)g.yp, ydn = (int)g.yd, py = (int)g.py;
@@ -355,12 +353,13 @@ __global__ __launch_bounds__(64) void k_mean_y(int mode, const float* __restrict
 /* x / z running-sum mean with offset O (LocalIntensityScale.hpp:365-494), in place;
    thread = one y of one line along the outer axis (length n, stride s) */
 template <int O>
-__global__ __launch_bounds__(256) void k_mean_outer(float* A, int64_t y_num, int64_t n, int64_t s, int64_t outer_stride) {
+__device__ __forceinline__ void mean_outer_body(float* A, int64_t y_num, int64_t n, int64_t s, int64_t outer_stride, int64_t bx,
+                                                int64_t by) {
     constexpr int H = 2 * O + 2;
     constexpr int W = 2 * O + 1;
-    const int64_t y = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t y = bx * blockDim.x + threadIdx.x;
     if (y >= y_num) return;
-    float* base = A + (int64_t)blockIdx.y * outer_stride + y;
+    float* base = A + by * outer_stride + y;
     float h[H];
     for (int i = 0; i < H; ++i) h[i] = 0.0f;
     float c = 0.0f;
@@ -384,11 +383,96 @@ __global__ __launch_bounds__(256) void k_mean_outer(float* A, int64_t y_num, int
     }
 }
 
+/* the same pass with two y per lane (8-byte accesses) and D steps of the line loaded ahead:
+   step j reads sample j and writes output j - O, so every sample is in registers before its
+   output overwrites it */
+template <int O, int D>
+__device__ __forceinline__ void mean_outer2_body(float* A, int64_t y_num, int64_t n, int64_t s, int64_t outer_stride, int64_t bx,
+                                                 int64_t by) {
+    constexpr int H = 2 * O + 2;
+    constexpr int W = 2 * O + 1;
+    const int64_t y2 = bx * blockDim.x + threadIdx.x;
+    if (2 * y2 >= y_num) return;
+    float2* base = (float2*)(A + by * outer_stride + 2 * y2);
+    const int64_t sv = s / 2;
+    float ha[H], hb[H];
+#pragma unroll
+    for (int i = 0; i < H; ++i) ha[i] = hb[i] = 0.0f;
+    float ca = 0.0f, cb = 0.0f;
+    for (int64_t j0 = 0; j0 < n; j0 += D) {
+        float2 buf[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            if (j0 + d < n) buf[d] = base[(j0 + d) * sv];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int64_t j = j0 + d;
+            if (j >= n) break;
+            ca = (j == 0) ? buf[d].x : buf[d].x + ca;
+            cb = (j == 0) ? buf[d].y : buf[d].y + cb;
+#pragma unroll
+            for (int i = 0; i + 1 < H; ++i) {
+                ha[i] = ha[i + 1];
+                hb[i] = hb[i + 1];
+            }
+            ha[H - 1] = ca;
+            hb[H - 1] = cb;
+            const int64_t i = j - O;
+            if (i < 0) continue;
+            if (i <= O) {
+                const float dv = (float)(i + O + 1);
+                base[i * sv] = make_float2(ca / dv, cb / dv);
+            } else if (i < n - O) {
+                base[i * sv] = make_float2((ca - ha[0]) / (float)W, (cb - hb[0]) / (float)W);
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < O; ++r) {
+        const int64_t i = n - O + r;
+        const float dv = (float)(n - i + O);
+        base[i * sv] = make_float2((ha[H - 1] - ha[1 + r]) / dv, (hb[H - 1] - hb[1 + r]) / dv);
+    }
+}
+
+template <int O>
+struct MeanY {
+    struct Args {
+        int mode;
+        const float* inten;
+        float* A;
+        PadGeom g;
+    };
+    static constexpr int THREADS = 64;
+    static const char* name() { return "lis_mean_y"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) { mean_y_body<O>(a.mode, a.inten, a.A, a.g, t); }
+};
+
+struct OuterArgs {
+    float* A;
+    int64_t y_num, n, s, outer_stride, gx;
+};
+template <int O>
+struct MeanOuter {
+    using Args = OuterArgs;
+    static constexpr int THREADS = 256;
+    static const char* name() { return "lis_mean_outer"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) { mean_outer_body<O>(a.A, a.y_num, a.n, a.s, a.outer_stride, t % a.gx, t / a.gx); }
+};
+template <int O, int D>
+struct MeanOuter2 {
+    using Args = OuterArgs;
+    static constexpr int THREADS = 256;
+    static const char* name() { return "lis_mean_outer2"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) {
+        mean_outer2_body<O, D>(a.A, a.y_num, a.n, a.s, a.outer_stride, t % a.gx, t / a.gx);
+    }
+};
+
 template <int O>
 void mean_y(aprh_context* ctx, int mode, const float* inten, float* A, const PadGeom& g) {
     const int64_t lines = g.xp * g.zp;
-    hipLaunchKernelGGL(k_mean_y<O>, dim3((unsigned)cdiv(lines, 64)), dim3(64), 0, ctx->stream, mode, inten, A, g);
-    HIP_LAUNCH_CHECK();
+    launch_tiles<MeanY<O>>(ctx, {mode, inten, A, g}, cdiv(lines, 64));
 }
 
 void mean_y_dispatch(aprh_context* ctx, int O, int mode, const float* inten, float* A, const PadGeom& g) {
@@ -406,9 +490,13 @@ void mean_y_dispatch(aprh_context* ctx, int O, int mode, const float* inten, flo
 
 template <int O>
 void mean_outer(aprh_context* ctx, float* A, int64_t y_num, int64_t n, int64_t s, int64_t outer_stride, int64_t n_outer) {
-    hipLaunchKernelGGL(k_mean_outer<O>, dim3((unsigned)cdiv(y_num, 256), (unsigned)n_outer), dim3(256), 0, ctx->stream, A, y_num, n,
-                       s, outer_stride);
-    HIP_LAUNCH_CHECK();
+    if (y_num % 2 == 0 && ((uintptr_t)A & 7u) == 0) {
+        const int64_t gx = cdiv(y_num / 2, 256);
+        launch_tiles<MeanOuter2<O, 8>>(ctx, {A, y_num, n, s, outer_stride, gx}, gx * n_outer);
+    } else {
+        const int64_t gx = cdiv(y_num, 256);
+        launch_tiles<MeanOuter<O>>(ctx, {A, y_num, n, s, outer_stride, gx}, gx * n_outer);
+    }
 }
 
 void mean_outer_dispatch(aprh_context* ctx, int O, float* A, int64_t y_num, int64_t n, int64_t s, int64_t outer_stride,
@@ -435,12 +523,35 @@ void mean_z(aprh_context* ctx, int O, float* A, int64_t y, int64_t x, int64_t z)
 
 /* applyParameters + computeLevels, one lane per downsampled cell */
 template <typename G>
-__global__ void k_levels(const float* __restrict__ A, PadGeom g, float var_rescale, const float* __restrict__ inten,
-                         G* grad, float* sigma_raw, float* sigma_out, uint8_t* level, const float* d_off, float Ip_th,
-                         float sigma_th, float sigma_th_max, float grad_th, float mult, G* grad_keep) {
-    const float ipth = Ip_th + *d_off;
+struct LevelsArgs {
+    const float* A;
+    PadGeom g;
+    float var_rescale;
+    const float* inten;
+    G* grad;
+    float* sigma_raw;
+    float* sigma_out;
+    uint8_t* level;
+    const float* d_off;
+    float Ip_th, sigma_th, sigma_th_max, grad_th, mult;
+    G* grad_keep;
+    int64_t Gs;   /* virtual grid */
+};
+template <typename G>
+__device__ __forceinline__ void levels_body(const LevelsArgs<G>& a, int64_t t) {
+    const float* __restrict__ A = a.A;
+    const PadGeom& g = a.g;
+    const float var_rescale = a.var_rescale;
+    const float* __restrict__ inten = a.inten;
+    G* grad = a.grad;
+    float* sigma_raw = a.sigma_raw;
+    float* sigma_out = a.sigma_out;
+    uint8_t* level = a.level;
+    const float sigma_th = a.sigma_th, sigma_th_max = a.sigma_th_max, grad_th = a.grad_th, mult = a.mult;
+    G* grad_keep = a.grad_keep;
+    const float ipth = a.Ip_th + *a.d_off;
     const size_t n = (size_t)(g.yd * g.xd * g.zd);
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    for (size_t i = (size_t)t * blockDim.x + threadIdx.x; i < n; i += (size_t)a.Gs * blockDim.x) {
         const int64_t y = (int64_t)(i % (size_t)g.yd);
         const int64_t r = (int64_t)(i / (size_t)g.yd);
         const int64_t x = r % g.xd, z = r / g.xd;
@@ -461,9 +572,10 @@ __global__ void k_levels(const float* __restrict__ A, PadGeom g, float var_resca
 }
 
 /* rescale_var + unpaddPixels (LocalIntensityScale.hpp:173-184, PixelData.hpp:1189-1210) */
-__global__ void k_unpad_rescale(const float* __restrict__ A, PadGeom g, float var_rescale, float* out) {
+__device__ __forceinline__ void unpad_rescale_body(const float* __restrict__ A, const PadGeom& g, float var_rescale, float* out,
+                                                   int64_t G, int64_t t) {
     const size_t n = (size_t)(g.yd * g.xd * g.zd);
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    for (size_t i = (size_t)t * blockDim.x + threadIdx.x; i < n; i += (size_t)G * blockDim.x) {
         const int64_t y = (int64_t)(i % (size_t)g.yd);
         const int64_t r = (int64_t)(i / (size_t)g.yd);
         const int64_t x = r % g.xd, z = r / g.xd;
@@ -473,8 +585,8 @@ __global__ void k_unpad_rescale(const float* __restrict__ A, PadGeom g, float va
 
 /* constant local intensity scale (LocalIntensityScale.hpp:131-148): a sequential double
    sum in index order, kept sequential (single lane) to reproduce its rounding */
-__global__ void k_constant_scale_sum(const float* __restrict__ m, size_t n, float* out_val) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+__device__ __forceinline__ void constant_scale_sum_body(const float* __restrict__ m, size_t n, float* out_val) {
+    if (threadIdx.x != 0) return;
     float mn = 660000.0f;
     double sum = 0.0;
     for (size_t i = 0; i < n; ++i) {
@@ -485,10 +597,102 @@ __global__ void k_constant_scale_sum(const float* __restrict__ m, size_t n, floa
     *out_val = (float)(sum / (double)(float)n - (double)mn);
 }
 
-__global__ void k_fill(float* m, size_t n, const float* __restrict__ val) {
+__device__ __forceinline__ void fill_body(float* m, size_t n, const float* __restrict__ val, int64_t G, int64_t t) {
     const float v = *val;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) m[i] = v;
+    for (size_t i = (size_t)t * blockDim.x + threadIdx.x; i < n; i += (size_t)G * blockDim.x) m[i] = v;
 }
+
+/* tile bodies of the gradient, inverse B-spline and level stages */
+struct GradArgs {
+    const void* img;
+    void* grad;
+    float* mean;
+    int64_t y_num, x_num, z_num, yd_num;
+    float h2x, h2y, h2z;
+    int accum;
+    int64_t gx, xd_num;
+};
+template <typename T>
+struct GradDs {
+    using Args = GradArgs;
+    static constexpr int THREADS = 256;
+    static const char* name() { return "gradient_ds"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) {
+        const int64_t bx = t % a.gx, r = t / a.gx;
+        grad_ds_body<T>((const T*)a.img, (T*)a.grad, a.mean, a.y_num, a.x_num, a.z_num, a.yd_num, a.h2x, a.h2y, a.h2z, a.accum, bx,
+                        r % a.xd_num, r / a.xd_num, a.xd_num);
+    }
+};
+template <bool UNIT>
+struct GradDsVec {
+    using Args = GradArgs;
+    static constexpr int THREADS = 256;
+    static const char* name() { return "gradient_ds_vec"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) {
+        const int64_t bx = t % a.gx, r = t / a.gx;
+        grad_ds_vec_body<UNIT>((const uint16_t*)a.img, (uint16_t*)a.grad, a.mean, a.y_num, a.x_num, a.z_num, a.yd_num, a.h2x, a.h2y,
+                               a.h2z, a.accum, bx, r % a.xd_num, r / a.xd_num, a.xd_num);
+    }
+};
+struct InvArgs {
+    const float* in;
+    float* out;
+    int64_t n_axis, stride;
+    size_t n;
+    int64_t G;
+};
+struct InvY {
+    using Args = InvArgs;
+    static constexpr int THREADS = 256;
+    static const char* name() { return "inv_bspline_y"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) { inv_y_body(a.in, a.out, a.n_axis, a.n, a.G, t); }
+};
+struct InvOuter {
+    using Args = InvArgs;
+    static constexpr int THREADS = 256;
+    static const char* name() { return "inv_bspline_outer"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) { inv_outer_body(a.in, a.out, a.n_axis, a.stride, a.n, a.G, t); }
+};
+template <typename G>
+struct Levels {
+    using Args = LevelsArgs<G>;
+    static constexpr int THREADS = 256;
+    static const char* name() { return "levels"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) { levels_body<G>(a, t); }
+};
+struct UnpadRescale {
+    struct Args {
+        const float* A;
+        PadGeom g;
+        float var_rescale;
+        float* out;
+        int64_t G;
+    };
+    static constexpr int THREADS = 256;
+    static const char* name() { return "unpad_rescale"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) { unpad_rescale_body(a.A, a.g, a.var_rescale, a.out, a.G, t); }
+};
+struct ConstantScaleSum {
+    struct Args {
+        const float* m;
+        size_t n;
+        float* out_val;
+    };
+    static constexpr int THREADS = 64;
+    static const char* name() { return "constant_scale_sum"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t) { constant_scale_sum_body(a.m, a.n, a.out_val); }
+};
+struct Fill {
+    struct Args {
+        float* m;
+        size_t n;
+        const float* val;
+        int64_t G;
+    };
+    static constexpr int THREADS = 256;
+    static const char* name() { return "fill"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) { fill_body(a.m, a.n, a.val, a.G, t); }
+};
 
 PadGeom pad_geom(int64_t yd, int64_t xd, int64_t zd, const LisParams& lp) {
     PadGeom g;
@@ -503,21 +707,18 @@ PadGeom pad_geom(int64_t yd, int64_t xd, int64_t zd, const LisParams& lp) {
 void launch_grad_ds_u16(aprh_context* ctx, const uint16_t* img, uint16_t* grad, float* mean, int64_t y, int64_t x, int64_t z,
                         float hx, float hy, float hz, bool accum) {
     const int64_t yd = cdiv(y, 2), xd = cdiv(x, 2), zd = cdiv(z, 2);
-    if (xd > 65535 || zd > 65535) throw Error(APRH_ERR_UNSUPPORTED, "x or z dimension too large");
     const bool aligned = (((uintptr_t)img | (uintptr_t)grad | (uintptr_t)mean) & 15u) == 0;
     if (y % 8 == 0 && aligned) {
-        const dim3 grid((unsigned)cdiv(y / 8, 256), (unsigned)xd, (unsigned)zd);
-        if (hx == 1.0f && hy == 1.0f && hz == 1.0f)
-            hipLaunchKernelGGL(k_grad_ds_vec<true>, grid, dim3(256), 0, ctx->stream, img, grad, mean, y, x, z, yd, 2.0f, 2.0f, 2.0f,
-                               accum ? 1 : 0);
-        else
-            hipLaunchKernelGGL(k_grad_ds_vec<false>, grid, dim3(256), 0, ctx->stream, img, grad, mean, y, x, z, yd, 2.0f * hx,
-                               2.0f * hy, 2.0f * hz, accum ? 1 : 0);
+        const int64_t gx = cdiv(y / 8, 256);
+        const bool unit = hx == 1.0f && hy == 1.0f && hz == 1.0f;
+        GradArgs a{img, grad, mean, y, x, z, yd, 2.0f * hx, 2.0f * hy, 2.0f * hz, accum ? 1 : 0, gx, xd};
+        if (unit) launch_tiles<GradDsVec<true>>(ctx, a, gx * xd * zd);
+        else launch_tiles<GradDsVec<false>>(ctx, a, gx * xd * zd);
     } else {
-        hipLaunchKernelGGL(k_grad_ds<uint16_t>, dim3((unsigned)cdiv(yd, 256), (unsigned)xd, (unsigned)zd), dim3(256), 0, ctx->stream, img, grad,
-                           mean, y, x, z, yd, 2.0f * hx, 2.0f * hy, 2.0f * hz, accum ? 1 : 0);
+        const int64_t gx = cdiv(yd, 256);
+        launch_tiles<GradDs<uint16_t>>(ctx, GradArgs{img, grad, mean, y, x, z, yd, 2.0f * hx, 2.0f * hy, 2.0f * hz, accum ? 1 : 0, gx, xd},
+                                       gx * xd * zd);
     }
-    HIP_LAUNCH_CHECK();
 }
 
 void launch_inv_bspline(aprh_context* ctx, float* m, float* tmp, int64_t y, int64_t x, int64_t z, bool ay, bool ax, bool az) {
@@ -525,23 +726,20 @@ void launch_inv_bspline(aprh_context* ctx, float* m, float* tmp, int64_t y, int6
     const size_t n = (size_t)(y * x * z);
     float* src = m;
     float* dst = tmp;
-    const dim3 grid(4096), block(256);
+    const int64_t G = stride_tiles((int64_t)n, 256 * 4);
     if (ay) {
-        hipLaunchKernelGGL(k_inv_y, grid, block, 0, ctx->stream, src, dst, y, n);
-        HIP_LAUNCH_CHECK();
+        launch_tiles<InvY>(ctx, {src, dst, y, 1, n, G}, G);
         std::swap(src, dst);
     }
     if (ax) {
-        hipLaunchKernelGGL(k_inv_outer, grid, block, 0, ctx->stream, src, dst, x, y, n);
-        HIP_LAUNCH_CHECK();
+        launch_tiles<InvOuter>(ctx, {src, dst, x, y, n, G}, G);
         std::swap(src, dst);
     }
     if (az) {
-        hipLaunchKernelGGL(k_inv_outer, grid, block, 0, ctx->stream, src, dst, z, x * y, n);
-        HIP_LAUNCH_CHECK();
+        launch_tiles<InvOuter>(ctx, {src, dst, z, x * y, n, G}, G);
         std::swap(src, dst);
     }
-    if (src != m) HIP_CHECK(hipMemcpyAsync(m, src, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    if (src != m) launch_copy(ctx, m, src, n * 4);
 }
 
 LisParams lis_params(const aprh_params& par, int64_t yd, int64_t xd, int64_t zd) {
@@ -605,9 +803,10 @@ void launch_levels_padded(aprh_context* ctx, const float* A, const LisParams& lp
                           const float* inten, G* grad, float* sigma_raw, float* sigma_out, uint8_t* level,
                           const aprh_params& par, const float* d_offset, float mult, G* grad_keep) {
     PadGeom g = pad_geom(yd, xd, zd, lp);
-    hipLaunchKernelGGL(k_levels<G>, dim3(4096), dim3(256), 0, ctx->stream, A, g, lp.var_rescale, inten, grad, sigma_raw, sigma_out, level,
-                       d_offset, par.Ip_th, par.sigma_th, par.sigma_th_max, par.grad_th, mult, grad_keep);
-    HIP_LAUNCH_CHECK();
+    const int64_t Gs = stride_tiles(yd * xd * zd, 256 * 4);
+    launch_tiles<Levels<G>>(ctx, LevelsArgs<G>{A, g, lp.var_rescale, inten, grad, sigma_raw, sigma_out, level, d_offset, par.Ip_th,
+                                               par.sigma_th, par.sigma_th_max, par.grad_th, mult, grad_keep, Gs},
+                            Gs);
 }
 
 template void launch_levels_padded<uint16_t>(aprh_context*, const float*, const LisParams&, int64_t, int64_t, int64_t, const float*,
@@ -618,32 +817,46 @@ template void launch_levels_padded<float>(aprh_context*, const float*, const Lis
 void launch_grad_ds_f32(aprh_context* ctx, const float* img, float* grad, float* mean, int64_t y, int64_t x, int64_t z, float hx,
                         float hy, float hz, bool accum) {
     const int64_t yd = cdiv(y, 2), xd = cdiv(x, 2), zd = cdiv(z, 2);
-    hipLaunchKernelGGL(k_grad_ds<float>, dim3((unsigned)cdiv(yd, 256), (unsigned)xd, (unsigned)zd), dim3(256), 0, ctx->stream, img, grad,
-                       mean, y, x, z, yd, 2.0f * hx, 2.0f * hy, 2.0f * hz, accum ? 1 : 0);
-    HIP_LAUNCH_CHECK();
+    const int64_t gx = cdiv(yd, 256);
+    launch_tiles<GradDs<float>>(ctx, GradArgs{img, grad, mean, y, x, z, yd, 2.0f * hx, 2.0f * hy, 2.0f * hz, accum ? 1 : 0, gx, xd},
+                                gx * xd * zd);
 }
 
 void launch_unpad_rescale(aprh_context* ctx, const float* A, const LisParams& lp, int64_t yd, int64_t xd, int64_t zd, float* out) {
     PadGeom g = pad_geom(yd, xd, zd, lp);
-    hipLaunchKernelGGL(k_unpad_rescale, dim3(4096), dim3(256), 0, ctx->stream, A, g, lp.var_rescale, out);
-    HIP_LAUNCH_CHECK();
+    const int64_t G = stride_tiles(yd * xd * zd, 256 * 4);
+    launch_tiles<UnpadRescale>(ctx, {A, g, lp.var_rescale, out, G}, G);
 }
 
-__global__ void k_u8_to_f32(const uint8_t* __restrict__ in, float* out, size_t n) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) out[i] = (float)in[i];
-}
+namespace {
+struct U8ToF32 {
+    struct Args {
+        const uint8_t* in;
+        float* out;
+        size_t n;
+        int64_t G;
+    };
+    static constexpr int THREADS = 256;
+    static const char* name() { return "u8_to_f32"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) {
+        for (size_t i = (size_t)t * blockDim.x + threadIdx.x; i < a.n; i += (size_t)a.G * blockDim.x) a.out[i] = (float)a.in[i];
+    }
+};
+}  // namespace
 
 void launch_u8_to_f32(aprh_context* ctx, const uint8_t* in, float* out, size_t n) {
-    hipLaunchKernelGGL(k_u8_to_f32, dim3(4096), dim3(256), 0, ctx->stream, in, out, n);
-    HIP_LAUNCH_CHECK();
+    const int64_t G = stride_tiles((int64_t)n, 256 * 4);
+    launch_tiles<U8ToF32>(ctx, {in, out, n, G}, G);
 }
 
 void launch_constant_scale(aprh_context* ctx, float* m, size_t n) {
-    float* d_val = (float*)ctx->d_flags + 32;
-    hipLaunchKernelGGL(k_constant_scale_sum, dim3(1), dim3(64), 0, ctx->stream, m, n, d_val);
-    HIP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_fill, dim3(4096), dim3(256), 0, ctx->stream, m, n, d_val);
-    HIP_LAUNCH_CHECK();
+    /* a recorded (batched) volume needs its own value slot: the plan's pool keeps every block */
+    Buf own;
+    if (ctx->recorder) own.reset(ctx->pool, 256);
+    float* d_val = ctx->recorder ? own.as<float>() : (float*)ctx->d_flags + 32;
+    launch_tiles<ConstantScaleSum>(ctx, {m, n, d_val}, 1);
+    const int64_t G = stride_tiles((int64_t)n, 256 * 4);
+    launch_tiles<Fill>(ctx, {m, n, d_val, G}, G);
 }
 
 size_t lis_padded_elems(int64_t yd, int64_t xd, int64_t zd, const LisParams& lp) {

@@ -18,6 +18,7 @@
 #include <type_traits>
 
 #include "aprh_internal.hpp"
+#include "tiles.hpp"
 
 namespace aprh {
 
@@ -29,22 +30,36 @@ __device__ __forceinline__ uint16_t f2u16(float v) {
     return (uint16_t)(int32_t)v;
 }
 
+/* roundf (half away from zero) as trunc(v + copysign(0.5 - 2^-25, v)): three VALU operations,
+   and equal to roundf bit for bit on every non-NaN float (v + 0.49999997 crosses the next
+   integer exactly when the fraction of |v| is at least one half; checked over all 2^32
+   patterns against the C library) */
+__device__ __forceinline__ float round_away(float v) { return truncf(v + copysignf(0.49999997f, v)); }
+
+/* an integral float clamped to [-1, 65536] and converted: bits above 15 are set exactly when
+   the value lies outside [0, 65535]; inside, the low 16 bits are the u16 value */
+__device__ __forceinline__ int clamp_q(float v) { return (int)__builtin_amdgcn_fmed3f(v, -1.0f, 65536.0f); }
+
 template <typename T> struct Val;
 template <> struct Val<uint16_t> {
     /* value as it sits in the u16 image after a float store */
     static __device__ __forceinline__ float store(float v) { return (float)f2u16(v); }
-    /* round<T> (ComputeGradient.hpp:236-244) */
-    static __device__ __forceinline__ float rnd(float v, int& err) {
-        v = roundf(v);
-        if (v < 0.0f || v > 65535.0f) err++;
+    /* round<T> (ComputeGradient.hpp:236-244); the range test is folded into acc, the OR of the
+       clamped values (a bit above 15 set <=> some value left [0, 65535], where the reference
+       throws) */
+    static __device__ __forceinline__ float rnd(float v, int& acc) {
+        v = round_away(v);
+        acc |= clamp_q(v);
         return v;
     }
     static __device__ __forceinline__ uint16_t to_t(float v) { return f2u16(v); }
+    static __device__ __forceinline__ bool failed(int acc) { return (acc & ~0xFFFF) != 0; }
 };
 template <> struct Val<float> {
     static __device__ __forceinline__ float store(float v) { return v; }
     static __device__ __forceinline__ float rnd(float v, int&) { return v; }
     static __device__ __forceinline__ float to_t(float v) { return v; }
+    static __device__ __forceinline__ bool failed(int) { return false; }
 };
 
 /* input sample: u16 image + bspline offset (copyFromMeshWithUnaryOp, PixelData.hpp:788-811) */
@@ -54,20 +69,28 @@ __device__ __forceinline__ float load_in(const T* p, float off, bool has_off) {
     return (float)(*p);
 }
 
-__global__ void k_minmax_init(uint32_t* mm) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        mm[0] = 65535u;
-        mm[1] = 0u;
+struct MinMaxInit {
+    struct Args {
+        uint32_t* mm;
+    };
+    static constexpr int THREADS = 64;
+    static const char* name() { return "minmax_init"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t) {
+        if (threadIdx.x == 0) {
+            a.mm[0] = 65535u;
+            a.mm[1] = 0u;
+        }
     }
-}
+};
 
-__global__ void k_minmax(const uint16_t* img, size_t n, uint32_t* mm) {
+/* grid-stride over a virtual grid of G blocks (tile = block) */
+__device__ __forceinline__ void minmax_body(const uint16_t* img, size_t n, uint32_t* mm, int64_t G, int64_t tile) {
     uint32_t lo = 65535u, hi = 0u;
     /* 16-byte loads over the aligned body, scalar tail */
     const size_t nv = n / 8;
     const uint4* v4 = (const uint4*)img;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)G * blockDim.x;
+    size_t i = (size_t)tile * blockDim.x + threadIdx.x;
     for (; i + 3 * stride < nv; i += 4 * stride) {
         uint4 a[4];
 #pragma unroll
@@ -93,7 +116,7 @@ __global__ void k_minmax(const uint16_t* img, size_t n, uint32_t* mm) {
             hi = max(hi, max(l, h));
         }
     }
-    for (size_t j = nv * 8 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+    for (size_t j = nv * 8 + (size_t)tile * blockDim.x + threadIdx.x; j < n; j += stride) {
         uint32_t v = img[j];
         lo = v < lo ? v : lo;
         hi = v > hi ? v : hi;
@@ -109,31 +132,64 @@ __global__ void k_minmax(const uint16_t* img, size_t n, uint32_t* mm) {
     }
 }
 
-__global__ void k_offset(const uint32_t* mm, float lambda, float* off) {
-    if (lambda <= 0.0f) {
-        *off = 0.0f;
-        return;
-    }
-    int mn = (int)mm[0], mx = (int)mm[1];
-    float o = (float)((65535 - (mx - mn)) / 2 - mn);
-    *off = fmaxf(fminf(o, 100.0f), -100.0f);
-}
+struct MinMax {
+    struct Args {
+        const uint16_t* img;
+        size_t n;
+        uint32_t* mm;
+        int64_t G;
+    };
+    static constexpr int THREADS = 256;
+    static const char* name() { return "minmax"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) { minmax_body(a.img, a.n, a.mm, a.G, t); }
+};
 
-__global__ void k_offset_copy(const uint16_t* in, uint16_t* out, size_t n, const float* d_off) {
-    const float off = *d_off;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        out[i] = f2u16((float)in[i] + off);
-}
+struct Offset {
+    struct Args {
+        const uint32_t* mm;
+        float lambda;
+        float* off;
+    };
+    static constexpr int THREADS = 64;
+    static const char* name() { return "bspline_offset"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t) {
+        if (threadIdx.x != 0) return;
+        if (a.lambda <= 0.0f) {
+            *a.off = 0.0f;
+            return;
+        }
+        int mn = (int)a.mm[0], mx = (int)a.mm[1];
+        float o = (float)((65535 - (mx - mn)) / 2 - mn);
+        *a.off = fmaxf(fminf(o, 100.0f), -100.0f);
+    }
+};
+
+struct OffsetCopy {
+    struct Args {
+        const uint16_t* in;
+        uint16_t* out;
+        size_t n;
+        const float* d_off;
+        int64_t G;
+    };
+    static constexpr int THREADS = 256;
+    static const char* name() { return "offset_copy"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) {
+        const float off = *a.d_off;
+        for (size_t i = (size_t)t * blockDim.x + threadIdx.x; i < a.n; i += (size_t)a.G * blockDim.x)
+            a.out[i] = f2u16((float)a.in[i] + off);
+    }
+};
 
 constexpr int TY = 64;   /* samples per tile row  */
 constexpr int TL = 64;   /* lines per tile (wave) */
 
 template <typename T>
-__global__ __launch_bounds__(64) void k_iir_y(const T* __restrict__ in, T* out, int64_t y_num, int64_t n_lines, IIRCoeffs c,
-                                              const float* d_off, int* d_err) {
+__device__ __forceinline__ void iir_y_body(const T* __restrict__ in, T* out, int64_t y_num, int64_t n_lines, const IIRCoeffs& c,
+                                           const float* d_off, int* d_err, int64_t blk) {
     __shared__ float tile[TL][TY + 1];
     const int lane = threadIdx.x;
-    const int64_t line0 = (int64_t)blockIdx.x * TL;
+    const int64_t line0 = blk * TL;
     const int64_t me = line0 + lane;
     const bool valid = me < n_lines;
     const bool has_off = d_off != nullptr;
@@ -230,17 +286,17 @@ __global__ __launch_bounds__(64) void k_iir_y(const T* __restrict__ in, T* out, 
         }
         __syncthreads();
     }
-    if (err) atomicAdd(d_err, err);
+    if (Val<T>::failed(err)) atomicAdd(d_err, 1);
 }
 
 /* x / z pass: lines along an outer axis of length n and element stride s;
    thread = one y of one line (ComputeGradient.hpp:415-599, 601-779) */
 template <typename T>
-__global__ __launch_bounds__(256) void k_iir_outer(T* img, int64_t y_num, int64_t n, int64_t s, int64_t outer_stride,
-                                                   IIRCoeffs c, int* d_err) {
-    const int64_t y = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void iir_outer_body(T* img, int64_t y_num, int64_t n, int64_t s, int64_t outer_stride, const IIRCoeffs& c,
+                                               int* d_err, int64_t bx, int64_t by) {
+    const int64_t y = bx * blockDim.x + threadIdx.x;
     if (y >= y_num) return;
-    T* base = img + (int64_t)blockIdx.y * outer_stride + y;
+    T* base = img + by * outer_stride + y;
     const float b1 = c.b1, b2 = c.b2, nf = c.nf;
     int err = 0;
     float v1 = 0, v2 = 0, v3 = 0, v4 = 0;
@@ -274,7 +330,7 @@ __global__ __launch_bounds__(256) void k_iir_outer(T* img, int64_t y_num, int64_
         q4 = q3;
         q3 = t;
     }
-    if (err) atomicAdd(d_err, err);
+    if (Val<T>::failed(err)) atomicAdd(d_err, 1);
 }
 
 
@@ -295,6 +351,12 @@ template <> struct Vec16<uint16_t> {
         uint32_t* w = (i < 2) ? &v.x : (i < 4) ? &v.y : (i < 6) ? &v.z : &v.w;
         *w = (i & 1) ? ((*w & 0xFFFFu) | (h << 16)) : ((*w & 0xFFFF0000u) | h);
     }
+    /* store of an integral value that is in [0, 65535] unless the pass reports an overflow */
+    static __device__ __forceinline__ void set_r(uint4& v, int i, float f) {
+        const uint32_t h = (uint32_t)clamp_q(f) & 0xFFFFu;
+        uint32_t* w = (i < 2) ? &v.x : (i < 4) ? &v.y : (i < 6) ? &v.z : &v.w;
+        *w = (i & 1) ? ((*w & 0xFFFFu) | (h << 16)) : ((*w & 0xFFFF0000u) | h);
+    }
 };
 template <> struct Vec16<float> {
     static constexpr int N = 4;
@@ -306,19 +368,21 @@ template <> struct Vec16<float> {
         const uint32_t b = __float_as_uint(f);
         if (i == 0) v.x = b; else if (i == 1) v.y = b; else if (i == 2) v.z = b; else v.w = b;
     }
+    static __device__ __forceinline__ void set_r(uint4& v, int i, float f) { set(v, i, f); }
 };
 
 template <typename T>
 __device__ __forceinline__ float vin(const uint4& v, int i, float off, bool has_off) {
     const float x = Vec16<T>::get(v, i);
-    return has_off ? (float)f2u16(x + off) : x;
+    /* x + offset lies in [-100, 65635]: the int32 conversion is the x86 one */
+    return has_off ? (float)((uint32_t)(int)(x + off) & 0xFFFFu) : x;
 }
 
 template <typename T, int D>
-__global__ __launch_bounds__(256) void k_iir_y_vec(const T* __restrict__ in, T* out, int64_t y_num, int64_t n_lines, IIRCoeffs c,
-                                                   const float* d_off, int* d_err) {
+__device__ __forceinline__ void iir_y_vec_body(const T* __restrict__ in, T* out, int64_t y_num, int64_t n_lines, const IIRCoeffs& c,
+                                               const float* d_off, int* d_err, int64_t blk) {
     constexpr int V = Vec16<T>::N;
-    const int64_t me = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t me = blk * blockDim.x + threadIdx.x;
     if (me >= n_lines) return;
     const bool has_off = d_off != nullptr;
     const float off = has_off ? *d_off : 0.0f;
@@ -358,6 +422,8 @@ __global__ __launch_bounds__(256) void k_iir_y_vec(const T* __restrict__ in, T* 
     }
     /* causal pass: state unrounded (ComputeGradient.hpp:359-373) */
     float s1 = t1, s2 = t2;
+    /* y(0), y(1) are stored truncated, without the round<T> check (ComputeGradient.hpp:365-366) */
+    const float t2q = Val<T>::store(t2), t1q = Val<T>::store(t1);
     for (int64_t b0 = 0; b0 < nv; b0 += D) {
         uint4 buf[D];
 #pragma unroll
@@ -372,16 +438,16 @@ __global__ __launch_bounds__(256) void k_iir_y_vec(const T* __restrict__ in, T* 
                 const int64_t k = (b0 + d) * V + i;
                 float r;
                 if (k == 0) {
-                    r = t2;
+                    r = t2q;
                 } else if (k == 1) {
-                    r = t1;
+                    r = t1q;
                 } else {
                     const float t = s1 * b1 + s2 * b2 + vin<T>(buf[d], i, off, has_off);
                     r = Val<T>::rnd(t, err);
                     s2 = s1;
                     s1 = t;
                 }
-                Vec16<T>::set(o, i, r);
+                Vec16<T>::set_r(o, i, r);
             }
             dst[b0 + d] = o;
         }
@@ -416,22 +482,22 @@ __global__ __launch_bounds__(256) void k_iir_y_vec(const T* __restrict__ in, T* 
                     r2 = r1;
                     r1 = t;
                 }
-                Vec16<T>::set(o, i, r);
+                Vec16<T>::set_r(o, i, r);
             }
             dst[b0 - d] = o;
         }
     }
-    if (err) atomicAdd(d_err, err);
+    if (Val<T>::failed(err)) atomicAdd(d_err, 1);
 }
 
 /* x / z pass, 16-byte vectors along y, D steps of the recursion loaded ahead */
 template <typename T, int D>
-__global__ __launch_bounds__(256) void k_iir_outer_vec(T* img, int64_t y_num, int64_t n, int64_t s, int64_t outer_stride, IIRCoeffs c,
-                                                       int* d_err) {
+__device__ __forceinline__ void iir_outer_vec_body(T* img, int64_t y_num, int64_t n, int64_t s, int64_t outer_stride, const IIRCoeffs& c,
+                                                   int* d_err, int64_t bx, int64_t by) {
     constexpr int V = Vec16<T>::N;
-    const int64_t yv = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t yv = bx * blockDim.x + threadIdx.x;
     if (yv * V >= y_num) return;
-    T* basep = img + (int64_t)blockIdx.y * outer_stride + yv * V;
+    T* basep = img + by * outer_stride + yv * V;
     const int64_t sv = s / V;   /* stride in vectors */
     uint4* base = (uint4*)basep;
     const float b1 = c.b1, b2 = c.b2, nf = c.nf;
@@ -456,8 +522,8 @@ __global__ __launch_bounds__(256) void k_iir_outer_vec(T* img, int64_t y_num, in
         uint4 o0 = make_uint4(0, 0, 0, 0), o1 = make_uint4(0, 0, 0, 0);
 #pragma unroll
         for (int i = 0; i < V; ++i) {
-            Vec16<T>::set(o0, i, Val<T>::rnd(v2[i], err));
-            Vec16<T>::set(o1, i, Val<T>::rnd(v1[i], err));
+            Vec16<T>::set_r(o0, i, Val<T>::rnd(v2[i], err));
+            Vec16<T>::set_r(o1, i, Val<T>::rnd(v1[i], err));
         }
         base[0] = o0;
         base[sv] = o1;
@@ -481,7 +547,7 @@ __global__ __launch_bounds__(256) void k_iir_outer_vec(T* img, int64_t y_num, in
 #pragma unroll
             for (int i = 0; i < V; ++i) {
                 const float t = Val<T>::rnd(Vec16<T>::get(buf[d], i) + b1 * p1[i] + b2 * p2[i], err);
-                Vec16<T>::set(o, i, t);
+                Vec16<T>::set_r(o, i, t);
                 p2[i] = p1[i];
                 p1[i] = t;
             }
@@ -493,8 +559,8 @@ __global__ __launch_bounds__(256) void k_iir_outer_vec(T* img, int64_t y_num, in
         uint4 oa = make_uint4(0, 0, 0, 0), ob = make_uint4(0, 0, 0, 0);
 #pragma unroll
         for (int i = 0; i < V; ++i) {
-            Vec16<T>::set(oa, i, Val<T>::rnd(v4[i] * nf, err));
-            Vec16<T>::set(ob, i, Val<T>::rnd(v3[i] * nf, err));
+            Vec16<T>::set_r(oa, i, Val<T>::rnd(v4[i] * nf, err));
+            Vec16<T>::set_r(ob, i, Val<T>::rnd(v3[i] * nf, err));
         }
         base[(n - 1) * sv] = oa;
         base[(n - 2) * sv] = ob;
@@ -511,14 +577,14 @@ __global__ __launch_bounds__(256) void k_iir_outer_vec(T* img, int64_t y_num, in
 #pragma unroll
             for (int i = 0; i < V; ++i) {
                 const float t = (Vec16<T>::get(buf[d], i) + b1 * v3[i]) + b2 * v4[i];
-                Vec16<T>::set(o, i, Val<T>::rnd(t * nf, err));
+                Vec16<T>::set_r(o, i, Val<T>::rnd(t * nf, err));
                 v4[i] = v3[i];
                 v3[i] = t;
             }
             base[(j0 - d) * sv] = o;
         }
     }
-    if (err) atomicAdd(d_err, err);
+    if (Val<T>::failed(err)) atomicAdd(d_err, 1);
 }
 
 /* y pass through an LDS tile with full-line coalescing: a wave owns 64 lines;
@@ -527,13 +593,13 @@ __global__ __launch_bounds__(256) void k_iir_outer_vec(T* img, int64_t y_num, in
  * free for 16-byte LDS accesses), and each lane then runs the recursion over
  * its own row.  The next chunk's global loads are issued before the current
  * chunk is processed.  u16 only; requires y_num % 64 == 0. */
-__global__ __launch_bounds__(64) void k_iir_y_lds(const uint16_t* __restrict__ in, uint16_t* out, int64_t y_num, int64_t n_lines,
-                                                  IIRCoeffs c, const float* d_off, int* d_err) {
+__device__ __forceinline__ void iir_y_lds_body(const uint16_t* __restrict__ in, uint16_t* out, int64_t y_num, int64_t n_lines,
+                                               const IIRCoeffs& c, const float* d_off, int* d_err, int64_t blk) {
     constexpr int CH = 64;            /* samples per line per chunk */
     constexpr int PITCH = 72;         /* u16 per LDS row (144 B)    */
     __shared__ __attribute__((aligned(16))) uint16_t tile[64 * PITCH];
     const int lane = threadIdx.x;
-    const int64_t line0 = (int64_t)blockIdx.x * 64;
+    const int64_t line0 = blk * 64;
     const int64_t me = line0 + lane;
     const bool valid = me < n_lines;
     const bool has_off = d_off != nullptr;
@@ -597,6 +663,8 @@ __global__ __launch_bounds__(64) void k_iir_y_lds(const uint16_t* __restrict__ i
 
     /* causal pass */
     float s1 = t1, s2 = t2;
+    /* y(0), y(1) are stored truncated, without the round<T> check (ComputeGradient.hpp:365-366) */
+    const float t2q = Val<uint16_t>::store(t2), t1q = Val<uint16_t>::store(t1);
     uint4 nxt[8];
     gload(in, 0, nxt);
     for (int64_t ch = 0; ch < nch; ++ch) {
@@ -615,16 +683,16 @@ __global__ __launch_bounds__(64) void k_iir_y_lds(const uint16_t* __restrict__ i
                     const int64_t k = ch * CH + q * 8 + i;
                     float r;
                     if (k == 0) {
-                        r = t2;
+                        r = t2q;
                     } else if (k == 1) {
-                        r = t1;
+                        r = t1q;
                     } else {
                         const float t = s1 * b1 + s2 * b2 + vin<uint16_t>(v, i, off, has_off);
                         r = Val<uint16_t>::rnd(t, err);
                         s2 = s1;
                         s1 = t;
                     }
-                    Vec16<uint16_t>::set(o, i, r);
+                    Vec16<uint16_t>::set_r(o, i, r);
                 }
                 *(uint4*)(row + q * 8) = o;
             }
@@ -667,7 +735,7 @@ __global__ __launch_bounds__(64) void k_iir_y_lds(const uint16_t* __restrict__ i
                         r2 = r1;
                         r1 = t;
                     }
-                    Vec16<uint16_t>::set(o, i, r);
+                    Vec16<uint16_t>::set_r(o, i, r);
                 }
                 *(uint4*)(row + q * 8) = o;
             }
@@ -675,8 +743,59 @@ __global__ __launch_bounds__(64) void k_iir_y_lds(const uint16_t* __restrict__ i
         __syncthreads();
         gstore(out, ch);
     }
-    if (err) atomicAdd(d_err, err);
+    if (Val<uint16_t>::failed(err)) atomicAdd(d_err, 1);
 }
+
+/* per-volume arguments of the smoothing passes */
+template <typename T>
+struct IirArgs {
+    const T* in;          /* y pass input (may equal out) */
+    T* out;               /* y pass output / image of the x, z passes */
+    int64_t y_num, n;     /* y length; length of the filtered axis (x, z passes) or lines (y) */
+    int64_t s, outer_stride, gx;
+    IIRCoeffs c;
+    const float* d_off;
+    int* d_err;
+};
+
+template <typename T>
+struct IirY {
+    using Args = IirArgs<T>;
+    static constexpr int THREADS = 64;
+    static const char* name() { return "bspline_y"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) { iir_y_body<T>(a.in, a.out, a.y_num, a.n, a.c, a.d_off, a.d_err, t); }
+};
+template <typename T, int D>
+struct IirYVec {
+    using Args = IirArgs<T>;
+    static constexpr int THREADS = 256;
+    static const char* name() { return "bspline_y_vec"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) { iir_y_vec_body<T, D>(a.in, a.out, a.y_num, a.n, a.c, a.d_off, a.d_err, t); }
+};
+struct IirYLds {
+    using Args = IirArgs<uint16_t>;
+    static constexpr int THREADS = 64;
+    static const char* name() { return "bspline_y_lds"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) { iir_y_lds_body(a.in, a.out, a.y_num, a.n, a.c, a.d_off, a.d_err, t); }
+};
+template <typename T>
+struct IirOuter {
+    using Args = IirArgs<T>;
+    static constexpr int THREADS = 256;
+    static const char* name() { return "bspline_outer"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) {
+        iir_outer_body<T>(a.out, a.y_num, a.n, a.s, a.outer_stride, a.c, a.d_err, t % a.gx, t / a.gx);
+    }
+};
+template <typename T, int D>
+struct IirOuterVec {
+    using Args = IirArgs<T>;
+    static constexpr int THREADS = 256;
+    static const char* name() { return "bspline_outer_vec"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) {
+        iir_outer_vec_body<T, D>(a.out, a.y_num, a.n, a.s, a.outer_stride, a.c, a.d_err, t % a.gx, t / a.gx);
+    }
+};
 
 inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
@@ -684,17 +803,15 @@ inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
 void launch_minmax_offset(aprh_context* ctx, const uint16_t* img, size_t n, float lambda, float* d_offset, uint32_t* d_mm) {
     /* initial (min, max) set by a kernel, so the sequence can be captured in a graph */
-    hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(1), 0, ctx->stream, d_mm);
-    HIP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_minmax, dim3(2048), dim3(256), 0, ctx->stream, img, n, d_mm);
-    HIP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_offset, dim3(1), dim3(1), 0, ctx->stream, d_mm, lambda, d_offset);
-    HIP_LAUNCH_CHECK();
+    launch_tiles<MinMaxInit>(ctx, {d_mm}, 1);
+    const int64_t G = stride_tiles((int64_t)n, 256 * 8 * 4, 2048);
+    launch_tiles<MinMax>(ctx, {img, n, d_mm, G}, G);
+    launch_tiles<Offset>(ctx, {d_mm, lambda, d_offset}, 1);
 }
 
 void launch_offset_copy(aprh_context* ctx, const uint16_t* in, uint16_t* out, size_t n, const float* d_offset) {
-    hipLaunchKernelGGL(k_offset_copy, dim3(4096), dim3(256), 0, ctx->stream, in, out, n, d_offset);
-    HIP_LAUNCH_CHECK();
+    const int64_t G = stride_tiles((int64_t)n, 256 * 4);
+    launch_tiles<OffsetCopy>(ctx, {in, out, n, d_offset, G}, G);
 }
 
 template <typename T>
@@ -702,42 +819,33 @@ void launch_iir_y(aprh_context* ctx, const T* in, T* out, int64_t y, int64_t x, 
                   const float* d_offset, int* d_err) {
     const int64_t lines = x * z;
     constexpr int V = Vec16<T>::N;
-    if (std::is_same<T, uint16_t>::value && y % 64 == 0 && aligned16(in) && aligned16(out)) {
-        hipLaunchKernelGGL(k_iir_y_lds, dim3((unsigned)cdiv(lines, 64)), dim3(64), 0, ctx->stream, (const uint16_t*)in,
-                           (uint16_t*)out, y, lines, c, d_offset, d_err);
-    } else if (y % V == 0 && y >= 2 * V && aligned16(in) && aligned16(out)) {
-        hipLaunchKernelGGL((k_iir_y_vec<T, 4>), dim3((unsigned)cdiv(lines, 256)), dim3(256), 0, ctx->stream, in, out, y, lines, c,
-                           d_offset, d_err);
-    } else {
-        hipLaunchKernelGGL(k_iir_y<T>, dim3((unsigned)cdiv(lines, TL)), dim3(64), 0, ctx->stream, in, out, y, lines, c, d_offset,
-                           d_err);
+    IirArgs<T> a{in, out, y, lines, 0, 0, 1, c, d_offset, d_err};
+    if constexpr (std::is_same<T, uint16_t>::value) {
+        if (y % 64 == 0 && aligned16(in) && aligned16(out)) {
+            launch_tiles<IirYLds>(ctx, a, cdiv(lines, 64));
+            return;
+        }
     }
-    HIP_LAUNCH_CHECK();
+    if (y % V == 0 && y >= 2 * V && aligned16(in) && aligned16(out)) {
+        launch_tiles<IirYVec<T, 4>>(ctx, a, cdiv(lines, 256));
+    } else {
+        launch_tiles<IirY<T>>(ctx, a, cdiv(lines, TL));
+    }
 }
 
 template <typename T>
 void launch_iir_outer(aprh_context* ctx, T* img, int64_t y, int64_t x, int64_t z, int axis, const IIRCoeffs& c, int* d_err) {
     constexpr int V = Vec16<T>::N;
+    /* x pass: lines along x (stride y), one per (y, z); z pass: along z (stride x*y), one per (y, x) */
+    const int64_t n = axis == 1 ? x : z, s = axis == 1 ? y : x * y, outer = axis == 1 ? z : x;
+    const int64_t ostride = axis == 1 ? x * y : y;
     if (y % V == 0 && aligned16(img)) {
-        const unsigned gx = (unsigned)cdiv(y / V, 256);
-        if (axis == 1) {
-            hipLaunchKernelGGL((k_iir_outer_vec<T, 8>), dim3(gx, (unsigned)z), dim3(256), 0, ctx->stream, img, y, x, y, x * y, c, d_err);
-        } else {
-            hipLaunchKernelGGL((k_iir_outer_vec<T, 8>), dim3(gx, (unsigned)x), dim3(256), 0, ctx->stream, img, y, z, x * y, y, c, d_err);
-        }
-        HIP_LAUNCH_CHECK();
+        const int64_t gx = cdiv(y / V, 256);
+        launch_tiles<IirOuterVec<T, 8>>(ctx, IirArgs<T>{img, img, y, n, s, ostride, gx, c, nullptr, d_err}, gx * outer);
         return;
     }
-    if (axis == 1) {
-        /* x pass: outer = z */
-        hipLaunchKernelGGL(k_iir_outer<T>, dim3((unsigned)cdiv(y, 256), (unsigned)z), dim3(256), 0, ctx->stream, img, y, x, y, x * y,
-                           c, d_err);
-    } else {
-        /* z pass: outer = x */
-        hipLaunchKernelGGL(k_iir_outer<T>, dim3((unsigned)cdiv(y, 256), (unsigned)x), dim3(256), 0, ctx->stream, img, y, z, x * y, y,
-                           c, d_err);
-    }
-    HIP_LAUNCH_CHECK();
+    const int64_t gx = cdiv(y, 256);
+    launch_tiles<IirOuter<T>>(ctx, IirArgs<T>{img, img, y, n, s, ostride, gx, c, nullptr, d_err}, gx * outer);
 }
 
 template void launch_iir_y<uint16_t>(aprh_context*, const uint16_t*, uint16_t*, int64_t, int64_t, int64_t, const IIRCoeffs&,

@@ -21,6 +21,7 @@
 #include <cmath>
 
 #include "aprh_internal.hpp"
+#include "tiles.hpp"
 
 namespace aprh {
 
@@ -33,9 +34,10 @@ struct Dims {
 };
 
 /* level map of level l from level l+1 (max of the clamped 2x2x2 block) + seeds of level l */
-__global__ void k_level_down(const uint8_t* __restrict__ fine, Dims fd, uint8_t* coarse, Dims cd, uint8_t* pct, int level, int mode) {
+__device__ __forceinline__ void level_down_body(const uint8_t* __restrict__ fine, Dims fd, uint8_t* coarse, Dims cd, uint8_t* pct,
+                                                int level, int mode, int64_t G, int64_t t) {
     const size_t n = (size_t)(cd.y * cd.x * cd.z);
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    for (size_t i = (size_t)t * blockDim.x + threadIdx.x; i < n; i += (size_t)G * blockDim.x) {
         const int64_t y = (int64_t)(i % (size_t)cd.y);
         const int64_t r = (int64_t)(i / (size_t)cd.y);
         const int64_t x = r % cd.x, z = r / cd.x;
@@ -56,8 +58,9 @@ __global__ void k_level_down(const uint8_t* __restrict__ fine, Dims fd, uint8_t*
 }
 
 /* seeds of the top level straight from the level map (PullingScheme::fill, k == l_max) */
-__global__ void k_seed(const uint8_t* __restrict__ lev, uint8_t* pct, size_t n, int level, int mode) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+__device__ __forceinline__ void seed_body(const uint8_t* __restrict__ lev, uint8_t* pct, size_t n, int level, int mode, int64_t G,
+                                          int64_t t) {
+    for (size_t i = (size_t)t * blockDim.x + threadIdx.x; i < n; i += (size_t)G * blockDim.x) {
         const uint8_t m = lev[i];
         bool seed = (mode == 0) ? (m >= level) : (mode == 1) ? (m <= level) : (m == level);
         pct[i] = seed ? SEED : EMPTY;
@@ -82,13 +85,13 @@ __device__ __forceinline__ void load_row8(const uint8_t* row, int yn, int y0, bo
     b[9] = (y0 + 8 < yn) ? row[y0 + 8] : (uint8_t)0;
 }
 
-__global__ __launch_bounds__(256) void k_pull_level8(uint8_t* P, Dims d, uint8_t* Pc, Dims dc, uint8_t* Pp, Dims dp, int is_top,
-                                                     int has_parent) {
+__device__ __forceinline__ void pull_level8_body(uint8_t* P, Dims d, uint8_t* Pc, Dims dc, uint8_t* Pp, Dims dp, int is_top,
+                                                 int has_parent, int64_t G, int64_t t) {
     const int yn = (int)d.y, xn = (int)d.x, zn = (int)d.z;
     const int nwr = (yn + 7) >> 3;
     const int64_t total = (int64_t)nwr * xn * zn;
     const bool aligned8 = (yn & 7) == 0 && ((uintptr_t)P & 7) == 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t i = t * blockDim.x + threadIdx.x; i < total; i += G * blockDim.x) {
         const int w = (int)(i % nwr);
         const int r = (int)(i / nwr);
         const int x = r % xn, z = r / xn;
@@ -168,9 +171,10 @@ __global__ __launch_bounds__(256) void k_pull_level8(uint8_t* P, Dims d, uint8_t
 }
 
 /* equivalence optimisation step for one level (LinearAccess.hpp:265-294) */
-__global__ void k_upseed(uint8_t* P, Dims d, const uint8_t* __restrict__ Pp, Dims dp, int min_type) {
+__device__ __forceinline__ void upseed_body(uint8_t* P, Dims d, const uint8_t* __restrict__ Pp, Dims dp, int min_type, int64_t G,
+                                            int64_t t) {
     const size_t n = (size_t)(d.y * d.x * d.z);
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    for (size_t i = (size_t)t * blockDim.x + threadIdx.x; i < n; i += (size_t)G * blockDim.x) {
         const int64_t y = (int64_t)(i % (size_t)d.y);
         const int64_t r = (int64_t)(i / (size_t)d.y);
         const int64_t x = r % d.x, z = r / d.x;
@@ -236,10 +240,11 @@ __device__ __forceinline__ void chunk_weights(const RowTable& t, int li, const u
 }
 
 /* particles of each APR row (one wave per row, 1024 cells per step) */
-__global__ __launch_bounds__(256) void k_row_count(const uint8_t* __restrict__ maps, RowTable t, int64_t nrows, uint32_t* cnt) {
+__device__ __forceinline__ void row_count_body(const uint8_t* __restrict__ maps, const RowTable& t, int64_t nrows, uint32_t* cnt,
+                                               int64_t G, int64_t blk) {
     const int lane = threadIdx.x & 63;
-    const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int64_t wid = (blk * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = (G * blockDim.x) >> 6;
     for (int64_t r = wid; r < nrows; r += nw) {
         const int li = row_level_of(t, r);
         const uint8_t* p = apr_row_cells(t, maps, li, r - t.row_prefix[li]);
@@ -258,11 +263,11 @@ __global__ __launch_bounds__(256) void k_row_count(const uint8_t* __restrict__ m
 
 /* y values of each APR row from its scanned offset: one cell per lane per step, positions
    from ballot prefix counts, so consecutive lanes write consecutive y_vec entries */
-__global__ __launch_bounds__(256) void k_row_fill(const uint8_t* __restrict__ maps, RowTable t, int64_t nrows,
-                                                  const uint64_t* __restrict__ row_off, uint16_t* y_vec) {
+__device__ __forceinline__ void row_fill_body(const uint8_t* __restrict__ maps, const RowTable& t, int64_t nrows,
+                                              const uint64_t* __restrict__ row_off, uint16_t* y_vec, int64_t G, int64_t blk) {
     const int lane = threadIdx.x & 63;
-    const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int64_t wid = (blk * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = (G * blockDim.x) >> 6;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (int64_t r = wid; r < nrows; r += nw) {
         const int li = row_level_of(t, r);
@@ -290,18 +295,19 @@ __global__ __launch_bounds__(256) void k_row_fill(const uint8_t* __restrict__ ma
 }
 
 /* xz_end_vec of each row: its offset + its count */
-__global__ void k_rows_xz(const uint64_t* __restrict__ row_off, const uint32_t* __restrict__ cnt, RowTable t, uint64_t* xz_end,
-                          int64_t nrows) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void rows_xz_body(const uint64_t* __restrict__ row_off, const uint32_t* __restrict__ cnt, const RowTable& t,
+                                             uint64_t* xz_end, int64_t nrows, int64_t blk) {
+    const int64_t r = blk * blockDim.x + threadIdx.x;
     if (r >= nrows) return;
     const int li = row_level_of(t, r);
     xz_end[t.row_begin[li] + (r - t.row_prefix[li])] = row_off[r] + cnt[r];
 }
 
 /* exclusive scan, three phases over blocks of 1024 */
-__global__ __launch_bounds__(1024) void k_scan_blocks(const uint32_t* __restrict__ in, uint64_t* out, uint64_t* block_sums, int64_t n) {
+__device__ __forceinline__ void scan_blocks_body(const uint32_t* __restrict__ in, uint64_t* out, uint64_t* block_sums, int64_t n,
+                                                 int64_t blk) {
     __shared__ uint64_t sh[1024];
-    const int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+    const int64_t i = blk * 1024 + threadIdx.x;
     uint64_t v = i < n ? in[i] : 0;
     sh[threadIdx.x] = v;
     __syncthreads();
@@ -312,10 +318,11 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(const uint32_t* __restrict
         __syncthreads();
     }
     if (i < n) out[i] = sh[threadIdx.x] - v;
-    if (threadIdx.x == 1023) block_sums[blockIdx.x] = sh[1023];
+    if (threadIdx.x == 1023) block_sums[blk] = sh[1023];
+    __syncthreads();   /* sh is reused by the next tile of a persistent block */
 }
 
-__global__ __launch_bounds__(1024) void k_scan_sums(uint64_t* block_sums, int64_t nb, uint64_t* total) {
+__device__ __forceinline__ void scan_sums_body(uint64_t* block_sums, int64_t nb, uint64_t* total) {
     /* single block: exclusive scan of the block sums (serial chunks per thread) */
     __shared__ uint64_t sh[1024];
     const int64_t per = (nb + 1023) / 1024;
@@ -339,10 +346,136 @@ __global__ __launch_bounds__(1024) void k_scan_sums(uint64_t* block_sums, int64_
     if (threadIdx.x == 1023) *total = sh[1023];
 }
 
-__global__ void k_scan_add(uint64_t* out, const uint64_t* __restrict__ block_sums, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void scan_add_body(uint64_t* out, const uint64_t* __restrict__ block_sums, int64_t n, int64_t blk) {
+    const int64_t i = blk * blockDim.x + threadIdx.x;
     if (i < n) out[i] += block_sums[i / 1024];
 }
+
+/* tile bodies of the pulling scheme and the structure */
+struct Seed {
+    struct Args {
+        const uint8_t* lev;
+        uint8_t* pct;
+        size_t n;
+        int level, mode;
+        int64_t G;
+    };
+    static constexpr int THREADS = 256;
+    static const char* name() { return "pulling_seed"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) { seed_body(a.lev, a.pct, a.n, a.level, a.mode, a.G, t); }
+};
+struct LevelDown {
+    struct Args {
+        const uint8_t* fine;
+        Dims fd;
+        uint8_t* coarse;
+        Dims cd;
+        uint8_t* pct;
+        int level, mode;
+        int64_t G;
+    };
+    static constexpr int THREADS = 256;
+    static const char* name() { return "pulling_level_down"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) { level_down_body(a.fine, a.fd, a.coarse, a.cd, a.pct, a.level, a.mode, a.G, t); }
+};
+struct PullLevel {
+    struct Args {
+        uint8_t* P;
+        Dims d;
+        uint8_t* Pc;
+        Dims dc;
+        uint8_t* Pp;
+        Dims dp;
+        int is_top, has_parent;
+        int64_t G;
+    };
+    static constexpr int THREADS = 256;
+    static const char* name() { return "pulling_level"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) {
+        pull_level8_body(a.P, a.d, a.Pc, a.dc, a.Pp, a.dp, a.is_top, a.has_parent, a.G, t);
+    }
+};
+struct Upseed {
+    struct Args {
+        uint8_t* P;
+        Dims d;
+        const uint8_t* Pp;
+        Dims dp;
+        int min_type;
+        int64_t G;
+    };
+    static constexpr int THREADS = 256;
+    static const char* name() { return "upseed"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) { upseed_body(a.P, a.d, a.Pp, a.dp, a.min_type, a.G, t); }
+};
+struct RowCount {
+    struct Args {
+        const uint8_t* maps;
+        RowTable t;
+        int64_t nrows;
+        uint32_t* cnt;
+        int64_t G;
+    };
+    static constexpr int THREADS = 256;
+    static const char* name() { return "row_count"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) { row_count_body(a.maps, a.t, a.nrows, a.cnt, a.G, t); }
+};
+struct RowFill {
+    struct Args {
+        const uint8_t* maps;
+        RowTable t;
+        int64_t nrows;
+        const uint64_t* row_off;
+        uint16_t* y_vec;
+        int64_t G;
+    };
+    static constexpr int THREADS = 256;
+    static const char* name() { return "row_fill"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) { row_fill_body(a.maps, a.t, a.nrows, a.row_off, a.y_vec, a.G, t); }
+};
+struct RowsXz {
+    struct Args {
+        const uint64_t* row_off;
+        const uint32_t* cnt;
+        RowTable t;
+        uint64_t* xz_end;
+        int64_t nrows;
+    };
+    static constexpr int THREADS = 256;
+    static const char* name() { return "rows_xz"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) { rows_xz_body(a.row_off, a.cnt, a.t, a.xz_end, a.nrows, t); }
+};
+struct ScanBlocks {
+    struct Args {
+        const uint32_t* in;
+        uint64_t* out;
+        uint64_t* block_sums;
+        int64_t n;
+    };
+    static constexpr int THREADS = 1024;
+    static const char* name() { return "scan_blocks"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) { scan_blocks_body(a.in, a.out, a.block_sums, a.n, t); }
+};
+struct ScanSums {
+    struct Args {
+        uint64_t* block_sums;
+        int64_t nb;
+        uint64_t* total;
+    };
+    static constexpr int THREADS = 1024;
+    static const char* name() { return "scan_sums"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t) { scan_sums_body(a.block_sums, a.nb, a.total); }
+};
+struct ScanAdd {
+    struct Args {
+        uint64_t* out;
+        const uint64_t* block_sums;
+        int64_t n;
+    };
+    static constexpr int THREADS = 256;
+    static const char* name() { return "scan_add"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) { scan_add_body(a.out, a.block_sums, a.n, t); }
+};
 
 /* mean of the 2x2x2 block in the reference's summation order (PixelData.hpp:952-962), stored
    as the pyramid's element type (u16: truncation like a float -> uint16 assignment) */
@@ -414,19 +547,15 @@ __global__ __launch_bounds__(256) void k_gather(LevelPtrs<T> L, const uint64_t* 
 }  // namespace
 
 void exclusive_scan_u32_async(aprh_context* ctx, const uint32_t* in, uint64_t* out, int64_t n, uint64_t* d_total) {
-    hipStream_t st = ctx->stream;
     if (n <= 0) {
         launch_zero(ctx, d_total, 8);
         return;
     }
     const int64_t nb = cdiv(n, 1024);
     Buf bsum(ctx->pool, (size_t)(nb + 1) * 8);
-    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(1024), 0, st, in, out, bsum.as<uint64_t>(), n);
-    HIP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, st, bsum.as<uint64_t>(), nb, d_total);
-    HIP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st, out, bsum.as<uint64_t>(), n);
-    HIP_LAUNCH_CHECK();
+    launch_tiles<ScanBlocks>(ctx, {in, out, bsum.as<uint64_t>(), n}, nb);
+    launch_tiles<ScanSums>(ctx, {bsum.as<uint64_t>(), nb, d_total}, 1);
+    launch_tiles<ScanAdd>(ctx, {out, bsum.as<uint64_t>(), n}, cdiv(n, 256));
 }
 
 uint64_t exclusive_scan_u32(aprh_context* ctx, const uint32_t* in, uint64_t* out, int64_t n) {
@@ -443,7 +572,6 @@ uint64_t exclusive_scan_u32(aprh_context* ctx, const uint32_t* in, uint64_t* out
 void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, const aprh_params& par) {
     hipStream_t st = ctx->stream;
     const int lmax = a.l_max, lmin = a.l_min;
-    const dim3 grid(4096), block(256);
     if (lmax <= 2) {
         if (ctx->async_mode) throw Error(APRH_ERR_UNSUPPORTED, "batched conversion of volumes with l_max <= 2");
         /* LinearAccess.hpp:229-260: every pixel is a particle at l_max */
@@ -484,13 +612,13 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
     /* seeds of the top level straight from the level map, then the max pyramid */
     {
         const size_t n = (size_t)(a.y_num[top] * a.x_num[top] * a.z_num[top]);
-        hipLaunchKernelGGL(k_seed, grid, block, 0, st, level_top, P + off[top], n, top, mode_of(top));
-        HIP_LAUNCH_CHECK();
+        const int64_t G = stride_tiles((int64_t)n, 256 * 4);
+        launch_tiles<Seed>(ctx, {level_top, P + off[top], n, top, mode_of(top), G}, G);
     }
     const uint8_t* fine = level_top;
     for (int l = top - 1; l >= lmin; --l) {
-        hipLaunchKernelGGL(k_level_down, grid, block, 0, st, fine, D(l + 1), LV + off[l], D(l), P + off[l], l, mode_of(l));
-        HIP_LAUNCH_CHECK();
+        const int64_t G = stride_tiles(off[l + 1] - off[l], 256 * 4);
+        launch_tiles<LevelDown>(ctx, {fine, D(l + 1), LV + off[l], D(l), P + off[l], l, mode_of(l), G}, G);
         fine = LV + off[l];
     }
     lev.release();
@@ -500,16 +628,17 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
         const bool is_top = (l == top);
         uint8_t* Pc = is_top ? nullptr : P + off[l + 1];
         uint8_t* Pp = has_parent ? P + off[l - 1] : nullptr;
-        hipLaunchKernelGGL(k_pull_level8, grid, block, 0, st, P + off[l], D(l), Pc, is_top ? Dims{1, 1, 1} : D(l + 1), Pp,
-                           has_parent ? D(l - 1) : Dims{1, 1, 1}, is_top ? 1 : 0, has_parent ? 1 : 0);
-        HIP_LAUNCH_CHECK();
+        const int64_t G = stride_tiles(cdiv(a.y_num[l], 8) * a.x_num[l] * a.z_num[l], 256 * 2);
+        launch_tiles<PullLevel>(ctx, {P + off[l], D(l), Pc, is_top ? Dims{1, 1, 1} : D(l + 1), Pp, has_parent ? D(l - 1) : Dims{1, 1, 1},
+                                      is_top ? 1 : 0, has_parent ? 1 : 0, G},
+                                G);
     }
     const int min_type = par.neighborhood_optimization ? 1 : 2;
     mark(ctx, ST_STRUCTURE);
     /* equivalence optimisation, levels upward in order */
     for (int l = lmin + 1; l <= top; ++l) {
-        hipLaunchKernelGGL(k_upseed, grid, block, 0, st, P + off[l], D(l), P + off[l - 1], D(l - 1), min_type);
-        HIP_LAUNCH_CHECK();
+        const int64_t G = stride_tiles(off[l + 1] - off[l], 256 * 4);
+        launch_tiles<Upseed>(ctx, {P + off[l], D(l), P + off[l - 1], D(l - 1), min_type, G}, G);
     }
     /* segment table over APR levels lmin..lmax */
     RowTable t;
@@ -536,9 +665,8 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
     /* per-row counts, exclusive scan over rows in particle order, xz_end_vec, y_vec */
     Buf cnt(ctx->pool, (size_t)rp * 4);
     Buf rowoff(ctx->pool, (size_t)rp * 8);
-    const dim3 rgrid((unsigned)std::min<int64_t>(cdiv(rp, 4), 16384));
-    hipLaunchKernelGGL(k_row_count, rgrid, dim3(256), 0, st, P, t, rp, cnt.as<uint32_t>());
-    HIP_LAUNCH_CHECK();
+    const int64_t RG = stride_tiles(rp, 4, 16384);   /* a wave per row, four waves per block */
+    launch_tiles<RowCount>(ctx, {P, t, rp, cnt.as<uint32_t>(), RG}, RG);
     if (ctx->async_mode) {
         /* y_vec and xz_end_vec are preallocated (y_vec for the upper bound P <= N); the count
            stays on the device */
@@ -549,11 +677,8 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
         a.y_vec.reset(ctx->pool, std::max<uint64_t>(a.total, 1) * 2);
     }
     launch_zero(ctx, a.xz_end.ptr(), a.n_xz * 8);
-    hipLaunchKernelGGL(k_rows_xz, dim3((unsigned)cdiv(rp, 256)), dim3(256), 0, st, (const uint64_t*)rowoff.as<uint64_t>(),
-                       (const uint32_t*)cnt.as<uint32_t>(), t, a.xz_end.as<uint64_t>(), rp);
-    HIP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_row_fill, rgrid, dim3(256), 0, st, P, t, rp, (const uint64_t*)rowoff.as<uint64_t>(), a.y_vec.as<uint16_t>());
-    HIP_LAUNCH_CHECK();
+    launch_tiles<RowsXz>(ctx, {rowoff.as<uint64_t>(), cnt.as<uint32_t>(), t, a.xz_end.as<uint64_t>(), rp}, cdiv(rp, 256));
+    launch_tiles<RowFill>(ctx, {P, t, rp, rowoff.as<uint64_t>(), a.y_vec.as<uint16_t>(), RG}, RG);
     if (ctx->keep_intermediates) {
         a.pct.clear();
         a.pct_dims.assign(3 * (top + 1), 0);

@@ -33,6 +33,11 @@ def _images(seed0):
 
 def test_batch_matches_single_and_oracle(oracle, ctx):
     batch = A.APRBatch(DIMS, par=_apar(), ctx=ctx)
+    stages = batch.stages()
+    # one launch per stage over the volumes: far fewer launches than volumes x stages
+    assert 0 < len(stages) < 3 * len(DIMS) * 10
+    assert all(1 <= st["grid"] <= st["tiles"] and 1 <= st["volumes"] <= len(DIMS) for st in stages)
+    assert max(st["volumes"] for st in stages) == len(DIMS)
     conv = A.APRConverter(ctx)
     conv.par = _apar()
     for rep, seed0 in enumerate((10, 50)):

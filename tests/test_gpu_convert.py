@@ -65,11 +65,12 @@ def test_bspline_axis(oracle, ctx, axis, dtype):
     assert np.array_equal(g.view(np.uint8), o.view(np.uint8))
 
 
+@pytest.mark.parametrize("shape", [(19, 21, 23), (17, 30, 26)])   # odd y: one sample per lane; even y: two
 @pytest.mark.parametrize("axis", [0, 1, 2])
 @pytest.mark.parametrize("off", [1, 2, 3, 6])
-def test_sat_mean_axis(oracle, ctx, axis, off):
+def test_sat_mean_axis(oracle, ctx, axis, off, shape):
     rng = np.random.default_rng(7 + off)
-    img = (rng.random((19, 21, 23)) * 100).astype(np.float32)
+    img = (rng.random(shape) * 100).astype(np.float32)
     g = img.copy()
     A.sat_mean(g, axis, off, ctx)
     o = img.copy()

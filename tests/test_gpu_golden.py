@@ -80,6 +80,18 @@ def test_overflow_detection_gpu(ctx):
         conv.get_apr(img)
 
 
+@pytest.mark.parametrize("shape", [(8, 16, 128), (8, 24, 40), (5, 9, 13)])   # LDS, 16-byte and scalar y passes
+def test_overflow_detection_spike(ctx, shape):
+    """a lone spike near 65535 in a dark volume: the smoothing's negative side lobes leave [0, 65535]"""
+    img = np.zeros(shape, np.uint16)
+    z, x, y = shape
+    img[z // 2, x // 2, y // 2] = 65435
+    conv = A.APRConverter(ctx)
+    conv.par = A.APRParameters(lambda_=2.0)
+    with pytest.raises(A.AprhOverflowError):
+        conv.get_apr(img)
+
+
 @pytest.mark.parametrize("shape,nb,seed", [((2, 3, 4), 1, 4), ((6, 6, 6), 2, 10), ((1, 1, 40), 2, 3), ((24, 1, 24), 4, 11)])
 def test_tiny_volumes_gpu(oracle, ctx, shape, nb, seed):
     """tiny and degenerate shapes (l_max <= 2, constant local scale, 1-wide axes) against the oracle"""
