@@ -202,6 +202,8 @@ void launch_iir_outer(aprh_context* ctx, T* img, int64_t y, int64_t x, int64_t z
 void launch_grad_ds_u16(aprh_context* ctx, const uint16_t* img, uint16_t* grad, float* mean, int64_t y, int64_t x, int64_t z,
                         float hx, float hy, float hz, bool accum);
 void launch_inv_bspline(aprh_context* ctx, float* m, float* tmp, int64_t y, int64_t x, int64_t z, bool ay, bool ax, bool az);
+/* the same passes alternating between a (input) and b; returns the buffer holding the result */
+float* launch_inv_bspline_pp(aprh_context* ctx, float* a, float* b, int64_t y, int64_t x, int64_t z, bool ay, bool ax, bool az);
 struct LisParams {
     int win[6];
     bool active[3];

@@ -98,8 +98,13 @@ void get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x,
     mark(ctx, ST_GRADIENT);
     /* gradient magnitude + 2x2x2 max (ComputeGradient.hpp:92-93), mean downsample and inverse B-spline (:95-118) */
     Buf grad(ctx->pool, nd * 2);
-    Buf mean(ctx->pool, nd * 4);
-    launch_grad_ds_u16(ctx, I, grad.as<uint16_t>(), mean.as<float>(), y, x, z, par.dx, par.dy, par.dz, false);
+    /* the inverse B-spline ping-pongs between mean and tmp: the gradient kernel writes the
+       downsampled mean where an odd number of passes ends in mean (no copy back) */
+    const int npass = par.lambda > 0 ? (int)(y > 2) + (int)(x > 2) + (int)(z > 2) : 0;
+    Buf mean(ctx->pool, nd * 4), tmp;
+    if (npass) tmp.reset(ctx->pool, nd * 4);
+    float* mean_first = (npass & 1) ? tmp.as<float>() : mean.as<float>();
+    launch_grad_ds_u16(ctx, I, grad.as<uint16_t>(), mean_first, y, x, z, par.dx, par.dy, par.dz, false);
     int herr[3] = {0, 0, 0};
     if (ctx->async_mode)
         launch_copy(ctx, ctx->slot_err, d_err, sizeof(herr));
@@ -111,10 +116,8 @@ void get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x,
     }
     img.release();
     mark(ctx, ST_INV_BSPLINE);
-    if (par.lambda > 0) {
-        Buf tmp(ctx->pool, nd * 4);
-        launch_inv_bspline(ctx, mean.as<float>(), tmp.as<float>(), yd, xd, zd, y > 2, x > 2, z > 2);
-    }
+    if (npass) launch_inv_bspline_pp(ctx, mean_first, (npass & 1) ? mean.as<float>() : tmp.as<float>(), yd, xd, zd, y > 2, x > 2, z > 2);
+    tmp.release();
     if (!ctx->async_mode) HIP_CHECK(hipStreamSynchronize(st));
     static const char* axis_name[3] = {"y", "x", "z"};
     for (int i = 0; i < 3; ++i)
@@ -193,18 +196,19 @@ void get_apr_f32(aprh_context* ctx, const float* image, int64_t y, int64_t x, in
     if (par.lambda > 0 && z > 2) launch_iir_outer<float>(ctx, I, y, x, z, 2, iir_coeffs(par.lambda, tol, z), d_err + 2);
     mark(ctx, ST_GRADIENT);
     Buf grad(ctx->pool, nd * 4);
-    Buf mean(ctx->pool, nd * 4);
-    launch_grad_ds_f32(ctx, I, grad.as<float>(), mean.as<float>(), y, x, z, par.dx, par.dy, par.dz, false);
+    const int npass = par.lambda > 0 ? (int)(y > 2) + (int)(x > 2) + (int)(z > 2) : 0;
+    Buf mean(ctx->pool, nd * 4), tmp;
+    if (npass) tmp.reset(ctx->pool, nd * 4);
+    float* mean_first = (npass & 1) ? tmp.as<float>() : mean.as<float>();
+    launch_grad_ds_f32(ctx, I, grad.as<float>(), mean_first, y, x, z, par.dx, par.dy, par.dz, false);
     if (ctx->keep_intermediates) {
         keep_copy(ctx, a->smoothed, I, n * 4);
         keep_copy(ctx, a->grad_raw, grad.ptr(), nd * 4);
     }
     img.release();
     mark(ctx, ST_INV_BSPLINE);
-    if (par.lambda > 0) {
-        Buf tmp(ctx->pool, nd * 4);
-        launch_inv_bspline(ctx, mean.as<float>(), tmp.as<float>(), yd, xd, zd, y > 2, x > 2, z > 2);
-    }
+    if (npass) launch_inv_bspline_pp(ctx, mean_first, (npass & 1) ? mean.as<float>() : tmp.as<float>(), yd, xd, zd, y > 2, x > 2, z > 2);
+    tmp.release();
     lis_and_solve<float>(ctx, a, par, mean, grad.as<float>(), d_off);
 }
 

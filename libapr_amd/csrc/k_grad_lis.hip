@@ -721,11 +721,10 @@ void launch_grad_ds_u16(aprh_context* ctx, const uint16_t* img, uint16_t* grad, 
     }
 }
 
-void launch_inv_bspline(aprh_context* ctx, float* m, float* tmp, int64_t y, int64_t x, int64_t z, bool ay, bool ax, bool az) {
-    /* ping-pong m -> tmp -> m ...; result always ends in m */
+float* launch_inv_bspline_pp(aprh_context* ctx, float* a, float* b, int64_t y, int64_t x, int64_t z, bool ay, bool ax, bool az) {
     const size_t n = (size_t)(y * x * z);
-    float* src = m;
-    float* dst = tmp;
+    float* src = a;
+    float* dst = b;
     const int64_t G = stride_tiles((int64_t)n, 256 * 4);
     if (ay) {
         launch_tiles<InvY>(ctx, {src, dst, y, 1, n, G}, G);
@@ -739,7 +738,13 @@ void launch_inv_bspline(aprh_context* ctx, float* m, float* tmp, int64_t y, int6
         launch_tiles<InvOuter>(ctx, {src, dst, z, x * y, n, G}, G);
         std::swap(src, dst);
     }
-    if (src != m) launch_copy(ctx, m, src, n * 4);
+    return src;
+}
+
+void launch_inv_bspline(aprh_context* ctx, float* m, float* tmp, int64_t y, int64_t x, int64_t z, bool ay, bool ax, bool az) {
+    /* ping-pong m -> tmp -> m ...; result always ends in m */
+    float* fin = launch_inv_bspline_pp(ctx, m, tmp, y, x, z, ay, ax, az);
+    if (fin != m) launch_copy(ctx, m, fin, (size_t)(y * x * z) * 4);
 }
 
 LisParams lis_params(const aprh_params& par, int64_t yd, int64_t xd, int64_t zd) {

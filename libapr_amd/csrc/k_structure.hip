@@ -195,6 +195,11 @@ struct RowTable {
     int64_t y_max;                /* y_num at l_max                    */
     int64_t row_prefix[MAXL + 1]; /* first global row of each APR level */
     int min_type;
+    /* the l_max rows (z, x) read pct row (z/2, x/2) of l_max - 1, so up to four of them hold
+       identical particles: work items are the rows below l_max, then one per such pct row */
+    int64_t top_z;                /* z_num at l_max                    */
+    int64_t top_pz;               /* z_num at l_max - 1                */
+    int64_t n_work;               /* row_prefix[top] + map_x * top_pz  */
 };
 
 /* weight of one pct cell for APR level li (0 or 1; 0..2 at l_max) */
@@ -204,16 +209,6 @@ __device__ __forceinline__ int cell_weight(const RowTable& t, int li, uint8_t st
         return 0;
     }
     return (st > t.min_type && st <= UPSEED) ? 1 : 0;
-}
-
-/* pct row of APR row `row` of level index li (the max level reads row (z/2, x/2) of l_max - 1) */
-__device__ __forceinline__ const uint8_t* apr_row_cells(const RowTable& t, const uint8_t* maps, int li, int64_t row) {
-    int64_t prow = row;
-    if (li == t.n_levels - 1) {
-        const int64_t zm = row / t.rows_x[li], xm = row - zm * t.rows_x[li];
-        prow = (zm / 2) * t.map_x[li] + xm / 2;
-    }
-    return maps + t.map_off[li] + prow * t.map_y[li];
 }
 
 __device__ __forceinline__ int row_level_of(const RowTable& t, int64_t r) {
@@ -239,57 +234,115 @@ __device__ __forceinline__ void chunk_weights(const RowTable& t, int li, const u
     (void)top;
 }
 
-/* particles of each APR row (one wave per row, 1024 cells per step) */
+/* the APR rows of work item w (1, or up to 4 sharing a pct row at l_max) and its cells */
+struct WorkRows {
+    int li;
+    const uint8_t* p;     /* pct row          */
+    int ylen;
+    int n;                /* destination rows */
+    int64_t r[4];
+};
+__device__ __forceinline__ WorkRows work_rows(const RowTable& t, const uint8_t* maps, int64_t w) {
+    WorkRows o;
+    const int top = t.n_levels - 1;
+    if (w < t.row_prefix[top]) {
+        o.li = row_level_of(t, w);
+        o.p = maps + t.map_off[o.li] + (w - t.row_prefix[o.li]) * t.map_y[o.li];
+        o.n = 1;
+        o.r[0] = w;
+    } else {
+        const int64_t prow = w - t.row_prefix[top];
+        const int64_t zm = prow / t.map_x[top], xm = prow - zm * t.map_x[top];
+        o.li = top;
+        o.p = maps + t.map_off[top] + prow * t.map_y[top];
+        /* rows (2zm, 2xm), (2zm, 2xm+1), (2zm+1, 2xm), (2zm+1, 2xm+1) that exist, in that order */
+        const bool hx = 2 * xm + 1 < t.rows_x[top], hz = 2 * zm + 1 < t.top_z;
+        const int64_t r00 = t.row_prefix[top] + 2 * zm * t.rows_x[top] + 2 * xm;
+        o.r[0] = r00;
+        o.r[1] = hx ? r00 + 1 : r00 + t.rows_x[top];
+        o.r[2] = hx ? r00 + t.rows_x[top] : r00;
+        o.r[3] = r00 + t.rows_x[top] + 1;
+        o.n = 1 + (hx ? 1 : 0) + (hz ? 1 : 0) + ((hx && hz) ? 1 : 0);
+    }
+    o.ylen = (int)t.map_y[o.li];
+    return o;
+}
+
+/* particles of each APR row: one wave per work item, 1024 cells per step */
 __device__ __forceinline__ void row_count_body(const uint8_t* __restrict__ maps, const RowTable& t, int64_t nrows, uint32_t* cnt,
                                                int64_t G, int64_t blk) {
     const int lane = threadIdx.x & 63;
     const int64_t wid = (blk * blockDim.x + threadIdx.x) >> 6;
     const int64_t nw = (G * blockDim.x) >> 6;
-    for (int64_t r = wid; r < nrows; r += nw) {
-        const int li = row_level_of(t, r);
-        const uint8_t* p = apr_row_cells(t, maps, li, r - t.row_prefix[li]);
-        const int ylen = (int)t.map_y[li];
+    (void)nrows;
+    for (int64_t w = wid; w < t.n_work; w += nw) {
+        const WorkRows wr = work_rows(t, maps, w);
         uint32_t c = 0;
-        for (int base = 0; base < ylen; base += 64 * CPL) {
-            int w[CPL];
-            chunk_weights(t, li, p, ylen, base + lane * CPL, w);
+        for (int base = 0; base < wr.ylen; base += 64 * CPL) {
+            int wt[CPL];
+            chunk_weights(t, wr.li, wr.p, wr.ylen, base + lane * CPL, wt);
 #pragma unroll
-            for (int j = 0; j < CPL; ++j) c += (uint32_t)w[j];
+            for (int j = 0; j < CPL; ++j) c += (uint32_t)wt[j];
         }
         for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
-        if (lane == 0) cnt[r] = c;
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (k < wr.n) cnt[wr.r[k]] = c;
+        }
     }
 }
 
-/* y values of each APR row from its scanned offset: one cell per lane per step, positions
-   from ballot prefix counts, so consecutive lanes write consecutive y_vec entries */
+/* y values of each APR row from its scanned offset.  A lane takes CPL consecutive cells of a
+   1024-cell chunk; a wave prefix sum of the lanes' counts places each lane's values in an LDS
+   stage, which the wave then copies to y_vec with consecutive lanes on consecutive entries
+   (once per destination row). */
+constexpr int FILL_WAVES = 4;
 __device__ __forceinline__ void row_fill_body(const uint8_t* __restrict__ maps, const RowTable& t, int64_t nrows,
                                               const uint64_t* __restrict__ row_off, uint16_t* y_vec, int64_t G, int64_t blk) {
+    __shared__ uint16_t stage[FILL_WAVES][2 * 64 * CPL];
     const int lane = threadIdx.x & 63;
+    uint16_t* st = stage[threadIdx.x >> 6];
     const int64_t wid = (blk * blockDim.x + threadIdx.x) >> 6;
     const int64_t nw = (G * blockDim.x) >> 6;
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int64_t r = wid; r < nrows; r += nw) {
-        const int li = row_level_of(t, r);
-        const bool top = (li == t.n_levels - 1);
-        const uint8_t* p = apr_row_cells(t, maps, li, r - t.row_prefix[li]);
-        const int ylen = (int)t.map_y[li];
-        uint64_t run = row_off[r];
-        for (int c0 = 0; c0 < ylen; c0 += 64) {
-            const int yy = c0 + lane;
-            const int w = (yy < ylen) ? cell_weight(t, li, p[yy], yy) : 0;
-            const uint64_t b1 = __ballot(w >= 1);
-            const uint64_t b2 = __ballot(w >= 2);
-            if (w) {
-                const uint64_t pos = run + __popcll(b1 & lt) + __popcll(b2 & lt);
-                if (top) {
-                    y_vec[pos] = (uint16_t)(2 * yy);
-                    if (w == 2) y_vec[pos + 1] = (uint16_t)(2 * yy + 1);
-                } else {
-                    y_vec[pos] = (uint16_t)yy;
-                }
+    (void)nrows;
+    for (int64_t w = wid; w < t.n_work; w += nw) {
+        const WorkRows wr = work_rows(t, maps, w);
+        const bool top = (wr.li == t.n_levels - 1);
+        uint64_t pos[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (k < wr.n) pos[k] = row_off[wr.r[k]];
+        for (int base = 0; base < wr.ylen; base += 64 * CPL) {
+            int wt[CPL];
+            const int c0 = base + lane * CPL;
+            chunk_weights(t, wr.li, wr.p, wr.ylen, c0, wt);
+            int c = 0;
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) c += wt[j];
+            /* inclusive prefix over the wave */
+            int incl = c;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int v = __shfl_up(incl, o);
+                if (lane >= o) incl += v;
             }
-            run += (uint64_t)(__popcll(b1) + __popcll(b2));
+            const int total = __shfl(incl, 63);
+            int q = incl - c;
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+                const int yy = c0 + j;
+                if (wt[j] >= 1) st[q++] = (uint16_t)(top ? 2 * yy : yy);
+                if (wt[j] == 2) st[q++] = (uint16_t)(2 * yy + 1);
+            }
+            __threadfence_block();
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (k >= wr.n) break;
+                uint16_t* dst = y_vec + pos[k];
+                for (int i = lane; i < total; i += 64) dst[i] = st[i];
+                pos[k] += (uint64_t)total;
+            }
+            __threadfence_block();   /* the stage is rewritten by the next chunk */
         }
     }
 }
@@ -662,10 +715,13 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
         rp += a.x_num[l] * a.z_num[l];
     }
     t.row_prefix[t.n_levels] = rp;
+    t.top_z = a.z_num[lmax];
+    t.top_pz = a.z_num[top];
+    t.n_work = t.row_prefix[t.n_levels - 1] + a.x_num[top] * a.z_num[top];
     /* per-row counts, exclusive scan over rows in particle order, xz_end_vec, y_vec */
     Buf cnt(ctx->pool, (size_t)rp * 4);
     Buf rowoff(ctx->pool, (size_t)rp * 8);
-    const int64_t RG = stride_tiles(rp, 4, 16384);   /* a wave per row, four waves per block */
+    const int64_t RG = stride_tiles(t.n_work, 4, 16384);   /* a wave per work item, four waves per block */
     launch_tiles<RowCount>(ctx, {P, t, rp, cnt.as<uint32_t>(), RG}, RG);
     if (ctx->async_mode) {
         /* y_vec and xz_end_vec are preallocated (y_vec for the upper bound P <= N); the count

@@ -54,16 +54,26 @@ __device__ __forceinline__ int tile_owner(const int64_t* prefix, int nv, int64_t
     return lo;
 }
 
+/* Blocks are handed to the 8 XCDs round-robin (block b runs on XCD b % 8), and each XCD has
+   its own L2.  Renumbering so that XCD k owns one contiguous run of the grid's block indices
+   puts neighbouring tiles (which share halo rows) in the same L2. */
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t grid) {
+    constexpr int64_t NX = 8;
+    const int64_t q = grid / NX, r = grid % NX, x = b % NX;
+    return x * q + min(x, r) + b / NX;
+}
+
 template <typename B>
 __global__ __launch_bounds__(B::THREADS) void k_tiles(typename B::Args a, int64_t tiles) {
-    for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) B::run(a, t);
+    const int64_t b = xcd_block(blockIdx.x, gridDim.x);
+    for (int64_t t = b; t < tiles; t += gridDim.x) B::run(a, t);
 }
 
 template <typename B>
 __global__ __launch_bounds__(B::THREADS) void k_tiles_multi(const typename B::Args* __restrict__ a, const int64_t* __restrict__ prefix,
                                                             int nv) {
     const int64_t total = prefix[nv];
-    for (int64_t t = blockIdx.x; t < total; t += gridDim.x) {
+    for (int64_t t = xcd_block(blockIdx.x, gridDim.x); t < total; t += gridDim.x) {
         const int v = tile_owner(prefix, nv, t);
         B::run(a[v], t - prefix[v]);
     }
