@@ -264,7 +264,7 @@ struct PadGeom {
    source offsets are computed once per block. */
 template <int O>
 __device__ __forceinline__ void mean_y_body(int mode, const float* __restrict__ inten, float* A, const PadGeom& g, int64_t blk) {
-    constexpr int TY = 64;
+    constexpr int TY = 32;   /* samples per tile: 8.4 KB of LDS per wave, so ~19 waves per CU */
     constexpr int H = 2 * O + 2;
     __shared__ float tile[64][TY + 1];
     __shared__ int64_t src_row[64];   /* inten offset of the (reflected) source row of each line */
@@ -302,9 +302,11 @@ __device__ __forceinline__ void mean_y_body(int mode, const float* __restrict__ 
     }
     for (int p0 = 0; p0 < yn; p0 += TY) {
         /* inputs for outputs [p0, p0+64) are samples [p0+O, p0+O+64) */
-        const int yy = p0 + O + lane;
-        if (yy < yn)
-            for (int l = 0; l < nl; ++l) tile[l][lane] = src(l, yy);
+        /* 64 lines x TY samples, 64 / TY lines per load instruction */
+        for (int idx = lane; idx < 64 * TY; idx += 64) {
+            const int l = idx / TY, k = idx % TY, yy = p0 + O + k;
+            if (l < nl && yy < yn) tile[l][k] = src(l, yy);
+        }
         __syncthreads();
         if (valid) {
             for (int k = 0; k < TY; ++k) {
@@ -327,9 +329,10 @@ __device__ __forceinline__ void mean_y_body(int mode, const float* __restrict__ 
             }
         }
         __syncthreads();
-        const int q = p0 + lane;
-        if (q < yn - O)
-            for (int l = 0; l < nl; ++l) A[(line0 + l) * yn + q] = tile[l][lane];
+        for (int idx = lane; idx < 64 * TY; idx += 64) {
+            const int l = idx / TY, k = idx % TY, q = p0 + k;
+            if (l < nl && q < yn - O) A[(line0 + l) * yn + q] = tile[l][k];
+        }
         __syncthreads();
     }
     /* right-hand end (LocalIntensityScale.hpp:346-352) */

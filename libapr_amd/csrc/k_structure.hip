@@ -85,87 +85,122 @@ __device__ __forceinline__ void load_row8(const uint8_t* row, int yn, int y0, bo
     b[9] = (y0 + 8 < yn) ? row[y0 + 8] : (uint8_t)0;
 }
 
+/* bit masks (bit k <-> y0 - 1 + k) of ASCENDANT and of SEED-or-PROPOGATE cells in a row */
+__device__ __forceinline__ void row_masks(const uint8_t b[10], uint32_t& am, uint32_t& sm) {
+    am = 0;
+    sm = 0;
+#pragma unroll
+    for (int k = 0; k < 10; ++k) {
+        am |= (uint32_t)(b[k] == ASC) << k;
+        sm |= (uint32_t)((b[k] == SEED) | (b[k] == PROP)) << k;
+    }
+}
+
+/* One lane owns 8 consecutive y cells of one (x) column over a run of ZC planes and marches
+   along z, keeping the ASCENDANT / SEED-or-PROPOGATE masks of the planes z-1, z, z+1 (each the
+   OR over rows x-1, x, x+1): every plane's three rows are loaded once instead of three times.
+   A cell's state is written only by its owner lane, so the cached bytes of the own row are
+   current. */
+constexpr int PULL_ZC = 4;
 __device__ __forceinline__ void pull_level8_body(uint8_t* P, Dims d, uint8_t* Pc, Dims dc, uint8_t* Pp, Dims dp, int is_top,
                                                  int has_parent, int64_t G, int64_t t) {
     const int yn = (int)d.y, xn = (int)d.x, zn = (int)d.z;
     const int nwr = (yn + 7) >> 3;
-    const int64_t total = (int64_t)nwr * xn * zn;
+    const int nzc = (zn + PULL_ZC - 1) / PULL_ZC;
+    const int64_t total = (int64_t)nwr * xn * nzc;
     const bool aligned8 = (yn & 7) == 0 && ((uintptr_t)P & 7) == 0;
     for (int64_t i = t * blockDim.x + threadIdx.x; i < total; i += G * blockDim.x) {
         const int w = (int)(i % nwr);
-        const int r = (int)(i / nwr);
-        const int x = r % xn, z = r / xn;
+        const int64_t r = i / nwr;
+        const int x = (int)(r % xn), zc = (int)(r / xn);
         const int y0 = w * 8;
-        uint32_t am = 0, sm = 0;   /* bit k <-> y0 - 1 + k */
-        for (int dz = -1; dz <= 1; ++dz) {
-            const int zz = z + dz;
-            if (zz < 0 || zz >= zn) continue;
+        const int z_begin = zc * PULL_ZC, z_end = min(zn, z_begin + PULL_ZC);
+        /* masks of plane zz (0 outside the volume); own row bytes of that plane in own[] */
+        auto plane = [&](int zz, uint32_t& am, uint32_t& sm, uint8_t own[10]) {
+            am = 0;
+            sm = 0;
+            if (zz < 0 || zz >= zn) return;
+#pragma unroll
             for (int dx = -1; dx <= 1; ++dx) {
                 const int xx = x + dx;
                 if (xx < 0 || xx >= xn) continue;
                 uint8_t b[10];
                 load_row8(P + ((int64_t)zz * xn + xx) * yn, yn, y0, aligned8, b);
+                uint32_t a1, s1;
+                row_masks(b, a1, s1);
+                am |= a1;
+                sm |= s1;
+                if (dx == 0)
 #pragma unroll
-                for (int k = 0; k < 10; ++k) {
-                    am |= (uint32_t)(b[k] == ASC) << k;
-                    sm |= (uint32_t)((b[k] == SEED) | (b[k] == PROP)) << k;
+                    for (int k = 0; k < 10; ++k) own[k] = b[k];
+            }
+        };
+        uint32_t pa0, ps0, pa1, ps1, pa2, ps2;
+        uint8_t cur[10], nxt[10];
+        plane(z_begin - 1, pa0, ps0, nxt);
+        plane(z_begin, pa1, ps1, cur);
+        for (int z = z_begin; z < z_end; ++z) {
+            plane(z + 1, pa2, ps2, nxt);
+            const uint32_t am = pa0 | pa1 | pa2, sm = ps0 | ps1 | ps2;
+            const uint32_t ad = am | (am << 1) | (am >> 1), sd = sm | (sm << 1) | (sm >> 1);
+            uint8_t* row = P + ((int64_t)z * xn + x) * yn;
+            uint8_t nb[8];
+            bool changed = false;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint8_t old = cur[1 + j];
+                const bool any_asc = (ad >> (j + 1)) & 1u, any_sp = (sd >> (j + 1)) & 1u;
+                uint8_t nw = old;
+                if (!is_top) {
+                    if (old == EMPTY) nw = any_asc ? ASCN : (any_sp ? BOUNDARY : EMPTY);
+                    else if (old == SEED) nw = any_asc ? PROP : SEED;
+                } else if (old == EMPTY && any_sp) {
+                    nw = BOUNDARY;
+                }
+                nb[j] = nw;
+                changed |= (nw != old) && (y0 + j < yn);
+            }
+            if (changed) {
+                if (aligned8) {
+                    uint64_t v = 0;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) v |= (uint64_t)nb[j] << (8 * j);
+                    *(uint64_t*)(row + y0) = v;
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 8; ++j)
+                        if (y0 + j < yn && nb[j] != cur[1 + j]) row[y0 + j] = nb[j];
                 }
             }
-        }
-        const uint32_t ad = am | (am << 1) | (am >> 1), sd = sm | (sm << 1) | (sm >> 1);
-        uint8_t* row = P + ((int64_t)z * xn + x) * yn;
-        uint8_t ob[10];
-        load_row8(row, yn, y0, aligned8, ob);
-        uint8_t nb[8];
-        bool changed = false;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const uint8_t old = ob[1 + j];
-            const bool any_asc = (ad >> (j + 1)) & 1u, any_sp = (sd >> (j + 1)) & 1u;
-            uint8_t nw = old;
-            if (!is_top) {
-                if (old == EMPTY) nw = any_asc ? ASCN : (any_sp ? BOUNDARY : EMPTY);
-                else if (old == SEED) nw = any_asc ? PROP : SEED;
-            } else if (old == EMPTY && any_sp) {
-                nw = BOUNDARY;
+            for (int j = 0; j < 8; ++j) {
+                const int y = y0 + j;
+                if (y >= yn) break;
+                const uint8_t nw = nb[j];
+                /* set_filler: empty children of ASCENDANTNEIGHBOUR / PROPOGATE become FILLER */
+                if (!is_top && (nw == ASCN || nw == PROP)) {
+                    const int cz = (z == zn - 1 && (dc.z & 1)) ? 1 : 2;
+                    const int cx = (x == xn - 1 && (dc.x & 1)) ? 1 : 2;
+                    const int cy = (y == yn - 1 && (dc.y & 1)) ? 1 : 2;
+                    for (int a = 0; a < cz; ++a)
+                        for (int bb = 0; bb < cx; ++bb)
+                            for (int c = 0; c < cy; ++c) {
+                                uint8_t* ch = &Pc[((int64_t)(2 * z + a) * dc.x + (2 * x + bb)) * dc.y + (2 * y + c)];
+                                if (*ch == EMPTY) *ch = FILLER;
+                            }
+                }
+                /* fill_parent */
+                if (has_parent && (nw == SEED || nw == PROP || nw == ASC)) {
+                    uint8_t* pp = &Pp[((int64_t)(z / 2) * dp.x + (x / 2)) * dp.y + (y / 2)];
+                    if (*pp != SEED) *pp = ASC;
+                }
             }
-            nb[j] = nw;
-            changed |= (nw != old) && (y0 + j < yn);
-        }
-        if (changed) {
-            if (aligned8) {
-                uint64_t v = 0;
+            pa0 = pa1;
+            ps0 = ps1;
+            pa1 = pa2;
+            ps1 = ps2;
 #pragma unroll
-                for (int j = 0; j < 8; ++j) v |= (uint64_t)nb[j] << (8 * j);
-                *(uint64_t*)(row + y0) = v;
-            } else {
-#pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if (y0 + j < yn && nb[j] != ob[1 + j]) row[y0 + j] = nb[j];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int y = y0 + j;
-            if (y >= yn) break;
-            const uint8_t nw = nb[j];
-            /* set_filler: empty children of ASCENDANTNEIGHBOUR / PROPOGATE become FILLER */
-            if (!is_top && (nw == ASCN || nw == PROP)) {
-                const int cz = (z == zn - 1 && (dc.z & 1)) ? 1 : 2;
-                const int cx = (x == xn - 1 && (dc.x & 1)) ? 1 : 2;
-                const int cy = (y == yn - 1 && (dc.y & 1)) ? 1 : 2;
-                for (int a = 0; a < cz; ++a)
-                    for (int bb = 0; bb < cx; ++bb)
-                        for (int c = 0; c < cy; ++c) {
-                            uint8_t* ch = &Pc[((int64_t)(2 * z + a) * dc.x + (2 * x + bb)) * dc.y + (2 * y + c)];
-                            if (*ch == EMPTY) *ch = FILLER;
-                        }
-            }
-            /* fill_parent */
-            if (has_parent && (nw == SEED || nw == PROP || nw == ASC)) {
-                uint8_t* pp = &Pp[((int64_t)(z / 2) * dp.x + (x / 2)) * dp.y + (y / 2)];
-                if (*pp != SEED) *pp = ASC;
-            }
+            for (int k = 0; k < 10; ++k) cur[k] = nxt[k];
         }
     }
 }
@@ -681,7 +716,7 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
         const bool is_top = (l == top);
         uint8_t* Pc = is_top ? nullptr : P + off[l + 1];
         uint8_t* Pp = has_parent ? P + off[l - 1] : nullptr;
-        const int64_t G = stride_tiles(cdiv(a.y_num[l], 8) * a.x_num[l] * a.z_num[l], 256 * 2);
+        const int64_t G = stride_tiles(cdiv(a.y_num[l], 8) * a.x_num[l] * cdiv(a.z_num[l], PULL_ZC), 256 * 2);
         launch_tiles<PullLevel>(ctx, {P + off[l], D(l), Pc, is_top ? Dims{1, 1, 1} : D(l + 1), Pp, has_parent ? D(l - 1) : Dims{1, 1, 1},
                                       is_top ? 1 : 0, has_parent ? 1 : 0, G},
                                 G);
