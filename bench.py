@@ -84,7 +84,7 @@ def run_copy(args):
     return {"bytes": nbytes, "ms": round(ms, 3), "GBps_read_plus_write": round(2 * nbytes / (ms / 1000.0) / 1e9, 1)}
 
 
-def run_gpu(args, conv=True):
+def run_gpu(args, with_conv=True):
     import libapr_amd as A
     ctx = A.Context(0, timing=True)
     L = A.lib()
@@ -154,7 +154,7 @@ def run_gpu(args, conv=True):
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "algorithmic_bytes": ab,
                 "avg_ms": round(cands[dom], 4)}
     total_ab = 23 * N + 6 * P
-    if not conv:
+    if not with_conv:
         last.pop("apr").close()
         A._check(L.aprh_free(ctx.h, d_img))
         A._check(L.aprh_free(ctx.h, d_parts))
@@ -480,7 +480,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.only:
-        part = {"get_apr": lambda: run_gpu(args, conv=False)[:4], "batch": lambda: run_batch(args, check=False),
+        part = {"get_apr": lambda: run_gpu(args, with_conv=False)[:4], "batch": lambda: run_batch(args, check=False),
                 "copy": lambda: run_copy(args)}[args.only]()
         print(json.dumps({"only": args.only, "result": part}), flush=True)
         return

@@ -17,6 +17,7 @@ point raises ``RuntimeError`` when libaprh.so has not been built
 (``python -m libapr_amd.build``).  There is no CPU fallback.
 """
 import ctypes as C
+import weakref
 import os
 
 import numpy as np
@@ -205,6 +206,13 @@ class APRParameters:
                           int(self.neighborhood_optimization), int(self.constant_intensity_scale))
 
 
+def _register(ctx, obj):
+    """obj holds device state of ctx: Context.close() closes it first"""
+    deps = getattr(ctx, "_deps", None)
+    if deps is not None:
+        deps.add(obj)
+
+
 class Context:
     """One HIP stream on one device (aprh_context)."""
 
@@ -213,6 +221,8 @@ class Context:
         _check(lib().aprh_context_create(device, C.byref(h)))
         self.h = h
         self.device = device
+        # objects holding device state of this context: closed before the context is destroyed
+        self._deps = weakref.WeakSet()
         if keep_intermediates:
             _check(lib().aprh_context_set_keep_intermediates(h, 1))
         if timing:
@@ -240,6 +250,8 @@ class Context:
 
     def close(self):
         if getattr(self, "h", None):
+            for d in list(self._deps):
+                d.close()
             lib().aprh_context_destroy(self.h)
             self.h = None
 
@@ -266,6 +278,7 @@ class APR:
     def __init__(self, ctx, handle, owner=None):
         self.ctx, self.h = ctx, handle
         self._owner = owner            # a batch plan that owns the handle (never destroyed here)
+        _register(ctx, self)
         info = AprhAprInfo()
         _check(lib().aprh_apr_info_get(handle, C.byref(info)))
         self.l_min, self.l_max = info.l_min, info.l_max
@@ -418,6 +431,7 @@ class APRBatch:
         h = vp()
         _check(lib().aprh_batch_create(self.ctx.h, len(self.dims), _ptr(d), C.byref(p), C.byref(h)))
         self.h = h
+        _register(self.ctx, self)
         self.voxels = int(sum(y * x * z for y, x, z in self.dims))
 
     def input_address(self, i):
@@ -760,6 +774,7 @@ class WarmStartCNN:
         h = vp()
         _check(lib().aprh_cnn_create(self.ctx.h, seed, C.byref(h)))
         self.h = h
+        _register(self.ctx, self)
         if params is not None:
             self.set_params(params)
 
@@ -842,6 +857,7 @@ class DeviceArray:
         _check(lib().aprh_malloc(self.ctx.h, max(int(nbytes), 1), C.byref(p)))
         self.ptr = p.value
         self.nbytes = int(nbytes)
+        _register(self.ctx, self)
 
     def __int__(self):
         return self.ptr
@@ -864,6 +880,8 @@ class DeviceArray:
         if getattr(self, "ptr", None):
             lib().aprh_free(self.ctx.h, C.c_void_p(self.ptr))
             self.ptr = None
+
+    close = free
 
     def __del__(self):
         try:

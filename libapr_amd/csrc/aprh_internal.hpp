@@ -332,7 +332,11 @@ inline void check_ctx(aprh_context* ctx) {
 }  // namespace aprh
 
 /* exception -> status translation used by every extern "C" entry point */
-#define APRH_TRY try {
+/* every entry point starts from a clean HIP error state, so an error left by an earlier,
+   unrelated call is never reported by this one */
+#define APRH_TRY                 \
+    (void)hipGetLastError();     \
+    try {
 #define APRH_CATCH                                              \
     }                                                           \
     catch (aprh::Error & e) {                                   \

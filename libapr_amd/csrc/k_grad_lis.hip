@@ -120,86 +120,110 @@ __device__ __forceinline__ int u16at(const uint4& v, int i) {
     return (int)((i & 1) ? (w >> 16) : (w & 0xFFFFu));
 }
 
+/* One lane owns 8 consecutive y of one downsampled x column pair and marches over a run of
+ * GZC downsampled z planes.  Step zd needs the full-resolution planes 2zd-1 .. 2zd+2; the
+ * rows (x0, x1) of plane 2zd-1 and of plane 2zd are the previous step's plane-(2zd-1) centre
+ * rows and its down rows, so a step loads 8 rows instead of 12. */
+constexpr int GZC = 8;
 template <bool UNIT>
 __device__ __forceinline__ void grad_ds_vec_body(const uint16_t* __restrict__ img, uint16_t* grad, float* mean, int64_t y_num,
                                                  int64_t x_num, int64_t z_num, int64_t yd_num, float h2x, float h2y, float h2z,
-                                                 int accum, int64_t bx, int64_t xd, int64_t zd, int64_t xd_num) {
+                                                 int accum, int64_t bx, int64_t xd, int64_t zc, int64_t xd_num, int64_t zd_num) {
     const int64_t yg = bx * blockDim.x + threadIdx.x;   /* group of 8 y */
     if (yg * 8 >= y_num) return;
     const int64_t xy = x_num * y_num;
     const int64_t y0 = yg * 8;
-    const int64_t x0 = 2 * xd, z0 = 2 * zd;
-    const bool hx1 = x0 + 1 < x_num, hz1 = z0 + 1 < z_num;
-    const int64_t x1 = hx1 ? x0 + 1 : x0, z1 = hz1 ? z0 + 1 : z0;
+    const int64_t x0 = 2 * xd;
+    const bool hx1 = x0 + 1 < x_num;
+    const int64_t x1 = hx1 ? x0 + 1 : x0;
     auto row = [&](int64_t x, int64_t z) -> const uint16_t* { return img + z * xy + x * y_num + y0; };
     auto ld = [&](int64_t x, int64_t z) -> uint4 { return *(const uint4*)row(x, z); };
     const int64_t xl0 = x0 > 0 ? x0 - 1 : 0, xr1 = min(x1 + 1, x_num - 1);
-    const int64_t zu0 = z0 > 0 ? z0 - 1 : 0, zd1 = min(z1 + 1, z_num - 1);
-    /* centre rows c[a][b]: x = a ? x1 : x0, z = b ? z1 : z0 */
-    const uint4 c00 = ld(x0, z0), c10 = ld(x1, z0), c01 = ld(x0, z1), c11 = ld(x1, z1);
-    const uint4 l0 = ld(xl0, z0), l1 = ld(xl0, z1), r0 = ld(xr1, z0), r1 = ld(xr1, z1);
-    const uint4 u0 = ld(x0, zu0), u1 = ld(x1, zu0), d0 = ld(x0, zd1), d1 = ld(x1, zd1);
     /* y neighbours beyond the vector (replicate at the line ends) */
     auto edge = [&](int64_t x, int64_t z, int which) -> int {
         const uint16_t* p = img + z * xy + x * y_num;
         if (which == 0) return y0 > 0 ? (int)p[y0 - 1] : -1;
         return y0 + 8 < y_num ? (int)p[y0 + 8] : -1;
     };
-    const int e00m = edge(x0, z0, 0), e00p = edge(x0, z0, 1), e10m = edge(x1, z0, 0), e10p = edge(x1, z0, 1);
-    const int e01m = edge(x0, z1, 0), e01p = edge(x0, z1, 1), e11m = edge(x1, z1, 0), e11p = edge(x1, z1, 1);
-    uint32_t gmax[4] = {0, 0, 0, 0};
-    auto one_row = [&](const uint4& c, const uint4& lft, const uint4& rgt, const uint4& up, const uint4& dn, int em, int ep) {
+    const int64_t zd_begin = zc * GZC, zd_end = min(zd_num, zd_begin + GZC);
+    uint4 keep_u0 = make_uint4(0, 0, 0, 0), keep_u1 = keep_u0, keep_c0 = keep_u0, keep_c1 = keep_u0;
+    int64_t keep_uz = -1, keep_cz = -1;   /* planes the kept rows belong to */
+    for (int64_t zd = zd_begin; zd < zd_end; ++zd) {
+        const int64_t z0 = 2 * zd;
+        const bool hz1 = z0 + 1 < z_num;
+        const int64_t z1 = hz1 ? z0 + 1 : z0;
+        const int64_t zu0 = z0 > 0 ? z0 - 1 : 0, zd1 = min(z1 + 1, z_num - 1);
+        /* centre rows c[a][b]: x = a ? x1 : x0, z = b ? z1 : z0 */
+        const bool kc = keep_cz == z0, ku = keep_uz == zu0;
+        const uint4 c00 = kc ? keep_c0 : ld(x0, z0), c10 = kc ? keep_c1 : ld(x1, z0);
+        const uint4 c01 = ld(x0, z1), c11 = ld(x1, z1);
+        const uint4 l0 = ld(xl0, z0), l1 = ld(xl0, z1), r0 = ld(xr1, z0), r1 = ld(xr1, z1);
+        const uint4 u0 = ku ? keep_u0 : ld(x0, zu0), u1 = ku ? keep_u1 : ld(x1, zu0);
+        const uint4 d0 = ld(x0, zd1), d1 = ld(x1, zd1);
+        const int e00m = edge(x0, z0, 0), e00p = edge(x0, z0, 1), e10m = edge(x1, z0, 0), e10p = edge(x1, z0, 1);
+        const int e01m = edge(x0, z1, 0), e01p = edge(x0, z1, 1), e11m = edge(x1, z1, 0), e11p = edge(x1, z1, 1);
+        uint32_t gmax[4] = {0, 0, 0, 0};
+        auto one_row = [&](const uint4& c, const uint4& lft, const uint4& rgt, const uint4& up, const uint4& dn, int em, int ep) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int64_t y = y0 + i;
-            int dyv;
-            /* y == 0: c[1]-c[0]; y == Y-1: c[Y-1]-c[Y-2]; else c[y+1]-c[y-1] */
-            const int cm = (i > 0) ? u16at(c, i - 1) : em;
-            const int cp = (i < 7) ? u16at(c, i + 1) : ep;
-            if (y == 0) dyv = u16at(c, 1) - u16at(c, 0);
-            else if (y == y_num - 1) dyv = u16at(c, i) - cm;
-            else dyv = cp - cm;
-            const int dxv = u16at(rgt, i) - u16at(lft, i);
-            const int dzv = u16at(dn, i) - u16at(up, i);
-            uint32_t g;
-            if (UNIT) {
-                const int64_t k = (int64_t)dxv * dxv + (int64_t)dzv * dzv + (int64_t)dyv * dyv;
-                g = ((uint32_t)sqrt((double)k)) >> 1;
-            } else {
-                const double q = sq_term(dxv, h2x) + sq_term(dzv, h2z) + sq_term(dyv, h2y);
-                g = d2u16(sqrt(q));
+            for (int i = 0; i < 8; ++i) {
+                const int64_t y = y0 + i;
+                int dyv;
+                /* y == 0: c[1]-c[0]; y == Y-1: c[Y-1]-c[Y-2]; else c[y+1]-c[y-1] */
+                const int cm = (i > 0) ? u16at(c, i - 1) : em;
+                const int cp = (i < 7) ? u16at(c, i + 1) : ep;
+                if (y == 0) dyv = u16at(c, 1) - u16at(c, 0);
+                else if (y == y_num - 1) dyv = u16at(c, i) - cm;
+                else dyv = cp - cm;
+                const int dxv = u16at(rgt, i) - u16at(lft, i);
+                const int dzv = u16at(dn, i) - u16at(up, i);
+                uint32_t g;
+                if (UNIT) {
+                    const int64_t k = (int64_t)dxv * dxv + (int64_t)dzv * dzv + (int64_t)dyv * dyv;
+                    g = ((uint32_t)sqrt((double)k)) >> 1;
+                } else {
+                    const double q = sq_term(dxv, h2x) + sq_term(dzv, h2z) + sq_term(dyv, h2y);
+                    g = d2u16(sqrt(q));
+                }
+                gmax[i >> 1] = max(gmax[i >> 1], g);
             }
-            gmax[i >> 1] = max(gmax[i >> 1], g);
+        };
+        /* rows (x0,z0), (x1,z0), (x0,z1), (x1,z1); left/right neighbours of x1 are x0 / xr1 etc. */
+        one_row(c00, l0, hx1 ? c10 : r0, u0, hz1 ? c01 : d0, e00m, e00p);
+        if (hx1) one_row(c10, c00, r0, u1, hz1 ? c11 : d1, e10m, e10p);
+        if (hz1) {
+            one_row(c01, l1, hx1 ? c11 : r1, c00, d0, e01m, e01p);
+            if (hx1) one_row(c11, c01, r1, c10, d1, e11m, e11p);
         }
-    };
-    /* rows (x0,z0), (x1,z0), (x0,z1), (x1,z1); left/right neighbours of x1 are x0 / xr1 etc. */
-    one_row(c00, l0, hx1 ? c10 : r0, u0, hz1 ? c01 : d0, e00m, e00p);
-    if (hx1) one_row(c10, c00, r0, u1, hz1 ? c11 : d1, e10m, e10p);
-    if (hz1) {
-        one_row(c01, l1, hx1 ? c11 : r1, c00, d0, e01m, e01p);
-        if (hx1) one_row(c11, c01, r1, c10, d1, e11m, e11p);
-    }
-    const int64_t o = (zd * xd_num + xd) * yd_num + yg * 4;
-    if (accum) {
+        const int64_t o = (zd * xd_num + xd) * yd_num + yg * 4;
+        if (accum) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) gmax[j] = max(gmax[j], (uint32_t)grad[o + j]);
-    }
-    uint2 gw;
-    gw.x = gmax[0] | (gmax[1] << 16);
-    gw.y = gmax[2] | (gmax[3] << 16);
-    *(uint2*)(grad + o) = gw;
-    if (mean) {
-        /* downsample(sum, /8.0) with clamped x/z (PixelData.hpp:944-962); exact integer sums */
-        float4 m;
-        float* mp = &m.x;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int a = 2 * j, b = 2 * j + 1;
-            const int sum = u16at(c00, a) + u16at(c00, b) + u16at(c10, a) + u16at(c10, b) + u16at(c01, a) + u16at(c01, b) +
-                            u16at(c11, a) + u16at(c11, b);
-            mp[j] = (float)((double)(float)sum / 8.0);
+            for (int j = 0; j < 4; ++j) gmax[j] = max(gmax[j], (uint32_t)grad[o + j]);
         }
-        *(float4*)(mean + o) = m;
+        uint2 gw;
+        gw.x = gmax[0] | (gmax[1] << 16);
+        gw.y = gmax[2] | (gmax[3] << 16);
+        *(uint2*)(grad + o) = gw;
+        if (mean) {
+            /* downsample(sum, /8.0) with clamped x/z (PixelData.hpp:944-962); exact integer sums */
+            float4 m;
+            float* mp = &m.x;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int a = 2 * j, b = 2 * j + 1;
+                const int sum = u16at(c00, a) + u16at(c00, b) + u16at(c10, a) + u16at(c10, b) + u16at(c01, a) + u16at(c01, b) +
+                                u16at(c11, a) + u16at(c11, b);
+                mp[j] = (float)((double)(float)sum / 8.0);
+            }
+            *(float4*)(mean + o) = m;
+        }
+        /* the next step's up rows are this step's (x0, x1) rows of plane z1, its first centre
+           rows this step's down rows */
+        keep_u0 = c01;
+        keep_u1 = c11;
+        keep_uz = z1;
+        keep_c0 = d0;
+        keep_c1 = d1;
+        keep_cz = zd1;
     }
 }
 
@@ -613,7 +637,7 @@ struct GradArgs {
     int64_t y_num, x_num, z_num, yd_num;
     float h2x, h2y, h2z;
     int accum;
-    int64_t gx, xd_num;
+    int64_t gx, xd_num, zd_num;
 };
 template <typename T>
 struct GradDs {
@@ -634,7 +658,7 @@ struct GradDsVec {
     static __device__ __forceinline__ void run(const Args& a, int64_t t) {
         const int64_t bx = t % a.gx, r = t / a.gx;
         grad_ds_vec_body<UNIT>((const uint16_t*)a.img, (uint16_t*)a.grad, a.mean, a.y_num, a.x_num, a.z_num, a.yd_num, a.h2x, a.h2y,
-                               a.h2z, a.accum, bx, r % a.xd_num, r / a.xd_num, a.xd_num);
+                               a.h2z, a.accum, bx, r % a.xd_num, r / a.xd_num, a.xd_num, a.zd_num);
     }
 };
 struct InvArgs {
@@ -714,12 +738,13 @@ void launch_grad_ds_u16(aprh_context* ctx, const uint16_t* img, uint16_t* grad, 
     if (y % 8 == 0 && aligned) {
         const int64_t gx = cdiv(y / 8, 256);
         const bool unit = hx == 1.0f && hy == 1.0f && hz == 1.0f;
-        GradArgs a{img, grad, mean, y, x, z, yd, 2.0f * hx, 2.0f * hy, 2.0f * hz, accum ? 1 : 0, gx, xd};
-        if (unit) launch_tiles<GradDsVec<true>>(ctx, a, gx * xd * zd);
-        else launch_tiles<GradDsVec<false>>(ctx, a, gx * xd * zd);
+        GradArgs a{img, grad, mean, y, x, z, yd, 2.0f * hx, 2.0f * hy, 2.0f * hz, accum ? 1 : 0, gx, xd, zd};
+        const int64_t tiles = gx * xd * cdiv(zd, GZC);   /* (y block, x column pair, run of GZC z planes) */
+        if (unit) launch_tiles<GradDsVec<true>>(ctx, a, tiles);
+        else launch_tiles<GradDsVec<false>>(ctx, a, tiles);
     } else {
         const int64_t gx = cdiv(yd, 256);
-        launch_tiles<GradDs<uint16_t>>(ctx, GradArgs{img, grad, mean, y, x, z, yd, 2.0f * hx, 2.0f * hy, 2.0f * hz, accum ? 1 : 0, gx, xd},
+        launch_tiles<GradDs<uint16_t>>(ctx, GradArgs{img, grad, mean, y, x, z, yd, 2.0f * hx, 2.0f * hy, 2.0f * hz, accum ? 1 : 0, gx, xd, zd},
                                        gx * xd * zd);
     }
 }
@@ -826,7 +851,7 @@ void launch_grad_ds_f32(aprh_context* ctx, const float* img, float* grad, float*
                         float hy, float hz, bool accum) {
     const int64_t yd = cdiv(y, 2), xd = cdiv(x, 2), zd = cdiv(z, 2);
     const int64_t gx = cdiv(yd, 256);
-    launch_tiles<GradDs<float>>(ctx, GradArgs{img, grad, mean, y, x, z, yd, 2.0f * hx, 2.0f * hy, 2.0f * hz, accum ? 1 : 0, gx, xd},
+    launch_tiles<GradDs<float>>(ctx, GradArgs{img, grad, mean, y, x, z, yd, 2.0f * hx, 2.0f * hy, 2.0f * hz, accum ? 1 : 0, gx, xd, zd},
                                 gx * xd * zd);
 }
 
