@@ -124,58 +124,8 @@ __global__ __launch_bounds__(256) void k_l1_fwd(const float* __restrict__ x, int
 }
 
 /* 16 -> 16 convolution: out(v) = sum_tap W[tap] in(v + tap + off) (zero outside the input),
-   then + bias, ReLU, and / or the mask (ReLU derivative of a stored activation) */
-__global__ __launch_bounds__(256) void k_conv16(const uint16_t* __restrict__ in_cl, int sin, int sout, int off,
-                                                const u32x4* __restrict__ Wp, const float* __restrict__ bias,
-                                                const uint16_t* __restrict__ mask_cl, uint16_t* out_cl, uint16_t* out_pl, int relu, int T) {
-    const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
-    const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    const int nv = sout * sout * sout, nin = sin * sin * sin;
-    const int nbt = (nv + 15) / 16;
-    const int64_t nblk = (int64_t)T * nbt;
-    u32x4 w[NPAIR];
-#pragma unroll
-    for (int p = 0; p < NPAIR; ++p) w[p] = Wp[p * 64 + lane];
-    const float bv = bias ? bias[m] : 0.0f;
-    const int c8 = (g & 1) * 8, th = g >> 1;
-    for (int64_t blk = wid; blk < nblk; blk += nw) {
-        const int t = (int)(blk / nbt), b = (int)(blk % nbt);
-        const int v = b * 16 + m;
-        int zo = 0, xo = 0, yo = 0;
-        const bool valid = v < nv;
-        if (valid) decompose(v, sout, zo, xo, yo);
-        const uint16_t* it = in_cl + (int64_t)t * nin * CH + c8;
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int p = 0; p < NPAIR; ++p) {
-            const int tap = 2 * p + th;
-            u32x4 av = {0u, 0u, 0u, 0u};
-            if (valid && tap < NT) {
-                int dz, dx, dy;
-                tap_offsets(tap, dz, dx, dy);
-                const int zi = zo + dz + off, xi = xo + dx + off, yi = yo + dy + off;
-                if (zi >= 0 && zi < sin && xi >= 0 && xi < sin && yi >= 0 && yi < sin)
-                    av = *(const u32x4*)(it + (int64_t)((zi * sin + xi) * sin + yi) * CH);
-            }
-            acc = mfma(av, w[p], acc);
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int vo = b * 16 + g * 4 + i;
-            if (vo < nv) {
-                float val = acc[i] + bv;
-                if (relu) val = fmaxf(val, 0.0f);
-                if (mask_cl && !(bf2f(mask_cl[((int64_t)t * nv + vo) * CH + m]) > 0.0f)) val = 0.0f;
-                const uint16_t hv = f2bf(val);
-                if (out_cl) out_cl[((int64_t)t * nv + vo) * CH + m] = hv;
-                if (out_pl) out_pl[((int64_t)t * CH + m) * nv + vo] = hv;
-            }
-        }
-    }
-}
-
-/* the same convolution with the input staged in LDS: a block owns an output brick of up to
+   then + bias, ReLU, and / or the mask (ReLU derivative of a stored activation).
+   The input is staged in LDS: a block owns an output brick of up to
    16 (y) x 8 (x) x 4 (z) voxels, loads the (brick + 2)^3 input voxels (zero outside the input)
    once, and its four waves take 16-voxel groups of the brick's voxels in turn, reading every
    tap's A operand from LDS (the brick's linear voxel order, so edge bricks waste no MFMA work) */

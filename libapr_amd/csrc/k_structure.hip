@@ -19,6 +19,7 @@
  */
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "aprh_internal.hpp"
 #include "tiles.hpp"
@@ -593,6 +594,33 @@ __global__ void k_pyr_down(const T* __restrict__ fine, Dims fd, T* coarse, Dims 
     }
 }
 
+/* the same mean for u16 with 8 fine y per lane (16-byte row loads, 4 coarse outputs); the
+   integer sum of eight u16 values is exact in float, so the order of the float additions
+   above does not matter and the result is identical */
+__global__ __launch_bounds__(256) void k_pyr_down_u16v(const uint16_t* __restrict__ fine, Dims fd, uint16_t* coarse, Dims cd) {
+    const int64_t ng = cd.y / 4;
+    const int64_t total = ng * cd.x * cd.z;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t g = i % ng, r = i / ng;
+        const int64_t x = r % cd.x, z = r / cd.x;
+        const int64_t x0 = 2 * x, x1 = min(2 * x + 1, fd.x - 1), z0 = 2 * z, z1 = min(2 * z + 1, fd.z - 1);
+        const uint4 a = *(const uint4*)(fine + (z0 * fd.x + x0) * fd.y + 8 * g);
+        const uint4 b = *(const uint4*)(fine + (z0 * fd.x + x1) * fd.y + 8 * g);
+        const uint4 c = *(const uint4*)(fine + (z1 * fd.x + x0) * fd.y + 8 * g);
+        const uint4 d = *(const uint4*)(fine + (z1 * fd.x + x1) * fd.y + 8 * g);
+        const uint32_t wa[4] = {a.x, a.y, a.z, a.w}, wb[4] = {b.x, b.y, b.z, b.w}, wc[4] = {c.x, c.y, c.z, c.w},
+                       wd[4] = {d.x, d.y, d.z, d.w};
+        uint32_t o[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t sum = (wa[j] & 0xFFFFu) + (wa[j] >> 16) + (wb[j] & 0xFFFFu) + (wb[j] >> 16) + (wc[j] & 0xFFFFu) +
+                                 (wc[j] >> 16) + (wd[j] & 0xFFFFu) + (wd[j] >> 16);
+            o[j] = (uint32_t)pyr_store<uint16_t>((float)sum / 8.0f);
+        }
+        *(uint2*)(coarse + (z * cd.x + x) * cd.y + 4 * g) = make_uint2(o[0] | (o[1] << 16), o[2] | (o[3] << 16));
+    }
+}
+
 template <typename T>
 struct LevelPtrs {
     const T* img[MAXL];
@@ -795,16 +823,24 @@ void launch_sample(aprh_context* ctx, const aprh_apr& a, const T* img, T* parts)
     int64_t tot = 0;
     for (int l = lmin; l < lmax; ++l) {
         off[l] = tot;
-        tot += a.y_num[l] * a.x_num[l] * a.z_num[l];
+        tot += (a.y_num[l] * a.x_num[l] * a.z_num[l] + 7) / 8 * 8;   /* 16-byte aligned levels */
     }
     Buf pyr(ctx->pool, (size_t)std::max<int64_t>(tot, 1) * sizeof(T));
     T* Pyr = pyr.as<T>();
     const T* fine = img;
     for (int l = lmax - 1; l >= lmin; --l) {
-        hipLaunchKernelGGL(k_pyr_down<T>, dim3(4096), dim3(256), 0, st, fine, Dims{a.y_num[l + 1], a.x_num[l + 1], a.z_num[l + 1]},
-                           Pyr + off[l], Dims{a.y_num[l], a.x_num[l], a.z_num[l]});
+        const Dims fd{a.y_num[l + 1], a.x_num[l + 1], a.z_num[l + 1]}, cd{a.y_num[l], a.x_num[l], a.z_num[l]};
+        T* coarse = Pyr + off[l];
+        if (std::is_same<T, uint16_t>::value && fd.y % 8 == 0 && cd.y * 2 == fd.y && ((uintptr_t)fine & 15u) == 0 &&
+            ((uintptr_t)coarse & 7u) == 0) {
+            const int64_t work = cd.y / 4 * cd.x * cd.z;
+            hipLaunchKernelGGL(k_pyr_down_u16v, dim3((unsigned)std::min<int64_t>(cdiv(work, 256), 16384)), dim3(256), 0, st,
+                               (const uint16_t*)fine, fd, (uint16_t*)coarse, cd);
+        } else {
+            hipLaunchKernelGGL(k_pyr_down<T>, dim3(4096), dim3(256), 0, st, fine, fd, coarse, cd);
+        }
         HIP_LAUNCH_CHECK();
-        fine = Pyr + off[l];
+        fine = coarse;
     }
     LevelPtrs<T> L;
     std::memset(&L, 0, sizeof(L));
