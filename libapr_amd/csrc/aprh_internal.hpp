@@ -274,6 +274,28 @@ void synth_device(aprh_context* ctx, uint16_t* out, int64_t y, int64_t x, int64_
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+/* Division of 32-bit unsigned values by a divisor fixed at launch: a multiply-high, a
+   subtract and two shifts (the round-up method of Granlund and Montgomery), exact for every
+   32-bit dividend.  Grid-stride kernels use it to split a linear index into (y, x, z)
+   instead of 64-bit division. */
+struct FastDiv {
+    uint32_t m = 0;
+    int l = 0;
+    FastDiv() = default;
+    explicit FastDiv(uint64_t d) {
+        if (d == 0 || d > 0xFFFFFFFFull) throw Error(APRH_ERR_UNSUPPORTED, "divisor outside 32 bits");
+        while ((1ull << l) < d) ++l;
+        m = (l == 0) ? 0u : (uint32_t)((((uint64_t)1 << 32) * (((uint64_t)1 << l) - d)) / d + 1);
+    }
+    __device__ __forceinline__ uint32_t div(uint32_t n) const {
+        if (l == 0) return n;
+        const uint32_t t = __umulhi(m, n);
+        return (t + ((n - t) >> 1)) >> (l - 1);
+    }
+};
+/* a grid-stride loop over n items may use 32-bit indices */
+inline bool fits32(int64_t n) { return n >= 0 && n < (int64_t)0x7FFFFFFF; }
+
 /* record the start of stage s (the end of the previous one) when timing is on */
 inline void mark(aprh_context* ctx, int s) {
     if (ctx->timing) {

@@ -228,10 +228,11 @@ __device__ __forceinline__ void grad_ds_vec_body(const uint16_t* __restrict__ im
 }
 
 /* ---------------- inverse cubic B-spline, [1 4 1]/6 ---------------- */
-__device__ __forceinline__ void inv_y_body(const float* __restrict__ in, float* out, int64_t y_num, size_t n, int64_t G, int64_t t) {
+__device__ __forceinline__ void inv_y_body(const float* __restrict__ in, float* out, int64_t y_num, size_t n, int64_t G, int64_t t,
+                                           const FastDiv& fy, bool i32) {
     const float a1 = (float)(1.0 / 6.0), a2 = (float)(4.0 / 6.0), a3 = (float)(1.0 / 6.0);
     for (size_t i = (size_t)t * blockDim.x + threadIdx.x; i < n; i += (size_t)G * blockDim.x) {
-        const int64_t k = (int64_t)(i % (size_t)y_num);
+        const int64_t k = i32 ? (int64_t)((uint32_t)i - fy.div((uint32_t)i) * (uint32_t)y_num) : (int64_t)(i % (size_t)y_num);
         const float* l = in + (i - (size_t)k);
         float v;
         if (k == 0) {
@@ -249,10 +250,16 @@ __device__ __forceinline__ void inv_y_body(const float* __restrict__ in, float* 
 
 /* x / z: out[0] = a1*in[1] + a2*in[0] + a3*in[1]; last = (a1+a3)*in[n-2] + a2*in[n-1] */
 __device__ __forceinline__ void inv_outer_body(const float* __restrict__ in, float* out, int64_t n_axis, int64_t stride, size_t n,
-                                               int64_t G, int64_t t) {
+                                               int64_t G, int64_t t, const FastDiv& fs, const FastDiv& fa, bool i32) {
     const float a1 = (float)(1.0 / 6.0), a2 = (float)(4.0 / 6.0), a3 = (float)(1.0 / 6.0);
     for (size_t i = (size_t)t * blockDim.x + threadIdx.x; i < n; i += (size_t)G * blockDim.x) {
-        const int64_t j = (int64_t)((i / (size_t)stride) % (size_t)n_axis);
+        int64_t j;
+        if (i32) {
+            const uint32_t q = fs.div((uint32_t)i);
+            j = (int64_t)(q - fa.div(q) * (uint32_t)n_axis);
+        } else {
+            j = (int64_t)((i / (size_t)stride) % (size_t)n_axis);
+        }
         float v;
         if (j == n_axis - 1) {
             v = (a1 + a3) * in[i - stride];
@@ -563,6 +570,8 @@ struct LevelsArgs {
     float Ip_th, sigma_th, sigma_th_max, grad_th, mult;
     G* grad_keep;
     int64_t Gs;   /* virtual grid */
+    FastDiv f_yd, f_xd;
+    int i32;
 };
 template <typename G>
 __device__ __forceinline__ void levels_body(const LevelsArgs<G>& a, int64_t t) {
@@ -579,9 +588,18 @@ __device__ __forceinline__ void levels_body(const LevelsArgs<G>& a, int64_t t) {
     const float ipth = a.Ip_th + *a.d_off;
     const size_t n = (size_t)(g.yd * g.xd * g.zd);
     for (size_t i = (size_t)t * blockDim.x + threadIdx.x; i < n; i += (size_t)a.Gs * blockDim.x) {
-        const int64_t y = (int64_t)(i % (size_t)g.yd);
-        const int64_t r = (int64_t)(i / (size_t)g.yd);
-        const int64_t x = r % g.xd, z = r / g.xd;
+        int64_t y, x, z;
+        if (a.i32) {
+            const uint32_t r = a.f_yd.div((uint32_t)i), zz = a.f_xd.div(r);
+            y = (int64_t)((uint32_t)i - r * (uint32_t)g.yd);
+            x = (int64_t)(r - zz * (uint32_t)g.xd);
+            z = zz;
+        } else {
+            y = (int64_t)(i % (size_t)g.yd);
+            const int64_t r = (int64_t)(i / (size_t)g.yd);
+            x = r % g.xd;
+            z = r / g.xd;
+        }
         /* rescale_var + unpad (LocalIntensityScale.hpp:173-184, PixelData.hpp:1189-1210) */
         float s = A[((z + g.pz) * g.xp + (x + g.px)) * g.yp + (y + g.py)] * var_rescale;
         if (sigma_raw) sigma_raw[i] = s;
@@ -667,18 +685,22 @@ struct InvArgs {
     int64_t n_axis, stride;
     size_t n;
     int64_t G;
+    FastDiv f_stride, f_axis;   /* valid when i32 */
+    int i32;
 };
 struct InvY {
     using Args = InvArgs;
     static constexpr int THREADS = 256;
     static const char* name() { return "inv_bspline_y"; }
-    static __device__ __forceinline__ void run(const Args& a, int64_t t) { inv_y_body(a.in, a.out, a.n_axis, a.n, a.G, t); }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) { inv_y_body(a.in, a.out, a.n_axis, a.n, a.G, t, a.f_axis, a.i32); }
 };
 struct InvOuter {
     using Args = InvArgs;
     static constexpr int THREADS = 256;
     static const char* name() { return "inv_bspline_outer"; }
-    static __device__ __forceinline__ void run(const Args& a, int64_t t) { inv_outer_body(a.in, a.out, a.n_axis, a.stride, a.n, a.G, t); }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) {
+        inv_outer_body(a.in, a.out, a.n_axis, a.stride, a.n, a.G, t, a.f_stride, a.f_axis, a.i32);
+    }
 };
 template <typename G>
 struct Levels {
@@ -754,16 +776,18 @@ float* launch_inv_bspline_pp(aprh_context* ctx, float* a, float* b, int64_t y, i
     float* src = a;
     float* dst = b;
     const int64_t G = stride_tiles((int64_t)n, 256 * 4);
+    const int i32 = fits32((int64_t)n) ? 1 : 0;
+    auto fd = [&](int64_t d) { return i32 ? FastDiv((uint64_t)d) : FastDiv(); };
     if (ay) {
-        launch_tiles<InvY>(ctx, {src, dst, y, 1, n, G}, G);
+        launch_tiles<InvY>(ctx, {src, dst, y, 1, n, G, FastDiv(), fd(y), i32}, G);
         std::swap(src, dst);
     }
     if (ax) {
-        launch_tiles<InvOuter>(ctx, {src, dst, x, y, n, G}, G);
+        launch_tiles<InvOuter>(ctx, {src, dst, x, y, n, G, fd(y), fd(x), i32}, G);
         std::swap(src, dst);
     }
     if (az) {
-        launch_tiles<InvOuter>(ctx, {src, dst, z, x * y, n, G}, G);
+        launch_tiles<InvOuter>(ctx, {src, dst, z, x * y, n, G, fd(x * y), fd(z), i32}, G);
         std::swap(src, dst);
     }
     return src;
@@ -837,8 +861,10 @@ void launch_levels_padded(aprh_context* ctx, const float* A, const LisParams& lp
                           const aprh_params& par, const float* d_offset, float mult, G* grad_keep) {
     PadGeom g = pad_geom(yd, xd, zd, lp);
     const int64_t Gs = stride_tiles(yd * xd * zd, 256 * 4);
+    const int i32 = fits32(yd * xd * zd) ? 1 : 0;
     launch_tiles<Levels<G>>(ctx, LevelsArgs<G>{A, g, lp.var_rescale, inten, grad, sigma_raw, sigma_out, level, d_offset, par.Ip_th,
-                                               par.sigma_th, par.sigma_th_max, par.grad_th, mult, grad_keep, Gs},
+                                               par.sigma_th, par.sigma_th_max, par.grad_th, mult, grad_keep, Gs,
+                                               i32 ? FastDiv((uint64_t)yd) : FastDiv(), i32 ? FastDiv((uint64_t)xd) : FastDiv(), i32},
                             Gs);
 }
 
