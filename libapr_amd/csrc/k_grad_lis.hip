@@ -293,15 +293,17 @@ struct PadGeom {
    One lane per line (the recursion is sequential in y); 64-line x 64-sample tiles are
    staged through LDS so global loads and stores stay coalesced along y.  The per-line
    source offsets are computed once per block. */
+constexpr int MEANY_LINES = 64;
 template <int O>
 __device__ __forceinline__ void mean_y_body(int mode, const float* __restrict__ inten, float* A, const PadGeom& g, int64_t blk) {
-    constexpr int TY = 32;   /* samples per tile: 8.4 KB of LDS per wave, so ~19 waves per CU */
+    constexpr int TY = 32;          /* samples per tile                     */
+    constexpr int NL = MEANY_LINES; /* lines per block, one per thread      */
     constexpr int H = 2 * O + 2;
-    __shared__ float tile[64][TY + 1];
-    __shared__ int64_t src_row[64];   /* inten offset of the (reflected) source row of each line */
+    __shared__ float tile[NL][TY + 1];
+    __shared__ int64_t src_row[NL];   /* inten offset of the (reflected) source row of each line */
     const int lane = threadIdx.x;
     const int64_t n_lines = g.xp * g.zp;
-    const int64_t line0 = blk * 64;
+    const int64_t line0 = blk * NL;
     const int64_t me = line0 + lane;
     const bool valid = me < n_lines;
     const int yn = (int)g.yp, ydn = (int)g.yd, py = (int)g.py;
@@ -311,7 +313,7 @@ __device__ __forceinline__ void mean_y_body(int mode, const float* __restrict__ 
         src_row[lane] = (refl(zp - g.pz, g.zd) * g.xd + refl(xp - g.px, g.xd)) * g.yd;
     }
     __syncthreads();
-    const int nl = (int)min((int64_t)64, n_lines - line0);
+    const int nl = (int)min((int64_t)NL, n_lines - line0);
     auto src = [&](int l, int yy) -> float {
         int ys = yy - py;
         if (ys < 0) ys = -ys;
@@ -332,9 +334,8 @@ __device__ __forceinline__ void mean_y_body(int mode, const float* __restrict__ 
         }
     }
     for (int p0 = 0; p0 < yn; p0 += TY) {
-        /* inputs for outputs [p0, p0+64) are samples [p0+O, p0+O+64) */
-        /* 64 lines x TY samples, 64 / TY lines per load instruction */
-        for (int idx = lane; idx < 64 * TY; idx += 64) {
+        /* inputs for outputs [p0, p0+TY) are samples [p0+O, p0+O+TY): NL lines x TY samples */
+        for (int idx = lane; idx < NL * TY; idx += NL) {
             const int l = idx / TY, k = idx % TY, yy = p0 + O + k;
             if (l < nl && yy < yn) tile[l][k] = src(l, yy);
         }
@@ -360,7 +361,7 @@ __device__ __forceinline__ void mean_y_body(int mode, const float* __restrict__ 
             }
         }
         __syncthreads();
-        for (int idx = lane; idx < 64 * TY; idx += 64) {
+        for (int idx = lane; idx < NL * TY; idx += NL) {
             const int l = idx / TY, k = idx % TY, q = p0 + k;
             if (l < nl && q < yn - O) A[(line0 + l) * yn + q] = tile[l][k];
         }
@@ -477,7 +478,7 @@ struct MeanY {
         float* A;
         PadGeom g;
     };
-    static constexpr int THREADS = 64;
+    static constexpr int THREADS = MEANY_LINES;
     static const char* name() { return "lis_mean_y"; }
     static __device__ __forceinline__ void run(const Args& a, int64_t t) { mean_y_body<O>(a.mode, a.inten, a.A, a.g, t); }
 };
@@ -506,7 +507,7 @@ struct MeanOuter2 {
 template <int O>
 void mean_y(aprh_context* ctx, int mode, const float* inten, float* A, const PadGeom& g) {
     const int64_t lines = g.xp * g.zp;
-    launch_tiles<MeanY<O>>(ctx, {mode, inten, A, g}, cdiv(lines, 64));
+    launch_tiles<MeanY<O>>(ctx, {mode, inten, A, g}, cdiv(lines, MEANY_LINES));
 }
 
 void mean_y_dispatch(aprh_context* ctx, int O, int mode, const float* inten, float* A, const PadGeom& g) {
