@@ -59,13 +59,24 @@ __device__ __forceinline__ void level_down_body(const uint8_t* __restrict__ fine
 }
 
 /* seeds of the top level straight from the level map (PullingScheme::fill, k == l_max) */
+__device__ __forceinline__ uint8_t seed_of(uint8_t m, int level, int mode) {
+    const bool seed = (mode == 0) ? (m >= level) : (mode == 1) ? (m <= level) : (m == level);
+    return seed ? SEED : EMPTY;
+}
+/* eight cells per thread (8-byte loads and stores) over the 8-aligned body, then the tail */
 __device__ __forceinline__ void seed_body(const uint8_t* __restrict__ lev, uint8_t* pct, size_t n, int level, int mode, int64_t G,
                                           int64_t t) {
-    for (size_t i = (size_t)t * blockDim.x + threadIdx.x; i < n; i += (size_t)G * blockDim.x) {
-        const uint8_t m = lev[i];
-        bool seed = (mode == 0) ? (m >= level) : (mode == 1) ? (m <= level) : (m == level);
-        pct[i] = seed ? SEED : EMPTY;
+    const bool wide = (((uintptr_t)lev | (uintptr_t)pct) & 7u) == 0;
+    const size_t nw = wide ? n / 8 : 0;
+    const size_t stride = (size_t)G * blockDim.x;
+    for (size_t i = (size_t)t * blockDim.x + threadIdx.x; i < nw; i += stride) {
+        const uint64_t w = ((const uint64_t*)lev)[i];
+        uint64_t o = 0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) o |= (uint64_t)seed_of((uint8_t)(w >> (8 * j)), level, mode) << (8 * j);
+        ((uint64_t*)pct)[i] = o;
     }
+    for (size_t i = nw * 8 + (size_t)t * blockDim.x + threadIdx.x; i < n; i += stride) pct[i] = seed_of(lev[i], level, mode);
 }
 
 /* one pulling-scheme level (PullingScheme.hpp:111-132 loop body): set_ascendant_neighbours
@@ -208,12 +219,21 @@ __device__ __forceinline__ void pull_level8_body(uint8_t* P, Dims d, uint8_t* Pc
 
 /* equivalence optimisation step for one level (LinearAccess.hpp:265-294) */
 __device__ __forceinline__ void upseed_body(uint8_t* P, Dims d, const uint8_t* __restrict__ Pp, Dims dp, int min_type, int64_t G,
-                                            int64_t t) {
+                                            int64_t t, const FastDiv& fy, const FastDiv& fx, bool i32) {
     const size_t n = (size_t)(d.y * d.x * d.z);
     for (size_t i = (size_t)t * blockDim.x + threadIdx.x; i < n; i += (size_t)G * blockDim.x) {
-        const int64_t y = (int64_t)(i % (size_t)d.y);
-        const int64_t r = (int64_t)(i / (size_t)d.y);
-        const int64_t x = r % d.x, z = r / d.x;
+        int64_t y, x, z;
+        if (i32) {
+            const uint32_t r = fy.div((uint32_t)i), zz = fx.div(r);
+            y = (int64_t)((uint32_t)i - r * (uint32_t)d.y);
+            x = (int64_t)(r - zz * (uint32_t)d.x);
+            z = zz;
+        } else {
+            y = (int64_t)(i % (size_t)d.y);
+            const int64_t r = (int64_t)(i / (size_t)d.y);
+            x = r % d.x;
+            z = r / d.x;
+        }
         const uint8_t s = Pp[((z / 2) * dp.x + (x / 2)) * dp.y + (y / 2)];
         if (s > 0 && s <= min_type) P[i] = UPSEED;
     }
@@ -492,10 +512,14 @@ struct Upseed {
         Dims dp;
         int min_type;
         int64_t G;
+        FastDiv fy, fx;
+        int i32;
     };
     static constexpr int THREADS = 256;
     static const char* name() { return "upseed"; }
-    static __device__ __forceinline__ void run(const Args& a, int64_t t) { upseed_body(a.P, a.d, a.Pp, a.dp, a.min_type, a.G, t); }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) {
+        upseed_body(a.P, a.d, a.Pp, a.dp, a.min_type, a.G, t, a.fy, a.fx, a.i32);
+    }
 };
 struct RowCount {
     struct Args {
@@ -717,7 +741,8 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
     /* particle-cell tree levels lmin .. top (= lmax - 1), one contiguous u8 buffer */
     const int top = lmax - 1;
     std::vector<int64_t> off(top + 2, 0);
-    for (int l = lmin; l <= top; ++l) off[l + 1] = off[l] + a.y_num[l] * a.x_num[l] * a.z_num[l];
+    /* level maps 8-byte aligned, so whole-word passes apply */
+    for (int l = lmin; l <= top; ++l) off[l + 1] = off[l] + (a.y_num[l] * a.x_num[l] * a.z_num[l] + 7) / 8 * 8;
     const int64_t pct_bytes = off[top + 1];
     Buf pct(ctx->pool, (size_t)pct_bytes);
     Buf lev(ctx->pool, (size_t)pct_bytes);
@@ -754,7 +779,10 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
     /* equivalence optimisation, levels upward in order */
     for (int l = lmin + 1; l <= top; ++l) {
         const int64_t G = stride_tiles(off[l + 1] - off[l], 256 * 4);
-        launch_tiles<Upseed>(ctx, {P + off[l], D(l), P + off[l - 1], D(l - 1), min_type, G}, G);
+        const int i32 = fits32(off[l + 1] - off[l]) ? 1 : 0;
+        launch_tiles<Upseed>(ctx, {P + off[l], D(l), P + off[l - 1], D(l - 1), min_type, G, i32 ? FastDiv((uint64_t)a.y_num[l]) : FastDiv(),
+                                   i32 ? FastDiv((uint64_t)a.x_num[l]) : FastDiv(), i32},
+                             G);
     }
     /* segment table over APR levels lmin..lmax */
     RowTable t;
@@ -803,7 +831,7 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
         a.pct_dims.assign(3 * (top + 1), 0);
         a.pct.resize(top + 1);
         for (int l = lmin; l <= top; ++l) {
-            const size_t nb2 = (size_t)(off[l + 1] - off[l]);
+            const size_t nb2 = (size_t)(a.y_num[l] * a.x_num[l] * a.z_num[l]);
             a.pct[l].reset(ctx->pool, nb2);
             HIP_CHECK(hipMemcpyAsync(a.pct[l].ptr(), P + off[l], nb2, hipMemcpyDeviceToDevice, st));
             a.pct_dims[3 * l] = a.y_num[l];
