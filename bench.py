@@ -143,8 +143,8 @@ def run_gpu(args, with_conv=True):
     Npad = (yd + 6) * (xd + 6) * (zd + 6)
     # single-kernel stages: (kernel, stages it serves = launches per step); the dominant kernel is
     # the one with the most device time per step, its roofline per launch
-    kernels = {"k_tiles<IirOuterVec<unsigned short, 8>>": ["smooth_bspline_x", "smooth_bspline_z"],
-               "k_tiles<IirYLds>": ["smooth_bspline_y"], "k_tiles<GradDsVec<true>>": ["gradient_ds"],
+    kernels = {"k_tiles<IirOuterCk<unsigned short, 16>>": ["smooth_bspline_x", "smooth_bspline_z"],
+               "k_tiles<IirYLdsCk>": ["smooth_bspline_y"], "k_tiles<GradDsVec<true>>": ["gradient_ds"],
                "k_tiles<Levels<unsigned short>>": ["levels"]}
     kt = {k: sum(stage_avg.get(s, 0.0) for s in v) for k, v in kernels.items()}
     dom = max(kt, key=kt.get)

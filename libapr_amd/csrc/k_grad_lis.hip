@@ -112,9 +112,20 @@ __device__ __forceinline__ void grad_ds_body(const T* __restrict__ img, T* grad,
  * the 32 gradient magnitudes, and writes 4 maxima and 4 means.
  * Unit spacing (hx = hy = hz = 1) uses the exact integer form of the
  * reference's double arithmetic: each squared term is d^2/4, the sum k/4 is
- * exact, and trunc(sqrt(k/4)) == floor(sqrt(k)) >> 1 == (uint)sqrt((double)k) >> 1
- * because a correctly rounded sqrt of an integer below 2^52 never rounds up
- * to the next integer. */
+ * exact, and trunc(sqrt(k/4)) == floor(sqrt(k)) >> 1.  floor(sqrt(k)) comes from
+ * isqrt_d2 below. */
+/* floor(sqrt(a + b + c)) for squares a, b, c < 2^32 (k = a + b + c < 2^34).  The float
+ * estimate carries a relative error below 2^-21 (three roundings of the sum, one of sqrtf),
+ * i.e. an absolute error below 0.06 for sqrt(k) < 2^17, so its truncation is within one of
+ * the answer and one exact 64-bit correction step finishes it. */
+__device__ __forceinline__ uint32_t isqrt_d2(uint32_t a, uint32_t b, uint32_t c) {
+    const uint64_t k = (uint64_t)a + b + c;
+    uint32_t s = (uint32_t)sqrtf(((float)a + (float)c) + (float)b);
+    if ((uint64_t)s * s > k) --s;
+    else if ((uint64_t)(s + 1) * (s + 1) <= k) ++s;
+    return s;
+}
+
 __device__ __forceinline__ int u16at(const uint4& v, int i) {
     const uint32_t w = (i < 2) ? v.x : (i < 4) ? v.y : (i < 6) ? v.z : v.w;
     return (int)((i & 1) ? (w >> 16) : (w & 0xFFFFu));
@@ -178,8 +189,9 @@ __device__ __forceinline__ void grad_ds_vec_body(const uint16_t* __restrict__ im
                 const int dzv = u16at(dn, i) - u16at(up, i);
                 uint32_t g;
                 if (UNIT) {
-                    const int64_t k = (int64_t)dxv * dxv + (int64_t)dzv * dzv + (int64_t)dyv * dyv;
-                    g = ((uint32_t)sqrt((double)k)) >> 1;
+                    /* |d| <= 65535: the square of (uint32)d modulo 2^32 is d^2 exactly */
+                    const uint32_t ux = (uint32_t)dxv, uy = (uint32_t)dyv, uz = (uint32_t)dzv;
+                    g = isqrt_d2(ux * ux, uy * uy, uz * uz) >> 1;
                 } else {
                     const double q = sq_term(dxv, h2x) + sq_term(dzv, h2z) + sq_term(dyv, h2y);
                     g = d2u16(sqrt(q));
@@ -212,7 +224,7 @@ __device__ __forceinline__ void grad_ds_vec_body(const uint16_t* __restrict__ im
                 const int a = 2 * j, b = 2 * j + 1;
                 const int sum = u16at(c00, a) + u16at(c00, b) + u16at(c10, a) + u16at(c10, b) + u16at(c01, a) + u16at(c01, b) +
                                 u16at(c11, a) + u16at(c11, b);
-                mp[j] = (float)((double)(float)sum / 8.0);
+                mp[j] = (float)sum * 0.125f;   /* sum < 2^24: exact, equal to the double division */
             }
             *(float4*)(mean + o) = m;
         }

@@ -2,15 +2,19 @@
  * k_iir.hip -- B-spline smoothing of the conversion pipeline on gfx950.
  *
  *  - min/max reduction + bspline offset     (AutoParameters.hpp:165-196)
- *  - y pass (ComputeGradient.hpp:248-413): one lane per y-line.  Lines whose
- *    length is a multiple of the 16-byte vector are streamed with 16-byte
- *    loads/stores, several vectors ahead of the recursion; other lengths go
- *    through an LDS tile (64 lines x 64 samples) so global accesses stay
- *    coalesced.  The offset copy of computeL (APRConverter.hpp:213-224) is
+ *  - y pass (ComputeGradient.hpp:248-413): one lane per y-line.  u16 lines of
+ *    a multiple of 64 samples go through an LDS tile (64 lines x 64 samples)
+ *    so global accesses stay coalesced, and do not store the causal result
+ *    (per-chunk recursion states are saved and the chunk is recomputed for the
+ *    anti-causal sweep); other lines whose length is a multiple of the 16-byte
+ *    vector are streamed with 16-byte loads/stores, several vectors ahead of
+ *    the recursion.  The offset copy of computeL (APRConverter.hpp:213-224) is
  *    fused into the first read.
  *  - x / z passes (ComputeGradient.hpp:415-599, 601-779): lanes run along y
  *    (8 u16 / 4 f32 samples per lane), so each recursion step is one
- *    coalesced 16-byte-per-lane row access; 8 steps are loaded ahead.
+ *    coalesced 16-byte-per-lane row access.  Axes of 64 samples or more save
+ *    the causal state every 16 steps instead of storing the causal result and
+ *    recompute each chunk in the anti-causal sweep, one chunk loaded ahead.
  *
  * Every lane keeps the reference's per-line FP32 operation order; the build
  * disables FP contraction, so results are bit-identical to the CPU path.
@@ -493,10 +497,13 @@ __device__ __forceinline__ void iir_y_vec_body(const T* __restrict__ in, T* out,
 /* x / z pass, 16-byte vectors along y, D steps of the recursion loaded ahead */
 template <typename T, int D>
 __device__ __forceinline__ void iir_outer_vec_body(T* img, int64_t y_num, int64_t n, int64_t s, int64_t outer_stride, const IIRCoeffs& c,
-                                                   int* d_err, int64_t bx, int64_t by) {
+                                                   int64_t lanes, int* d_err, int64_t t) {
     constexpr int V = Vec16<T>::N;
-    const int64_t yv = bx * blockDim.x + threadIdx.x;
-    if (yv * V >= y_num) return;
+    const int64_t ny = y_num / V;
+    /* lanes are numbered densely over (line, y vector), so short y rows still fill the blocks */
+    const int64_t lane = t * blockDim.x + threadIdx.x;
+    if (lane >= lanes) return;
+    const int64_t by = lane / ny, yv = lane - by * ny;
     T* basep = img + by * outer_stride + yv * V;
     const int64_t sv = s / V;   /* stride in vectors */
     uint4* base = (uint4*)basep;
@@ -587,16 +594,229 @@ __device__ __forceinline__ void iir_outer_vec_body(T* img, int64_t y_num, int64_
     if (Val<T>::failed(err)) atomicAdd(d_err, 1);
 }
 
-/* y pass through an LDS tile with full-line coalescing: a wave owns 64 lines;
- * each chunk of CH samples per line is loaded by 8 lanes per line (16 B each,
- * CH*2 = 128 B per line), staged in LDS with a 144-byte row pitch (conflict
- * free for 16-byte LDS accesses), and each lane then runs the recursion over
- * its own row.  The next chunk's global loads are issued before the current
- * chunk is processed.  u16 only; requires y_num % 64 == 0. */
-__device__ __forceinline__ void iir_y_lds_body(const uint16_t* __restrict__ in, uint16_t* out, int64_t y_num, int64_t n_lines,
-                                               const IIRCoeffs& c, const float* d_off, int* d_err, int64_t blk) {
-    constexpr int CH = 64;            /* samples per line per chunk */
-    constexpr int PITCH = 72;         /* u16 per LDS row (144 B)    */
+/* x / z pass without storing the causal result.  The causal recursion runs once over the
+ * whole line and saves its state (the two previous outputs) at the start of every chunk of
+ * C steps; the anti-causal sweep then walks the chunks from the end, re-reads each chunk's
+ * input, recomputes its causal outputs from the saved state into registers and runs the
+ * anti-causal recursion over them.  Per sample the line moves 2 reads and 1 write instead of
+ * 2 reads and 2 writes, plus 4/C bytes of saved state.  Every value is computed by the same
+ * operations in the same order as iir_outer_vec_body, so the outputs are identical; the causal
+ * outputs held in registers are packed like the stored ones, and saved states are packed the
+ * same way (exact for u16 unless the pass reports an overflow, where the reference throws).
+ * In place: chunk k is written only after its input was read, and chunks below k are read
+ * later but written later still. */
+template <typename T, int C>
+__device__ __forceinline__ void iir_outer_ck_body(T* img, int64_t y_num, int64_t n, int64_t s, int64_t outer_stride, const IIRCoeffs& c,
+                                                  int* d_err, uint4* __restrict__ ck, int64_t ck_lanes, int64_t t) {
+    constexpr int V = Vec16<T>::N;
+    const int64_t ny = y_num / V;
+    /* lanes are numbered densely over (line, y vector), so short y rows still fill the blocks */
+    const int64_t lane = t * blockDim.x + threadIdx.x;
+    if (lane >= ck_lanes) return;
+    /* lane counts stay below 2^32 (the launcher checks): 32-bit division */
+    const uint32_t by = (uint32_t)lane / (uint32_t)ny, yv = (uint32_t)lane - by * (uint32_t)ny;
+    uint4* base = (uint4*)(img + (int64_t)by * outer_stride + (int64_t)yv * V);
+    const int64_t sv = s / V;
+    const float b1 = c.b1, b2 = c.b2, nf = c.nf;
+    /* causal steps j = 2 .. n-1; the anti-causal sweep consumes j = 2 .. n-3 in chunks
+       [2 + kC, min(2 + (k+1)C, n-2)), k < K.  Scratch per lane: the packed states at the
+       starts of chunks 1 .. K-1, then the four boundary sums (unrounded floats), which are
+       parked there between the sweeps to keep registers free for the chunk buffers. */
+    const int64_t K = (n - 4 + C - 1) / C;
+    uint4* sums = ck + (K - 1) * ck_lanes * 2 + lane * V;
+    auto put_f = [&](uint4* dst, const float* v) {
+#pragma unroll
+        for (int q = 0; q < V / 4; ++q) dst[q] = make_uint4(__float_as_uint(v[4 * q]), __float_as_uint(v[4 * q + 1]),
+                                                             __float_as_uint(v[4 * q + 2]), __float_as_uint(v[4 * q + 3]));
+    };
+    auto get_f = [&](const uint4* src, float* v) {
+#pragma unroll
+        for (int q = 0; q < V / 4; ++q) {
+            const uint4 w = src[q];
+            v[4 * q] = __uint_as_float(w.x);
+            v[4 * q + 1] = __uint_as_float(w.y);
+            v[4 * q + 2] = __uint_as_float(w.z);
+            v[4 * q + 3] = __uint_as_float(w.w);
+        }
+    };
+    int err = 0;
+    float p1[V], p2[V];
+    {
+        float v1[V], v2[V], v3[V], v4[V];
+#pragma unroll
+        for (int i = 0; i < V; ++i) v1[i] = v2[i] = v3[i] = v4[i] = 0.0f;
+        for (int j = 0; j < c.mL; ++j) {
+            const uint4 a = base[(int64_t)j * sv];
+            const uint4 b = base[(n - 1 - j) * sv];
+            const float w1 = c.bc1[j], w2 = c.bc2[j], w3 = c.bc3[j], w4 = c.bc4[j];
+#pragma unroll
+            for (int i = 0; i < V; ++i) {
+                const float xa = Vec16<T>::get(a, i), xb = Vec16<T>::get(b, i);
+                v1[i] += w1 * xa;
+                v2[i] += w2 * xa;
+                v3[i] += w3 * xb;
+                v4[i] += w4 * xb;
+            }
+        }
+        put_f(sums, v1);
+        put_f(sums + V / 4, v2);
+        put_f(sums + 2 * (V / 4), v3);
+        put_f(sums + 3 * (V / 4), v4);
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            p1[i] = v1[i];
+            p2[i] = v2[i];
+        }
+    }
+    /* chunk loads are issued one chunk ahead of the recursion */
+    auto load_chunk = [&](uint4* dst, int64_t j0, int cnt) {
+#pragma unroll
+        for (int d = 0; d < C; ++d)
+            if (d < cnt) dst[d] = base[(j0 + d) * sv];
+    };
+    uint4 cur[C], nxt[C];
+    {
+        int cnt = (int)min((int64_t)C, n - 2);
+        load_chunk(cur, 2, cnt);
+        for (int64_t k = 0, j0 = 2; j0 < n; ++k, j0 += C) {
+            const int64_t jn = j0 + C;
+            const int cn = jn < n ? (int)min((int64_t)C, n - jn) : 0;
+            load_chunk(nxt, jn, cn);
+            if (k > 0 && k < K) {
+                uint4 q1 = make_uint4(0, 0, 0, 0), q2 = make_uint4(0, 0, 0, 0);
+#pragma unroll
+                for (int i = 0; i < V; ++i) {
+                    Vec16<T>::set_r(q1, i, p1[i]);
+                    Vec16<T>::set_r(q2, i, p2[i]);
+                }
+                uint4* cp = ck + ((k - 1) * ck_lanes + lane) * 2;
+                cp[0] = q1;
+                cp[1] = q2;
+            }
+#pragma unroll
+            for (int d = 0; d < C; ++d) {
+                if (d < cnt) {
+#pragma unroll
+                    for (int i = 0; i < V; ++i) {
+                        const float t = Val<T>::rnd(Vec16<T>::get(cur[d], i) + b1 * p1[i] + b2 * p2[i], err);
+                        p2[i] = p1[i];
+                        p1[i] = t;
+                    }
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < C; ++d) cur[d] = nxt[d];
+            cnt = cn;
+        }
+    }
+    /* anti-causal: boundary values from the unrounded sums */
+    float v3[V], v4[V];
+    get_f(sums + 2 * (V / 4), v3);
+    get_f(sums + 3 * (V / 4), v4);
+    {
+        uint4 oa = make_uint4(0, 0, 0, 0), ob = make_uint4(0, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            Vec16<T>::set_r(oa, i, Val<T>::rnd(v4[i] * nf, err));
+            Vec16<T>::set_r(ob, i, Val<T>::rnd(v3[i] * nf, err));
+        }
+        base[(n - 1) * sv] = oa;
+        base[(n - 2) * sv] = ob;
+    }
+    auto back = [&](const uint4& f, int64_t j) {
+        uint4 o = make_uint4(0, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            const float t = (Vec16<T>::get(f, i) + b1 * v3[i]) + b2 * v4[i];
+            Vec16<T>::set_r(o, i, Val<T>::rnd(t * nf, err));
+            v4[i] = v3[i];
+            v3[i] = t;
+        }
+        base[j * sv] = o;
+    };
+    /* causal outputs of chunk `buf` from state (p1, p2), then the anti-causal sweep over it */
+    auto chunk = [&](uint4* buf, int64_t j0, int cnt) {
+#pragma unroll
+        for (int d = 0; d < C; ++d) {
+            if (d < cnt) {
+                uint4 o = make_uint4(0, 0, 0, 0);
+#pragma unroll
+                for (int i = 0; i < V; ++i) {
+                    const float t = Val<T>::rnd(Vec16<T>::get(buf[d], i) + b1 * p1[i] + b2 * p2[i], err);
+                    Vec16<T>::set_r(o, i, t);
+                    p2[i] = p1[i];
+                    p1[i] = t;
+                }
+                buf[d] = o;
+            }
+        }
+#pragma unroll
+        for (int d = C - 1; d >= 0; --d)
+            if (d < cnt) back(buf[d], j0 + d);
+    };
+    if (K > 0) {
+        int64_t j0 = 2 + (K - 1) * C;
+        int cnt = (int)min((int64_t)C, n - 2 - j0);
+        load_chunk(cur, j0, cnt);
+        uint4 q1 = make_uint4(0, 0, 0, 0), q2 = q1;
+        if (K > 1) {
+            const uint4* cp = ck + ((K - 2) * ck_lanes + lane) * 2;
+            q1 = cp[0];
+            q2 = cp[1];
+        }
+        /* chunks K-1 .. 1 start from packed states; chunk 0 from the boundary sums */
+        for (int64_t k = K - 1; k >= 1; --k, j0 -= C) {
+            uint4 n1 = make_uint4(0, 0, 0, 0), n2 = n1;
+            load_chunk(nxt, j0 - C, C);
+            if (k > 1) {
+                const uint4* cp = ck + ((k - 2) * ck_lanes + lane) * 2;
+                n1 = cp[0];
+                n2 = cp[1];
+            }
+#pragma unroll
+            for (int i = 0; i < V; ++i) {
+                p1[i] = Vec16<T>::get(q1, i);
+                p2[i] = Vec16<T>::get(q2, i);
+            }
+            chunk(cur, j0, cnt);
+#pragma unroll
+            for (int d = 0; d < C; ++d) cur[d] = nxt[d];
+            q1 = n1;
+            q2 = n2;
+            cnt = C;
+        }
+        get_f(sums, p1);
+        get_f(sums + V / 4, p2);
+        chunk(cur, 2, cnt);
+    }
+    /* causal outputs at 0 and 1, as they would be stored */
+    float v1[V], v2[V];
+    get_f(sums, v1);
+    get_f(sums + V / 4, v2);
+    uint4 f0 = make_uint4(0, 0, 0, 0), f1 = make_uint4(0, 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+        Vec16<T>::set_r(f0, i, Val<T>::rnd(v2[i], err));
+        Vec16<T>::set_r(f1, i, Val<T>::rnd(v1[i], err));
+    }
+    if (n - 3 >= 1) back(f1, 1);
+    back(f0, 0);
+    if (Val<T>::failed(err)) atomicAdd(d_err, 1);
+}
+
+/* y pass without storing the causal result (u16, y_num % 64 == 0): the causal sweep streams
+ * the lines once and saves each line's recursion state (two floats) at the start of every
+ * chunk; the anti-causal sweep walks the chunks from the end, reloads each chunk of the input,
+ * recomputes its causal outputs into the LDS row from the saved state and runs the
+ * anti-causal recursion over the row before the chunk is written.  The same operations in the
+ * same order as a plain causal-then-anti-causal sweep, so the outputs are identical; the line
+ * moves 2 reads and one
+ * write instead of 2 reads and 2 writes (plus 8 bytes of state per 64 samples).  In place is
+ * safe: chunk ch is written after chunk ch - 1 has been loaded. */
+__device__ __forceinline__ void iir_y_lds_ck_body(const uint16_t* in, uint16_t* out, int64_t y_num, int64_t n_lines, const IIRCoeffs& c,
+                                                  const float* d_off, int* d_err, float2* __restrict__ ck, int64_t blk) {
+    constexpr int CH = 64;
+    constexpr int PITCH = 72;
     __shared__ __attribute__((aligned(16))) uint16_t tile[64 * PITCH];
     const int lane = threadIdx.x;
     const int64_t line0 = blk * 64;
@@ -606,7 +826,6 @@ __device__ __forceinline__ void iir_y_lds_body(const uint16_t* __restrict__ in, 
     const float off = has_off ? *d_off : 0.0f;
     const float b1 = c.b1, b2 = c.b2, nf = c.nf;
     const int64_t nch = y_num / CH;
-    /* cooperative layout: load instruction i covers lines 8i..8i+7, lane -> (line 8i + lane/8, segment lane%8) */
     const int seg = lane & 7;
     int err = 0;
 
@@ -629,7 +848,7 @@ __device__ __forceinline__ void iir_y_lds_body(const uint16_t* __restrict__ in, 
         }
     };
 
-    /* boundary sums (ComputeGradient.hpp:342-356), direct 16-byte loads of the own line */
+    /* boundary sums (ComputeGradient.hpp:342-356) */
     float t1 = 0, t2 = 0, t3 = 0, t4 = 0;
     if (valid) {
         const uint4* src = (const uint4*)(in + me * y_num);
@@ -660,63 +879,72 @@ __device__ __forceinline__ void iir_y_lds_body(const uint16_t* __restrict__ in, 
             }
         }
     }
-
-    /* causal pass */
-    float s1 = t1, s2 = t2;
-    /* y(0), y(1) are stored truncated, without the round<T> check (ComputeGradient.hpp:365-366) */
     const float t2q = Val<uint16_t>::store(t2), t1q = Val<uint16_t>::store(t1);
-    uint4 nxt[8];
-    gload(in, 0, nxt);
-    for (int64_t ch = 0; ch < nch; ++ch) {
-        __syncthreads();
-        lds_put(nxt);
-        if (ch + 1 < nch) gload(in, ch + 1, nxt);
-        __syncthreads();
-        if (valid) {
-            uint16_t* row = &tile[lane * PITCH];
+    /* causal recursion over one LDS row chunk from state (s1, s2); with `keep` the outputs
+       replace the row's samples */
+    auto causal = [&](uint16_t* row, int64_t ch, float& s1, float& s2, bool keep) {
 #pragma unroll
-            for (int q = 0; q < CH / 8; ++q) {
-                uint4 v = *(uint4*)(row + q * 8);
-                uint4 o = make_uint4(0, 0, 0, 0);
+        for (int q = 0; q < CH / 8; ++q) {
+            uint4 v = *(uint4*)(row + q * 8);
+            uint4 o = make_uint4(0, 0, 0, 0);
 #pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const int64_t k = ch * CH + q * 8 + i;
-                    float r;
-                    if (k == 0) {
-                        r = t2q;
-                    } else if (k == 1) {
-                        r = t1q;
-                    } else {
-                        const float t = s1 * b1 + s2 * b2 + vin<uint16_t>(v, i, off, has_off);
-                        r = Val<uint16_t>::rnd(t, err);
-                        s2 = s1;
-                        s1 = t;
-                    }
-                    Vec16<uint16_t>::set_r(o, i, r);
+            for (int i = 0; i < 8; ++i) {
+                const int64_t k = ch * CH + q * 8 + i;
+                float r;
+                if (k == 0) {
+                    r = t2q;
+                } else if (k == 1) {
+                    r = t1q;
+                } else {
+                    const float t = s1 * b1 + s2 * b2 + vin<uint16_t>(v, i, off, has_off);
+                    r = Val<uint16_t>::rnd(t, err);
+                    s2 = s1;
+                    s1 = t;
                 }
-                *(uint4*)(row + q * 8) = o;
+                Vec16<uint16_t>::set_r(o, i, r);
+            }
+            if (keep) *(uint4*)(row + q * 8) = o;
+        }
+    };
+
+    /* causal sweep: states only */
+    {
+        float s1 = t1, s2 = t2;
+        uint4 nxt[8];
+        gload(in, 0, nxt);
+        for (int64_t ch = 0; ch < nch; ++ch) {
+            __syncthreads();
+            lds_put(nxt);
+            if (ch + 1 < nch) gload(in, ch + 1, nxt);
+            __syncthreads();
+            if (valid) {
+                if (ch > 0) ck[(ch - 1) * n_lines + me] = make_float2(s1, s2);
+                causal(&tile[lane * PITCH], ch, s1, s2, false);
             }
         }
-        __syncthreads();
-        gstore(out, ch);
     }
     const float e2 = Val<uint16_t>::rnd(t3 * nf, err);
     const float e1 = Val<uint16_t>::rnd(t4 * nf, err);
     const float bv_m2 = Val<uint16_t>::store(e2), bv_m1 = Val<uint16_t>::store(e1);
     float r2 = bv_m1 / nf;
     float r1 = bv_m2 / nf;
-    /* every lane's forward stores must be visible before the tile loads below */
-    __threadfence_block();
-    __syncthreads();
-    /* anti-causal pass over the forward output */
-    gload(out, nch - 1, nxt);
+    /* anti-causal sweep: recompute each chunk's causal outputs, then run backwards over them */
+    uint4 nxt[8];
+    gload(in, nch - 1, nxt);
     for (int64_t ch = nch - 1; ch >= 0; --ch) {
         __syncthreads();
         lds_put(nxt);
-        if (ch - 1 >= 0) gload(out, ch - 1, nxt);
+        if (ch - 1 >= 0) gload(in, ch - 1, nxt);
         __syncthreads();
         if (valid) {
             uint16_t* row = &tile[lane * PITCH];
+            float s1 = t1, s2 = t2;
+            if (ch > 0) {
+                const float2 st = ck[(ch - 1) * n_lines + me];
+                s1 = st.x;
+                s2 = st.y;
+            }
+            causal(row, ch, s1, s2, true);
 #pragma unroll
             for (int q = CH / 8 - 1; q >= 0; --q) {
                 uint4 v = *(uint4*)(row + q * 8);
@@ -756,6 +984,8 @@ struct IirArgs {
     IIRCoeffs c;
     const float* d_off;
     int* d_err;
+    uint4* ck = nullptr;      /* saved causal states (checkpointed x, z passes) */
+    int64_t ck_lanes = 0;     /* lanes of a vector x / z launch; states of chunk k start at k * ck_lanes */
 };
 
 template <typename T>
@@ -772,11 +1002,13 @@ struct IirYVec {
     static const char* name() { return "bspline_y_vec"; }
     static __device__ __forceinline__ void run(const Args& a, int64_t t) { iir_y_vec_body<T, D>(a.in, a.out, a.y_num, a.n, a.c, a.d_off, a.d_err, t); }
 };
-struct IirYLds {
+struct IirYLdsCk {
     using Args = IirArgs<uint16_t>;
     static constexpr int THREADS = 64;
-    static const char* name() { return "bspline_y_lds"; }
-    static __device__ __forceinline__ void run(const Args& a, int64_t t) { iir_y_lds_body(a.in, a.out, a.y_num, a.n, a.c, a.d_off, a.d_err, t); }
+    static const char* name() { return "bspline_y_lds_ck"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) {
+        iir_y_lds_ck_body(a.in, a.out, a.y_num, a.n, a.c, a.d_off, a.d_err, (float2*)a.ck, t);
+    }
 };
 template <typename T>
 struct IirOuter {
@@ -793,7 +1025,18 @@ struct IirOuterVec {
     static constexpr int THREADS = 256;
     static const char* name() { return "bspline_outer_vec"; }
     static __device__ __forceinline__ void run(const Args& a, int64_t t) {
-        iir_outer_vec_body<T, D>(a.out, a.y_num, a.n, a.s, a.outer_stride, a.c, a.d_err, t % a.gx, t / a.gx);
+        iir_outer_vec_body<T, D>(a.out, a.y_num, a.n, a.s, a.outer_stride, a.c, a.ck_lanes, a.d_err, t);
+    }
+};
+
+template <typename T, int C>
+struct IirOuterCk {
+    using Args = IirArgs<T>;
+    static constexpr int THREADS = 256;
+    static constexpr int MIN_WAVES = 2;   /* two chunk buffers in registers; one wave per SIMD hides too little latency */
+    static const char* name() { return "bspline_outer_ck"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) {
+        iir_outer_ck_body<T, C>(a.out, a.y_num, a.n, a.s, a.outer_stride, a.c, a.d_err, a.ck, a.ck_lanes, t);
     }
 };
 
@@ -822,7 +1065,10 @@ void launch_iir_y(aprh_context* ctx, const T* in, T* out, int64_t y, int64_t x, 
     IirArgs<T> a{in, out, y, lines, 0, 0, 1, c, d_offset, d_err};
     if constexpr (std::is_same<T, uint16_t>::value) {
         if (y % 64 == 0 && aligned16(in) && aligned16(out)) {
-            launch_tiles<IirYLds>(ctx, a, cdiv(lines, 64));
+            /* line states at the start of chunks 1 .. y/64-1 */
+            Buf ck(ctx->pool, (size_t)std::max<int64_t>(y / 64 - 1, 1) * (size_t)lines * 8);
+            a.ck = ck.as<uint4>();
+            launch_tiles<IirYLdsCk>(ctx, a, cdiv(lines, 64));
             return;
         }
     }
@@ -833,6 +1079,10 @@ void launch_iir_y(aprh_context* ctx, const T* in, T* out, int64_t y, int64_t x, 
     }
 }
 
+/* checkpointed x / z pass: chunk length and the shortest axis it is used for */
+constexpr int OUTER_CK = 16;
+constexpr int64_t OUTER_CK_MIN = 64;
+
 template <typename T>
 void launch_iir_outer(aprh_context* ctx, T* img, int64_t y, int64_t x, int64_t z, int axis, const IIRCoeffs& c, int* d_err) {
     constexpr int V = Vec16<T>::N;
@@ -840,8 +1090,19 @@ void launch_iir_outer(aprh_context* ctx, T* img, int64_t y, int64_t x, int64_t z
     const int64_t n = axis == 1 ? x : z, s = axis == 1 ? y : x * y, outer = axis == 1 ? z : x;
     const int64_t ostride = axis == 1 ? x * y : y;
     if (y % V == 0 && aligned16(img)) {
-        const int64_t gx = cdiv(y / V, 256);
-        launch_tiles<IirOuterVec<T, 8>>(ctx, IirArgs<T>{img, img, y, n, s, ostride, gx, c, nullptr, d_err}, gx * outer);
+        const int64_t lanes = (y / V) * outer;
+        IirArgs<T> a{img, img, y, n, s, ostride, 1, c, nullptr, d_err};
+        a.ck_lanes = lanes;
+        if (n >= OUTER_CK_MIN && lanes < (int64_t(1) << 32)) {
+            const int64_t chunks = (n - 4 + OUTER_CK - 1) / OUTER_CK;
+            /* the chunk states live until the launch ends: from the pool, returned after the
+               launch is queued (stream order keeps later users behind it) */
+            Buf ck(ctx->pool, ((size_t)(chunks - 1) * 2 + V) * (size_t)lanes * 16);
+            a.ck = ck.as<uint4>();
+            launch_tiles<IirOuterCk<T, OUTER_CK>>(ctx, a, cdiv(lanes, 256));
+        } else {
+            launch_tiles<IirOuterVec<T, 8>>(ctx, a, cdiv(lanes, 256));
+        }
         return;
     }
     const int64_t gx = cdiv(y, 256);

@@ -63,14 +63,25 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t grid) {
     return x * q + min(x, r) + b / NX;
 }
 
+/* waves per SIMD a body asks the register allocator to leave room for (B::MIN_WAVES, default 1) */
+template <typename B, typename = void>
+struct min_waves {
+    static constexpr int value = 1;
+};
 template <typename B>
-__global__ __launch_bounds__(B::THREADS) void k_tiles(typename B::Args a, int64_t tiles) {
+struct min_waves<B, std::void_t<decltype(B::MIN_WAVES)>> {
+    static constexpr int value = B::MIN_WAVES;
+};
+
+template <typename B>
+__global__ __launch_bounds__(B::THREADS) __attribute__((amdgpu_waves_per_eu(min_waves<B>::value))) void k_tiles(typename B::Args a,
+                                                                                                             int64_t tiles) {
     const int64_t b = xcd_block(blockIdx.x, gridDim.x);
     for (int64_t t = b; t < tiles; t += gridDim.x) B::run(a, t);
 }
 
 template <typename B>
-__global__ __launch_bounds__(B::THREADS) void k_tiles_multi(const typename B::Args* __restrict__ a, const int64_t* __restrict__ prefix,
+__global__ __launch_bounds__(B::THREADS) __attribute__((amdgpu_waves_per_eu(min_waves<B>::value))) void k_tiles_multi(const typename B::Args* __restrict__ a, const int64_t* __restrict__ prefix,
                                                             int nv) {
     const int64_t total = prefix[nv];
     for (int64_t t = xcd_block(blockIdx.x, gridDim.x); t < total; t += gridDim.x) {

@@ -65,6 +65,24 @@ def test_bspline_axis(oracle, ctx, axis, dtype):
     assert np.array_equal(g.view(np.uint8), o.view(np.uint8))
 
 
+@pytest.mark.parametrize("n", [64, 65, 66, 67, 68, 69, 75, 83, 84, 85, 100, 260])
+@pytest.mark.parametrize("axis", [1, 2])
+@pytest.mark.parametrize("dtype", [np.uint16, np.float32])
+def test_bspline_outer_chunked(oracle, ctx, n, axis, dtype):
+    # x / z lines of 64 samples or more take the pass that saves causal states per chunk of
+    # 16 steps instead of storing the causal output: every remainder of (n - 4) mod 16
+    rng = np.random.default_rng(n + 7 * axis)
+    shape = (n, 5, 24) if axis == 2 else (3, n, 24)
+    img = (rng.random(shape) * 3000 + 100).astype(dtype)
+    g = img.copy()
+    A.bspline_filt_rec(g, axis, 3.0, 1e-4, ctx)
+    o = img.copy()
+    lib = oracle.load("oracle")
+    z, x, y = img.shape
+    (lib.orc_bspline_u16 if dtype == np.uint16 else lib.orc_bspline_f32)(o.reshape(-1), y, x, z, axis, 3.0, 1e-4)
+    assert np.array_equal(g.view(np.uint8), o.view(np.uint8))
+
+
 @pytest.mark.parametrize("shape", [(19, 21, 23), (17, 30, 26)])   # odd y: one sample per lane; even y: two
 @pytest.mark.parametrize("axis", [0, 1, 2])
 @pytest.mark.parametrize("off", [1, 2, 3, 6])
@@ -79,10 +97,20 @@ def test_sat_mean_axis(oracle, ctx, axis, off, shape):
     assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
 
 
-def test_gradient_sqrt_exact(oracle, ctx):
-    # extreme differences exercise the double sqrt + truncation
+@pytest.mark.parametrize("pattern", ["random", "checker", "ramps"])
+def test_gradient_sqrt_exact(oracle, ctx, pattern):
+    # extreme differences exercise the integer square root against the reference's double
+    # sqrt + truncation; the checkerboard of 0 / 65535 reaches the largest sum of squares
     rng = np.random.default_rng(3)
-    img = rng.integers(0, 65535, size=(8, 16, 40), dtype=np.uint16)
+    shape = (8, 16, 40)
+    if pattern == "random":
+        img = rng.integers(0, 65535, size=shape, dtype=np.uint16)
+    elif pattern == "checker":
+        zz, xx, yy = np.indices(shape)
+        img = (((zz + xx + yy) % 2) * 65535).astype(np.uint16)
+    else:
+        zz, xx, yy = np.indices(shape)
+        img = ((zz * 8191 + xx * 4093 + yy * 1637 + rng.integers(0, 3, size=shape)) % 65536).astype(np.uint16)
     g, m = A.gradient_ds(img, ctx=ctx)
     z, x, y = img.shape
     go = np.zeros_like(g)
