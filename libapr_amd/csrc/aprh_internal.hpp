@@ -209,6 +209,7 @@ struct LisParams {
     bool active[3];
     float var_rescale;
     int pad[3];
+    int pitch_align;   /* row pitch of the padded array: yp rounded up to this many floats (0 or 1: yp) */
 };
 LisParams lis_params(const aprh_params& par, int64_t yd, int64_t xd, int64_t zd);
 /* full local intensity scale over the reflect-padded array A (see lis_padded_elems) */

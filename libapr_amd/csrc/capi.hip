@@ -146,6 +146,7 @@ void lis_and_solve(aprh_context* ctx, aprh_apr* a, const aprh_params& par, Buf& 
         launch_copy(ctx, A.ptr(), mean.ptr(), nd * 4);
         launch_constant_scale(ctx, A.as<float>(), nd);
         lpa.pad[0] = lpa.pad[1] = lpa.pad[2] = 0;
+        lpa.pitch_align = 1;
         lpa.var_rescale = 1.0f;
         Ap = A.as<float>();
     } else {

@@ -297,21 +297,35 @@ struct PadGeom {
     int64_t yd, xd, zd;       /* unpadded (downsampled) dims */
     int64_t py, px, pz;       /* pads                         */
     int64_t yp, xp, zp;       /* padded dims                  */
+    int64_t yq;               /* row pitch (>= yp): the LIS array rounds it up to 32 floats so rows start on
+                                 128-byte lines and tile stores cover whole lines; unpadded arrays use yp */
 };
 
 /* y running-sum mean with offset O over padded lines (LocalIntensityScale.hpp:311-363).
    Source sample: mode 0 = padded copy of the intensity, mode 1 = |A - padded intensity|
    (calc_abs_diff, LocalIntensityScale.hpp:188-196).  O == 0 means "axis inactive": copy only.  Output written to A (may alias the source).
-   One lane per line (the recursion is sequential in y); 64-line x 64-sample tiles are
+   One lane per line (the recursion is sequential in y); 64-line x 32-sample tiles are
    staged through LDS so global loads and stores stay coalesced along y.  The per-line
    source offsets are computed once per block. */
+/* x / D for a small odd D (1..13): q0 = RN(x * RN(1/D)), the residual x - q0 * D exactly by
+   FMA, one correction step, and the sign of x for zero results.  Equal to the IEEE quotient
+   x / (float)D for every finite float x (checked over all 2^32 bit patterns for D = 1, 3, ..., 13). */
+template <int D>
+__device__ __forceinline__ float div_odd(float x) {
+    constexpr float r = 1.0f / (float)D;
+    const float q0 = x * r;
+    const float e = fmaf(-q0, (float)D, x);
+    return copysignf(fmaf(e, r, q0), x);
+}
+
 constexpr int MEANY_LINES = 64;
 template <int O>
 __device__ __forceinline__ void mean_y_body(int mode, const float* __restrict__ inten, float* A, const PadGeom& g, int64_t blk) {
     constexpr int TY = 32;          /* samples per tile                     */
     constexpr int NL = MEANY_LINES; /* lines per block, one per thread      */
     constexpr int H = 2 * O + 2;
-    __shared__ float tile[NL][TY + 1];
+    constexpr int P = TY + H + 1;   /* odd row pitch: lane-per-row accesses are conflict free */
+    __shared__ float tile[NL][P];
     __shared__ int64_t src_row[NL];   /* inten offset of the (reflected) source row of each line */
     const int lane = threadIdx.x;
     const int64_t n_lines = g.xp * g.zp;
@@ -319,6 +333,7 @@ __device__ __forceinline__ void mean_y_body(int mode, const float* __restrict__ 
     const int64_t me = line0 + lane;
     const bool valid = me < n_lines;
     const int yn = (int)g.yp, ydn = (int)g.yd, py = (int)g.py;
+    const int64_t yq = g.yq;
     const float d = (float)(2 * O + 1);
     if (valid) {
         const int64_t xp = me % g.xp, zp = me / g.xp;
@@ -332,7 +347,7 @@ __device__ __forceinline__ void mean_y_body(int mode, const float* __restrict__ 
         if (ys >= ydn) ys = 2 * (ydn - 1) - ys;
         const float b = inten[src_row[l] + ys];
         if (mode == 0) return b;
-        return fabsf(A[(line0 + l) * yn + yy] - b);
+        return fabsf(A[(line0 + l) * yq + yy] - b);
     };
     float h[H > 0 ? H : 1];
     for (int i = 0; i < H; ++i) h[i] = 0.0f;
@@ -345,7 +360,65 @@ __device__ __forceinline__ void mean_y_body(int mode, const float* __restrict__ 
             h[H - 1] = t;
         }
     }
+    /* Interior tiles (every output q has q > O and q + O < yn, every sample lies inside the
+       unpadded line) run in two phases.  A: each lane turns its line's TY samples into running
+       sums S (the only sequential dependence), stored after the H previous sums.  B: output
+       q = p0 + k is (-S(q-O-1) + S(q+O)) / d in the reference's order, all independent, computed
+       by lanes laid out along y so the stores are coalesced straight from registers. */
+    const int64_t int_lo = max(32, py - O), int_hi = min((int64_t)yn, (int64_t)py + ydn) - O - TY;
+    const int hb = lane >> 5, kk = lane & 31;
     for (int p0 = 0; p0 < yn; p0 += TY) {
+        if (O > 0 && p0 >= int_lo && p0 <= int_hi) {
+            /* loads go out in batches of LB before the first is used (lines past the end of the
+               block's range read its last line and are not stored) */
+            constexpr int LB = 16;
+            const int64_t ys0 = p0 + O - py + kk;
+            const float* arow = A + line0 * yq + p0 + O + kk;
+#pragma unroll 1
+            for (int i0 = 0; i0 < NL / 2; i0 += LB) {
+                float vv[LB];
+#pragma unroll
+                for (int i = 0; i < LB; ++i) vv[i] = inten[src_row[min(2 * (i0 + i) + hb, nl - 1)] + ys0];
+                if (mode != 0) {
+                    float aa[LB];
+#pragma unroll
+                    for (int i = 0; i < LB; ++i) aa[i] = arow[(int64_t)min(2 * (i0 + i) + hb, nl - 1) * yq];
+#pragma unroll
+                    for (int i = 0; i < LB; ++i) vv[i] = fabsf(aa[i] - vv[i]);
+                }
+#pragma unroll
+                for (int i = 0; i < LB; ++i)
+                    if (2 * (i0 + i) + hb < nl) tile[2 * (i0 + i) + hb][H + kk] = vv[i];
+            }
+            __syncthreads();
+            if (valid) {
+                float* row = tile[lane];
+                float xs[TY];
+#pragma unroll
+                for (int k = 0; k < TY; ++k) xs[k] = row[H + k];
+#pragma unroll
+                for (int m = 0; m < H; ++m) row[m] = h[m];
+#pragma unroll
+                for (int k = 0; k < TY; ++k) {
+                    t += xs[k];
+                    row[H + k] = t;
+#pragma unroll
+                    for (int i = 0; i + 1 < H; ++i) h[i] = h[i + 1];
+                    h[H - 1] = t;
+                }
+            }
+            __syncthreads();
+            float* orow = A + line0 * yq + p0 + kk;
+#pragma unroll 8
+            for (int i = 0; i < NL / 2; ++i) {
+                const int l = 2 * i + hb;
+                float v = div_odd<2 * O + 1>(-tile[l][kk + 1]);
+                v += div_odd<2 * O + 1>(tile[l][H + kk]);
+                if (l < nl) orow[(int64_t)l * yq] = v;
+            }
+            __syncthreads();
+            continue;
+        }
         /* inputs for outputs [p0, p0+TY) are samples [p0+O, p0+O+TY): NL lines x TY samples */
         for (int idx = lane; idx < NL * TY; idx += NL) {
             const int l = idx / TY, k = idx % TY, yy = p0 + O + k;
@@ -375,7 +448,7 @@ __device__ __forceinline__ void mean_y_body(int mode, const float* __restrict__ 
         __syncthreads();
         for (int idx = lane; idx < NL * TY; idx += NL) {
             const int l = idx / TY, k = idx % TY, q = p0 + k;
-            if (l < nl && q < yn - O) A[(line0 + l) * yn + q] = tile[l][k];
+            if (l < nl && q < yn - O) A[(line0 + l) * yq + q] = tile[l][k];
         }
         __syncthreads();
     }
@@ -392,7 +465,7 @@ __device__ __forceinline__ void mean_y_body(int mode, const float* __restrict__ 
             v = (float)((double)v * (1.0 / (double)(d - cnt)));
             if (q >= 1 && q <= O) v = (float)((double)v * ((double)d / ((double)(q + O) + 1.0)));
             if (q == 0) v = (float)((double)v * ((double)d / ((double)O + 1.0)));
-            A[me * yn + q] = v;
+            A[me * yq + q] = v;
         }
     }
 }
@@ -434,12 +507,16 @@ __device__ __forceinline__ void mean_outer_body(float* A, int64_t y_num, int64_t
    step j reads sample j and writes output j - O, so every sample is in registers before its
    output overwrites it */
 template <int O, int D>
-__device__ __forceinline__ void mean_outer2_body(float* A, int64_t y_num, int64_t n, int64_t s, int64_t outer_stride, int64_t bx,
-                                                 int64_t by) {
+__device__ __forceinline__ void mean_outer2_body(float* A, int64_t y_num, int64_t n, int64_t s, int64_t outer_stride, int64_t lanes,
+                                                 int64_t t) {
     constexpr int H = 2 * O + 2;
     constexpr int W = 2 * O + 1;
-    const int64_t y2 = bx * blockDim.x + threadIdx.x;
-    if (2 * y2 >= y_num) return;
+    /* lanes are numbered densely over (line, y pair), so rows whose length is not a multiple of
+       the block still fill the blocks; lane counts stay below 2^32 (the launcher checks) */
+    const int64_t lane = t * blockDim.x + threadIdx.x;
+    if (lane >= lanes) return;
+    const uint32_t ny2 = (uint32_t)(y_num / 2);
+    const uint32_t by = (uint32_t)lane / ny2, y2 = (uint32_t)lane - by * ny2;
     float2* base = (float2*)(A + by * outer_stride + 2 * y2);
     const int64_t sv = s / 2;
     float ha[H], hb[H];
@@ -470,7 +547,7 @@ __device__ __forceinline__ void mean_outer2_body(float* A, int64_t y_num, int64_
                 const float dv = (float)(i + O + 1);
                 base[i * sv] = make_float2(ca / dv, cb / dv);
             } else if (i < n - O) {
-                base[i * sv] = make_float2((ca - ha[0]) / (float)W, (cb - hb[0]) / (float)W);
+                base[i * sv] = make_float2(div_odd<W>(ca - ha[0]), div_odd<W>(cb - hb[0]));
             }
         }
     }
@@ -512,7 +589,7 @@ struct MeanOuter2 {
     static constexpr int THREADS = 256;
     static const char* name() { return "lis_mean_outer2"; }
     static __device__ __forceinline__ void run(const Args& a, int64_t t) {
-        mean_outer2_body<O, D>(a.A, a.y_num, a.n, a.s, a.outer_stride, t % a.gx, t / a.gx);
+        mean_outer2_body<O, D>(a.A, a.y_num, a.n, a.s, a.outer_stride, a.gx, t);
     }
 };
 
@@ -537,9 +614,10 @@ void mean_y_dispatch(aprh_context* ctx, int O, int mode, const float* inten, flo
 
 template <int O>
 void mean_outer(aprh_context* ctx, float* A, int64_t y_num, int64_t n, int64_t s, int64_t outer_stride, int64_t n_outer) {
-    if (y_num % 2 == 0 && ((uintptr_t)A & 7u) == 0) {
-        const int64_t gx = cdiv(y_num / 2, 256);
-        launch_tiles<MeanOuter2<O, 8>>(ctx, {A, y_num, n, s, outer_stride, gx}, gx * n_outer);
+    const int64_t lanes2 = y_num / 2 * n_outer;
+    if (y_num % 2 == 0 && s % 2 == 0 && outer_stride % 2 == 0 && ((uintptr_t)A & 7u) == 0 && lanes2 < (int64_t(1) << 32)) {
+        /* the gx field carries the lane count of the dense numbering */
+        launch_tiles<MeanOuter2<O, 8>>(ctx, {A, y_num, n, s, outer_stride, lanes2}, cdiv(lanes2, 256));
     } else {
         const int64_t gx = cdiv(y_num, 256);
         launch_tiles<MeanOuter<O>>(ctx, {A, y_num, n, s, outer_stride, gx}, gx * n_outer);
@@ -614,7 +692,7 @@ __device__ __forceinline__ void levels_body(const LevelsArgs<G>& a, int64_t t) {
             z = r / g.xd;
         }
         /* rescale_var + unpad (LocalIntensityScale.hpp:173-184, PixelData.hpp:1189-1210) */
-        float s = A[((z + g.pz) * g.xp + (x + g.px)) * g.yp + (y + g.py)] * var_rescale;
+        float s = A[((z + g.pz) * g.xp + (x + g.px)) * g.yq + (y + g.py)] * var_rescale;
         if (sigma_raw) sigma_raw[i] = s;
         G gv = grad[i];
         /* threshold_gradient (ComputeGradient.hpp:159-171) */
@@ -637,7 +715,7 @@ __device__ __forceinline__ void unpad_rescale_body(const float* __restrict__ A, 
         const int64_t y = (int64_t)(i % (size_t)g.yd);
         const int64_t r = (int64_t)(i / (size_t)g.yd);
         const int64_t x = r % g.xd, z = r / g.xd;
-        out[i] = A[((z + g.pz) * g.xp + (x + g.px)) * g.yp + (y + g.py)] * var_rescale;
+        out[i] = A[((z + g.pz) * g.xp + (x + g.px)) * g.yq + (y + g.py)] * var_rescale;
     }
 }
 
@@ -761,6 +839,8 @@ PadGeom pad_geom(int64_t yd, int64_t xd, int64_t zd, const LisParams& lp) {
     g.yd = yd; g.xd = xd; g.zd = zd;
     g.py = lp.pad[0]; g.px = lp.pad[1]; g.pz = lp.pad[2];
     g.yp = yd + 2 * g.py; g.xp = xd + 2 * g.px; g.zp = zd + 2 * g.pz;
+    const int64_t a = std::max(1, lp.pitch_align);
+    g.yq = (g.yp + a - 1) / a * a;
     return g;
 }
 
@@ -834,6 +914,7 @@ LisParams lis_params(const aprh_params& par, int64_t yd, int64_t xd, int64_t zd)
         lp.pad[a] = dims[a] > 1 ? (int)std::min<int64_t>(p, dims[a] - 1) : 0;
     }
     lp.var_rescale = (float)(nact == 3 ? r3[psf_ind] : nact == 2 ? r2[psf_ind] : r1[psf_ind]);
+    lp.pitch_align = 32;   /* 128-byte rows for the padded array launch_lis fills */
     return lp;
 }
 
@@ -844,7 +925,7 @@ void launch_sat_mean(aprh_context* ctx, float* m, int64_t y, int64_t x, int64_t 
         (void)g;
         /* y mean of an unpadded array: treat it as a padded array with zero pads whose source is itself */
         PadGeom gg;
-        gg.yd = y; gg.xd = x; gg.zd = z; gg.py = gg.px = gg.pz = 0; gg.yp = y; gg.xp = x; gg.zp = z;
+        gg.yd = y; gg.xd = x; gg.zd = z; gg.py = gg.px = gg.pz = 0; gg.yp = y; gg.xp = x; gg.zp = z; gg.yq = y;
         if (y < 2 * off + 2) throw Error(APRH_ERR_UNSUPPORTED, "mean window wider than the axis");
         mean_y_dispatch(ctx, off, 0, m, m, gg);
     } else if (axis == 1) {
@@ -860,12 +941,15 @@ void launch_lis(aprh_context* ctx, const float* intensity, float* sigma_padded, 
     float* A = sigma_padded;
     /* first mean (windows win[0..2]) of the padded intensity */
     mean_y_dispatch(ctx, lp.active[0] ? lp.win[0] : 0, 0, intensity, A, g);
-    if (lp.active[1]) mean_x(ctx, lp.win[1], A, g.yp, g.xp, g.zp);
-    if (lp.active[2]) mean_z(ctx, lp.win[2], A, g.yp, g.xp, g.zp);
+    /* x / z passes over the yp valid columns of rows with pitch yq */
+    auto mx = [&](int O) { mean_outer_dispatch(ctx, O, A, g.yp, g.xp, g.yq, g.xp * g.yq, g.zp); };
+    auto mz = [&](int O) { mean_outer_dispatch(ctx, O, A, g.yp, g.zp, g.xp * g.yq, g.yq, g.xp); };
+    if (lp.active[1]) mx(lp.win[1]);
+    if (lp.active[2]) mz(lp.win[2]);
     /* |mean - intensity| then the second mean (windows win[3..5]) */
     mean_y_dispatch(ctx, lp.active[0] ? lp.win[3] : 0, 1, intensity, A, g);
-    if (lp.active[1]) mean_x(ctx, lp.win[4], A, g.yp, g.xp, g.zp);
-    if (lp.active[2]) mean_z(ctx, lp.win[5], A, g.yp, g.xp, g.zp);
+    if (lp.active[1]) mx(lp.win[4]);
+    if (lp.active[2]) mz(lp.win[5]);
 }
 
 template <typename G>
@@ -932,7 +1016,7 @@ void launch_constant_scale(aprh_context* ctx, float* m, size_t n) {
 }
 
 size_t lis_padded_elems(int64_t yd, int64_t xd, int64_t zd, const LisParams& lp) {
-    return (size_t)((yd + 2 * lp.pad[0]) * (xd + 2 * lp.pad[1]) * (zd + 2 * lp.pad[2]));
+    return (size_t)(pad_geom(yd, xd, zd, lp).yq * (xd + 2 * lp.pad[1]) * (zd + 2 * lp.pad[2]));
 }
 
 }  // namespace aprh
