@@ -1,0 +1,109 @@
+/*
+ * Host checks of the exact-arithmetic shortcuts used in the HIP kernels, restated in C with the
+ * same operations (compiled with -ffp-contract=off; fmaf is a single rounding on x86-64 as on
+ * gfx950):
+ *   div_odd<D>(x)  = copysign(fma(fma(-q0, D, x), r, q0), x), q0 = x * r, r = RN(1/D)
+ *                    must equal x / (float)D for every finite x, D = 1, 3, ..., 13
+ *                    (k_grad_lis.hip, running means of the local intensity scale);
+ *   isqrt_d2(a,b,c) = floor(sqrt(a + b + c)) from a float estimate and one integer
+ *                    correction, for squares of differences of u16 values
+ *                    (k_grad_lis.hip, gradient magnitude), against the reference's
+ *                    trunc(sqrt(double)) form.
+ * Usage: arith_check [stride]   -- tests every stride-th float bit pattern (1 = all 2^32).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+static float div_odd(float x, float d, float r) {
+    const float q0 = x * r;
+    const float e = fmaf(-q0, d, x);
+    return copysignf(fmaf(e, r, q0), x);
+}
+
+static uint32_t isqrt_d2(uint32_t a, uint32_t b, uint32_t c) {
+    const uint64_t k = (uint64_t)a + b + c;
+    uint32_t s = (uint32_t)sqrtf(((float)a + (float)c) + (float)b);
+    if ((uint64_t)s * s > k) --s;
+    else if ((uint64_t)(s + 1) * (s + 1) <= k) ++s;
+    return s;
+}
+
+static uint64_t rng = 88172645463325252ull;
+static uint32_t next32(void) {
+    rng ^= rng << 13;
+    rng ^= rng >> 7;
+    rng ^= rng << 17;
+    return (uint32_t)(rng >> 16);
+}
+
+int main(int argc, char** argv) {
+    const uint64_t stride = argc > 1 ? strtoull(argv[1], NULL, 10) : 97;
+    uint64_t bad = 0, n = 0;
+    for (int D = 1; D <= 13; D += 2) {
+        const float d = (float)D, r = 1.0f / d;
+        for (uint64_t u = 0; u < (1ull << 32); u += stride) {
+            const uint32_t bits = (uint32_t)u;
+            float x;
+            memcpy(&x, &bits, 4);
+            if (!isfinite(x)) continue;
+            const float q = x / d, g = div_odd(x, d, r);
+            ++n;
+            if (memcmp(&q, &g, 4) != 0) {
+                if (bad < 5) printf("div_odd D=%d x=%a: %a != %a\n", D, x, g, q);
+                ++bad;
+            }
+        }
+    }
+    /* signed zeros, the smallest and largest magnitudes */
+    const uint32_t special[8] = {0x00000000u, 0x80000000u, 0x00000001u, 0x80000001u, 0x00800000u, 0x80800000u, 0x7F7FFFFFu, 0xFF7FFFFFu};
+    for (int D = 1; D <= 13; D += 2)
+        for (int i = 0; i < 8; ++i) {
+            float x;
+            memcpy(&x, &special[i], 4);
+            const float q = x / (float)D, g = div_odd(x, (float)D, 1.0f / (float)D);
+            ++n;
+            if (memcmp(&q, &g, 4) != 0) {
+                if (bad < 5) printf("div_odd D=%d x=%a: %a != %a\n", D, x, g, q);
+                ++bad;
+            }
+        }
+    printf("div_odd: %llu values, %llu mismatches\n", (unsigned long long)n, (unsigned long long)bad);
+    uint64_t bad2 = 0, n2 = 0;
+    for (uint64_t i = 0; i < 20000000ull; ++i) {
+        int dx = (int)(next32() % 131071u) - 65535, dy = (int)(next32() % 131071u) - 65535, dz = (int)(next32() % 131071u) - 65535;
+        if (i & 1) {
+            dx >>= next32() % 17;
+            dy >>= next32() % 17;
+            dz >>= next32() % 17;
+        }
+        if (i % 7 == 0) {   /* near perfect squares */
+            dy = (int)(next32() % 4) - 2;
+            dz = (int)(next32() % 4);
+        }
+        const uint32_t ux = (uint32_t)dx, uy = (uint32_t)dy, uz = (uint32_t)dz;
+        const uint32_t g = isqrt_d2(ux * ux, uy * uy, uz * uz) >> 1;
+        const double q = (double)dx * dx / 4.0 + (double)dz * dz / 4.0 + (double)dy * dy / 4.0;
+        const uint32_t ref = (uint32_t)sqrt(q);
+        ++n2;
+        if (g != ref) {
+            if (bad2 < 5) printf("isqrt_d2 %d %d %d: %u != %u\n", dx, dy, dz, g, ref);
+            ++bad2;
+        }
+    }
+    /* the extremes: every square sum of three u16 differences from 0 to 65535 */
+    const int ext[4] = {0, 1, 65534, 65535};
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b)
+            for (int c = 0; c < 4; ++c) {
+                const uint32_t ux = (uint32_t)ext[a], uy = (uint32_t)ext[b], uz = (uint32_t)ext[c];
+                const uint32_t g = isqrt_d2(ux * ux, uy * uy, uz * uz) >> 1;
+                const double q = (double)ext[a] * ext[a] / 4.0 + (double)ext[c] * ext[c] / 4.0 + (double)ext[b] * ext[b] / 4.0;
+                ++n2;
+                if (g != (uint32_t)sqrt(q)) ++bad2;
+            }
+    printf("isqrt_d2: %llu values, %llu mismatches\n", (unsigned long long)n2, (unsigned long long)bad2);
+    return (bad || bad2) ? 1 : 0;
+}
