@@ -204,6 +204,8 @@ void launch_grad_ds_u16(aprh_context* ctx, const uint16_t* img, uint16_t* grad, 
 void launch_inv_bspline(aprh_context* ctx, float* m, float* tmp, int64_t y, int64_t x, int64_t z, bool ay, bool ax, bool az);
 /* the same passes alternating between a (input) and b; returns the buffer holding the result */
 float* launch_inv_bspline_pp(aprh_context* ctx, float* a, float* b, int64_t y, int64_t x, int64_t z, bool ay, bool ax, bool az);
+/* number of a -> b launches launch_inv_bspline_pp makes (its result is in b when odd) */
+int inv_bspline_launches(bool ay, bool ax, bool az);
 struct LisParams {
     int win[6];
     bool active[3];

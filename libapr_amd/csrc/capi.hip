@@ -98,9 +98,9 @@ void get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x,
     mark(ctx, ST_GRADIENT);
     /* gradient magnitude + 2x2x2 max (ComputeGradient.hpp:92-93), mean downsample and inverse B-spline (:95-118) */
     Buf grad(ctx->pool, nd * 2);
-    /* the inverse B-spline ping-pongs between mean and tmp: the gradient kernel writes the
-       downsampled mean where an odd number of passes ends in mean (no copy back) */
-    const int npass = par.lambda > 0 ? (int)(y > 2) + (int)(x > 2) + (int)(z > 2) : 0;
+    /* the inverse B-spline runs tmp -> mean: the gradient kernel writes the downsampled mean
+       where an odd number of launches ends in mean (no copy back) */
+    const int npass = par.lambda > 0 ? inv_bspline_launches(y > 2, x > 2, z > 2) : 0;
     Buf mean(ctx->pool, nd * 4), tmp;
     if (npass) tmp.reset(ctx->pool, nd * 4);
     float* mean_first = (npass & 1) ? tmp.as<float>() : mean.as<float>();
@@ -197,7 +197,7 @@ void get_apr_f32(aprh_context* ctx, const float* image, int64_t y, int64_t x, in
     if (par.lambda > 0 && z > 2) launch_iir_outer<float>(ctx, I, y, x, z, 2, iir_coeffs(par.lambda, tol, z), d_err + 2);
     mark(ctx, ST_GRADIENT);
     Buf grad(ctx->pool, nd * 4);
-    const int npass = par.lambda > 0 ? (int)(y > 2) + (int)(x > 2) + (int)(z > 2) : 0;
+    const int npass = par.lambda > 0 ? inv_bspline_launches(y > 2, x > 2, z > 2) : 0;
     Buf mean(ctx->pool, nd * 4), tmp;
     if (npass) tmp.reset(ctx->pool, nd * 4);
     float* mean_first = (npass & 1) ? tmp.as<float>() : mean.as<float>();

@@ -5,7 +5,8 @@
  *  - k_grad_ds: one lane per downsampled cell; gradient magnitude of its
  *    2x2x2 voxels (ComputeGradient.hpp:975-1051, double squares/sqrt like the
  *    reference) fused with the 2x2x2 mean downsample (ComputeGradient.hpp:95-98).
- *  - inverse B-spline: one pass per axis, out of place (ComputeGradient.hpp:784-964).
+ *  - inverse B-spline: all active axes in one z-marching sweep, out of place
+ *    (ComputeGradient.hpp:784-964).
  *  - local intensity scale (LocalIntensityScale.hpp:26-494): the six
  *    separable running-sum means over the reflect-padded array.  The y mean
  *    is a per-line sequential prefix sum, so lines are staged through an LDS
@@ -239,51 +240,151 @@ __device__ __forceinline__ void grad_ds_vec_body(const uint16_t* __restrict__ im
     }
 }
 
-/* ---------------- inverse cubic B-spline, [1 4 1]/6 ---------------- */
-__device__ __forceinline__ void inv_y_body(const float* __restrict__ in, float* out, int64_t y_num, size_t n, int64_t G, int64_t t,
-                                           const FastDiv& fy, bool i32) {
+/* ---------------- inverse cubic B-spline, [1 4 1]/6 (ComputeGradient.hpp:784-964) ---------------- */
+/* All active passes of the inverse B-spline in one sweep.  A block owns a 64 (y) x 16 (x) tile
+ * and marches along a run of INV_ZC z planes: each plane is loaded once with a one-sample halo
+ * into LDS, the y pass is applied to the 18 rows of the tile (the halo rows' y outputs are the
+ * x neighbours), then the x pass, and the x-y result of three consecutive planes gives the z
+ * pass of the middle one.  Each value is formed by the same operations in the same order as the
+ * reference's separate passes (y: ComputeGradient.hpp:785-830; z: 832-899; x: 901-964; boundary
+ * samples with their own two-term forms), so the result is identical to running them one after
+ * another. */
+constexpr int INV_TY = 64, INV_TX = 16, INV_ZC = 16;
+
+__device__ __forceinline__ float inv_y_at(float l, float c, float r, int64_t k, int64_t n) {
     const float a1 = (float)(1.0 / 6.0), a2 = (float)(4.0 / 6.0), a3 = (float)(1.0 / 6.0);
-    for (size_t i = (size_t)t * blockDim.x + threadIdx.x; i < n; i += (size_t)G * blockDim.x) {
-        const int64_t k = i32 ? (int64_t)((uint32_t)i - fy.div((uint32_t)i) * (uint32_t)y_num) : (int64_t)(i % (size_t)y_num);
-        const float* l = in + (i - (size_t)k);
-        float v;
-        if (k == 0) {
-            v = a2 * l[0];
-            v += (a1 + a3) * l[1];
-        } else if (k == y_num - 1) {
-            v = (a1 + a3) * l[y_num - 2];
-            v += a2 * l[y_num - 1];
-        } else {
-            v = a1 * l[k - 1] + a2 * l[k] + a3 * l[k + 1];
+    float v;
+    if (k == 0) {
+        v = a2 * c;
+        v += (a1 + a3) * r;
+    } else if (k == n - 1) {
+        v = (a1 + a3) * l;
+        v += a2 * c;
+    } else {
+        v = a1 * l + a2 * c + a3 * r;
+    }
+    return v;
+}
+__device__ __forceinline__ float inv_outer_at(float l, float c, float r, int64_t j, int64_t n) {
+    const float a1 = (float)(1.0 / 6.0), a2 = (float)(4.0 / 6.0), a3 = (float)(1.0 / 6.0);
+    float v;
+    if (j == n - 1) {
+        v = (a1 + a3) * l;
+        v += a2 * c;
+    } else if (j == 0) {
+        v = a1 * r + a2 * c + a3 * r;
+    } else {
+        v = a1 * l + a2 * c + a3 * r;
+    }
+    return v;
+}
+
+struct InvFusedArgs {
+    const float* in;
+    float* out;
+    int64_t y, x, z;
+    int ay, ax, az;
+    int64_t nty, ntx;   /* tiles along y and x */
+};
+
+__device__ __forceinline__ void inv_fused_body(const InvFusedArgs& a, int64_t t) {
+    constexpr int RW = INV_TY + 2, NR = INV_TX + 2;   /* halo'd row width (y) and row count (x) */
+    constexpr int NLD = (RW * NR + 255) / 256;
+    __shared__ float s_in[NR][RW + 2];
+    __shared__ float s_y[NR][INV_TY + 1];
+    const int64_t Y = a.y, X = a.x, Z = a.z;
+    const int64_t ty = t % a.nty, r0 = t / a.nty, tx = r0 % a.ntx, tz = r0 / a.ntx;
+    const int64_t y0 = ty * INV_TY, x0 = tx * INV_TX, zs = tz * INV_ZC, ze = min(zs + INV_ZC, Z);
+    const int ly = threadIdx.x & 63, lx = threadIdx.x >> 6;
+    const int64_t yy = y0 + ly;
+    /* the plane's halo'd tile, loaded cooperatively into registers */
+    auto gload = [&](int64_t zz, float* v) {
+#pragma unroll
+        for (int i = 0; i < NLD; ++i) {
+            const int idx = threadIdx.x + 256 * i;
+            const int r = idx / RW, c = idx - r * RW;
+            const int64_t gx = x0 - 1 + r, gy = y0 - 1 + c;
+            v[i] = (r < NR && gx >= 0 && gx < X && gy >= 0 && gy < Y) ? a.in[(zz * X + gx) * Y + gy] : 0.0f;
         }
-        out[i] = v;
+    };
+    auto lput = [&](const float* v) {
+#pragma unroll
+        for (int i = 0; i < NLD; ++i) {
+            const int idx = threadIdx.x + 256 * i;
+            const int r = idx / RW, c = idx - r * RW;
+            if (r < NR) s_in[r][c] = v[i];
+        }
+    };
+    /* x-y result of one plane for this thread's four x columns */
+    auto plane_xy = [&](float* o) {
+        __syncthreads();   /* s_in written */
+        if (yy < Y) {
+            for (int r = lx; r < NR; r += 4) {
+                const int64_t gx = x0 - 1 + r;
+                if (gx < 0 || gx >= X) continue;
+                const float c = s_in[r][ly + 1];
+                s_y[r][ly] = a.ay ? inv_y_at(s_in[r][ly], c, s_in[r][ly + 2], yy, Y) : c;
+            }
+        }
+        __syncthreads();   /* s_y written; s_in free */
+#pragma unroll
+        for (int cI = 0; cI < 4; ++cI) {
+            const int r = 1 + lx * 4 + cI;
+            const int64_t gx = x0 - 1 + r;
+            float v = 0.0f;
+            if (yy < Y && gx < X) v = a.ax ? inv_outer_at(s_y[r - 1][ly], s_y[r][ly], s_y[r + 1][ly], gx, X) : s_y[r][ly];
+            o[cI] = v;
+        }
+    };
+    auto store = [&](int64_t zz, const float* v) {
+        if (yy >= Y) return;
+#pragma unroll
+        for (int cI = 0; cI < 4; ++cI) {
+            const int64_t gx = x0 + lx * 4 + cI;
+            if (gx < X) a.out[(zz * X + gx) * Y + yy] = v[cI];
+        }
+    };
+    const int64_t zf = a.az ? max(zs - 1, (int64_t)0) : zs, zl = a.az ? min(ze, Z - 1) : ze - 1;
+    float pm[4] = {0, 0, 0, 0}, pc[4] = {0, 0, 0, 0}, pn[4];
+    float cur[NLD], nxt[NLD];
+    gload(zf, cur);
+    for (int64_t zz = zf; zz <= zl; ++zz) {
+        if (zz < zl) gload(zz + 1, nxt);
+        lput(cur);
+        plane_xy(pn);
+        if (!a.az) {
+            store(zz, pn);
+        } else if (zz - 1 >= zs && zz - 1 < ze) {
+            /* z pass of plane zz - 1 from planes zz - 2 (if any), zz - 1, zz */
+            float o[4];
+#pragma unroll
+            for (int cI = 0; cI < 4; ++cI) o[cI] = inv_outer_at(pm[cI], pc[cI], pn[cI], zz - 1, Z);
+            store(zz - 1, o);
+        }
+#pragma unroll
+        for (int cI = 0; cI < 4; ++cI) {
+            pm[cI] = pc[cI];
+            pc[cI] = pn[cI];
+        }
+#pragma unroll
+        for (int i = 0; i < NLD; ++i) cur[i] = nxt[i];
+        __syncthreads();   /* s_y reads done before the next plane rewrites it */
+    }
+    /* the last plane of the volume has no successor */
+    if (a.az && zl == Z - 1 && Z - 1 >= zs && Z - 1 < ze) {
+        float o[4];
+#pragma unroll
+        for (int cI = 0; cI < 4; ++cI) o[cI] = inv_outer_at(pm[cI], pc[cI], 0.0f, Z - 1, Z);
+        store(Z - 1, o);
     }
 }
 
-/* x / z: out[0] = a1*in[1] + a2*in[0] + a3*in[1]; last = (a1+a3)*in[n-2] + a2*in[n-1] */
-__device__ __forceinline__ void inv_outer_body(const float* __restrict__ in, float* out, int64_t n_axis, int64_t stride, size_t n,
-                                               int64_t G, int64_t t, const FastDiv& fs, const FastDiv& fa, bool i32) {
-    const float a1 = (float)(1.0 / 6.0), a2 = (float)(4.0 / 6.0), a3 = (float)(1.0 / 6.0);
-    for (size_t i = (size_t)t * blockDim.x + threadIdx.x; i < n; i += (size_t)G * blockDim.x) {
-        int64_t j;
-        if (i32) {
-            const uint32_t q = fs.div((uint32_t)i);
-            j = (int64_t)(q - fa.div(q) * (uint32_t)n_axis);
-        } else {
-            j = (int64_t)((i / (size_t)stride) % (size_t)n_axis);
-        }
-        float v;
-        if (j == n_axis - 1) {
-            v = (a1 + a3) * in[i - stride];
-            v += a2 * in[i];
-        } else if (j == 0) {
-            v = a1 * in[i + stride] + a2 * in[i] + a3 * in[i + stride];
-        } else {
-            v = a1 * in[i - stride] + a2 * in[i] + a3 * in[i + stride];
-        }
-        out[i] = v;
-    }
-}
+struct InvFused {
+    using Args = InvFusedArgs;
+    static constexpr int THREADS = 256;
+    static const char* name() { return "inv_bspline_fused"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) { inv_fused_body(a, t); }
+};
 
 /* ---------------- local intensity scale ---------------- */
 __device__ __forceinline__ int64_t refl(int64_t i, int64_t n) {
@@ -770,29 +871,6 @@ struct GradDsVec {
                                a.h2z, a.accum, bx, r % a.xd_num, r / a.xd_num, a.xd_num, a.zd_num);
     }
 };
-struct InvArgs {
-    const float* in;
-    float* out;
-    int64_t n_axis, stride;
-    size_t n;
-    int64_t G;
-    FastDiv f_stride, f_axis;   /* valid when i32 */
-    int i32;
-};
-struct InvY {
-    using Args = InvArgs;
-    static constexpr int THREADS = 256;
-    static const char* name() { return "inv_bspline_y"; }
-    static __device__ __forceinline__ void run(const Args& a, int64_t t) { inv_y_body(a.in, a.out, a.n_axis, a.n, a.G, t, a.f_axis, a.i32); }
-};
-struct InvOuter {
-    using Args = InvArgs;
-    static constexpr int THREADS = 256;
-    static const char* name() { return "inv_bspline_outer"; }
-    static __device__ __forceinline__ void run(const Args& a, int64_t t) {
-        inv_outer_body(a.in, a.out, a.n_axis, a.stride, a.n, a.G, t, a.f_stride, a.f_axis, a.i32);
-    }
-};
 template <typename G>
 struct Levels {
     using Args = LevelsArgs<G>;
@@ -864,30 +942,21 @@ void launch_grad_ds_u16(aprh_context* ctx, const uint16_t* img, uint16_t* grad, 
     }
 }
 
+int inv_bspline_launches(bool ay, bool ax, bool az) { return (ay || ax || az) ? 1 : 0; }
+
 float* launch_inv_bspline_pp(aprh_context* ctx, float* a, float* b, int64_t y, int64_t x, int64_t z, bool ay, bool ax, bool az) {
-    const size_t n = (size_t)(y * x * z);
-    float* src = a;
-    float* dst = b;
-    const int64_t G = stride_tiles((int64_t)n, 256 * 4);
-    const int i32 = fits32((int64_t)n) ? 1 : 0;
-    auto fd = [&](int64_t d) { return i32 ? FastDiv((uint64_t)d) : FastDiv(); };
-    if (ay) {
-        launch_tiles<InvY>(ctx, {src, dst, y, 1, n, G, FastDiv(), fd(y), i32}, G);
-        std::swap(src, dst);
+    if (ay || ax || az) {
+        /* one sweep for all active axes: a -> b */
+        if ((ay && y < 2) || (ax && x < 2) || (az && z < 2)) throw Error(APRH_ERR_UNSUPPORTED, "inverse B-spline over an axis shorter than 2");
+        const int64_t nty = cdiv(y, INV_TY), ntx = cdiv(x, INV_TX), ntz = cdiv(z, INV_ZC);
+        launch_tiles<InvFused>(ctx, InvFusedArgs{a, b, y, x, z, ay ? 1 : 0, ax ? 1 : 0, az ? 1 : 0, nty, ntx}, nty * ntx * ntz);
+        return b;
     }
-    if (ax) {
-        launch_tiles<InvOuter>(ctx, {src, dst, x, y, n, G, fd(y), fd(x), i32}, G);
-        std::swap(src, dst);
-    }
-    if (az) {
-        launch_tiles<InvOuter>(ctx, {src, dst, z, x * y, n, G, fd(x * y), fd(z), i32}, G);
-        std::swap(src, dst);
-    }
-    return src;
+    return a;
 }
 
 void launch_inv_bspline(aprh_context* ctx, float* m, float* tmp, int64_t y, int64_t x, int64_t z, bool ay, bool ax, bool az) {
-    /* ping-pong m -> tmp -> m ...; result always ends in m */
+    /* m -> tmp, copied back so the result ends in m */
     float* fin = launch_inv_bspline_pp(ctx, m, tmp, y, x, z, ay, ax, az);
     if (fin != m) launch_copy(ctx, m, fin, (size_t)(y * x * z) * 4);
 }
