@@ -193,6 +193,11 @@ void launch_offset_copy(aprh_context* ctx, const uint16_t* in, uint16_t* out, si
 template <typename T>
 void launch_iir_y(aprh_context* ctx, const T* in, T* out, int64_t y, int64_t x, int64_t z, const IIRCoeffs& c,
                   const float* d_offset, int* d_err);
+/* u16 y pass that also finds the bspline offset: it runs with the offset most images get, gathers
+   the input's (min, max) on the way, and runs again only when the true offset differs */
+bool iir_y_fuses_offset(int64_t y, const void* in, const void* out);
+void launch_iir_y_offset(aprh_context* ctx, const uint16_t* in, uint16_t* out, int64_t y, int64_t x, int64_t z, const IIRCoeffs& c,
+                         float lambda, float* d_offset, uint32_t* d_mm, int* d_rerun, int* d_err);
 template <typename T>
 void launch_iir_outer(aprh_context* ctx, T* img, int64_t y, int64_t x, int64_t z, int axis, const IIRCoeffs& c, int* d_err);
 

@@ -78,18 +78,26 @@ void get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x,
     float* d_off = (float*)flags;
     uint32_t* d_mm = (uint32_t*)flags + 2;
     int* d_err = flags + 4;
+    int* d_rerun = flags + 8;
     launch_zero(ctx, d_err, 3 * sizeof(int));
-    /* computeL: bspline offset (APRConverter.hpp:213-224) */
-    launch_minmax_offset(ctx, (const uint16_t*)in.ptr, n, par.lambda, d_off, d_mm);
-    mark(ctx, ST_SMOOTH_Y);
-    /* smoothing (ComputeGradient.hpp:173-205) with the offset copy fused into the first pass */
     Buf img(ctx->pool, n * 2);
     uint16_t* I = img.as<uint16_t>();
     const float tol = 0.0001f;
-    if (par.lambda > 0 && y > 2) {
-        launch_iir_y<uint16_t>(ctx, (const uint16_t*)in.ptr, I, y, x, z, iir_coeffs(par.lambda, tol, y), d_off, d_err + 0);
+    if (par.lambda > 0 && y > 2 && iir_y_fuses_offset(y, in.ptr, I)) {
+        /* computeL's bspline offset (APRConverter.hpp:213-224) found by the y pass itself */
+        mark(ctx, ST_SMOOTH_Y);
+        launch_iir_y_offset(ctx, (const uint16_t*)in.ptr, I, y, x, z, iir_coeffs(par.lambda, tol, y), par.lambda, d_off, d_mm, d_rerun,
+                            d_err + 0);
     } else {
-        launch_offset_copy(ctx, (const uint16_t*)in.ptr, I, n, d_off);
+        /* computeL: bspline offset (APRConverter.hpp:213-224) */
+        launch_minmax_offset(ctx, (const uint16_t*)in.ptr, n, par.lambda, d_off, d_mm);
+        mark(ctx, ST_SMOOTH_Y);
+        /* smoothing (ComputeGradient.hpp:173-205) with the offset copy fused into the first pass */
+        if (par.lambda > 0 && y > 2) {
+            launch_iir_y<uint16_t>(ctx, (const uint16_t*)in.ptr, I, y, x, z, iir_coeffs(par.lambda, tol, y), d_off, d_err + 0);
+        } else {
+            launch_offset_copy(ctx, (const uint16_t*)in.ptr, I, n, d_off);
+        }
     }
     mark(ctx, ST_SMOOTH_X);
     if (par.lambda > 0 && x > 2) launch_iir_outer<uint16_t>(ctx, I, y, x, z, 1, iir_coeffs(par.lambda, tol, x), d_err + 1);

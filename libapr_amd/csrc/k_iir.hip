@@ -168,6 +168,54 @@ struct Offset {
     }
 };
 
+/* offset speculation for the fused y pass: before it, the min/max words are reset, the offset
+   is set to the value every image with max + min <= 65335 gets (100 for lambda > 0) and the
+   re-run flag is cleared */
+struct OffsetGuess {
+    struct Args {
+        uint32_t* mm;
+        float lambda;
+        float* off;
+        int* rerun;
+    };
+    static constexpr int THREADS = 64;
+    static const char* name() { return "bspline_offset_guess"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t) {
+        if (threadIdx.x != 0) return;
+        a.mm[0] = 65535u;
+        a.mm[1] = 0u;
+        *a.off = a.lambda > 0.0f ? 100.0f : 0.0f;
+        *a.rerun = 0;
+    }
+};
+/* after the y pass: the offset from the min/max it gathered (computeL, AutoParameters.hpp:185-196);
+   when the guess was wrong the offset is replaced, the y pass's overflow word cleared and the
+   re-run flag set */
+struct OffsetVerify {
+    struct Args {
+        const uint32_t* mm;
+        float lambda;
+        float* off;
+        int* rerun;
+        int* err_y;
+    };
+    static constexpr int THREADS = 64;
+    static const char* name() { return "bspline_offset_verify"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t) {
+        if (threadIdx.x != 0) return;
+        float o = 0.0f;
+        if (a.lambda > 0.0f) {
+            const int mn = (int)a.mm[0], mx = (int)a.mm[1];
+            o = fmaxf(fminf((float)((65535 - (mx - mn)) / 2 - mn), 100.0f), -100.0f);
+        }
+        if (o != *a.off) {
+            *a.off = o;
+            *a.err_y = 0;
+            *a.rerun = 1;
+        }
+    }
+};
+
 struct OffsetCopy {
     struct Args {
         const uint16_t* in;
@@ -814,10 +862,12 @@ __device__ __forceinline__ void iir_outer_ck_body(T* img, int64_t y_num, int64_t
  * write instead of 2 reads and 2 writes (plus 8 bytes of state per 64 samples).  In place is
  * safe: chunk ch is written after chunk ch - 1 has been loaded. */
 __device__ __forceinline__ void iir_y_lds_ck_body(const uint16_t* in, uint16_t* out, int64_t y_num, int64_t n_lines, const IIRCoeffs& c,
-                                                  const float* d_off, int* d_err, float2* __restrict__ ck, int64_t blk) {
+                                                  const float* d_off, int* d_err, float2* __restrict__ ck, uint32_t* mm,
+                                                  const int* run_if, int64_t blk) {
     constexpr int CH = 64;
     constexpr int PITCH = 72;
     __shared__ __attribute__((aligned(16))) uint16_t tile[64 * PITCH];
+    if (run_if && *run_if == 0) return;
     const int lane = threadIdx.x;
     const int64_t line0 = blk * 64;
     const int64_t me = line0 + lane;
@@ -907,19 +957,47 @@ __device__ __forceinline__ void iir_y_lds_ck_body(const uint16_t* in, uint16_t* 
         }
     };
 
-    /* causal sweep: states only */
+    /* causal sweep: states only; with mm, also the (min, max) of the raw samples (packed u16
+       min / max over the loaded vectors; lines past the end load zeros, so they are skipped) */
     {
         float s1 = t1, s2 = t2;
         uint4 nxt[8];
+        typedef unsigned short us2 __attribute__((ext_vector_type(2)));
+        us2 lo = {65535, 65535}, hi = {0, 0};
         gload(in, 0, nxt);
         for (int64_t ch = 0; ch < nch; ++ch) {
             __syncthreads();
+            if (mm) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    if (line0 + 8 * i + (lane >> 3) >= n_lines) continue;
+                    const uint32_t w[4] = {nxt[i].x, nxt[i].y, nxt[i].z, nxt[i].w};
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        us2 v;
+                        __builtin_memcpy(&v, &w[q], 4);
+                        lo = __builtin_elementwise_min(lo, v);
+                        hi = __builtin_elementwise_max(hi, v);
+                    }
+                }
+            }
             lds_put(nxt);
             if (ch + 1 < nch) gload(in, ch + 1, nxt);
             __syncthreads();
             if (valid) {
                 if (ch > 0) ck[(ch - 1) * n_lines + me] = make_float2(s1, s2);
                 causal(&tile[lane * PITCH], ch, s1, s2, false);
+            }
+        }
+        if (mm) {
+            uint32_t l = min((uint32_t)lo.x, (uint32_t)lo.y), h = max((uint32_t)hi.x, (uint32_t)hi.y);
+            for (int o = 32; o > 0; o >>= 1) {
+                l = min(l, (uint32_t)__shfl_xor((int)l, o));
+                h = max(h, (uint32_t)__shfl_xor((int)h, o));
+            }
+            if (lane == 0) {
+                atomicMin(&mm[0], l);
+                atomicMax(&mm[1], h);
             }
         }
     }
@@ -986,6 +1064,8 @@ struct IirArgs {
     int* d_err;
     uint4* ck = nullptr;      /* saved causal states (checkpointed x, z passes) */
     int64_t ck_lanes = 0;     /* lanes of a vector x / z launch; states of chunk k start at k * ck_lanes */
+    uint32_t* mm = nullptr;   /* y pass: (min, max) of the raw input gathered on the way */
+    const int* run_if = nullptr;   /* y pass: runs only when *run_if != 0 */
 };
 
 template <typename T>
@@ -1007,7 +1087,7 @@ struct IirYLdsCk {
     static constexpr int THREADS = 64;
     static const char* name() { return "bspline_y_lds_ck"; }
     static __device__ __forceinline__ void run(const Args& a, int64_t t) {
-        iir_y_lds_ck_body(a.in, a.out, a.y_num, a.n, a.c, a.d_off, a.d_err, (float2*)a.ck, t);
+        iir_y_lds_ck_body(a.in, a.out, a.y_num, a.n, a.c, a.d_off, a.d_err, (float2*)a.ck, a.mm, a.run_if, t);
     }
 };
 template <typename T>
@@ -1055,6 +1135,24 @@ void launch_minmax_offset(aprh_context* ctx, const uint16_t* img, size_t n, floa
 void launch_offset_copy(aprh_context* ctx, const uint16_t* in, uint16_t* out, size_t n, const float* d_offset) {
     const int64_t G = stride_tiles((int64_t)n, 256 * 4);
     launch_tiles<OffsetCopy>(ctx, {in, out, n, d_offset, G}, G);
+}
+
+bool iir_y_fuses_offset(int64_t y, const void* in, const void* out) { return y % 64 == 0 && aligned16(in) && aligned16(out); }
+
+void launch_iir_y_offset(aprh_context* ctx, const uint16_t* in, uint16_t* out, int64_t y, int64_t x, int64_t z, const IIRCoeffs& c,
+                         float lambda, float* d_offset, uint32_t* d_mm, int* d_rerun, int* d_err) {
+    const int64_t lines = x * z;
+    IirArgs<uint16_t> a{in, out, y, lines, 0, 0, 1, c, d_offset, d_err};
+    Buf ck(ctx->pool, (size_t)std::max<int64_t>(y / 64 - 1, 1) * (size_t)lines * 8);
+    a.ck = ck.as<uint4>();
+    launch_tiles<OffsetGuess>(ctx, {d_mm, lambda, d_offset, d_rerun}, 1);
+    a.mm = d_mm;
+    launch_tiles<IirYLdsCk>(ctx, a, cdiv(lines, 64));
+    launch_tiles<OffsetVerify>(ctx, {d_mm, lambda, d_offset, d_rerun, d_err}, 1);
+    /* the pass again with the true offset, as blocks that return at once unless the guess was wrong */
+    a.mm = nullptr;
+    a.run_if = d_rerun;
+    launch_tiles<IirYLdsCk>(ctx, a, cdiv(lines, 64));
 }
 
 template <typename T>

@@ -78,3 +78,39 @@ def test_batch_apr_is_borrowed(ctx):
     assert A.lib().aprh_apr_destroy(apr.h) == A.APRH_ERR_INVALID_ARGUMENT
     assert apr.total > 0
     batch.close()
+
+
+def test_batch_offset_rerun_per_volume(oracle, ctx):
+    # volumes whose lines are a multiple of 64 find the bspline offset in the y pass: a guess,
+    # then a second run inside the graph for a volume whose (min, max) give another offset;
+    # the two replays swap which volume needs it
+    dims = [(128, 40, 24), (64, 72, 20)]
+    par = A.APRParameters(rel_error=0.1, sigma_th=50, Ip_th=120, grad_th=10, lambda_=0.2)
+    opar = oracle.params(rel_error=0.1, sigma_th=50, Ip_th=120, grad_th=10, lambda_=0.2)
+
+    def image(i, mid):
+        y, x, z = dims[i]
+        img = A.synth_blobs((z, x, y), 20 + i, 12).astype(np.float64)
+        lo, hi = (20000, 47000) if mid else (100, 5000)
+        return (lo + (img - img.min()) * (hi - lo) / max(img.max() - img.min(), 1.0)).round().astype(np.uint16)
+
+    batch = A.APRBatch(dims, par=par, ctx=ctx)
+    conv = A.APRConverter(ctx)
+    conv.par = par
+    for rep in range(2):
+        imgs = [image(0, rep == 0), image(1, rep == 1)]
+        for i, img in enumerate(imgs):
+            batch.set_input(i, img)
+        batch.run()
+        for i, img in enumerate(imgs):
+            got = batch.get(i)
+            single = conv.get_apr(img)
+            ref = oracle.get_apr(img, opar, "oracle")
+            print(f"replay {rep} volume {i}: offset {got.bspline_offset} P={got.total} oracle={ref.total}", flush=True)
+            assert got.bspline_offset == single.bspline_offset
+            assert got.total == single.total == ref.total
+            lxz, xze, yv = got.structure()
+            assert np.array_equal(xze, ref.xz_end_vec) and np.array_equal(yv, ref.y_vec)
+            got.close()
+            single.close()
+    batch.close()

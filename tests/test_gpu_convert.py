@@ -51,6 +51,31 @@ def test_get_apr_matches_oracle(oracle, ctx, shape, nb, seed):
     assert np.array_equal(parts, ref.sample(img))
 
 
+@pytest.mark.parametrize("lo,hi,overflow", [(20000, 47000, False), (30000, 35500, False), (5, 40000, False),
+                                             (2000, 64000, True)])
+def test_get_apr_offset_not_guessed(oracle, ctx, lo, hi, overflow):
+    # the y pass starts from the offset that images with max + min <= 65335 get (100) and runs
+    # again with the true offset when the (min, max) it gathers say otherwise: the first two
+    # images (offsets -100 and 17) take the second run, the third the first; the last overflows
+    # the smoothing under either offset.  lambda 0.2 keeps the wide ranges below overflow.
+    z, x, y = 24, 40, 128
+    img = A.synth_blobs((z, x, y), 5, 40).astype(np.float64)
+    img = (lo + (img - img.min()) * (hi - lo) / max(img.max() - img.min(), 1.0)).round().astype(np.uint16)
+    conv = A.APRConverter(ctx)
+    conv.par = A.APRParameters(rel_error=0.1, sigma_th=50, Ip_th=120, grad_th=10, lambda_=0.2)
+    if overflow:
+        with pytest.raises(A.AprhOverflowError):
+            conv.get_apr(img)
+        return
+    ref = oracle.get_apr(img, oracle.params(rel_error=0.1, sigma_th=50, Ip_th=120, grad_th=10, lambda_=0.2), "oracle")
+    apr = conv.get_apr(img)
+    for w in range(6):
+        assert np.array_equal(apr.intermediate(w), ref.intermediate(w)), w
+    assert apr.total == ref.total
+    lxz, xze, yv = apr.structure()
+    assert np.array_equal(xze, ref.xz_end_vec) and np.array_equal(yv, ref.y_vec)
+
+
 @pytest.mark.parametrize("axis", [0, 1, 2])
 @pytest.mark.parametrize("dtype", [np.uint16, np.float32])
 def test_bspline_axis(oracle, ctx, axis, dtype):
