@@ -105,20 +105,33 @@ def run_gpu(args, with_conv=True):
     P = None
 
     last = {}
+    # setup (untimed): one conversion to learn the particle count and size the particle buffer
+    apr0 = conv.get_apr(d_img.value, dims=(y, x, z))
+    P = apr0.total
+    apr0.close()
+    d_parts = C.c_void_p()
+    A._check(L.aprh_malloc(ctx.h, max(P, 1) * 2, C.byref(d_parts)))
 
     def step():
-        nonlocal d_parts, P
+        """get_apr + sample_image of the same image in one call (the row fill gathers the
+        particle intensities; the mean pyramid is built inside the structure stage)"""
+        if "apr" in last:
+            last.pop("apr").close()
+        apr, sampled = conv.get_apr_and_sample(d_img.value, d_parts.value, P, dims=(y, x, z))
+        if not sampled or apr.total != P:
+            raise RuntimeError("particle count changed between conversions of the same image")
+        st = ctx.stage_times()
+        last["apr"] = apr
+        return st
+
+    def step_separate():
+        """the same work as two calls: get_apr, then sample_image (pyramid + gather over y_vec)"""
         if "apr" in last:
             last.pop("apr").close()
         apr = conv.get_apr(d_img.value, dims=(y, x, z))
         st = ctx.stage_times()
-        if d_parts is None:
-            P = apr.total
-            d_parts = C.c_void_p()
-            A._check(L.aprh_malloc(ctx.h, max(P, 1) * 2, C.byref(d_parts)))
         pd.sample_image(apr, d_img.value, out=d_parts.value)
-        st2 = ctx.stage_times()
-        st["sample_image"] = st2["sample_image"]
+        st["sample_image"] = ctx.stage_times()["sample_image"]
         last["apr"] = apr
         return st
 
@@ -135,6 +148,15 @@ def run_gpu(args, with_conv=True):
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
     log(f"[bench] {args.steps} steps: {ms_per_step:.2f} ms/step, P={P}")
+    # reference point (untimed for the metric): the same work as separate get_apr + sample_image calls
+    step_separate()
+    A._check(L.aprh_memcpy(ctx.h, None, None, 0))
+    ts = time.perf_counter()
+    sep = step_separate()
+    A._check(L.aprh_memcpy(ctx.h, None, None, 0))
+    separate_ms = (time.perf_counter() - ts) * 1000.0
+    log(f"[bench] as separate get_apr + sample_image calls: {separate_ms:.2f} ms (sample_image {sep['sample_image']:.3f} ms)")
+    stage_extra = {"separate_calls_ms": round(separate_ms, 3), "separate_sample_image_ms": round(sep["sample_image"], 3)}
     stage_avg = {k: v / args.steps for k, v in stage_acc.items()}
     for k, v in sorted(stage_avg.items(), key=lambda kv: -kv[1]):
         log(f"[bench]   {k:24s} {v:8.3f} ms")
@@ -169,7 +191,7 @@ def run_gpu(args, with_conv=True):
         A._check(L.aprh_free(ctx.h, d_img))
         A._check(L.aprh_free(ctx.h, d_parts))
         ctx.close()
-        return ms_per_step, P, stage_avg, roofline, total_ab, None
+        return ms_per_step, P, stage_avg, roofline, total_ab, None, stage_extra
     # 5x5x5 Gaussian (sigma 1) APR convolution on these particles (u16 in, f32 out)
     apr = last.pop("apr")
     A._check(L.aprh_free(ctx.h, d_img))
@@ -187,7 +209,7 @@ def run_gpu(args, with_conv=True):
     A._check(L.aprh_free(ctx.h, d_out))
     A._check(L.aprh_free(ctx.h, d_parts))
     ctx.close()
-    return ms_per_step, P, stage_avg, roofline, total_ab, c2
+    return ms_per_step, P, stage_avg, roofline, total_ab, c2, stage_extra
 
 
 def conv_algorithmic_bytes(P, T, R, s_in):
@@ -490,11 +512,11 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.only:
-        part = {"get_apr": lambda: run_gpu(args, with_conv=False)[:4], "batch": lambda: run_batch(args, check=False),
+        part = {"get_apr": lambda: (lambda r: list(r[:4]) + [r[6]])(run_gpu(args, with_conv=False)), "batch": lambda: run_batch(args, check=False),
                 "copy": lambda: run_copy(args)}[args.only]()
         print(json.dumps({"only": args.only, "result": part}), flush=True)
         return
-    ms, P, stages, roofline, total_ab, conv5 = run_gpu(args)
+    ms, P, stages, roofline, total_ab, conv5, extra = run_gpu(args)
     z, x, y = SHAPE_ZXY
     N = x * y * z
     conv3 = run_conv_small(args)
@@ -518,6 +540,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "get_apr + sample_image, 2048x2048x512 u16, 200k blobs sigma 1.5-8 amp 300-4000, "
                                "bg 100 + shot noise, seed 1; rel_error 0.1 lambda 3 sigma_th 50 Ip_th 120 grad_th 10",
+                   "call": "aprh_get_apr_sample_u16 (conversion and sampling in one call)",
+                   "as_separate_calls_ms": extra["separate_calls_ms"],
                    "particles": P, "pipeline_algorithmic_bytes": total_ab,
                    "pipeline_GBps": round(total_ab / (ms / 1000.0) / 1e9, 1),
                    "stage_ms": {k: round(v, 3) for k, v in stages.items()},

@@ -81,6 +81,14 @@ int aprh_context_release_cache(aprh_context* ctx);
 /* APRConverter<uint16_t>::get_apr (src/algorithm/APRConverter.hpp:591-600, get_apr_cpu :534-582) */
 int aprh_get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y_num, int64_t x_num, int64_t z_num,
                      const aprh_params* par, aprh_apr** out);
+/* get_apr followed by ParticleData<uint16_t>::sample_image of the same image (the usual pair,
+   APRConverter.hpp:591-600 then ParticleData.hpp:122-127), in one call: the row fill that
+   writes y_vec_ also gathers the particle intensities.  parts is device memory with room for
+   parts_capacity particles; *sampled = 1 when they were written (the APR's particle count is at
+   most parts_capacity), else 0 and the caller samples with aprh_sample_image_u16.  The APR and
+   the intensities equal those of aprh_get_apr_u16 + aprh_sample_image_u16. */
+int aprh_get_apr_sample_u16(aprh_context* ctx, const uint16_t* image, int64_t y_num, int64_t x_num, int64_t z_num,
+                            const aprh_params* par, aprh_apr** out, uint16_t* parts, uint64_t parts_capacity, int32_t* sampled);
 /* APRConverter<float>::get_apr (APRConverter.hpp:534-600 with ImageType = float): no B-spline
    offset (:218-220), float smoothing and gradient; same structure semantics */
 int aprh_get_apr_f32(aprh_context* ctx, const float* image, int64_t y_num, int64_t x_num, int64_t z_num, const aprh_params* par,

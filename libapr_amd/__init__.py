@@ -101,6 +101,8 @@ def lib():
     L.aprh_stage_name.argtypes = [C.c_int]
     L.aprh_context_release_cache.argtypes = [vp]
     L.aprh_get_apr_u16.argtypes = [vp, vp, i64, i64, i64, C.POINTER(AprhParams), C.POINTER(vp)]
+    L.aprh_get_apr_sample_u16.argtypes = [vp, vp, i64, i64, i64, C.POINTER(AprhParams), C.POINTER(vp), vp, C.c_uint64,
+                                          C.POINTER(C.c_int32)]
     L.aprh_get_apr_f32.argtypes = [vp, vp, i64, i64, i64, C.POINTER(AprhParams), C.POINTER(vp)]
     L.aprh_apr_from_grad_lis.argtypes = [vp, vp, vp, vp, i64, i64, i64, C.POINTER(AprhParams), C.c_float, C.POINTER(vp)]
     L.aprh_apr_info_get.argtypes = [vp, C.POINTER(AprhAprInfo)]
@@ -402,6 +404,25 @@ class APRConverter:
         f = lib().aprh_get_apr_f32 if self.image_type == np.float32 else lib().aprh_get_apr_u16
         _check(f(self.ctx.h, _ptr(image), y, x, z, C.byref(p), C.byref(h)))
         return APR(self.ctx, h)
+
+    def get_apr_and_sample(self, image, parts, capacity, dims=None):
+        """get_apr followed by ParticleData.sample_image of the same u16 image in one call.  parts is
+        a device address with room for `capacity` u16 particles; returns (apr, sampled), sampled
+        False when the APR has more particles (then sample with ParticleData.sample_image)."""
+        if self.image_type != np.uint16:
+            raise TypeError("get_apr_and_sample converts u16 images")
+        if dims is None:
+            z, x, y = image.shape
+            if image.dtype != np.uint16:
+                raise TypeError("get_apr_and_sample needs a uint16 image")
+        else:
+            y, x, z = dims
+        h = vp()
+        p = self.par.c()
+        done = C.c_int32(0)
+        _check(lib().aprh_get_apr_sample_u16(self.ctx.h, _ptr(image), y, x, z, C.byref(p), C.byref(h), parts, capacity,
+                                             C.byref(done)))
+        return APR(self.ctx, h), bool(done.value)
 
     def get_apr_custom_grad_scale(self, grad_ds, lis_ds, intensity_ds, dims, bspline_offset=0.0):
         y, x, z = dims

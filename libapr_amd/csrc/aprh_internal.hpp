@@ -118,6 +118,17 @@ enum Stage {
 
 namespace aprh { struct TileRecorder; }
 
+namespace aprh {
+/* conversion with sampling (aprh_get_apr_sample_u16): the row fill of the structure also
+   gathers the image's particle intensities when they fit into parts */
+struct SampleReq {
+    const uint16_t* img = nullptr;   /* device copy of the converted image */
+    uint16_t* parts = nullptr;
+    uint64_t capacity = 0;
+    bool done = false;
+};
+}  // namespace aprh
+
 struct aprh_context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -137,6 +148,7 @@ struct aprh_context {
     int* slot_err = nullptr;          /* 3 IIR overflow counters (y, x, z)     */
     /* recording mode of a batch plan: stage launches are appended here instead of run */
     aprh::TileRecorder* recorder = nullptr;
+    aprh::SampleReq* sample_req = nullptr;   /* set for one aprh_get_apr_sample_u16 call */
 };
 
 /* device-resident APR: GenInfo + LinearAccess (+ optional intermediates) */

@@ -69,6 +69,7 @@ void get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x,
     const size_t nd = (size_t)(yd * xd * zd);
     hipStream_t st = ctx->stream;
     DevIn in(ctx, image, n * 2);
+    if (ctx->sample_req) ctx->sample_req->img = (const uint16_t*)in.ptr;
     geninfo_init(*a, y, x, z);
     mark(ctx, ST_OFFSET);
     /* offset, min/max and overflow words: the context's, or a recorded volume's own */
@@ -248,6 +249,42 @@ int aprh_get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_
         delete a;
         throw;
     }
+    *out = a;
+    APRH_CATCH
+}
+
+int aprh_get_apr_sample_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x, int64_t z, const aprh_params* par,
+                            aprh_apr** out, uint16_t* parts, uint64_t parts_capacity, int32_t* sampled) {
+    APRH_TRY
+    check_ctx(ctx);
+    check_dims(y, x, z);
+    if (!image || !par || !out || !sampled) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (parts_capacity > 0 && !parts) throw Error(APRH_ERR_INVALID_ARGUMENT, "parts is NULL with a nonzero capacity");
+    *sampled = 0;
+    if (parts && !is_device_ptr(parts)) throw Error(APRH_ERR_INVALID_ARGUMENT, "parts must be device memory");
+    aprh_apr* a = new aprh_apr();
+    a->ctx = ctx;
+    SampleReq req;
+    req.parts = parts;
+    req.capacity = parts ? parts_capacity : 0;
+    try {
+        ctx->sample_req = &req;
+        get_apr_u16(ctx, image, y, x, z, *par, a);
+        ctx->sample_req = nullptr;
+        if (!req.done && parts && a->total <= parts_capacity) {
+            /* conversions whose structure is not built by the row fill (l_max <= 2) */
+            DevIn in(ctx, image, (size_t)(y * x * z) * 2);
+            launch_sample_u16(ctx, *a, (const uint16_t*)in.ptr, parts);
+            req.done = true;
+        }
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+        ctx->sample_req = nullptr;
+        (void)hipStreamSynchronize(ctx->stream);
+        delete a;
+        throw;
+    }
+    *sampled = req.done ? 1 : 0;
     *out = a;
     APRH_CATCH
 }

@@ -354,8 +354,17 @@ __device__ __forceinline__ void row_count_body(const uint8_t* __restrict__ maps,
    stage, which the wave then copies to y_vec with consecutive lanes on consecutive entries
    (once per destination row). */
 constexpr int FILL_WAVES = 4;
+/* level images for a row fill that also samples: APR level index li -> image of that level
+   (the converted image at l_max, the mean pyramid below), rows in (z, x) order like the APR rows */
+struct RowImages {
+    const uint16_t* img[MAXL];
+    int64_t y[MAXL];
+    uint16_t* parts;   /* nullptr: no sampling */
+};
+
 __device__ __forceinline__ void row_fill_body(const uint8_t* __restrict__ maps, const RowTable& t, int64_t nrows,
-                                              const uint64_t* __restrict__ row_off, uint16_t* y_vec, int64_t G, int64_t blk) {
+                                              const uint64_t* __restrict__ row_off, uint16_t* y_vec, const RowImages& ri, int64_t G,
+                                              int64_t blk) {
     __shared__ uint16_t stage[FILL_WAVES][2 * 64 * CPL];
     const int lane = threadIdx.x & 63;
     uint16_t* st = stage[threadIdx.x >> 6];
@@ -395,7 +404,32 @@ __device__ __forceinline__ void row_fill_body(const uint8_t* __restrict__ maps, 
             for (int k = 0; k < 4; ++k) {
                 if (k >= wr.n) break;
                 uint16_t* dst = y_vec + pos[k];
-                for (int i = lane; i < total; i += 64) dst[i] = st[i];
+                if (ri.parts) {
+                    /* sample_image's gather (ParticleData.hpp:221-261) for this row's particles */
+                    const uint16_t* irow = ri.img[wr.li] + (wr.r[k] - t.row_prefix[wr.li]) * ri.y[wr.li];
+                    uint16_t* pdst = ri.parts + pos[k];
+                    int i = lane;
+                    /* four particles per lane in flight */
+                    for (; i + 192 < total; i += 256) {
+                        const uint16_t y0 = st[i], y1 = st[i + 64], y2 = st[i + 128], y3 = st[i + 192];
+                        const uint16_t v0 = irow[y0], v1 = irow[y1], v2 = irow[y2], v3 = irow[y3];
+                        dst[i] = y0;
+                        dst[i + 64] = y1;
+                        dst[i + 128] = y2;
+                        dst[i + 192] = y3;
+                        pdst[i] = v0;
+                        pdst[i + 64] = v1;
+                        pdst[i + 128] = v2;
+                        pdst[i + 192] = v3;
+                    }
+                    for (; i < total; i += 64) {
+                        const uint16_t yv = st[i];
+                        dst[i] = yv;
+                        pdst[i] = irow[yv];
+                    }
+                } else {
+                    for (int i = lane; i < total; i += 64) dst[i] = st[i];
+                }
                 pos[k] += (uint64_t)total;
             }
             __threadfence_block();   /* the stage is rewritten by the next chunk */
@@ -541,10 +575,13 @@ struct RowFill {
         const uint64_t* row_off;
         uint16_t* y_vec;
         int64_t G;
+        RowImages ri;
     };
     static constexpr int THREADS = 256;
     static const char* name() { return "row_fill"; }
-    static __device__ __forceinline__ void run(const Args& a, int64_t t) { row_fill_body(a.maps, a.t, a.nrows, a.row_off, a.y_vec, a.G, t); }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) {
+        row_fill_body(a.maps, a.t, a.nrows, a.row_off, a.y_vec, a.ri, a.G, t);
+    }
 };
 struct RowsXz {
     struct Args {
@@ -709,6 +746,9 @@ uint64_t exclusive_scan_u32(aprh_context* ctx, const uint32_t* in, uint64_t* out
     return total;
 }
 
+template <typename T>
+std::vector<int64_t> build_pyramid(aprh_context* ctx, const aprh_apr& a, const T* img, Buf& pyr);
+
 void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, const aprh_params& par) {
     hipStream_t st = ctx->stream;
     const int lmax = a.l_max, lmin = a.l_min;
@@ -825,7 +865,20 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
     }
     launch_zero(ctx, a.xz_end.ptr(), a.n_xz * 8);
     launch_tiles<RowsXz>(ctx, {rowoff.as<uint64_t>(), cnt.as<uint32_t>(), t, a.xz_end.as<uint64_t>(), rp}, cdiv(rp, 256));
-    launch_tiles<RowFill>(ctx, {P, t, rp, rowoff.as<uint64_t>(), a.y_vec.as<uint16_t>(), RG}, RG);
+    RowImages ri;
+    std::memset(&ri, 0, sizeof(ri));
+    Buf pyr;
+    SampleReq* sr = ctx->sample_req;
+    if (sr && sr->img && !ctx->async_mode && !ctx->recorder && a.total <= sr->capacity) {
+        const std::vector<int64_t> poff = build_pyramid<uint16_t>(ctx, a, sr->img, pyr);
+        for (int l = lmin; l <= lmax; ++l) {
+            ri.img[l - lmin] = (l == lmax) ? sr->img : pyr.as<uint16_t>() + poff[l];
+            ri.y[l - lmin] = a.y_num[l];
+        }
+        ri.parts = sr->parts;
+        sr->done = true;
+    }
+    launch_tiles<RowFill>(ctx, {P, t, rp, rowoff.as<uint64_t>(), a.y_vec.as<uint16_t>(), RG, ri}, RG);
     if (ctx->keep_intermediates) {
         a.pct.clear();
         a.pct_dims.assign(3 * (top + 1), 0);
@@ -842,18 +895,19 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
     if (!ctx->async_mode) HIP_CHECK(hipStreamSynchronize(st));
 }
 
+/* the mean pyramid of an image over levels lmin .. lmax-1 (PixelData.hpp:1036-1058), one
+   buffer with 16-byte aligned levels; returns the per-level element offsets */
 template <typename T>
-void launch_sample(aprh_context* ctx, const aprh_apr& a, const T* img, T* parts) {
+std::vector<int64_t> build_pyramid(aprh_context* ctx, const aprh_apr& a, const T* img, Buf& pyr) {
     hipStream_t st = ctx->stream;
     const int lmax = a.l_max, lmin = a.l_min;
-    /* pyramid levels lmin..lmax-1 in one buffer */
     std::vector<int64_t> off(lmax + 1, 0);
     int64_t tot = 0;
     for (int l = lmin; l < lmax; ++l) {
         off[l] = tot;
-        tot += (a.y_num[l] * a.x_num[l] * a.z_num[l] + 7) / 8 * 8;   /* 16-byte aligned levels */
+        tot += (a.y_num[l] * a.x_num[l] * a.z_num[l] + 7) / 8 * 8;
     }
-    Buf pyr(ctx->pool, (size_t)std::max<int64_t>(tot, 1) * sizeof(T));
+    pyr.reset(ctx->pool, (size_t)std::max<int64_t>(tot, 1) * sizeof(T));
     T* Pyr = pyr.as<T>();
     const T* fine = img;
     for (int l = lmax - 1; l >= lmin; --l) {
@@ -870,6 +924,16 @@ void launch_sample(aprh_context* ctx, const aprh_apr& a, const T* img, T* parts)
         HIP_LAUNCH_CHECK();
         fine = coarse;
     }
+    return off;
+}
+
+template <typename T>
+void launch_sample(aprh_context* ctx, const aprh_apr& a, const T* img, T* parts) {
+    hipStream_t st = ctx->stream;
+    const int lmax = a.l_max, lmin = a.l_min;
+    Buf pyr;
+    const std::vector<int64_t> off = build_pyramid<T>(ctx, a, img, pyr);
+    T* Pyr = pyr.as<T>();
     LevelPtrs<T> L;
     std::memset(&L, 0, sizeof(L));
     L.n_levels = lmax - lmin + 1;

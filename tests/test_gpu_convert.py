@@ -51,6 +51,38 @@ def test_get_apr_matches_oracle(oracle, ctx, shape, nb, seed):
     assert np.array_equal(parts, ref.sample(img))
 
 
+@pytest.mark.parametrize("shape,nb,seed", CASES + [((4, 6, 5), 1, 3), ((17, 45, 130), 12, 4)])
+def test_get_apr_and_sample_matches_separate_calls(oracle, ctx, shape, nb, seed):
+    # one call that converts and samples (the row fill gathers the intensities) against
+    # get_apr + sample_image and the oracle; host and device images; a buffer one particle too
+    # small leaves the sampling to the caller
+    img = A.synth_blobs(shape, seed, nb)
+    conv = A.APRConverter(ctx)
+    conv.par = _apar()
+    ref_apr = conv.get_apr(img)
+    ref_parts = A.ParticleData().sample_image(ref_apr, img)
+    P = ref_apr.total
+    print(f"case {shape} P={P} l_max={ref_apr.l_max}", flush=True)
+    d_img = A.DeviceArray(img.nbytes, ctx)
+    d_img.from_host(img)
+    for src, dims in ((img, None), (d_img.ptr, shape[::-1])):
+        parts = A.DeviceArray(P * 2 + 2, ctx)
+        apr, sampled = conv.get_apr_and_sample(src, parts.ptr, P, dims=dims)
+        assert sampled and apr.total == P
+        lxz, xze, yv = apr.structure()
+        rl, rx, ry = ref_apr.structure()
+        assert np.array_equal(xze, rx) and np.array_equal(yv, ry) and np.array_equal(lxz, rl)
+        assert np.array_equal(parts.to_host(np.uint16, P), ref_parts)
+        apr.close()
+        if P > 0:
+            apr, sampled = conv.get_apr_and_sample(src, parts.ptr, P - 1, dims=dims)
+            assert not sampled and apr.total == P
+            apr.close()
+        parts.free()
+    assert np.array_equal(ref_parts, oracle.get_apr(img, _par(oracle), "oracle").sample(img))
+    d_img.free()
+
+
 @pytest.mark.parametrize("lo,hi,overflow", [(20000, 47000, False), (30000, 35500, False), (5, 40000, False),
                                              (2000, 64000, True)])
 def test_get_apr_offset_not_guessed(oracle, ctx, lo, hi, overflow):
