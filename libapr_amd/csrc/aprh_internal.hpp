@@ -126,6 +126,9 @@ struct SampleReq {
     uint16_t* parts = nullptr;
     uint64_t capacity = 0;
     bool done = false;
+    Buf pyr;                         /* the image's mean pyramid (levels l_min .. l_max-1)     */
+    std::vector<int64_t> pyr_off;    /* element offset of each level in pyr                   */
+    bool top_built = false;          /* level l_max-1 already written (by the y pass)         */
 };
 }  // namespace aprh
 
@@ -208,8 +211,10 @@ void launch_iir_y(aprh_context* ctx, const T* in, T* out, int64_t y, int64_t x, 
 /* u16 y pass that also finds the bspline offset: it runs with the offset most images get, gathers
    the input's (min, max) on the way, and runs again only when the true offset differs */
 bool iir_y_fuses_offset(int64_t y, const void* in, const void* out);
+/* pyr (optional): receives the first level of the input's mean pyramid (ceil(x/2) x ceil(z/2) rows
+   of y/2, 16-byte aligned) */
 void launch_iir_y_offset(aprh_context* ctx, const uint16_t* in, uint16_t* out, int64_t y, int64_t x, int64_t z, const IIRCoeffs& c,
-                         float lambda, float* d_offset, uint32_t* d_mm, int* d_rerun, int* d_err);
+                         float lambda, float* d_offset, uint32_t* d_mm, int* d_rerun, int* d_err, uint16_t* pyr);
 template <typename T>
 void launch_iir_outer(aprh_context* ctx, T* img, int64_t y, int64_t x, int64_t z, int axis, const IIRCoeffs& c, int* d_err);
 
@@ -253,6 +258,8 @@ void launch_grad_ds_f32(aprh_context* ctx, const float* img, float* grad, float*
 /* ---- pulling scheme / structure / sampling (k_structure.hip) ---- */
 void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, const aprh_params& par);
 void launch_sample_u16(aprh_context* ctx, const aprh_apr& a, const uint16_t* img, uint16_t* parts);
+/* element offsets of the mean pyramid's levels l_min .. l_max-1 in one buffer; *total elements */
+std::vector<int64_t> pyramid_layout(const aprh_apr& a, int64_t* total);
 /* ParticleData<float>::sample_image of a float image (same pyramid, float means) */
 void launch_sample_f32(aprh_context* ctx, const aprh_apr& a, const float* img, float* parts);
 

@@ -87,8 +87,18 @@ void get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x,
     if (par.lambda > 0 && y > 2 && iir_y_fuses_offset(y, in.ptr, I)) {
         /* computeL's bspline offset (APRConverter.hpp:213-224) found by the y pass itself */
         mark(ctx, ST_SMOOTH_Y);
+        /* with sampling requested, the pass also writes the first level of the image's mean pyramid */
+        uint16_t* pyr_top = nullptr;
+        SampleReq* sr = ctx->sample_req;
+        if (sr && !ctx->recorder && a->l_max - 1 >= a->l_min && a->l_max > 2) {
+            int64_t tot = 0;
+            sr->pyr_off = pyramid_layout(*a, &tot);
+            sr->pyr.reset(ctx->pool, (size_t)std::max<int64_t>(tot, 1) * 2);
+            pyr_top = sr->pyr.as<uint16_t>() + sr->pyr_off[a->l_max - 1];
+            sr->top_built = true;
+        }
         launch_iir_y_offset(ctx, (const uint16_t*)in.ptr, I, y, x, z, iir_coeffs(par.lambda, tol, y), par.lambda, d_off, d_mm, d_rerun,
-                            d_err + 0);
+                            d_err + 0, pyr_top);
     } else {
         /* computeL: bspline offset (APRConverter.hpp:213-224) */
         launch_minmax_offset(ctx, (const uint16_t*)in.ptr, n, par.lambda, d_off, d_mm);

@@ -863,15 +863,25 @@ __device__ __forceinline__ void iir_outer_ck_body(T* img, int64_t y_num, int64_t
  * safe: chunk ch is written after chunk ch - 1 has been loaded. */
 __device__ __forceinline__ void iir_y_lds_ck_body(const uint16_t* in, uint16_t* out, int64_t y_num, int64_t n_lines, const IIRCoeffs& c,
                                                   const float* d_off, int* d_err, float2* __restrict__ ck, uint32_t* mm,
-                                                  const int* run_if, int64_t blk) {
+                                                  const int* run_if, uint16_t* pyr, int64_t px_num, int64_t pz_num, int64_t blk) {
     constexpr int CH = 64;
     constexpr int PITCH = 72;
     __shared__ __attribute__((aligned(16))) uint16_t tile[64 * PITCH];
     if (run_if && *run_if == 0) return;
     const int lane = threadIdx.x;
+    /* LDS row R -> line (z * x_num + x), or -1.  Plain: 64 consecutive lines.  With pyr: rows
+       0..31 are 32 consecutive x of plane 2 zp, rows 32..63 the same x of plane 2 zp + 1, so the
+       2x2 (x, z) neighbourhoods of the mean pyramid's cells are all in the tile */
+    const int64_t nxb = pyr ? (px_num + 31) / 32 : 1;
+    const int64_t pzp = pyr ? blk / nxb : 0, pxb = pyr ? blk - pzp * nxb : 0;
     const int64_t line0 = blk * 64;
-    const int64_t me = line0 + lane;
-    const bool valid = me < n_lines;
+    auto line_of = [&](int R) -> int64_t {
+        if (!pyr) return line0 + R < n_lines ? line0 + R : -1;
+        const int64_t zz = 2 * pzp + (R >> 5), xx = pxb * 32 + (R & 31);
+        return (zz < pz_num && xx < px_num) ? zz * px_num + xx : -1;
+    };
+    const int64_t me = line_of(lane);
+    const bool valid = me >= 0;
     const bool has_off = d_off != nullptr;
     const float off = has_off ? *d_off : 0.0f;
     const float b1 = c.b1, b2 = c.b2, nf = c.nf;
@@ -879,12 +889,14 @@ __device__ __forceinline__ void iir_y_lds_ck_body(const uint16_t* in, uint16_t* 
     const int seg = lane & 7;
     int err = 0;
 
+    /* lines of the cooperative layout: load instruction i covers LDS rows 8i..8i+7 */
+    int64_t lrow[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) lrow[i] = line_of(8 * i + (lane >> 3));
     auto gload = [&](const uint16_t* base, int64_t ch, uint4* r) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int64_t ln = line0 + 8 * i + (lane >> 3);
-            r[i] = (ln < n_lines) ? *(const uint4*)(base + ln * y_num + ch * CH + seg * 8) : make_uint4(0, 0, 0, 0);
-        }
+        for (int i = 0; i < 8; ++i)
+            r[i] = (lrow[i] >= 0) ? *(const uint4*)(base + lrow[i] * y_num + ch * CH + seg * 8) : make_uint4(0, 0, 0, 0);
     };
     auto lds_put = [&](const uint4* r) {
 #pragma unroll
@@ -892,10 +904,37 @@ __device__ __forceinline__ void iir_y_lds_ck_body(const uint16_t* in, uint16_t* 
     };
     auto gstore = [&](uint16_t* base, int64_t ch) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int64_t ln = line0 + 8 * i + (lane >> 3);
-            if (ln < n_lines) *(uint4*)(base + ln * y_num + ch * CH + seg * 8) = *(const uint4*)(&tile[(8 * i + (lane >> 3)) * PITCH + seg * 8]);
+        for (int i = 0; i < 8; ++i)
+            if (lrow[i] >= 0) *(uint4*)(base + lrow[i] * y_num + ch * CH + seg * 8) = *(const uint4*)(&tile[(8 * i + (lane >> 3)) * PITCH + seg * 8]);
+    };
+    /* the mean pyramid's cells of one chunk (PixelData.hpp:944-962 with u16 storage): 16 x pairs
+       x 32 y pairs of the plane pair; a lane sums 8 y pairs of one x pair over its 4 rows (eight
+       u16 values per cell, exact in float) and stores them as one 16-byte vector */
+    auto pyr_chunk = [&](int64_t ch) {
+        const int xcl = lane & 15, ycg = lane >> 4;
+        const int64_t Xc = (px_num + 1) / 2, Yc = y_num / 2;
+        const int64_t xc = pxb * 16 + xcl;
+        if (xc >= Xc) return;
+        const int r00 = 2 * xcl;
+        const int r01 = (pxb * 32 + 2 * xcl + 1 < px_num) ? r00 + 1 : r00;
+        const bool hz = 2 * pzp + 1 < pz_num;
+        const int rows[4] = {r00, r01, hz ? r00 + 32 : r00, hz ? r01 + 32 : r01};
+        uint32_t sum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const uint4 v = *(const uint4*)(&tile[rows[q] * PITCH + ycg * 16 + h * 8]);
+                const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) sum[h * 4 + j] += (w[j] & 0xFFFFu) + (w[j] >> 16);
+            }
         }
+        uint32_t o[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) o[j] = (uint32_t)(uint16_t)(int32_t)((float)sum[j] / 8.0f);
+        *(uint4*)(pyr + (pzp * Xc + xc) * Yc + ch * (CH / 2) + ycg * 8) =
+            make_uint4(o[0] | (o[1] << 16), o[2] | (o[3] << 16), o[4] | (o[5] << 16), o[6] | (o[7] << 16));
     };
 
     /* boundary sums (ComputeGradient.hpp:342-356) */
@@ -970,7 +1009,7 @@ __device__ __forceinline__ void iir_y_lds_ck_body(const uint16_t* in, uint16_t* 
             if (mm) {
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
-                    if (line0 + 8 * i + (lane >> 3) >= n_lines) continue;
+                    if (lrow[i] < 0) continue;
                     const uint32_t w[4] = {nxt[i].x, nxt[i].y, nxt[i].z, nxt[i].w};
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
@@ -984,6 +1023,7 @@ __device__ __forceinline__ void iir_y_lds_ck_body(const uint16_t* in, uint16_t* 
             lds_put(nxt);
             if (ch + 1 < nch) gload(in, ch + 1, nxt);
             __syncthreads();
+            if (pyr) pyr_chunk(ch);
             if (valid) {
                 if (ch > 0) ck[(ch - 1) * n_lines + me] = make_float2(s1, s2);
                 causal(&tile[lane * PITCH], ch, s1, s2, false);
@@ -1066,6 +1106,8 @@ struct IirArgs {
     int64_t ck_lanes = 0;     /* lanes of a vector x / z launch; states of chunk k start at k * ck_lanes */
     uint32_t* mm = nullptr;   /* y pass: (min, max) of the raw input gathered on the way */
     const int* run_if = nullptr;   /* y pass: runs only when *run_if != 0 */
+    uint16_t* pyr = nullptr;  /* y pass: also write the 2x2x2 mean level of the raw input here */
+    int64_t px = 0, pz = 0;   /* y pass with pyr: x and z sizes (lines = px * pz) */
 };
 
 template <typename T>
@@ -1087,7 +1129,7 @@ struct IirYLdsCk {
     static constexpr int THREADS = 64;
     static const char* name() { return "bspline_y_lds_ck"; }
     static __device__ __forceinline__ void run(const Args& a, int64_t t) {
-        iir_y_lds_ck_body(a.in, a.out, a.y_num, a.n, a.c, a.d_off, a.d_err, (float2*)a.ck, a.mm, a.run_if, t);
+        iir_y_lds_ck_body(a.in, a.out, a.y_num, a.n, a.c, a.d_off, a.d_err, (float2*)a.ck, a.mm, a.run_if, a.pyr, a.px, a.pz, t);
     }
 };
 template <typename T>
@@ -1140,17 +1182,26 @@ void launch_offset_copy(aprh_context* ctx, const uint16_t* in, uint16_t* out, si
 bool iir_y_fuses_offset(int64_t y, const void* in, const void* out) { return y % 64 == 0 && aligned16(in) && aligned16(out); }
 
 void launch_iir_y_offset(aprh_context* ctx, const uint16_t* in, uint16_t* out, int64_t y, int64_t x, int64_t z, const IIRCoeffs& c,
-                         float lambda, float* d_offset, uint32_t* d_mm, int* d_rerun, int* d_err) {
+                         float lambda, float* d_offset, uint32_t* d_mm, int* d_rerun, int* d_err, uint16_t* pyr) {
     const int64_t lines = x * z;
     IirArgs<uint16_t> a{in, out, y, lines, 0, 0, 1, c, d_offset, d_err};
     Buf ck(ctx->pool, (size_t)std::max<int64_t>(y / 64 - 1, 1) * (size_t)lines * 8);
     a.ck = ck.as<uint4>();
     launch_tiles<OffsetGuess>(ctx, {d_mm, lambda, d_offset, d_rerun}, 1);
     a.mm = d_mm;
-    launch_tiles<IirYLdsCk>(ctx, a, cdiv(lines, 64));
+    if (pyr) {
+        /* blocks over (plane pair, 32 x) so the first level of the mean pyramid comes along */
+        a.pyr = pyr;
+        a.px = x;
+        a.pz = z;
+        launch_tiles<IirYLdsCk>(ctx, a, cdiv(z, 2) * cdiv(x, 32));
+    } else {
+        launch_tiles<IirYLdsCk>(ctx, a, cdiv(lines, 64));
+    }
     launch_tiles<OffsetVerify>(ctx, {d_mm, lambda, d_offset, d_rerun, d_err}, 1);
     /* the pass again with the true offset, as blocks that return at once unless the guess was wrong */
     a.mm = nullptr;
+    a.pyr = nullptr;
     a.run_if = d_rerun;
     launch_tiles<IirYLdsCk>(ctx, a, cdiv(lines, 64));
 }

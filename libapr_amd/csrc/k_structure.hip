@@ -748,6 +748,8 @@ uint64_t exclusive_scan_u32(aprh_context* ctx, const uint32_t* in, uint64_t* out
 
 template <typename T>
 std::vector<int64_t> build_pyramid(aprh_context* ctx, const aprh_apr& a, const T* img, Buf& pyr);
+template <typename T>
+void pyramid_levels(aprh_context* ctx, const aprh_apr& a, const T* img, T* Pyr, const std::vector<int64_t>& off, int first);
 
 void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, const aprh_params& par) {
     hipStream_t st = ctx->stream;
@@ -867,12 +869,16 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
     launch_tiles<RowsXz>(ctx, {rowoff.as<uint64_t>(), cnt.as<uint32_t>(), t, a.xz_end.as<uint64_t>(), rp}, cdiv(rp, 256));
     RowImages ri;
     std::memset(&ri, 0, sizeof(ri));
-    Buf pyr;
     SampleReq* sr = ctx->sample_req;
     if (sr && sr->img && !ctx->async_mode && !ctx->recorder && a.total <= sr->capacity) {
-        const std::vector<int64_t> poff = build_pyramid<uint16_t>(ctx, a, sr->img, pyr);
+        if (sr->top_built) {
+            /* level lmax-1 came with the y pass */
+            pyramid_levels<uint16_t>(ctx, a, sr->img, sr->pyr.as<uint16_t>(), sr->pyr_off, lmax - 2);
+        } else {
+            sr->pyr_off = build_pyramid<uint16_t>(ctx, a, sr->img, sr->pyr);
+        }
         for (int l = lmin; l <= lmax; ++l) {
-            ri.img[l - lmin] = (l == lmax) ? sr->img : pyr.as<uint16_t>() + poff[l];
+            ri.img[l - lmin] = (l == lmax) ? sr->img : sr->pyr.as<uint16_t>() + sr->pyr_off[l];
             ri.y[l - lmin] = a.y_num[l];
         }
         ri.parts = sr->parts;
@@ -895,22 +901,26 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
     if (!ctx->async_mode) HIP_CHECK(hipStreamSynchronize(st));
 }
 
-/* the mean pyramid of an image over levels lmin .. lmax-1 (PixelData.hpp:1036-1058), one
-   buffer with 16-byte aligned levels; returns the per-level element offsets */
-template <typename T>
-std::vector<int64_t> build_pyramid(aprh_context* ctx, const aprh_apr& a, const T* img, Buf& pyr) {
-    hipStream_t st = ctx->stream;
-    const int lmax = a.l_max, lmin = a.l_min;
-    std::vector<int64_t> off(lmax + 1, 0);
+/* layout of the mean pyramid of an image over levels lmin .. lmax-1 (PixelData.hpp:1036-1058):
+   one buffer, 16-byte aligned levels; per-level element offsets and the total */
+std::vector<int64_t> pyramid_layout(const aprh_apr& a, int64_t* total) {
+    std::vector<int64_t> off(a.l_max + 1, 0);
     int64_t tot = 0;
-    for (int l = lmin; l < lmax; ++l) {
+    for (int l = a.l_min; l < a.l_max; ++l) {
         off[l] = tot;
         tot += (a.y_num[l] * a.x_num[l] * a.z_num[l] + 7) / 8 * 8;
     }
-    pyr.reset(ctx->pool, (size_t)std::max<int64_t>(tot, 1) * sizeof(T));
-    T* Pyr = pyr.as<T>();
-    const T* fine = img;
-    for (int l = lmax - 1; l >= lmin; --l) {
+    *total = tot;
+    return off;
+}
+
+/* levels `first` .. lmin of the pyramid, each from the level above (the image above lmax-1) */
+template <typename T>
+void pyramid_levels(aprh_context* ctx, const aprh_apr& a, const T* img, T* Pyr, const std::vector<int64_t>& off, int first) {
+    hipStream_t st = ctx->stream;
+    const int lmax = a.l_max, lmin = a.l_min;
+    for (int l = first; l >= lmin; --l) {
+        const T* fine = (l == lmax - 1) ? img : Pyr + off[l + 1];
         const Dims fd{a.y_num[l + 1], a.x_num[l + 1], a.z_num[l + 1]}, cd{a.y_num[l], a.x_num[l], a.z_num[l]};
         T* coarse = Pyr + off[l];
         if (std::is_same<T, uint16_t>::value && fd.y % 8 == 0 && cd.y * 2 == fd.y && ((uintptr_t)fine & 15u) == 0 &&
@@ -922,8 +932,15 @@ std::vector<int64_t> build_pyramid(aprh_context* ctx, const aprh_apr& a, const T
             hipLaunchKernelGGL(k_pyr_down<T>, dim3(4096), dim3(256), 0, st, fine, fd, coarse, cd);
         }
         HIP_LAUNCH_CHECK();
-        fine = coarse;
     }
+}
+
+template <typename T>
+std::vector<int64_t> build_pyramid(aprh_context* ctx, const aprh_apr& a, const T* img, Buf& pyr) {
+    int64_t tot = 0;
+    std::vector<int64_t> off = pyramid_layout(a, &tot);
+    pyr.reset(ctx->pool, (size_t)std::max<int64_t>(tot, 1) * sizeof(T));
+    pyramid_levels<T>(ctx, a, img, pyr.as<T>(), off, a.l_max - 1);
     return off;
 }
 

@@ -51,7 +51,7 @@ def test_get_apr_matches_oracle(oracle, ctx, shape, nb, seed):
     assert np.array_equal(parts, ref.sample(img))
 
 
-@pytest.mark.parametrize("shape,nb,seed", CASES + [((4, 6, 5), 1, 3), ((17, 45, 130), 12, 4)])
+@pytest.mark.parametrize("shape,nb,seed", CASES + [((4, 6, 5), 1, 3), ((17, 45, 128), 12, 4), ((9, 70, 192), 9, 5)])
 def test_get_apr_and_sample_matches_separate_calls(oracle, ctx, shape, nb, seed):
     # one call that converts and samples (the row fill gathers the intensities) against
     # get_apr + sample_image and the oracle; host and device images; a buffer one particle too
