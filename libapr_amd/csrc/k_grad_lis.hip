@@ -113,15 +113,16 @@ __device__ __forceinline__ void grad_ds_body(const T* __restrict__ img, T* grad,
  * the 32 gradient magnitudes, and writes 4 maxima and 4 means.
  * Unit spacing (hx = hy = hz = 1) uses the exact integer form of the
  * reference's double arithmetic: each squared term is d^2/4, the sum k/4 is
- * exact, and trunc(sqrt(k/4)) == floor(sqrt(k)) >> 1.  floor(sqrt(k)) comes from
- * isqrt_d2 below. */
-/* floor(sqrt(a + b + c)) for squares a, b, c < 2^32 (k = a + b + c < 2^34).  The float
- * estimate carries a relative error below 2^-21 (three roundings of the sum, one of sqrtf),
- * i.e. an absolute error below 0.06 for sqrt(k) < 2^17, so its truncation is within one of
- * the answer and one exact 64-bit correction step finishes it. */
-__device__ __forceinline__ uint32_t isqrt_d2(uint32_t a, uint32_t b, uint32_t c) {
-    const uint64_t k = (uint64_t)a + b + c;
-    uint32_t s = (uint32_t)sqrtf(((float)a + (float)c) + (float)b);
+ * exact, and trunc(sqrt(k/4)) == floor(sqrt(k)) >> 1.  Both the square root and the
+ * truncation are monotone, so a cell's maximum magnitude is that of its largest k: the
+ * lanes take the integer maximum of k over the cell's voxels and one square root per cell
+ * (the same holds for the double form of other spacings). */
+/* floor(sqrt(k)) for k < 2^35: the hardware square root of the float-converted k carries a
+ * relative error below 2^-22 (one rounding of k, one ulp of v_sqrt_f32), i.e. an absolute
+ * error below 0.05 for sqrt(k) < 2^17.5, so its truncation is within one of the answer and
+ * one exact 64-bit correction step finishes it. */
+__device__ __forceinline__ uint32_t isqrt_k(uint64_t k) {
+    uint32_t s = (uint32_t)__builtin_amdgcn_sqrtf((float)k);
     if ((uint64_t)s * s > k) --s;
     else if ((uint64_t)(s + 1) * (s + 1) <= k) ++s;
     return s;
@@ -174,7 +175,10 @@ __device__ __forceinline__ void grad_ds_vec_body(const uint16_t* __restrict__ im
         const uint4 d0 = ld(x0, zd1), d1 = ld(x1, zd1);
         const int e00m = edge(x0, z0, 0), e00p = edge(x0, z0, 1), e10m = edge(x1, z0, 0), e10p = edge(x1, z0, 1);
         const int e01m = edge(x0, z1, 0), e01p = edge(x0, z1, 1), e11m = edge(x1, z1, 0), e11p = edge(x1, z1, 1);
-        uint32_t gmax[4] = {0, 0, 0, 0};
+        /* per cell (y pair): the largest squared-difference sum (unit spacing) or the largest
+           double magnitude before the square root */
+        uint64_t kmax[4] = {0, 0, 0, 0};
+        double qmax[4] = {0.0, 0.0, 0.0, 0.0};
         auto one_row = [&](const uint4& c, const uint4& lft, const uint4& rgt, const uint4& up, const uint4& dn, int em, int ep) {
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
@@ -188,16 +192,15 @@ __device__ __forceinline__ void grad_ds_vec_body(const uint16_t* __restrict__ im
                 else dyv = cp - cm;
                 const int dxv = u16at(rgt, i) - u16at(lft, i);
                 const int dzv = u16at(dn, i) - u16at(up, i);
-                uint32_t g;
                 if (UNIT) {
                     /* |d| <= 65535: the square of (uint32)d modulo 2^32 is d^2 exactly */
                     const uint32_t ux = (uint32_t)dxv, uy = (uint32_t)dyv, uz = (uint32_t)dzv;
-                    g = isqrt_d2(ux * ux, uy * uy, uz * uz) >> 1;
+                    const uint64_t k = (uint64_t)(ux * ux) + (uint64_t)(uy * uy) + (uint64_t)(uz * uz);
+                    kmax[i >> 1] = max(kmax[i >> 1], k);
                 } else {
                     const double q = sq_term(dxv, h2x) + sq_term(dzv, h2z) + sq_term(dyv, h2y);
-                    g = d2u16(sqrt(q));
+                    qmax[i >> 1] = fmax(qmax[i >> 1], q);
                 }
-                gmax[i >> 1] = max(gmax[i >> 1], g);
             }
         };
         /* rows (x0,z0), (x1,z0), (x0,z1), (x1,z1); left/right neighbours of x1 are x0 / xr1 etc. */
@@ -207,6 +210,9 @@ __device__ __forceinline__ void grad_ds_vec_body(const uint16_t* __restrict__ im
             one_row(c01, l1, hx1 ? c11 : r1, c00, d0, e01m, e01p);
             if (hx1) one_row(c11, c01, r1, c10, d1, e11m, e11p);
         }
+        uint32_t gmax[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) gmax[j] = UNIT ? isqrt_k(kmax[j]) >> 1 : (uint32_t)d2u16(sqrt(qmax[j]));
         const int64_t o = (zd * xd_num + xd) * yd_num + yg * 4;
         if (accum) {
 #pragma unroll

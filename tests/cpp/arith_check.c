@@ -5,10 +5,12 @@
  *   div_odd<D>(x)  = copysign(fma(fma(-q0, D, x), r, q0), x), q0 = x * r, r = RN(1/D)
  *                    must equal x / (float)D for every finite x, D = 1, 3, ..., 13
  *                    (k_grad_lis.hip, running means of the local intensity scale);
- *   isqrt_d2(a,b,c) = floor(sqrt(a + b + c)) from a float estimate and one integer
- *                    correction, for squares of differences of u16 values
+ *   isqrt_k(k)      = floor(sqrt(k)) from a float square root estimate and one integer
+ *                    correction, for sums of three squares of differences of u16 values
  *                    (k_grad_lis.hip, gradient magnitude), against the reference's
- *                    trunc(sqrt(double)) form.
+ *                    trunc(sqrt(double)) form; the estimate is taken one ulp low, exact and
+ *                    one ulp high (the hardware square root is within one ulp), and the
+ *                    maximum over a cell is taken before the root (both maps are monotone).
  * Usage: arith_check [stride]   -- tests every stride-th float bit pattern (1 = all 2^32).
  */
 #include <math.h>
@@ -23,13 +25,16 @@ static float div_odd(float x, float d, float r) {
     return copysignf(fmaf(e, r, q0), x);
 }
 
-static uint32_t isqrt_d2(uint32_t a, uint32_t b, uint32_t c) {
-    const uint64_t k = (uint64_t)a + b + c;
-    uint32_t s = (uint32_t)sqrtf(((float)a + (float)c) + (float)b);
+static uint32_t isqrt_k(uint64_t k, int ulp) {
+    float r = sqrtf((float)k);
+    if (ulp < 0) r = nextafterf(r, 0.0f);
+    if (ulp > 0) r = nextafterf(r, INFINITY);
+    uint32_t s = (uint32_t)r;
     if ((uint64_t)s * s > k) --s;
     else if ((uint64_t)(s + 1) * (s + 1) <= k) ++s;
     return s;
 }
+static uint32_t isqrt_d2(uint32_t a, uint32_t b, uint32_t c, int ulp) { return isqrt_k((uint64_t)a + b + c, ulp); }
 
 static uint64_t rng = 88172645463325252ull;
 static uint32_t next32(void) {
@@ -84,13 +89,15 @@ int main(int argc, char** argv) {
             dz = (int)(next32() % 4);
         }
         const uint32_t ux = (uint32_t)dx, uy = (uint32_t)dy, uz = (uint32_t)dz;
-        const uint32_t g = isqrt_d2(ux * ux, uy * uy, uz * uz) >> 1;
         const double q = (double)dx * dx / 4.0 + (double)dz * dz / 4.0 + (double)dy * dy / 4.0;
         const uint32_t ref = (uint32_t)sqrt(q);
-        ++n2;
-        if (g != ref) {
-            if (bad2 < 5) printf("isqrt_d2 %d %d %d: %u != %u\n", dx, dy, dz, g, ref);
-            ++bad2;
+        for (int ulp = -1; ulp <= 1; ++ulp) {
+            const uint32_t g = isqrt_d2(ux * ux, uy * uy, uz * uz, ulp) >> 1;
+            ++n2;
+            if (g != ref) {
+                if (bad2 < 5) printf("isqrt_k %d %d %d (ulp %d): %u != %u\n", dx, dy, dz, ulp, g, ref);
+                ++bad2;
+            }
         }
     }
     /* the extremes: every square sum of three u16 differences from 0 to 65535 */
@@ -99,11 +106,27 @@ int main(int argc, char** argv) {
         for (int b = 0; b < 4; ++b)
             for (int c = 0; c < 4; ++c) {
                 const uint32_t ux = (uint32_t)ext[a], uy = (uint32_t)ext[b], uz = (uint32_t)ext[c];
-                const uint32_t g = isqrt_d2(ux * ux, uy * uy, uz * uz) >> 1;
+                const uint32_t g = isqrt_d2(ux * ux, uy * uy, uz * uz, 0) >> 1;
                 const double q = (double)ext[a] * ext[a] / 4.0 + (double)ext[c] * ext[c] / 4.0 + (double)ext[b] * ext[b] / 4.0;
                 ++n2;
                 if (g != (uint32_t)sqrt(q)) ++bad2;
             }
-    printf("isqrt_d2: %llu values, %llu mismatches\n", (unsigned long long)n2, (unsigned long long)bad2);
+    /* a cell's maximum: the root of the largest sum equals the largest root (8 random voxels) */
+    for (uint64_t i = 0; i < 2000000ull; ++i) {
+        uint64_t kmax = 0;
+        uint32_t gmax = 0;
+        for (int v = 0; v < 8; ++v) {
+            const int dx = (int)(next32() % 131071u) - 65535, dy = (int)(next32() % 2001u) - 1000, dz = (int)(next32() % 257u) - 128;
+            const uint32_t ux = (uint32_t)dx, uy = (uint32_t)dy, uz = (uint32_t)dz;
+            const uint64_t k = (uint64_t)(ux * ux) + (uint64_t)(uy * uy) + (uint64_t)(uz * uz);
+            kmax = k > kmax ? k : kmax;
+            const double q = (double)dx * dx / 4.0 + (double)dz * dz / 4.0 + (double)dy * dy / 4.0;
+            const uint32_t g = (uint32_t)sqrt(q);
+            gmax = g > gmax ? g : gmax;
+        }
+        ++n2;
+        if ((isqrt_k(kmax, 0) >> 1) != gmax) ++bad2;
+    }
+    printf("isqrt_k: %llu values, %llu mismatches\n", (unsigned long long)n2, (unsigned long long)bad2);
     return (bad || bad2) ? 1 : 0;
 }
