@@ -971,28 +971,48 @@ __device__ __forceinline__ void iir_y_lds_ck_body(const uint16_t* in, uint16_t* 
     const float t2q = Val<uint16_t>::store(t2), t1q = Val<uint16_t>::store(t1);
     /* causal recursion over one LDS row chunk from state (s1, s2); with `keep` the outputs
        replace the row's samples */
-    auto causal = [&](uint16_t* row, int64_t ch, float& s1, float& s2, bool keep) {
+    /* the first chunk holds samples 0 and 1, which take the boundary values; the others run
+       without per-sample tests.  e collects the range check (the re-computing sweep passes a
+       scratch word: its values were checked by the first sweep) */
+    auto causal = [&](uint16_t* row, int64_t ch, float& s1, float& s2, bool keep, int& e) {
+        if (ch == 0) {
 #pragma unroll
-        for (int q = 0; q < CH / 8; ++q) {
-            uint4 v = *(uint4*)(row + q * 8);
-            uint4 o = make_uint4(0, 0, 0, 0);
+            for (int q = 0; q < CH / 8; ++q) {
+                uint4 v = *(uint4*)(row + q * 8);
+                uint4 o = make_uint4(0, 0, 0, 0);
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int64_t k = ch * CH + q * 8 + i;
-                float r;
-                if (k == 0) {
-                    r = t2q;
-                } else if (k == 1) {
-                    r = t1q;
-                } else {
+                for (int i = 0; i < 8; ++i) {
+                    const int k = q * 8 + i;
+                    float r;
+                    if (k == 0) {
+                        r = t2q;
+                    } else if (k == 1) {
+                        r = t1q;
+                    } else {
+                        const float t = s1 * b1 + s2 * b2 + vin<uint16_t>(v, i, off, has_off);
+                        r = Val<uint16_t>::rnd(t, e);
+                        s2 = s1;
+                        s1 = t;
+                    }
+                    Vec16<uint16_t>::set_r(o, i, r);
+                }
+                if (keep) *(uint4*)(row + q * 8) = o;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < CH / 8; ++q) {
+                uint4 v = *(uint4*)(row + q * 8);
+                uint4 o = make_uint4(0, 0, 0, 0);
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
                     const float t = s1 * b1 + s2 * b2 + vin<uint16_t>(v, i, off, has_off);
-                    r = Val<uint16_t>::rnd(t, err);
+                    const float r = Val<uint16_t>::rnd(t, e);
                     s2 = s1;
                     s1 = t;
+                    Vec16<uint16_t>::set_r(o, i, r);
                 }
-                Vec16<uint16_t>::set_r(o, i, r);
+                if (keep) *(uint4*)(row + q * 8) = o;
             }
-            if (keep) *(uint4*)(row + q * 8) = o;
         }
     };
 
@@ -1026,7 +1046,7 @@ __device__ __forceinline__ void iir_y_lds_ck_body(const uint16_t* in, uint16_t* 
             if (pyr) pyr_chunk(ch);
             if (valid) {
                 if (ch > 0) ck[(ch - 1) * n_lines + me] = make_float2(s1, s2);
-                causal(&tile[lane * PITCH], ch, s1, s2, false);
+                causal(&tile[lane * PITCH], ch, s1, s2, false, err);
             }
         }
         if (mm) {
@@ -1062,28 +1082,46 @@ __device__ __forceinline__ void iir_y_lds_ck_body(const uint16_t* in, uint16_t* 
                 s1 = st.x;
                 s2 = st.y;
             }
-            causal(row, ch, s1, s2, true);
+            int checked = 0;
+            causal(row, ch, s1, s2, true, checked);
+            if (ch == nch - 1) {
+                /* the last chunk holds samples y_num-2 and y_num-1, which take the boundary values */
 #pragma unroll
-            for (int q = CH / 8 - 1; q >= 0; --q) {
-                uint4 v = *(uint4*)(row + q * 8);
-                uint4 o = make_uint4(0, 0, 0, 0);
+                for (int q = CH / 8 - 1; q >= 0; --q) {
+                    uint4 v = *(uint4*)(row + q * 8);
+                    uint4 o = make_uint4(0, 0, 0, 0);
 #pragma unroll
-                for (int i = 7; i >= 0; --i) {
-                    const int64_t k = ch * CH + q * 8 + i;
-                    float r;
-                    if (k == y_num - 1) {
-                        r = bv_m1;
-                    } else if (k == y_num - 2) {
-                        r = bv_m2;
-                    } else {
+                    for (int i = 7; i >= 0; --i) {
+                        const int k = q * 8 + i;
+                        float r;
+                        if (k == CH - 1) {
+                            r = bv_m1;
+                        } else if (k == CH - 2) {
+                            r = bv_m2;
+                        } else {
+                            const float t = r1 * b1 + r2 * b2 + Vec16<uint16_t>::get(v, i);
+                            r = Val<uint16_t>::rnd(t * nf, err);
+                            r2 = r1;
+                            r1 = t;
+                        }
+                        Vec16<uint16_t>::set_r(o, i, r);
+                    }
+                    *(uint4*)(row + q * 8) = o;
+                }
+            } else {
+#pragma unroll
+                for (int q = CH / 8 - 1; q >= 0; --q) {
+                    uint4 v = *(uint4*)(row + q * 8);
+                    uint4 o = make_uint4(0, 0, 0, 0);
+#pragma unroll
+                    for (int i = 7; i >= 0; --i) {
                         const float t = r1 * b1 + r2 * b2 + Vec16<uint16_t>::get(v, i);
-                        r = Val<uint16_t>::rnd(t * nf, err);
+                        Vec16<uint16_t>::set_r(o, i, Val<uint16_t>::rnd(t * nf, err));
                         r2 = r1;
                         r1 = t;
                     }
-                    Vec16<uint16_t>::set_r(o, i, r);
+                    *(uint4*)(row + q * 8) = o;
                 }
-                *(uint4*)(row + q * 8) = o;
             }
         }
         __syncthreads();
