@@ -121,6 +121,8 @@ __device__ __forceinline__ void pull_level8_body(uint8_t* P, Dims d, uint8_t* Pc
     const int nzc = (zn + PULL_ZC - 1) / PULL_ZC;
     const int64_t total = (int64_t)nwr * xn * nzc;
     const bool aligned8 = (yn & 7) == 0 && ((uintptr_t)P & 7) == 0;
+    /* parent rows whose words never straddle two rows */
+    const bool par_words = has_parent && (dp.y & 3) == 0 && ((uintptr_t)Pp & 3) == 0;
     for (int64_t i = t * blockDim.x + threadIdx.x; i < total; i += G * blockDim.x) {
         const int w = (int)(i % nwr);
         const int64_t r = i / nwr;
@@ -184,6 +186,28 @@ __device__ __forceinline__ void pull_level8_body(uint8_t* P, Dims d, uint8_t* Pc
                         if (y0 + j < yn && nb[j] != cur[1 + j]) row[y0 + j] = nb[j];
                 }
             }
+            /* fill_parent for the lane's 4 parent cells with one word: the parent level holds only
+               EMPTY, SEED and ASCENDANT now (its seeds, and ASCENDANT from this pass), so OR-ing
+               ASCENDANT's bits into the non-SEED bytes wanted gives each the reference's value, in
+               any order between lanes that share the word */
+            const bool word_parent = has_parent && par_words;
+            if (word_parent) {
+                uint32_t want = 0;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const uint8_t nw = nb[j];
+                    if (y0 + j < yn && (nw == SEED || nw == PROP || nw == ASC)) want |= 0xFFu << (8 * (j >> 1));
+                }
+                if (want) {
+                    uint32_t* pw = (uint32_t*)&Pp[((int64_t)(z / 2) * dp.x + (x / 2)) * dp.y + (y0 / 2)];
+                    const uint32_t cur4 = *pw;
+                    uint32_t mask = 0;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        if (((want >> (8 * q)) & 0xFFu) && ((cur4 >> (8 * q)) & 0xFFu) != SEED) mask |= (uint32_t)ASC << (8 * q);
+                    if (mask) atomicOr(pw, mask);
+                }
+            }
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const int y = y0 + j;
@@ -202,7 +226,7 @@ __device__ __forceinline__ void pull_level8_body(uint8_t* P, Dims d, uint8_t* Pc
                             }
                 }
                 /* fill_parent */
-                if (has_parent && (nw == SEED || nw == PROP || nw == ASC)) {
+                if (!word_parent && has_parent && (nw == SEED || nw == PROP || nw == ASC)) {
                     uint8_t* pp = &Pp[((int64_t)(z / 2) * dp.x + (x / 2)) * dp.y + (y / 2)];
                     if (*pp != SEED) *pp = ASC;
                 }
