@@ -11,6 +11,7 @@
  *   APRFilter::convolve(_pencil)         numerics/APRFilter.hpp:67-149
  *   APRNumerics::richardson_lucy         numerics/APRNumerics.hpp:126-134
  *   APRReconstruction::reconstruct_constant  numerics/APRReconstruction.hpp:80, :297
+ *   APRReconstruction::reconstruct_level     numerics/APRReconstruction.hpp:131-171
  *
  * Error conventions follow the reference: get_apr returns false on bad
  * dimensions (APRConverter.hpp:86-91), u16 smoothing overflow throws
@@ -299,6 +300,13 @@ void reconstruct_constant(APR& apr, PixelData<U>& img, const ParticleData<T>& pa
     img.init((apr.org_dims(0) + s - 1) / s, (apr.org_dims(1) + s - 1) / s, (apr.org_dims(2) + s - 1) / s);
     detail::check(aprh_reconstruct_constant(apr.context().get(), apr.get(), parts.data.data(), detail::dtype<T>::code, level_delta,
                                             img.mesh.data(), detail::dtype<U>::code));
+}
+
+/* reconstruct_level (APRReconstruction.hpp:131-171): each pixel holds the level of its particle */
+template <typename U>
+void reconstruct_level(APR& apr, PixelData<U>& img) {
+    img.init(apr.org_dims(0), apr.org_dims(1), apr.org_dims(2));
+    detail::check(aprh_reconstruct_level(apr.context().get(), apr.get(), img.mesh.data(), detail::dtype<U>::code));
 }
 }  // namespace APRReconstruction
 

@@ -210,6 +210,10 @@ int aprh_richardson_lucy(aprh_context* ctx, aprh_apr* apr, const void* parts_in,
    dimensions of level l_max + level_delta, (y, x, z) with y fastest. */
 int aprh_reconstruct_constant(aprh_context* ctx, aprh_apr* apr, const void* parts, int parts_dtype, int level_delta, void* image,
                               int image_dtype);
+/* APRReconstruction::reconstruct_level (APRReconstruction.hpp:131-171): each pixel of the full
+   volume (y, x, z, y fastest) takes the level of the particle covering it; image_dtype U16 or
+   F32 (the reference writes any pixel type) */
+int aprh_reconstruct_level(aprh_context* ctx, aprh_apr* apr, void* image, int image_dtype);
 
 /* ---- learned Richardson-Lucy warm start (no reference counterpart) ----
    A 3-D CNN: four valid 3x3x3 convolutions 1 -> 16 -> 16 -> 16 -> 1 channels, ReLU after

@@ -138,6 +138,7 @@ def lib():
     L.aprh_richardson_lucy.argtypes = [vp, vp, vp, C.c_int, vp, vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int,
                                        C.c_int, C.c_int, C.c_double, C.POINTER(AprhRLStats)]
     L.aprh_reconstruct_constant.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp, C.c_int]
+    L.aprh_reconstruct_level.argtypes = [vp, vp, vp, C.c_int]
     L.aprh_synth_deconv_device.argtypes = [vp, vp, vp, i64, i64, i64, C.c_uint64, i64, C.c_float, C.c_float, C.c_float,
                                            C.c_float, C.c_float, vp, C.c_int32]
     L.aprh_sample_image_f32.argtypes = [vp, vp, vp, vp]
@@ -725,7 +726,8 @@ class APRNumerics:
 
 
 class APRReconstruction:
-    """APRReconstruction::reconstruct_constant (APRReconstruction.hpp:80-120, :297-394)."""
+    """APRReconstruction::reconstruct_constant (APRReconstruction.hpp:80-120, :297-394) and
+    reconstruct_level (:131-171)."""
 
     @staticmethod
     def reconstruct_constant(apr, parts, level_delta=0, out_dtype=None, out=None, dtype=None):
@@ -739,6 +741,19 @@ class APRReconstruction:
             s = 2 ** (-level_delta)
             out = np.zeros((-(-z // s), -(-x // s), -(-y // s)), odt)
         _check(lib().aprh_reconstruct_constant(apr.ctx.h, apr.h, _ptr(parts), code, int(level_delta), _ptr(out), ocode))
+        return out
+
+
+    @staticmethod
+    def reconstruct_level(apr, out_dtype=np.uint16, out=None):
+        """APRReconstruction::reconstruct_level (APRReconstruction.hpp:131-171): the level of the
+        particle covering each pixel, [z, x, y] of the original size."""
+        odt = np.dtype(out_dtype)
+        ocode = _dtype_of(None, str(odt))
+        if out is None:
+            y, x, z = apr.org_dims
+            out = np.zeros((z, x, y), odt)
+        _check(lib().aprh_reconstruct_level(apr.ctx.h, apr.h, _ptr(out), ocode))
         return out
 
 

@@ -281,4 +281,16 @@ int aprh_reconstruct_constant(aprh_context* ctx, aprh_apr* a, const void* parts,
     APRH_CATCH
 }
 
+int aprh_reconstruct_level(aprh_context* ctx, aprh_apr* a, void* image, int image_dtype) {
+    APRH_TRY
+    check_apr(ctx, a);
+    if (!image) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    const size_t ies = dtype_size(image_dtype);
+    const size_t n = (size_t)(a->y_num[a->l_max] * a->x_num[a->l_max] * a->z_num[a->l_max]);
+    DevOut out(ctx, image, n * ies);
+    reconstruct_level(ctx, *a, out.ptr, image_dtype);
+    out.finish(ctx);
+    APRH_CATCH
+}
+
 }  // extern "C"

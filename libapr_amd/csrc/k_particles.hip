@@ -28,6 +28,7 @@
 #include <cmath>
 
 #include "aprh_internal.hpp"
+#include "tiles.hpp"
 
 namespace aprh {
 
@@ -740,6 +741,43 @@ void reconstruct(aprh_context* ctx, aprh_apr& a, const T* parts, int level, U* i
         prev = cur;
     }
     HIP_CHECK(hipStreamSynchronize(st));
+}
+
+namespace {
+/* value v over n u16 elements (grid-stride over a virtual grid of G blocks) */
+struct FillU16 {
+    struct Args {
+        uint16_t* p;
+        uint64_t n;
+        uint16_t v;
+        int64_t G;
+    };
+    static constexpr int THREADS = 256;
+    static const char* name() { return "fill_u16"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) {
+        for (uint64_t i = (uint64_t)t * blockDim.x + threadIdx.x; i < a.n; i += (uint64_t)a.G * blockDim.x) a.p[i] = a.v;
+    }
+};
+}  // namespace
+
+/* APRReconstruction::reconstruct_level (APRReconstruction.hpp:131-171): every pixel takes the
+   level of the particle covering it, which is the constant reconstruction of particles whose
+   values are their levels (particles are stored level by level, so each level is one range) */
+void reconstruct_level(aprh_context* ctx, aprh_apr& a, void* img, int idt) {
+    Buf lev(ctx->pool, std::max<uint64_t>(a.total, 1) * 2);
+    for (int l = a.l_min; l <= a.l_max; ++l) {
+        const uint64_t r0 = a.level_xz_vec[l], nrows = (uint64_t)(a.x_num[l] * a.z_num[l]);
+        if (nrows == 0) continue;
+        uint64_t b = 0, e = 0;
+        if (r0 > 0) HIP_CHECK(hipMemcpyAsync(&b, a.xz_end.as<uint64_t>() + r0 - 1, 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipMemcpyAsync(&e, a.xz_end.as<uint64_t>() + r0 + nrows - 1, 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        if (e > b) {
+            const int64_t G = stride_tiles((int64_t)(e - b), 256 * 4);
+            launch_tiles<FillU16>(ctx, {lev.as<uint16_t>() + b, e - b, (uint16_t)l, G}, G);
+        }
+    }
+    reconstruct_any(ctx, a, lev.ptr(), APRH_DTYPE_U16, a.l_max, img, idt);
 }
 
 void reconstruct_any(aprh_context* ctx, aprh_apr& a, const void* parts, int pdt, int level, void* img, int idt) {

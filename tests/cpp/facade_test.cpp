@@ -124,6 +124,29 @@ int main() {
         CHECK(same_bytes(rec.mesh, orec.data(), orec.size()));
     }
 
+    // level image: the constant reconstruction of particles valued by their levels
+    {
+        std::vector<uint64_t> lxz, xze;
+        std::vector<uint16_t> yv;
+        apr.linear_access(lxz, xze, yv);
+        std::vector<uint16_t> lev(P, 0);
+        for (size_t l = 1; l + 1 < lxz.size(); ++l) {
+            if (lxz[l + 1] == lxz[l]) continue;
+            const uint64_t b = lxz[l] > 0 ? xze[lxz[l] - 1] : 0, e = xze[lxz[l + 1] - 1];
+            for (uint64_t i = b; i < e; ++i) lev[i] = (uint16_t)l;
+        }
+        PixelData<uint16_t> rl;
+        APRReconstruction::reconstruct_level(apr, rl);
+        std::vector<uint16_t> orl((size_t)(y * x * z));
+        orc_reconstruct(o, lev.data(), 0, 0, orl.data(), 0);
+        CHECK(same_bytes(rl.mesh, orl.data(), orl.size()));
+        PixelData<float> rlf;
+        APRReconstruction::reconstruct_level(apr, rlf);
+        bool eq = rlf.mesh.size() == orl.size();
+        for (size_t i = 0; eq && i < orl.size(); ++i) eq = rlf.mesh[i] == (float)orl[i];
+        CHECK(eq);
+    }
+
     // error conventions
     PixelData<uint16_t> empty;
     APR a2;

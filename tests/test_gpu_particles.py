@@ -1,6 +1,6 @@
 """GPU parity of the particle operations against the CPU oracle, through the C-ABI:
-tree structure, fill_tree_mean, convolve_pencil / convolve, Richardson-Lucy and
-reconstruct_constant.
+tree structure, fill_tree_mean, convolve_pencil / convolve, Richardson-Lucy,
+reconstruct_constant and reconstruct_level.
 
 The GPU sums in the reference's order (tree children: particles then tree nodes,
 (z, x, y); convolution taps: (z, x, y)), so every float result is bit-equal to the
@@ -238,3 +238,39 @@ def test_richardson_lucy_separable_mode(oracle, sep_ctx):
     r = _rel(ug, uo).max()
     print(f"RL 3 iterations, separable vs reference order: max rel {r:.2e}")
     assert st["iterations"] == 3 and r < 1e-5
+
+
+def _level_image(o):
+    """reconstruct_level restated from the structure (APRReconstruction.hpp:131-171): the
+    particle of level l at row (z, x), position y covers the block [z, x, y] * 2^(l_max - l)"""
+    y, x, z = o.dims
+    img = np.zeros((z, x, y), np.uint16)
+    lxz, xze, yv = o.level_xz_vec, o.xz_end_vec, o.y_vec
+    for lev in range(o.l_min, o.l_max + 1):
+        s = 2 ** (o.l_max - lev)
+        xl, zl = -(-x // s), -(-z // s)
+        for zi in range(zl):
+            for xi in range(xl):
+                r = int(lxz[lev]) + zi * xl + xi
+                b = int(xze[r - 1]) if r > 0 else 0
+                for yy in yv[b:int(xze[r])]:
+                    img[zi * s:(zi + 1) * s, xi * s:(xi + 1) * s, int(yy) * s:(int(yy) + 1) * s] = lev
+    return img
+
+
+@pytest.mark.parametrize("shape,nb,seed", CASES[:3])
+def test_reconstruct_level(oracle, ctx, shape, nb, seed):
+    img, o, g = _pair(oracle, ctx, shape, nb, seed)
+    expect = _level_image(o)
+    # the same image as the constant reconstruction of particles valued by their levels
+    lev = np.zeros(o.total, np.uint16)
+    for l in range(o.l_min, o.l_max + 1):
+        r0 = int(o.level_xz_vec[l])
+        b = int(o.xz_end_vec[r0 - 1]) if r0 > 0 else 0
+        e = int(o.xz_end_vec[int(o.level_xz_vec[l + 1]) - 1])
+        lev[b:e] = l
+    assert np.array_equal(o.reconstruct(lev, 0, np.uint16), expect)
+    assert int(expect.min()) >= o.l_min and int(expect.max()) == o.l_max
+    for out_dtype in (np.uint16, np.float32):
+        got = A.APRReconstruction.reconstruct_level(g, out_dtype)
+        assert got.dtype == out_dtype and np.array_equal(got, expect.astype(out_dtype))
