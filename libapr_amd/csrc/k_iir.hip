@@ -782,6 +782,22 @@ __device__ __forceinline__ void iir_outer_ck_body(T* img, int64_t y_num, int64_t
         }
         base[j * sv] = o;
     };
+    /* two anti-causal steps (j, then j - 1) with the states swapping roles instead of moving */
+    auto back2 = [&](const uint4& f, const uint4& g, int64_t j) {
+        uint4 o = make_uint4(0, 0, 0, 0), u = o;
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            v4[i] = (Vec16<T>::get(f, i) + b1 * v3[i]) + b2 * v4[i];
+            Vec16<T>::set_r(o, i, Val<T>::rnd(v4[i] * nf, err));
+        }
+        base[j * sv] = o;
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            v3[i] = (Vec16<T>::get(g, i) + b1 * v4[i]) + b2 * v3[i];
+            Vec16<T>::set_r(u, i, Val<T>::rnd(v3[i] * nf, err));
+        }
+        base[(j - 1) * sv] = u;
+    };
     /* causal outputs of chunk `buf` from state (p1, p2), then the anti-causal sweep over it */
     auto chunk = [&](uint4* buf, int64_t j0, int cnt) {
 #pragma unroll
@@ -972,9 +988,13 @@ __device__ __forceinline__ void iir_y_lds_ck_body(const uint16_t* in, uint16_t* 
     /* causal recursion over one LDS row chunk from state (s1, s2); with `keep` the outputs
        replace the row's samples */
     /* the first chunk holds samples 0 and 1, which take the boundary values; the others run
-       without per-sample tests.  e collects the range check (the re-computing sweep passes a
-       scratch word: its values were checked by the first sweep) */
-    auto causal = [&](uint16_t* row, int64_t ch, float& s1, float& s2, bool keep, int& e) {
+       without per-sample tests.  Without `keep` (the first sweep) nothing is rounded: the
+       recursion carries the unrounded values, and the range check of the rounded outputs
+       becomes a running min / max of the unrounded ones, since roundf(t) lies in [0, 65535]
+       exactly when -0.5 < t < 65535.5.  The re-computing sweep rounds the same values again
+       and skips the check. */
+    auto causal = [&](uint16_t* row, int64_t ch, float& s1, float& s2, bool keep, float& lo, float& hi) {
+        int e = 0;
         if (ch == 0) {
 #pragma unroll
             for (int q = 0; q < CH / 8; ++q) {
@@ -990,11 +1010,17 @@ __device__ __forceinline__ void iir_y_lds_ck_body(const uint16_t* in, uint16_t* 
                         r = t1q;
                     } else {
                         const float t = s1 * b1 + s2 * b2 + vin<uint16_t>(v, i, off, has_off);
-                        r = Val<uint16_t>::rnd(t, e);
+                        if (keep) {
+                            r = Val<uint16_t>::rnd(t, e);
+                        } else {
+                            lo = fminf(lo, t);
+                            hi = fmaxf(hi, t);
+                            r = 0.0f;
+                        }
                         s2 = s1;
                         s1 = t;
                     }
-                    Vec16<uint16_t>::set_r(o, i, r);
+                    if (keep) Vec16<uint16_t>::set_r(o, i, r);
                 }
                 if (keep) *(uint4*)(row + q * 8) = o;
             }
@@ -1006,10 +1032,14 @@ __device__ __forceinline__ void iir_y_lds_ck_body(const uint16_t* in, uint16_t* 
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
                     const float t = s1 * b1 + s2 * b2 + vin<uint16_t>(v, i, off, has_off);
-                    const float r = Val<uint16_t>::rnd(t, e);
+                    if (keep) {
+                        Vec16<uint16_t>::set_r(o, i, Val<uint16_t>::rnd(t, e));
+                    } else {
+                        lo = fminf(lo, t);
+                        hi = fmaxf(hi, t);
+                    }
                     s2 = s1;
                     s1 = t;
-                    Vec16<uint16_t>::set_r(o, i, r);
                 }
                 if (keep) *(uint4*)(row + q * 8) = o;
             }
@@ -1019,7 +1049,7 @@ __device__ __forceinline__ void iir_y_lds_ck_body(const uint16_t* in, uint16_t* 
     /* causal sweep: states only; with mm, also the (min, max) of the raw samples (packed u16
        min / max over the loaded vectors; lines past the end load zeros, so they are skipped) */
     {
-        float s1 = t1, s2 = t2;
+        float s1 = t1, s2 = t2, tlo = 0.0f, thi = 0.0f;
         uint4 nxt[8];
         typedef unsigned short us2 __attribute__((ext_vector_type(2)));
         us2 lo = {65535, 65535}, hi = {0, 0};
@@ -1046,9 +1076,10 @@ __device__ __forceinline__ void iir_y_lds_ck_body(const uint16_t* in, uint16_t* 
             if (pyr) pyr_chunk(ch);
             if (valid) {
                 if (ch > 0) ck[(ch - 1) * n_lines + me] = make_float2(s1, s2);
-                causal(&tile[lane * PITCH], ch, s1, s2, false, err);
+                causal(&tile[lane * PITCH], ch, s1, s2, false, tlo, thi);
             }
         }
+        if (!(tlo > -0.5f && thi < 65535.5f)) err |= 0x10000;
         if (mm) {
             uint32_t l = min((uint32_t)lo.x, (uint32_t)lo.y), h = max((uint32_t)hi.x, (uint32_t)hi.y);
             for (int o = 32; o > 0; o >>= 1) {
@@ -1082,8 +1113,8 @@ __device__ __forceinline__ void iir_y_lds_ck_body(const uint16_t* in, uint16_t* 
                 s1 = st.x;
                 s2 = st.y;
             }
-            int checked = 0;
-            causal(row, ch, s1, s2, true, checked);
+            float unused_lo = 0.0f, unused_hi = 0.0f;
+            causal(row, ch, s1, s2, true, unused_lo, unused_hi);
             if (ch == nch - 1) {
                 /* the last chunk holds samples y_num-2 and y_num-1, which take the boundary values */
 #pragma unroll

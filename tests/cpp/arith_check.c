@@ -128,5 +128,25 @@ int main(int argc, char** argv) {
         if ((isqrt_k(kmax, 0) >> 1) != gmax) ++bad2;
     }
     printf("isqrt_k: %llu values, %llu mismatches\n", (unsigned long long)n2, (unsigned long long)bad2);
-    return (bad || bad2) ? 1 : 0;
+    /* range check of the y smoothing's first sweep: roundf(t) lies in [0, 65535] exactly when
+       -0.5 < t < 65535.5 (every float near both bounds, and every stride-th bit pattern) */
+    uint64_t n3 = 0, bad3 = 0;
+    const float lim[2][2] = {{-0.75f, -0.25f}, {65533.0f, 65538.0f}};
+    for (int r = 0; r < 2; ++r)
+        for (float t = lim[r][0]; t <= lim[r][1]; t = nextafterf(t, INFINITY)) {
+            const float q = roundf(t);
+            ++n3;
+            if ((q >= 0.0f && q <= 65535.0f) != (t > -0.5f && t < 65535.5f)) ++bad3;
+        }
+    for (uint64_t b = 0; b < (1ull << 32); b += stride) {
+        const uint32_t u = (uint32_t)b;
+        float t;
+        memcpy(&t, &u, 4);
+        if (t != t) continue;
+        const float q = roundf(t);
+        ++n3;
+        if ((q >= 0.0f && q <= 65535.0f) != (t > -0.5f && t < 65535.5f)) ++bad3;
+    }
+    printf("u16 range: %llu values, %llu mismatches\n", (unsigned long long)n3, (unsigned long long)bad3);
+    return (bad || bad2 || bad3) ? 1 : 0;
 }

@@ -1,7 +1,8 @@
 """Host checks of the exact-arithmetic shortcuts in the HIP kernels (division by the odd mean
 window via multiply and FMA correction; integer square root of the gradient from a float
-estimate): tests/cpp/arith_check.c restates both with the same operations and compares them
-with the plain IEEE division and the reference's double sqrt + truncation."""
+estimate; the u16 range test of the y smoothing as bounds on the unrounded value):
+tests/cpp/arith_check.c restates them with the same operations and compares them with the
+plain IEEE division, the reference's double sqrt + truncation and roundf."""
 import os
 import shutil
 import subprocess
@@ -22,4 +23,5 @@ def test_exact_arithmetic_shortcuts():
         r = subprocess.run([exe, "97"], capture_output=True, text=True, timeout=300)
     print(r.stdout)
     assert r.returncode == 0, r.stdout
-    assert " 0 mismatches" in r.stdout.splitlines()[0] and " 0 mismatches" in r.stdout.splitlines()[1]
+    lines = r.stdout.splitlines()
+    assert len(lines) == 3 and all(" 0 mismatches" in l for l in lines), r.stdout
