@@ -12,6 +12,7 @@
  *   APRNumerics::richardson_lucy         numerics/APRNumerics.hpp:126-134
  *   APRReconstruction::reconstruct_constant  numerics/APRReconstruction.hpp:80, :297
  *   APRReconstruction::reconstruct_level     numerics/APRReconstruction.hpp:131-171
+ *   ReconPatch and the patch reconstructions numerics/APRReconstruction.hpp:20-66, :186-196, :296-490
  *
  * Error conventions follow the reference: get_apr returns false on bad
  * dimensions (APRConverter.hpp:86-91), u16 smoothing overflow throws
@@ -23,6 +24,7 @@
 #define APR_AMD_HPP
 
 #include <algorithm>
+#include <iostream>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -292,6 +294,23 @@ aprh_rl_stats richardson_lucy(APR& apr, ParticleData<In>& in, ParticleData<float
 }
 }  // namespace APRNumerics
 
+/* ReconPatch (APRReconstruction.hpp:20-66) */
+struct ReconPatch {
+    int x_begin = 0, x_end = -1, y_begin = 0, y_end = -1, z_begin = 0, z_end = -1;
+    int level_delta = 0;
+    size_t size() const { return (size_t)(z_end - z_begin) * (x_end - x_begin) * (y_end - y_begin); }
+    /* clamps to the image at level l_max + level_delta; false (and a message) when a range is empty */
+    bool check_limits(APR& apr) {
+        aprh_patch p{x_begin, x_end, y_begin, y_end, z_begin, z_end, level_delta};
+        if (aprh_patch_check_limits(apr.get(), &p) != APRH_OK) {
+            std::wcerr << "ReconPatch expects begin < end in all dimensions" << std::endl;
+            return false;
+        }
+        x_begin = p.x_begin, x_end = p.x_end, y_begin = p.y_begin, y_end = p.y_end, z_begin = p.z_begin, z_end = p.z_end;
+        return true;
+    }
+};
+
 namespace APRReconstruction {
 /* reconstruct_constant (APRReconstruction.hpp:80-120); level_delta < 0 as the patch form (:297-394) */
 template <typename U, typename T>
@@ -307,6 +326,32 @@ template <typename U>
 void reconstruct_level(APR& apr, PixelData<U>& img) {
     img.init(apr.org_dims(0), apr.org_dims(1), apr.org_dims(2));
     detail::check(aprh_reconstruct_level(apr.context().get(), apr.get(), img.mesh.data(), detail::dtype<U>::code));
+}
+
+/* reconstruct_constant over a patch (APRReconstruction.hpp:186-193, :296-394): as the reference,
+   an invalid patch prints a message and leaves img untouched */
+template <typename U, typename T>
+void reconstruct_constant(APR& apr, PixelData<U>& img, const ParticleData<T>& parts, ReconPatch& patch) {
+    if (!patch.check_limits(apr)) {
+        std::wcerr << "APRReconstruction::reconstruct_constant: invalid patch size - exiting" << std::endl;
+        return;
+    }
+    img.init(patch.y_end - patch.y_begin, patch.x_end - patch.x_begin, patch.z_end - patch.z_begin);
+    const aprh_patch p{patch.x_begin, patch.x_end, patch.y_begin, patch.y_end, patch.z_begin, patch.z_end, patch.level_delta};
+    detail::check(aprh_reconstruct_constant_patch(apr.context().get(), apr.get(), parts.data.data(), detail::dtype<T>::code, &p,
+                                                  img.mesh.data(), detail::dtype<U>::code));
+}
+
+/* reconstruct_level over a patch (APRReconstruction.hpp:397-490) */
+template <typename U>
+void reconstruct_level(APR& apr, PixelData<U>& img, ReconPatch& patch) {
+    if (!patch.check_limits(apr)) {
+        std::wcerr << "APRReconstruction::reconstruct_level: invalid patch size - exiting" << std::endl;
+        return;
+    }
+    img.init(patch.y_end - patch.y_begin, patch.x_end - patch.x_begin, patch.z_end - patch.z_begin);
+    const aprh_patch p{patch.x_begin, patch.x_end, patch.y_begin, patch.y_end, patch.z_begin, patch.z_end, patch.level_delta};
+    detail::check(aprh_reconstruct_level_patch(apr.context().get(), apr.get(), &p, img.mesh.data(), detail::dtype<U>::code));
 }
 }  // namespace APRReconstruction
 

@@ -214,6 +214,22 @@ int aprh_reconstruct_constant(aprh_context* ctx, aprh_apr* apr, const void* part
    volume (y, x, z, y fastest) takes the level of the particle covering it; image_dtype U16 or
    F32 (the reference writes any pixel type) */
 int aprh_reconstruct_level(aprh_context* ctx, aprh_apr* apr, void* image, int image_dtype);
+/* ReconPatch (APRReconstruction.hpp:20-66): a box in pixels of the image at level
+   l_max + level_delta (level_delta > 0 upsamples by 2^level_delta); begins below 0 become 0,
+   ends below 0 the full extent ceil(dim * 2^level_delta), larger ends are cut to it */
+typedef struct aprh_patch {
+    int32_t x_begin, x_end, y_begin, y_end, z_begin, z_end;
+    int32_t level_delta;
+} aprh_patch;
+/* ReconPatch::check_limits (APRReconstruction.hpp:38-64): clamps the bounds in place;
+   APRH_ERR_INVALID_ARGUMENT when a range is empty afterwards */
+int aprh_patch_check_limits(const aprh_apr* apr, aprh_patch* patch);
+/* reconstruct_constant over a patch (APRReconstruction.hpp:186-193 -> :296-394; tree values
+   from fill_tree_mean) and reconstruct_level over a patch (:397-490).  image holds the clamped
+   patch, (y_end - y_begin, x_end - x_begin, z_end - z_begin) with y fastest */
+int aprh_reconstruct_constant_patch(aprh_context* ctx, aprh_apr* apr, const void* parts, int parts_dtype, const aprh_patch* patch,
+                                    void* image, int image_dtype);
+int aprh_reconstruct_level_patch(aprh_context* ctx, aprh_apr* apr, const aprh_patch* patch, void* image, int image_dtype);
 
 /* ---- learned Richardson-Lucy warm start (no reference counterpart) ----
    A 3-D CNN: four valid 3x3x3 convolutions 1 -> 16 -> 16 -> 16 -> 1 channels, ReLU after

@@ -68,6 +68,10 @@ class AprhTreeInfo(C.Structure):
                 ("xz_end_size", C.c_uint64), ("level_xz_size", C.c_uint64)]
 
 
+class AprhPatch(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("x_begin", "x_end", "y_begin", "y_end", "z_begin", "z_end", "level_delta")]
+
+
 class AprhRLStats(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("last_rel_update", C.c_double)]
 
@@ -139,6 +143,9 @@ def lib():
                                        C.c_int, C.c_int, C.c_double, C.POINTER(AprhRLStats)]
     L.aprh_reconstruct_constant.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp, C.c_int]
     L.aprh_reconstruct_level.argtypes = [vp, vp, vp, C.c_int]
+    L.aprh_patch_check_limits.argtypes = [vp, C.POINTER(AprhPatch)]
+    L.aprh_reconstruct_constant_patch.argtypes = [vp, vp, vp, C.c_int, C.POINTER(AprhPatch), vp, C.c_int]
+    L.aprh_reconstruct_level_patch.argtypes = [vp, vp, C.POINTER(AprhPatch), vp, C.c_int]
     L.aprh_synth_deconv_device.argtypes = [vp, vp, vp, i64, i64, i64, C.c_uint64, i64, C.c_float, C.c_float, C.c_float,
                                            C.c_float, C.c_float, vp, C.c_int32]
     L.aprh_sample_image_f32.argtypes = [vp, vp, vp, vp]
@@ -725,17 +732,66 @@ class APRNumerics:
         return out, {"iterations": stats.iterations, "last_rel_update": stats.last_rel_update}
 
 
+class ReconPatch:
+    """ReconPatch (APRReconstruction.hpp:20-66): a box in pixels of the image at level
+    l_max + level_delta; an end below 0 means the full extent."""
+
+    _names = ("x_begin", "x_end", "y_begin", "y_end", "z_begin", "z_end", "level_delta")
+
+    def __init__(self, x_begin=0, x_end=-1, y_begin=0, y_end=-1, z_begin=0, z_end=-1, level_delta=0):
+        self.x_begin, self.x_end, self.y_begin, self.y_end = x_begin, x_end, y_begin, y_end
+        self.z_begin, self.z_end, self.level_delta = z_begin, z_end, level_delta
+
+    def size(self):
+        return (self.z_end - self.z_begin) * (self.x_end - self.x_begin) * (self.y_end - self.y_begin)
+
+    def _c(self):
+        return AprhPatch(*[int(getattr(self, n)) for n in self._names])
+
+    def check_limits(self, apr):
+        """clamps the bounds to the APR's image at this level; False when a range is empty
+        (the reference prints a message and returns false, :38-64)"""
+        c = self._c()
+        if lib().aprh_patch_check_limits(apr.h, C.byref(c)) != APRH_OK:
+            return False
+        for n in self._names:
+            setattr(self, n, getattr(c, n))
+        return True
+
+    def __repr__(self):
+        return "ReconPatch(" + ", ".join(f"{n}={getattr(self, n)}" for n in self._names) + ")"
+
+
+def _clamped_patch(apr, patch):
+    p = ReconPatch(*[getattr(patch, n) for n in ReconPatch._names])
+    if not p.check_limits(apr):
+        raise ValueError(f"invalid patch {patch!r}: ReconPatch expects begin < end in all dimensions")
+    return p
+
+
 class APRReconstruction:
     """APRReconstruction::reconstruct_constant (APRReconstruction.hpp:80-120, :297-394) and
-    reconstruct_level (:131-171)."""
+    reconstruct_level (:131-171), each also over a ReconPatch (:186-196, :296-490)."""
 
     @staticmethod
-    def reconstruct_constant(apr, parts, level_delta=0, out_dtype=None, out=None, dtype=None):
+    def reconstruct_constant(apr, parts, level_delta=0, out_dtype=None, out=None, dtype=None, patch=None):
+        """with `patch` (a ReconPatch; its level_delta replaces the argument) the result is
+        the clamped patch, [z, x, y]"""
         parts = _host_parts(parts)
         code = _dtype_of(parts, dtype)
         odt = np.dtype(out_dtype) if out_dtype is not None else (np.dtype(np.uint16) if code == APRH_DTYPE_U16
                                                                   else np.dtype(np.float32))
         ocode = _dtype_of(None, str(odt))
+        if patch is not None:
+            p = _clamped_patch(apr, patch)
+            shape = (p.z_end - p.z_begin, p.x_end - p.x_begin, p.y_end - p.y_begin)
+            if out is None:
+                out = np.zeros(shape, odt)
+            elif out.size != int(np.prod(shape)) or out.dtype != odt or not out.flags.c_contiguous:
+                raise ValueError(f"out must be a contiguous {odt} array of {shape}")
+            c = p._c()
+            _check(lib().aprh_reconstruct_constant_patch(apr.ctx.h, apr.h, _ptr(parts), code, C.byref(c), _ptr(out), ocode))
+            return out
         if out is None:
             y, x, z = apr.org_dims
             s = 2 ** (-level_delta)
@@ -745,11 +801,22 @@ class APRReconstruction:
 
 
     @staticmethod
-    def reconstruct_level(apr, out_dtype=np.uint16, out=None):
+    def reconstruct_level(apr, out_dtype=np.uint16, out=None, patch=None):
         """APRReconstruction::reconstruct_level (APRReconstruction.hpp:131-171): the level of the
-        particle covering each pixel, [z, x, y] of the original size."""
+        particle covering each pixel, [z, x, y] of the original size; with `patch`, the clamped
+        patch (:397-490)"""
         odt = np.dtype(out_dtype)
         ocode = _dtype_of(None, str(odt))
+        if patch is not None:
+            p = _clamped_patch(apr, patch)
+            shape = (p.z_end - p.z_begin, p.x_end - p.x_begin, p.y_end - p.y_begin)
+            if out is None:
+                out = np.zeros(shape, odt)
+            elif out.size != int(np.prod(shape)) or out.dtype != odt or not out.flags.c_contiguous:
+                raise ValueError(f"out must be a contiguous {odt} array of {shape}")
+            c = p._c()
+            _check(lib().aprh_reconstruct_level_patch(apr.ctx.h, apr.h, C.byref(c), _ptr(out), ocode))
+            return out
         if out is None:
             y, x, z = apr.org_dims
             out = np.zeros((z, x, y), odt)

@@ -292,7 +292,12 @@ int richardson_lucy(aprh_context* ctx, aprh_apr& a, const T* in, float* u, const
 /* piecewise-constant image of level `level` (l_max: APRReconstruction::reconstruct_constant);
    dtypes 0 = u16, 1 = f32 for particles (pdt) and image (idt) */
 void reconstruct_any(aprh_context* ctx, aprh_apr& a, const void* parts, int pdt, int level, void* img, int idt);
-void reconstruct_level(aprh_context* ctx, aprh_apr& a, void* img, int idt);
+/* level image at `level` (<= l_max) with pixels of finer particles set to `level` */
+void reconstruct_level(aprh_context* ctx, aprh_apr& a, int level, void* img, int idt);
+/* ReconPatch::check_limits: clamps the bounds; false when a range is empty */
+bool patch_limits(const aprh_apr& a, aprh_patch& p);
+/* patch of the constant (parts != nullptr) or level reconstruction; p already clamped */
+void reconstruct_patch(aprh_context* ctx, aprh_apr& a, const void* parts, int pdt, const aprh_patch& p, void* img, int idt);
 
 /* ---- synthetic data (k_synth.hip) ---- */
 void synth_deconv_device(aprh_context* ctx, float* gt, uint16_t* noisy, int64_t y, int64_t x, int64_t z, uint64_t seed, int64_t n_blobs,

@@ -288,7 +288,51 @@ int aprh_reconstruct_level(aprh_context* ctx, aprh_apr* a, void* image, int imag
     const size_t ies = dtype_size(image_dtype);
     const size_t n = (size_t)(a->y_num[a->l_max] * a->x_num[a->l_max] * a->z_num[a->l_max]);
     DevOut out(ctx, image, n * ies);
-    reconstruct_level(ctx, *a, out.ptr, image_dtype);
+    reconstruct_level(ctx, *a, a->l_max, out.ptr, image_dtype);
+    out.finish(ctx);
+    APRH_CATCH
+}
+
+int aprh_patch_check_limits(const aprh_apr* a, aprh_patch* patch) {
+    APRH_TRY
+    if (!a || !patch) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!patch_limits(*a, *patch)) throw Error(APRH_ERR_INVALID_ARGUMENT, "ReconPatch expects begin < end in all dimensions (and extents within int32)");
+    APRH_CATCH
+}
+
+/* shared checks of the two patch entry points: clamps *p and returns the patch's pixel count */
+static size_t patch_setup(aprh_apr* a, const aprh_patch* patch, aprh_patch* p) {
+    if (!patch) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    *p = *patch;
+    if (p->level_delta < a->l_min - a->l_max) throw Error(APRH_ERR_INVALID_ARGUMENT, "level_delta below the APR's minimum level");
+    if (!patch_limits(*a, *p)) throw Error(APRH_ERR_INVALID_ARGUMENT, "ReconPatch expects begin < end in all dimensions");
+    return (size_t)(p->y_end - p->y_begin) * (size_t)(p->x_end - p->x_begin) * (size_t)(p->z_end - p->z_begin);
+}
+
+int aprh_reconstruct_constant_patch(aprh_context* ctx, aprh_apr* a, const void* parts, int parts_dtype, const aprh_patch* patch,
+                                    void* image, int image_dtype) {
+    APRH_TRY
+    check_apr(ctx, a);
+    if (!parts || !image) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    const size_t pes = dtype_size(parts_dtype), ies = dtype_size(image_dtype);
+    aprh_patch p;
+    const size_t n = patch_setup(a, patch, &p);
+    DevIn in(ctx, parts, std::max<uint64_t>(a->total, 1) * pes);
+    DevOut out(ctx, image, n * ies);
+    reconstruct_patch(ctx, *a, in.ptr, parts_dtype, p, out.ptr, image_dtype);
+    out.finish(ctx);
+    APRH_CATCH
+}
+
+int aprh_reconstruct_level_patch(aprh_context* ctx, aprh_apr* a, const aprh_patch* patch, void* image, int image_dtype) {
+    APRH_TRY
+    check_apr(ctx, a);
+    if (!image) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    const size_t ies = dtype_size(image_dtype);
+    aprh_patch p;
+    const size_t n = patch_setup(a, patch, &p);
+    DevOut out(ctx, image, n * ies);
+    reconstruct_patch(ctx, *a, nullptr, 0, p, out.ptr, image_dtype);
     out.finish(ctx);
     APRH_CATCH
 }

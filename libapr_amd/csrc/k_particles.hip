@@ -762,8 +762,10 @@ struct FillU16 {
 
 /* APRReconstruction::reconstruct_level (APRReconstruction.hpp:131-171): every pixel takes the
    level of the particle covering it, which is the constant reconstruction of particles whose
-   values are their levels (particles are stored level by level, so each level is one range) */
-void reconstruct_level(aprh_context* ctx, aprh_apr& a, void* img, int idt) {
+   values are their levels (particles are stored level by level, so each level is one range).
+   At a coarser level L (the patch form, :397-490) a pixel covered by finer particles takes L
+   from the tree: the particles carry min(level, L), so every tree mean there is exactly L. */
+void reconstruct_level(aprh_context* ctx, aprh_apr& a, int level, void* img, int idt) {
     Buf lev(ctx->pool, std::max<uint64_t>(a.total, 1) * 2);
     for (int l = a.l_min; l <= a.l_max; ++l) {
         const uint64_t r0 = a.level_xz_vec[l], nrows = (uint64_t)(a.x_num[l] * a.z_num[l]);
@@ -774,10 +776,83 @@ void reconstruct_level(aprh_context* ctx, aprh_apr& a, void* img, int idt) {
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
         if (e > b) {
             const int64_t G = stride_tiles((int64_t)(e - b), 256 * 4);
-            launch_tiles<FillU16>(ctx, {lev.as<uint16_t>() + b, e - b, (uint16_t)l, G}, G);
+            launch_tiles<FillU16>(ctx, {lev.as<uint16_t>() + b, e - b, (uint16_t)std::min(l, level), G}, G);
         }
     }
-    reconstruct_any(ctx, a, lev.ptr(), APRH_DTYPE_U16, a.l_max, img, idt);
+    reconstruct_any(ctx, a, lev.ptr(), APRH_DTYPE_U16, level, img, idt);
+}
+
+namespace {
+/* the patch out of a level image: out[z, x, y] = full[(z + zb) >> s, (x + xb) >> s, (y + yb) >> s]
+   (s > 0: nearest-neighbour upsampling by 2^s), y fastest in both */
+template <typename U>
+struct CropPatch {
+    struct Args {
+        const U* full;
+        U* out;
+        int64_t fy, fx;
+        int64_t py, px, n;
+        int64_t yb, xb, zb;
+        int s;
+        int64_t G;
+    };
+    static constexpr int THREADS = 256;
+    static const char* name() { return "crop_patch"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) {
+        for (int64_t i = t * blockDim.x + threadIdx.x; i < a.n; i += a.G * blockDim.x) {
+            const int64_t y = i % a.py, r = i / a.py;
+            const int64_t x = r % a.px, z = r / a.px;
+            a.out[i] = a.full[(((z + a.zb) >> a.s) * a.fx + ((x + a.xb) >> a.s)) * a.fy + ((y + a.yb) >> a.s)];
+        }
+    }
+};
+}  // namespace
+
+bool patch_limits(const aprh_apr& a, aprh_patch& p) {
+    /* ReconPatch::check_limits (APRReconstruction.hpp:38-64): extents ceil(dim * 2^level_delta),
+       which must fit an int32 */
+    if (p.level_delta > 30 || p.level_delta < -30) return false;
+    int64_t m[3];
+    for (int d = 0; d < 3; ++d) {
+        const int64_t dim = a.org[d];
+        m[d] = p.level_delta >= 0 ? (dim << p.level_delta) : ((dim + ((int64_t)1 << -p.level_delta) - 1) >> -p.level_delta);
+        if (m[d] > INT32_MAX) return false;
+    }
+    int32_t* b[3] = {&p.y_begin, &p.x_begin, &p.z_begin};
+    int32_t* e[3] = {&p.y_end, &p.x_end, &p.z_end};
+    for (int d = 0; d < 3; ++d) {
+        *b[d] = std::max(0, *b[d]);
+        *e[d] = *e[d] < 0 ? (int32_t)m[d] : (int32_t)std::min<int64_t>(m[d], *e[d]);
+    }
+    return p.y_begin < p.y_end && p.x_begin < p.x_end && p.z_begin < p.z_end;
+}
+
+/* APRReconstruction::reconstruct_constant / reconstruct_level over a patch (:296-394, :397-490):
+   the level image of level min(l_max, l_max + level_delta), cut to the patch (and upsampled by
+   2^level_delta when level_delta > 0).  parts == nullptr: the level image. */
+void reconstruct_patch(aprh_context* ctx, aprh_apr& a, const void* parts, int pdt, const aprh_patch& p, void* img, int idt) {
+    const int level = a.l_max + std::min(p.level_delta, 0);
+    const int s = std::max(p.level_delta, 0);
+    const int64_t fy = a.y_num[level], fx = a.x_num[level], fz = a.z_num[level];
+    const int64_t py = p.y_end - p.y_begin, px = p.x_end - p.x_begin, pz = p.z_end - p.z_begin;
+    const size_t es = idt == APRH_DTYPE_U16 ? 2 : 4;
+    auto full_image = [&](void* dst) {
+        if (parts) reconstruct_any(ctx, a, parts, pdt, level, dst, idt);
+        else reconstruct_level(ctx, a, level, dst, idt);
+    };
+    if (s == 0 && py == fy && px == fx && pz == fz) {
+        full_image(img);
+        return;
+    }
+    Buf full(ctx->pool, (size_t)(fy * fx * fz) * es);
+    full_image(full.ptr());
+    const int64_t n = py * px * pz;
+    const int64_t G = stride_tiles(n, 256 * 4);
+    if (idt == APRH_DTYPE_U16)
+        launch_tiles<CropPatch<uint16_t>>(ctx, {full.as<uint16_t>(), (uint16_t*)img, fy, fx, py, px, n, p.y_begin, p.x_begin, p.z_begin, s, G}, G);
+    else
+        launch_tiles<CropPatch<float>>(ctx, {full.as<float>(), (float*)img, fy, fx, py, px, n, p.y_begin, p.x_begin, p.z_begin, s, G}, G);
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
 }
 
 void reconstruct_any(aprh_context* ctx, aprh_apr& a, const void* parts, int pdt, int level, void* img, int idt) {

@@ -145,6 +145,29 @@ int main() {
         bool eq = rlf.mesh.size() == orl.size();
         for (size_t i = 0; eq && i < orl.size(); ++i) eq = rlf.mesh[i] == (float)orl[i];
         CHECK(eq);
+
+        // patches: a box of the full-resolution images (the end past the image is clamped)
+        std::vector<uint16_t> orec((size_t)(y * x * z));
+        orc_reconstruct(o, oparts.data(), 0, 0, orec.data(), 0);
+        ReconPatch pp;
+        pp.x_begin = 5, pp.x_end = 60, pp.y_begin = -2, pp.y_end = 500, pp.z_begin = 3, pp.z_end = 17;
+        PixelData<uint16_t> pc, pl;
+        APRReconstruction::reconstruct_constant(apr, pc, parts, pp);
+        APRReconstruction::reconstruct_level(apr, pl, pp);
+        CHECK(pp.y_begin == 0 && pp.y_end == y && pc.y_num == y && pc.x_num == 55 && pc.z_num == 14);
+        bool peq = pc.mesh.size() == (size_t)(55 * 14 * y) && pl.mesh.size() == pc.mesh.size();
+        for (int zz = 0; peq && zz < 14; ++zz)
+            for (int xx = 0; peq && xx < 55; ++xx)
+                for (int yy = 0; peq && yy < y; ++yy) {
+                    const size_t src = ((size_t)(zz + 3) * x + (xx + 5)) * y + yy, dst = ((size_t)zz * 55 + xx) * y + yy;
+                    peq = pc.mesh[dst] == orec[src] && pl.mesh[dst] == orl[src];
+                }
+        CHECK(peq);
+        // an empty range: a message, the image untouched
+        ReconPatch bad;
+        bad.x_begin = 9, bad.x_end = 9;
+        APRReconstruction::reconstruct_level(apr, pl, bad);
+        CHECK(pl.mesh.size() == pc.mesh.size());
     }
 
     // error conventions

@@ -274,3 +274,108 @@ def test_reconstruct_level(oracle, ctx, shape, nb, seed):
     for out_dtype in (np.uint16, np.float32):
         got = A.APRReconstruction.reconstruct_level(g, out_dtype)
         assert got.dtype == out_dtype and np.array_equal(got, expect.astype(out_dtype))
+
+
+def _patch_restated(o, values, tree_values, p, out_dtype):
+    """the patch loop of APRReconstruction.hpp:296-394 (reconstruct_level :397-490 is the same
+    loop with level-valued particles and the patch level for tree nodes), row by row"""
+    y, x, z = o.dims
+    max_level = o.l_max + p.level_delta
+    img = np.zeros((p.z_end - p.z_begin, p.x_end - p.x_begin, p.y_end - p.y_begin), out_dtype)
+    lxz, xze, yv = o.level_xz_vec, o.xz_end_vec, o.y_vec
+
+    def row(lxz_, xze_, lev, zz, xx):
+        xl = -(-x // 2 ** (o.l_max - lev))
+        r = int(lxz_[lev]) + zz * xl + xx
+        return (int(xze_[r - 1]) if r > 0 else 0), int(xze_[r])
+
+    for xx in range(p.x_begin, p.x_end):
+        for zz in range(p.z_begin, p.z_end):
+            for lev in range(min(max_level, o.l_max), o.l_min - 1, -1):
+                ylev = -(-y // 2 ** (o.l_max - lev))
+                if lev == max_level:
+                    b, e = row(lxz, xze, lev, zz, xx)
+                    for i in range(b, e):
+                        if p.y_begin <= yv[i] < p.y_end:
+                            img[zz - p.z_begin, xx - p.x_begin, int(yv[i]) - p.y_begin] = values[i]
+                else:
+                    step = 2 ** (max_level - lev)
+                    b, e = row(lxz, xze, lev, zz // step, xx // step)
+                    ybl, yel = p.y_begin // step, min(-(-p.y_end // step), ylev)
+                    for i in range(b, e):
+                        if ybl <= yv[i] < yel:
+                            lo = max(int(yv[i]) * step, p.y_begin) - p.y_begin
+                            hi = min((int(yv[i]) + 1) * step, p.y_end) - p.y_begin
+                            img[zz - p.z_begin, xx - p.x_begin, lo:hi] = values[i]
+    if max_level < o.l_max:
+        t = o.tree()
+        for xx in range(p.x_begin, p.x_end):
+            for zz in range(p.z_begin, p.z_end):
+                b, e = row(t["level_xz_vec"], t["xz_end_vec"], max_level, zz, xx)
+                for i in range(b, e):
+                    yy = int(t["y_vec"][i])
+                    if p.y_begin <= yy < p.y_end:
+                        img[zz - p.z_begin, xx - p.x_begin, yy - p.y_begin] = tree_values[i]
+    return img
+
+
+def _levels_of(o):
+    lev = np.zeros(o.total, np.uint16)
+    for l in range(o.l_min, o.l_max + 1):
+        r0 = int(o.level_xz_vec[l])
+        b = int(o.xz_end_vec[r0 - 1]) if r0 > 0 else 0
+        lev[b:int(o.xz_end_vec[int(o.level_xz_vec[l + 1]) - 1])] = l
+    return lev
+
+
+PATCHES = [dict(x_begin=3, x_end=40, y_begin=5, y_end=50, z_begin=2, z_end=20),
+           dict(x_begin=-4, x_end=1000, y_begin=17, y_end=-1, z_begin=9, z_end=-1),
+           dict(x_begin=1, x_end=12, y_begin=3, y_end=21, z_begin=0, z_end=7, level_delta=-1),
+           dict(x_begin=2, x_end=9, y_begin=0, y_end=-1, z_begin=1, z_end=5, level_delta=-2),
+           dict(x_begin=7, x_end=50, y_begin=11, y_end=90, z_begin=3, z_end=19, level_delta=1)]
+
+
+@pytest.mark.parametrize("shape,nb,seed", CASES[:2])
+@pytest.mark.parametrize("pi", range(len(PATCHES)))
+def test_reconstruct_patch(oracle, ctx, shape, nb, seed, pi):
+    img, o, g = _pair(oracle, ctx, shape, nb, seed)
+    parts = o.sample(img)
+    p = A.ReconPatch(**PATCHES[pi])
+    assert p.check_limits(g)
+    d = p.level_delta
+    # the restated loop against a cut of the oracle's whole-image reconstruction
+    full = o.reconstruct(parts, min(d, 0), np.uint16)
+    if d > 0:
+        full = full.repeat(2 ** d, 0).repeat(2 ** d, 1).repeat(2 ** d, 2)
+    cut = full[p.z_begin:p.z_end, p.x_begin:p.x_end, p.y_begin:p.y_end]
+    tree = o.fill_tree_mean(parts) if d < 0 else None
+    exp = _patch_restated(o, parts, tree, p, np.uint16)
+    assert np.array_equal(exp, cut)
+    got = A.APRReconstruction.reconstruct_constant(g, parts, patch=A.ReconPatch(**PATCHES[pi]))
+    assert got.dtype == np.uint16 and np.array_equal(got, exp)
+    gotf = A.APRReconstruction.reconstruct_constant(g, parts, out_dtype=np.float32, patch=A.ReconPatch(**PATCHES[pi]))
+    expf = _patch_restated(o, parts.astype(np.float32), tree, p, np.float32)
+    assert np.array_equal(_bits(gotf), _bits(expf))
+    # levels: the particles' levels, capped at the patch level (tree nodes take the patch level)
+    lev = _levels_of(o)
+    maxl = o.l_max + d
+    tl = np.full(o.tree()["total"], maxl, np.uint16) if d < 0 else None
+    expl = _patch_restated(o, np.minimum(lev, maxl), tl, p, np.uint16)
+    gotl = A.APRReconstruction.reconstruct_level(g, patch=A.ReconPatch(**PATCHES[pi]))
+    assert np.array_equal(gotl, expl)
+    if d == 0:
+        assert np.array_equal(expl, _level_image(o)[p.z_begin:p.z_end, p.x_begin:p.x_end, p.y_begin:p.y_end])
+
+
+def test_reconstruct_patch_limits(oracle, ctx):
+    img, o, g = _pair(oracle, ctx, *CASES[1])
+    y, x, z = o.dims
+    p = A.ReconPatch(x_begin=-3, y_end=10 ** 6, z_begin=2, level_delta=-1)
+    assert p.check_limits(g)
+    assert (p.x_begin, p.x_end, p.y_begin, p.y_end, p.z_begin, p.z_end) == (0, -(-x // 2), 0, -(-y // 2), 2, -(-z // 2))
+    for bad in (dict(x_begin=5, x_end=5), dict(y_begin=10 ** 6), dict(z_begin=3, z_end=2), dict(level_delta=40)):
+        assert not A.ReconPatch(**bad).check_limits(g)
+        with pytest.raises(ValueError):
+            A.APRReconstruction.reconstruct_level(g, patch=A.ReconPatch(**bad))
+    with pytest.raises(A.AprhError):
+        A.APRReconstruction.reconstruct_level(g, patch=A.ReconPatch(level_delta=-(o.l_max - o.l_min) - 1))
