@@ -391,13 +391,20 @@ def run_rl(args):
     w.rl(u, iters=2, tol=0.0)
     exact_ms = (time.perf_counter() - t) * 500.0
     log(f"[bench] RL iteration, reference tap order: {exact_ms:.1f} ms")
+    pf = None
     if os.path.exists(W.DEFAULT_WEIGHTS):
         net = A.WarmStartCNN.load(W.DEFAULT_WEIGHTS, ctx)
+        pf = net.post_filter
+        net.set_post_filter(None)
         trained = "libapr_amd/data/warm_start_cnn.npy"
     else:
         net = A.WarmStartCNN(ctx, params=None)
         trained = "untrained (weights file missing)"
     ev = W.evaluate(ctx, net, workload=w, log=log)
+    evf = None
+    if pf:
+        net.set_post_filter(pf["sigma"], pf["size"])
+        evf = W.evaluate(ctx, net, workload=w, log=log)
     # data residual ||psf * u - input|| / ||input|| of the plain result
     ctx.set_conv_mode(A.CONV_SEPARABLE)
     w.rl(u)
@@ -418,6 +425,12 @@ def run_rl(args):
                           "time_ratio_warm_over_plain": ev["time_ratio_warm_over_plain"],
                           "error_ratio_warm_over_plain": ev["final_error_ratio_warm_over_plain"],
                           "rel_l2_diff_to_plain_result": ev["warm_vs_plain_result_rel_l2"]}}
+    if evf:
+        res["warm_start_post_filter"] = {"post_filter": pf, "iterations": evf["warm_iterations"],
+                                         "time_to_tol_s": evf["warm_time_s"],
+                                         "rel_l2_error_vs_gt": evf["warm_rel_l2_error_vs_gt"],
+                                         "time_ratio_warm_over_plain": evf["time_ratio_warm_over_plain"],
+                                         "error_ratio_warm_over_plain": evf["final_error_ratio_warm_over_plain"]}
     net.close()
     ctx.close()
     return res

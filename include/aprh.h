@@ -263,6 +263,12 @@ int aprh_cnn_apply(aprh_context* ctx, aprh_cnn* net, const void* image, int dtyp
    piecewise-constant reconstruction of the particles (normalised by its mean), floored at
    1e-3 of the mean, sampled back onto the particles (ParticleData::sample_image) */
 int aprh_cnn_warm_start(aprh_context* ctx, aprh_cnn* net, aprh_apr* apr, const void* parts, int dtype, float* u0);
+/* optional last step of aprh_cnn_warm_start: the estimate convolved (convolve_pencil, stencils
+   downsampled per level and normalised, reflected boundary) with the Gaussian of
+   APRStencil::create_gaussian_filter(sigma3, size3) in (y, x, z) order.  It removes the
+   network's fine-scale error, which RL iterations do not correct, at the cost of a few more
+   iterations.  Sizes odd in 1..15; NULL pointers switch it off (the default). */
+int aprh_cnn_set_post_filter(aprh_cnn* net, const float* sigma3, const int32_t* size3);
 /* mean of n u16 / f32 values (host or device), accumulated in double */
 int aprh_mean(aprh_context* ctx, const void* data, int dtype, uint64_t n, double* mean);
 

@@ -158,6 +158,7 @@ def lib():
     L.aprh_cnn_extract_tiles.argtypes = [vp, vp, C.c_int, i64, i64, i64, vp, C.c_int32, C.c_int32, C.c_int32, C.c_float, vp]
     L.aprh_cnn_apply.argtypes = [vp, vp, vp, C.c_int, i64, i64, i64, C.c_float, vp]
     L.aprh_cnn_warm_start.argtypes = [vp, vp, vp, vp, C.c_int, vp]
+    L.aprh_cnn_set_post_filter.argtypes = [vp, vp, vp]
     L.aprh_mean.argtypes = [vp, vp, C.c_int, C.c_uint64, C.POINTER(C.c_double)]
     L.aprh_batch_create.argtypes = [vp, C.c_int32, vp, C.POINTER(AprhParams), C.POINTER(vp)]
     L.aprh_batch_input.argtypes = [vp, C.c_int32, C.POINTER(vp)]
@@ -877,6 +878,7 @@ class WarmStartCNN:
         h = vp()
         _check(lib().aprh_cnn_create(self.ctx.h, seed, C.byref(h)))
         self.h = h
+        self.post_filter = None
         _register(self.ctx, self)
         if params is not None:
             self.set_params(params)
@@ -892,12 +894,36 @@ class WarmStartCNN:
             raise ValueError(f"expected {CNN_PARAMS} parameters")
         _check(lib().aprh_cnn_set_params(self.ctx.h, self.h, _ptr(p)))
 
+    def set_post_filter(self, sigma=None, size=None):
+        """Gaussian (sigma, size in (y, x, z)) applied to the warm-start particles; None: off"""
+        if sigma is None:
+            _check(lib().aprh_cnn_set_post_filter(self.h, None, None))
+            self.post_filter = None
+            return
+        sg = np.ascontiguousarray(sigma, np.float32)
+        sz = np.ascontiguousarray(size, np.int32)
+        _check(lib().aprh_cnn_set_post_filter(self.h, _ptr(sg), _ptr(sz)))
+        self.post_filter = {"sigma": [float(v) for v in sg], "size": [int(v) for v in sz]}
+
     def save(self, path):
         np.save(path, self.get_params(), allow_pickle=False)
 
     @classmethod
-    def load(cls, path, ctx=None):
-        return cls(ctx, params=np.load(path, allow_pickle=False))
+    def load(cls, path, ctx=None, post_filter="recorded"):
+        """weights from `path`; post_filter "recorded" takes the filter recorded beside the
+        weights (the .json of the same name, if it names one), None switches it off, or a dict
+        {"sigma": (y, x, z), "size": (y, x, z)}"""
+        net = cls(ctx, params=np.load(path, allow_pickle=False))
+        if post_filter == "recorded":
+            meta = os.path.splitext(path)[0] + ".json"
+            post_filter = None
+            if os.path.exists(meta):
+                import json
+                with open(meta) as f:
+                    post_filter = json.load(f).get("post_filter")
+        if post_filter:
+            net.set_post_filter(post_filter["sigma"], post_filter["size"])
+        return net
 
     def forward_tiles(self, x, S):
         x = np.ascontiguousarray(x, np.float32)

@@ -592,6 +592,9 @@ struct aprh_cnn {
     aprh::Buf W1p, W2p, W3p, W2t, W3t;           /* packed bf16 weights                  */
     int step = 0;
     float last_loss = 0;
+    /* Gaussian applied to the warm-start particles (sizes 1: none) */
+    float post_sigma[3] = {0.0f, 0.0f, 0.0f};
+    int32_t post_size[3] = {1, 1, 1};
 };
 
 namespace aprh {
@@ -967,7 +970,37 @@ int aprh_cnn_warm_start(aprh_context* ctx, aprh_cnn* n, aprh_apr* a, const void*
     /* the estimate must stay positive for the multiplicative updates */
     apply_volume(n, rec.ptr(), APRH_DTYPE_F32, yn, xn, zn, scale, 1e-3f * scale, pred.as<float>());
     launch_sample_f32(ctx, *a, pred.as<float>(), (float*)u.ptr);
+    if (n->post_size[0] * n->post_size[1] * n->post_size[2] > 1 && a->total > 0) {
+        /* the post filter: APRFilter::convolve_pencil of the estimate with a normalised Gaussian
+           (downsampled per level, reflected boundary) */
+        HostStencil hs;
+        hs.sy = n->post_size[0];
+        hs.sx = n->post_size[1];
+        hs.sz = n->post_size[2];
+        hs.w.assign((size_t)(hs.sy * hs.sx * hs.sz), 0.0f);
+        if (aprh_gaussian_stencil(n->post_sigma, n->post_size, 1, hs.w.data()) != APRH_OK)
+            throw Error(APRH_ERR_INVALID_ARGUMENT, "post filter stencil");
+        const std::vector<HostStencil> v = downsampled_stencils(hs, a->l_max - a->l_min, true);
+        Buf tmp(ctx->pool, a->total * 4);
+        convolve<float>(ctx, *a, v, (const float*)u.ptr, tmp.as<float>(), true);
+        launch_copy(ctx, u.ptr, tmp.ptr(), a->total * 4);
+    }
     u.finish(ctx);
+    APRH_CATCH
+}
+
+int aprh_cnn_set_post_filter(aprh_cnn* n, const float* sigma3, const int32_t* size3) {
+    APRH_TRY
+    if (!n) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL network");
+    if (!sigma3 || !size3) {
+        for (int d = 0; d < 3; ++d) n->post_sigma[d] = 0.0f, n->post_size[d] = 1;
+    } else {
+        for (int d = 0; d < 3; ++d) {
+            if (size3[d] < 1 || size3[d] > 15 || size3[d] % 2 == 0) throw Error(APRH_ERR_INVALID_ARGUMENT, "post filter sizes: odd, 1..15");
+            if (size3[d] > 1 && !(sigma3[d] > 0.0f)) throw Error(APRH_ERR_INVALID_ARGUMENT, "post filter sigma must be positive");
+        }
+        for (int d = 0; d < 3; ++d) n->post_sigma[d] = sigma3[d], n->post_size[d] = size3[d];
+    }
     APRH_CATCH
 }
 

@@ -13,7 +13,8 @@ that of the converged estimate, on 64^3 tiles (4-voxel halo), both divided by
 the mean of the observed reconstruction.
 
 Evaluation: on a held-out volume, plain RL from u = 1 against RL started from
-the network's estimate (aprh_cnn_warm_start), both to the same tolerance;
+the network's estimate (aprh_cnn_warm_start, without and with the post filter
+that is recorded beside the weights), both to the same tolerance;
 reports iterations, wall time (the warm start's includes the network), the
 relative L2 error of each reconstruction against the ground truth, and the
 difference between the two results.
@@ -163,6 +164,8 @@ def main():
     ap.add_argument("--weights", default=DEFAULT_WEIGHTS)
     ap.add_argument("--eval-shape", default="256,1024,1024")
     ap.add_argument("--report", default=None)
+    ap.add_argument("--post-sigma", default="0.55,0.55,0.3", help="y,x,z sigma of the post filter evaluated")
+    ap.add_argument("--post-size", default="3,3,1", help="y,x,z size of the post filter evaluated")
     ap.add_argument("--target-iters", type=int, default=0,
                     help="train on plain RL after this many iterations (0: run to the tolerance)")
     args = ap.parse_args()
@@ -181,9 +184,15 @@ def main():
                            "first_loss": float(np.mean(losses[:20])), "last_loss": float(np.mean(losses[-50:]))}
         del pairs
     else:
-        net = A.WarmStartCNN.load(args.weights, ctx)
+        net = A.WarmStartCNN.load(args.weights, ctx, post_filter=None)
     es = tuple(int(v) for v in args.eval_shape.split(","))
-    report["eval"] = evaluate(ctx, net, es, log=log)
+    w = Workload(ctx, es, 3, 50_000)
+    net.set_post_filter(None)
+    report["eval"] = evaluate(ctx, net, es, log=log, workload=w)
+    pf = {"sigma": [float(v) for v in args.post_sigma.split(",")], "size": [int(v) for v in args.post_size.split(",")]}
+    net.set_post_filter(pf["sigma"], pf["size"])
+    report["eval_with_post_filter"] = dict(evaluate(ctx, net, es, log=log, workload=w), post_filter=pf)
+    report["post_filter"] = pf
     print(json.dumps(report))
     if args.report:
         with open(args.report, "w") as f:

@@ -158,7 +158,7 @@ def test_trained_warm_start_gain(ctx):
     and the final error against the ground truth stays close to plain RL's."""
     from libapr_amd import warm_start as W
     w = W.Workload(ctx, (128, 256, 256), seed=7)
-    net = A.WarmStartCNN.load(W.DEFAULT_WEIGHTS, ctx)
+    net = A.WarmStartCNN.load(W.DEFAULT_WEIGHTS, ctx, post_filter=None)
     ctx.set_conv_mode(A.CONV_SEPARABLE)
     try:
         u = A.DeviceArray(w.P * 4, ctx)
@@ -179,5 +179,34 @@ def test_trained_warm_start_gain(ctx):
         assert st_w["iterations"] <= 0.6 * st_p["iterations"]
         assert d_net < 0.5 * d_obs
         assert err_w < 1.1 * err_p
+        # with the post filter recorded beside the weights: more iterations, error nearer plain RL's
+        netf = A.WarmStartCNN.load(W.DEFAULT_WEIGHTS, ctx)
+        assert netf.post_filter is not None
+        netf.warm_start(w.apr, w.parts.ptr, out=uw.ptr, dtype="uint16")
+        st_f = w.rl(uw, resume=True)
+        err_f = w.error_vs_gt(uw)
+        print(f"with the post filter {netf.post_filter}: {st_f['iterations']} iterations, error {err_f:.4f}")
+        assert st_f["iterations"] <= 0.8 * st_p["iterations"]
+        assert err_f < err_w and err_f < 1.03 * err_p
     finally:
         ctx.set_conv_mode(A.CONV_EXACT)
+
+
+def test_warm_start_post_filter_is_a_convolution(ctx):
+    """the post filter is convolve_pencil of the network's estimate with the normalised
+    Gaussian (bit for bit), and switching it off restores the plain estimate"""
+    from libapr_amd import warm_start as W
+    w = W.Workload(ctx, (64, 96, 128), seed=11)
+    net = A.WarmStartCNN.load(W.DEFAULT_WEIGHTS, ctx, post_filter=None)
+    plain = net.warm_start(w.apr, w.parts.ptr, dtype="uint16").copy()
+    sigma, size = (0.55, 0.55, 0.3), (3, 3, 1)
+    net.set_post_filter(sigma, size)
+    got = net.warm_start(w.apr, w.parts.ptr, dtype="uint16").copy()
+    st = A.APRStencil.create_gaussian_filter(sigma, size, True)
+    exp = A.APRFilter.convolve_pencil(w.apr, st, plain, normalize=True)
+    assert np.array_equal(got.view(np.uint32), exp[:w.P].view(np.uint32))
+    assert not np.array_equal(got, plain)
+    net.set_post_filter(None)
+    assert np.array_equal(net.warm_start(w.apr, w.parts.ptr, dtype="uint16"), plain)
+    with pytest.raises(A.AprhError):
+        net.set_post_filter((0.5, 0.5, 0.5), (2, 3, 3))
