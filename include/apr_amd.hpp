@@ -10,6 +10,7 @@
  *   APRStencil::*                        numerics/APRStencil.hpp:12-244
  *   APRFilter::convolve(_pencil)         numerics/APRFilter.hpp:67-149
  *   APRNumerics::richardson_lucy         numerics/APRNumerics.hpp:126-134
+ *   APRNumerics::gradient_* (cfd, sobel, magnitude)  numerics/APRNumerics.hpp:165-283
  *   APRReconstruction::reconstruct_constant  numerics/APRReconstruction.hpp:80, :297
  *   APRReconstruction::reconstruct_level     numerics/APRReconstruction.hpp:131-171
  *   ReconPatch and the patch reconstructions numerics/APRReconstruction.hpp:20-66, :186-196, :296-490
@@ -291,6 +292,32 @@ aprh_rl_stats richardson_lucy(APR& apr, ParticleData<In>& in, ParticleData<float
                                        psf.mesh.data(), (int32_t)psf.y_num, (int32_t)psf.x_num, (int32_t)psf.z_num, number_iterations,
                                        use_stencil_downsample ? 1 : 0, normalize ? 1 : 0, resume ? 1 : 0, tol, &st));
     return st;
+}
+
+/* gradient_cfd / gradient_sobel (APRNumerics.hpp:165-213); std::invalid_argument on a bad
+   dimension, as the reference */
+template <typename In>
+void gradient_cfd(APR& apr, const ParticleData<In>& in, ParticleData<float>& out, int dimension, float delta = 1.0f) {
+    if (dimension < 0 || dimension > 2) throw std::invalid_argument("APRNumerics::gradient_cfd argument 'dimension' must be 0 (y), 1 (x) or 2 (z)");
+    out.init(apr.total_number_particles());
+    detail::check(aprh_gradient_cfd(apr.context().get(), apr.get(), in.data.data(), detail::dtype<In>::code, dimension, delta, out.data.data()));
+}
+template <typename In>
+void gradient_sobel(APR& apr, const ParticleData<In>& in, ParticleData<float>& out, int dimension, float delta = 1.0f) {
+    if (dimension < 0 || dimension > 2) throw std::invalid_argument("APRNumerics::gradient_sobel argument 'dimension' must be 0 (y), 1 (x) or 2 (z)");
+    out.init(apr.total_number_particles());
+    detail::check(aprh_gradient_sobel(apr.context().get(), apr.get(), in.data.data(), detail::dtype<In>::code, dimension, delta, out.data.data()));
+}
+/* gradient_magnitude_cfd / _sobel (APRNumerics.hpp:216-283) */
+template <typename In>
+void gradient_magnitude_cfd(APR& apr, const ParticleData<In>& in, ParticleData<float>& out, const std::vector<float>& deltas = {1.0f, 1.0f, 1.0f}) {
+    out.init(apr.total_number_particles());
+    detail::check(aprh_gradient_magnitude_cfd(apr.context().get(), apr.get(), in.data.data(), detail::dtype<In>::code, deltas.data(), out.data.data()));
+}
+template <typename In>
+void gradient_magnitude_sobel(APR& apr, const ParticleData<In>& in, ParticleData<float>& out, const std::vector<float>& deltas = {1.0f, 1.0f, 1.0f}) {
+    out.init(apr.total_number_particles());
+    detail::check(aprh_gradient_magnitude_sobel(apr.context().get(), apr.get(), in.data.data(), detail::dtype<In>::code, deltas.data(), out.data.data()));
 }
 }  // namespace APRNumerics
 

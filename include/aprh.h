@@ -214,6 +214,19 @@ int aprh_reconstruct_constant(aprh_context* ctx, aprh_apr* apr, const void* part
    volume (y, x, z, y fastest) takes the level of the particle covering it; image_dtype U16 or
    F32 (the reference writes any pixel type) */
 int aprh_reconstruct_level(aprh_context* ctx, aprh_apr* apr, void* image, int image_dtype);
+/* APRNumerics::gradient_cfd (APRNumerics.hpp:165-184) and gradient_sobel (:187-213): the
+   particles convolved (convolve_pencil, reflected boundary) with the central difference
+   [-1, 0, 1] / (2 delta) along `dimension` (0 y, 1 x, 2 z), or with the Sobel stencil (that
+   difference times [1/4, 1/2, 1/4] smoothing along the other axes; the 2-D form for 2-D
+   images, the central difference below), divided by 2^d at level delta d (rescaled, not
+   downsampled, stencils).  out: float32[total]. */
+int aprh_gradient_cfd(aprh_context* ctx, aprh_apr* apr, const void* parts, int dtype, int dimension, float delta, float* out);
+int aprh_gradient_sobel(aprh_context* ctx, aprh_apr* apr, const void* parts, int dtype, int dimension, float delta, float* out);
+/* gradient_magnitude_cfd / _sobel (APRNumerics.hpp:216-283): sqrtf(g_y^2 + g_x^2 + g_z^2) with
+   deltas[3] the pixel sizes in (y, x, z); axes of size 1 are left out */
+int aprh_gradient_magnitude_cfd(aprh_context* ctx, aprh_apr* apr, const void* parts, int dtype, const float* deltas, float* out);
+int aprh_gradient_magnitude_sobel(aprh_context* ctx, aprh_apr* apr, const void* parts, int dtype, const float* deltas, float* out);
+
 /* ReconPatch (APRReconstruction.hpp:20-66): a box in pixels of the image at level
    l_max + level_delta (level_delta > 0 upsamples by 2^level_delta); begins below 0 become 0,
    ends below 0 the full extent ceil(dim * 2^level_delta), larger ends are cut to it */

@@ -144,6 +144,10 @@ def lib():
     L.aprh_reconstruct_constant.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp, C.c_int]
     L.aprh_reconstruct_level.argtypes = [vp, vp, vp, C.c_int]
     L.aprh_patch_check_limits.argtypes = [vp, C.POINTER(AprhPatch)]
+    for f in ("aprh_gradient_cfd", "aprh_gradient_sobel"):
+        getattr(L, f).argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_float, vp]
+    for f in ("aprh_gradient_magnitude_cfd", "aprh_gradient_magnitude_sobel"):
+        getattr(L, f).argtypes = [vp, vp, vp, C.c_int, vp, vp]
     L.aprh_reconstruct_constant_patch.argtypes = [vp, vp, vp, C.c_int, C.POINTER(AprhPatch), vp, C.c_int]
     L.aprh_reconstruct_level_patch.argtypes = [vp, vp, C.POINTER(AprhPatch), vp, C.c_int]
     L.aprh_synth_deconv_device.argtypes = [vp, vp, vp, i64, i64, i64, C.c_uint64, i64, C.c_float, C.c_float, C.c_float,
@@ -711,6 +715,47 @@ class APRFilter:
 
 class APRNumerics:
     """APRNumerics::richardson_lucy (APRNumerics.hpp:130-134, :549-593)."""
+
+    @staticmethod
+    def _gradient(fn, apr, particle_input, particle_output, dimension, delta, dtype):
+        parts = _host_parts(particle_input)
+        code = _dtype_of(parts, dtype)
+        if particle_output is None:
+            particle_output = np.zeros(max(apr.total, 1), np.float32)
+        _check(getattr(lib(), fn)(apr.ctx.h, apr.h, _ptr(parts), code, int(dimension), float(delta), _ptr(particle_output)))
+        return particle_output[:apr.total] if isinstance(particle_output, np.ndarray) else particle_output
+
+    @staticmethod
+    def gradient_cfd(apr, particle_input, particle_output=None, dimension=0, delta=1.0, dtype=None):
+        """APRNumerics::gradient_cfd (APRNumerics.hpp:165-184); dimension 0 y, 1 x, 2 z"""
+        return APRNumerics._gradient("aprh_gradient_cfd", apr, particle_input, particle_output, dimension, delta, dtype)
+
+    @staticmethod
+    def gradient_sobel(apr, particle_input, particle_output=None, dimension=0, delta=1.0, dtype=None):
+        """APRNumerics::gradient_sobel (APRNumerics.hpp:187-213)"""
+        return APRNumerics._gradient("aprh_gradient_sobel", apr, particle_input, particle_output, dimension, delta, dtype)
+
+    @staticmethod
+    def _magnitude(fn, apr, particle_input, particle_output, deltas, dtype):
+        parts = _host_parts(particle_input)
+        code = _dtype_of(parts, dtype)
+        if particle_output is None:
+            particle_output = np.zeros(max(apr.total, 1), np.float32)
+        d = np.ascontiguousarray(deltas, np.float32)
+        if d.size != 3:
+            raise ValueError("deltas: three pixel sizes (y, x, z)")
+        _check(getattr(lib(), fn)(apr.ctx.h, apr.h, _ptr(parts), code, _ptr(d), _ptr(particle_output)))
+        return particle_output[:apr.total] if isinstance(particle_output, np.ndarray) else particle_output
+
+    @staticmethod
+    def gradient_magnitude_cfd(apr, particle_input, particle_output=None, deltas=(1.0, 1.0, 1.0), dtype=None):
+        """APRNumerics::gradient_magnitude_cfd (APRNumerics.hpp:216-248)"""
+        return APRNumerics._magnitude("aprh_gradient_magnitude_cfd", apr, particle_input, particle_output, deltas, dtype)
+
+    @staticmethod
+    def gradient_magnitude_sobel(apr, particle_input, particle_output=None, deltas=(1.0, 1.0, 1.0), dtype=None):
+        """APRNumerics::gradient_magnitude_sobel (APRNumerics.hpp:251-283)"""
+        return APRNumerics._magnitude("aprh_gradient_magnitude_sobel", apr, particle_input, particle_output, deltas, dtype)
 
     @staticmethod
     def richardson_lucy(apr, particle_input, particle_output, psf, number_iterations, use_stencil_downsample=True,

@@ -294,6 +294,8 @@ int richardson_lucy(aprh_context* ctx, aprh_apr& a, const T* in, float* u, const
 void reconstruct_any(aprh_context* ctx, aprh_apr& a, const void* parts, int pdt, int level, void* img, int idt);
 /* level image at `level` (<= l_max) with pixels of finer particles set to `level` */
 void reconstruct_level(aprh_context* ctx, aprh_apr& a, int level, void* img, int idt);
+/* g = sqrtf(g*g [+ gx*gx] [+ gz*gz]) elementwise (gx / gz may be nullptr) */
+void gradient_magnitude_combine(aprh_context* ctx, float* g, const float* gx, const float* gz, uint64_t n);
 /* ReconPatch::check_limits: clamps the bounds; false when a range is empty */
 bool patch_limits(const aprh_apr& a, aprh_patch& p);
 /* patch of the constant (parts != nullptr) or level reconstruction; p already clamped */

@@ -44,6 +44,75 @@ std::vector<HostStencil> stencil_list(const float* stencils, const int32_t* dims
 
 int n_stencil_levels(const aprh_apr& a, int use_ds) { return use_ds ? a.l_max - a.l_min : 1; }
 
+/* APRStencil::get_rescaled_stencils (APRStencil.hpp:135-152): the stencil divided by 2^d for
+   level deltas d = 0 .. nlevels-1 */
+std::vector<HostStencil> rescaled_stencils(const HostStencil& s, int nlevels) {
+    std::vector<HostStencil> v((size_t)std::max(nlevels, 1), s);
+    for (size_t d = 0; d < v.size(); ++d) {
+        const float step = (float)std::pow(2.0, (double)d);
+        for (float& w : v[d].w) w = w / step;
+    }
+    return v;
+}
+
+/* central difference along `dim` (APRNumerics.hpp:165-184) */
+HostStencil cfd_stencil(int dim, float delta) {
+    HostStencil s;
+    s.sy = dim == 0 ? 3 : 1;
+    s.sx = dim == 1 ? 3 : 1;
+    s.sz = dim == 2 ? 3 : 1;
+    s.w = {-1.0f / (2 * delta), 0.0f, 1.0f / (2 * delta)};
+    return s;
+}
+
+/* APRStencil::create_sobel_filter (APRStencil.hpp:254-276) and the 2-D form (:278-297):
+   difference along `dim`, [1/4, 1/2, 1/4] smoothing along the others, (y, x, z) products */
+HostStencil sobel_stencil(int dim, float delta, bool two_d) {
+    const float sm[3] = {0.25f, 0.5f, 0.25f};
+    const float df[3] = {-1.0f / (2 * delta), 0.0f, 1.0f / (2 * delta)};
+    const float* f[3] = {dim == 0 ? df : sm, dim == 1 ? df : sm, dim == 2 ? df : sm};
+    HostStencil s;
+    s.sy = 3;
+    s.sx = 3;
+    s.sz = two_d ? 1 : 3;
+    s.w.assign((size_t)(9 * s.sz), 0.0f);
+    for (int k = 0; k < s.sz; ++k)
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i) s.w[(size_t)((k * 3 + j) * 3 + i)] = two_d ? f[0][i] * f[1][j] : f[0][i] * f[1][j] * f[2][k];
+    return s;
+}
+
+/* the stencil APRNumerics::gradient_cfd / gradient_sobel convolve with (Sobel falls back to the
+   central difference below three dimensions, except along y or x of a 2-D image) */
+HostStencil gradient_stencil(const aprh_apr& a, int dim, float delta, bool sobel) {
+    if (dim < 0 || dim > 2) throw Error(APRH_ERR_INVALID_ARGUMENT, "dimension must be 0 (y), 1 (x) or 2 (z)");
+    const int nd = (a.org[0] > 1) + (a.org[1] > 1) + (a.org[2] > 1);
+    if (sobel && nd == 3) return sobel_stencil(dim, delta, false);
+    if (sobel && nd == 2 && dim != 2) return sobel_stencil(dim, delta, true);
+    return cfd_stencil(dim, delta);
+}
+
+template <typename T>
+void gradient_along(aprh_context* ctx, aprh_apr& a, const T* in, int dim, float delta, bool sobel, float* out) {
+    const std::vector<HostStencil> v = rescaled_stencils(gradient_stencil(a, dim, delta, sobel), a.l_max - a.l_min);
+    convolve<T>(ctx, a, v, in, out, true);
+}
+
+template <typename T>
+void gradient_magnitude(aprh_context* ctx, aprh_apr& a, const T* in, const float* deltas, bool sobel, float* out) {
+    gradient_along<T>(ctx, a, in, 0, deltas[0], sobel, out);
+    Buf gx, gz;
+    if (a.org[1] > 1) {
+        gx.reset(ctx->pool, a.total * 4);
+        gradient_along<T>(ctx, a, in, 1, deltas[1], sobel, gx.as<float>());
+    }
+    if (a.org[2] > 1) {
+        gz.reset(ctx->pool, a.total * 4);
+        gradient_along<T>(ctx, a, in, 2, deltas[2], sobel, gz.as<float>());
+    }
+    gradient_magnitude_combine(ctx, out, a.org[1] > 1 ? gx.as<float>() : nullptr, a.org[2] > 1 ? gz.as<float>() : nullptr, a.total);
+}
+
 void check_apr(aprh_context* ctx, aprh_apr* a) {
     check_ctx(ctx);
     if (!a) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL apr");
@@ -291,6 +360,45 @@ int aprh_reconstruct_level(aprh_context* ctx, aprh_apr* a, void* image, int imag
     reconstruct_level(ctx, *a, a->l_max, out.ptr, image_dtype);
     out.finish(ctx);
     APRH_CATCH
+}
+
+static int gradient_entry(aprh_context* ctx, aprh_apr* a, const void* parts, int dtype, int dim, const float* deltas, bool sobel,
+                          float* out) {
+    APRH_TRY
+    check_apr(ctx, a);
+    if (!parts || !out || (dim < 0 && !deltas)) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (dim >= 0) gradient_stencil(*a, dim, 1.0f, sobel);   /* rejects a bad dimension before any copy */
+    const size_t es = dtype_size(dtype);
+    DevIn in(ctx, parts, std::max<uint64_t>(a->total, 1) * es);
+    DevOut o(ctx, out, std::max<uint64_t>(a->total, 1) * 4);
+    if (dim >= 0) {
+        if (dtype == APRH_DTYPE_U16) gradient_along<uint16_t>(ctx, *a, (const uint16_t*)in.ptr, dim, deltas[0], sobel, (float*)o.ptr);
+        else gradient_along<float>(ctx, *a, (const float*)in.ptr, dim, deltas[0], sobel, (float*)o.ptr);
+    } else {
+        if (dtype == APRH_DTYPE_U16) gradient_magnitude<uint16_t>(ctx, *a, (const uint16_t*)in.ptr, deltas, sobel, (float*)o.ptr);
+        else gradient_magnitude<float>(ctx, *a, (const float*)in.ptr, deltas, sobel, (float*)o.ptr);
+    }
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    o.finish(ctx);
+    APRH_CATCH
+}
+
+int aprh_gradient_cfd(aprh_context* ctx, aprh_apr* a, const void* parts, int dtype, int dimension, float delta, float* out) {
+    if (dimension < 0) dimension = 3;   /* reported as out of range */
+    return gradient_entry(ctx, a, parts, dtype, dimension, &delta, false, out);
+}
+
+int aprh_gradient_sobel(aprh_context* ctx, aprh_apr* a, const void* parts, int dtype, int dimension, float delta, float* out) {
+    if (dimension < 0) dimension = 3;
+    return gradient_entry(ctx, a, parts, dtype, dimension, &delta, true, out);
+}
+
+int aprh_gradient_magnitude_cfd(aprh_context* ctx, aprh_apr* a, const void* parts, int dtype, const float* deltas, float* out) {
+    return gradient_entry(ctx, a, parts, dtype, -1, deltas, false, out);
+}
+
+int aprh_gradient_magnitude_sobel(aprh_context* ctx, aprh_apr* a, const void* parts, int dtype, const float* deltas, float* out) {
+    return gradient_entry(ctx, a, parts, dtype, -1, deltas, true, out);
 }
 
 int aprh_patch_check_limits(const aprh_apr* a, aprh_patch* patch) {

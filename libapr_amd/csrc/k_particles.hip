@@ -808,6 +808,36 @@ struct CropPatch {
 };
 }  // namespace
 
+namespace {
+/* |grad| from the per-axis gradients, in APRNumerics::gradient_magnitude_*'s order
+   (APRNumerics.hpp:216-283): g_y^2, + g_x^2, + g_z^2 (absent axes skipped), sqrtf */
+struct GradMag {
+    struct Args {
+        float* g;          /* in: g_y, out: magnitude */
+        const float* gx;   /* nullptr: no x axis */
+        const float* gz;   /* nullptr: no z axis */
+        uint64_t n;
+        int64_t G;
+    };
+    static constexpr int THREADS = 256;
+    static const char* name() { return "grad_mag"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) {
+        for (uint64_t i = (uint64_t)t * blockDim.x + threadIdx.x; i < a.n; i += (uint64_t)a.G * blockDim.x) {
+            float m = a.g[i] * a.g[i];
+            if (a.gx) m = m + a.gx[i] * a.gx[i];
+            if (a.gz) m = m + a.gz[i] * a.gz[i];
+            a.g[i] = sqrtf(m);
+        }
+    }
+};
+}  // namespace
+
+void gradient_magnitude_combine(aprh_context* ctx, float* g, const float* gx, const float* gz, uint64_t n) {
+    if (n == 0) return;
+    const int64_t G = stride_tiles((int64_t)n, 256 * 4);
+    launch_tiles<GradMag>(ctx, {g, gx, gz, n, G}, G);
+}
+
 bool patch_limits(const aprh_apr& a, aprh_patch& p) {
     /* ReconPatch::check_limits (APRReconstruction.hpp:38-64): extents ceil(dim * 2^level_delta),
        which must fit an int32 */

@@ -1,6 +1,7 @@
 // C++ drop-in check: the apr_amd.hpp facade (reference-shaped API) against the
 // CPU oracle (liboracle.so), on a small synthetic volume.  Exit status 0 and
 // "facade ok" on success.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -168,6 +169,25 @@ int main() {
         bad.x_begin = 9, bad.x_end = 9;
         APRReconstruction::reconstruct_level(apr, pl, bad);
         CHECK(pl.mesh.size() == pc.mesh.size());
+    }
+
+    // gradients: a constant field has none; a bad dimension throws std::invalid_argument
+    {
+        ParticleData<uint16_t> cst;
+        cst.init(P);
+        std::fill(cst.data.begin(), cst.data.end(), (uint16_t)777);
+        ParticleData<float> gm;
+        APRNumerics::gradient_magnitude_sobel(apr, cst, gm);
+        bool zero = gm.data.size() == P;
+        for (float v : gm.data) zero = zero && v == 0.0f;
+        CHECK(zero);
+        bool threw_dim = false;
+        try {
+            APRNumerics::gradient_cfd(apr, cst, gm, 3);
+        } catch (std::invalid_argument&) {
+            threw_dim = true;
+        }
+        CHECK(threw_dim);
     }
 
     // error conventions

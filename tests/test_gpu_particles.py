@@ -379,3 +379,57 @@ def test_reconstruct_patch_limits(oracle, ctx):
             A.APRReconstruction.reconstruct_level(g, patch=A.ReconPatch(**bad))
     with pytest.raises(A.AprhError):
         A.APRReconstruction.reconstruct_level(g, patch=A.ReconPatch(level_delta=-(o.l_max - o.l_min) - 1))
+
+
+def _gradient_stack(o, dim, delta, sobel):
+    """APRNumerics::gradient_cfd / gradient_sobel's stencils (APRNumerics.hpp:165-213,
+    APRStencil.hpp:135-152, :254-297) restated in float32, [z, x, y]"""
+    y, x, z = o.dims
+    nd = int(y > 1) + int(x > 1) + int(z > 1)
+    h = np.float32(1.0) / (np.float32(2.0) * np.float32(delta))
+    diff = np.array([-h, 0.0, h], np.float32)
+    sm = np.array([0.25, 0.5, 0.25], np.float32)
+    fy, fx, fz = (diff if dim == 0 else sm), (diff if dim == 1 else sm), (diff if dim == 2 else sm)
+    if sobel and nd == 3:
+        st = (fy[None, None, :] * fx[None, :, None]) * fz[:, None, None]
+    elif sobel and nd == 2 and dim != 2:
+        st = (fy[None, None, :] * fx[None, :, None]).astype(np.float32)
+    else:
+        st = diff.reshape([(3, 1, 1), (1, 3, 1), (1, 1, 3)][2 - dim])
+    st = st.astype(np.float32)
+    return [st / np.float32(2.0 ** d) for d in range(max(o.l_max - o.l_min, 1))]
+
+
+@pytest.mark.parametrize("shape,nb,seed", CASES[:3])
+@pytest.mark.parametrize("sobel", [False, True])
+def test_gradients(oracle, ctx, shape, nb, seed, sobel):
+    img, o, g = _pair(oracle, ctx, shape, nb, seed)
+    parts = o.sample(img)
+    pf = (parts.astype(np.float32) * np.float32(0.731)).astype(np.float32)
+    fn = A.APRNumerics.gradient_sobel if sobel else A.APRNumerics.gradient_cfd
+    grads = []
+    for dim, delta in ((0, 1.0), (1, 0.7), (2, 2.5)):
+        stack = _gradient_stack(o, dim, delta, sobel)
+        exp = o.convolve_pencil(stack, parts, True)
+        got = fn(g, parts, dimension=dim, delta=delta)
+        assert np.array_equal(_bits(got), _bits(exp)), (dim, int((_bits(got) != _bits(exp)).sum()))
+        assert np.array_equal(_bits(fn(g, pf, dimension=dim, delta=delta)), _bits(o.convolve_pencil(stack, pf, True)))
+        grads.append(exp)
+    # the magnitude in the reference's order: g_y^2, + g_x^2, + g_z^2, sqrtf
+    m = grads[0] * grads[0]
+    m = (m + grads[1] * grads[1]).astype(np.float32)
+    m = (m + grads[2] * grads[2]).astype(np.float32)
+    m = np.sqrt(m).astype(np.float32)
+    mfn = A.APRNumerics.gradient_magnitude_sobel if sobel else A.APRNumerics.gradient_magnitude_cfd
+    assert np.array_equal(_bits(mfn(g, parts, deltas=(1.0, 0.7, 2.5))), _bits(m))
+    with pytest.raises(A.AprhError):
+        fn(g, parts, dimension=3)
+
+
+def test_gradient_of_a_constant_is_zero(oracle, ctx):
+    img, o, g = _pair(oracle, ctx, *CASES[3])
+    c = np.full(o.total, 1234, np.uint16)
+    for fn in (A.APRNumerics.gradient_cfd, A.APRNumerics.gradient_sobel):
+        for dim in range(3):
+            assert np.all(fn(g, c, dimension=dim) == 0.0)
+    assert np.all(A.APRNumerics.gradient_magnitude_sobel(g, c) == 0.0)
