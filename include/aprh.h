@@ -227,6 +227,11 @@ int aprh_gradient_sobel(aprh_context* ctx, aprh_apr* apr, const void* parts, int
 int aprh_gradient_magnitude_cfd(aprh_context* ctx, aprh_apr* apr, const void* parts, int dtype, const float* deltas, float* out);
 int aprh_gradient_magnitude_sobel(aprh_context* ctx, aprh_apr* apr, const void* parts, int dtype, const float* deltas, float* out);
 
+/* APRNumerics::local_std (APRNumerics.hpp:357-389): the box mean (size3 in (y, x, z), stencils
+   downsampled and normalised) of the particles and of their squares (tree values: the squared
+   tree means), then sqrtf(max(E[x^2] - E[x]^2, 0)).  out: float32[total]. */
+int aprh_local_std(aprh_context* ctx, aprh_apr* apr, const void* parts, int dtype, const int32_t* size3, float* out);
+
 /* ReconPatch (APRReconstruction.hpp:20-66): a box in pixels of the image at level
    l_max + level_delta (level_delta > 0 upsamples by 2^level_delta); begins below 0 become 0,
    ends below 0 the full extent ceil(dim * 2^level_delta), larger ends are cut to it */

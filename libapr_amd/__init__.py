@@ -148,6 +148,7 @@ def lib():
         getattr(L, f).argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_float, vp]
     for f in ("aprh_gradient_magnitude_cfd", "aprh_gradient_magnitude_sobel"):
         getattr(L, f).argtypes = [vp, vp, vp, C.c_int, vp, vp]
+    L.aprh_local_std.argtypes = [vp, vp, vp, C.c_int, vp, vp]
     L.aprh_reconstruct_constant_patch.argtypes = [vp, vp, vp, C.c_int, C.POINTER(AprhPatch), vp, C.c_int]
     L.aprh_reconstruct_level_patch.argtypes = [vp, vp, C.POINTER(AprhPatch), vp, C.c_int]
     L.aprh_synth_deconv_device.argtypes = [vp, vp, vp, i64, i64, i64, C.c_uint64, i64, C.c_float, C.c_float, C.c_float,
@@ -756,6 +757,19 @@ class APRNumerics:
     def gradient_magnitude_sobel(apr, particle_input, particle_output=None, deltas=(1.0, 1.0, 1.0), dtype=None):
         """APRNumerics::gradient_magnitude_sobel (APRNumerics.hpp:251-283)"""
         return APRNumerics._magnitude("aprh_gradient_magnitude_sobel", apr, particle_input, particle_output, deltas, dtype)
+
+    @staticmethod
+    def local_std(apr, particle_input, particle_output=None, size=(3, 3, 3), dtype=None):
+        """APRNumerics::local_std (APRNumerics.hpp:357-389); size in (y, x, z)"""
+        parts = _host_parts(particle_input)
+        code = _dtype_of(parts, dtype)
+        if particle_output is None:
+            particle_output = np.zeros(max(apr.total, 1), np.float32)
+        sz = np.ascontiguousarray(size, np.int32)
+        if sz.size != 3:
+            raise ValueError("size: three window sizes (y, x, z)")
+        _check(lib().aprh_local_std(apr.ctx.h, apr.h, _ptr(parts), code, _ptr(sz), _ptr(particle_output)))
+        return particle_output[:apr.total] if isinstance(particle_output, np.ndarray) else particle_output
 
     @staticmethod
     def richardson_lucy(apr, particle_input, particle_output, psf, number_iterations, use_stencil_downsample=True,

@@ -284,8 +284,14 @@ void validate_structure(aprh_context* ctx, const aprh_apr& a);
 void ensure_tree(aprh_context* ctx, aprh_apr& a);
 template <typename T>
 void tree_mean(aprh_context* ctx, aprh_apr& a, const T* parts, float* tree);
+/* tree_in: the tree values to use (APRFilter.hpp:1094-1100's tree_data); nullptr: the tree
+   means of `in` (fill_tree_mean, APRFilter.hpp:1088) */
 template <typename T>
-void convolve(aprh_context* ctx, aprh_apr& a, const std::vector<HostStencil>& stencils, const T* in, float* out, bool reflect);
+void convolve(aprh_context* ctx, aprh_apr& a, const std::vector<HostStencil>& stencils, const T* in, float* out, bool reflect,
+              const float* tree_in = nullptr);
+/* elementwise steps of APRNumerics::local_std: mode 0 p = float(u16 in), 1 p = p * p,
+   2 p = sqrtf(max(p - m * m, 0)) */
+void local_std_step(aprh_context* ctx, int mode, float* p, const void* in_or_m, uint64_t n);
 template <typename T>
 int richardson_lucy(aprh_context* ctx, aprh_apr& a, const T* in, float* u, const std::vector<HostStencil>& psf,
                     const std::vector<HostStencil>& psf_flipped, int max_iter, bool resume, double tol, double* last_rel);

@@ -401,6 +401,38 @@ int aprh_gradient_magnitude_sobel(aprh_context* ctx, aprh_apr* a, const void* pa
     return gradient_entry(ctx, a, parts, dtype, -1, deltas, true, out);
 }
 
+int aprh_local_std(aprh_context* ctx, aprh_apr* a, const void* parts, int dtype, const int32_t* size3, float* out) {
+    APRH_TRY
+    check_apr(ctx, a);
+    if (!parts || !out || !size3) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    const size_t es = dtype_size(dtype);
+    /* APRStencil::create_mean_filter (APRStencil.hpp:236-244): 1 / (sy sx sz) from a double */
+    std::vector<float> box((size_t)std::max(size3[0], 1) * std::max(size3[1], 1) * std::max(size3[2], 1));
+    HostStencil hs = host_stencil(box.data(), size3[0], size3[1], size3[2]);
+    std::fill(hs.w.begin(), hs.w.end(), (float)(1.0 / (float)hs.w.size()));
+    const std::vector<HostStencil> v = downsampled_stencils(hs, n_stencil_levels(*a, 1), true);
+    DevIn in(ctx, parts, std::max<uint64_t>(a->total, 1) * es);
+    DevOut o(ctx, out, std::max<uint64_t>(a->total, 1) * 4);
+    if (a->total > 0) {
+        /* APRNumerics::local_std (APRNumerics.hpp:357-389): a float copy x with its tree means t;
+           m = box(x; t); out = box(x^2; t^2); out = sqrtf(max(out - m^2, 0)) */
+        Buf x(ctx->pool, a->total * 4), m(ctx->pool, a->total * 4);
+        if (dtype == APRH_DTYPE_U16) local_std_step(ctx, 0, x.as<float>(), in.ptr, a->total);
+        else launch_copy(ctx, x.ptr(), in.ptr, a->total * 4);
+        ensure_tree(ctx, *a);
+        Buf t(ctx->pool, std::max<uint64_t>(a->t_total, 1) * 4);
+        tree_mean<float>(ctx, *a, x.as<float>(), t.as<float>());
+        convolve<float>(ctx, *a, v, x.as<float>(), m.as<float>(), true, t.as<float>());
+        local_std_step(ctx, 1, x.as<float>(), nullptr, a->total);
+        local_std_step(ctx, 1, t.as<float>(), nullptr, a->t_total);
+        convolve<float>(ctx, *a, v, x.as<float>(), (float*)o.ptr, true, t.as<float>());
+        local_std_step(ctx, 2, (float*)o.ptr, m.as<float>(), a->total);
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    }
+    o.finish(ctx);
+    APRH_CATCH
+}
+
 int aprh_patch_check_limits(const aprh_apr* a, aprh_patch* patch) {
     APRH_TRY
     if (!a || !patch) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");

@@ -648,14 +648,20 @@ bool rank1_factors(const HostStencil& s, std::vector<float>& fy, std::vector<flo
 }
 
 template <typename T>
-void convolve(aprh_context* ctx, aprh_apr& a, const std::vector<HostStencil>& stencils, const T* in, float* out, bool reflect) {
+void convolve(aprh_context* ctx, aprh_apr& a, const std::vector<HostStencil>& stencils, const T* in, float* out, bool reflect,
+              const float* tree_in) {
     if (stencils.empty()) throw Error(APRH_ERR_INVALID_ARGUMENT, "no stencils");
     if (a.total == 0) return;
     ensure_tree(ctx, a);
     hipStream_t st = ctx->stream;
-    Buf tree(ctx->pool, std::max<uint64_t>(a.t_total, 1) * 4);
-    /* APRFilter.hpp:1088: tree means of the input */
-    tree_mean(ctx, a, in, tree.as<float>());
+    Buf tree_buf;
+    const float* tree_vals = tree_in;
+    if (!tree_vals) {
+        /* APRFilter.hpp:1088: tree means of the input */
+        tree_buf.reset(ctx->pool, std::max<uint64_t>(a.t_total, 1) * 4);
+        tree_mean(ctx, a, in, tree_buf.as<float>());
+        tree_vals = tree_buf.as<float>();
+    }
     std::vector<float> flat;
     std::vector<StencilDev> sd;
     bool any_sep = false;
@@ -690,7 +696,7 @@ void convolve(aprh_context* ctx, aprh_apr& a, const std::vector<HostStencil>& st
     float* cur = D0.as<float>();
     float* other = D1.as<float>();
     for (int l = a.l_min; l <= a.l_max; ++l) {
-        build_level<T, float>(ctx, a, AR, TR, l, prev, cur, in, tree.as<float>(), true);
+        build_level<T, float>(ctx, a, AR, TR, l, prev, cur, in, tree_vals, true);
         const int sn = std::min((int)stencils.size() - 1, a.l_max - l);
         const StencilDev& d = sd[(size_t)sn];
         if (d.sep) {
@@ -838,6 +844,39 @@ void gradient_magnitude_combine(aprh_context* ctx, float* g, const float* gx, co
     launch_tiles<GradMag>(ctx, {g, gx, gz, n, G}, G);
 }
 
+namespace {
+struct LocalStdStep {
+    struct Args {
+        int mode;
+        float* p;
+        const void* q;
+        uint64_t n;
+        int64_t G;
+    };
+    static constexpr int THREADS = 256;
+    static const char* name() { return "local_std_step"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) {
+        for (uint64_t i = (uint64_t)t * blockDim.x + threadIdx.x; i < a.n; i += (uint64_t)a.G * blockDim.x) {
+            if (a.mode == 0) {
+                a.p[i] = (float)((const uint16_t*)a.q)[i];
+            } else if (a.mode == 1) {
+                a.p[i] = a.p[i] * a.p[i];
+            } else {
+                const float m = ((const float*)a.q)[i];
+                const float d = a.p[i] - m * m;
+                a.p[i] = sqrtf(d < 0.0f ? 0.0f : d);
+            }
+        }
+    }
+};
+}  // namespace
+
+void local_std_step(aprh_context* ctx, int mode, float* p, const void* q, uint64_t n) {
+    if (n == 0) return;
+    const int64_t G = stride_tiles((int64_t)n, 256 * 4);
+    launch_tiles<LocalStdStep>(ctx, {mode, p, q, n, G}, G);
+}
+
 bool patch_limits(const aprh_apr& a, aprh_patch& p) {
     /* ReconPatch::check_limits (APRReconstruction.hpp:38-64): extents ceil(dim * 2^level_delta),
        which must fit an int32 */
@@ -894,8 +933,8 @@ void reconstruct_any(aprh_context* ctx, aprh_apr& a, const void* parts, int pdt,
 
 template void tree_mean<uint16_t>(aprh_context*, aprh_apr&, const uint16_t*, float*);
 template void tree_mean<float>(aprh_context*, aprh_apr&, const float*, float*);
-template void convolve<uint16_t>(aprh_context*, aprh_apr&, const std::vector<HostStencil>&, const uint16_t*, float*, bool);
-template void convolve<float>(aprh_context*, aprh_apr&, const std::vector<HostStencil>&, const float*, float*, bool);
+template void convolve<uint16_t>(aprh_context*, aprh_apr&, const std::vector<HostStencil>&, const uint16_t*, float*, bool, const float*);
+template void convolve<float>(aprh_context*, aprh_apr&, const std::vector<HostStencil>&, const float*, float*, bool, const float*);
 
 template <typename T>
 int richardson_lucy(aprh_context* ctx, aprh_apr& a, const T* in, float* u, const std::vector<HostStencil>& psf,

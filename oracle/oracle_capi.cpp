@@ -378,6 +378,38 @@ int orc_richardson_lucy_f32(void* hp, const float* in, float* u, const float* ps
     return richardson_lucy(h->r.info, h->r.lin, h->get_tree(), in, u, m, niter, use_ds != 0, normalize != 0, resume != 0, tol, rel);
 }
 
+/* APRNumerics::local_std (APRNumerics.hpp:357-389): the box mean (create_mean_filter,
+   APRStencil.hpp:236-244, stencils downsampled and normalised) of the float copy of the input
+   with its tree means, and of its square with the squared tree means; then
+   sqrtf(max(E[x^2] - E[x]^2, 0)) */
+extern "C++" {
+template <typename PartT>
+static void local_std_impl(Handle* h, const PartT* in, const int* size, float* out) {
+    const GenInfo& g = h->r.info;
+    const Linear& L = h->r.lin;
+    const size_t P = (size_t)L.total;
+    Mesh<float> box(size[0], size[1], size[2]);
+    const float cnt = (float)box.size();
+    std::fill(box.v.begin(), box.v.end(), (float)(1.0 / cnt));
+    std::vector<Mesh<float>> st;
+    downsampled_stencils(box, st, g.l_max - g.l_min, true);
+    std::vector<float> x(P), tr, mean(P);
+    for (size_t i = 0; i < P; ++i) x[i] = (float)in[i];
+    fill_tree_mean(g, L, h->get_tree(), x.data(), tr);
+    convolve_pencil(g, L, h->get_tree(), st, x.data(), tr.data(), mean.data(), true);
+    for (auto& v : x) v = v * v;
+    for (auto& v : tr) v = v * v;
+    convolve_pencil(g, L, h->get_tree(), st, x.data(), tr.data(), out, true);
+    for (size_t i = 0; i < P; ++i) {
+        const float d = out[i] - mean[i] * mean[i];
+        out[i] = std::sqrt(std::max(d, 0.0f));
+    }
+}
+}  // extern "C++"
+
+void orc_local_std_u16(void* hp, const uint16_t* in, const int* size, float* out) { local_std_impl((Handle*)hp, in, size, out); }
+void orc_local_std_f32(void* hp, const float* in, const int* size, float* out) { local_std_impl((Handle*)hp, in, size, out); }
+
 /* piecewise-constant reconstruction at level l_max + level_delta; dtypes 0 u16, 1 f32 */
 void orc_reconstruct(void* hp, const void* parts, int pdt, int level_delta, void* img, int idt) {
     Handle* h = (Handle*)hp;

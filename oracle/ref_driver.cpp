@@ -483,6 +483,23 @@ void orc_convolve_f32(void* hp, const float* stencils, const int* dims, int n, c
     std::copy(po.data.begin(), po.data.end(), out);
 }
 
+void orc_local_std_u16(void* hp, const uint16_t* in, const int* size, float* out) {
+    Handle* h = (Handle*)hp;
+    ParticleData<uint16_t> pi;
+    ParticleData<float> po;
+    fill_parts(pi, in, h->apr.total_number_particles());
+    APRNumerics::local_std(h->apr, pi, po, std::vector<int>{size[0], size[1], size[2]});
+    std::copy(po.data.begin(), po.data.end(), out);
+}
+
+void orc_local_std_f32(void* hp, const float* in, const int* size, float* out) {
+    Handle* h = (Handle*)hp;
+    ParticleData<float> pi, po;
+    fill_parts(pi, in, h->apr.total_number_particles());
+    APRNumerics::local_std(h->apr, pi, po, std::vector<int>{size[0], size[1], size[2]});
+    std::copy(po.data.begin(), po.data.end(), out);
+}
+
 int orc_richardson_lucy_u16(void* hp, const uint16_t* in, float* u, const float* psf, int sy, int sx, int sz, int niter, int use_ds,
                             int normalize, int resume, double tol, double* rel) {
     (void)tol;

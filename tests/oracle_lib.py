@@ -79,6 +79,8 @@ def _bind(lib):
     for n in ("orc_convolve_pencil_f32", "orc_convolve_f32"):
         getattr(lib, n).argtypes = [C.c_void_p, f32p, i32p, C.c_int, f32p, f32p, C.c_int]
     lib.orc_convolve_pencil_u16.argtypes = [C.c_void_p, f32p, i32p, C.c_int, u16p, f32p, C.c_int]
+    lib.orc_local_std_u16.argtypes = [C.c_void_p, u16p, i32p, f32p]
+    lib.orc_local_std_f32.argtypes = [C.c_void_p, f32p, i32p, f32p]
     for n, t in (("orc_richardson_lucy_u16", u16p), ("orc_richardson_lucy_f32", f32p)):
         getattr(lib, n).argtypes = [C.c_void_p, t, f32p, f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_double, C.POINTER(C.c_double)]
@@ -180,6 +182,16 @@ class OracleAPR:
         else:
             f = self.lib.orc_convolve_pencil_f32 if which == "pencil" else self.lib.orc_convolve_f32
             f(self.h, flat, dims, len(stencils), np.ascontiguousarray(parts, np.float32), out, int(reflect))
+        return out[:self.total]
+
+    def local_std(self, parts, size=(3, 3, 3)):
+        """APRNumerics::local_std (APRNumerics.hpp:357-389); size in (y, x, z)"""
+        out = np.zeros(max(self.total, 1), np.float32)
+        sz = np.ascontiguousarray(size, np.int32)
+        if parts.dtype == np.uint16:
+            self.lib.orc_local_std_u16(self.h, np.ascontiguousarray(parts), sz, out)
+        else:
+            self.lib.orc_local_std_f32(self.h, np.ascontiguousarray(parts, np.float32), sz, out)
         return out[:self.total]
 
     def richardson_lucy(self, parts, psf, niter, use_ds=True, normalize=True, u=None, tol=0.0):

@@ -433,3 +433,13 @@ def test_gradient_of_a_constant_is_zero(oracle, ctx):
         for dim in range(3):
             assert np.all(fn(g, c, dimension=dim) == 0.0)
     assert np.all(A.APRNumerics.gradient_magnitude_sobel(g, c) == 0.0)
+
+
+@pytest.mark.parametrize("shape,nb,seed", CASES[:3])
+@pytest.mark.parametrize("size", [(3, 3, 3), (5, 3, 1)])
+def test_local_std(oracle, ctx, shape, nb, seed, size):
+    img, o, g = _pair(oracle, ctx, shape, nb, seed)
+    parts = o.sample(img)
+    assert np.array_equal(_bits(A.APRNumerics.local_std(g, parts, size=size)), _bits(o.local_std(parts, size)))
+    pf = (parts.astype(np.float32) * np.float32(0.37)).astype(np.float32)
+    assert np.array_equal(_bits(A.APRNumerics.local_std(g, pf, size=size)), _bits(o.local_std(pf, size)))

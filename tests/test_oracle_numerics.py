@@ -240,3 +240,23 @@ def test_reconstruct_then_sample_is_identity():
     rec = o.reconstruct(parts)
     assert rec.shape == img.shape
     assert np.array_equal(o.sample(rec), parts)
+
+
+@pytest.mark.parametrize("shape,nb,seed", CASES[:2])
+@pytest.mark.parametrize("size", [(3, 3, 3), (5, 3, 1)])
+def test_local_std_equal_reference(shape, nb, seed, size):
+    """APRNumerics::local_std: the oracle's restatement against the reference build; a constant
+    gives a deviation at the float rounding level of E[x^2] - E[x]^2 only (the method subtracts
+    two box means, so it is not exactly zero)"""
+    img, o = make_apr(shape, nb, seed, "oracle")
+    parts = o.sample(img)
+    got = o.local_std(parts, size)
+    c = o.local_std(np.full(o.total, 321, np.uint16), size)
+    assert float(c.max()) < 2e-3 * 321
+    pf = parts.astype(np.float32)
+    assert float(got.max()) > 1.0
+    if O.load("ref") is None:
+        pytest.skip("reference build (oracle/_ref) not available")
+    _, r = make_apr(shape, nb, seed, "ref")
+    assert np.array_equal(got.view(np.uint32), r.local_std(parts, size).view(np.uint32))
+    assert np.array_equal(o.local_std(pf, size).view(np.uint32), r.local_std(pf, size).view(np.uint32))
