@@ -11,6 +11,7 @@
  *   APRFilter::convolve(_pencil)         numerics/APRFilter.hpp:67-149
  *   APRNumerics::richardson_lucy         numerics/APRNumerics.hpp:126-134
  *   APRNumerics::gradient_* (cfd, sobel, magnitude)  numerics/APRNumerics.hpp:165-283
+ *   APRNumerics::local_std               numerics/APRNumerics.hpp:357-389
  *   APRReconstruction::reconstruct_constant  numerics/APRReconstruction.hpp:80, :297
  *   APRReconstruction::reconstruct_level     numerics/APRReconstruction.hpp:131-171
  *   ReconPatch and the patch reconstructions numerics/APRReconstruction.hpp:20-66, :186-196, :296-490
@@ -318,6 +319,13 @@ template <typename In>
 void gradient_magnitude_sobel(APR& apr, const ParticleData<In>& in, ParticleData<float>& out, const std::vector<float>& deltas = {1.0f, 1.0f, 1.0f}) {
     out.init(apr.total_number_particles());
     detail::check(aprh_gradient_magnitude_sobel(apr.context().get(), apr.get(), in.data.data(), detail::dtype<In>::code, deltas.data(), out.data.data()));
+}
+/* local_std (APRNumerics.hpp:357-389); size in (y, x, z) */
+template <typename In>
+void local_std(APR& apr, const ParticleData<In>& in, ParticleData<float>& out, const std::vector<int>& size = {3, 3, 3}) {
+    out.init(apr.total_number_particles());
+    const int32_t sz[3] = {size[0], size[1], size[2]};
+    detail::check(aprh_local_std(apr.context().get(), apr.get(), in.data.data(), detail::dtype<In>::code, sz, out.data.data()));
 }
 }  // namespace APRNumerics
 
