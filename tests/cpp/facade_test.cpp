@@ -23,6 +23,7 @@ void orc_convolve_pencil_u16(void* h, const float* stencils, const int* dims, in
 int orc_richardson_lucy_u16(void* h, const uint16_t* in, float* u, const float* psf, int sy, int sx, int sz, int niter, int use_ds,
                             int normalize, int resume, double tol, double* rel);
 void orc_reconstruct(void* h, const void* parts, int pdt, int level_delta, void* img, int idt);
+void orc_local_std_u16(void* h, const uint16_t* in, const int* size, float* out);
 void orc_apr_free(void* h);
 }
 
@@ -169,6 +170,16 @@ int main() {
         bad.x_begin = 9, bad.x_end = 9;
         APRReconstruction::reconstruct_level(apr, pl, bad);
         CHECK(pl.mesh.size() == pc.mesh.size());
+    }
+
+    // local standard deviation in a 3x3x3 window
+    {
+        ParticleData<float> ls;
+        APRNumerics::local_std(apr, parts, ls);
+        std::vector<float> ols(P);
+        const int sz3[3] = {3, 3, 3};
+        orc_local_std_u16(o, oparts.data(), sz3, ols.data());
+        CHECK(same_bytes(ls.data, ols.data(), P));
     }
 
     // gradients: a constant field has none; a bad dimension throws std::invalid_argument
