@@ -348,23 +348,26 @@ __device__ __forceinline__ WorkRows work_rows(const RowTable& t, const uint8_t* 
     return o;
 }
 
-/* particles of each APR row: one wave per work item, 1024 cells per step */
+/* particles of each APR row: 16 lanes per work item, 256 cells per step (short rows of small
+   volumes and coarse levels keep the lanes busy; long rows loop) */
+constexpr int COUNT_LANES = 16;
 __device__ __forceinline__ void row_count_body(const uint8_t* __restrict__ maps, const RowTable& t, int64_t nrows, uint32_t* cnt,
                                                int64_t G, int64_t blk) {
-    const int lane = threadIdx.x & 63;
-    const int64_t wid = (blk * blockDim.x + threadIdx.x) >> 6;
-    const int64_t nw = (G * blockDim.x) >> 6;
+    const int lane = threadIdx.x & (COUNT_LANES - 1);
+    const int64_t wid = (blk * blockDim.x + threadIdx.x) / COUNT_LANES;
+    const int64_t nw = (G * blockDim.x) / COUNT_LANES;
     (void)nrows;
     for (int64_t w = wid; w < t.n_work; w += nw) {
         const WorkRows wr = work_rows(t, maps, w);
         uint32_t c = 0;
-        for (int base = 0; base < wr.ylen; base += 64 * CPL) {
+        for (int base = 0; base < wr.ylen; base += COUNT_LANES * CPL) {
             int wt[CPL];
             chunk_weights(t, wr.li, wr.p, wr.ylen, base + lane * CPL, wt);
 #pragma unroll
             for (int j = 0; j < CPL; ++j) c += (uint32_t)wt[j];
         }
-        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+        /* the 16 lanes of a group share w, so they are active together */
+        for (int o = COUNT_LANES / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, COUNT_LANES);
         if (lane == 0) {
 #pragma unroll
             for (int k = 0; k < 4; ++k)
@@ -878,7 +881,7 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
     /* per-row counts, exclusive scan over rows in particle order, xz_end_vec, y_vec */
     Buf cnt(ctx->pool, (size_t)rp * 4);
     Buf rowoff(ctx->pool, (size_t)rp * 8);
-    const int64_t RG = stride_tiles(t.n_work, 4, 16384);   /* a wave per work item, four waves per block */
+    const int64_t RG = stride_tiles(t.n_work, 16, 16384);   /* 16 lanes per work item, 16 items per block */
     launch_tiles<RowCount>(ctx, {P, t, rp, cnt.as<uint32_t>(), RG}, RG);
     if (ctx->async_mode) {
         /* y_vec and xz_end_vec are preallocated (y_vec for the upper bound P <= N); the count
@@ -908,7 +911,8 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
         ri.parts = sr->parts;
         sr->done = true;
     }
-    launch_tiles<RowFill>(ctx, {P, t, rp, rowoff.as<uint64_t>(), a.y_vec.as<uint16_t>(), RG, ri}, RG);
+    const int64_t FG = stride_tiles(t.n_work, 4, 16384);    /* a wave per work item, four waves per block */
+    launch_tiles<RowFill>(ctx, {P, t, rp, rowoff.as<uint64_t>(), a.y_vec.as<uint16_t>(), FG, ri}, FG);
     if (ctx->keep_intermediates) {
         a.pct.clear();
         a.pct_dims.assign(3 * (top + 1), 0);
