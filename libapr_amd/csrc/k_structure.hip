@@ -376,10 +376,11 @@ __device__ __forceinline__ void row_count_body(const uint8_t* __restrict__ maps,
     }
 }
 
-/* y values of each APR row from its scanned offset.  A lane takes CPL consecutive cells of a
-   1024-cell chunk; a wave prefix sum of the lanes' counts places each lane's values in an LDS
-   stage, which the wave then copies to y_vec with consecutive lanes on consecutive entries
-   (once per destination row). */
+/* y values of each APR row from its scanned offset.  A group of FILL_LANES lanes (a whole wave
+   for long rows, 16 lanes for short ones) takes one work item; a lane takes CPL consecutive
+   cells of a FILL_LANES * CPL-cell chunk; a prefix sum over the group places each lane's values
+   in the group's LDS stage, which the group then copies to y_vec with consecutive lanes on
+   consecutive entries (once per destination row). */
 constexpr int FILL_WAVES = 4;
 /* level images for a row fill that also samples: APR level index li -> image of that level
    (the converted image at l_max, the mean pyramid below), rows in (z, x) order like the APR rows */
@@ -389,15 +390,18 @@ struct RowImages {
     uint16_t* parts;   /* nullptr: no sampling */
 };
 
+template <int FILL_LANES>
 __device__ __forceinline__ void row_fill_body(const uint8_t* __restrict__ maps, const RowTable& t, int64_t nrows,
                                               const uint64_t* __restrict__ row_off, uint16_t* y_vec, const RowImages& ri, int64_t G,
                                               int64_t blk) {
-    __shared__ uint16_t stage[FILL_WAVES][2 * 64 * CPL];
-    const int lane = threadIdx.x & 63;
-    uint16_t* st = stage[threadIdx.x >> 6];
-    const int64_t wid = (blk * blockDim.x + threadIdx.x) >> 6;
-    const int64_t nw = (G * blockDim.x) >> 6;
+    constexpr int FILL_GROUPS = FILL_WAVES * 64 / FILL_LANES;
+    __shared__ uint16_t stage[FILL_GROUPS][2 * FILL_LANES * CPL];
+    const int lane = threadIdx.x & (FILL_LANES - 1);
+    uint16_t* st = stage[threadIdx.x / FILL_LANES];
+    const int64_t wid = (blk * blockDim.x + threadIdx.x) / FILL_LANES;
+    const int64_t nw = (G * blockDim.x) / FILL_LANES;
     (void)nrows;
+    /* the 16 lanes of a group share w (and every loop bound below), so they are active together */
     for (int64_t w = wid; w < t.n_work; w += nw) {
         const WorkRows wr = work_rows(t, maps, w);
         const bool top = (wr.li == t.n_levels - 1);
@@ -405,20 +409,20 @@ __device__ __forceinline__ void row_fill_body(const uint8_t* __restrict__ maps, 
 #pragma unroll
         for (int k = 0; k < 4; ++k)
             if (k < wr.n) pos[k] = row_off[wr.r[k]];
-        for (int base = 0; base < wr.ylen; base += 64 * CPL) {
+        for (int base = 0; base < wr.ylen; base += FILL_LANES * CPL) {
             int wt[CPL];
             const int c0 = base + lane * CPL;
             chunk_weights(t, wr.li, wr.p, wr.ylen, c0, wt);
             int c = 0;
 #pragma unroll
             for (int j = 0; j < CPL; ++j) c += wt[j];
-            /* inclusive prefix over the wave */
+            /* inclusive prefix over the group */
             int incl = c;
-            for (int o = 1; o < 64; o <<= 1) {
-                const int v = __shfl_up(incl, o);
+            for (int o = 1; o < FILL_LANES; o <<= 1) {
+                const int v = __shfl_up(incl, o, FILL_LANES);
                 if (lane >= o) incl += v;
             }
-            const int total = __shfl(incl, 63);
+            const int total = __shfl(incl, FILL_LANES - 1, FILL_LANES);
             int q = incl - c;
 #pragma unroll
             for (int j = 0; j < CPL; ++j) {
@@ -437,25 +441,25 @@ __device__ __forceinline__ void row_fill_body(const uint8_t* __restrict__ maps, 
                     uint16_t* pdst = ri.parts + pos[k];
                     int i = lane;
                     /* four particles per lane in flight */
-                    for (; i + 192 < total; i += 256) {
-                        const uint16_t y0 = st[i], y1 = st[i + 64], y2 = st[i + 128], y3 = st[i + 192];
+                    for (; i + 3 * FILL_LANES < total; i += 4 * FILL_LANES) {
+                        const uint16_t y0 = st[i], y1 = st[i + FILL_LANES], y2 = st[i + 2 * FILL_LANES], y3 = st[i + 3 * FILL_LANES];
                         const uint16_t v0 = irow[y0], v1 = irow[y1], v2 = irow[y2], v3 = irow[y3];
                         dst[i] = y0;
-                        dst[i + 64] = y1;
-                        dst[i + 128] = y2;
-                        dst[i + 192] = y3;
+                        dst[i + FILL_LANES] = y1;
+                        dst[i + 2 * FILL_LANES] = y2;
+                        dst[i + 3 * FILL_LANES] = y3;
                         pdst[i] = v0;
-                        pdst[i + 64] = v1;
-                        pdst[i + 128] = v2;
-                        pdst[i + 192] = v3;
+                        pdst[i + FILL_LANES] = v1;
+                        pdst[i + 2 * FILL_LANES] = v2;
+                        pdst[i + 3 * FILL_LANES] = v3;
                     }
-                    for (; i < total; i += 64) {
+                    for (; i < total; i += FILL_LANES) {
                         const uint16_t yv = st[i];
                         dst[i] = yv;
                         pdst[i] = irow[yv];
                     }
                 } else {
-                    for (int i = lane; i < total; i += 64) dst[i] = st[i];
+                    for (int i = lane; i < total; i += FILL_LANES) dst[i] = st[i];
                 }
                 pos[k] += (uint64_t)total;
             }
@@ -594,6 +598,7 @@ struct RowCount {
     static const char* name() { return "row_count"; }
     static __device__ __forceinline__ void run(const Args& a, int64_t t) { row_count_body(a.maps, a.t, a.nrows, a.cnt, a.G, t); }
 };
+template <int LANES>
 struct RowFill {
     struct Args {
         const uint8_t* maps;
@@ -605,9 +610,9 @@ struct RowFill {
         RowImages ri;
     };
     static constexpr int THREADS = 256;
-    static const char* name() { return "row_fill"; }
+    static const char* name() { return LANES == 64 ? "row_fill" : "row_fill_short"; }
     static __device__ __forceinline__ void run(const Args& a, int64_t t) {
-        row_fill_body(a.maps, a.t, a.nrows, a.row_off, a.y_vec, a.ri, a.G, t);
+        row_fill_body<LANES>(a.maps, a.t, a.nrows, a.row_off, a.y_vec, a.ri, a.G, t);
     }
 };
 struct RowsXz {
@@ -911,8 +916,14 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
         ri.parts = sr->parts;
         sr->done = true;
     }
-    const int64_t FG = stride_tiles(t.n_work, 4, 16384);    /* a wave per work item, four waves per block */
-    launch_tiles<RowFill>(ctx, {P, t, rp, rowoff.as<uint64_t>(), a.y_vec.as<uint16_t>(), FG, ri}, FG);
+    /* a wave per work item when the longest rows fill one (1024 cells), else 16 lanes */
+    if (t.map_y[t.n_levels - 1] >= 512) {
+        const int64_t FG = stride_tiles(t.n_work, FILL_WAVES, 16384);
+        launch_tiles<RowFill<64>>(ctx, {P, t, rp, rowoff.as<uint64_t>(), a.y_vec.as<uint16_t>(), FG, ri}, FG);
+    } else {
+        const int64_t FG = stride_tiles(t.n_work, FILL_WAVES * 4, 16384);
+        launch_tiles<RowFill<16>>(ctx, {P, t, rp, rowoff.as<uint64_t>(), a.y_vec.as<uint16_t>(), FG, ri}, FG);
+    }
     if (ctx->keep_intermediates) {
         a.pct.clear();
         a.pct_dims.assign(3 * (top + 1), 0);
