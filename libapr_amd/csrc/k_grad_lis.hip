@@ -141,8 +141,8 @@ constexpr int GZC = 8;
 template <bool UNIT>
 __device__ __forceinline__ void grad_ds_vec_body(const uint16_t* __restrict__ img, uint16_t* grad, float* mean, int64_t y_num,
                                                  int64_t x_num, int64_t z_num, int64_t yd_num, float h2x, float h2y, float h2z,
-                                                 int accum, int64_t bx, int64_t xd, int64_t zc, int64_t xd_num, int64_t zd_num) {
-    const int64_t yg = bx * blockDim.x + threadIdx.x;   /* group of 8 y */
+                                                 int accum, int64_t yg, int64_t xd, int64_t zc, int64_t xd_num, int64_t zd_num) {
+    /* yg: group of 8 y */
     if (yg * 8 >= y_num) return;
     const int64_t xy = x_num * y_num;
     const int64_t y0 = yg * 8;
@@ -872,9 +872,14 @@ struct GradDsVec {
     static constexpr int THREADS = 256;
     static const char* name() { return "gradient_ds_vec"; }
     static __device__ __forceinline__ void run(const Args& a, int64_t t) {
-        const int64_t bx = t % a.gx, r = t / a.gx;
+        /* lanes numbered densely over (y group, x column pair, run of GZC z planes), so short y
+           rows still fill the blocks */
+        const int64_t ny = a.y_num / 8, nz = (a.zd_num + GZC - 1) / GZC;
+        const int64_t L = t * blockDim.x + threadIdx.x;
+        if (L >= ny * a.xd_num * nz) return;
+        const int64_t yg = L % ny, r = L / ny;
         grad_ds_vec_body<UNIT>((const uint16_t*)a.img, (uint16_t*)a.grad, a.mean, a.y_num, a.x_num, a.z_num, a.yd_num, a.h2x, a.h2y,
-                               a.h2z, a.accum, bx, r % a.xd_num, r / a.xd_num, a.xd_num, a.zd_num);
+                               a.h2z, a.accum, yg, r % a.xd_num, r / a.xd_num, a.xd_num, a.zd_num);
     }
 };
 template <typename G>
@@ -938,7 +943,7 @@ void launch_grad_ds_u16(aprh_context* ctx, const uint16_t* img, uint16_t* grad, 
         const int64_t gx = cdiv(y / 8, 256);
         const bool unit = hx == 1.0f && hy == 1.0f && hz == 1.0f;
         GradArgs a{img, grad, mean, y, x, z, yd, 2.0f * hx, 2.0f * hy, 2.0f * hz, accum ? 1 : 0, gx, xd, zd};
-        const int64_t tiles = gx * xd * cdiv(zd, GZC);   /* (y block, x column pair, run of GZC z planes) */
+        const int64_t tiles = cdiv((y / 8) * xd * cdiv(zd, GZC), 256);   /* 256 lanes per tile */
         if (unit) launch_tiles<GradDsVec<true>>(ctx, a, tiles);
         else launch_tiles<GradDsVec<false>>(ctx, a, tiles);
     } else {
