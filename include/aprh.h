@@ -326,6 +326,11 @@ int aprh_synth_blobs_device(aprh_context* ctx, uint16_t* out, int64_t y_num, int
 int aprh_synth_deconv_device(aprh_context* ctx, float* gt, uint16_t* noisy, int64_t y_num, int64_t x_num, int64_t z_num,
                              uint64_t seed, int64_t n_blobs, float sigma_min, float sigma_max, float amp_min, float amp_max,
                              float background, const float* psf_sigma3, int32_t psf_taps);
+/* the same deconvolution workload computed on the host (gt and noisy host memory), byte for
+   byte equal to aprh_synth_deconv_device */
+int aprh_synth_deconv_host(float* gt, uint16_t* noisy, int64_t y_num, int64_t x_num, int64_t z_num, uint64_t seed,
+                           int64_t n_blobs, float sigma_min, float sigma_max, float amp_min, float amp_max, float background,
+                           const float* psf_sigma3, int32_t psf_taps);
 
 /* ---- device memory helpers for callers without their own allocator ---- */
 int aprh_malloc(aprh_context* ctx, size_t bytes, void** out);

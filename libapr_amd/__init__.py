@@ -153,6 +153,8 @@ def lib():
     L.aprh_reconstruct_level_patch.argtypes = [vp, vp, C.POINTER(AprhPatch), vp, C.c_int]
     L.aprh_synth_deconv_device.argtypes = [vp, vp, vp, i64, i64, i64, C.c_uint64, i64, C.c_float, C.c_float, C.c_float,
                                            C.c_float, C.c_float, vp, C.c_int32]
+    L.aprh_synth_deconv_host.argtypes = [vp, vp, i64, i64, i64, C.c_uint64, i64, C.c_float, C.c_float, C.c_float,
+                                         C.c_float, C.c_float, vp, C.c_int32]
     L.aprh_sample_image_f32.argtypes = [vp, vp, vp, vp]
     L.aprh_cnn_create.argtypes = [vp, C.c_uint64, C.POINTER(vp)]
     L.aprh_cnn_destroy.argtypes = [vp]
@@ -894,6 +896,19 @@ def synth_deconv(shape_zxy, seed, n_blobs, gt_out, noisy_out, sigma=(1.0, 4.0), 
     ps = np.array(psf_sigma, np.float32)
     _check(lib().aprh_synth_deconv_device(ctx.h, _ptr(gt_out), _ptr(noisy_out), y, x, z, seed, n_blobs, sigma[0], sigma[1],
                                           amp[0], amp[1], background, _ptr(ps), psf_taps))
+
+
+def synth_deconv_host(shape_zxy, seed, n_blobs, sigma=(1.0, 4.0), amp=(200.0, 2000.0), background=100.0,
+                      psf_sigma=(1.0, 1.0, 2.0), psf_taps=9):
+    """the deconvolution workload of synth_deconv computed on the host (same bytes):
+    returns (gt f32 (z, x, y), noisy u16 (z, x, y))"""
+    z, x, y = shape_zxy
+    gt = np.zeros((z, x, y), np.float32)
+    noisy = np.zeros((z, x, y), np.uint16)
+    ps = np.array(psf_sigma, np.float32)
+    _check(lib().aprh_synth_deconv_host(_ptr(gt), _ptr(noisy), y, x, z, seed, n_blobs, sigma[0], sigma[1], amp[0], amp[1],
+                                        background, _ptr(ps), psf_taps))
+    return gt, noisy
 
 
 # ---------------- learned Richardson-Lucy warm start ----------------

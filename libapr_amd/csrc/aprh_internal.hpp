@@ -13,11 +13,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <map>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/aprh.h"
@@ -97,6 +99,18 @@ private:
     void* p_ = nullptr;
     size_t bytes_ = 0;
 };
+
+/* host loop over [0, n) split into contiguous ranges, one per hardware thread; f(begin, end) */
+template <typename F>
+void host_parallel_for(int64_t n, F f) {
+    const int64_t nt = std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (int64_t t = 0; t < nt; ++t) {
+        const int64_t b = n * t / nt, e = n * (t + 1) / nt;
+        th.emplace_back([=] { f(b, e); });
+    }
+    for (auto& t : th) t.join();
+}
 
 /* timed stages of get_apr / sample_image (APRConverter's timers, split per kernel group) */
 enum Stage {

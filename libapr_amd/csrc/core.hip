@@ -456,25 +456,30 @@ int aprh_synth_blobs_host(uint16_t* out, int64_t y, int64_t x, int64_t z, uint64
     if (!out || y <= 0 || x <= 0 || z <= 0) throw Error(APRH_ERR_INVALID_ARGUMENT, "bad synth arguments");
     const size_t n = (size_t)(y * x * z);
     std::vector<uint32_t> acc(n, 0);
-    for (int64_t i = 0; i < n_blobs; ++i) {
-        synth::Blob b = synth::blob(seed, (uint64_t)i, y, x, z, smin, smax, amin, amax);
-        int64_t y0, y1, x0, x1, z0, z1;
-        synth::box(b.cy, b.sigma, y, y0, y1);
-        synth::box(b.cx, b.sigma, x, x0, x1);
-        synth::box(b.cz, b.sigma, z, z0, z1);
-        for (int64_t zz = z0; zz <= z1; ++zz)
-            for (int64_t xx = x0; xx <= x1; ++xx)
-                for (int64_t yy = y0; yy <= y1; ++yy) acc[(size_t)(zz * x * y + xx * y + yy)] += synth::contribution(b, yy, xx, zz);
-    }
-    for (int64_t i = 0; i < (int64_t)n; ++i) {
-        float lam = background + (float)acc[i] * (1.0f / 64.0f);
-        if (noise) {
-            out[i] = synth::noisy(seed, (uint64_t)i, lam);
-        } else {
-            float k = lam + 0.5f;
-            out[i] = (uint16_t)(uint32_t)(k > 65535.0f ? 65535.0f : k);
+    /* threads own z-slabs; the integer accumulation makes the bytes independent of the split */
+    host_parallel_for(z, [&](int64_t zb, int64_t ze) {
+        for (int64_t i = 0; i < n_blobs; ++i) {
+            synth::Blob b = synth::blob(seed, (uint64_t)i, y, x, z, smin, smax, amin, amax);
+            int64_t y0, y1, x0, x1, z0, z1;
+            synth::box(b.cy, b.sigma, y, y0, y1);
+            synth::box(b.cx, b.sigma, x, x0, x1);
+            synth::box(b.cz, b.sigma, z, z0, z1);
+            z0 = std::max(z0, zb);
+            z1 = std::min(z1, ze - 1);
+            for (int64_t zz = z0; zz <= z1; ++zz)
+                for (int64_t xx = x0; xx <= x1; ++xx)
+                    for (int64_t yy = y0; yy <= y1; ++yy) acc[(size_t)(zz * x * y + xx * y + yy)] += synth::contribution(b, yy, xx, zz);
         }
-    }
+        for (size_t i = (size_t)(zb * x * y); i < (size_t)(ze * x * y); ++i) {
+            float lam = background + (float)acc[i] * (1.0f / 64.0f);
+            if (noise) {
+                out[i] = synth::noisy(seed, (uint64_t)i, lam);
+            } else {
+                float k = lam + 0.5f;
+                out[i] = (uint16_t)(uint32_t)(k > 65535.0f ? 65535.0f : k);
+            }
+        }
+    });
     APRH_CATCH
 }
 

@@ -72,7 +72,85 @@ __global__ void k_noise_u16(const float* __restrict__ lam, uint16_t* out, size_t
     }
 }
 
+/* normalised 1-D Gaussian factors of the PSF, (y, x, z) one after the other */
+std::vector<float> psf_factors(const float* psf_sigma3, int taps) {
+    std::vector<float> w(3 * (size_t)taps);
+    for (int d = 0; d < 3; ++d) {
+        double s = 0;
+        for (int k = 0; k < taps; ++k) {
+            const double t = k - taps / 2;
+            const double v = psf_sigma3[d] > 0 ? std::exp(-t * t / (2.0 * psf_sigma3[d] * psf_sigma3[d])) : (t == 0 ? 1.0 : 0.0);
+            w[(size_t)d * taps + k] = (float)v;
+            s += v;
+        }
+        for (int k = 0; k < taps; ++k) w[(size_t)d * taps + k] = (float)(w[(size_t)d * taps + k] / s);
+    }
+    return w;
+}
+
+int64_t reflect_host(int64_t q, int64_t n) {
+    if (q < 0) q = -q;
+    if (q >= n) q = 2 * (n - 1) - q;
+    return q < 0 ? 0 : (q >= n ? n - 1 : q);
+}
+
 }  // namespace
+
+/* the same workload on the host, operation for operation (blob sums in fixed point, each
+   blur tap a float multiply then add in tap order, per-voxel noise), so the bytes equal the
+   device's; threads own z-slabs */
+void synth_deconv_host(float* gt, uint16_t* noisy, int64_t y, int64_t x, int64_t z, uint64_t seed, int64_t n_blobs, float smin,
+                       float smax, float amin, float amax, float background, const float* psf_sigma3, int taps) {
+    const size_t n = (size_t)(y * x * z);
+    const size_t plane = (size_t)(y * x);
+    {
+        std::vector<uint32_t> acc(n, 0);
+        host_parallel_for(z, [&](int64_t zb, int64_t ze) {
+            for (int64_t i = 0; i < n_blobs; ++i) {
+                synth::Blob b = synth::blob(seed, (uint64_t)i, y, x, z, smin, smax, amin, amax);
+                int64_t y0, y1, x0, x1, z0, z1;
+                synth::box(b.cy, b.sigma, y, y0, y1);
+                synth::box(b.cx, b.sigma, x, x0, x1);
+                synth::box(b.cz, b.sigma, z, z0, z1);
+                z0 = std::max(z0, zb);
+                z1 = std::min(z1, ze - 1);
+                for (int64_t zz = z0; zz <= z1; ++zz)
+                    for (int64_t xx = x0; xx <= x1; ++xx)
+                        for (int64_t yy = y0; yy <= y1; ++yy) acc[(size_t)(zz * x * y + xx * y + yy)] += synth::contribution(b, yy, xx, zz);
+            }
+            for (size_t i = (size_t)zb * plane; i < (size_t)ze * plane; ++i) gt[i] = background + (float)acc[i] * (1.0f / 64.0f);
+        });
+    }
+    const std::vector<float> w = psf_factors(psf_sigma3, taps);
+    const int h = taps / 2;
+    std::vector<float> a(n), b(n);
+    auto pass = [&](const float* in, float* out, int axis) {
+        const int64_t stride = axis == 0 ? 1 : (axis == 1 ? y : y * x);
+        const int64_t len = axis == 0 ? y : (axis == 1 ? x : z);
+        const float* wk = w.data() + (size_t)axis * taps;
+        host_parallel_for(z, [&](int64_t zb, int64_t ze) {
+            for (int64_t zz = zb; zz < ze; ++zz)
+                for (int64_t xx = 0; xx < x; ++xx)
+                    for (int64_t yy = 0; yy < y; ++yy) {
+                        const int64_t i = (zz * x + xx) * y + yy;
+                        const int64_t c = axis == 0 ? yy : (axis == 1 ? xx : zz);
+                        const float* base = in + i - c * stride;
+                        float s = 0.0f;
+                        for (int k = 0; k < taps; ++k) s += wk[k] * base[reflect_host(c + k - h, len) * stride];
+                        out[i] = s;
+                    }
+        });
+    };
+    pass(gt, a.data(), 0);
+    pass(a.data(), b.data(), 1);
+    pass(b.data(), a.data(), 2);
+    host_parallel_for(z, [&](int64_t zb, int64_t ze) {
+        for (size_t i = (size_t)zb * plane; i < (size_t)ze * plane; ++i) {
+            const float l = a[i] < 0.0f ? 0.0f : a[i];
+            noisy[i] = synth::noisy(seed, (uint64_t)i, l);
+        }
+    });
+}
 
 void synth_deconv_device(aprh_context* ctx, float* gt, uint16_t* noisy, int64_t y, int64_t x, int64_t z, uint64_t seed, int64_t n_blobs,
                          float smin, float smax, float amin, float amax, float background, const float* psf_sigma3, int taps) {
@@ -89,18 +167,7 @@ void synth_deconv_device(aprh_context* ctx, float* gt, uint16_t* noisy, int64_t 
         hipLaunchKernelGGL(k_acc_to_f32, dim3(4096), dim3(256), 0, st, acc.as<uint32_t>(), gt, n, background);
         HIP_LAUNCH_CHECK();
     }
-    /* normalised 1-D Gaussian factors of the PSF */
-    std::vector<float> w(3 * (size_t)taps);
-    for (int d = 0; d < 3; ++d) {
-        double s = 0;
-        for (int k = 0; k < taps; ++k) {
-            const double t = k - taps / 2;
-            const double v = psf_sigma3[d] > 0 ? std::exp(-t * t / (2.0 * psf_sigma3[d] * psf_sigma3[d])) : (t == 0 ? 1.0 : 0.0);
-            w[(size_t)d * taps + k] = (float)v;
-            s += v;
-        }
-        for (int k = 0; k < taps; ++k) w[(size_t)d * taps + k] = (float)(w[(size_t)d * taps + k] / s);
-    }
+    const std::vector<float> w = psf_factors(psf_sigma3, taps);
     Buf dw(ctx->pool, w.size() * 4), a(ctx->pool, n * 4), b(ctx->pool, n * 4);
     HIP_CHECK(hipMemcpyAsync(dw.ptr(), w.data(), w.size() * 4, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_blur_axis, dim3(8192), dim3(256), 0, st, (const float*)gt, a.as<float>(), y, x, z, 0, dw.as<float>(), taps);
@@ -128,5 +195,16 @@ extern "C" int aprh_synth_deconv_device(aprh_context* ctx, float* gt, uint16_t* 
     if (!aprh::is_device_ptr(gt) || !aprh::is_device_ptr(noisy)) throw aprh::Error(APRH_ERR_INVALID_ARGUMENT, "outputs must be device memory");
     aprh::synth_deconv_device(ctx, gt, noisy, y, x, z, seed, n_blobs, sigma_min, sigma_max, amp_min, amp_max, background, psf_sigma3,
                               psf_taps);
+    APRH_CATCH
+}
+
+extern "C" int aprh_synth_deconv_host(float* gt, uint16_t* noisy, int64_t y, int64_t x, int64_t z, uint64_t seed, int64_t n_blobs,
+                                      float sigma_min, float sigma_max, float amp_min, float amp_max, float background,
+                                      const float* psf_sigma3, int32_t psf_taps) {
+    APRH_TRY
+    if (!gt || !noisy || !psf_sigma3 || y <= 0 || x <= 0 || z <= 0 || psf_taps < 1 || psf_taps % 2 == 0 || psf_taps > 63)
+        throw aprh::Error(APRH_ERR_INVALID_ARGUMENT, "bad synth arguments");
+    aprh::synth_deconv_host(gt, noisy, y, x, z, seed, n_blobs, sigma_min, sigma_max, amp_min, amp_max, background, psf_sigma3,
+                            psf_taps);
     APRH_CATCH
 }

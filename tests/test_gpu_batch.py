@@ -64,6 +64,29 @@ def test_batch_matches_single_and_oracle(oracle, ctx):
     batch.close()
 
 
+def test_batch_mixes_long_and_short_rows(oracle, ctx):
+    # a volume with rows of 1024+ cells at the top level (64-lane row fill) merged into one plan
+    # with small volumes (16-lane row fill)
+    dims = [(2048, 8, 4), (64, 48, 32), (1030, 10, 6), (40, 72, 24)]
+    batch = A.APRBatch(dims, par=_apar(), ctx=ctx)
+    imgs = []
+    for i, (y, x, z) in enumerate(dims):
+        imgs.append(A.synth_blobs((z, x, y), 70 + i, 6))
+        batch.set_input(i, imgs[-1])
+    batch.run()
+    for i, img in enumerate(imgs):
+        got = batch.get(i)
+        ref = oracle.get_apr(img, _par(oracle), "oracle")
+        print(f"volume {i} dims {dims[i]}: P={got.total} oracle={ref.total}", flush=True)
+        assert got.total == ref.total
+        lxz, xze, yv = got.structure()
+        assert np.array_equal(lxz, ref.level_xz_vec)
+        assert np.array_equal(xze, ref.xz_end_vec)
+        assert np.array_equal(yv, ref.y_vec)
+        got.close()
+    batch.close()
+
+
 def test_batch_rejects_small_volumes(ctx):
     with pytest.raises(A.AprhError) as e:
         A.APRBatch([(64, 64, 64), (4, 4, 4)], ctx=ctx)

@@ -14,6 +14,10 @@ CASES = [
     ((61, 63, 65), 30, 1),
     ((3, 37, 200), 10, 2),
     ((128, 256, 256), 500, 0),   # the 256x256x128 conversion workload
+    # long y rows: the top particle-cell level has rows of 512+ cells, which take the 64-lane
+    # row fill and the multi-chunk row count used at 2048x2048x512
+    ((6, 10, 1030), 4, 6),
+    ((4, 8, 2048), 4, 7),
 ]
 
 
