@@ -223,6 +223,13 @@ def conv_algorithmic_bytes(P, T, R, s_in):
     return P * (s_in + 6) + 8 * T + 16 * R
 
 
+def hbm_roofline(ab, ms):
+    """roofline entry of a call moving ab algorithmic bytes in ms"""
+    gbps = ab / (ms / 1000.0) / 1e9
+    return {"bound": "hbm", "achieved": round(gbps, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbps / HBM_PEAK_GBS, 4), "algorithmic_bytes": int(ab)}
+
+
 def conv_timing(A, apr, stencil, d_in, d_out, dtype, s_in, reps):
     apr.tree_info()   # build the tree once (cached per APR, like APR::initialize_tree_dense)
     A.APRFilter.convolve_pencil(apr, stencil, d_in, d_out, dtype=dtype)
@@ -234,7 +241,8 @@ def conv_timing(A, apr, stencil, d_in, d_out, dtype, s_in, reps):
     R = int(apr.xz_end_size) + int(apr.tree_info().xz_end_size)
     ab = conv_algorithmic_bytes(P, T, R, s_in)
     return {"ms": round(ms, 3), "particles": P, "tree_particles": T, "algorithmic_bytes": ab,
-            "algorithmic_GBps": round(ab / (ms / 1000.0) / 1e9, 1), "stencil": list(stencil.shape[::-1])}
+            "algorithmic_GBps": round(ab / (ms / 1000.0) / 1e9, 1), "stencil": list(stencil.shape[::-1]),
+            "roofline": hbm_roofline(ab, ms)}
 
 
 def run_conv_small(args):
@@ -384,13 +392,19 @@ def run_rl(args):
     psf = W.psf_stencil()
     u = A.DeviceArray(w.P * 4, ctx)
     w.apr.tree_info()
-    # per-iteration time with every tap in the reference's order
-    ctx.set_conv_mode(A.CONV_EXACT)
-    w.rl(u, iters=1, tol=0.0)
-    t = time.perf_counter()
-    w.rl(u, iters=2, tol=0.0)
-    exact_ms = (time.perf_counter() - t) * 500.0
-    log(f"[bench] RL iteration, reference tap order: {exact_ms:.1f} ms")
+    # per-iteration time with every tap in the reference's order, and separable
+    per_it = {}
+    for name, mode in (("exact", A.CONV_EXACT), ("separable", A.CONV_SEPARABLE)):
+        ctx.set_conv_mode(mode)
+        w.rl(u, iters=1, tol=0.0)
+        t = time.perf_counter()
+        w.rl(u, iters=4, tol=0.0)
+        per_it[name] = (time.perf_counter() - t) * 250.0
+    exact_ms = per_it["exact"]
+    log(f"[bench] RL iteration: reference tap order {per_it['exact']:.2f} ms, separable {per_it['separable']:.2f} ms")
+    Tn = w.apr.total_number_tree_particles()
+    R = int(w.apr.xz_end_size) + int(w.apr.tree_info().xz_end_size)
+    rl_ab = 2 * conv_algorithmic_bytes(w.P, Tn, R, 4) + w.P * (2 + 12)
     pf = None
     if os.path.exists(W.DEFAULT_WEIGHTS):
         net = A.WarmStartCNN.load(W.DEFAULT_WEIGHTS, ctx)
@@ -419,6 +433,10 @@ def run_rl(args):
            "final_rel_update": ev["plain_final_rel_update"], "data_residual": round(resid, 5),
            "rel_l2_error_vs_gt": ev["plain_rel_l2_error_vs_gt"], "conv_mode": "separable",
            "exact_order_ms_per_iteration": round(exact_ms, 2),
+           "separable_ms_per_iteration": round(per_it["separable"], 3),
+           "iteration_algorithmic_bytes": int(rl_ab),
+           "roofline_separable_iteration": hbm_roofline(rl_ab, per_it["separable"]),
+           "roofline_exact_iteration": hbm_roofline(rl_ab, per_it["exact"]),
            "warm_start": {"weights": trained, "iterations": ev["warm_iterations"], "time_to_tol_s": ev["warm_time_s"],
                           "network_s": ev["warm_network_time_s"], "final_rel_update": ev["warm_final_rel_update"],
                           "rel_l2_error_vs_gt": ev["warm_rel_l2_error_vs_gt"],

@@ -89,6 +89,15 @@ int aprh_get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y_num, in
    the intensities equal those of aprh_get_apr_u16 + aprh_sample_image_u16. */
 int aprh_get_apr_sample_u16(aprh_context* ctx, const uint16_t* image, int64_t y_num, int64_t x_num, int64_t z_num,
                             const aprh_params* par, aprh_apr** out, uint16_t* parts, uint64_t parts_capacity, int32_t* sampled);
+/* APRConverterBatch<uint16_t>::get_apr (src/algorithm/APRConverterBatch.hpp:44-175): the image in
+   z blocks of z_block_size planes (the last takes the remainder) with up to ghost_z ghost planes
+   on each side; each block runs the pixel pipeline on its own (get_apr_method_patch :186-276,
+   its own B-spline offset, the whole image's l_max) and seeds its non-ghost planes into one
+   particle-cell tree (fill_patch, PullingScheme.hpp:175-244); one pulling scheme and linear
+   structure follow.  ghost_z >= z_num gives get_apr's result.  Rows a block would place past a
+   level's z extent are skipped (the reference writes them beyond the level's rows). */
+int aprh_get_apr_by_block_u16(aprh_context* ctx, const uint16_t* image, int64_t y_num, int64_t x_num, int64_t z_num,
+                              const aprh_params* par, int64_t z_block_size, int64_t ghost_z, aprh_apr** out);
 /* APRConverter<float>::get_apr (APRConverter.hpp:534-600 with ImageType = float): no B-spline
    offset (:218-220), float smoothing and gradient; same structure semantics */
 int aprh_get_apr_f32(aprh_context* ctx, const float* image, int64_t y_num, int64_t x_num, int64_t z_num, const aprh_params* par,
@@ -144,6 +153,12 @@ int aprh_batch_stage_info(const aprh_batch* batch, int32_t i, aprh_batch_stage* 
 
 /* ParticleData<uint16_t>::sample_image (ParticleData.hpp:122-127, :221-261) */
 int aprh_sample_image_u16(aprh_context* ctx, const aprh_apr* apr, const uint16_t* image, uint16_t* parts);
+/* ParticleData<uint16_t>::sample_parts_from_img_blocked (ParticleData.hpp:140-141, 267-362): per z
+   block (as aprh_get_apr_by_block_u16) the mean pyramid of the block with its ghost planes,
+   sampled at the particles of the block's non-ghost planes of every level; particles no block
+   reaches are 0 */
+int aprh_sample_image_blocked_u16(aprh_context* ctx, const aprh_apr* apr, const uint16_t* image, int64_t z_block_size,
+                                  int64_t ghost_z, uint16_t* parts);
 /* ParticleData<float>::sample_image of a float image (same mean pyramid, float means) */
 int aprh_sample_image_f32(aprh_context* ctx, const aprh_apr* apr, const float* image, float* parts);
 

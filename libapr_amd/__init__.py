@@ -116,6 +116,8 @@ def lib():
     L.aprh_apr_particle_cell_tree.argtypes = [vp, C.c_int, vp, C.POINTER(C.c_int64)]
     L.aprh_apr_destroy.argtypes = [vp]
     L.aprh_sample_image_u16.argtypes = [vp, vp, vp, vp]
+    L.aprh_get_apr_by_block_u16.argtypes = [vp, vp, i64, i64, i64, C.POINTER(AprhParams), i64, i64, C.POINTER(vp)]
+    L.aprh_sample_image_blocked_u16.argtypes = [vp, vp, vp, i64, i64, vp]
     L.aprh_bspline_filt_rec_u16.argtypes = [vp, vp, i64, i64, i64, C.c_int, C.c_float, C.c_float]
     L.aprh_bspline_filt_rec_f32.argtypes = [vp, vp, i64, i64, i64, C.c_int, C.c_float, C.c_float]
     L.aprh_inv_bspline_f32.argtypes = [vp, vp, i64, i64, i64, C.c_int]
@@ -514,6 +516,31 @@ class APRBatch:
             pass
 
 
+class APRConverterBatch:
+    """APRConverterBatch<uint16_t> (APRConverterBatch.hpp:32-175): get_apr over z blocks of
+    z_block_size planes with ghost_z ghost planes on each side (the reference reads the blocks from
+    a TIFF file; here the image is a numpy [z, x, y] uint16 array or a device address with dims)."""
+
+    def __init__(self, ctx=None):
+        self.par = APRParameters()
+        self.ctx = ctx or default_context()
+        self.z_block_size = 128
+        self.ghost_z = 32
+
+    def get_apr(self, image, dims=None):
+        if dims is None:
+            z, x, y = image.shape
+            if image.dtype != np.uint16:
+                raise TypeError("APRConverterBatch<uint16_t> needs a uint16 image")
+        else:
+            y, x, z = dims
+        h = vp()
+        p = self.par.c()
+        _check(lib().aprh_get_apr_by_block_u16(self.ctx.h, _ptr(image), y, x, z, C.byref(p), int(self.z_block_size),
+                                               int(self.ghost_z), C.byref(h)))
+        return APR(self.ctx, h)
+
+
 class ParticleData:
     """ParticleData<T>::sample_image (ParticleData.hpp:122-127) for uint16 and float images."""
 
@@ -528,6 +555,15 @@ class ParticleData:
             out = np.zeros(max(apr.total, 1), np.uint16 if code == APRH_DTYPE_U16 else np.float32)
         f = lib().aprh_sample_image_u16 if code == APRH_DTYPE_U16 else lib().aprh_sample_image_f32
         _check(f(apr.ctx.h, apr.h, _ptr(image), _ptr(out)))
+        self.data = out[:apr.total] if isinstance(out, np.ndarray) else out
+        return self.data
+
+    def sample_parts_from_img_blocked(self, apr, image, z_block_size=256, ghost_z=32, out=None):
+        """ParticleData::sample_parts_from_img_blocked (ParticleData.hpp:140-141, 267-362) of a uint16
+        image (numpy [z, x, y] or a device address)"""
+        if out is None:
+            out = np.zeros(max(apr.total, 1), np.uint16)
+        _check(lib().aprh_sample_image_blocked_u16(apr.ctx.h, apr.h, _ptr(image), int(z_block_size), int(ghost_z), _ptr(out)))
         self.data = out[:apr.total] if isinstance(out, np.ndarray) else out
         return self.data
 
@@ -909,6 +945,140 @@ def synth_deconv_host(shape_zxy, seed, n_blobs, sigma=(1.0, 4.0), amp=(200.0, 20
     _check(lib().aprh_synth_deconv_host(_ptr(gt), _ptr(noisy), y, x, z, seed, n_blobs, sigma[0], sigma[1], amp[0], amp[1],
                                         background, _ptr(ps), psf_taps))
     return gt, noisy
+
+
+# ---------------- APR files (libaprh_io.so) ----------------
+
+IO_LIB_PATH = os.path.join(_HERE, "_lib", "libaprh_io.so")
+_io = None
+
+
+def io_lib():
+    """The loaded libaprh_io.so (raises if it has not been built)."""
+    global _io
+    if _io is not None:
+        return _io
+    if not os.path.exists(IO_LIB_PATH):
+        raise RuntimeError(f"libaprh_io.so not built ({IO_LIB_PATH}); run `python -m libapr_amd.build` "
+                           "(needs the system HDF5 and c-blosc)")
+    lib()
+    L = C.CDLL(IO_LIB_PATH)
+    L.aprh_io_last_error.restype = C.c_char_p
+    L.aprh_file_open.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(vp)]
+    L.aprh_file_close.argtypes = [vp]
+    L.aprh_file_write_apr.argtypes = [vp, vp, vp, C.c_uint64, C.c_char_p, C.c_int, C.POINTER(AprhParams)]
+    L.aprh_file_write_structure.argtypes = [vp, C.c_uint64, C.c_char_p, vp, vp, C.c_uint64, vp, C.c_uint64,
+                                            C.POINTER(AprhParams)]
+    L.aprh_file_read_apr.argtypes = [vp, vp, C.c_uint64, C.c_char_p, C.POINTER(vp)]
+    L.aprh_file_read_info.argtypes = [vp, C.c_uint64, C.c_char_p, vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.aprh_file_read_structure.argtypes = [vp, C.c_uint64, C.c_char_p, vp, vp]
+    L.aprh_file_write_particles.argtypes = [vp, vp, C.c_char_p, vp, C.c_int, C.c_uint64, C.c_int, C.c_uint64, C.c_char_p]
+    L.aprh_file_read_particles.argtypes = [vp, vp, C.c_char_p, C.c_int, C.c_uint64, C.c_char_p, vp, C.c_uint64,
+                                           C.POINTER(C.c_int), C.POINTER(C.c_uint64)]
+    _io = L
+    return L
+
+
+def _io_check(code):
+    if code != APRH_OK:
+        msg = io_lib().aprh_io_last_error().decode(errors="replace")
+        raise AprhOverflowError(code, msg) if code == APRH_ERR_OVERFLOW else AprhError(code, msg)
+
+
+class APRFile:
+    """APRFile (APRFile.hpp:19-141): HDF5 .apr files, interchangeable with the reference's.
+
+        f = APRFile(); f.open("x.apr", "WRITE"); f.write_apr(apr); f.write_particles("particles", parts); f.close()
+        f.open("x.apr", "READ"); apr = f.read_apr(); parts = f.read_particles(apr, "particles")
+    """
+
+    def __init__(self):
+        self.h = None
+
+    def open(self, file_name, read_write="WRITE"):
+        h = vp()
+        _io_check(io_lib().aprh_file_open(os.fsencode(file_name), read_write.encode(), C.byref(h)))
+        self.h = h
+        return True
+
+    def close(self):
+        if self.h:
+            io_lib().aprh_file_close(self.h)
+            self.h = None
+
+    def write_apr(self, apr, t=0, channel_name="t", write_tree=True, parameters=None):
+        p = C.byref(parameters.c()) if parameters is not None else None
+        _io_check(io_lib().aprh_file_write_apr(apr.ctx.h, self.h, apr.h, t, channel_name.encode(), int(write_tree), p))
+        return True
+
+    def write_structure(self, dims, xz_end_vec, y_vec, t=0, channel_name="t", parameters=None):
+        """the linear access of an image of dims (y, x, z) from host arrays (no device)"""
+        d = np.ascontiguousarray(dims, np.int64)
+        xze = np.ascontiguousarray(xz_end_vec, np.uint64)
+        yv = np.ascontiguousarray(y_vec, np.uint16)
+        p = C.byref(parameters.c()) if parameters is not None else None
+        _io_check(io_lib().aprh_file_write_structure(self.h, t, channel_name.encode(), _ptr(d), _ptr(xze), xze.size,
+                                                      _ptr(yv) if yv.size else None, yv.size, p))
+
+    def read_apr(self, t=0, channel_name="t", ctx=None):
+        ctx = ctx or default_context()
+        h = vp()
+        _io_check(io_lib().aprh_file_read_apr(ctx.h, self.h, t, channel_name.encode(), C.byref(h)))
+        return APR(ctx, h)
+
+    def read_info(self, t=0, channel_name="t"):
+        """(dims (y, x, z), total particles, xz_end_vec size)"""
+        d = np.zeros(3, np.int64)
+        tot, nx = C.c_uint64(0), C.c_uint64(0)
+        _io_check(io_lib().aprh_file_read_info(self.h, t, channel_name.encode(), _ptr(d), C.byref(tot), C.byref(nx)))
+        return tuple(int(v) for v in d), tot.value, nx.value
+
+    def read_structure(self, t=0, channel_name="t"):
+        """(dims, xz_end_vec, y_vec) as host arrays (no device)"""
+        dims, tot, nx = self.read_info(t, channel_name)
+        xze = np.zeros(max(nx, 1), np.uint64)
+        yv = np.zeros(max(tot, 1), np.uint16)
+        _io_check(io_lib().aprh_file_read_structure(self.h, t, channel_name.encode(), _ptr(xze), _ptr(yv)))
+        return dims, xze[:nx], yv[:tot]
+
+    def write_particles(self, particles_name, particles, apr_or_tree=True, t=0, channel_name="t", ctx=None, dtype=None):
+        """particles: numpy array (host) or a device address with dtype and ctx"""
+        if isinstance(particles, np.ndarray):
+            code = _dtype_of(particles, dtype)
+            n, cx = particles.size, None
+            particles = np.ascontiguousarray(particles)
+        else:
+            code = _dtype_of(particles, dtype)
+            n = getattr(particles, "size", None)
+            cx = (ctx or default_context()).h
+        if n is None:
+            raise ValueError("the number of particles of a device buffer: pass a numpy array or an object with .size")
+        _io_check(io_lib().aprh_file_write_particles(cx, self.h, particles_name.encode(), _ptr(particles), code, n,
+                                                      int(apr_or_tree), t, channel_name.encode()))
+        return True
+
+    def read_particles(self, particles_name, apr_or_tree=True, t=0, channel_name="t"):
+        """host numpy array of the dataset (u16 or f32)"""
+        dt, n = C.c_int(0), C.c_uint64(0)
+        L = io_lib()
+        _io_check(L.aprh_file_read_particles(None, self.h, particles_name.encode(), int(apr_or_tree), t, channel_name.encode(),
+                                             None, 0, C.byref(dt), C.byref(n)))
+        out = np.zeros(max(n.value, 1), np.uint16 if dt.value == APRH_DTYPE_U16 else np.float32)
+        _io_check(L.aprh_file_read_particles(None, self.h, particles_name.encode(), int(apr_or_tree), t, channel_name.encode(),
+                                             _ptr(out), n.value, None, None))
+        return out[:n.value]
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------- learned Richardson-Lucy warm start ----------------

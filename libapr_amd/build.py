@@ -5,6 +5,8 @@
 Every .hip under libapr_amd/csrc is compiled with hipcc for gfx950 with FP
 contraction disabled (the kernels keep the reference's FP32 operation order)
 and packed-FP32 instructions disabled, then linked into one shared library.
+libapr_amd/csrc_io/aprfile.cpp (the .apr file format, host code) is linked
+into libapr_amd/_lib/libaprh_io.so over the system HDF5 and c-blosc.
 """
 import concurrent.futures as cf
 import os
@@ -43,6 +45,31 @@ def _compile(src):
     return obj, err
 
 
+IO_SRC = os.path.join(HERE, "csrc_io", "aprfile.cpp")
+IO_LIB = os.path.join(OUT_DIR, "libaprh_io.so")
+IO_PREFIX = os.environ.get("APRH_IO_PREFIX", "/opt/conda")   # where hdf5.h / blosc.h and their libraries are
+
+
+def build_io(verbose=True):
+    """libaprh_io.so: the .apr file format over the system HDF5 and c-blosc (host code, g++);
+    skipped with a message when those libraries are absent"""
+    inc, lib = os.path.join(IO_PREFIX, "include"), os.path.join(IO_PREFIX, "lib")
+    if not (os.path.exists(os.path.join(inc, "hdf5.h")) and os.path.exists(os.path.join(inc, "blosc.h"))):
+        if verbose:
+            print(f"libaprh_io.so not built: hdf5.h / blosc.h not found under {inc}")
+        return None
+    tmp = IO_LIB + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", IO_SRC, "-I" + inc, "-L" + OUT_DIR, "-laprh", "-L" + lib,
+           "-lhdf5", "-lblosc", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + lib, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"libaprh_io build failed:\n{r.stderr}")
+    os.replace(tmp, IO_LIB)
+    if verbose:
+        print(f"built {IO_LIB}")
+    return IO_LIB
+
+
 def build(verbose=True):
     os.makedirs(OBJ_DIR, exist_ok=True)
     srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
@@ -61,6 +88,7 @@ def build(verbose=True):
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_io(verbose)
     return LIB
 
 

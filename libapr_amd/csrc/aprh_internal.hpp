@@ -133,6 +133,15 @@ enum Stage {
 namespace aprh { struct TileRecorder; }
 
 namespace aprh {
+/* one z block of a blocked conversion (aprh_get_apr_by_block_u16): the patch's pipeline stops
+   after computeLevels (with the whole image's l_max) and seeds the global particle-cell tree */
+struct BlockReq {
+    int l_max = 0;                   /* the whole image's l_max                     */
+    const aprh_apr* global = nullptr;
+    uint8_t* pct = nullptr;          /* global particle-cell tree (pct_layout)      */
+    std::vector<int64_t> off;
+    int64_t z0 = 0, gl = 0, gr = 0;  /* block start, ghost planes below and above  */
+};
 /* conversion with sampling (aprh_get_apr_sample_u16): the row fill of the structure also
    gathers the image's particle intensities when they fit into parts */
 struct SampleReq {
@@ -166,6 +175,7 @@ struct aprh_context {
     /* recording mode of a batch plan: stage launches are appended here instead of run */
     aprh::TileRecorder* recorder = nullptr;
     aprh::SampleReq* sample_req = nullptr;   /* set for one aprh_get_apr_sample_u16 call */
+    aprh::BlockReq* block_req = nullptr;     /* set while a blocked conversion runs a patch */
 };
 
 /* device-resident APR: GenInfo + LinearAccess (+ optional intermediates) */
@@ -195,6 +205,11 @@ struct aprh_apr {
     uint64_t t_n_xz = 0, t_total = 0;
     aprh::Buf t_xz_end;                            /* u64[t_n_xz]               */
     aprh::Buf t_y_vec;                             /* u16[t_total]              */
+    /* per-row index of 64-cell y blocks (APR and tree rows), built on first convolution */
+    bool yidx_built = false;
+    aprh::Buf yidx, t_yidx;                        /* u16                       */
+    std::vector<int64_t> yidx_off, t_yidx_off;     /* per level                 */
+    int top_tree_regular = -1;                     /* top tree level readable straight from the particle rows: -1 unknown */
 };
 
 namespace aprh {
@@ -271,7 +286,19 @@ void launch_grad_ds_f32(aprh_context* ctx, const float* img, float* grad, float*
 
 /* ---- pulling scheme / structure / sampling (k_structure.hip) ---- */
 void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, const aprh_params& par);
-void launch_sample_u16(aprh_context* ctx, const aprh_apr& a, const uint16_t* img, uint16_t* parts);
+/* the same from a particle-cell tree already seeded (seeded != nullptr; pct_layout), or from
+   level_top as above */
+void build_structure_seeded(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, uint8_t* seeded, const aprh_params& par);
+std::vector<int64_t> pct_layout(const aprh_apr& a);
+/* fill_patch of every level for one z block (patch of zp planes starting at z0 - gl) */
+void seed_patch(aprh_context* ctx, const aprh_apr& a, uint8_t* P, const std::vector<int64_t>& off, const uint8_t* level_top_patch,
+                int64_t zp, int64_t z0, int64_t gl, int64_t gr);
+/* ParticleData::sample_parts_from_img_blocked on a device image */
+void sample_blocked_u16(aprh_context* ctx, const aprh_apr& a, const uint16_t* img, int64_t z_block_size, int64_t ghost_z,
+                        uint16_t* parts);
+void launch_sample_u16(aprh_context* ctx, const aprh_apr& a, const uint16_t* img, uint16_t* parts, uint16_t* pyr_ext = nullptr);
+/* bytes of the mean pyramid (levels l_min .. l_max-1) sample_image builds, elements of `elem` bytes */
+size_t sample_pyramid_bytes(const aprh_apr& a, size_t elem);
 /* element offsets of the mean pyramid's levels l_min .. l_max-1 in one buffer; *total elements */
 std::vector<int64_t> pyramid_layout(const aprh_apr& a, int64_t* total);
 /* ParticleData<float>::sample_image of a float image (same pyramid, float means) */
@@ -303,6 +330,16 @@ void tree_mean(aprh_context* ctx, aprh_apr& a, const T* parts, float* tree);
 template <typename T>
 void convolve(aprh_context* ctx, aprh_apr& a, const std::vector<HostStencil>& stencils, const T* in, float* out, bool reflect,
               const float* tree_in = nullptr);
+/* 1-D factors (fy, fx, fz) of a rank-1 stencil (to 1e-6 of its largest weight), if it is one */
+bool rank1_factors(const HostStencil& s, std::vector<float>& fy, std::vector<float>& fx, std::vector<float>& fz);
+/* ---- level-tile convolution (k_conv.hip) ---- */
+void ensure_yidx(aprh_context* ctx, aprh_apr& a);
+template <typename T>
+bool tree_mean_top(aprh_context* ctx, aprh_apr& a, const T* parts, float* tree);
+template <typename T>
+void conv_levels(aprh_context* ctx, aprh_apr& a, const std::vector<HostStencil>& stencils, bool sep_mode, const T* in, const float* tree,
+                 float* out, bool reflect, int epi, const void* epi_in, bool epi_u16, Buf* part, int64_t* nparts);
+void reduce_norm_parts(aprh_context* ctx, const double* part, int64_t nparts, double* d_out);
 /* elementwise steps of APRNumerics::local_std: mode 0 p = float(u16 in), 1 p = p * p,
    2 p = sqrtf(max(p - m * m, 0)) */
 void local_std_step(aprh_context* ctx, int mode, float* p, const void* in_or_m, uint64_t n);

@@ -49,8 +49,14 @@ void solve(aprh_context* ctx, aprh_apr& a, const aprh_params& par, const float* 
         sig = a.lis.as<float>();
         gk = a.grad.as<G>();
     }
+    BlockReq* br = ctx->block_req;
     launch_levels_padded<G>(ctx, A, lp, yd, xd, zd, inten, grad, sig_raw, sig, level.as<uint8_t>(), par, d_off,
-                            level_mult(par, a.l_max), gk);
+                            level_mult(par, br ? br->l_max : a.l_max), gk);
+    if (br) {
+        /* a z block of a blocked conversion: seed the global particle-cell tree (fill_patch) */
+        if (br->l_max > 2) seed_patch(ctx, *br->global, br->pct, br->off, level.as<uint8_t>(), a.org[2], br->z0, br->gl, br->gr);
+        return;
+    }
     if (keep) {
         /* levels as float, like local_scale_temp after computeLevels */
         a.levels.reset(ctx->pool, nd * 4);
@@ -296,6 +302,79 @@ int aprh_get_apr_sample_u16(aprh_context* ctx, const uint16_t* image, int64_t y,
     }
     *sampled = req.done ? 1 : 0;
     *out = a;
+    APRH_CATCH
+}
+
+/* APRConverterBatch<uint16_t>::get_apr (APRConverterBatch.hpp:111-175) on an image in host or
+   device memory: z blocks with ghost slabs, each through the pixel pipeline on its own, seeds
+   merged into one particle-cell tree, one pulling scheme and linear structure */
+int aprh_get_apr_by_block_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x, int64_t z, const aprh_params* par,
+                              int64_t z_block_size, int64_t ghost_z, aprh_apr** out) {
+    APRH_TRY
+    check_ctx(ctx);
+    check_dims(y, x, z);
+    if (!image || !par || !out) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (z_block_size <= 0 || ghost_z < 0) throw Error(APRH_ERR_INVALID_ARGUMENT, "z_block_size must be positive and ghost_z >= 0");
+    aprh_apr* a = new aprh_apr();
+    a->ctx = ctx;
+    const int keep = ctx->keep_intermediates;
+    BlockReq req;
+    try {
+        geninfo_init(*a, y, x, z);
+        req.l_max = a->l_max;
+        req.global = a;
+        req.off = a->l_max > 2 ? pct_layout(*a) : std::vector<int64_t>();
+        Buf pct;
+        if (a->l_max > 2) {
+            pct.reset(ctx->pool, (size_t)std::max<int64_t>(req.off[a->l_max], 8));
+            launch_zero(ctx, pct.ptr(), (size_t)req.off[a->l_max]);
+            req.pct = pct.as<uint8_t>();
+        }
+        ctx->keep_intermediates = 0;
+        const int64_t plane = y * x;
+        const int64_t nb = std::max<int64_t>(z / z_block_size, 1);
+        float offset = 0;
+        for (int64_t b = 0; b < nb; ++b) {
+            req.z0 = b * z_block_size;
+            const int64_t zf = (b == nb - 1) ? z : (b + 1) * z_block_size;
+            req.gl = std::min(req.z0, ghost_z);
+            req.gr = std::min(z - zf, ghost_z);
+            const int64_t zb = req.z0 - req.gl, zp = zf + req.gr - zb;
+            aprh_apr patch;
+            patch.ctx = ctx;
+            ctx->block_req = &req;
+            get_apr_u16(ctx, image + zb * plane, y, x, zp, *par, &patch);
+            ctx->block_req = nullptr;
+            offset = patch.bspline_offset;
+        }
+        ctx->keep_intermediates = keep;
+        build_structure_seeded(ctx, *a, nullptr, req.pct, *par);
+        a->bspline_offset = offset;
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+        ctx->block_req = nullptr;
+        ctx->keep_intermediates = keep;
+        (void)hipStreamSynchronize(ctx->stream);
+        delete a;
+        throw;
+    }
+    *out = a;
+    APRH_CATCH
+}
+
+/* ParticleData<uint16_t>::sample_parts_from_img_blocked (ParticleData.hpp:140-141, 292-362) on an
+   image in host or device memory */
+int aprh_sample_image_blocked_u16(aprh_context* ctx, const aprh_apr* a, const uint16_t* image, int64_t z_block_size, int64_t ghost_z,
+                                  uint16_t* parts) {
+    APRH_TRY
+    check_ctx(ctx);
+    if (!a || !image || !parts) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (z_block_size <= 0 || ghost_z < 0) throw Error(APRH_ERR_INVALID_ARGUMENT, "z_block_size must be positive and ghost_z >= 0");
+    const size_t n = (size_t)(a->org[0] * a->org[1] * a->org[2]);
+    DevIn in(ctx, image, n * 2);
+    DevOut o(ctx, parts, std::max<uint64_t>(a->total, 1) * 2);
+    sample_blocked_u16(ctx, *a, (const uint16_t*)in.ptr, z_block_size, ghost_z, (uint16_t*)o.ptr);
+    o.finish(ctx);
     APRH_CATCH
 }
 

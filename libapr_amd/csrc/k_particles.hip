@@ -236,159 +236,6 @@ __global__ void k_upsample(const D* __restrict__ C, int64_t cyn, int64_t cxn, D*
     }
 }
 
-struct StencilDev {
-    int sy, sx, sz;
-    int64_t off;   /* float offset in the stencil buffer */
-    int sep;       /* evaluated as 1-D factors (fy, fx, fz at fy_off, fx_off, fz_off) */
-    int64_t fy_off, fx_off, fz_off;
-};
-
-__device__ __forceinline__ int64_t refl_idx(int64_t q, int64_t n) {
-    /* reflection about the edge sample (no repeat) */
-    if (q < 0) q = -q;
-    if (q >= n) q = 2 * (n - 1) - q;
-    return q < 0 ? 0 : (q >= n ? n - 1 : q);
-}
-
-/* stencil x level image at every particle of level li (one wave per particle row) */
-__global__ __launch_bounds__(256) void k_conv_level(Rows R, int li, const uint64_t* __restrict__ xz_end, const uint16_t* __restrict__ y_vec,
-                                                    const float* __restrict__ D, const float* __restrict__ W, StencilDev sd, int reflect,
-                                                    float* out) {
-    const int64_t xn = R.x_num[li], zn = R.z_num[li], yn = R.y_num[li];
-    const int64_t nrows = xn * zn;
-    const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
-    const int lane = threadIdx.x & 63;
-    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / 64;
-    const int hy = (sd.sy - 1) / 2, hx = (sd.sx - 1) / 2, hz = (sd.sz - 1) / 2;
-    const float* w = W + sd.off;
-    for (int64_t row = wid; row < nrows; row += nw) {
-        const int64_t z = row / xn, x = row % xn;
-        const uint64_t b = xz_end[R.row_begin[li] + row - 1], e = xz_end[R.row_begin[li] + row];
-        for (uint64_t i = b + lane; i < e; i += 64) {
-            const int64_t y = y_vec[i];
-            float val = 0.0f;
-            int c = 0;
-            for (int iz = 0; iz < sd.sz; ++iz) {
-                int64_t zq = z + iz - hz;
-                const bool zin = zq >= 0 && zq < zn;
-                if (reflect) zq = refl_idx(zq, zn);
-                for (int ix = 0; ix < sd.sx; ++ix) {
-                    int64_t xq = x + ix - hx;
-                    const bool xin = xq >= 0 && xq < xn;
-                    if (reflect) xq = refl_idx(xq, xn);
-                    if (reflect) {
-                        const float* drow = D + (zq * xn + xq) * yn;
-                        for (int iy = 0; iy < sd.sy; ++iy) val += drow[refl_idx(y + iy - hy, yn)] * w[c++];
-                    } else if (zin && xin) {
-                        const float* drow = D + (zq * xn + xq) * yn;
-                        for (int iy = 0; iy < sd.sy; ++iy) {
-                            const int64_t yq = y + iy - hy;
-                            const float dv = (yq >= 0 && yq < yn) ? drow[yq] : 0.0f;
-                            val += dv * w[c++];
-                        }
-                    } else {
-                        for (int iy = 0; iy < sd.sy; ++iy) val += 0.0f * w[c++];
-                    }
-                }
-            }
-            out[i] = val;
-        }
-    }
-}
-
-/* one 1-D pass of a separable stencil along y (AXIS 0) or x (AXIS 1) over a level image */
-template <int AXIS>
-__global__ __launch_bounds__(256) void k_sep_pass(const float* __restrict__ in, float* out, int64_t yn, int64_t xn, int64_t zn,
-                                                  const float* __restrict__ w, int taps, int reflect) {
-    const size_t n = (size_t)(yn * xn * zn);
-    const int h = (taps - 1) / 2;
-    const int64_t stride = AXIS == 0 ? 1 : yn;
-    const int64_t len = AXIS == 0 ? yn : xn;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const int64_t c = AXIS == 0 ? (int64_t)(i % (size_t)yn) : (int64_t)((i / (size_t)yn) % (size_t)xn);
-        const float* base = in + ((int64_t)i - c * stride);
-        float acc = 0.0f;
-        for (int k = 0; k < taps; ++k) {
-            const int64_t q = c + k - h;
-            if (reflect) acc += w[k] * base[refl_idx(q, len) * stride];
-            else if (q >= 0 && q < len) acc += w[k] * base[q * stride];
-        }
-        out[i] = acc;
-    }
-}
-
-/* last (z) pass of a separable stencil, evaluated only at the particles of level li */
-__global__ __launch_bounds__(256) void k_sep_z_gather(Rows R, int li, const uint64_t* __restrict__ xz_end,
-                                                      const uint16_t* __restrict__ y_vec, const float* __restrict__ T,
-                                                      const float* __restrict__ w, int taps, int reflect, float* out) {
-    const int64_t xn = R.x_num[li], zn = R.z_num[li], yn = R.y_num[li];
-    const int64_t nrows = xn * zn;
-    const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
-    const int lane = threadIdx.x & 63;
-    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / 64;
-    const int h = (taps - 1) / 2;
-    for (int64_t row = wid; row < nrows; row += nw) {
-        const int64_t z = row / xn, x = row % xn;
-        const uint64_t b = xz_end[R.row_begin[li] + row - 1], e = xz_end[R.row_begin[li] + row];
-        for (uint64_t i = b + lane; i < e; i += 64) {
-            const int64_t y = y_vec[i];
-            float acc = 0.0f;
-            for (int k = 0; k < taps; ++k) {
-                const int64_t q = z + k - h;
-                if (reflect) acc += w[k] * T[(refl_idx(q, zn) * xn + x) * yn + y];
-                else if (q >= 0 && q < zn) acc += w[k] * T[(q * xn + x) * yn + y];
-            }
-            out[i] = acc;
-        }
-    }
-}
-
-/* relative_blur = input / blur (APRNumerics.hpp:553, 569) */
-template <typename T>
-__global__ void k_divide(const T* __restrict__ in, float* blur, size_t n) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        blur[i] = (float)in[i] / blur[i];
-}
-
-/* u *= est, with per-block partial sums of (u_new - u)^2 and u^2 */
-__global__ __launch_bounds__(256) void k_multiply_norms(float* u, const float* __restrict__ est, size_t n, double* part) {
-    __shared__ double s1[256], s2[256];
-    double a = 0, b = 0;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const float uo = u[i];
-        const float nu = uo * est[i];
-        const double d = (double)nu - (double)uo;
-        a += d * d;
-        b += (double)uo * (double)uo;
-        u[i] = nu;
-    }
-    s1[threadIdx.x] = a;
-    s2[threadIdx.x] = b;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) {
-            s1[threadIdx.x] += s1[threadIdx.x + o];
-            s2[threadIdx.x] += s2[threadIdx.x + o];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = s1[0];
-        part[2 * blockIdx.x + 1] = s2[0];
-    }
-}
-
-__global__ void k_sum_parts(const double* __restrict__ part, int nb, double* out) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    double a = 0, b = 0;
-    for (int i = 0; i < nb; ++i) {
-        a += part[2 * i];
-        b += part[2 * i + 1];
-    }
-    out[0] = a;
-    out[1] = b;
-}
-
 __global__ void k_fill_f32(float* p, size_t n, float v) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -543,10 +390,14 @@ void tree_mean(aprh_context* ctx, aprh_apr& a, const T* parts, float* tree) {
     if (a.t_total == 0) return;
     Rows AR = make_rows(a.y_num, a.x_num, a.z_num, a.level_xz_vec, a.l_min, a.l_max);
     Rows TR = make_rows(a.t_y, a.t_x, a.t_z, a.t_level_xz, 0, a.t_l_max);
-    /* largest children level: APR level t_l_max + 1 */
-    const size_t maxn = level_elems(a, a.t_l_max + 1);
+    /* the top tree level straight from the particle rows of l_max when they have the regular
+       shape; the levels below through dense scratch of the children level */
+    const bool top = tree_mean_top<T>(ctx, a, parts, tree);
+    const int t_first = top ? a.t_l_max - 1 : a.t_l_max;
+    if (t_first < 0) return;
+    const size_t maxn = level_elems(a, t_first + 1);
     Buf V(ctx->pool, maxn * 4), Ty(ctx->pool, maxn);
-    for (int t = a.t_l_max; t >= 0; --t) {
+    for (int t = t_first; t >= 0; --t) {
         const int c = t + 1;   /* children level; APR and tree levels share the level geometry */
         const int64_t cyn = a.y_num[c], cxn = a.x_num[c], czn = a.z_num[c];
         HIP_CHECK(hipMemsetAsync(Ty.ptr(), 0, level_elems(a, c), st));
@@ -653,7 +504,6 @@ void convolve(aprh_context* ctx, aprh_apr& a, const std::vector<HostStencil>& st
     if (stencils.empty()) throw Error(APRH_ERR_INVALID_ARGUMENT, "no stencils");
     if (a.total == 0) return;
     ensure_tree(ctx, a);
-    hipStream_t st = ctx->stream;
     Buf tree_buf;
     const float* tree_vals = tree_in;
     if (!tree_vals) {
@@ -662,64 +512,9 @@ void convolve(aprh_context* ctx, aprh_apr& a, const std::vector<HostStencil>& st
         tree_mean(ctx, a, in, tree_buf.as<float>());
         tree_vals = tree_buf.as<float>();
     }
-    std::vector<float> flat;
-    std::vector<StencilDev> sd;
-    bool any_sep = false;
-    for (const auto& s : stencils) {
-        StencilDev d{s.sy, s.sx, s.sz, (int64_t)flat.size(), 0, 0, 0, 0};
-        flat.insert(flat.end(), s.w.begin(), s.w.end());
-        std::vector<float> fy, fx, fz;
-        if (ctx->conv_mode == APRH_CONV_SEPARABLE && rank1_factors(s, fy, fx, fz)) {
-            d.sep = 1;
-            d.fy_off = (int64_t)flat.size();
-            flat.insert(flat.end(), fy.begin(), fy.end());
-            d.fx_off = (int64_t)flat.size();
-            flat.insert(flat.end(), fx.begin(), fx.end());
-            d.fz_off = (int64_t)flat.size();
-            flat.insert(flat.end(), fz.begin(), fz.end());
-            any_sep = true;
-        }
-        sd.push_back(d);
-    }
-    Buf W(ctx->pool, flat.size() * 4);
-    HIP_CHECK(hipMemcpyAsync(W.ptr(), flat.data(), flat.size() * 4, hipMemcpyHostToDevice, st));
-    Rows AR = make_rows(a.y_num, a.x_num, a.z_num, a.level_xz_vec, a.l_min, a.l_max);
-    Rows TR = make_rows(a.t_y, a.t_x, a.t_z, a.t_level_xz, 0, a.t_l_max);
-    const size_t maxn = level_elems(a, a.l_max);
-    Buf D0(ctx->pool, maxn * 4), D1(ctx->pool, maxn * 4);
-    Buf T1, T2;
-    if (any_sep) {
-        T1.reset(ctx->pool, maxn * 4);
-        T2.reset(ctx->pool, maxn * 4);
-    }
-    float* prev = nullptr;
-    float* cur = D0.as<float>();
-    float* other = D1.as<float>();
-    for (int l = a.l_min; l <= a.l_max; ++l) {
-        build_level<T, float>(ctx, a, AR, TR, l, prev, cur, in, tree_vals, true);
-        const int sn = std::min((int)stencils.size() - 1, a.l_max - l);
-        const StencilDev& d = sd[(size_t)sn];
-        if (d.sep) {
-            const int64_t yn = a.y_num[l], xn = a.x_num[l], zn = a.z_num[l];
-            hipLaunchKernelGGL(k_sep_pass<0>, dim3(8192), kBlock, 0, st, (const float*)cur, T1.as<float>(), yn, xn, zn,
-                               (const float*)(W.as<float>() + d.fy_off), d.sy, reflect ? 1 : 0);
-            HIP_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_sep_pass<1>, dim3(8192), kBlock, 0, st, (const float*)T1.as<float>(), T2.as<float>(), yn, xn, zn,
-                               (const float*)(W.as<float>() + d.fx_off), d.sx, reflect ? 1 : 0);
-            HIP_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_sep_z_gather, dim3(8192), kBlock, 0, st, AR, l - a.l_min, a.xz_end.as<uint64_t>(),
-                               a.y_vec.as<uint16_t>(), (const float*)T2.as<float>(), (const float*)(W.as<float>() + d.fz_off), d.sz,
-                               reflect ? 1 : 0, out);
-            HIP_LAUNCH_CHECK();
-        } else {
-            hipLaunchKernelGGL(k_conv_level, dim3(8192), kBlock, 0, st, AR, l - a.l_min, a.xz_end.as<uint64_t>(),
-                               a.y_vec.as<uint16_t>(), (const float*)cur, W.as<float>(), d, reflect ? 1 : 0, out);
-            HIP_LAUNCH_CHECK();
-        }
-        prev = cur;
-        std::swap(cur, other);
-    }
-    HIP_CHECK(hipStreamSynchronize(st));
+    conv_levels<T>(ctx, a, stencils, ctx->conv_mode == APRH_CONV_SEPARABLE, in, tree_vals, out, reflect, 0, nullptr, false, nullptr,
+                   nullptr);
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
 }
 
 template <typename T, typename U>
@@ -936,6 +731,10 @@ template void tree_mean<float>(aprh_context*, aprh_apr&, const float*, float*);
 template void convolve<uint16_t>(aprh_context*, aprh_apr&, const std::vector<HostStencil>&, const uint16_t*, float*, bool, const float*);
 template void convolve<float>(aprh_context*, aprh_apr&, const std::vector<HostStencil>&, const float*, float*, bool, const float*);
 
+/* APRNumerics::richardson_lucy (APRNumerics.hpp:549-593): per iteration the tree means of u,
+   blur = u (*) flip(psf) with relative_blur = in / blur in the convolution's epilogue, the tree
+   means of relative_blur, and u *= relative_blur (*) psf in the second convolution's epilogue
+   with the partial sums of the relative-update norm (stopping rule, when tol > 0) */
 template <typename T>
 int richardson_lucy(aprh_context* ctx, aprh_apr& a, const T* in, float* u, const std::vector<HostStencil>& psf,
                     const std::vector<HostStencil>& psf_flipped, int max_iter, bool resume, double tol, double* last_rel) {
@@ -945,20 +744,20 @@ int richardson_lucy(aprh_context* ctx, aprh_apr& a, const T* in, float* u, const
         hipLaunchKernelGGL(k_fill_f32, kGrid, kBlock, 0, st, u, P, 1.0f);
         HIP_LAUNCH_CHECK();
     }
-    Buf blur(ctx->pool, std::max<size_t>(P, 1) * 4), est(ctx->pool, std::max<size_t>(P, 1) * 4);
-    const int nb = 1024;
-    Buf part(ctx->pool, nb * 2 * sizeof(double)), sums(ctx->pool, 2 * sizeof(double));
+    if (P > 0) ensure_tree(ctx, a);
+    const bool sep = ctx->conv_mode == APRH_CONV_SEPARABLE;
+    const bool in_u16 = std::is_same<T, uint16_t>::value;
+    Buf blur(ctx->pool, std::max<size_t>(P, 1) * 4), tree(ctx->pool, std::max<uint64_t>(a.t_total, 1) * 4);
+    Buf part, sums(ctx->pool, 2 * sizeof(double));
     int it = 0;
     double rel = 0;
     for (it = 0; it < max_iter && P > 0; ++it) {
-        convolve<float>(ctx, a, psf_flipped, u, blur.as<float>(), true);
-        hipLaunchKernelGGL(k_divide<T>, kGrid, kBlock, 0, st, in, blur.as<float>(), P);
-        HIP_LAUNCH_CHECK();
-        convolve<float>(ctx, a, psf, blur.as<float>(), est.as<float>(), true);
-        hipLaunchKernelGGL(k_multiply_norms, dim3(nb), kBlock, 0, st, u, est.as<float>(), P, part.as<double>());
-        HIP_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, part.as<double>(), nb, sums.as<double>());
-        HIP_LAUNCH_CHECK();
+        tree_mean<float>(ctx, a, u, tree.as<float>());
+        conv_levels<float>(ctx, a, psf_flipped, sep, u, tree.as<float>(), blur.as<float>(), true, 1, in, in_u16, nullptr, nullptr);
+        tree_mean<float>(ctx, a, blur.as<float>(), tree.as<float>());
+        int64_t np = 0;
+        conv_levels<float>(ctx, a, psf, sep, blur.as<float>(), tree.as<float>(), u, true, 2, nullptr, false, &part, &np);
+        reduce_norm_parts(ctx, part.as<double>(), np, sums.as<double>());
         double h[2];
         HIP_CHECK(hipMemcpyAsync(h, sums.ptr(), 16, hipMemcpyDeviceToHost, st));
         HIP_CHECK(hipStreamSynchronize(st));

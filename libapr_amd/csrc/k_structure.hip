@@ -783,7 +783,168 @@ std::vector<int64_t> build_pyramid(aprh_context* ctx, const aprh_apr& a, const T
 template <typename T>
 void pyramid_levels(aprh_context* ctx, const aprh_apr& a, const T* img, T* Pyr, const std::vector<int64_t>& off, int first);
 
+/* layout of the particle-cell tree levels lmin .. lmax-1 in one u8 buffer (8-byte aligned
+   levels, so whole-word passes apply): element offsets, off[top + 1] = total */
+std::vector<int64_t> pct_layout(const aprh_apr& a) {
+    const int top = a.l_max - 1;
+    std::vector<int64_t> off(std::max(top + 2, 1), 0);
+    for (int l = a.l_min; l <= top; ++l) off[l + 1] = off[l] + (a.y_num[l] * a.x_num[l] * a.z_num[l] + 7) / 8 * 8;
+    return off;
+}
+
+/* PullingScheme::fill of every level (LocalParticleCellSet.hpp:58-91): seeds of the top level
+   straight from the level map, then the 2x2x2 max pyramid of the map with the seeds of each
+   coarser level.  zn: z extent per level (the APR's, or a patch's) */
+void seed_levels(aprh_context* ctx, const aprh_apr& a, const uint8_t* level_top, const std::vector<int64_t>& zn, uint8_t* P,
+                 const std::vector<int64_t>& off) {
+    const int top = a.l_max - 1, lmin = a.l_min;
+    auto D = [&](int l) { return Dims{a.y_num[l], a.x_num[l], zn[l]}; };
+    auto mode_of = [&](int l) { return l == top ? 0 : (l == lmin ? 1 : 2); };
+    Buf lev(ctx->pool, (size_t)std::max<int64_t>(off[top + 1], 8));
+    uint8_t* LV = lev.as<uint8_t>();
+    {
+        const size_t n = (size_t)(a.y_num[top] * a.x_num[top] * zn[top]);
+        const int64_t G = stride_tiles((int64_t)n, 256 * 4);
+        launch_tiles<Seed>(ctx, {level_top, P + off[top], n, top, mode_of(top), G}, G);
+    }
+    const uint8_t* fine = level_top;
+    for (int l = top - 1; l >= lmin; --l) {
+        const int64_t G = stride_tiles(off[l + 1] - off[l], 256 * 4);
+        launch_tiles<LevelDown>(ctx, {fine, D(l + 1), LV + off[l], D(l), P + off[l], l, mode_of(l), G}, G);
+        fine = LV + off[l];
+    }
+}
+
+namespace {
+/* seeds of one patch's plane range into the global particle-cell tree level:
+   global(z + oz) |= SEED where local(z) == SEED, z in [z_lo, z_hi), z + oz < global z extent */
+struct SeedMerge {
+    struct Args {
+        const uint8_t* local;
+        uint8_t* global;
+        int64_t plane;        /* y * x of the level */
+        int64_t z_lo, z_hi, oz, gz;
+        int64_t G;
+    };
+    static constexpr int THREADS = 256;
+    static const char* name() { return "seed_merge"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) {
+        const int64_t zh = min(a.z_hi, a.gz - a.oz);
+        if (zh <= a.z_lo) return;
+        const int64_t n = (zh - a.z_lo) * a.plane;
+        const uint8_t* src = a.local + a.z_lo * a.plane;
+        uint8_t* dst = a.global + (a.z_lo + a.oz) * a.plane;
+        for (int64_t i = t * blockDim.x + threadIdx.x; i < n; i += a.G * blockDim.x)
+            if (src[i] == SEED) dst[i] = SEED;
+    }
+};
+
+/* blocked sampling of one level: rows (z + oz, x) for patch planes z in [z_lo, z_hi) take the
+   patch pyramid's values at (y, x, z) (ParticleData.hpp:329-362); one wave per row */
+template <typename T>
+struct GatherPatch {
+    struct Args {
+        const T* img;                 /* patch pyramid level, (z, x, y) with y fastest */
+        int64_t Y, X;
+        const uint64_t* xz_end;
+        const uint16_t* y_vec;
+        int64_t row_begin;            /* xz_end index of the level's row (0, 0) */
+        int64_t z_lo, z_hi, oz;
+        T* parts;
+        int64_t G;
+    };
+    static constexpr int THREADS = 256;
+    static const char* name() { return "gather_patch"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) {
+        const int lane = threadIdx.x & 63;
+        const int64_t nrows = (a.z_hi - a.z_lo) * a.X;
+        const int64_t wpb = blockDim.x / 64;
+        for (int64_t r = t * wpb + (threadIdx.x >> 6); r < nrows; r += a.G * wpb) {
+            const int64_t z = a.z_lo + r / a.X, x = r % a.X;
+            const int64_t rid = a.row_begin + (z + a.oz) * a.X + x;
+            const uint64_t b = a.xz_end[rid - 1], e = a.xz_end[rid];
+            const T* row = a.img + (z * a.X + x) * a.Y;
+            for (uint64_t i = b + lane; i < e; i += 64) {
+                const int64_t y = a.y_vec[i];
+                if (y < a.Y) a.parts[i] = row[y];
+            }
+        }
+    }
+};
+}  // namespace
+
+/* z extents per level of a patch of zp planes (the pyramid / particle-cell tree geometry of the
+   patch: ceil(zp / 2^(l_max - l))) */
+std::vector<int64_t> patch_z_extents(const aprh_apr& a, int64_t zp) {
+    std::vector<int64_t> zn(a.l_max + 1, 0);
+    for (int l = 0; l <= a.l_max; ++l) zn[l] = cdiv(zp, (int64_t)1 << (a.l_max - l));
+    return zn;
+}
+
+/* PullingScheme::fill_patch for every level (APRConverterBatch.hpp:287-322,
+   PullingScheme.hpp:175-244): the patch's own seeds (its level map and the max pyramid of it),
+   merged into the global tree P over the patch's non-ghost planes of each level */
+void seed_patch(aprh_context* ctx, const aprh_apr& a, uint8_t* P, const std::vector<int64_t>& off, const uint8_t* level_top_patch,
+                int64_t zp, int64_t z0, int64_t gl, int64_t gr) {
+    const int top = a.l_max - 1;
+    const std::vector<int64_t> zn = patch_z_extents(a, zp);
+    std::vector<int64_t> loff(top + 2, 0);
+    for (int l = a.l_min; l <= top; ++l) loff[l + 1] = loff[l] + (a.y_num[l] * a.x_num[l] * zn[l] + 7) / 8 * 8;
+    Buf local(ctx->pool, (size_t)std::max<int64_t>(loff[top + 1], 8));
+    seed_levels(ctx, a, level_top_patch, zn, local.as<uint8_t>(), loff);
+    for (int l = a.l_min; l <= top; ++l) {
+        const int64_t f = (int64_t)1 << (a.l_max - l);
+        const int64_t zl = gl / f, zr = gr / f, oz = z0 / f - zl;
+        const int64_t plane = a.y_num[l] * a.x_num[l];
+        const int64_t G = stride_tiles(std::max<int64_t>((zn[l] - zl - zr) * plane, 1), 256 * 4);
+        launch_tiles<SeedMerge>(ctx, {local.as<uint8_t>() + loff[l], P + off[l], plane, zl, zn[l] - zr, oz, a.z_num[l], G}, G);
+    }
+}
+
+/* ParticleData::sample_parts_from_img_blocked (ParticleData.hpp:292-362) on a device image:
+   per z block the mean pyramid of the patch (ghosts included) and the rows of its non-ghost
+   planes of each level; rows past a level's z extent are skipped; particles no block reaches
+   stay 0 */
+void sample_blocked_u16(aprh_context* ctx, const aprh_apr& a, const uint16_t* img, int64_t z_block_size, int64_t ghost_z,
+                        uint16_t* parts) {
+    launch_zero(ctx, parts, std::max<uint64_t>(a.total, 1) * 2);
+    const int64_t Z = a.org[2], plane = a.org[0] * a.org[1];
+    const int64_t nb = std::max<int64_t>(Z / z_block_size, 1);
+    for (int64_t bk = 0; bk < nb; ++bk) {
+        const int64_t z0 = bk * z_block_size, zf = (bk == nb - 1) ? Z : (bk + 1) * z_block_size;
+        const int64_t gl = std::min(z0, ghost_z), gr = std::min(Z - zf, ghost_z);
+        const int64_t zp = zf + gr - (z0 - gl);
+        const uint16_t* pimg = img + (z0 - gl) * plane;
+        aprh_apr pa;
+        pa.l_min = a.l_min;
+        pa.l_max = a.l_max;
+        pa.y_num = a.y_num;
+        pa.x_num = a.x_num;
+        pa.z_num = patch_z_extents(a, zp);
+        int64_t tot = 0;
+        const std::vector<int64_t> poff = pyramid_layout(pa, &tot);
+        Buf pyr(ctx->pool, (size_t)std::max<int64_t>(tot, 1) * 2);
+        if (a.l_max > a.l_min) pyramid_levels<uint16_t>(ctx, pa, pimg, pyr.as<uint16_t>(), poff, a.l_max - 1);
+        for (int l = a.l_min; l <= a.l_max; ++l) {
+            const int64_t f = (int64_t)1 << (a.l_max - l);
+            const int64_t zl = gl / f, zr = gr / f, oz = z0 / f - zl;
+            const int64_t zh = std::min(pa.z_num[l] - zr, a.z_num[l] - oz);
+            if (zh <= zl) continue;
+            const uint16_t* lvimg = (l == a.l_max) ? pimg : pyr.as<uint16_t>() + poff[l];
+            const int64_t G = stride_tiles((zh - zl) * a.x_num[l], 4, 16384);
+            launch_tiles<GatherPatch<uint16_t>>(ctx, {lvimg, a.y_num[l], a.x_num[l], a.xz_end.as<uint64_t>(), a.y_vec.as<uint16_t>(),
+                                                      (int64_t)a.level_xz_vec[l], zl, zh, oz, parts, G},
+                                                G);
+        }
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    }
+}
+
 void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, const aprh_params& par) {
+    build_structure_seeded(ctx, a, level_top, nullptr, par);
+}
+
+void build_structure_seeded(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, uint8_t* seeded, const aprh_params& par) {
     hipStream_t st = ctx->stream;
     const int lmax = a.l_max, lmin = a.l_min;
     if (lmax <= 2) {
@@ -814,29 +975,15 @@ void build_structure(aprh_context* ctx, aprh_apr& a, const uint8_t* level_top, c
     mark(ctx, ST_PULLING);
     /* particle-cell tree levels lmin .. top (= lmax - 1), one contiguous u8 buffer */
     const int top = lmax - 1;
-    std::vector<int64_t> off(top + 2, 0);
-    /* level maps 8-byte aligned, so whole-word passes apply */
-    for (int l = lmin; l <= top; ++l) off[l + 1] = off[l] + (a.y_num[l] * a.x_num[l] * a.z_num[l] + 7) / 8 * 8;
-    const int64_t pct_bytes = off[top + 1];
-    Buf pct(ctx->pool, (size_t)pct_bytes);
-    Buf lev(ctx->pool, (size_t)pct_bytes);
-    uint8_t* P = pct.as<uint8_t>();
-    uint8_t* LV = lev.as<uint8_t>();
+    const std::vector<int64_t> off = pct_layout(a);
+    Buf pct;
+    uint8_t* P = seeded;
     auto D = [&](int l) { return Dims{a.y_num[l], a.x_num[l], a.z_num[l]}; };
-    auto mode_of = [&](int l) { return l == top ? 0 : (l == lmin ? 1 : 2); };
-    /* seeds of the top level straight from the level map, then the max pyramid */
-    {
-        const size_t n = (size_t)(a.y_num[top] * a.x_num[top] * a.z_num[top]);
-        const int64_t G = stride_tiles((int64_t)n, 256 * 4);
-        launch_tiles<Seed>(ctx, {level_top, P + off[top], n, top, mode_of(top), G}, G);
+    if (!P) {
+        pct.reset(ctx->pool, (size_t)off[top + 1]);
+        P = pct.as<uint8_t>();
+        seed_levels(ctx, a, level_top, a.z_num, P, off);
     }
-    const uint8_t* fine = level_top;
-    for (int l = top - 1; l >= lmin; --l) {
-        const int64_t G = stride_tiles(off[l + 1] - off[l], 256 * 4);
-        launch_tiles<LevelDown>(ctx, {fine, D(l + 1), LV + off[l], D(l), P + off[l], l, mode_of(l), G}, G);
-        fine = LV + off[l];
-    }
-    lev.release();
     /* pulling scheme, top to bottom */
     for (int l = top; l >= lmin; --l) {
         const bool has_parent = (l - 1) >= lmin;
@@ -983,13 +1130,23 @@ std::vector<int64_t> build_pyramid(aprh_context* ctx, const aprh_apr& a, const T
     return off;
 }
 
+/* pyr_ext: caller-provided pyramid memory of at least sample_pyramid_bytes() (nothing is
+   allocated then); nullptr: the pyramid comes from the context's pool */
 template <typename T>
-void launch_sample(aprh_context* ctx, const aprh_apr& a, const T* img, T* parts) {
+void launch_sample(aprh_context* ctx, const aprh_apr& a, const T* img, T* parts, T* pyr_ext) {
     hipStream_t st = ctx->stream;
     const int lmax = a.l_max, lmin = a.l_min;
     Buf pyr;
-    const std::vector<int64_t> off = build_pyramid<T>(ctx, a, img, pyr);
-    T* Pyr = pyr.as<T>();
+    std::vector<int64_t> off;
+    T* Pyr = pyr_ext;
+    if (pyr_ext) {
+        int64_t tot = 0;
+        off = pyramid_layout(a, &tot);
+        pyramid_levels<T>(ctx, a, img, pyr_ext, off, a.l_max - 1);
+    } else {
+        off = build_pyramid<T>(ctx, a, img, pyr);
+        Pyr = pyr.as<T>();
+    }
     LevelPtrs<T> L;
     std::memset(&L, 0, sizeof(L));
     L.n_levels = lmax - lmin + 1;
@@ -1012,7 +1169,15 @@ void launch_sample(aprh_context* ctx, const aprh_apr& a, const T* img, T* parts)
     HIP_CHECK(hipStreamSynchronize(st));
 }
 
-void launch_sample_u16(aprh_context* ctx, const aprh_apr& a, const uint16_t* img, uint16_t* parts) { launch_sample<uint16_t>(ctx, a, img, parts); }
-void launch_sample_f32(aprh_context* ctx, const aprh_apr& a, const float* img, float* parts) { launch_sample<float>(ctx, a, img, parts); }
+void launch_sample_u16(aprh_context* ctx, const aprh_apr& a, const uint16_t* img, uint16_t* parts, uint16_t* pyr_ext) {
+    launch_sample<uint16_t>(ctx, a, img, parts, pyr_ext);
+}
+void launch_sample_f32(aprh_context* ctx, const aprh_apr& a, const float* img, float* parts) { launch_sample<float>(ctx, a, img, parts, nullptr); }
+
+size_t sample_pyramid_bytes(const aprh_apr& a, size_t elem) {
+    int64_t tot = 0;
+    pyramid_layout(a, &tot);
+    return (size_t)tot * elem;
+}
 
 }  // namespace aprh

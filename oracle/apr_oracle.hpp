@@ -1114,6 +1114,138 @@ static inline void get_apr_u16(AprResult& r, const Mesh<uint16_t>& input, const 
     apr_from_grad_lis(r, grad, lis, intensity, par, keep);
 }
 
+/* ------------------------------------------------------------------------ */
+/* Blocked conversion: APRConverterBatch::get_apr_batch_method_from_file     */
+/* (APRConverterBatch.hpp:111-175) on an image held in memory.  z blocks of  */
+/* z_block_size planes (the last takes the remainder) with ghost slabs of up */
+/* to ghost_z planes; each patch runs the pixel pipeline on its own (its own */
+/* B-spline offset, get_apr_method_patch :186-276) with the global l_max in  */
+/* computeLevels, and fills the global particle-cell tree over its non-ghost */
+/* planes (fill_global_particle_cell_tree :287-322, PullingScheme::fill_patch */
+/* PullingScheme.hpp:175-244); then one pulling scheme and linear structure. */
+/* ------------------------------------------------------------------------ */
+struct ZBlock {
+    int64_t z0, zf, gl, gr;   /* block [z0, zf), ghosts below / above */
+};
+
+static inline std::vector<ZBlock> z_blocks(int64_t z_num, int64_t z_block_size, int64_t ghost_z) {
+    const int64_t nb = std::max<int64_t>(z_num / z_block_size, 1);
+    std::vector<ZBlock> out;
+    for (int64_t b = 0; b < nb; ++b) {
+        ZBlock k;
+        k.z0 = b * z_block_size;
+        k.zf = (b == nb - 1) ? z_num : (b + 1) * z_block_size;
+        k.gl = std::min(k.z0, ghost_z);
+        k.gr = std::min(z_num - k.zf, ghost_z);
+        out.push_back(k);
+    }
+    return out;
+}
+
+/* fill_patch: SEED where the patch's level map at pct level k satisfies the level's test, over
+   the patch planes [gl / f, z_num - gr / f) moved by z0 / f - gl / f (f = 2^(l_max_pct + 1 - k)) */
+static inline void fill_patch(PullingScheme& ps, int k, const Mesh<float>& in, const ZBlock& b) {
+    const int64_t f = (int64_t)1 << (ps.l_max + 1 - k);
+    const int64_t zl = b.gl / f, zr = b.gr / f;
+    const int64_t oz = (b.z0 - b.gl + b.gl) / f - zl;
+    Mesh<uint8_t>& m = ps.pct[k];
+    const float kf = (float)k;
+    for (int64_t z = zl; z < in.z_num - zr; ++z) {
+        if (z + oz >= m.z_num) continue;
+        for (int64_t x = 0; x < in.x_num; ++x)
+            for (int64_t y = 0; y < in.y_num; ++y) {
+                const float v = in.at(y, x, z);
+                const bool seed = (k == ps.l_max) ? (v >= kf) : (k == ps.l_min) ? (v <= kf) : (v == kf);
+                if (seed) m.at(y, x, z + oz) = PS_SEED;
+            }
+    }
+}
+
+static inline void get_apr_by_block_u16(AprResult& r, const Mesh<uint16_t>& input, const aprh_params& par, int64_t z_block_size,
+                                        int64_t ghost_z) {
+    r.info.init(input.y_num, input.x_num, input.z_num);
+    PullingScheme ps;
+    ps.init(r.info);
+    const int64_t plane = input.y_num * input.x_num;
+    for (const ZBlock& b : z_blocks(input.z_num, z_block_size, ghost_z)) {
+        const int64_t zb = b.z0 - b.gl, ze = b.zf + b.gr;
+        Mesh<uint16_t> patch(input.y_num, input.x_num, ze - zb);
+        std::copy(input.v.begin() + zb * plane, input.v.begin() + ze * plane, patch.v.begin());
+        const int64_t zd = (int64_t)std::ceil(1.0 * patch.z_num / 2.0), xd = (int64_t)std::ceil(1.0 * patch.x_num / 2.0),
+                      yd = (int64_t)std::ceil(1.0 * patch.y_num / 2.0);
+        /* get_apr_method_patch: the pixel pipeline of get_apr_u16 on the patch */
+        Mesh<uint16_t> grad(yd, xd, zd, 0);
+        Mesh<uint16_t> img(patch.y_num, patch.x_num, patch.z_num);
+        const float off = bspline_offset_u16(patch.v.data(), patch.size(), par.lambda);
+        for (size_t i = 0; i < patch.size(); ++i) img.v[i] = f2u16((float)patch.v[i] + off);
+        if (par.lambda > 0) smooth_bspline_3d(img, par.lambda);
+        grad_mag_ds(img, grad, par.dx, par.dy, par.dz);
+        Mesh<float> lis;
+        downsample(img, lis, [](float a, float c) { return a + c; }, [](float a) { return (float)((double)a / 8.0); });
+        if (par.lambda > 0) {
+            if (img.y_num > 2) inv_bspline_y(lis);
+            if (img.x_num > 2) inv_bspline_x(lis);
+            if (img.z_num > 2) inv_bspline_z(lis);
+        }
+        Mesh<float> intensity;
+        local_intensity_scale(lis, intensity, par);
+        /* applyParameters (APRConverterBatch.hpp:396-433) */
+        const float ipth = par.Ip_th + off;
+        for (size_t i = 0; i < intensity.size(); ++i) if (intensity.v[i] <= ipth) grad.v[i] = 0;
+        for (size_t i = 0; i < grad.size(); ++i) {
+            float s = lis.v[i];
+            if (s < par.sigma_th) {
+                s = (s < par.sigma_th_max) ? 60000.0f : par.sigma_th;
+                lis.v[i] = s;
+            }
+        }
+        for (size_t i = 0; i < grad.size(); ++i) if ((float)grad.v[i] < par.grad_th) grad.v[i] = 0;
+        compute_levels(grad, lis, r.info.l_max, par.rel_error, par.dx, par.dy, par.dz);
+        /* fill_global_particle_cell_tree */
+        fill_patch(ps, ps.l_max, lis, b);
+        Mesh<float> cur = lis, nxt;
+        for (int l = ps.l_max - 1; l >= ps.l_min; --l) {
+            downsample(cur, nxt, [](float a, float c) { return std::max(a, c); }, [](float a) { return a; });
+            fill_patch(ps, l, nxt, b);
+            std::swap(cur, nxt);
+        }
+    }
+    ps.run();
+    linear_structure(r.info, r.lin, ps.pct, par.neighborhood_optimization != 0);
+}
+
+/* ParticleData::sample_parts_from_img_blocked (ParticleData.hpp:292-362): per z block the mean
+   pyramid of the patch (ghosts included) is sampled at the rows of the block's non-ghost planes
+   of each level (f = 2^(l_max - level), planes [gl / f, z_num - gr / f) moved by z0 / f - gl / f);
+   particles of rows no block reaches stay 0 */
+template <typename ImageType, typename PartType>
+void sample_image_blocked(const GenInfo& g, const Linear& L, const Mesh<ImageType>& img, int64_t z_block_size, int64_t ghost_z,
+                          std::vector<PartType>& parts) {
+    parts.assign(L.total, 0);
+    const int64_t plane = img.y_num * img.x_num;
+    for (const ZBlock& b : z_blocks(img.z_num, z_block_size, ghost_z)) {
+        const int64_t zb = b.z0 - b.gl, ze = b.zf + b.gr;
+        std::vector<Mesh<ImageType>> pyr(g.l_max + 1);
+        pyr[g.l_max].resize(img.y_num, img.x_num, ze - zb);
+        std::copy(img.v.begin() + zb * plane, img.v.begin() + ze * plane, pyr[g.l_max].v.begin());
+        for (int level = g.l_max; level > g.l_min; --level)
+            downsample(pyr[level], pyr[level - 1], [](float a, float c) { return a + c; },
+                       [](float a) { return Store<ImageType>::cvt(a / 8.0f); });
+        for (int level = g.l_min; level <= g.l_max; ++level) {
+            const int64_t f = (int64_t)1 << (g.l_max - level);
+            const int64_t zl = b.gl / f, zr = b.gr / f;
+            const int64_t oz = (b.z0 - b.gl + b.gl) / f - zl;
+            const Mesh<ImageType>& P = pyr[level];
+            for (int64_t z = zl; z < P.z_num - zr; ++z) {
+                if (z + oz >= g.z_num[level]) continue;
+                for (int64_t x = 0; x < P.x_num; ++x)
+                    for (uint64_t i = L.row_begin(level, z + oz, x, g); i < L.row_end(level, z + oz, x, g); ++i)
+                        if ((int64_t)L.y_vec[i] < P.y_num) parts[i] = (PartType)P.at(L.y_vec[i], x, z);
+            }
+        }
+    }
+}
+
 /* get_apr for ImageType = float (APRConverter<float>): no B-spline offset   */
 /* (APRConverter.hpp:218-220), float smoothing, float gradient (grad_temp is */
 /* PixelData<float>), then the same stages.                                  */
@@ -1448,6 +1580,106 @@ void convolve_pencil(const GenInfo& g, const Linear& L, const Tree& t, const std
             }
         }
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Face-neighbour filters: APRNumerics::face_neighbour_filter                 */
+/* (APRNumerics.hpp:305-355) with the random iterator's neighbour search      */
+/* (APRIterator.hpp:283-394, RandomAccess.hpp:92-170, RandomIterator.hpp:     */
+/* 76-91).  For each face (+d, then -d) of a particle at (l, z, x, y):        */
+/*   the same-level cell, else (l > l_min) the parent-level cell containing   */
+/*   the face neighbour, else (l < l_max) the four children on the face, each */
+/*   counted when it is a particle -- provided the first of them is one;      */
+/* out = in * f[1], then += f[2] (face +d) and f[0] (face -d) times the mean  */
+/* of the neighbours, or times the particle's own value when it has none.    */
+/* Coordinates are unsigned 16-bit in the reference: -1 wraps out of range.  */
+/* ------------------------------------------------------------------------ */
+/* index of the particle at (level, z, x, y), or -1 */
+static inline int64_t find_particle(const GenInfo& g, const Linear& L, int level, int64_t z, int64_t x, int64_t y) {
+    if (level < g.l_min || level > g.l_max) return -1;
+    if (z < 0 || x < 0 || y < 0 || z >= g.z_num[level] || x >= g.x_num[level] || y >= g.y_num[level]) return -1;
+    uint64_t b = L.row_begin(level, z, x, g), e = L.row_end(level, z, x, g);
+    const uint16_t* yv = L.y_vec.data();
+    uint64_t lo = b, hi = e;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) / 2;
+        if ((int64_t)yv[mid] < y) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < e && (int64_t)yv[lo] == y) ? (int64_t)lo : -1;
+}
+
+/* the reference's uint16 coordinate arithmetic: a step below 0 wraps to 65535 */
+static inline int64_t u16c(int64_t v) { return (int64_t)(uint16_t)v; }
+
+template <typename S>
+void face_neighbour_filter(const GenInfo& g, const Linear& L, const S* in, float* out, const float* filter, int dim) {
+    static const int DY[6] = {1, -1, 0, 0, 0, 0}, DX[6] = {0, 0, 1, -1, 0, 0}, DZ[6] = {0, 0, 0, 0, 1, -1};
+    static const int CO[4][2] = {{0, 0}, {1, 0}, {0, 1}, {1, 1}};
+    const float ft[2] = {filter[2], filter[0]};
+    for (int level = g.l_min; level <= g.l_max; ++level) {
+#pragma omp parallel for schedule(dynamic)
+        for (int64_t z = 0; z < g.z_num[level]; ++z)
+            for (int64_t x = 0; x < g.x_num[level]; ++x)
+                for (uint64_t i = L.row_begin(level, z, x, g); i < L.row_end(level, z, x, g); ++i) {
+                    const int64_t y = L.y_vec[i];
+                    const float cur = (float)in[i];
+                    float o = cur * filter[1];
+                    for (int k = 0; k < 2; ++k) {
+                        const int f = 2 * dim + k;
+                        float sum = 0, cnt = 0;
+                        int64_t n = find_particle(g, L, level, u16c(z + DZ[f]), u16c(x + DX[f]), u16c(y + DY[f]));
+                        if (n >= 0) {
+                            sum += (float)in[n];
+                            cnt++;
+                        } else {
+                            n = (level > g.l_min) ? find_particle(g, L, level - 1, u16c(z + DZ[f]) / 2, u16c(x + DX[f]) / 2,
+                                                                  u16c(y + DY[f]) / 2)
+                                                  : -1;
+                            if (n >= 0) {
+                                sum += (float)in[n];
+                                cnt++;
+                            } else if (level < g.l_max) {
+                                /* children on the face: the face axis at 2 (c + d) + (d < 0), the other two
+                                   axes 2 c + offsets (CO), the first of them decides */
+                                for (int c = 0; c < 4; ++c) {
+                                    int64_t cy = u16c(u16c(y + DY[f]) * 2 + (DY[f] < 0)), cx = u16c(u16c(x + DX[f]) * 2 + (DX[f] < 0)),
+                                            cz = u16c(u16c(z + DZ[f]) * 2 + (DZ[f] < 0));
+                                    if (dim == 0) { cx += CO[c][0]; cz += CO[c][1]; }
+                                    else if (dim == 1) { cy += CO[c][0]; cz += CO[c][1]; }
+                                    else { cy += CO[c][0]; cx += CO[c][1]; }
+                                    const int64_t m = find_particle(g, L, level + 1, u16c(cz), u16c(cx), u16c(cy));
+                                    if (m < 0) {
+                                        if (c == 0) break;
+                                        continue;
+                                    }
+                                    sum += (float)in[m];
+                                    cnt++;
+                                }
+                            }
+                        }
+                        if (cnt > 0) o += ft[k] * sum / cnt;
+                        else o += ft[k] * cur;
+                    }
+                    out[i] = o;
+                }
+    }
+}
+
+/* seperable_face_neighbour_filter (APRNumerics.hpp:286-302): y, x, z per repeat */
+template <typename S>
+void seperable_face_neighbour_filter(const GenInfo& g, const Linear& L, const S* in, float* out, const float* filter, int repeats) {
+    const size_t P = L.total;
+    std::vector<float> tmp(P), o(P, 0.0f);
+    for (size_t i = 0; i < P; ++i) tmp[i] = (float)in[i];
+    for (int r = 0; r < repeats; ++r) {
+        face_neighbour_filter(g, L, tmp.data(), o.data(), filter, 0);
+        face_neighbour_filter(g, L, o.data(), tmp.data(), filter, 1);
+        face_neighbour_filter(g, L, tmp.data(), o.data(), filter, 2);
+        std::swap(o, tmp);
+    }
+    std::swap(o, tmp);
+    std::copy(o.begin(), o.end(), out);
 }
 
 /* APRNumerics::richardson_lucy (APRNumerics.hpp:549-593).  Optional stopping

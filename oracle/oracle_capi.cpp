@@ -198,6 +198,37 @@ void* orc_get_apr_f32(const float* img, int64_t y, int64_t x, int64_t z, const a
     }
 }
 
+/* APRNumerics::face_neighbour_filter / seperable_face_neighbour_filter (repeats >= 0) */
+void orc_face_neighbour_filter_f32(void* hp, const float* in, const float* filter3, int dim, int repeats, float* out) {
+    Handle* h = (Handle*)hp;
+    if (repeats < 0) face_neighbour_filter(h->r.info, h->r.lin, in, out, filter3, dim);
+    else seperable_face_neighbour_filter(h->r.info, h->r.lin, in, out, filter3, repeats);
+}
+
+/* APRConverterBatch (APRConverterBatch.hpp:111-175) on an image in memory */
+void* orc_get_apr_by_block_u16(const uint16_t* img, int64_t y, int64_t x, int64_t z, const aprh_params* par, int64_t z_block_size,
+                               int64_t ghost_z, char* errbuf, int errlen) {
+    try {
+        Handle* h = new Handle();
+        h->dims[0] = y; h->dims[1] = x; h->dims[2] = z;
+        Mesh<uint16_t> m = wrap(img, y, x, z);
+        get_apr_by_block_u16(h->r, m, *par, z_block_size, ghost_z);
+        return h;
+    } catch (std::exception& e) {
+        set_err(errbuf, errlen, e.what());
+        return nullptr;
+    }
+}
+
+/* ParticleData::sample_parts_from_img_blocked (ParticleData.hpp:292-362) */
+void orc_sample_image_blocked_u16(void* hp, const uint16_t* img, int64_t z_block_size, int64_t ghost_z, uint16_t* parts) {
+    Handle* h = (Handle*)hp;
+    Mesh<uint16_t> m = wrap(img, h->dims[0], h->dims[1], h->dims[2]);
+    std::vector<uint16_t> p;
+    sample_image_blocked(h->r.info, h->r.lin, m, z_block_size, ghost_z, p);
+    std::copy(p.begin(), p.end(), parts);
+}
+
 /* get_apr_custom_grad_scale (APRConverter.hpp:169-191): structure from given   */
 /* downsampled gradient and local intensity scale; intensity = Ip_th reference  */
 void* orc_apr_from_grad_lis(const uint16_t* grad, const float* lis, const float* intensity, int64_t y, int64_t x, int64_t z,

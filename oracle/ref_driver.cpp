@@ -531,6 +531,55 @@ int orc_richardson_lucy_f32(void* hp, const float* in, float* u, const float* ps
     return niter;
 }
 
+/* APRNumerics::gradient_cfd / gradient_sobel (APRNumerics.hpp:165-213); kind 0 cfd, 1 sobel */
+}  // extern "C"
+template <typename T>
+static void ref_gradient(void* hp, const T* in, int kind, int dim, float delta, float* out) {
+    Handle* h = (Handle*)hp;
+    ParticleData<T> pi;
+    ParticleData<float> po;
+    fill_parts(pi, in, h->apr.total_number_particles());
+    if (kind == 0) APRNumerics::gradient_cfd(h->apr, pi, po, dim, delta);
+    else APRNumerics::gradient_sobel(h->apr, pi, po, dim, delta);
+    std::copy(po.data.begin(), po.data.end(), out);
+}
+extern "C" {
+void orc_gradient_u16(void* hp, const uint16_t* in, int kind, int dim, float delta, float* out) { ref_gradient(hp, in, kind, dim, delta, out); }
+void orc_gradient_f32(void* hp, const float* in, int kind, int dim, float delta, float* out) { ref_gradient(hp, in, kind, dim, delta, out); }
+
+/* APRNumerics::gradient_magnitude_cfd / _sobel (APRNumerics.hpp:216-283) */
+}  // extern "C"
+template <typename T>
+static void ref_gradient_magnitude(void* hp, const T* in, int kind, const float* deltas, float* out) {
+    Handle* h = (Handle*)hp;
+    ParticleData<T> pi;
+    ParticleData<float> po;
+    fill_parts(pi, in, h->apr.total_number_particles());
+    const std::vector<float> d = {deltas[0], deltas[1], deltas[2]};
+    if (kind == 0) APRNumerics::gradient_magnitude_cfd(h->apr, pi, po, d);
+    else APRNumerics::gradient_magnitude_sobel(h->apr, pi, po, d);
+    std::copy(po.data.begin(), po.data.end(), out);
+}
+extern "C" {
+void orc_gradient_magnitude_u16(void* hp, const uint16_t* in, int kind, const float* deltas, float* out) {
+    ref_gradient_magnitude(hp, in, kind, deltas, out);
+}
+void orc_gradient_magnitude_f32(void* hp, const float* in, int kind, const float* deltas, float* out) {
+    ref_gradient_magnitude(hp, in, kind, deltas, out);
+}
+
+/* APRNumerics::face_neighbour_filter (APRNumerics.hpp:305-355) and seperable_face_neighbour_filter
+   (:286-302) on float particles; repeats < 0: the single-dimension filter along `dim` */
+void orc_face_neighbour_filter_f32(void* hp, const float* in, const float* filter3, int dim, int repeats, float* out) {
+    Handle* h = (Handle*)hp;
+    ParticleData<float> pi, po;
+    fill_parts(pi, in, h->apr.total_number_particles());
+    const std::vector<float> f = {filter3[0], filter3[1], filter3[2]};
+    if (repeats < 0) APRNumerics::face_neighbour_filter(h->apr, pi, po, f, dim);
+    else APRNumerics::seperable_face_neighbour_filter(h->apr, pi, po, f, repeats);
+    std::copy(po.data.begin(), po.data.end(), out);
+}
+
 void orc_gaussian_stencil(const float* sigma, const int* size, int normalize, float* out) {
     PixelData<float> m = APRStencil::create_gaussian_filter<float>({sigma[0], sigma[1], sigma[2]}, {size[0], size[1], size[2]},
                                                                    normalize != 0);

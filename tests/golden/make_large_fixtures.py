@@ -17,6 +17,7 @@ Workloads (sizes and seeds of the benchmark):
           voxels, seeds (2 << 16) + i): get_apr of each
   deconv  1024x1024x256 deconvolution volume (50k blobs sigma 1-4 amplitude 200-2000,
           9^3 Gaussian PSF sigma (1, 1, 2), shot noise, seed 3): get_apr + sample_image,
+          the tree means and the 9^3 convolution of the particles times 1.37 (float),
           then 3 Richardson-Lucy iterations from u = 1 (normalised, downsampled stencils)
 Test infrastructure only.
 """
@@ -124,6 +125,12 @@ def run_deconv(O, A):
     res.update(total=apr.total, l_min=apr.l_min, l_max=apr.l_max, level_xz_vec=[int(v) for v in apr.level_xz_vec],
                xz_end_vec=record(apr.xz_end_vec), y_vec=record(apr.y_vec), particles=record(parts))
     psf = O.gaussian_stencil(g["psf_sigma"], (g["psf_taps"],) * 3, True)
+    # float particles with fractional values: tree means and the reference-order convolution
+    # depend on the summation order there
+    pf = (parts.astype(np.float32) * np.float32(1.37)).astype(np.float32)
+    res["tree_mean_f32"] = record(apr.fill_tree_mean(pf))
+    stack = O.downsampled_stencils(psf, apr.l_max - apr.l_min, True)
+    res["conv_psf_f32"] = record(apr.convolve_pencil(stack, pf))
     t = time.time()
     u, it, rel = apr.richardson_lucy(parts, psf, g["rl_iters"], use_ds=True, normalize=True)
     log(f"deconv: {it} RL iterations {time.time() - t:.1f} s, P={apr.total}")

@@ -65,6 +65,10 @@ def _bind(lib):
     lib.orc_apr_pct.argtypes = [C.c_void_p, C.c_int, u8p]
     lib.orc_apr_pct.restype = C.c_int
     lib.orc_sample_image_u16.argtypes = [C.c_void_p, u16p, u16p]
+    if hasattr(lib, "orc_get_apr_by_block_u16"):
+        lib.orc_get_apr_by_block_u16.argtypes = [u16p, i64, i64, i64, C.POINTER(AprhParams), i64, i64, C.c_char_p, C.c_int]
+        lib.orc_get_apr_by_block_u16.restype = C.c_void_p
+        lib.orc_sample_image_blocked_u16.argtypes = [C.c_void_p, u16p, i64, i64, u16p]
     lib.orc_apr_free.argtypes = [C.c_void_p]
     lib.orc_num_threads.restype = C.c_int
     lib.orc_apr_from_structure.argtypes = [i64, i64, i64, u64p, C.c_uint64, u16p, C.c_uint64]
@@ -93,6 +97,13 @@ def _bind(lib):
         lib.orc_cnn_forward.argtypes = [f32p, f32p, C.c_int, C.c_int, C.c_int, f32p]
         lib.orc_cnn_grad.argtypes = [f32p, f32p, f32p, C.c_int, C.c_int, C.c_int, f64p]
         lib.orc_cnn_grad.restype = C.c_double
+    if hasattr(lib, "orc_gradient_u16"):
+        lib.orc_gradient_u16.argtypes = [C.c_void_p, u16p, C.c_int, C.c_int, C.c_float, f32p]
+        lib.orc_gradient_f32.argtypes = [C.c_void_p, f32p, C.c_int, C.c_int, C.c_float, f32p]
+        lib.orc_gradient_magnitude_u16.argtypes = [C.c_void_p, u16p, C.c_int, f32p, f32p]
+        lib.orc_gradient_magnitude_f32.argtypes = [C.c_void_p, f32p, C.c_int, f32p, f32p]
+    if hasattr(lib, "orc_face_neighbour_filter_f32"):
+        lib.orc_face_neighbour_filter_f32.argtypes = [C.c_void_p, f32p, f32p, C.c_int, C.c_int, f32p]
     if hasattr(lib, "orc_reconstruct"):
         lib.orc_reconstruct.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int]
     return lib
@@ -147,6 +158,12 @@ class OracleAPR:
         if self.lib.orc_apr_pct(self.h, level, out.reshape(-1)) != 0:
             return None
         return out
+
+    def sample_blocked(self, img, z_block_size, ghost_z):
+        """ParticleData::sample_parts_from_img_blocked (ParticleData.hpp:292-362)"""
+        parts = np.zeros(max(self.total, 1), np.uint16)
+        self.lib.orc_sample_image_blocked_u16(self.h, np.ascontiguousarray(img).reshape(-1), z_block_size, ghost_z, parts)
+        return parts[:self.total]
 
     def sample(self, img):
         parts = np.zeros(max(self.total, 1), np.uint16)
@@ -205,6 +222,33 @@ class OracleAPR:
                int(resume), tol, C.byref(rel))
         return u[:self.total], it, rel.value
 
+    def gradient(self, parts, sobel, dim, delta):
+        """APRNumerics::gradient_cfd / gradient_sobel (reference build only)"""
+        out = np.zeros(max(self.total, 1), np.float32)
+        if parts.dtype == np.uint16:
+            self.lib.orc_gradient_u16(self.h, np.ascontiguousarray(parts), int(sobel), dim, delta, out)
+        else:
+            self.lib.orc_gradient_f32(self.h, np.ascontiguousarray(parts, np.float32), int(sobel), dim, delta, out)
+        return out[:self.total]
+
+    def gradient_magnitude(self, parts, sobel, deltas):
+        """APRNumerics::gradient_magnitude_cfd / _sobel (reference build only)"""
+        out = np.zeros(max(self.total, 1), np.float32)
+        d = np.ascontiguousarray(deltas, np.float32)
+        if parts.dtype == np.uint16:
+            self.lib.orc_gradient_magnitude_u16(self.h, np.ascontiguousarray(parts), int(sobel), d, out)
+        else:
+            self.lib.orc_gradient_magnitude_f32(self.h, np.ascontiguousarray(parts, np.float32), int(sobel), d, out)
+        return out[:self.total]
+
+    def face_neighbour_filter(self, parts, filt, dim=0, repeats=-1):
+        """APRNumerics::face_neighbour_filter (repeats < 0, along dim) or
+        seperable_face_neighbour_filter (repeats >= 0) of float particles"""
+        out = np.zeros(max(self.total, 1), np.float32)
+        self.lib.orc_face_neighbour_filter_f32(self.h, np.ascontiguousarray(parts, np.float32),
+                                               np.ascontiguousarray(filt, np.float32), dim, repeats, out)
+        return out[:self.total]
+
     def sample_f32(self, img):
         parts = np.zeros(max(self.total, 1), np.float32)
         self.lib.orc_sample_image_f32(self.h, np.ascontiguousarray(img, np.float32).reshape(-1), parts)
@@ -236,6 +280,18 @@ def get_apr(img_zxy, par, which="oracle", keep=True):
     z, x, y = img_zxy.shape
     err = C.create_string_buffer(512)
     h = lib.orc_get_apr_u16(np.ascontiguousarray(img_zxy).reshape(-1), y, x, z, C.byref(par), int(keep), err, 512)
+    if not h:
+        raise ValueError(err.value.decode())
+    return OracleAPR(lib, h, (y, x, z))
+
+
+def get_apr_by_block(img_zxy, par, z_block_size, ghost_z, which="oracle"):
+    """APRConverterBatch<uint16_t> (APRConverterBatch.hpp:111-175) on a (z, x, y) uint16 image"""
+    lib = load(which)
+    z, x, y = img_zxy.shape
+    err = C.create_string_buffer(512)
+    h = lib.orc_get_apr_by_block_u16(np.ascontiguousarray(img_zxy).reshape(-1), y, x, z, C.byref(par), z_block_size, ghost_z,
+                                     err, 512)
     if not h:
         raise ValueError(err.value.decode())
     return OracleAPR(lib, h, (y, x, z))

@@ -177,6 +177,18 @@ def test_deconvolution_volume_rl(lctx):
     img.free()
     _check("particles", parts.to_host(np.uint16, P), f["particles"])
     psf = A.APRStencil.create_gaussian_filter(g["psf_sigma"], (g["psf_taps"],) * 3, True)
+    # float particles with fractional values: tree means and reference-order convolution
+    pf = (parts.to_host(np.uint16, P).astype(np.float32) * np.float32(1.37)).astype(np.float32)
+    d_pf = A.DeviceArray(P * 4, lctx)
+    d_pf.from_host(pf)
+    tree = A.DeviceArray(max(apr.total_number_tree_particles(), 1) * 4, lctx)
+    A.APRTreeNumerics.fill_tree_mean(apr, d_pf.ptr, tree.ptr, dtype="float32")
+    _check("tree_mean_f32", tree.to_host(np.float32, apr.total_number_tree_particles()), f["tree_mean_f32"])
+    conv_out = A.DeviceArray(P * 4, lctx)
+    A.APRFilter.convolve_pencil(apr, psf, d_pf.ptr, conv_out.ptr, normalize=True, dtype="float32")
+    _check("conv_psf_f32", conv_out.to_host(np.float32, P), f["conv_psf_f32"])
+    for arr in (d_pf, tree, conv_out):
+        arr.free()
     u = A.DeviceArray(P * 4, lctx)
     lctx.set_conv_mode(A.CONV_EXACT)
     _, st = A.APRNumerics.richardson_lucy(apr, parts.ptr, u.ptr, psf, g["rl_iters"], normalize=True, dtype="uint16")

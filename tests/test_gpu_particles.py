@@ -422,6 +422,13 @@ def test_gradients(oracle, ctx, shape, nb, seed, sobel):
     m = np.sqrt(m).astype(np.float32)
     mfn = A.APRNumerics.gradient_magnitude_sobel if sobel else A.APRNumerics.gradient_magnitude_cfd
     assert np.array_equal(_bits(mfn(g, parts, deltas=(1.0, 0.7, 2.5))), _bits(m))
+    # and against the reference's own APRNumerics, where its build is present
+    if oracle.load("ref") is not None and hasattr(oracle.load("ref"), "orc_gradient_u16"):
+        r = oracle.apr_from_structure(o.dims, o.xz_end_vec, o.y_vec, "ref")
+        for dim, delta in ((0, 1.0), (1, 0.7), (2, 2.5)):
+            for p in (parts, pf):
+                assert np.array_equal(_bits(fn(g, p, dimension=dim, delta=delta)), _bits(r.gradient(p, sobel, dim, delta)))
+        assert np.array_equal(_bits(mfn(g, pf, deltas=(1.0, 0.7, 2.5))), _bits(r.gradient_magnitude(pf, sobel, (1.0, 0.7, 2.5))))
     with pytest.raises(A.AprhError):
         fn(g, parts, dimension=3)
 

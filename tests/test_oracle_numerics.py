@@ -260,3 +260,52 @@ def test_local_std_equal_reference(shape, nb, seed, size):
     _, r = make_apr(shape, nb, seed, "ref")
     assert np.array_equal(got.view(np.uint32), r.local_std(parts, size).view(np.uint32))
     assert np.array_equal(o.local_std(pf, size).view(np.uint32), r.local_std(pf, size).view(np.uint32))
+
+
+def _gradient_stack(o, dim, delta, sobel):
+    """gradient_cfd / gradient_sobel's stencils (APRNumerics.hpp:165-213, APRStencil.hpp:135-152,
+    :254-297) in float32 [z, x, y], one per level delta, rescaled by 2^-d"""
+    y, x, z = o.dims
+    nd = int(y > 1) + int(x > 1) + int(z > 1)
+    h = np.float32(1.0) / (np.float32(2.0) * np.float32(delta))
+    diff = np.array([-h, 0.0, h], np.float32)
+    sm = np.array([0.25, 0.5, 0.25], np.float32)
+    fy, fx, fz = (diff if dim == 0 else sm), (diff if dim == 1 else sm), (diff if dim == 2 else sm)
+    if sobel and nd == 3:
+        st = (fy[None, None, :] * fx[None, :, None]) * fz[:, None, None]
+    elif sobel and nd == 2 and dim != 2:
+        st = (fy[None, None, :] * fx[None, :, None]).astype(np.float32)
+    else:
+        st = diff.reshape([(3, 1, 1), (1, 3, 1), (1, 1, 3)][2 - dim])
+    st = st.astype(np.float32)
+    return [st / np.float32(2.0 ** d) for d in range(max(o.l_max - o.l_min, 1))]
+
+
+@pytest.mark.parametrize("shape,nb,seed", CASES[:2] + [((1, 70, 90), 6, 5)])
+@pytest.mark.parametrize("sobel", [False, True])
+def test_gradients_equal_reference(shape, nb, seed, sobel):
+    """APRNumerics::gradient_cfd / gradient_sobel / gradient_magnitude_*: the stencils restated
+    here (incl. the 2-D Sobel of a one-plane image) with the oracle's convolution, against the
+    reference build's own functions"""
+    need_ref()
+    img, o = make_apr(shape, nb, seed, "oracle")
+    r = O.apr_from_structure(o.dims, o.xz_end_vec, o.y_vec, "ref")
+    if not hasattr(r.lib, "orc_gradient_u16"):
+        pytest.skip("reference build without the gradient exports")
+    parts = o.sample(img)
+    pf = (parts.astype(np.float32) * np.float32(0.731)).astype(np.float32)
+    dims = [d for d in (0, 1, 2) if o.dims[d] > 1]
+    grads = {}
+    for dim, delta in ((0, 1.0), (1, 0.7), (2, 2.5)):
+        if dim not in dims:
+            continue
+        stack = _gradient_stack(o, dim, delta, sobel)
+        for p in (parts, pf):
+            exp = o.convolve_pencil(stack, p, True)
+            assert np.array_equal(exp.view(np.uint32), r.gradient(p, sobel, dim, delta).view(np.uint32)), (dim, p.dtype)
+        grads[dim] = o.convolve_pencil(stack, pf, True)
+    m = grads[0] * grads[0]
+    for dim in dims[1:]:
+        m = (m + grads[dim] * grads[dim]).astype(np.float32)
+    m = np.sqrt(m).astype(np.float32)
+    assert np.array_equal(m.view(np.uint32), r.gradient_magnitude(pf, sobel, (1.0, 0.7, 2.5)).view(np.uint32))
