@@ -242,6 +242,14 @@ int aprh_gradient_sobel(aprh_context* ctx, aprh_apr* apr, const void* parts, int
 int aprh_gradient_magnitude_cfd(aprh_context* ctx, aprh_apr* apr, const void* parts, int dtype, const float* deltas, float* out);
 int aprh_gradient_magnitude_sobel(aprh_context* ctx, aprh_apr* apr, const void* parts, int dtype, const float* deltas, float* out);
 
+/* APRNumerics::face_neighbour_filter (APRNumerics.hpp:305-355): out = filter3[1] * p, plus
+   filter3[2] times the mean of the neighbours across the +dimension face and filter3[0] across
+   the -dimension face (the particle itself where a face has none); dimension 0 y, 1 x, 2 z */
+int aprh_face_neighbour_filter(aprh_context* ctx, aprh_apr* apr, const void* parts, int dtype, const float* filter3, int dimension,
+                               float* out);
+/* APRNumerics::seperable_face_neighbour_filter (APRNumerics.hpp:286-302): repeats x (y, x, z) */
+int aprh_seperable_face_neighbour_filter(aprh_context* ctx, aprh_apr* apr, const void* parts, int dtype, const float* filter3,
+                                         int32_t repeats, float* out);
 /* APRNumerics::local_std (APRNumerics.hpp:357-389): the box mean (size3 in (y, x, z), stencils
    downsampled and normalised) of the particles and of their squares (tree values: the squared
    tree means), then sqrtf(max(E[x^2] - E[x]^2, 0)).  out: float32[total]. */

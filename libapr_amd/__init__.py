@@ -151,6 +151,8 @@ def lib():
     for f in ("aprh_gradient_magnitude_cfd", "aprh_gradient_magnitude_sobel"):
         getattr(L, f).argtypes = [vp, vp, vp, C.c_int, vp, vp]
     L.aprh_local_std.argtypes = [vp, vp, vp, C.c_int, vp, vp]
+    L.aprh_face_neighbour_filter.argtypes = [vp, vp, vp, C.c_int, vp, C.c_int, vp]
+    L.aprh_seperable_face_neighbour_filter.argtypes = [vp, vp, vp, C.c_int, vp, C.c_int32, vp]
     L.aprh_reconstruct_constant_patch.argtypes = [vp, vp, vp, C.c_int, C.POINTER(AprhPatch), vp, C.c_int]
     L.aprh_reconstruct_level_patch.argtypes = [vp, vp, C.POINTER(AprhPatch), vp, C.c_int]
     L.aprh_synth_deconv_device.argtypes = [vp, vp, vp, i64, i64, i64, C.c_uint64, i64, C.c_float, C.c_float, C.c_float,
@@ -807,6 +809,30 @@ class APRNumerics:
         if sz.size != 3:
             raise ValueError("size: three window sizes (y, x, z)")
         _check(lib().aprh_local_std(apr.ctx.h, apr.h, _ptr(parts), code, _ptr(sz), _ptr(particle_output)))
+        return particle_output[:apr.total] if isinstance(particle_output, np.ndarray) else particle_output
+
+    @staticmethod
+    def face_neighbour_filter(apr, particle_input, filter, dimension, particle_output=None, dtype=None):
+        """APRNumerics::face_neighbour_filter (APRNumerics.hpp:305-355); filter = (-face, centre, +face)"""
+        parts = _host_parts(particle_input)
+        code = _dtype_of(parts, dtype)
+        if particle_output is None:
+            particle_output = np.zeros(max(apr.total, 1), np.float32)
+        f = np.ascontiguousarray(filter, np.float32)
+        _check(lib().aprh_face_neighbour_filter(apr.ctx.h, apr.h, _ptr(parts), code, _ptr(f), int(dimension),
+                                                _ptr(particle_output)))
+        return particle_output[:apr.total] if isinstance(particle_output, np.ndarray) else particle_output
+
+    @staticmethod
+    def seperable_face_neighbour_filter(apr, particle_input, filter, repeats=1, particle_output=None, dtype=None):
+        """APRNumerics::seperable_face_neighbour_filter (APRNumerics.hpp:286-302)"""
+        parts = _host_parts(particle_input)
+        code = _dtype_of(parts, dtype)
+        if particle_output is None:
+            particle_output = np.zeros(max(apr.total, 1), np.float32)
+        f = np.ascontiguousarray(filter, np.float32)
+        _check(lib().aprh_seperable_face_neighbour_filter(apr.ctx.h, apr.h, _ptr(parts), code, _ptr(f), int(repeats),
+                                                          _ptr(particle_output)))
         return particle_output[:apr.total] if isinstance(particle_output, np.ndarray) else particle_output
 
     @staticmethod

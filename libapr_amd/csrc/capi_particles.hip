@@ -401,6 +401,48 @@ int aprh_gradient_magnitude_sobel(aprh_context* ctx, aprh_apr* a, const void* pa
     return gradient_entry(ctx, a, parts, dtype, -1, deltas, true, out);
 }
 
+int aprh_face_neighbour_filter(aprh_context* ctx, aprh_apr* a, const void* parts, int dtype, const float* filter3, int dimension,
+                               float* out) {
+    APRH_TRY
+    check_apr(ctx, a);
+    if (!parts || !out || !filter3) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (dimension < 0 || dimension > 2) throw Error(APRH_ERR_INVALID_ARGUMENT, "dimension must be 0 (y), 1 (x) or 2 (z)");
+    const size_t es = dtype_size(dtype);
+    DevIn in(ctx, parts, std::max<uint64_t>(a->total, 1) * es);
+    DevOut o(ctx, out, std::max<uint64_t>(a->total, 1) * 4);
+    if (in.ptr == o.ptr) throw Error(APRH_ERR_INVALID_ARGUMENT, "input and output must be different buffers");
+    face_neighbour_filter(ctx, *a, in.ptr, dtype, (float*)o.ptr, filter3, dimension);
+    o.finish(ctx);
+    APRH_CATCH
+}
+
+int aprh_seperable_face_neighbour_filter(aprh_context* ctx, aprh_apr* a, const void* parts, int dtype, const float* filter3,
+                                         int32_t repeats, float* out) {
+    APRH_TRY
+    check_apr(ctx, a);
+    if (!parts || !out || !filter3) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (repeats < 0) throw Error(APRH_ERR_INVALID_ARGUMENT, "negative repeats");
+    const size_t es = dtype_size(dtype);
+    const uint64_t P = a->total;
+    DevIn in(ctx, parts, std::max<uint64_t>(P, 1) * es);
+    DevOut o(ctx, out, std::max<uint64_t>(P, 1) * 4);
+    if (P > 0) {
+        /* APRNumerics.hpp:286-302: tmp = input (as float); per repeat y, x, z passes */
+        Buf tmp(ctx->pool, P * 4), w(ctx->pool, P * 4);
+        if (dtype == APRH_DTYPE_U16) local_std_step(ctx, 0, tmp.as<float>(), in.ptr, P);
+        else launch_copy(ctx, tmp.ptr(), in.ptr, P * 4);
+        for (int r = 0; r < repeats; ++r) {
+            face_neighbour_filter(ctx, *a, tmp.ptr(), APRH_DTYPE_F32, w.as<float>(), filter3, 0);
+            face_neighbour_filter(ctx, *a, w.ptr(), APRH_DTYPE_F32, tmp.as<float>(), filter3, 1);
+            face_neighbour_filter(ctx, *a, tmp.ptr(), APRH_DTYPE_F32, w.as<float>(), filter3, 2);
+            launch_copy(ctx, tmp.ptr(), w.ptr(), P * 4);
+        }
+        launch_copy(ctx, o.ptr, tmp.ptr(), P * 4);
+    }
+    o.finish(ctx);
+    APRH_CATCH
+}
+
 int aprh_local_std(aprh_context* ctx, aprh_apr* a, const void* parts, int dtype, const int32_t* size3, float* out) {
     APRH_TRY
     check_apr(ctx, a);

@@ -633,6 +633,119 @@ struct GradMag {
 };
 }  // namespace
 
+namespace {
+/* APRNumerics::face_neighbour_filter (APRNumerics.hpp:305-355) along one dimension: one lane
+   per particle (a wave per row).  The neighbours of each face come from the random iterator's
+   search (APRIterator.hpp:283-394, RandomAccess.hpp:92-170): the same-level cell, else (above
+   l_min) the parent-level cell containing the face neighbour, else (below l_max) the four
+   children on the face, counted where they are particles provided the first one is.  Cells
+   are found by binary search in their row; coordinates follow the reference's unsigned 16-bit
+   arithmetic, and the four l_max rows under the first parent row read row (0, 0), as the
+   reference's random access resolves them. */
+struct FaceFilter {
+    struct Args {
+        Rows R;
+        const uint64_t* xz_end;
+        const uint16_t* y_vec;
+        const void* in;
+        int in_u16;
+        float* out;
+        float f0, f1, f2;
+        int dim;
+        int64_t total_rows;
+        int64_t G;
+    };
+    static constexpr int THREADS = 256;
+    static const char* name() { return "face_neighbour_filter"; }
+
+    static __device__ __forceinline__ float val(const Args& a, int64_t i) {
+        return a.in_u16 ? (float)((const uint16_t*)a.in)[i] : ((const float*)a.in)[i];
+    }
+    /* particle index at (level, z, x, y) or -1 */
+    static __device__ __forceinline__ int64_t find(const Args& a, int level, int64_t z, int64_t x, int64_t y) {
+        const Rows& R = a.R;
+        const int li = level - R.l_lo;
+        if (li < 0 || li >= R.n_levels) return -1;
+        if (z < 0 || x < 0 || y < 0 || z >= R.z_num[li] || x >= R.x_num[li] || y >= R.y_num[li]) return -1;
+        if (li == R.n_levels - 1 && z < 2 && x < 2) {
+            z = 0;
+            x = 0;
+        }
+        const int64_t rid = R.row_begin[li] + z * R.x_num[li] + x;
+        uint64_t lo = a.xz_end[rid - 1], hi = a.xz_end[rid];
+        const uint64_t e = hi;
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if ((int64_t)a.y_vec[mid] < y) lo = mid + 1;
+            else hi = mid;
+        }
+        return (lo < e && (int64_t)a.y_vec[lo] == y) ? (int64_t)lo : -1;
+    }
+    static __device__ __forceinline__ int64_t u16c(int64_t v) { return (int64_t)(uint16_t)v; }
+
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) {
+        const int lane = threadIdx.x & 63;
+        const int64_t wpb = blockDim.x / 64;
+        const Rows& R = a.R;
+        const int l_min = R.l_lo, l_max = R.l_lo + R.n_levels - 1;
+        for (int64_t r = t * wpb + (threadIdx.x >> 6); r < a.total_rows; r += a.G * wpb) {
+            const int li = row_level(R, r);
+            const int level = R.l_lo + li;
+            const int64_t row = r - R.prefix[li];
+            const int64_t z = row / R.x_num[li], x = row % R.x_num[li];
+            const uint64_t b = a.xz_end[R.row_begin[li] + row - 1], e = a.xz_end[R.row_begin[li] + row];
+            for (uint64_t i = b + lane; i < e; i += 64) {
+                const int64_t y = a.y_vec[i];
+                const float cur = val(a, (int64_t)i);
+                float o = cur * a.f1;
+                for (int k = 0; k < 2; ++k) {
+                    const int sgn = k == 0 ? 1 : -1;
+                    const int dy = a.dim == 0 ? sgn : 0, dx = a.dim == 1 ? sgn : 0, dz = a.dim == 2 ? sgn : 0;
+                    const float ft = k == 0 ? a.f2 : a.f0;
+                    float sum = 0.0f, cnt = 0.0f;
+                    const int64_t ny = u16c(y + dy), nx = u16c(x + dx), nz = u16c(z + dz);
+                    int64_t n = find(a, level, nz, nx, ny);
+                    if (n < 0 && level > l_min) n = find(a, level - 1, nz / 2, nx / 2, ny / 2);
+                    if (n >= 0) {
+                        sum += val(a, n);
+                        cnt += 1.0f;
+                    } else if (level < l_max) {
+                        const int64_t cy = u16c(ny * 2 + (dy < 0)), cx = u16c(nx * 2 + (dx < 0)), cz = u16c(nz * 2 + (dz < 0));
+                        for (int c = 0; c < 4; ++c) {
+                            const int o0 = c & 1, o1 = c >> 1;
+                            int64_t yy = cy, xx = cx, zz = cz;
+                            if (a.dim == 0) { xx += o0; zz += o1; }
+                            else if (a.dim == 1) { yy += o0; zz += o1; }
+                            else { yy += o0; xx += o1; }
+                            const int64_t m = find(a, level + 1, u16c(zz), u16c(xx), u16c(yy));
+                            if (m < 0) {
+                                if (c == 0) break;
+                                continue;
+                            }
+                            sum += val(a, m);
+                            cnt += 1.0f;
+                        }
+                    }
+                    if (cnt > 0.0f) o += ft * sum / cnt;
+                    else o += ft * cur;
+                }
+                a.out[i] = o;
+            }
+        }
+    }
+};
+}  // namespace
+
+void face_neighbour_filter(aprh_context* ctx, const aprh_apr& a, const void* in, int in_dtype, float* out, const float* filter3, int dim) {
+    if (a.total == 0) return;
+    Rows R = make_rows(a.y_num, a.x_num, a.z_num, a.level_xz_vec, a.l_min, a.l_max);
+    const int64_t rows = R.prefix[R.n_levels];
+    const int64_t G = stride_tiles(rows, 4, 16384);
+    launch_tiles<FaceFilter>(ctx, {R, a.xz_end.as<uint64_t>(), a.y_vec.as<uint16_t>(), in, in_dtype == APRH_DTYPE_U16 ? 1 : 0, out,
+                                   filter3[0], filter3[1], filter3[2], dim, rows, G},
+                             G);
+}
+
 void gradient_magnitude_combine(aprh_context* ctx, float* g, const float* gx, const float* gz, uint64_t n) {
     if (n == 0) return;
     const int64_t G = stride_tiles((int64_t)n, 256 * 4);

@@ -1594,10 +1594,17 @@ void convolve_pencil(const GenInfo& g, const Linear& L, const Tree& t, const std
 /* of the neighbours, or times the particle's own value when it has none.    */
 /* Coordinates are unsigned 16-bit in the reference: -1 wraps out of range.  */
 /* ------------------------------------------------------------------------ */
-/* index of the particle at (level, z, x, y), or -1 */
+/* index of the particle at (level, z, x, y), or -1.  The reference's random access resolves
+   the four l_max rows under the first parent row, (z, x) in {0, 1}^2, to row (0, 0): their
+   particles read the values of row (0, 0) with the same y. */
 static inline int64_t find_particle(const GenInfo& g, const Linear& L, int level, int64_t z, int64_t x, int64_t y) {
     if (level < g.l_min || level > g.l_max) return -1;
     if (z < 0 || x < 0 || y < 0 || z >= g.z_num[level] || x >= g.x_num[level] || y >= g.y_num[level]) return -1;
+    if (level == g.l_max && z < 2 && x < 2) {
+        z = 0;
+        x = 0;
+    }
+
     uint64_t b = L.row_begin(level, z, x, g), e = L.row_end(level, z, x, g);
     const uint16_t* yv = L.y_vec.data();
     uint64_t lo = b, hi = e;
@@ -1756,6 +1763,162 @@ void reconstruct_constant(const GenInfo& g, const Linear& L, const Tree* t, cons
                 for (uint64_t i = t->lin.row_begin(lv, z, x, ti); i < t->lin.row_end(lv, z, x, ti); ++i)
                     img.at(t->lin.y_vec[i], x, z) = ImgCvt<ImgT>::of(tree[i]);
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* APRReconstruction::reconstruct_smooth (APRReconstruction.hpp:270-295): the  */
+/* constant reconstruction, then per axis a running-sum mean whose half-width */
+/* at a pixel is min(floor(2^(l_max - level) / scale), offset_max) with       */
+/* offset_max = min(20, n/2 - 1) (calc_sat_adaptive_y/x/z, :528-876).  The    */
+/* arithmetic is the reference's: cumulative sums in float, some scale        */
+/* factors in double, every store converted to the image type T as a C++     */
+/* assignment on x86-64 (float / double -> u16 through int32), the half-width */
+/* as T then as float (y) or int64 (x, z).  As in the reference, the z pass's */
+/* low-end boundary test compares the x index (not z) with the half-width.    */
+/* ------------------------------------------------------------------------ */
+template <typename T> struct SmCvt;
+template <> struct SmCvt<float> {
+    static float f(float v) { return v; }
+    static float d(double v) { return (float)v; }
+    static float off(double v) { return (float)v; }
+};
+template <> struct SmCvt<uint16_t> {
+    static uint16_t f(float v) { return f2u16(v); }
+    static uint16_t d(double v) { return d2u16(v); }
+    static uint16_t off(double v) { return d2u16(v); }
+};
+
+template <typename T>
+static inline T sm_offset(uint16_t level, float scale, int om, int d_max) {
+    const T a = SmCvt<T>::off(std::floor(std::pow(2.0, (double)(d_max - (int)level)) / (double)scale));
+    const T b = (T)om;
+    return std::min(a, b);
+}
+
+template <typename T>
+void sat_adaptive_y(Mesh<T>& in, const Mesh<uint16_t>& kimg, float scale, int om_in, int d_max) {
+    const int64_t yn = in.y_num, xn = in.x_num, zn = in.z_num;
+    const int om = std::min(om_in, (int)(yn / 2 - 1));
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < zn; ++j) {
+        std::vector<float> tv((size_t)yn), ov((size_t)yn);
+        for (int64_t i = 0; i < xn; ++i) {
+            T* m = in.v.data() + (size_t)((j * xn + i) * yn);
+            const uint16_t* kl = kimg.v.data() + (size_t)((j * xn + i) * yn);
+            for (int64_t k = 0; k < yn; ++k) ov[k] = (float)sm_offset<T>(kl[k], scale, om, d_max);
+            float temp = 0;
+            for (int64_t k = 0; k < yn; ++k) {
+                temp += (float)m[k];
+                tv[k] = temp;
+            }
+            m[0] = 0;
+            for (int64_t k = 1; k <= om + 1; ++k) {
+                const float div = 2 * ov[k] + 1, off = ov[k];
+                if ((float)k <= off + 1) m[k] = SmCvt<T>::f(-tv[0] / div);
+            }
+            for (int64_t k = 0; k < yn; ++k) {
+                const float div = 2 * ov[k] + 1, off = ov[k];
+                if ((float)k >= off + 1) m[k] = SmCvt<T>::f(-tv[(size_t)(k - (int64_t)off - 1)] / div);
+            }
+            for (int64_t k = 0; k < yn; ++k) {
+                const float div = 2 * ov[k] + 1, off = ov[k];
+                if ((float)k < (float)yn - off) m[k] = SmCvt<T>::f((float)m[k] + tv[(size_t)(k + (int64_t)off)] / div);
+            }
+            for (int64_t k = yn - om; k < yn; ++k) {
+                const float div = 2 * ov[k] + 1, off = ov[k];
+                if ((float)k >= (float)yn - off) {
+                    const float counter = (float)k - ((float)yn - off) + 1;
+                    m[k] = SmCvt<T>::f((float)m[k] * div);
+                    m[k] = SmCvt<T>::f((float)m[k] + tv[(size_t)(yn - 1)]);
+                    m[k] = SmCvt<T>::d((double)m[k] * (1.0 / (double)(div - counter)));
+                }
+            }
+            for (int64_t k = 1; k < om + 1; ++k) {
+                const float div = 2 * ov[k] + 1, off = ov[k];
+                if ((float)k < off + 1) m[k] = SmCvt<T>::d((double)m[k] * ((double)div / (1.0 * k + (double)off)));
+            }
+            {
+                const float div = 2 * ov[0] + 1, off = ov[0];
+                m[0] = SmCvt<T>::f((float)m[0] * (div / (off + 1)));
+            }
+        }
+    }
+}
+
+/* x (axis 1) or z (axis 2) pass: a ring of 2 offset_max + 2 cumulative-sum rows */
+template <typename T>
+void sat_adaptive_outer(Mesh<T>& in, const Mesh<uint16_t>& kimg, float scale, int om_in, int d_max, int axis) {
+    const int64_t yn = in.y_num, xn = in.x_num, zn = in.z_num;
+    const int64_t n = axis == 1 ? xn : zn;
+    const int64_t om = std::min(om_in, (int)(n / 2 - 1));
+    const int64_t R = 2 * om + 2;
+    const int64_t outer = axis == 1 ? zn : xn;
+    const int64_t stride = axis == 1 ? yn : xn * yn;
+#pragma omp parallel for schedule(static)
+    for (int64_t o = 0; o < outer; ++o) {
+        std::vector<float> tv((size_t)(yn * R), 0.0f);
+        const int64_t base = axis == 1 ? o * xn * yn : o * yn;
+        T* m = in.v.data() + base;
+        const uint16_t* kl = kimg.v.data() + base;
+        auto M = [&](int64_t i, int64_t k) -> T& { return m[i * stride + k]; };
+        auto K = [&](int64_t i, int64_t k) { return kl[i * stride + k]; };
+        for (int64_t k = 0; k < yn; ++k) tv[(size_t)k] = (float)M(0, k);
+        for (int64_t i = 1; i < 2 * om + 1; ++i)
+            for (int64_t k = 0; k < yn; ++k) tv[(size_t)(i * yn + k)] = (float)M(i, k) + tv[(size_t)((i - 1) * yn + k)];
+        for (int64_t i = 0; i < om + 1; ++i)
+            for (int64_t k = 0; k < yn; ++k) {
+                const int64_t off = (int64_t)sm_offset<T>(K(i, k), scale, (int)om, d_max);
+                /* the z pass tests the x index here (APRReconstruction.hpp:821) */
+                const int64_t t = axis == 1 ? i : o;
+                if (t < off + 1) M(i, k) = SmCvt<T>::f(tv[(size_t)((i + off) * yn + k)] / (float)(i + off + 1));
+            }
+        for (int64_t i = 1; i < n; ++i)
+            for (int64_t k = 0; k < yn; ++k) {
+                const int64_t off = (int64_t)sm_offset<T>(K(i, k), scale, (int)om, d_max);
+                if ((i >= om + 1) & (i < n - om)) {
+                    const int64_t im = (i + om) % R, pm = (i + om - 1) % R;
+                    tv[(size_t)(im * yn + k)] = (float)M(i + om, k) + tv[(size_t)(pm * yn + k)];
+                }
+                if ((i >= off + 1) & (i < n - off)) {
+                    const int64_t fm = (i + off) % R, bm = (i - off - 1) % R;
+                    M(i, k) = SmCvt<T>::f((tv[(size_t)(fm * yn + k)] - tv[(size_t)(bm * yn + k)]) / (float)(2 * off + 1));
+                }
+            }
+        for (int64_t i = n - om; i < n; ++i)
+            for (int64_t k = 0; k < yn; ++k) {
+                const int64_t off = (int64_t)sm_offset<T>(K(i, k), scale, (int)om, d_max);
+                if (i >= n - off) {
+                    const int64_t bm = (i - off - 1) % R, fm = (n - 1) % R;
+                    M(i, k) = SmCvt<T>::f((tv[(size_t)(fm * yn + k)] - tv[(size_t)(bm * yn + k)]) / (float)(n - i + off));
+                }
+            }
+    }
+}
+
+/* the level of the particle covering each pixel (reconstruct_level, APRReconstruction.hpp:131-171) */
+static inline void level_image(const GenInfo& g, const Linear& L, Mesh<uint16_t>& img) {
+    img.resize(g.y_num[g.l_max], g.x_num[g.l_max], g.z_num[g.l_max], 0);
+    for (int level = g.l_min; level <= g.l_max; ++level) {
+        const int64_t s = (int64_t)1 << (g.l_max - level);
+        for (int64_t z = 0; z < g.z_num[level]; ++z)
+            for (int64_t x = 0; x < g.x_num[level]; ++x)
+                for (uint64_t i = L.row_begin(level, z, x, g); i < L.row_end(level, z, x, g); ++i)
+                    for (int64_t zz = z * s; zz < std::min((z + 1) * s, img.z_num); ++zz)
+                        for (int64_t xx = x * s; xx < std::min((x + 1) * s, img.x_num); ++xx)
+                            for (int64_t yy = L.y_vec[i] * s; yy < std::min((L.y_vec[i] + 1) * s, img.y_num); ++yy)
+                                img.at(yy, xx, zz) = (uint16_t)level;
+    }
+}
+
+template <typename PartT, typename ImgT>
+void reconstruct_smooth(const GenInfo& g, const Linear& L, const PartT* parts, const float* scale_d, Mesh<ImgT>& img) {
+    reconstruct_constant<PartT, ImgT>(g, L, nullptr, parts, nullptr, 0, img);
+    Mesh<uint16_t> k;
+    level_image(g, L, k);
+    const int om = 20;
+    if (img.y_num > 1) sat_adaptive_y(img, k, scale_d[0], om, g.l_max);
+    if (img.x_num > 1) sat_adaptive_outer(img, k, scale_d[1], om, g.l_max, 1);
+    if (img.z_num > 1) sat_adaptive_outer(img, k, scale_d[2], om, g.l_max, 2);
 }
 
 /* APRStencil::create_gaussian_filter (APRStencil.hpp:173-228) */

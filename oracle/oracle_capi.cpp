@@ -198,6 +198,23 @@ void* orc_get_apr_f32(const float* img, int64_t y, int64_t x, int64_t z, const a
     }
 }
 
+/* APRReconstruction::reconstruct_smooth (APRReconstruction.hpp:270-295); parts dtype 0 u16 /
+   1 f32, image dtype 0 u16 / 1 f32 */
+void orc_reconstruct_smooth(void* hp, const void* parts, int pdt, const float* scale3, void* out, int idt) {
+    Handle* h = (Handle*)hp;
+    if (idt == 0) {
+        Mesh<uint16_t> img;
+        if (pdt == 0) reconstruct_smooth(h->r.info, h->r.lin, (const uint16_t*)parts, scale3, img);
+        else reconstruct_smooth(h->r.info, h->r.lin, (const float*)parts, scale3, img);
+        std::copy(img.v.begin(), img.v.end(), (uint16_t*)out);
+    } else {
+        Mesh<float> img;
+        if (pdt == 0) reconstruct_smooth(h->r.info, h->r.lin, (const uint16_t*)parts, scale3, img);
+        else reconstruct_smooth(h->r.info, h->r.lin, (const float*)parts, scale3, img);
+        std::copy(img.v.begin(), img.v.end(), (float*)out);
+    }
+}
+
 /* APRNumerics::face_neighbour_filter / seperable_face_neighbour_filter (repeats >= 0) */
 void orc_face_neighbour_filter_f32(void* hp, const float* in, const float* filter3, int dim, int repeats, float* out) {
     Handle* h = (Handle*)hp;

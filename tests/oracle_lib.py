@@ -104,6 +104,8 @@ def _bind(lib):
         lib.orc_gradient_magnitude_f32.argtypes = [C.c_void_p, f32p, C.c_int, f32p, f32p]
     if hasattr(lib, "orc_face_neighbour_filter_f32"):
         lib.orc_face_neighbour_filter_f32.argtypes = [C.c_void_p, f32p, f32p, C.c_int, C.c_int, f32p]
+    if hasattr(lib, "orc_reconstruct_smooth"):
+        lib.orc_reconstruct_smooth.argtypes = [C.c_void_p, C.c_void_p, C.c_int, f32p, C.c_void_p, C.c_int]
     if hasattr(lib, "orc_reconstruct"):
         lib.orc_reconstruct.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int]
     return lib
@@ -253,6 +255,19 @@ class OracleAPR:
         parts = np.zeros(max(self.total, 1), np.float32)
         self.lib.orc_sample_image_f32(self.h, np.ascontiguousarray(img, np.float32).reshape(-1), parts)
         return parts[:self.total]
+
+    def reconstruct_smooth(self, parts, scale=(2.0, 2.0, 2.0), out_dtype=None):
+        """APRReconstruction::reconstruct_smooth (APRReconstruction.hpp:270-295): image [z, x, y]"""
+        parts = np.ascontiguousarray(parts)
+        pdt = 0 if parts.dtype == np.uint16 else 1
+        if pdt == 1:
+            parts = np.ascontiguousarray(parts, np.float32)
+        odt = np.dtype(out_dtype) if out_dtype is not None else parts.dtype
+        y, x, z = self.dims
+        out = np.zeros((z, x, y), odt)
+        self.lib.orc_reconstruct_smooth(self.h, parts.ctypes.data, pdt, np.ascontiguousarray(scale, np.float32), out.ctypes.data,
+                                        0 if odt == np.uint16 else 1)
+        return out
 
     def reconstruct(self, parts, level_delta=0, out_dtype=None):
         """piecewise-constant image [z, x, y] at level l_max + level_delta"""

@@ -450,3 +450,22 @@ def test_local_std(oracle, ctx, shape, nb, seed, size):
     assert np.array_equal(_bits(A.APRNumerics.local_std(g, parts, size=size)), _bits(o.local_std(parts, size)))
     pf = (parts.astype(np.float32) * np.float32(0.37)).astype(np.float32)
     assert np.array_equal(_bits(A.APRNumerics.local_std(g, pf, size=size)), _bits(o.local_std(pf, size)))
+
+
+@pytest.mark.parametrize("shape,nb,seed", CASES + [((1, 70, 90), 6, 5), ((33, 17, 9), 5, 6)])
+def test_face_neighbour_filters(oracle, ctx, shape, nb, seed):
+    """APRNumerics::face_neighbour_filter / seperable_face_neighbour_filter, bit-equal to the
+    oracle (itself bit-equal to the reference build), u16 and f32 input"""
+    img, o, g = _pair(oracle, ctx, shape, nb, seed)
+    parts = o.sample(img)
+    pf = (parts.astype(np.float32) * np.float32(0.37)).astype(np.float32)
+    f = np.array([0.25, 0.5, 0.3], np.float32)
+    for p in (parts, pf):
+        for dim in range(3):
+            got = A.APRNumerics.face_neighbour_filter(g, p, f, dim)
+            exp = o.face_neighbour_filter(p.astype(np.float32), f, dim)
+            assert np.array_equal(_bits(got), _bits(exp)), (dim, int((_bits(got) != _bits(exp)).sum()))
+    for rep in (0, 1, 2):
+        got = A.APRNumerics.seperable_face_neighbour_filter(g, pf, f, rep)
+        assert np.array_equal(_bits(got), _bits(o.face_neighbour_filter(pf, f, 0, rep))), rep
+

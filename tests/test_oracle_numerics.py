@@ -309,3 +309,37 @@ def test_gradients_equal_reference(shape, nb, seed, sobel):
         m = (m + grads[dim] * grads[dim]).astype(np.float32)
     m = np.sqrt(m).astype(np.float32)
     assert np.array_equal(m.view(np.uint32), r.gradient_magnitude(pf, sobel, (1.0, 0.7, 2.5)).view(np.uint32))
+
+
+FACE_CASES = CASES + [((1, 70, 90), 6, 5), ((40, 1, 50), 5, 2), ((33, 17, 9), 5, 6), ((61, 63, 65), 30, 1)]
+
+
+@pytest.mark.parametrize("shape,nb,seed", FACE_CASES)
+def test_face_neighbour_filters_equal_reference(shape, nb, seed):
+    """APRNumerics::face_neighbour_filter per dimension and seperable_face_neighbour_filter
+    (0, 1, 2 repeats): the oracle's restatement of the random iterator's neighbour search
+    against the reference build, bit for bit, on odd, even and one-plane images"""
+    need_ref()
+    img, o = make_apr(shape, nb, seed, "oracle")
+    r = O.apr_from_structure(o.dims, o.xz_end_vec, o.y_vec, "ref")
+    if not hasattr(r.lib, "orc_face_neighbour_filter_f32"):
+        pytest.skip("reference build without the face-neighbour export")
+    pf = (o.sample(img).astype(np.float32) * np.float32(0.37)).astype(np.float32)
+    for f in (np.array([0.25, 0.5, 0.3], np.float32), np.array([1.0, 0.0, 0.0], np.float32)):
+        for dim in range(3):
+            a, b = o.face_neighbour_filter(pf, f, dim), r.face_neighbour_filter(pf, f, dim)
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (dim, int((a != b).sum()))
+        for rep in (0, 1, 2):
+            a, b = o.face_neighbour_filter(pf, f, 0, rep), r.face_neighbour_filter(pf, f, 0, rep)
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (rep, int((a != b).sum()))
+
+
+def test_face_neighbour_filter_closed_forms():
+    """a constant stays constant under weights summing to 1; the identity filter is the input"""
+    img, o = make_apr((29, 45, 61), 12, 3, "oracle")
+    c = np.full(o.total, 3.5, np.float32)
+    f = np.array([0.25, 0.5, 0.25], np.float32)
+    for dim in range(3):
+        assert np.all(o.face_neighbour_filter(c, f, dim) == np.float32(3.5))
+    p = o.sample(img).astype(np.float32)
+    assert np.array_equal(o.face_neighbour_filter(p, np.array([0, 1, 0], np.float32), 1), p)
