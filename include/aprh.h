@@ -229,6 +229,12 @@ int aprh_reconstruct_constant(aprh_context* ctx, aprh_apr* apr, const void* part
    volume (y, x, z, y fastest) takes the level of the particle covering it; image_dtype U16 or
    F32 (the reference writes any pixel type) */
 int aprh_reconstruct_level(aprh_context* ctx, aprh_apr* apr, void* image, int image_dtype);
+/* APRReconstruction::reconstruct_smooth(apr, img, parts, scale_d) (APRReconstruction.hpp:270-295):
+   the constant reconstruction smoothed along y, x, z by running-sum means whose half-width at a
+   pixel is min(floor(2^(l_max - level) / scale_d[axis]), 20) (calc_sat_adaptive_y/x/z, :528-876);
+   scale3 = NULL means the reference's default {2, 2, 2} */
+int aprh_reconstruct_smooth(aprh_context* ctx, aprh_apr* apr, const void* parts, int parts_dtype, const float* scale3, void* image,
+                            int image_dtype);
 /* APRNumerics::gradient_cfd (APRNumerics.hpp:165-184) and gradient_sobel (:187-213): the
    particles convolved (convolve_pencil, reflected boundary) with the central difference
    [-1, 0, 1] / (2 delta) along `dimension` (0 y, 1 x, 2 z), or with the Sobel stencil (that

@@ -145,6 +145,7 @@ def lib():
                                        C.c_int, C.c_int, C.c_double, C.POINTER(AprhRLStats)]
     L.aprh_reconstruct_constant.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp, C.c_int]
     L.aprh_reconstruct_level.argtypes = [vp, vp, vp, C.c_int]
+    L.aprh_reconstruct_smooth.argtypes = [vp, vp, vp, C.c_int, C.POINTER(C.c_float), vp, C.c_int]
     L.aprh_patch_check_limits.argtypes = [vp, C.POINTER(AprhPatch)]
     for f in ("aprh_gradient_cfd", "aprh_gradient_sobel"):
         getattr(L, f).argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_float, vp]
@@ -895,7 +896,8 @@ def _clamped_patch(apr, patch):
 
 class APRReconstruction:
     """APRReconstruction::reconstruct_constant (APRReconstruction.hpp:80-120, :297-394) and
-    reconstruct_level (:131-171), each also over a ReconPatch (:186-196, :296-490)."""
+    reconstruct_level (:131-171), each also over a ReconPatch (:186-196, :296-490), and
+    reconstruct_smooth (:270-295)."""
 
     @staticmethod
     def reconstruct_constant(apr, parts, level_delta=0, out_dtype=None, out=None, dtype=None, patch=None):
@@ -945,6 +947,23 @@ class APRReconstruction:
             y, x, z = apr.org_dims
             out = np.zeros((z, x, y), odt)
         _check(lib().aprh_reconstruct_level(apr.ctx.h, apr.h, _ptr(out), ocode))
+        return out
+
+    @staticmethod
+    def reconstruct_smooth(apr, parts, scale_d=(2.0, 2.0, 2.0), out_dtype=None, out=None, dtype=None):
+        """APRReconstruction::reconstruct_smooth (APRReconstruction.hpp:270-295): the constant
+        reconstruction smoothed per axis by running means whose half-width follows the particle
+        level (calc_sat_adaptive_y/x/z, :528-876); scale_d in (y, x, z) order; [z, x, y]"""
+        parts = _host_parts(parts)
+        code = _dtype_of(parts, dtype)
+        odt = np.dtype(out_dtype) if out_dtype is not None else (np.dtype(np.uint16) if code == APRH_DTYPE_U16
+                                                                  else np.dtype(np.float32))
+        ocode = _dtype_of(None, str(odt))
+        if out is None:
+            y, x, z = apr.org_dims
+            out = np.zeros((z, x, y), odt)
+        sc = (C.c_float * 3)(*[float(v) for v in scale_d])
+        _check(lib().aprh_reconstruct_smooth(apr.ctx.h, apr.h, _ptr(parts), code, sc, _ptr(out), ocode))
         return out
 
 

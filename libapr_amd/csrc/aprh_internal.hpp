@@ -351,6 +351,8 @@ int richardson_lucy(aprh_context* ctx, aprh_apr& a, const T* in, float* u, const
 void reconstruct_any(aprh_context* ctx, aprh_apr& a, const void* parts, int pdt, int level, void* img, int idt);
 /* level image at `level` (<= l_max) with pixels of finer particles set to `level` */
 void reconstruct_level(aprh_context* ctx, aprh_apr& a, int level, void* img, int idt);
+/* APRReconstruction::reconstruct_smooth of the whole image (scale3: y, x, z) */
+void reconstruct_smooth(aprh_context* ctx, aprh_apr& a, const void* parts, int pdt, const float* scale3, void* img, int idt);
 /* APRNumerics::face_neighbour_filter along dim (filter3: -face, centre, +face weights); in and
    out must differ */
 void face_neighbour_filter(aprh_context* ctx, const aprh_apr& a, const void* in, int in_dtype, float* out, const float* filter3, int dim);

@@ -362,6 +362,23 @@ int aprh_reconstruct_level(aprh_context* ctx, aprh_apr* a, void* image, int imag
     APRH_CATCH
 }
 
+int aprh_reconstruct_smooth(aprh_context* ctx, aprh_apr* a, const void* parts, int parts_dtype, const float* scale3, void* image,
+                            int image_dtype) {
+    APRH_TRY
+    check_apr(ctx, a);
+    if (!parts || !image) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    const size_t pes = dtype_size(parts_dtype), ies = dtype_size(image_dtype);
+    float sc[3] = {2.0f, 2.0f, 2.0f};
+    if (scale3)
+        for (int i = 0; i < 3; ++i) sc[i] = scale3[i];
+    const size_t n = (size_t)(a->y_num[a->l_max] * a->x_num[a->l_max] * a->z_num[a->l_max]);
+    DevIn in(ctx, parts, std::max<uint64_t>(a->total, 1) * pes);
+    DevOut out(ctx, image, n * ies);
+    reconstruct_smooth(ctx, *a, in.ptr, parts_dtype, sc, out.ptr, image_dtype);
+    out.finish(ctx);
+    APRH_CATCH
+}
+
 static int gradient_entry(aprh_context* ctx, aprh_apr* a, const void* parts, int dtype, int dim, const float* deltas, bool sobel,
                           float* out) {
     APRH_TRY

@@ -215,6 +215,21 @@ void orc_reconstruct_smooth(void* hp, const void* parts, int pdt, const float* s
     }
 }
 
+/* one adaptive mean pass of reconstruct_smooth on a float image (axis 0 y, 1 x, 2 z) with a
+   caller-supplied level image: calc_sat_adaptive_y/x/z (APRReconstruction.hpp:528-876) */
+void orc_sat_adaptive_f32(float* img, const uint16_t* levels, int64_t y, int64_t x, int64_t z, float scale, int offset_max,
+                          int d_max, int axis) {
+    Mesh<float> m;
+    m.resize(y, x, z, 0.0f);
+    std::copy(img, img + y * x * z, m.v.begin());
+    Mesh<uint16_t> k;
+    k.resize(y, x, z, 0);
+    std::copy(levels, levels + y * x * z, k.v.begin());
+    if (axis == 0) sat_adaptive_y(m, k, scale, offset_max, d_max);
+    else sat_adaptive_outer(m, k, scale, offset_max, d_max, axis);
+    std::copy(m.v.begin(), m.v.end(), img);
+}
+
 /* APRNumerics::face_neighbour_filter / seperable_face_neighbour_filter (repeats >= 0) */
 void orc_face_neighbour_filter_f32(void* hp, const float* in, const float* filter3, int dim, int repeats, float* out) {
     Handle* h = (Handle*)hp;

@@ -104,6 +104,8 @@ def _bind(lib):
         lib.orc_gradient_magnitude_f32.argtypes = [C.c_void_p, f32p, C.c_int, f32p, f32p]
     if hasattr(lib, "orc_face_neighbour_filter_f32"):
         lib.orc_face_neighbour_filter_f32.argtypes = [C.c_void_p, f32p, f32p, C.c_int, C.c_int, f32p]
+    if hasattr(lib, "orc_sat_adaptive_f32"):
+        lib.orc_sat_adaptive_f32.argtypes = [f32p, u16p, C.c_int64, C.c_int64, C.c_int64, C.c_float, C.c_int, C.c_int, C.c_int]
     if hasattr(lib, "orc_reconstruct_smooth"):
         lib.orc_reconstruct_smooth.argtypes = [C.c_void_p, C.c_void_p, C.c_int, f32p, C.c_void_p, C.c_int]
     if hasattr(lib, "orc_reconstruct"):

@@ -1,6 +1,6 @@
 """GPU parity of the particle operations against the CPU oracle, through the C-ABI:
 tree structure, fill_tree_mean, convolve_pencil / convolve, Richardson-Lucy,
-reconstruct_constant and reconstruct_level.
+reconstruct_constant, reconstruct_level and reconstruct_smooth.
 
 The GPU sums in the reference's order (tree children: particles then tree nodes,
 (z, x, y); convolution taps: (z, x, y)), so every float result is bit-equal to the
@@ -274,6 +274,26 @@ def test_reconstruct_level(oracle, ctx, shape, nb, seed):
     for out_dtype in (np.uint16, np.float32):
         got = A.APRReconstruction.reconstruct_level(g, out_dtype)
         assert got.dtype == out_dtype and np.array_equal(got, expect.astype(out_dtype))
+
+
+@pytest.mark.parametrize("shape,nb,seed", CASES[:3] + [((3, 9, 40), 2, 8)])
+def test_reconstruct_smooth(oracle, ctx, shape, nb, seed):
+    """reconstruct_smooth bit-equal to the oracle: u16 and f32 particles into u16 and f32 images,
+    the default scale and an anisotropic one, and a scale so large that no pixel is smoothed"""
+    img, o, g = _pair(oracle, ctx, shape, nb, seed)
+    pu = o.sample(img)
+    pf = (pu.astype(np.float32) * np.float32(0.37)) + np.float32(1.5)
+    for parts in (pu, pf):
+        for out_dtype in (np.uint16, np.float32):
+            for sc in ((2.0, 2.0, 2.0), (1.0, 0.7, 3.0), (1e6, 1e6, 1e6)):
+                exp = o.reconstruct_smooth(parts, scale=sc, out_dtype=out_dtype)
+                got = A.APRReconstruction.reconstruct_smooth(g, parts, scale_d=sc, out_dtype=out_dtype)
+                assert got.dtype == exp.dtype
+                if out_dtype == np.float32:
+                    ok = np.array_equal(_bits(got), _bits(exp))
+                else:
+                    ok = np.array_equal(got, exp)
+                assert ok, (parts.dtype, out_dtype, sc, int((got != exp).sum()))
 
 
 def _patch_restated(o, values, tree_values, p, out_dtype):

@@ -343,3 +343,76 @@ def test_face_neighbour_filter_closed_forms():
         assert np.all(o.face_neighbour_filter(c, f, dim) == np.float32(3.5))
     p = o.sample(img).astype(np.float32)
     assert np.array_equal(o.face_neighbour_filter(p, np.array([0, 1, 0], np.float32), 1), p)
+
+
+def _sat_windows_f64(img, lev, scale, om_in, d_max, axis):
+    """One adaptive mean pass of reconstruct_smooth written as explicit windows in float64 (the
+    mean of each pixel's window, not the reference's cumulative sums): half-width
+    off = min(floor(2^(d_max - level) / scale), om), om = min(om_in, n // 2 - 1); along y the
+    window is [0, off] at k = 0, [1, k + off] for 1 <= k <= off, [k - off, k + off] inside and
+    [k - off, n - 1] at the high end; along x / z it is [0, i + off] at the low end (i <= off;
+    along z only where the x index is <= off, elsewhere the pixel keeps its value),
+    [i - off, i + off] inside and [i - off, n - 1] at the high end (APRReconstruction.hpp:528-876)"""
+    a = np.moveaxis(img.astype(np.float64), 2 - axis, -1)          # [.., n] along the pass axis
+    L = np.moveaxis(lev, 2 - axis, -1)
+    n = a.shape[-1]
+    om = min(om_in, n // 2 - 1)
+    out = a.copy()
+    offs = np.minimum(np.floor(2.0 ** (d_max - L.astype(np.int64)) / np.float64(np.float32(scale))), om).astype(np.int64)
+    xidx = np.moveaxis(np.broadcast_to(np.arange(img.shape[1])[None, :, None], img.shape), 2 - axis, -1)
+    for idx in np.ndindex(a.shape[:-1]):
+        line, o = a[idx], offs[idx]
+        for i in range(n):
+            f = int(o[i])
+            if axis == 0:
+                if i == 0:
+                    lo, hi = 0, f
+                elif i <= f:
+                    lo, hi = 1, i + f
+                elif i < n - f:
+                    lo, hi = i - f, i + f
+                else:
+                    lo, hi = i - f, n - 1
+            else:
+                if i >= n - f:
+                    lo, hi = i - f, n - 1
+                elif i >= f + 1:
+                    lo, hi = i - f, i + f
+                elif axis == 1 or xidx[idx][i] < f + 1:
+                    lo, hi = 0, i + f
+                else:
+                    continue
+            out[idx + (i,)] = line[lo:hi + 1].mean()
+    return np.moveaxis(out, -1, 2 - axis)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_reconstruct_smooth_pass_windows(axis):
+    """each adaptive pass of the oracle equals its window means (float64, 1e-5 relative); the
+    level image mixes every half-width from 0 to the clamp"""
+    rng = np.random.default_rng(10 + axis)
+    shape = (30, 34, 38)                                   # z, x, y: every axis > 2 * om + 2
+    img = rng.integers(0, 100, shape).astype(np.float32)      # integer samples: the float prefix sums are exact
+    lev = rng.integers(2, 7, shape).astype(np.uint16)     # d_max 6: 2^(6 - l) / 2 = 0.5 .. 8
+    got = img.copy()
+    O.load("oracle").orc_sat_adaptive_f32(got.reshape(-1), lev.reshape(-1), shape[2], shape[1], shape[0], 2.0, 20, 6, axis)
+    exp = _sat_windows_f64(img, lev, 2.0, 20, 6, axis)
+    err = np.abs(got - exp) / np.maximum(np.abs(exp), 1.0)
+    assert err.max() < 1e-5, (axis, float(err.max()))
+    # the clamp: offset_max 3 caps every half-width
+    got = img.copy()
+    O.load("oracle").orc_sat_adaptive_f32(got.reshape(-1), lev.reshape(-1), shape[2], shape[1], shape[0], 0.25, 3, 6, axis)
+    exp = _sat_windows_f64(img, lev, 0.25, 3, 6, axis)
+    assert (np.abs(got - exp) / np.maximum(np.abs(exp), 1.0)).max() < 1e-5
+
+
+def test_reconstruct_smooth_closed_forms():
+    """half-width 0 everywhere (scale far above 2^(l_max - l_min)) leaves the constant
+    reconstruction unchanged; a constant stays constant (float) for any scale"""
+    img, o = make_apr((29, 45, 61), 12, 3, "oracle")
+    p = o.sample(img)
+    assert np.array_equal(o.reconstruct_smooth(p, scale=(1e6, 1e6, 1e6)), o.reconstruct(p))
+    c = np.full(o.total, 7.25, np.float32)
+    for sc in ((2.0, 2.0, 2.0), (1.0, 0.5, 3.0)):
+        out = o.reconstruct_smooth(c, scale=sc)
+        assert np.abs(out - np.float32(7.25)).max() < 1e-5 * 7.25, sc
