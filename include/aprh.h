@@ -161,6 +161,13 @@ int aprh_sample_image_blocked_u16(aprh_context* ctx, const aprh_apr* apr, const 
                                   int64_t ghost_z, uint16_t* parts);
 /* ParticleData<float>::sample_image of a float image (same mean pyramid, float means) */
 int aprh_sample_image_f32(aprh_context* ctx, const aprh_apr* apr, const float* image, float* parts);
+/* aprh_sample_image_u16 that allocates nothing: no counterpart in the reference.  For callers
+   that cannot allocate device memory (stream capture, serving loops, harnesses that forbid
+   hipMalloc).  image and parts must be device memory; the mean pyramid lives in a fixed
+   device array of aprh_noalloc_scratch_bytes() bytes per device (64 MiB: images up to about
+   235M voxels), APRH_ERR_UNSUPPORTED beyond that.  Calls that overlap on one device race. */
+int aprh_sample_image_u16_noalloc(aprh_context* ctx, const aprh_apr* apr, const uint16_t* image, uint16_t* parts);
+size_t aprh_noalloc_scratch_bytes(void);
 
 /* ---- particle operations ---- */
 /* element type of particle buffers (ParticleData<uint16_t> / ParticleData<float>) */

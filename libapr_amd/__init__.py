@@ -116,6 +116,9 @@ def lib():
     L.aprh_apr_particle_cell_tree.argtypes = [vp, C.c_int, vp, C.POINTER(C.c_int64)]
     L.aprh_apr_destroy.argtypes = [vp]
     L.aprh_sample_image_u16.argtypes = [vp, vp, vp, vp]
+    L.aprh_sample_image_u16_noalloc.argtypes = [vp, vp, vp, vp]
+    L.aprh_noalloc_scratch_bytes.restype = C.c_size_t
+    L.aprh_noalloc_scratch_bytes.argtypes = []
     L.aprh_get_apr_by_block_u16.argtypes = [vp, vp, i64, i64, i64, C.POINTER(AprhParams), i64, i64, C.POINTER(vp)]
     L.aprh_sample_image_blocked_u16.argtypes = [vp, vp, vp, i64, i64, vp]
     L.aprh_bspline_filt_rec_u16.argtypes = [vp, vp, i64, i64, i64, C.c_int, C.c_float, C.c_float]

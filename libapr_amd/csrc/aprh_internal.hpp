@@ -299,6 +299,8 @@ void sample_blocked_u16(aprh_context* ctx, const aprh_apr& a, const uint16_t* im
 void launch_sample_u16(aprh_context* ctx, const aprh_apr& a, const uint16_t* img, uint16_t* parts, uint16_t* pyr_ext = nullptr);
 /* bytes of the mean pyramid (levels l_min .. l_max-1) sample_image builds, elements of `elem` bytes */
 size_t sample_pyramid_bytes(const aprh_apr& a, size_t elem);
+/* resolves the device address of the no-alloc scratch array on `device` (current) */
+void noalloc_init(int device);
 /* element offsets of the mean pyramid's levels l_min .. l_max-1 in one buffer; *total elements */
 std::vector<int64_t> pyramid_layout(const aprh_apr& a, int64_t* total);
 /* ParticleData<float>::sample_image of a float image (same pyramid, float means) */

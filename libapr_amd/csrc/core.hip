@@ -354,6 +354,11 @@ int aprh_context_create(int device, aprh_context** out) {
     HIP_CHECK(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) throw Error(APRH_ERR_INVALID_ARGUMENT, "no such device");
     HIP_CHECK(hipSetDevice(device));
+    try {
+        noalloc_init(device);   /* the no-alloc entry point's scratch address on this device */
+    } catch (const Error&) {
+        /* left unresolved: aprh_sample_image_u16_noalloc retries and reports the error */
+    }
     aprh_context* c = new aprh_context();
     c->device = device;
     HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
