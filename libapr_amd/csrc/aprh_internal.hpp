@@ -263,12 +263,15 @@ struct LisParams {
     float var_rescale;
     int pad[3];
     int pitch_align;   /* row pitch of the padded array: yp rounded up to this many floats (0 or 1: yp) */
+    bool fuse_levels = false;   /* launch_lis leaves out the last z mean; the levels launch runs it with its epilogue */
 };
 LisParams lis_params(const aprh_params& par, int64_t yd, int64_t xd, int64_t zd);
 /* full local intensity scale over the reflect-padded array A (see lis_padded_elems) */
 void launch_lis(aprh_context* ctx, const float* intensity, float* A, float* unused, int64_t yd, int64_t xd, int64_t zd,
                 const LisParams& lp);
 size_t lis_padded_elems(int64_t yd, int64_t xd, int64_t zd, const LisParams& lp);
+/* whether the last z mean can run inside the levels launch (LisParams::fuse_levels) */
+bool lis_can_fuse_levels(const LisParams& lp, int64_t yd, int64_t xd, int64_t zd);
 void launch_unpad_rescale(aprh_context* ctx, const float* A, const LisParams& lp, int64_t yd, int64_t xd, int64_t zd, float* out);
 /* constant-scale branch of get_local_intensity_scale, on the device */
 void launch_constant_scale(aprh_context* ctx, float* m, size_t n);

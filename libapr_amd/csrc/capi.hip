@@ -176,6 +176,9 @@ void lis_and_solve(aprh_context* ctx, aprh_apr* a, const aprh_params& par, Buf& 
         Ap = A.as<float>();
     } else {
         A.reset(ctx->pool, lis_padded_elems(yd, xd, zd, lp) * 4);
+        /* the levels launch runs the last z mean (its epilogue also keeps the intermediates) */
+        lp.fuse_levels = lis_can_fuse_levels(lp, yd, xd, zd);
+        lpa = lp;
         launch_lis(ctx, mean.as<float>(), A.as<float>(), nullptr, yd, xd, zd, lp);
         Ap = A.as<float>();
     }

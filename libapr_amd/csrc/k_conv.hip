@@ -116,6 +116,8 @@ struct Lds {
     uint8_t* ylist;   /* sz x TX x 64 particle y offsets of interior rows                        */
     int* cnt;         /* sz x TX                                                                 */
     uint64_t* base;   /* sz x TX first particle index                                            */
+    uint64_t* mlo;    /* 2 WX: first index to load of each window row (particle rows, tree rows) */
+    int* mpre;        /* 2 WX + 1: exclusive prefix of the rows' load counts                     */
 };
 
 /* weights through the constant address space: uniform indices become scalar loads */
@@ -146,6 +148,10 @@ __global__ __launch_bounds__(CT) void k_conv_tile(ConvArgs<TIn> a) {
     uint8_t* p = smem;
     s.base = (uint64_t*)p;
     p += (size_t)sz * TX * 8;
+    s.mlo = (uint64_t*)p;
+    p += (size_t)2 * WX * 8;
+    s.mpre = (int*)p;
+    p += (size_t)(2 * WX + 2) * 4;
     s.cnt = (int*)p;
     p += (size_t)sz * TX * 4;
     s.ylist = p;
@@ -190,59 +196,131 @@ __global__ __launch_bounds__(CT) void k_conv_tile(ConvArgs<TIn> a) {
         const bool zin = zq >= 0 && zq < L.Z;
         const bool plane_zero = !a.reflect && !zin;
         const int64_t zs = a.reflect ? refl_c(zq, L.Z) : zq;
-        /* 1. coarse image (or zero) */
-        for (int xw = wv; xw < WX; xw += NWAVE) {
+        /* 1. row lookups of the window plane, one lane per row (particle rows, then tree rows):
+           the particle range to load and, for interior rows, the block's particles; and the
+           coarse image (or zero) under the whole window.  Every load of this step is
+           independent of the others. */
+        const int nmd = 2 * WX;
+        for (int t = threadIdx.x; t < nmd; t += CT) {
+            const bool trow = t >= WX;
+            const int xw = trow ? t - WX : t;
             const int64_t x = x0 - hx + xw;
             const bool xin = x >= 0 && x < L.X;
-            const int64_t xs = a.reflect ? refl_c(x, L.X) : x;
-            const bool row_zero = plane_zero || (!a.reflect && !xin);
-            float* vrow = V + xw * WYS;
-            const float* crow = (!row_zero && a.coarse) ? a.coarse + ((zs >> 1) * L.cX + (xs >> 1)) * L.cY : nullptr;
-            for (int yw = lane; yw < WY; yw += 64) {
-                const int64_t y = y0 - hy + yw;
-                float v = 0.0f;
-                if (crow && y >= 0 && y < L.Y) v = crow[y >> 1];
-                vrow[yw] = v;
+            const bool interior = !trow && xw >= hx && xw < hx + TX;
+            uint64_t lo = 0;
+            int n = 0;
+            if (!(plane_zero || (!a.reflect && !xin) || (trow && L.t_row_begin < 0))) {
+                const int64_t xs = a.reflect ? refl_c(x, L.X) : x;
+                const int64_t row = zs * L.X + xs;
+                uint64_t b, e, s0, s1;
+                if (!trow) row_block(a.xz_end, a.yidx + L.yidx_off, row, L.nyb, L.row_begin + row, yb0, b, e, s0, s1);
+                else row_block(a.t_xz_end, a.t_yidx + L.t_yidx_off, row, L.nyb, L.t_row_begin + row, yb0, b, e, s0, s1);
+                lo = (s0 - b > (uint64_t)hy) ? s0 - hy : b;
+                const uint64_t hi = (e - s1 > (uint64_t)hy) ? s1 + hy : e;
+                n = (int)(hi - lo);
+                if (interior) {
+                    cnt[xw - hx] = (zin && xin) ? (int)min(s1 - s0, (uint64_t)TY) : 0;
+                    base[xw - hx] = s0;
+                }
+            } else if (interior) {
+                cnt[xw - hx] = 0;
+                base[xw - hx] = 0;
+            }
+            s.mlo[t] = lo;
+            s.mpre[t] = n;
+        }
+        /* the coarse image under the window, cells numbered over (row, y): eight loads per lane
+           in flight before their stores */
+        {
+            const int ncell = WX * WY;
+            for (int c0 = threadIdx.x; c0 < ncell; c0 += 8 * CT) {
+                float cv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int c = c0 + u * CT;
+                    cv[u] = 0.0f;
+                    if (c < ncell && a.coarse && !plane_zero) {
+                        const int xw = c / WY, yw = c - xw * WY;
+                        const int64_t x = x0 - hx + xw, y = y0 - hy + yw;
+                        if ((a.reflect || (x >= 0 && x < L.X)) && y >= 0 && y < L.Y) {
+                            const int64_t xs = a.reflect ? refl_c(x, L.X) : x;
+                            cv[u] = a.coarse[((zs >> 1) * L.cX + (xs >> 1)) * L.cY + (y >> 1)];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int c = c0 + u * CT;
+                    if (c < ncell) V[(c / WY) * WYS + (c % WY)] = cv[u];
+                }
             }
         }
         __syncthreads();
-        /* 2. particles and tree nodes of the level */
-        for (int xw = wv; xw < WX; xw += NWAVE) {
-            const int64_t x = x0 - hx + xw;
-            const bool xin = x >= 0 && x < L.X;
-            const bool interior = xw >= hx && xw < hx + TX;
-            if (plane_zero || (!a.reflect && !xin)) {
-                if (interior && lane == 0) {
-                    cnt[xw - hx] = 0;
-                    base[xw - hx] = 0;
+        /* 2. exclusive prefix of the load counts (wave 0; nmd <= 64 here, else one lane walks) */
+        if (wv == 0) {
+            if (nmd <= 64) {
+                const int c = lane < nmd ? s.mpre[lane] : 0;
+                int inc = c;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int u = __shfl_up(inc, o, 64);
+                    if (lane >= o) inc += u;
                 }
-                continue;
+                if (lane < nmd) s.mpre[lane] = inc - c;
+                if (lane == nmd - 1) s.mpre[nmd] = inc;
+            } else if (lane == 0) {
+                int acc = 0;
+                for (int t = 0; t < nmd; ++t) {
+                    const int c = s.mpre[t];
+                    s.mpre[t] = acc;
+                    acc += c;
+                }
+                s.mpre[nmd] = acc;
             }
-            const int64_t xs = a.reflect ? refl_c(x, L.X) : x;
-            const int64_t row = zs * L.X + xs;
-            float* vrow = V + xw * WYS;
-            uint64_t b, e, s0, s1;
-            row_block(a.xz_end, a.yidx + L.yidx_off, row, L.nyb, L.row_begin + row, yb0, b, e, s0, s1);
-            const uint64_t lo = (s0 - b > (uint64_t)hy) ? s0 - hy : b;
-            const uint64_t hi = (e - s1 > (uint64_t)hy) ? s1 + hy : e;
-            const bool list = interior && zin && xin;
-            for (uint64_t i = lo + lane; i < hi; i += 64) {
-                const int y = a.y_vec[i];
-                const int yw = y - (int)(y0 - hy);
-                if (yw >= 0 && yw < WY) vrow[yw] = tof(a.in[i]);
-                if (list && i >= s0 && i < s1 && i - s0 < (uint64_t)TY) ylist[(xw - hx) * TY + (int)(i - s0)] = (uint8_t)(y - (int)y0);
-            }
-            if (interior && lane == 0) {
-                cnt[xw - hx] = list ? (int)min(s1 - s0, (uint64_t)TY) : 0;
-                base[xw - hx] = s0;
-            }
-            if (L.t_row_begin >= 0) {
-                row_block(a.t_xz_end, a.t_yidx + L.t_yidx_off, row, L.nyb, L.t_row_begin + row, yb0, b, e, s0, s1);
-                const uint64_t tlo = (s0 - b > (uint64_t)hy) ? s0 - hy : b;
-                const uint64_t thi = (e - s1 > (uint64_t)hy) ? s1 + hy : e;
-                for (uint64_t i = tlo + lane; i < thi; i += 64) {
-                    const int yw = (int)a.t_y_vec[i] - (int)(y0 - hy);
-                    if (yw >= 0 && yw < WY) vrow[yw] = a.tree[i];
+        }
+        __syncthreads();
+        /* 3. particles and tree nodes of the level over the coarse image (the two never share a
+           cell), all rows' loads at once: four entries per lane before any is stored */
+        {
+            const int total = s.mpre[nmd];
+            for (int k0 = threadIdx.x; k0 < total; k0 += 4 * CT) {
+                int rr[4];
+                uint64_t ii[4];
+                int yy[4];
+                float vv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int k = k0 + u * CT;
+                    int lo = 0, hi = nmd;   /* last row r with mpre[r] <= k */
+                    while (hi - lo > 1) {
+                        const int mid = (lo + hi) >> 1;
+                        if (s.mpre[mid] <= k) lo = mid;
+                        else hi = mid;
+                    }
+                    rr[u] = k < total ? lo : -1;
+                    ii[u] = k < total ? s.mlo[lo] + (uint64_t)(k - s.mpre[lo]) : 0;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (rr[u] < 0) continue;
+                    if (rr[u] < WX) {
+                        yy[u] = a.y_vec[ii[u]];
+                        vv[u] = tof(a.in[ii[u]]);
+                    } else {
+                        yy[u] = a.t_y_vec[ii[u]];
+                        vv[u] = a.tree[ii[u]];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (rr[u] < 0) continue;
+                    const int xw = rr[u] < WX ? rr[u] : rr[u] - WX;
+                    const int yw = yy[u] - (int)(y0 - hy);
+                    if (yw >= 0 && yw < WY) V[xw * WYS + yw] = vv[u];
+                    if (rr[u] < WX && xw >= hx && xw < hx + TX) {
+                        const uint64_t d = ii[u] - base[xw - hx];
+                        if (d < (uint64_t)cnt[xw - hx]) ylist[(xw - hx) * TY + (int)d] = (uint8_t)(yy[u] - (int)y0);
+                    }
                 }
             }
         }
@@ -361,6 +439,7 @@ size_t conv_lds_bytes(int tx, bool sep, int sy, int sx, int sz) {
     const int WX = tx + (sx - 1), WYS = TY + (sy - 1);
     size_t b = sep ? ((size_t)sz * tx * TY + (size_t)WX * WYS + (size_t)WX * TY) * 4 : (size_t)sz * WX * WYS * 4;
     b += (size_t)sz * tx * (8 + 4 + TY);
+    b += (size_t)2 * WX * 8 + (size_t)(2 * WX + 2) * 4;
     return std::max<size_t>(b, 2 * CT * sizeof(double));
 }
 

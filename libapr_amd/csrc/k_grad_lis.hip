@@ -610,22 +610,14 @@ __device__ __forceinline__ void mean_outer_body(float* A, int64_t y_num, int64_t
     }
 }
 
-/* the same pass with two y per lane (8-byte accesses) and D steps of the line loaded ahead:
-   step j reads sample j and writes output j - O, so every sample is in registers before its
-   output overwrites it */
-template <int O, int D>
-__device__ __forceinline__ void mean_outer2_body(float* A, int64_t y_num, int64_t n, int64_t s, int64_t outer_stride, int64_t lanes,
-                                                 int64_t t) {
+/* the running mean of one pair of lines (two y, 8-byte accesses) along an outer axis of length
+   n (stride sv in float2), D steps loaded ahead; emit(i, mean pair) for every output i.  Step j
+   reads sample j and emits output j - O, so an emit that stores in place never overwrites a
+   sample still to be read */
+template <int O, int D, typename Emit>
+__device__ __forceinline__ void mean_outer2_walk(const float2* base, int64_t n, int64_t sv, Emit emit) {
     constexpr int H = 2 * O + 2;
     constexpr int W = 2 * O + 1;
-    /* lanes are numbered densely over (line, y pair), so rows whose length is not a multiple of
-       the block still fill the blocks; lane counts stay below 2^32 (the launcher checks) */
-    const int64_t lane = t * blockDim.x + threadIdx.x;
-    if (lane >= lanes) return;
-    const uint32_t ny2 = (uint32_t)(y_num / 2);
-    const uint32_t by = (uint32_t)lane / ny2, y2 = (uint32_t)lane - by * ny2;
-    float2* base = (float2*)(A + by * outer_stride + 2 * y2);
-    const int64_t sv = s / 2;
     float ha[H], hb[H];
 #pragma unroll
     for (int i = 0; i < H; ++i) ha[i] = hb[i] = 0.0f;
@@ -652,9 +644,9 @@ __device__ __forceinline__ void mean_outer2_body(float* A, int64_t y_num, int64_
             if (i < 0) continue;
             if (i <= O) {
                 const float dv = (float)(i + O + 1);
-                base[i * sv] = make_float2(ca / dv, cb / dv);
+                emit(i, make_float2(ca / dv, cb / dv));
             } else if (i < n - O) {
-                base[i * sv] = make_float2(div_odd<W>(ca - ha[0]), div_odd<W>(cb - hb[0]));
+                emit(i, make_float2(div_odd<W>(ca - ha[0]), div_odd<W>(cb - hb[0])));
             }
         }
     }
@@ -662,8 +654,23 @@ __device__ __forceinline__ void mean_outer2_body(float* A, int64_t y_num, int64_
     for (int r = 0; r < O; ++r) {
         const int64_t i = n - O + r;
         const float dv = (float)(n - i + O);
-        base[i * sv] = make_float2((ha[H - 1] - ha[1 + r]) / dv, (hb[H - 1] - hb[1 + r]) / dv);
+        emit(i, make_float2((ha[H - 1] - ha[1 + r]) / dv, (hb[H - 1] - hb[1 + r]) / dv));
     }
+}
+
+/* the pass in place over lines of pairs numbered densely over (line, y pair), so rows whose
+   length is not a multiple of the block still fill the blocks; lane counts stay below 2^32 (the
+   launcher checks) */
+template <int O, int D>
+__device__ __forceinline__ void mean_outer2_body(float* A, int64_t y_num, int64_t n, int64_t s, int64_t outer_stride, int64_t lanes,
+                                                 int64_t t) {
+    const int64_t lane = t * blockDim.x + threadIdx.x;
+    if (lane >= lanes) return;
+    const uint32_t ny2 = (uint32_t)(y_num / 2);
+    const uint32_t by = (uint32_t)lane / ny2, y2 = (uint32_t)lane - by * ny2;
+    float2* base = (float2*)(A + by * outer_stride + 2 * y2);
+    const int64_t sv = s / 2;
+    mean_outer2_walk<O, D>(base, n, sv, [&](int64_t i, float2 v) { base[i * sv] = v; });
 }
 
 template <int O>
@@ -771,18 +778,28 @@ struct LevelsArgs {
     FastDiv f_yd, f_xd;
     int i32;
 };
+/* applyParameters and computeLevels of downsampled cell i from its rescaled scale s
+   (threshold_gradient ComputeGradient.hpp:159-171, APRConverter.hpp:249-297,
+   LocalParticleCellSet.hpp:128-171); ipth = Ip_th + bspline offset */
+template <typename G>
+__device__ __forceinline__ void levels_at(const LevelsArgs<G>& a, size_t i, float s, float ipth) {
+    if (a.sigma_raw) a.sigma_raw[i] = s;
+    G gv = a.grad[i];
+    if (a.inten[i] <= ipth) gv = (G)0;
+    if (s < a.sigma_th) s = (s < a.sigma_th_max) ? 60000.0f : a.sigma_th;
+    if ((float)gv < a.grad_th) gv = (G)0;
+    if (a.sigma_out) a.sigma_out[i] = s;
+    if (a.grad_keep) a.grad_keep[i] = gv;
+    const float L = (float)gv / s;
+    const uint32_t u = f2u32(L * a.mult);
+    a.level[i] = (uint8_t)(u == 0u ? 0 : 31 - __clz(u));
+}
+
 template <typename G>
 __device__ __forceinline__ void levels_body(const LevelsArgs<G>& a, int64_t t) {
     const float* __restrict__ A = a.A;
     const PadGeom& g = a.g;
     const float var_rescale = a.var_rescale;
-    const float* __restrict__ inten = a.inten;
-    G* grad = a.grad;
-    float* sigma_raw = a.sigma_raw;
-    float* sigma_out = a.sigma_out;
-    uint8_t* level = a.level;
-    const float sigma_th = a.sigma_th, sigma_th_max = a.sigma_th_max, grad_th = a.grad_th, mult = a.mult;
-    G* grad_keep = a.grad_keep;
     const float ipth = a.Ip_th + *a.d_off;
     const size_t n = (size_t)(g.yd * g.xd * g.zd);
     for (size_t i = (size_t)t * blockDim.x + threadIdx.x; i < n; i += (size_t)a.Gs * blockDim.x) {
@@ -799,19 +816,35 @@ __device__ __forceinline__ void levels_body(const LevelsArgs<G>& a, int64_t t) {
             z = r / g.xd;
         }
         /* rescale_var + unpad (LocalIntensityScale.hpp:173-184, PixelData.hpp:1189-1210) */
-        float s = A[((z + g.pz) * g.xp + (x + g.px)) * g.yq + (y + g.py)] * var_rescale;
-        if (sigma_raw) sigma_raw[i] = s;
-        G gv = grad[i];
-        /* threshold_gradient (ComputeGradient.hpp:159-171) */
-        if (inten[i] <= ipth) gv = (G)0;
-        if (s < sigma_th) s = (s < sigma_th_max) ? 60000.0f : sigma_th;
-        if ((float)gv < grad_th) gv = (G)0;
-        if (sigma_out) sigma_out[i] = s;
-        if (grad_keep) grad_keep[i] = gv;
-        const float L = (float)gv / s;
-        const uint32_t u = f2u32(L * mult);
-        level[i] = (uint8_t)(u == 0u ? 0 : 31 - __clz(u));
+        levels_at(a, i, A[((z + g.pz) * g.xp + (x + g.px)) * g.yq + (y + g.py)] * var_rescale, ipth);
     }
+}
+
+/* the local intensity scale's last pass (the z mean of offset O, LocalIntensityScale.hpp:434-494)
+   with the levels epilogue: lanes (x column, y pair) of the unpadded cells walk the padded z
+   line, and each unpadded output, rescaled, goes to levels_at instead of back to A.  Lanes are
+   numbered densely over (x, pair); pairs start at the even y at or below the first unpadded y,
+   and the half of a pair outside the unpadded range is not used. */
+template <int O, int D, typename G>
+__device__ __forceinline__ void mean_z_levels_body(const LevelsArgs<G>& a, int64_t p0, int64_t npair, int64_t lanes, int64_t t) {
+    const int64_t lane = t * blockDim.x + threadIdx.x;
+    if (lane >= lanes) return;
+    const PadGeom& g = a.g;
+    const uint32_t xi = (uint32_t)lane / (uint32_t)npair, pi = (uint32_t)lane - xi * (uint32_t)npair;
+    const int64_t yp0 = 2 * (p0 + pi);              /* padded y of the pair's first cell */
+    const float2* base = (const float2*)(a.A + (xi + g.px) * g.yq + yp0);
+    const int64_t sv = g.xp * g.yq / 2;
+    const float ipth = a.Ip_th + *a.d_off;
+    const float vr = a.var_rescale;
+    const int64_t y0 = yp0 - g.py;                  /* unpadded y of the first cell (may be -1) */
+    const bool in0 = y0 >= 0 && y0 < g.yd, in1 = y0 + 1 >= 0 && y0 + 1 < g.yd;
+    mean_outer2_walk<O, D>(base, g.zp, sv, [&](int64_t i, float2 v) {
+        const int64_t z = i - g.pz;
+        if (z < 0 || z >= g.zd) return;
+        const size_t row = (size_t)((z * g.xd + xi) * g.yd);
+        if (in0) levels_at(a, row + (size_t)y0, v.x * vr, ipth);
+        if (in1) levels_at(a, row + (size_t)(y0 + 1), v.y * vr, ipth);
+    });
 }
 
 /* rescale_var + unpaddPixels (LocalIntensityScale.hpp:173-184, PixelData.hpp:1189-1210) */
@@ -881,6 +914,16 @@ struct GradDsVec {
         grad_ds_vec_body<UNIT>((const uint16_t*)a.img, (uint16_t*)a.grad, a.mean, a.y_num, a.x_num, a.z_num, a.yd_num, a.h2x, a.h2y,
                                a.h2z, a.accum, yg, r % a.xd_num, r / a.xd_num, a.xd_num, a.zd_num);
     }
+};
+template <int O, typename G>
+struct MeanZLevels {
+    struct Args {
+        LevelsArgs<G> la;
+        int64_t p0, npair, lanes;
+    };
+    static constexpr int THREADS = 256;
+    static const char* name() { return "lis_mean_z_levels"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) { mean_z_levels_body<O, 8, G>(a.la, a.p0, a.npair, a.lanes, t); }
 };
 template <typename G>
 struct Levels {
@@ -1029,7 +1072,16 @@ void launch_lis(aprh_context* ctx, const float* intensity, float* sigma_padded, 
     /* |mean - intensity| then the second mean (windows win[3..5]) */
     mean_y_dispatch(ctx, lp.active[0] ? lp.win[3] : 0, 1, intensity, A, g);
     if (lp.active[1]) mx(lp.win[4]);
-    if (lp.active[2]) mz(lp.win[5]);
+    /* with fuse_levels the last z mean runs in the levels launch (launch_levels_padded) */
+    if (lp.active[2] && !lp.fuse_levels) mz(lp.win[5]);
+}
+
+bool lis_can_fuse_levels(const LisParams& lp, int64_t yd, int64_t xd, int64_t zd) {
+    if (!lp.active[2] || lp.win[5] < 1 || lp.win[5] > 6) return false;
+    PadGeom g = pad_geom(yd, xd, zd, lp);
+    if (g.yq % 2 != 0 || g.zp < 2 * lp.win[5] + 2) return false;
+    const int64_t npair = (g.py + g.yd + 1) / 2 - g.py / 2;
+    return npair * g.xd < (int64_t(1) << 32);
 }
 
 template <typename G>
@@ -1039,10 +1091,24 @@ void launch_levels_padded(aprh_context* ctx, const float* A, const LisParams& lp
     PadGeom g = pad_geom(yd, xd, zd, lp);
     const int64_t Gs = stride_tiles(yd * xd * zd, 256 * 4);
     const int i32 = fits32(yd * xd * zd) ? 1 : 0;
-    launch_tiles<Levels<G>>(ctx, LevelsArgs<G>{A, g, lp.var_rescale, inten, grad, sigma_raw, sigma_out, level, d_offset, par.Ip_th,
-                                               par.sigma_th, par.sigma_th_max, par.grad_th, mult, grad_keep, Gs,
-                                               i32 ? FastDiv((uint64_t)yd) : FastDiv(), i32 ? FastDiv((uint64_t)xd) : FastDiv(), i32},
-                            Gs);
+    const LevelsArgs<G> la{A, g, lp.var_rescale, inten, grad, sigma_raw, sigma_out, level, d_offset, par.Ip_th,
+                           par.sigma_th, par.sigma_th_max, par.grad_th, mult, grad_keep, Gs,
+                           i32 ? FastDiv((uint64_t)yd) : FastDiv(), i32 ? FastDiv((uint64_t)xd) : FastDiv(), i32};
+    if (lp.fuse_levels) {
+        /* the z mean of the second window (left out by launch_lis) with the levels epilogue */
+        const int64_t p0 = g.py / 2, npair = (g.py + g.yd + 1) / 2 - p0, lanes = npair * g.xd;
+        switch (lp.win[5]) {
+            case 1: launch_tiles<MeanZLevels<1, G>>(ctx, {la, p0, npair, lanes}, cdiv(lanes, 256)); break;
+            case 2: launch_tiles<MeanZLevels<2, G>>(ctx, {la, p0, npair, lanes}, cdiv(lanes, 256)); break;
+            case 3: launch_tiles<MeanZLevels<3, G>>(ctx, {la, p0, npair, lanes}, cdiv(lanes, 256)); break;
+            case 4: launch_tiles<MeanZLevels<4, G>>(ctx, {la, p0, npair, lanes}, cdiv(lanes, 256)); break;
+            case 5: launch_tiles<MeanZLevels<5, G>>(ctx, {la, p0, npair, lanes}, cdiv(lanes, 256)); break;
+            case 6: launch_tiles<MeanZLevels<6, G>>(ctx, {la, p0, npair, lanes}, cdiv(lanes, 256)); break;
+            default: throw Error(APRH_ERR_UNSUPPORTED, "mean window offset must be 1..6");
+        }
+        return;
+    }
+    launch_tiles<Levels<G>>(ctx, la, Gs);
 }
 
 template void launch_levels_padded<uint16_t>(aprh_context*, const float*, const LisParams&, int64_t, int64_t, int64_t, const float*,
