@@ -372,12 +372,34 @@ __global__ __launch_bounds__(CT) void k_conv_tile(ConvArgs<TIn> a) {
         if (q >= 2 * hz) {
             const int64_t qo = q - 2 * hz;           /* window plane of z = zo - hz */
             const int oslot = (int)((qo + hz) % sz); /* plane zo itself             */
-            for (int r = wv; r < TX; r += NWAVE) {
-                const int c = __builtin_amdgcn_readfirstlane(s.cnt[oslot * TX + r]);
-                if (c == 0) continue;
-                const uint64_t pb = s.base[oslot * TX + r];
-                const bool act = lane < c;
-                const int yo = act ? (int)s.ylist[((size_t)oslot * TX + r) * TY + lane] : 0;
+            /* the plane's particles numbered over the tile's rows (row r holds cnt[r]), one lane
+               each; SEP keeps a wave per row */
+            int ntot = 0;
+            if (!SEP)
+                for (int r = 0; r < TX; ++r) ntot += s.cnt[oslot * TX + r];
+            for (int it = SEP ? wv : threadIdx.x; SEP ? it < TX : it - lane < ntot; it += SEP ? NWAVE : CT) {
+                int r, d;
+                bool act;
+                if (SEP) {
+                    r = it;
+                    d = lane;
+                    act = lane < s.cnt[oslot * TX + r];
+                } else {
+                    /* row of particle it: the last r with (sum of cnt below r) <= it */
+                    int k = it, rr = 0;
+                    act = it < ntot;
+                    if (act)
+                        for (; rr < TX - 1; ++rr) {
+                            const int c = s.cnt[oslot * TX + rr];
+                            if (k < c) break;
+                            k -= c;
+                        }
+                    r = rr;
+                    d = act ? k : 0;
+                }
+                if (SEP && __builtin_amdgcn_readfirstlane(s.cnt[oslot * TX + r]) == 0) continue;
+                const uint64_t pb = s.base[oslot * TX + r] + (uint64_t)d;
+                const int yo = act ? (int)s.ylist[((size_t)oslot * TX + r) * TY + d] : 0;
                 float val = 0.0f;
                 if (SEP) {
                     const CWeights fz = Wt + L.sy + L.sx;
@@ -396,7 +418,7 @@ __global__ __launch_bounds__(CT) void k_conv_tile(ConvArgs<TIn> a) {
                     }
                 }
                 if (act) {
-                    const uint64_t i = pb + lane;
+                    const uint64_t i = pb;
                     if (a.epi == 0) {
                         a.out[i] = val;
                     } else if (a.epi == 1) {
@@ -433,6 +455,307 @@ __global__ __launch_bounds__(CT) void k_conv_tile(ConvArgs<TIn> a) {
             a.part[2 * (int64_t)blockIdx.x + 1] = red[CT];
         }
     }
+}
+
+/* Separable mode with the z pass in registers.  Each thread owns one y (its lane) in TX / 4
+   rows of the tile and keeps the x-passed values of the last SZ planes of those cells in
+   registers; a per-plane map from interior cells to their particle (offset in the row's block,
+   0xFF for none) sits in an LDS ring of SZ + 1 planes, so the tile needs no LDS ring of planes
+   and more blocks fit on a CU.  Per plane: assemble the window (as k_conv_tile), y pass into A,
+   x pass into the registers, and the outputs of plane z - hz at the cells that hold particles. */
+template <typename TIn, int TX, int SZ>
+__global__ __launch_bounds__(CT) void k_conv_sep(ConvArgs<TIn> a) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    constexpr int SZR = SZ + 1;
+    constexpr int HZ = (SZ - 1) / 2;
+    constexpr int RPT = TX / NWAVE;
+    const LevelDesc& L = a.L;
+    constexpr int hy = HZ, hx = HZ;
+    constexpr int WX = TX + 2 * hx, WY = TY + 2 * hy, WYS = WY;
+    uint8_t* p = smem;
+    uint64_t* sbase = (uint64_t*)p;      /* SZR x TX: first particle of each interior row's block */
+    p += (size_t)SZR * TX * 8;
+    uint64_t* mlo = (uint64_t*)p;        /* 2 WX */
+    p += (size_t)2 * WX * 8;
+    int* mpre = (int*)p;                 /* 2 WX + 2 */
+    p += (size_t)(2 * WX + 2) * 4;
+    float* V = (float*)p;                /* WX x WYS */
+    p += (size_t)WX * WYS * 4;
+    float* Ab = (float*)p;               /* WX x TY */
+    p += (size_t)WX * TY * 4;
+    uint8_t* pidx = p;                   /* SZR x TX x TY */
+
+    const int64_t ntiles = (int64_t)L.ntx * L.nty * L.ntz;
+    const int64_t tile = xcd_block(blockIdx.x, gridDim.x);
+    if (tile >= ntiles) return;
+    const int tx = (int)(tile % L.ntx);
+    const int ty = (int)((tile / L.ntx) % L.nty);
+    const int tz = (int)(tile / ((int64_t)L.ntx * L.nty));
+    const int64_t x0 = (int64_t)tx * TX, y0 = (int64_t)ty * TY;
+    const int yb0 = ty;
+    const int64_t z_lo = (int64_t)tz * L.zc, z_hi = min(z_lo + L.zc, L.Z);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    float fy[SZ], fx[SZ], fz[SZ];
+    {
+        const CWeights w = (CWeights)(a.W + L.w_off);
+#pragma unroll
+        for (int k = 0; k < SZ; ++k) {
+            fy[k] = w[k];
+            fx[k] = w[SZ + k];
+            fz[k] = w[2 * SZ + k];
+        }
+    }
+    constexpr int nmd = 2 * WX;
+
+    double n1 = 0.0, n2 = 0.0;
+    float breg[RPT][SZ];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j)
+#pragma unroll
+        for (int k = 0; k < SZ; ++k) breg[j][k] = 0.0f;
+
+    const int64_t nq = (z_hi - z_lo) + 2 * HZ;
+    for (int64_t q = 0; q < nq; ++q) {
+        const int64_t zq = z_lo - HZ + q;
+        const int slot = (int)(q % SZR);
+        uint8_t* pid = pidx + (size_t)slot * TX * TY;
+        uint64_t* bs = sbase + (size_t)slot * TX;
+        const bool zin = zq >= 0 && zq < L.Z;
+        const bool plane_zero = !a.reflect && !zin;
+        const int64_t zs = a.reflect ? refl_c(zq, L.Z) : zq;
+        /* 1. wave 0: row lookups and the exclusive prefix of the load counts; every thread:
+           clear this plane's particle map; all waves: the coarse image under the window */
+        for (int c = threadIdx.x * 8; c < TX * TY; c += CT * 8) *(uint64_t*)(pid + c) = ~(uint64_t)0;
+        if (wv == 0) {
+            int carry = 0;
+            for (int t0 = 0; t0 < nmd; t0 += 64) {
+                const int t = t0 + lane;
+                uint64_t lo = 0;
+                int n = 0;
+                if (t < nmd) {
+                    const bool trow = t >= WX;
+                    const int xw = trow ? t - WX : t;
+                    const int64_t x = x0 - hx + xw;
+                    const bool xin = x >= 0 && x < L.X;
+                    const bool interior = !trow && xw >= hx && xw < hx + TX;
+                    if (!(plane_zero || (!a.reflect && !xin) || (trow && L.t_row_begin < 0))) {
+                        const int64_t xs = a.reflect ? refl_c(x, L.X) : x;
+                        const int64_t row = zs * L.X + xs;
+                        uint64_t b, e, s0, s1;
+                        if (!trow) row_block(a.xz_end, a.yidx + L.yidx_off, row, L.nyb, L.row_begin + row, yb0, b, e, s0, s1);
+                        else row_block(a.t_xz_end, a.t_yidx + L.t_yidx_off, row, L.nyb, L.t_row_begin + row, yb0, b, e, s0, s1);
+                        lo = (s0 - b > (uint64_t)hy) ? s0 - hy : b;
+                        const uint64_t hi = (e - s1 > (uint64_t)hy) ? s1 + hy : e;
+                        n = (int)(hi - lo);
+                        /* interior rows of planes inside the level list their block's particles */
+                        if (interior) bs[xw - hx] = (zin && xin) ? s0 : ~(uint64_t)0;
+                    } else if (interior) {
+                        bs[xw - hx] = ~(uint64_t)0;
+                    }
+                    mlo[t] = lo;
+                }
+                int inc = n;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int u = __shfl_up(inc, o, 64);
+                    if (lane >= o) inc += u;
+                }
+                if (t < nmd) mpre[t] = carry + inc - n;
+                carry += __shfl(inc, 63, 64);
+            }
+            if (lane == 0) mpre[nmd] = carry;
+        }
+        /* the coarse image under the window: a wave per window row, lanes along y (a second
+           pass for the y halo beyond 64 cells), four rows' loads in flight before their stores */
+        for (int xb = wv; xb < WX; xb += 4 * NWAVE) {
+            float cv[4][2];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int xw = xb + u * NWAVE;
+                const int64_t x = x0 - hx + xw;
+                const bool rin = xw < WX && !plane_zero && a.coarse && (a.reflect || (x >= 0 && x < L.X));
+                const float* crow = rin ? a.coarse + ((zs >> 1) * L.cX + ((a.reflect ? refl_c(x, L.X) : x) >> 1)) * L.cY : nullptr;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int yw = lane + 64 * h;
+                    const int64_t y = y0 - hy + yw;
+                    cv[u][h] = (crow && yw < WY && y >= 0 && y < L.Y) ? crow[y >> 1] : 0.0f;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int xw = xb + u * NWAVE;
+                if (xw >= WX) break;
+                V[xw * WYS + lane] = cv[u][0];
+                if (lane + 64 < WY) V[xw * WYS + lane + 64] = cv[u][1];
+            }
+        }
+        __syncthreads();
+        /* 2. particles and tree nodes of the level over the coarse image; interior particles of
+           planes inside the level enter the plane's particle map */
+        {
+            const int total = mpre[nmd];
+            for (int k0 = threadIdx.x; k0 < total; k0 += 4 * CT) {
+                int rr[4];
+                uint64_t ii[4];
+                int yy[4];
+                float vv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int k = k0 + u * CT;
+                    int lo = 0, hi = nmd;
+                    while (hi - lo > 1) {
+                        const int mid = (lo + hi) >> 1;
+                        if (mpre[mid] <= k) lo = mid;
+                        else hi = mid;
+                    }
+                    rr[u] = k < total ? lo : -1;
+                    ii[u] = k < total ? mlo[lo] + (uint64_t)(k - mpre[lo]) : 0;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (rr[u] < 0) continue;
+                    if (rr[u] < WX) {
+                        yy[u] = a.y_vec[ii[u]];
+                        vv[u] = tof(a.in[ii[u]]);
+                    } else {
+                        yy[u] = a.t_y_vec[ii[u]];
+                        vv[u] = a.tree[ii[u]];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (rr[u] < 0) continue;
+                    const int xw = rr[u] < WX ? rr[u] : rr[u] - WX;
+                    const int yw = yy[u] - (int)(y0 - hy);
+                    if (yw >= 0 && yw < WY) V[xw * WYS + yw] = vv[u];
+                    if (rr[u] < WX && xw >= hx && xw < hx + TX) {
+                        const uint64_t b0 = bs[xw - hx];
+                        const uint64_t d = ii[u] - b0;
+                        const int yo = yy[u] - (int)y0;
+                        if (b0 != ~(uint64_t)0 && d < (uint64_t)TY && yo >= 0 && yo < TY) pid[(xw - hx) * TY + yo] = (uint8_t)d;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        /* 3. y ends: mirror inside the window */
+        if (a.reflect && (y0 - hy < 0 || y0 + TY + hy > L.Y)) {
+            for (int xw = wv; xw < WX; xw += NWAVE) {
+                float* vrow = V + xw * WYS;
+                for (int yw = lane; yw < WY; yw += 64) {
+                    const int64_t y = y0 - hy + yw;
+                    if ((y < 0 || y >= L.Y) && y < L.Y + hy) {
+                        const int64_t src = refl_c(y, L.Y) - (y0 - hy);
+                        if (src >= 0 && src < WY) vrow[yw] = vrow[src];
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        /* the assembled interior is the next level's coarse image */
+        if (a.vout && zq >= z_lo && zq < z_hi) {
+            for (int r = wv; r < TX; r += NWAVE) {
+                const int64_t x = x0 + r;
+                const int64_t y = y0 + lane;
+                if (x < L.X && y < L.Y) a.vout[(zq * L.X + x) * L.Y + y] = V[(r + hx) * WYS + hy + lane];
+            }
+        }
+        /* 4. y pass of every window row */
+        for (int xw = wv; xw < WX; xw += NWAVE) {
+            const float* vrow = V + xw * WYS + lane;
+            float acc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < SZ; ++k) acc = fmaf(fy[k], vrow[k], acc);
+            Ab[xw * TY + lane] = acc;
+        }
+        __syncthreads();
+        /* 5. x pass of the thread's rows into the register ring; then the outputs of plane
+           zq - HZ, whose z neighbourhood is now in the ring */
+        const bool emit = q >= 2 * HZ;
+        const int oslot = (int)((q - HZ + SZR) % SZR);
+        const uint8_t* opid = pidx + (size_t)oslot * TX * TY;
+        const uint64_t* obs = sbase + (size_t)oslot * TX;
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const int r = wv + j * NWAVE;
+            float acc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < SZ; ++k) acc = fmaf(fx[k], Ab[(r + k) * TY + lane], acc);
+#pragma unroll
+            for (int k = 0; k + 1 < SZ; ++k) breg[j][k] = breg[j][k + 1];
+            breg[j][SZ - 1] = acc;
+        }
+        if (emit) {
+            uint64_t oi[RPT];
+            float ov[RPT];
+            bool on[RPT];
+#pragma unroll
+            for (int j = 0; j < RPT; ++j) {
+                const int r = wv + j * NWAVE;
+                const int d = opid[r * TY + lane];
+                on[j] = d != 0xFF;
+                oi[j] = on[j] ? obs[r] + (uint64_t)d : 0;
+                float val = 0.0f;
+                if (on[j]) {
+#pragma unroll
+                    for (int k = 0; k < SZ; ++k) val = fmaf(fz[k], breg[j][k], val);
+                }
+                ov[j] = val;
+            }
+            if (a.epi == 0) {
+#pragma unroll
+                for (int j = 0; j < RPT; ++j)
+                    if (on[j]) a.out[oi[j]] = ov[j];
+            } else if (a.epi == 1) {
+                float num[RPT];
+#pragma unroll
+                for (int j = 0; j < RPT; ++j)
+                    num[j] = on[j] ? (a.epi_in_u16 ? (float)((const uint16_t*)a.epi_in)[oi[j]] : ((const float*)a.epi_in)[oi[j]]) : 1.0f;
+#pragma unroll
+                for (int j = 0; j < RPT; ++j)
+                    if (on[j]) a.out[oi[j]] = num[j] / ov[j];
+            } else {
+                float uo[RPT];
+#pragma unroll
+                for (int j = 0; j < RPT; ++j) uo[j] = on[j] ? a.out[oi[j]] : 0.0f;
+#pragma unroll
+                for (int j = 0; j < RPT; ++j)
+                    if (on[j]) {
+                        const float nu = uo[j] * ov[j];
+                        const double d = (double)nu - (double)uo[j];
+                        n1 += d * d;
+                        n2 += (double)uo[j] * (double)uo[j];
+                        a.out[oi[j]] = nu;
+                    }
+            }
+        }
+    }
+    if (a.epi == 2) {
+        __syncthreads();
+        double* red = (double*)smem;
+        red[threadIdx.x] = n1;
+        red[CT + threadIdx.x] = n2;
+        __syncthreads();
+        for (int o = CT / 2; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) {
+                red[threadIdx.x] += red[threadIdx.x + o];
+                red[CT + threadIdx.x] += red[CT + threadIdx.x + o];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            a.part[2 * (int64_t)blockIdx.x] = red[0];
+            a.part[2 * (int64_t)blockIdx.x + 1] = red[CT];
+        }
+    }
+}
+
+size_t conv_sep_lds_bytes(int tx, int sy, int sx, int sz) {
+    const int WX = tx + (sx - 1), WYS = TY + (sy - 1), SZR = sz + 1;
+    size_t b = (size_t)SZR * tx * 8 + (size_t)2 * WX * 8 + (size_t)(2 * WX + 2) * 4 + (size_t)WX * WYS * 4 + (size_t)WX * TY * 4 +
+               (size_t)SZR * tx * TY;
+    return std::max<size_t>(b, 2 * CT * sizeof(double));
 }
 
 size_t conv_lds_bytes(int tx, bool sep, int sy, int sx, int sz) {
@@ -620,6 +943,31 @@ void launch_conv_tile(const ConvArgs<TIn>& args, unsigned grid, size_t lds, hipS
     HIP_LAUNCH_CHECK();
 }
 
+template <typename TIn, int TX, int SZ>
+void launch_conv_sep_tile(const ConvArgs<TIn>& args, unsigned grid, size_t lds, hipStream_t st) {
+    if (lds > 65536)
+        (void)hipFuncSetAttribute((const void*)k_conv_sep<TIn, TX, SZ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((k_conv_sep<TIn, TX, SZ>), dim3(grid), dim3(CT), lds, st, args);
+    HIP_LAUNCH_CHECK();
+}
+
+template <typename TIn, int TX>
+void launch_conv_sep_sz(const ConvArgs<TIn>& ca, unsigned grid, size_t lds, hipStream_t st) {
+    switch (ca.L.sz) {
+        case 1: launch_conv_sep_tile<TIn, TX, 1>(ca, grid, lds, st); return;
+        case 3: launch_conv_sep_tile<TIn, TX, 3>(ca, grid, lds, st); return;
+        case 5: launch_conv_sep_tile<TIn, TX, 5>(ca, grid, lds, st); return;
+        case 7: launch_conv_sep_tile<TIn, TX, 7>(ca, grid, lds, st); return;
+        case 9: launch_conv_sep_tile<TIn, TX, 9>(ca, grid, lds, st); return;
+        case 11: launch_conv_sep_tile<TIn, TX, 11>(ca, grid, lds, st); return;
+        case 13: launch_conv_sep_tile<TIn, TX, 13>(ca, grid, lds, st); return;
+        default: throw Error(APRH_ERR_UNSUPPORTED, "register z ring: stencil depth not instantiated");
+    }
+}
+
+/* z sizes with a register-ring separable kernel */
+bool sep_regz_size(int sz) { return sz >= 1 && sz <= 13 && (sz & 1); }
+
 /* the kernel for one level: separable or reference order, tile width, unrolled y taps */
 template <typename TIn>
 void launch_conv_level(const ConvArgs<TIn>& ca, int tx, bool sep, unsigned grid, size_t lds, hipStream_t st) {
@@ -679,7 +1027,7 @@ void conv_levels(aprh_context* ctx, aprh_apr& a, const std::vector<HostStencil>&
     struct Plan {
         LevelDesc L;
         int tx;
-        bool sep;
+        bool sep, regz;
         size_t lds;
         unsigned grid;
     };
@@ -706,20 +1054,37 @@ void conv_levels(aprh_context* ctx, aprh_apr& a, const std::vector<HostStencil>&
         L.sz = s.sz;
         pl.sep = off_sep[(size_t)sn] >= 0;
         L.w_off = pl.sep ? off_sep[(size_t)sn] : off_exact[(size_t)sn];
-        pl.tx = 16;
-        pl.lds = conv_lds_bytes(16, pl.sep, s.sy, s.sx, s.sz);
-        if (pl.lds > 96 * 1024) {
-            pl.tx = 8;
-            pl.lds = conv_lds_bytes(8, pl.sep, s.sy, s.sx, s.sz);
+        pl.regz = pl.sep && sep_regz_size(s.sz) && s.sy == s.sz && s.sx == s.sz;
+        if (pl.regz) {
+            /* 16-row tiles: more blocks per CU hide the plane's global reads better than the
+               smaller x halo of 32-row tiles saves (measured on the 9x9x9 RL workload) */
+            pl.tx = 16;
+            pl.lds = conv_sep_lds_bytes(pl.tx, s.sy, s.sx, s.sz);
+        } else {
+            pl.tx = 16;
+            pl.lds = conv_lds_bytes(16, pl.sep, s.sy, s.sx, s.sz);
+            if (pl.lds > 96 * 1024) {
+                pl.tx = 8;
+                pl.lds = conv_lds_bytes(8, pl.sep, s.sy, s.sx, s.sz);
+            }
         }
         if (pl.lds > 160 * 1024) throw Error(APRH_ERR_UNSUPPORTED, "stencil too large for the tile convolution");
         L.ntx = (int)cdiv(L.X, pl.tx);
         L.nty = L.nyb;
         const int64_t nxy = (int64_t)L.ntx * L.nty;
         const int hz = (s.sz - 1) / 2;
-        int64_t zc = cdiv(L.Z * nxy, 8192);
-        zc = std::max<int64_t>(zc, std::min<int64_t>(L.Z, std::max(8, 4 * hz)));
-        zc = std::min<int64_t>(zc, L.Z);
+        int64_t zc;
+        if (pl.regz) {
+            /* about 4096 blocks; runs of at least 4 hz planes unless that leaves fewer than ~1024 */
+            zc = cdiv(L.Z * nxy, 4096);
+            const int64_t zmin = std::min<int64_t>(L.Z, std::max(8, 4 * hz));
+            if (nxy * cdiv(L.Z, zmin) >= 1024) zc = std::max(zc, zmin);
+            zc = std::max<int64_t>(1, std::min<int64_t>(zc, L.Z));
+        } else {
+            zc = cdiv(L.Z * nxy, 8192);
+            zc = std::max<int64_t>(zc, std::min<int64_t>(L.Z, std::max(8, 4 * hz)));
+            zc = std::min<int64_t>(zc, L.Z);
+        }
         L.zc = (int)zc;
         L.ntz = (int)cdiv(L.Z, zc);
         const int64_t g = nxy * L.ntz;
@@ -764,6 +1129,11 @@ void conv_levels(aprh_context* ctx, aprh_apr& a, const std::vector<HostStencil>&
         ca.epi_in_u16 = epi_u16 ? 1 : 0;
         boff += pl.grid;
         if (pl.grid == 0) continue;
+        if (pl.regz) {
+            if (pl.tx == 32) launch_conv_sep_sz<T, 32>(ca, pl.grid, pl.lds, st);
+            else launch_conv_sep_sz<T, 16>(ca, pl.grid, pl.lds, st);
+            continue;
+        }
         launch_conv_level<T>(ca, pl.tx, pl.sep, pl.grid, pl.lds, st);
     }
 }
