@@ -463,7 +463,7 @@ __global__ __launch_bounds__(CT) void k_conv_tile(ConvArgs<TIn> a) {
    0xFF for none) sits in an LDS ring of SZ + 1 planes, so the tile needs no LDS ring of planes
    and more blocks fit on a CU.  Per plane: assemble the window (as k_conv_tile), y pass into A,
    x pass into the registers, and the outputs of plane z - hz at the cells that hold particles. */
-template <typename TIn, int TX, int SZ>
+template <typename TIn, int TX, int SZ, int EPI, bool TOP>
 __global__ __launch_bounds__(CT) void k_conv_sep(ConvArgs<TIn> a) {
     extern __shared__ __align__(16) uint8_t smem[];
     constexpr int SZR = SZ + 1;
@@ -505,7 +505,7 @@ __global__ __launch_bounds__(CT) void k_conv_sep(ConvArgs<TIn> a) {
             fz[k] = w[2 * SZ + k];
         }
     }
-    constexpr int nmd = 2 * WX;
+    constexpr int nmd = TOP ? WX : 2 * WX;   /* window rows looked up: particle rows, tree rows */
 
     double n1 = 0.0, n2 = 0.0;
     float breg[RPT][SZ];
@@ -538,11 +538,11 @@ __global__ __launch_bounds__(CT) void k_conv_sep(ConvArgs<TIn> a) {
                     const int64_t x = x0 - hx + xw;
                     const bool xin = x >= 0 && x < L.X;
                     const bool interior = !trow && xw >= hx && xw < hx + TX;
-                    if (!(plane_zero || (!a.reflect && !xin) || (trow && L.t_row_begin < 0))) {
+                    if (!(plane_zero || (!a.reflect && !xin) || (trow && (TOP || L.t_row_begin < 0)))) {
                         const int64_t xs = a.reflect ? refl_c(x, L.X) : x;
                         const int64_t row = zs * L.X + xs;
                         uint64_t b, e, s0, s1;
-                        if (!trow) row_block(a.xz_end, a.yidx + L.yidx_off, row, L.nyb, L.row_begin + row, yb0, b, e, s0, s1);
+                        if (TOP || !trow) row_block(a.xz_end, a.yidx + L.yidx_off, row, L.nyb, L.row_begin + row, yb0, b, e, s0, s1);
                         else row_block(a.t_xz_end, a.t_yidx + L.t_yidx_off, row, L.nyb, L.t_row_begin + row, yb0, b, e, s0, s1);
                         lo = (s0 - b > (uint64_t)hy) ? s0 - hy : b;
                         const uint64_t hi = (e - s1 > (uint64_t)hy) ? s1 + hy : e;
@@ -615,7 +615,7 @@ __global__ __launch_bounds__(CT) void k_conv_sep(ConvArgs<TIn> a) {
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     if (rr[u] < 0) continue;
-                    if (rr[u] < WX) {
+                    if (TOP || rr[u] < WX) {
                         yy[u] = a.y_vec[ii[u]];
                         vv[u] = tof(a.in[ii[u]]);
                     } else {
@@ -654,7 +654,7 @@ __global__ __launch_bounds__(CT) void k_conv_sep(ConvArgs<TIn> a) {
             __syncthreads();
         }
         /* the assembled interior is the next level's coarse image */
-        if (a.vout && zq >= z_lo && zq < z_hi) {
+        if (!TOP && a.vout && zq >= z_lo && zq < z_hi) {
             for (int r = wv; r < TX; r += NWAVE) {
                 const int64_t x = x0 + r;
                 const int64_t y = y0 + lane;
@@ -703,11 +703,11 @@ __global__ __launch_bounds__(CT) void k_conv_sep(ConvArgs<TIn> a) {
                 }
                 ov[j] = val;
             }
-            if (a.epi == 0) {
+            if constexpr (EPI == 0) {
 #pragma unroll
                 for (int j = 0; j < RPT; ++j)
                     if (on[j]) a.out[oi[j]] = ov[j];
-            } else if (a.epi == 1) {
+            } else if constexpr (EPI == 1) {
                 float num[RPT];
 #pragma unroll
                 for (int j = 0; j < RPT; ++j)
@@ -731,7 +731,7 @@ __global__ __launch_bounds__(CT) void k_conv_sep(ConvArgs<TIn> a) {
             }
         }
     }
-    if (a.epi == 2) {
+    if constexpr (EPI == 2) {
         __syncthreads();
         double* red = (double*)smem;
         red[threadIdx.x] = n1;
@@ -943,12 +943,33 @@ void launch_conv_tile(const ConvArgs<TIn>& args, unsigned grid, size_t lds, hipS
     HIP_LAUNCH_CHECK();
 }
 
-template <typename TIn, int TX, int SZ>
-void launch_conv_sep_tile(const ConvArgs<TIn>& args, unsigned grid, size_t lds, hipStream_t st) {
+template <typename TIn, int TX, int SZ, int EPI, bool TOP>
+void launch_conv_sep_k(const ConvArgs<TIn>& args, unsigned grid, size_t lds, hipStream_t st) {
     if (lds > 65536)
-        (void)hipFuncSetAttribute((const void*)k_conv_sep<TIn, TX, SZ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((k_conv_sep<TIn, TX, SZ>), dim3(grid), dim3(CT), lds, st, args);
+        (void)hipFuncSetAttribute((const void*)k_conv_sep<TIn, TX, SZ, EPI, TOP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((k_conv_sep<TIn, TX, SZ, EPI, TOP>), dim3(grid), dim3(CT), lds, st, args);
     HIP_LAUNCH_CHECK();
+}
+
+/* the variant for the launch's epilogue and level: the finest level has no tree nodes and no
+   next-level image; u16 particles only ever feed the plain store */
+template <typename TIn, int TX, int SZ>
+void launch_conv_sep_tile(const ConvArgs<TIn>& a, unsigned grid, size_t lds, hipStream_t st) {
+    const bool top = a.L.t_row_begin < 0 && a.vout == nullptr;
+    if constexpr (std::is_same<TIn, uint16_t>::value) {
+        if (a.epi != 0) throw Error(APRH_ERR_UNSUPPORTED, "convolution epilogue needs float particles");
+        if (top) launch_conv_sep_k<TIn, TX, SZ, 0, true>(a, grid, lds, st);
+        else launch_conv_sep_k<TIn, TX, SZ, 0, false>(a, grid, lds, st);
+    } else {
+        switch (a.epi * 2 + (top ? 1 : 0)) {
+            case 0: launch_conv_sep_k<TIn, TX, SZ, 0, false>(a, grid, lds, st); break;
+            case 1: launch_conv_sep_k<TIn, TX, SZ, 0, true>(a, grid, lds, st); break;
+            case 2: launch_conv_sep_k<TIn, TX, SZ, 1, false>(a, grid, lds, st); break;
+            case 3: launch_conv_sep_k<TIn, TX, SZ, 1, true>(a, grid, lds, st); break;
+            case 4: launch_conv_sep_k<TIn, TX, SZ, 2, false>(a, grid, lds, st); break;
+            default: launch_conv_sep_k<TIn, TX, SZ, 2, true>(a, grid, lds, st); break;
+        }
+    }
 }
 
 template <typename TIn, int TX>
