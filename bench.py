@@ -486,9 +486,38 @@ def run_cpu_rl():
     apr.richardson_lucy(parts, psf, 3)
     dt = (time.perf_counter() - t) / 3
     log(f"[bench] cpu RL ({kind}, {threads} threads): {dt * 1000:.1f} ms/iteration at P={apr.total}")
-    return {"ms_per_iteration": round(dt * 1000.0, 2), "particles": apr.total, "cores": threads, "kind": kind,
+    return {"ms_per_iteration": round(dt * 1000.0, 2), "particles": apr.total, "cores": threads, "kind": kind, **host_cpu(),
             "ns_per_particle_iteration": round(dt * 1e9 / apr.total, 2),
             "sample": f"3 RL iterations (9^3 PSF) on the APR of a 256x256x128 volume of the same generator ({nb} blobs)"}
+
+
+def host_cpu():
+    """CPU model and the host's logical CPUs (all of them, and those this process may use)"""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"cpu_model": model, "nproc": usable, "cpu_count": os.cpu_count(),
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+
+
+def fp64_errors():
+    """error of the float stages against float64 restatements, as recorded by
+    tests/test_fp64_error.py (APRH_FP64_ERRORS=tests/golden/fp64_errors.json)"""
+    path = os.path.join(ROOT, "tests", "golden", "fp64_errors.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return {"source": "tests/test_fp64_error.py (CPU; the GPU results are bit-equal to the oracle)", "max_abs_over_max_and_rel_l2": json.load(f)}
 
 
 def run_cpu_baseline():
@@ -525,7 +554,7 @@ def run_cpu_baseline():
     apr.sample(img)
     dt = time.perf_counter() - t
     log(f"[bench] cpu baseline: {dt:.2f} s, P={apr.total}")
-    return {"value": round(n / dt / 1e9, 4), "unit": "Gvoxels/s", "cores": threads, "kind": kind,
+    return {"value": round(n / dt / 1e9, 4), "unit": "Gvoxels/s", "cores": threads, "kind": kind, **host_cpu(),
             "sample": f"get_apr+sample_image on a {shape[2]}x{shape[1]}x{shape[0]} u16 volume of the same generator "
                       f"({nb} blobs, seed {SEED}), {dt:.2f} s wall"}
 
@@ -585,7 +614,8 @@ def main():
                    "stage_ms": {k: round(v, 3) for k, v in stages.items()},
                    "conv_3x3x3_256x256x128": conv3, "conv_5x5x5_2048x2048x512": conv5,
                    "batched_uneven_volumes": batch,
-                   "rl_time_to_tol": rl},
+                   "rl_time_to_tol": rl,
+                   "fp64_error": fp64_errors()},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -151,9 +151,49 @@ def evaluate(ctx, net, shape_zxy=(256, 1024, 1024), seed=3, n_blobs=50_000, log=
     return res
 
 
+HELD_OUT_SEEDS = (400, 401, 402, 403)   # used by neither training (100-299) nor evaluation (3)
+POST_FILTER_CANDIDATES = [((sy, sy, 0.3), (3, 3, 1)) for sy in (0.4, 0.45, 0.5, 0.55, 0.6, 0.65)] + \
+    [((0.5, 0.5, 0.5), (3, 3, 3)), ((0.6, 0.6, 0.5), (3, 3, 3))]
+
+
+def select_post_filter(ctx, net, seeds=HELD_OUT_SEEDS, shape_zxy=(128, 512, 512), max_error_ratio=1.01, log=print):
+    """The post filter chosen on held-out volumes (seeds outside training and evaluation): for
+    each candidate, RL from the filtered network estimate to the tolerance on every volume;
+    the candidate with the fewest mean iterations whose mean final-error ratio against plain RL
+    is at most max_error_ratio (else the lowest error ratio).  Returns (choice, table)."""
+    ctx.set_conv_mode(A.CONV_SEPARABLE)
+    vols = []
+    for seed in seeds:
+        w = Workload(ctx, shape_zxy, seed)
+        u = A.DeviceArray(w.P * 4, ctx)
+        st = w.rl(u)
+        vols.append((w, st["iterations"], w.error_vs_gt(u)))
+        u.free()
+    table = []
+    for sigma, size in POST_FILTER_CANDIDATES:
+        net.set_post_filter(sigma, size)
+        its, ratios = [], []
+        for w, it_plain, err_plain in vols:
+            uw = A.DeviceArray(w.P * 4, ctx)
+            net.warm_start(w.apr, w.parts.ptr, out=uw.ptr, dtype="uint16")
+            st = w.rl(uw, resume=True)
+            its.append(st["iterations"] / it_plain)
+            ratios.append(w.error_vs_gt(uw) / err_plain)
+            uw.free()
+        row = {"sigma": list(sigma), "size": list(size), "iterations_over_plain": round(float(np.mean(its)), 4),
+               "error_ratio": round(float(np.mean(ratios)), 5)}
+        log(f"[warm_start] held-out post filter {sigma} {size}: iterations/plain {row['iterations_over_plain']:.3f} "
+            f"error ratio {row['error_ratio']:.4f}")
+        table.append(row)
+    ok = [r for r in table if r["error_ratio"] <= max_error_ratio]
+    best = min(ok, key=lambda r: r["iterations_over_plain"]) if ok else min(table, key=lambda r: r["error_ratio"])
+    ctx.set_conv_mode(A.CONV_EXACT)
+    return {"sigma": best["sigma"], "size": best["size"]}, table
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("cmd", choices=["train", "eval"])
+    ap.add_argument("cmd", choices=["train", "eval", "select-post-filter"])
     ap.add_argument("--volumes", type=int, default=200)
     ap.add_argument("--first-seed", type=int, default=100)
     ap.add_argument("--shape", default="128,256,256", help="z,x,y of the training volumes")
@@ -185,6 +225,18 @@ def main():
         del pairs
     else:
         net = A.WarmStartCNN.load(args.weights, ctx, post_filter=None)
+    if args.cmd == "select-post-filter":
+        choice, table = select_post_filter(ctx, net, log=log)
+        rec_path = os.path.splitext(args.weights)[0] + ".json"
+        rec = json.load(open(rec_path)) if os.path.exists(rec_path) else {}
+        rec["post_filter"] = choice
+        rec["post_filter_selection"] = {"held_out_seeds": list(HELD_OUT_SEEDS), "shape_zxy": [128, 512, 512],
+                                        "rule": "fewest iterations with mean final-error ratio <= 1.01 vs plain RL",
+                                        "candidates": table}
+        with open(args.report or rec_path, "w") as f:
+            json.dump(rec, f, indent=1)
+        print(json.dumps({"post_filter": choice}))
+        return
     es = tuple(int(v) for v in args.eval_shape.split(","))
     w = Workload(ctx, es, 3, 50_000)
     net.set_post_filter(None)
