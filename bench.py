@@ -316,7 +316,8 @@ def run_batch(args, check=True):
     stages = batch.stages()
     log(f"[bench] batch graph holds {len(stages)} stage launches over {len(dims)} volumes")
     for st in sorted(stages, key=lambda r: -r["tuned_ms"])[:8]:
-        log(f"[bench]   {st['name']:22s} vols={st['volumes']:3d} tiles={st['tiles']:9d} grid={st['grid']:8d} {st['tuned_ms']:.3f} ms")
+        log(f"[bench]   {st['name']:22s} {str(st['shape'] or ''):18s} vols={st['volumes']:3d} tiles={st['tiles']:9d} "
+            f"grid={st['grid']:8d} {st['tuned_ms']:.3f} ms")
     if not check:
         batch.close()
         ctx.close()
@@ -352,7 +353,8 @@ def run_batch(args, check=True):
             "Gvoxels_per_s": round(nvox / (ms / 1000.0) / 1e9, 3),
             "single_volume_loop_ms": round(seq_ms, 3), "bit_exact_volumes": exact, "volumes": len(dims),
             "stage_launches": len(stages),
-            "autotuned_stages": [{k: st[k] for k in ("name", "volumes", "tiles", "grid", "tuned_ms")} for st in stages]}
+            "autotuned_stages": [{k: st[k] for k in ("name", "shape", "shapes_tried", "volumes", "tiles", "grid", "tuned_ms")}
+                                 for st in stages]}
 
 
 RL_SHAPE_ZXY = (256, 1024, 1024)
@@ -450,6 +452,27 @@ def run_rl(args):
                                          "time_ratio_warm_over_plain": evf["time_ratio_warm_over_plain"],
                                          "error_ratio_warm_over_plain": evf["final_error_ratio_warm_over_plain"]}
     net.close()
+    ctx.close()
+    return res
+
+
+def run_rl_iterations(args):
+    """--steps Richardson-Lucy iterations on the deconvolution workload in each convolution mode
+    (after one untimed iteration), for profiling"""
+    import libapr_amd as A
+    from libapr_amd import warm_start as W
+    ctx = A.Context(0)
+    w = W.Workload(ctx, RL_SHAPE_ZXY, RL_GEN["seed"], RL_GEN["n_blobs"])
+    u = A.DeviceArray(w.P * 4, ctx)
+    w.apr.tree_info()
+    res = {"particles": w.P}
+    for name, mode in (("separable", A.CONV_SEPARABLE), ("exact", A.CONV_EXACT)):
+        ctx.set_conv_mode(mode)
+        w.rl(u, iters=1, tol=0.0)
+        t = time.perf_counter()
+        w.rl(u, iters=max(args.steps, 1), tol=0.0)
+        res[name + "_ms_per_iteration"] = round((time.perf_counter() - t) * 1000.0 / max(args.steps, 1), 3)
+        log(f"[bench] RL ({name}): {res[name + '_ms_per_iteration']:.2f} ms per iteration")
     ctx.close()
     return res
 
@@ -570,9 +593,11 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-separate", action="store_true",
                     help="skip the reference timing of get_apr and sample_image as separate calls (profiling runs)")
-    ap.add_argument("--only", choices=["get_apr", "batch", "copy"],
+    ap.add_argument("--only", choices=["get_apr", "batch", "copy", "conv", "rl"],
                     help="run one part (for profiling): the headline conversion steps, the batched volumes, "
-                         "or the device copy of 4 GiB that calibrates the HBM byte counters")
+                         "the device copy of 4 GiB that calibrates the HBM byte counters, the 5x5x5 convolution "
+                         "of the headline volume's particles (both modes), or --steps Richardson-Lucy iterations "
+                         "on the deconvolution workload (both modes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -580,7 +605,7 @@ def main():
         sys.exit(2)
     if args.only:
         part = {"get_apr": lambda: (lambda r: list(r[:4]) + [r[6]])(run_gpu(args, with_conv=False)), "batch": lambda: run_batch(args, check=False),
-                "copy": lambda: run_copy(args)}[args.only]()
+                "copy": lambda: run_copy(args), "conv": lambda: run_gpu(args)[5], "rl": lambda: run_rl_iterations(args)}[args.only]()
         print(json.dumps({"only": args.only, "result": part}), flush=True)
         return
     ms, P, stages, roofline, total_ab, conv5, extra = run_gpu(args)

@@ -147,6 +147,8 @@ typedef struct aprh_batch_stage {
     uint32_t grid;        /* blocks launched            */
     uint32_t threads;     /* threads per block          */
     float tuned_ms;       /* time at the chosen grid    */
+    const char* shape;    /* chosen tile shape (static string), NULL for stages with one shape */
+    int32_t n_shapes;     /* tile-shape alternatives the autotuner timed */
 } aprh_batch_stage;
 int aprh_batch_stage_count(const aprh_batch* batch);
 int aprh_batch_stage_info(const aprh_batch* batch, int32_t i, aprh_batch_stage* info);

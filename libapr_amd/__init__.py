@@ -48,7 +48,7 @@ class AprhOverflowError(AprhError, ValueError):
 
 class AprhBatchStage(C.Structure):
     _fields_ = [("name", C.c_char_p), ("volumes", C.c_int32), ("tiles", C.c_int64), ("grid", C.c_uint32),
-                ("threads", C.c_uint32), ("tuned_ms", C.c_float)]
+                ("threads", C.c_uint32), ("tuned_ms", C.c_float), ("shape", C.c_char_p), ("n_shapes", C.c_int32)]
 
 
 class AprhParams(C.Structure):
@@ -496,13 +496,14 @@ class APRBatch:
         _check(lib().aprh_batch_run(self.ctx.h, self.h))
 
     def stages(self):
-        """the merged stage launches with the autotuned grid of each"""
+        """the merged stage launches with the autotuned tile shape and grid of each"""
         out = []
         for i in range(lib().aprh_batch_stage_count(self.h)):
             st = AprhBatchStage()
             _check(lib().aprh_batch_stage_info(self.h, i, C.byref(st)))
             out.append({"name": st.name.decode(), "volumes": st.volumes, "tiles": st.tiles, "grid": st.grid,
-                        "threads": st.threads, "tuned_ms": round(st.tuned_ms, 4)})
+                        "threads": st.threads, "tuned_ms": round(st.tuned_ms, 4),
+                        "shape": st.shape.decode() if st.shape else None, "shapes_tried": st.n_shapes})
         return out
 
     def get(self, i):

@@ -12,9 +12,12 @@
  *     volume's order is kept and volumes with fewer levels simply sit out the
  *     extra pulling-scheme launches;
  *  3. autotunes each merged launch: the whole sequence runs once per
- *     candidate grid (one block per tile, or 1, 2, 4, 8, 16 blocks per CU
- *     walking the tiles persistently), each launch timed with events, and
- *     the fastest grid of each launch kept;
+ *     candidate (tile-shape alternative of the stages that have them -- the
+ *     x/z smoothing's checkpoint spacing, the running means' look-ahead, the
+ *     pulling scheme's z run, the row fill's lanes per work item -- times
+ *     the grid: one block per tile, or 1, 2, 4, 8, 16 blocks per CU walking
+ *     the tiles persistently), each launch timed with events, and the
+ *     fastest shape and grid of each launch kept;
  *  4. captures the tuned sequence into one graph.
  * aprh_batch_run replays the graph and reads each volume's particle count,
  * B-spline offset and overflow flags from device slots.
@@ -34,13 +37,20 @@ using namespace aprh;
 
 namespace {
 
-struct Group {
-    const TileRec* proto = nullptr;           /* body, block size, launcher, name */
-    std::vector<const TileRec*> recs;         /* one per volume, in launch order  */
+/* the merged launch of one tile-shape alternative: arguments and tile prefix of every volume */
+struct GroupAlt {
+    const TileAlt* proto = nullptr;           /* body, block size, launcher, name, shape */
     int64_t tiles = 0;
     Buf d_args, d_prefix;
+};
+
+struct Group {
+    std::vector<const TileRec*> recs;         /* one per volume, in launch order  */
+    std::vector<GroupAlt> alts;               /* the stage's tile-shape alternatives */
+    int alt = 0;                              /* the one launched                */
     unsigned grid = 0;
     float ms = 0.0f;
+    const GroupAlt& cur() const { return alts[(size_t)alt]; }
 };
 
 constexpr int N_CAND = 6;   /* one block per tile; 1, 2, 4, 8, 16 blocks per CU */
@@ -138,59 +148,88 @@ void merge(aprh_batch* b) {
             const TileRec& r = b->recs[v].recs[head[v]];
             if (r.body != best) continue;
             g.recs.push_back(&r);
-            g.tiles += r.tiles;
             ++head[v];
         }
-        g.proto = g.recs.front();
+        /* the alternatives every volume of the launch has (the recording passes them in a fixed
+           order per stage, so the lists agree) */
+        size_t na = g.recs.front()->alts.size();
+        for (const TileRec* r : g.recs) na = std::min(na, r->alts.size());
+        for (size_t k = 0; k < na; ++k) {
+            bool same = true;
+            for (const TileRec* r : g.recs) same = same && r->alts[k].body == g.recs.front()->alts[k].body;
+            if (!same) {
+                na = k;
+                break;
+            }
+        }
+        g.alts.resize(std::max<size_t>(na, 1));
+        g.alt = std::min<int>(g.recs.front()->prefer, (int)g.alts.size() - 1);
         b->groups.push_back(std::move(g));
     }
-    /* arguments and tile prefixes of each launch, uploaded once */
+    /* arguments and tile prefixes of each launch and alternative, uploaded once */
     for (auto& g : b->groups) {
-        const size_t as = g.proto->args.size();
-        std::vector<uint8_t> args(as * g.recs.size());
-        std::vector<int64_t> prefix(g.recs.size() + 1, 0);
-        for (size_t k = 0; k < g.recs.size(); ++k) {
-            std::memcpy(args.data() + k * as, g.recs[k]->args.data(), as);
-            prefix[k + 1] = prefix[k] + g.recs[k]->tiles;
+        for (size_t k = 0; k < g.alts.size(); ++k) {
+            GroupAlt& ga = g.alts[k];
+            ga.proto = &g.recs.front()->alts[k];
+            const size_t as = ga.proto->args.size();
+            std::vector<uint8_t> args(as * g.recs.size());
+            std::vector<int64_t> prefix(g.recs.size() + 1, 0);
+            for (size_t v = 0; v < g.recs.size(); ++v) {
+                std::memcpy(args.data() + v * as, g.recs[v]->alts[k].args.data(), as);
+                prefix[v + 1] = prefix[v] + g.recs[v]->alts[k].tiles;
+            }
+            ga.tiles = prefix.back();
+            ga.d_args.reset(b->pool, args.size());
+            ga.d_prefix.reset(b->pool, prefix.size() * 8);
+            HIP_CHECK(hipMemcpy(ga.d_args.ptr(), args.data(), args.size(), hipMemcpyHostToDevice));
+            HIP_CHECK(hipMemcpy(ga.d_prefix.ptr(), prefix.data(), prefix.size() * 8, hipMemcpyHostToDevice));
         }
-        g.d_args.reset(b->pool, args.size());
-        g.d_prefix.reset(b->pool, prefix.size() * 8);
-        HIP_CHECK(hipMemcpy(g.d_args.ptr(), args.data(), args.size(), hipMemcpyHostToDevice));
-        HIP_CHECK(hipMemcpy(g.d_prefix.ptr(), prefix.data(), prefix.size() * 8, hipMemcpyHostToDevice));
     }
 }
 
-unsigned candidate_grid(const aprh_batch* b, const Group& g, int c) {
-    const int64_t cap = (int64_t)(0xFFFFFFFFu / (unsigned)g.proto->threads);
-    const int64_t want = (c == 0) ? g.tiles : (int64_t)b->cus * (1 << (c - 1));
-    return (unsigned)std::max<int64_t>(1, std::min<int64_t>({want, g.tiles, cap}));
+unsigned candidate_grid(const aprh_batch* b, const GroupAlt& ga, int c) {
+    const int64_t cap = (int64_t)(0xFFFFFFFFu / (unsigned)ga.proto->threads);
+    const int64_t want = (c == 0) ? ga.tiles : (int64_t)b->cus * (1 << (c - 1));
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>({want, ga.tiles, cap}));
 }
 
 void launch_all(aprh_batch* b, const std::vector<hipEvent_t>* ev) {
     hipStream_t st = b->ctx->stream;
     for (size_t k = 0; k < b->groups.size(); ++k) {
         const Group& g = b->groups[k];
+        const GroupAlt& ga = g.cur();
         if (ev) HIP_CHECK(hipEventRecord((*ev)[k], st));
-        g.proto->multi(st, g.d_args.ptr(), g.d_prefix.as<int64_t>(), (int)g.recs.size(), g.grid);
+        ga.proto->multi(st, ga.d_args.ptr(), ga.d_prefix.as<int64_t>(), (int)g.recs.size(), g.grid);
     }
     if (ev) HIP_CHECK(hipEventRecord(ev->back(), st));
 }
 
-/* step 3: per-launch grid from timed runs of the whole sequence (each run recomputes every
-   stage from the inputs, so the runs leave valid results behind) */
+/* step 3: per-launch tile shape and grid from timed runs of the whole sequence (each run
+   recomputes every stage from the inputs with alternatives that give the same results, so the
+   runs leave valid results behind) */
 void autotune(aprh_batch* b) {
     hipStream_t st = b->ctx->stream;
     std::vector<hipEvent_t> ev(b->groups.size() + 1);
     for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
-    std::vector<std::vector<float>> t(b->groups.size(), std::vector<float>(N_CAND, 1e30f));
+    size_t max_alts = 1;
+    for (const auto& g : b->groups) max_alts = std::max(max_alts, g.alts.size());
+    /* t[launch][alternative][grid candidate] */
+    std::vector<std::vector<std::vector<float>>> t(b->groups.size());
+    for (size_t k = 0; k < b->groups.size(); ++k)
+        t[k].assign(b->groups[k].alts.size(), std::vector<float>(N_CAND, 1e30f));
     try {
-        for (int c = -1; c < N_CAND; ++c) {   /* c = -1: warm-up */
-            for (auto& g : b->groups) g.grid = candidate_grid(b, g, std::max(c, 0));
-            launch_all(b, &ev);
-            HIP_CHECK(hipStreamSynchronize(st));
-            if (c < 0) continue;
-            for (size_t k = 0; k < b->groups.size(); ++k) HIP_CHECK(hipEventElapsedTime(&t[k][c], ev[k], ev[k + 1]));
-        }
+        for (size_t a = 0; a < max_alts; ++a)
+            for (int c = -1; c < N_CAND; ++c) {   /* c = -1: warm-up */
+                for (auto& g : b->groups) {
+                    g.alt = (int)std::min(a, g.alts.size() - 1);
+                    g.grid = candidate_grid(b, g.cur(), std::max(c, 0));
+                }
+                launch_all(b, &ev);
+                HIP_CHECK(hipStreamSynchronize(st));
+                if (c < 0) continue;
+                for (size_t k = 0; k < b->groups.size(); ++k)
+                    if (a < b->groups[k].alts.size()) HIP_CHECK(hipEventElapsedTime(&t[k][a][c], ev[k], ev[k + 1]));
+            }
     } catch (...) {
         for (auto& e : ev) (void)hipEventDestroy(e);
         throw;
@@ -198,11 +237,16 @@ void autotune(aprh_batch* b) {
     for (auto& e : ev) (void)hipEventDestroy(e);
     for (size_t k = 0; k < b->groups.size(); ++k) {
         Group& g = b->groups[k];
-        int bc = 0;
-        for (int c = 1; c < N_CAND; ++c)
-            if (t[k][c] < t[k][bc]) bc = c;
-        g.grid = candidate_grid(b, g, bc);
-        g.ms = t[k][bc];
+        int ba = 0, bc = 0;
+        for (size_t a = 0; a < g.alts.size(); ++a)
+            for (int c = 0; c < N_CAND; ++c)
+                if (t[k][a][c] < t[k][ba][bc]) {
+                    ba = (int)a;
+                    bc = c;
+                }
+        g.alt = ba;
+        g.grid = candidate_grid(b, g.cur(), bc);
+        g.ms = t[k][ba][bc];
     }
 }
 
@@ -317,12 +361,15 @@ int aprh_batch_stage_info(const aprh_batch* b, int32_t i, aprh_batch_stage* info
     APRH_TRY
     if (!b || !info || i < 0 || i >= (int)b->groups.size()) throw Error(APRH_ERR_INVALID_ARGUMENT, "bad stage index");
     const Group& g = b->groups[(size_t)i];
-    info->name = g.proto->name;
+    const GroupAlt& ga = g.cur();
+    info->name = ga.proto->name;
     info->volumes = (int32_t)g.recs.size();
-    info->tiles = g.tiles;
+    info->tiles = ga.tiles;
     info->grid = g.grid;
-    info->threads = (uint32_t)g.proto->threads;
+    info->threads = (uint32_t)ga.proto->threads;
     info->tuned_ms = g.ms;
+    info->shape = ga.proto->shape;
+    info->n_shapes = (int32_t)g.alts.size();
     APRH_CATCH
 }
 

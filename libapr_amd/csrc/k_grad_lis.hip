@@ -730,8 +730,17 @@ template <int O>
 void mean_outer(aprh_context* ctx, float* A, int64_t y_num, int64_t n, int64_t s, int64_t outer_stride, int64_t n_outer) {
     const int64_t lanes2 = y_num / 2 * n_outer;
     if (y_num % 2 == 0 && s % 2 == 0 && outer_stride % 2 == 0 && ((uintptr_t)A & 7u) == 0 && lanes2 < (int64_t(1) << 32)) {
-        /* the gx field carries the lane count of the dense numbering */
-        launch_tiles<MeanOuter2<O, 8>>(ctx, {A, y_num, n, s, outer_stride, lanes2}, cdiv(lanes2, 256));
+        /* the gx field carries the lane count of the dense numbering; a batch plan also times
+           4 and 16 steps of look-ahead */
+        const OuterArgs oa{A, y_num, n, s, outer_stride, lanes2};
+        const int64_t tiles = cdiv(lanes2, 256);
+        if (ctx->recorder)
+            launch_tiles_alts(ctx,
+                              {tile_alt<MeanOuter2<O, 4>>(oa, tiles, "look-ahead 4"), tile_alt<MeanOuter2<O, 8>>(oa, tiles, "look-ahead 8"),
+                               tile_alt<MeanOuter2<O, 16>>(oa, tiles, "look-ahead 16")},
+                              1);
+        else
+            launch_tiles<MeanOuter2<O, 8>>(ctx, oa, tiles);
     } else {
         const int64_t gx = cdiv(y_num, 256);
         launch_tiles<MeanOuter<O>>(ctx, {A, y_num, n, s, outer_stride, gx}, gx * n_outer);
@@ -915,15 +924,17 @@ struct GradDsVec {
                                a.h2z, a.accum, yg, r % a.xd_num, r / a.xd_num, a.xd_num, a.zd_num);
     }
 };
-template <int O, typename G>
+template <typename G>
+struct MeanZLevelsArgs {
+    LevelsArgs<G> la;
+    int64_t p0, npair, lanes;
+};
+template <int O, typename G, int D = 8>
 struct MeanZLevels {
-    struct Args {
-        LevelsArgs<G> la;
-        int64_t p0, npair, lanes;
-    };
+    using Args = MeanZLevelsArgs<G>;
     static constexpr int THREADS = 256;
     static const char* name() { return "lis_mean_z_levels"; }
-    static __device__ __forceinline__ void run(const Args& a, int64_t t) { mean_z_levels_body<O, 8, G>(a.la, a.p0, a.npair, a.lanes, t); }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) { mean_z_levels_body<O, D, G>(a.la, a.p0, a.npair, a.lanes, t); }
 };
 template <typename G>
 struct Levels {
@@ -1084,6 +1095,21 @@ bool lis_can_fuse_levels(const LisParams& lp, int64_t yd, int64_t xd, int64_t zd
     return npair * g.xd < (int64_t(1) << 32);
 }
 
+/* the fused last mean with 8 steps of look-ahead; a batch plan (u16) also times 4 and 16 */
+template <int O, typename G>
+void launch_mean_z_levels(aprh_context* ctx, const MeanZLevelsArgs<G>& za, int64_t tiles) {
+    if constexpr (std::is_same<G, uint16_t>::value) {
+        if (ctx->recorder) {
+            launch_tiles_alts(ctx,
+                              {tile_alt<MeanZLevels<O, G, 4>>(za, tiles, "look-ahead 4"), tile_alt<MeanZLevels<O, G, 8>>(za, tiles, "look-ahead 8"),
+                               tile_alt<MeanZLevels<O, G, 16>>(za, tiles, "look-ahead 16")},
+                              1);
+            return;
+        }
+    }
+    launch_tiles<MeanZLevels<O, G>>(ctx, za, tiles);
+}
+
 template <typename G>
 void launch_levels_padded(aprh_context* ctx, const float* A, const LisParams& lp, int64_t yd, int64_t xd, int64_t zd,
                           const float* inten, G* grad, float* sigma_raw, float* sigma_out, uint8_t* level,
@@ -1097,13 +1123,15 @@ void launch_levels_padded(aprh_context* ctx, const float* A, const LisParams& lp
     if (lp.fuse_levels) {
         /* the z mean of the second window (left out by launch_lis) with the levels epilogue */
         const int64_t p0 = g.py / 2, npair = (g.py + g.yd + 1) / 2 - p0, lanes = npair * g.xd;
+        const MeanZLevelsArgs<G> za{la, p0, npair, lanes};
+        const int64_t tiles = cdiv(lanes, 256);
         switch (lp.win[5]) {
-            case 1: launch_tiles<MeanZLevels<1, G>>(ctx, {la, p0, npair, lanes}, cdiv(lanes, 256)); break;
-            case 2: launch_tiles<MeanZLevels<2, G>>(ctx, {la, p0, npair, lanes}, cdiv(lanes, 256)); break;
-            case 3: launch_tiles<MeanZLevels<3, G>>(ctx, {la, p0, npair, lanes}, cdiv(lanes, 256)); break;
-            case 4: launch_tiles<MeanZLevels<4, G>>(ctx, {la, p0, npair, lanes}, cdiv(lanes, 256)); break;
-            case 5: launch_tiles<MeanZLevels<5, G>>(ctx, {la, p0, npair, lanes}, cdiv(lanes, 256)); break;
-            case 6: launch_tiles<MeanZLevels<6, G>>(ctx, {la, p0, npair, lanes}, cdiv(lanes, 256)); break;
+            case 1: launch_mean_z_levels<1, G>(ctx, za, tiles); break;
+            case 2: launch_mean_z_levels<2, G>(ctx, za, tiles); break;
+            case 3: launch_mean_z_levels<3, G>(ctx, za, tiles); break;
+            case 4: launch_mean_z_levels<4, G>(ctx, za, tiles); break;
+            case 5: launch_mean_z_levels<5, G>(ctx, za, tiles); break;
+            case 6: launch_mean_z_levels<6, G>(ctx, za, tiles); break;
             default: throw Error(APRH_ERR_UNSUPPORTED, "mean window offset must be 1..6");
         }
         return;

@@ -1312,12 +1312,23 @@ void launch_iir_outer(aprh_context* ctx, T* img, int64_t y, int64_t x, int64_t z
         IirArgs<T> a{img, img, y, n, s, ostride, 1, c, nullptr, d_err};
         a.ck_lanes = lanes;
         if (n >= OUTER_CK_MIN && lanes < (int64_t(1) << 32)) {
-            const int64_t chunks = (n - 4 + OUTER_CK - 1) / OUTER_CK;
+            /* a batch plan also times checkpoints every 8 and every 32 steps (the buffer then
+               holds the states of the densest spacing) */
+            const bool alts = std::is_same<T, uint16_t>::value && ctx->recorder;
+            const int64_t chunks = (n - 4 + (alts ? 8 : OUTER_CK) - 1) / (alts ? 8 : OUTER_CK);
             /* the chunk states live until the launch ends: from the pool, returned after the
                launch is queued (stream order keeps later users behind it) */
             Buf ck(ctx->pool, ((size_t)(chunks - 1) * 2 + V) * (size_t)lanes * 16);
             a.ck = ck.as<uint4>();
-            launch_tiles<IirOuterCk<T, OUTER_CK>>(ctx, a, cdiv(lanes, 256));
+            const int64_t tiles = cdiv(lanes, 256);
+            if (alts)
+                launch_tiles_alts(ctx,
+                                  {tile_alt<IirOuterCk<T, 8>>(a, tiles, "checkpoint every 8"),
+                                   tile_alt<IirOuterCk<T, OUTER_CK>>(a, tiles, "checkpoint every 16"),
+                                   tile_alt<IirOuterCk<T, 32>>(a, tiles, "checkpoint every 32")},
+                                  1);
+            else
+                launch_tiles<IirOuterCk<T, OUTER_CK>>(ctx, a, tiles);
         } else {
             launch_tiles<IirOuterVec<T, 8>>(ctx, a, cdiv(lanes, 256));
         }

@@ -113,12 +113,13 @@ __device__ __forceinline__ void row_masks(const uint8_t b[10], uint32_t& am, uin
    OR over rows x-1, x, x+1): every plane's three rows are loaded once instead of three times.
    A cell's state is written only by its owner lane, so the cached bytes of the own row are
    current. */
-constexpr int PULL_ZC = 4;
+constexpr int PULL_ZC = 4;   /* planes per lane of a single volume; a batch plan also times 8 */
+template <int ZC>
 __device__ __forceinline__ void pull_level8_body(uint8_t* P, Dims d, uint8_t* Pc, Dims dc, uint8_t* Pp, Dims dp, int is_top,
                                                  int has_parent, int64_t G, int64_t t) {
     const int yn = (int)d.y, xn = (int)d.x, zn = (int)d.z;
     const int nwr = (yn + 7) >> 3;
-    const int nzc = (zn + PULL_ZC - 1) / PULL_ZC;
+    const int nzc = (zn + ZC - 1) / ZC;
     const int64_t total = (int64_t)nwr * xn * nzc;
     const bool aligned8 = (yn & 7) == 0 && ((uintptr_t)P & 7) == 0;
     /* parent rows whose words never straddle two rows */
@@ -128,7 +129,7 @@ __device__ __forceinline__ void pull_level8_body(uint8_t* P, Dims d, uint8_t* Pc
         const int64_t r = i / nwr;
         const int x = (int)(r % xn), zc = (int)(r / xn);
         const int y0 = w * 8;
-        const int z_begin = zc * PULL_ZC, z_end = min(zn, z_begin + PULL_ZC);
+        const int z_begin = zc * ZC, z_end = min(zn, z_begin + ZC);
         /* masks of plane zz (0 outside the volume); own row bytes of that plane in own[] */
         auto plane = [&](int zz, uint32_t& am, uint32_t& sm, uint8_t own[10]) {
             am = 0;
@@ -552,21 +553,23 @@ struct LevelDown {
     static const char* name() { return "pulling_level_down"; }
     static __device__ __forceinline__ void run(const Args& a, int64_t t) { level_down_body(a.fine, a.fd, a.coarse, a.cd, a.pct, a.level, a.mode, a.G, t); }
 };
+struct PullLevelArgs {
+    uint8_t* P;
+    Dims d;
+    uint8_t* Pc;
+    Dims dc;
+    uint8_t* Pp;
+    Dims dp;
+    int is_top, has_parent;
+    int64_t G;
+};
+template <int ZC = PULL_ZC>
 struct PullLevel {
-    struct Args {
-        uint8_t* P;
-        Dims d;
-        uint8_t* Pc;
-        Dims dc;
-        uint8_t* Pp;
-        Dims dp;
-        int is_top, has_parent;
-        int64_t G;
-    };
+    using Args = PullLevelArgs;
     static constexpr int THREADS = 256;
     static const char* name() { return "pulling_level"; }
     static __device__ __forceinline__ void run(const Args& a, int64_t t) {
-        pull_level8_body(a.P, a.d, a.Pc, a.dc, a.Pp, a.dp, a.is_top, a.has_parent, a.G, t);
+        pull_level8_body<ZC>(a.P, a.d, a.Pc, a.dc, a.Pp, a.dp, a.is_top, a.has_parent, a.G, t);
     }
 };
 struct Upseed {
@@ -990,10 +993,19 @@ void build_structure_seeded(aprh_context* ctx, aprh_apr& a, const uint8_t* level
         const bool is_top = (l == top);
         uint8_t* Pc = is_top ? nullptr : P + off[l + 1];
         uint8_t* Pp = has_parent ? P + off[l - 1] : nullptr;
-        const int64_t G = stride_tiles(cdiv(a.y_num[l], 8) * a.x_num[l] * cdiv(a.z_num[l], PULL_ZC), 256 * 2);
-        launch_tiles<PullLevel>(ctx, {P + off[l], D(l), Pc, is_top ? Dims{1, 1, 1} : D(l + 1), Pp, has_parent ? D(l - 1) : Dims{1, 1, 1},
-                                      is_top ? 1 : 0, has_parent ? 1 : 0, G},
-                                G);
+        auto pargs = [&](int zc) {
+            const int64_t G = stride_tiles(cdiv(a.y_num[l], 8) * a.x_num[l] * cdiv(a.z_num[l], zc), 256 * 2);
+            return PullLevelArgs{P + off[l], D(l), Pc, is_top ? Dims{1, 1, 1} : D(l + 1), Pp, has_parent ? D(l - 1) : Dims{1, 1, 1},
+                                 is_top ? 1 : 0, has_parent ? 1 : 0, G};
+        };
+        const PullLevelArgs p4 = pargs(PULL_ZC);
+        if (ctx->recorder) {
+            const PullLevelArgs p8 = pargs(8);
+            launch_tiles_alts(ctx, {tile_alt<PullLevel<PULL_ZC>>(p4, p4.G, "4 planes per lane"), tile_alt<PullLevel<8>>(p8, p8.G, "8 planes per lane")},
+                              0);
+        } else {
+            launch_tiles<PullLevel<>>(ctx, p4, p4.G);
+        }
     }
     const int min_type = par.neighborhood_optimization ? 1 : 2;
     mark(ctx, ST_STRUCTURE);
@@ -1064,12 +1076,18 @@ void build_structure_seeded(aprh_context* ctx, aprh_apr& a, const uint8_t* level
         sr->done = true;
     }
     /* a wave per work item when the longest rows fill one (1024 cells), else 16 lanes */
-    if (t.map_y[t.n_levels - 1] >= 512) {
-        const int64_t FG = stride_tiles(t.n_work, FILL_WAVES, 16384);
-        launch_tiles<RowFill<64>>(ctx, {P, t, rp, rowoff.as<uint64_t>(), a.y_vec.as<uint16_t>(), FG, ri}, FG);
-    } else {
-        const int64_t FG = stride_tiles(t.n_work, FILL_WAVES * 4, 16384);
-        launch_tiles<RowFill<16>>(ctx, {P, t, rp, rowoff.as<uint64_t>(), a.y_vec.as<uint16_t>(), FG, ri}, FG);
+    {
+        const int64_t FG16 = stride_tiles(t.n_work, FILL_WAVES * 4, 16384), FG64 = stride_tiles(t.n_work, FILL_WAVES, 16384);
+        const typename RowFill<16>::Args f16{P, t, rp, rowoff.as<uint64_t>(), a.y_vec.as<uint16_t>(), FG16, ri};
+        const typename RowFill<64>::Args f64{P, t, rp, rowoff.as<uint64_t>(), a.y_vec.as<uint16_t>(), FG64, ri};
+        const int prefer = t.map_y[t.n_levels - 1] >= 512 ? 1 : 0;
+        if (ctx->recorder)
+            launch_tiles_alts(ctx, {tile_alt<RowFill<16>>(f16, FG16, "16 lanes per item"), tile_alt<RowFill<64>>(f64, FG64, "64 lanes per item")},
+                              prefer);
+        else if (prefer)
+            launch_tiles<RowFill<64>>(ctx, f64, FG64);
+        else
+            launch_tiles<RowFill<16>>(ctx, f16, FG16);
     }
     if (ctx->keep_intermediates) {
         a.pct.clear();

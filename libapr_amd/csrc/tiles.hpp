@@ -28,15 +28,26 @@
 
 namespace aprh {
 
-/* one recorded launch of a volume */
-struct TileRec {
+/* one way to run a stage of a volume: a body B with its arguments and tiles */
+struct TileAlt {
     const std::type_info* body = nullptr;   /* identifies B                      */
     std::vector<uint8_t> args;              /* bytes of B::Args                  */
     int64_t tiles = 0;
     int threads = 0;
     /* persistent launch of B over nv volumes (args and prefix in device memory) */
     void (*multi)(hipStream_t, const void* d_args, const int64_t* d_prefix, int nv, unsigned grid) = nullptr;
+    /* launch of B over one volume */
+    void (*single)(hipStream_t, const void* args, int64_t tiles) = nullptr;
     const char* name = nullptr;
+    const char* shape = nullptr;            /* tile shape of this alternative (or nullptr) */
+};
+
+/* one recorded launch of a volume: alts[0], or for a stage with tile-shape alternatives every
+   alternative (same results, in a fixed order per stage), which a batch plan times */
+struct TileRec {
+    const std::type_info* body = nullptr;   /* merge key: the body of alts[0]    */
+    std::vector<TileAlt> alts;
+    int prefer = 0;                         /* the alternative a single volume takes */
 };
 
 struct TileRecorder {
@@ -103,18 +114,54 @@ inline unsigned single_grid(int64_t tiles, int threads) {
 }
 
 template <typename B>
+void launch_tiles_single(hipStream_t st, const void* args, int64_t tiles) {
+    typename B::Args a;
+    std::memcpy(&a, args, sizeof(a));
+    hipLaunchKernelGGL(k_tiles<B>, dim3(single_grid(tiles, B::THREADS)), dim3(B::THREADS), 0, st, a, tiles);
+    HIP_LAUNCH_CHECK();
+}
+
+/* an alternative for launch_tiles_alts: body B over `tiles` tiles, shape named for reports */
+template <typename B>
+TileAlt tile_alt(const typename B::Args& a, int64_t tiles, const char* shape = nullptr) {
+    static_assert(std::is_trivially_copyable<typename B::Args>::value, "tile arguments must be plain data");
+    TileAlt t;
+    t.body = &typeid(B);
+    t.args.resize(sizeof(typename B::Args));
+    std::memcpy(t.args.data(), &a, sizeof(typename B::Args));
+    t.tiles = tiles;
+    t.threads = B::THREADS;
+    t.multi = &launch_tiles_multi<B>;
+    t.single = &launch_tiles_single<B>;
+    t.name = B::name();
+    t.shape = shape;
+    return t;
+}
+
+/* a stage with tile-shape alternatives that compute the same result: one volume runs
+   alts[prefer]; a batch plan records them all and keeps the fastest */
+inline void launch_tiles_alts(aprh_context* ctx, std::vector<TileAlt> alts, int prefer) {
+    if (alts.empty() || alts[(size_t)prefer].tiles <= 0) return;
+    if (ctx->recorder) {
+        TileRec r;
+        r.body = alts[0].body;
+        r.alts = std::move(alts);
+        r.prefer = prefer;
+        ctx->recorder->recs.push_back(std::move(r));
+        return;
+    }
+    const TileAlt& t = alts[(size_t)prefer];
+    t.single(ctx->stream, t.args.data(), t.tiles);
+}
+
+template <typename B>
 void launch_tiles(aprh_context* ctx, const typename B::Args& a, int64_t tiles) {
     static_assert(std::is_trivially_copyable<typename B::Args>::value, "tile arguments must be plain data");
     if (tiles <= 0) return;
     if (ctx->recorder) {
         TileRec r;
         r.body = &typeid(B);
-        r.args.resize(sizeof(typename B::Args));
-        std::memcpy(r.args.data(), &a, sizeof(typename B::Args));
-        r.tiles = tiles;
-        r.threads = B::THREADS;
-        r.multi = &launch_tiles_multi<B>;
-        r.name = B::name();
+        r.alts.push_back(tile_alt<B>(a, tiles));
         ctx->recorder->recs.push_back(std::move(r));
         return;
     }
