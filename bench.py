@@ -67,6 +67,22 @@ def algorithmic_bytes(stage, N, Nd, P, Npad):
     }.get(stage)
 
 
+def oracle_particle_count():
+    """the oracle's particle count for this workload, from the committed digests of the CPU
+    oracle's results (tests/golden/large_workloads.json; the structure and intensities are
+    compared bit for bit in the GPU tests), or None if that file describes another workload"""
+    try:
+        with open(os.path.join(ROOT, "tests", "golden", "large_workloads.json")) as f:
+            d = json.load(f)
+        prm = d["params"]["large"]
+        if (tuple(prm["shape_zxy"]) == SHAPE_ZXY and prm["seed"] == SEED and prm["n_blobs"] == N_BLOBS
+                and tuple(prm["sigma"]) == SIGMA and tuple(prm["amp"]) == AMP):
+            return int(d["large"]["total"])
+    except (OSError, KeyError, ValueError):
+        pass
+    return None
+
+
 def run_copy(args):
     """device-to-device copy of 4 GiB (steps times): the known-size kernel that calibrates FETCH_SIZE /
     WRITE_SIZE (each copy reads and writes 4 GiB)"""
@@ -110,6 +126,9 @@ def run_gpu(args, with_conv=True):
     apr0 = conv.get_apr(d_img.value, dims=(y, x, z))
     P = apr0.total
     apr0.close()
+    P_oracle = oracle_particle_count()
+    if P_oracle is not None and P != P_oracle:
+        raise RuntimeError(f"particle count {P} differs from the oracle's {P_oracle} for this volume")
     d_parts = C.c_void_p()
     A._check(L.aprh_malloc(ctx.h, max(P, 1) * 2, C.byref(d_parts)))
 
@@ -149,7 +168,7 @@ def run_gpu(args, with_conv=True):
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
     log(f"[bench] {args.steps} steps: {ms_per_step:.2f} ms/step, P={P}")
-    stage_extra = {"separate_calls_ms": None, "separate_sample_image_ms": None}
+    stage_extra = {"separate_calls_ms": None, "separate_sample_image_ms": None, "oracle_particles": P_oracle}
     if not args.no_separate:
         # reference point (untimed for the metric): the same work as separate get_apr + sample_image calls
         step_separate()
@@ -159,7 +178,7 @@ def run_gpu(args, with_conv=True):
         A._check(L.aprh_memcpy(ctx.h, None, None, 0))
         separate_ms = (time.perf_counter() - ts) * 1000.0
         log(f"[bench] as separate get_apr + sample_image calls: {separate_ms:.2f} ms (sample_image {sep['sample_image']:.3f} ms)")
-        stage_extra = {"separate_calls_ms": round(separate_ms, 3), "separate_sample_image_ms": round(sep["sample_image"], 3)}
+        stage_extra.update(separate_calls_ms=round(separate_ms, 3), separate_sample_image_ms=round(sep["sample_image"], 3))
     stage_avg = {k: v / args.steps for k, v in stage_acc.items()}
     for k, v in sorted(stage_avg.items(), key=lambda kv: -kv[1]):
         log(f"[bench]   {k:24s} {v:8.3f} ms")
@@ -634,7 +653,8 @@ def main():
                                "bg 100 + shot noise, seed 1; rel_error 0.1 lambda 3 sigma_th 50 Ip_th 120 grad_th 10",
                    "call": "aprh_get_apr_sample_u16 (conversion and sampling in one call)",
                    "as_separate_calls_ms": extra["separate_calls_ms"],
-                   "particles": P, "pipeline_algorithmic_bytes": total_ab,
+                   "particles": P, "particles_equal_oracle": extra["oracle_particles"] == P if extra["oracle_particles"] else None,
+                   "pipeline_algorithmic_bytes": total_ab,
                    "pipeline_GBps": round(total_ab / (ms / 1000.0) / 1e9, 1),
                    "stage_ms": {k: round(v, 3) for k, v in stages.items()},
                    "conv_3x3x3_256x256x128": conv3, "conv_5x5x5_2048x2048x512": conv5,

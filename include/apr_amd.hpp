@@ -4,16 +4,20 @@
  * shape:
  *
  *   APRConverter<uint16_t>::get_apr      algorithm/APRConverter.hpp:34-134
+ *   APRConverterBatch::get_apr           algorithm/APRConverterBatch.hpp:33-175
  *   APR (GenInfo + LinearAccess)         data_structures/APR/APR.hpp:18-61
  *   ParticleData<T>::sample_image        data_structures/APR/particles/ParticleData.hpp:122-127
+ *   ParticleData::sample_parts_from_img_blocked  data_structures/APR/particles/ParticleData.hpp:140-141, 267-362
  *   APRTreeNumerics::fill_tree_mean      numerics/APRTreeNumerics.hpp:19
  *   APRStencil::*                        numerics/APRStencil.hpp:12-244
  *   APRFilter::convolve(_pencil)         numerics/APRFilter.hpp:67-149
  *   APRNumerics::richardson_lucy         numerics/APRNumerics.hpp:126-134
  *   APRNumerics::gradient_* (cfd, sobel, magnitude)  numerics/APRNumerics.hpp:165-283
  *   APRNumerics::local_std               numerics/APRNumerics.hpp:357-389
+ *   APRNumerics::(seperable_)face_neighbour_filter  numerics/APRNumerics.hpp:286-355
  *   APRReconstruction::reconstruct_constant  numerics/APRReconstruction.hpp:80, :297
  *   APRReconstruction::reconstruct_level     numerics/APRReconstruction.hpp:131-171
+ *   APRReconstruction::reconstruct_smooth    numerics/APRReconstruction.hpp:270-295
  *   ReconPatch and the patch reconstructions numerics/APRReconstruction.hpp:20-66, :186-196, :296-490
  *
  * Error conventions follow the reference: get_apr returns false on bad
@@ -169,6 +173,14 @@ struct ParticleData {
             detail::check(aprh_sample_image_u16(apr.context().get(), apr.get(), (const uint16_t*)img.mesh.data(),
                                                 data.empty() ? nullptr : (uint16_t*)data.data()));
     }
+    /* sample_parts_from_img_blocked (ParticleData.hpp:140-141, 267-362), from an image in memory
+       instead of a TIFF file: per z block the block's mean pyramid with its ghost planes */
+    void sample_parts_from_img_blocked(APR& apr, const PixelData<uint16_t>& img, int z_block_size = 256, int ghost_z = 32) {
+        static_assert(std::is_same<T, uint16_t>::value, "blocked sampling takes uint16 images into uint16 particles");
+        init(apr.total_number_particles());
+        detail::check(aprh_sample_image_blocked_u16(apr.context().get(), apr.get(), img.mesh.data(), z_block_size, ghost_z,
+                                                    data.empty() ? nullptr : (uint16_t*)data.data()));
+    }
 };
 
 template <typename ImageType>
@@ -187,6 +199,29 @@ public:
             rc = aprh_get_apr_f32(ctx_->get(), (const float*)img.mesh.data(), img.y_num, img.x_num, img.z_num, &par, &h);
         else
             rc = aprh_get_apr_u16(ctx_->get(), (const uint16_t*)img.mesh.data(), img.y_num, img.x_num, img.z_num, &par, &h);
+        if (rc == APRH_ERR_BAD_DIMS) return false;
+        detail::check(rc);
+        apr.adopt(h, ctx_);
+        return true;
+    }
+
+private:
+    Context* ctx_;
+};
+
+/* APRConverterBatch<uint16_t> (APRConverterBatch.hpp:33-175): get_apr over z blocks of
+   z_block_size planes with ghost_z ghost planes each side, from an image in memory instead of
+   a TIFF file; false on bad dimensions */
+class APRConverterBatch {
+public:
+    APRParameters par;
+    int z_block_size = 128;
+    int ghost_z = 32;
+    explicit APRConverterBatch(Context& ctx = Context::default_context()) : ctx_(&ctx) {}
+    bool get_apr(APR& apr, const PixelData<uint16_t>& img) {
+        aprh_apr* h = nullptr;
+        const int rc = aprh_get_apr_by_block_u16(ctx_->get(), img.mesh.data(), img.y_num, img.x_num, img.z_num, &par, z_block_size,
+                                                 ghost_z, &h);
         if (rc == APRH_ERR_BAD_DIMS) return false;
         detail::check(rc);
         apr.adopt(h, ctx_);
@@ -327,6 +362,24 @@ void local_std(APR& apr, const ParticleData<In>& in, ParticleData<float>& out, c
     const int32_t sz[3] = {size[0], size[1], size[2]};
     detail::check(aprh_local_std(apr.context().get(), apr.get(), in.data.data(), detail::dtype<In>::code, sz, out.data.data()));
 }
+/* face_neighbour_filter (APRNumerics.hpp:305-355); filter = {w_minus, w_self, w_plus} */
+template <typename In>
+void face_neighbour_filter(APR& apr, const ParticleData<In>& in, ParticleData<float>& out, const std::vector<float>& filter,
+                           int dimension) {
+    if (filter.size() != 3) throw std::invalid_argument("APRNumerics::face_neighbour_filter expects a filter of size 3");
+    out.init(apr.total_number_particles());
+    detail::check(aprh_face_neighbour_filter(apr.context().get(), apr.get(), in.data.data(), detail::dtype<In>::code, filter.data(),
+                                             dimension, out.data.data()));
+}
+/* seperable_face_neighbour_filter (APRNumerics.hpp:286-302) */
+template <typename In>
+void seperable_face_neighbour_filter(APR& apr, const ParticleData<In>& in, ParticleData<float>& out, const std::vector<float>& filter,
+                                     int repeats = 1) {
+    if (filter.size() != 3) throw std::invalid_argument("APRNumerics::seperable_face_neighbour_filter expects a filter of size 3");
+    out.init(apr.total_number_particles());
+    detail::check(aprh_seperable_face_neighbour_filter(apr.context().get(), apr.get(), in.data.data(), detail::dtype<In>::code,
+                                                       filter.data(), repeats, out.data.data()));
+}
 }  // namespace APRNumerics
 
 /* ReconPatch (APRReconstruction.hpp:20-66) */
@@ -387,6 +440,14 @@ void reconstruct_level(APR& apr, PixelData<U>& img, ReconPatch& patch) {
     img.init(patch.y_end - patch.y_begin, patch.x_end - patch.x_begin, patch.z_end - patch.z_begin);
     const aprh_patch p{patch.x_begin, patch.x_end, patch.y_begin, patch.y_end, patch.z_begin, patch.z_end, patch.level_delta};
     detail::check(aprh_reconstruct_level_patch(apr.context().get(), apr.get(), &p, img.mesh.data(), detail::dtype<U>::code));
+}
+/* reconstruct_smooth (APRReconstruction.hpp:270-295); scale_d in (y, x, z) */
+template <typename U, typename T>
+void reconstruct_smooth(APR& apr, PixelData<U>& img, const ParticleData<T>& parts, const std::vector<float>& scale_d = {2, 2, 2}) {
+    if (scale_d.size() != 3) throw std::invalid_argument("APRReconstruction::reconstruct_smooth expects scale_d of size 3");
+    img.init(apr.org_dims(0), apr.org_dims(1), apr.org_dims(2));
+    detail::check(aprh_reconstruct_smooth(apr.context().get(), apr.get(), parts.data.data(), detail::dtype<T>::code, scale_d.data(),
+                                          img.mesh.data(), detail::dtype<U>::code));
 }
 }  // namespace APRReconstruction
 

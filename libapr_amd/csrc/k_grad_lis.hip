@@ -916,6 +916,7 @@ struct GradDsVec {
     static __device__ __forceinline__ void run(const Args& a, int64_t t) {
         /* lanes numbered densely over (y group, x column pair, run of GZC z planes), so short y
            rows still fill the blocks */
+        /* valid only for y_num % 8 == 0, which launch_grad_ds_u16 checks before choosing it */
         const int64_t ny = a.y_num / 8, nz = (a.zd_num + GZC - 1) / GZC;
         const int64_t L = t * blockDim.x + threadIdx.x;
         if (L >= ny * a.xd_num * nz) return;
@@ -994,9 +995,10 @@ void launch_grad_ds_u16(aprh_context* ctx, const uint16_t* img, uint16_t* grad, 
     const int64_t yd = cdiv(y, 2), xd = cdiv(x, 2), zd = cdiv(z, 2);
     const bool aligned = (((uintptr_t)img | (uintptr_t)grad | (uintptr_t)mean) & 15u) == 0;
     if (y % 8 == 0 && aligned) {
-        const int64_t gx = cdiv(y / 8, 256);
+        /* GradDsVec owns 8 consecutive y per lane and numbers its lanes densely, so it needs
+           y % 8 == 0 (tested here) and takes no row-tile count (gx = 0) */
         const bool unit = hx == 1.0f && hy == 1.0f && hz == 1.0f;
-        GradArgs a{img, grad, mean, y, x, z, yd, 2.0f * hx, 2.0f * hy, 2.0f * hz, accum ? 1 : 0, gx, xd, zd};
+        GradArgs a{img, grad, mean, y, x, z, yd, 2.0f * hx, 2.0f * hy, 2.0f * hz, accum ? 1 : 0, 0, xd, zd};
         const int64_t tiles = cdiv((y / 8) * xd * cdiv(zd, GZC), 256);   /* 256 lanes per tile */
         if (unit) launch_tiles<GradDsVec<true>>(ctx, a, tiles);
         else launch_tiles<GradDsVec<false>>(ctx, a, tiles);

@@ -24,6 +24,11 @@ int orc_richardson_lucy_u16(void* h, const uint16_t* in, float* u, const float* 
                             int normalize, int resume, double tol, double* rel);
 void orc_reconstruct(void* h, const void* parts, int pdt, int level_delta, void* img, int idt);
 void orc_local_std_u16(void* h, const uint16_t* in, const int* size, float* out);
+void orc_reconstruct_smooth(void* hp, const void* parts, int pdt, const float* scale3, void* out, int idt);
+void orc_face_neighbour_filter_f32(void* hp, const float* in, const float* filter3, int dim, int repeats, float* out);
+void* orc_get_apr_by_block_u16(const uint16_t* img, int64_t y, int64_t x, int64_t z, const aprh_params* par, int64_t z_block_size,
+                               int64_t ghost_z, char* errbuf, int errlen);
+void orc_sample_image_blocked_u16(void* hp, const uint16_t* img, int64_t z_block_size, int64_t ghost_z, uint16_t* parts);
 void orc_apr_free(void* h);
 }
 
@@ -180,6 +185,60 @@ int main() {
         const int sz3[3] = {3, 3, 3};
         orc_local_std_u16(o, oparts.data(), sz3, ols.data());
         CHECK(same_bytes(ls.data, ols.data(), P));
+    }
+
+    // smooth reconstruction, default scale {2, 2, 2}, float image
+    {
+        PixelData<float> rs;
+        APRReconstruction::reconstruct_smooth(apr, rs, parts);
+        std::vector<float> ors((size_t)(y * x * z));
+        const float sc[3] = {2.0f, 2.0f, 2.0f};
+        orc_reconstruct_smooth(o, oparts.data(), 0, sc, ors.data(), 1);
+        CHECK(same_bytes(rs.mesh, ors.data(), ors.size()));
+    }
+
+    // face-neighbour filters on float particles
+    {
+        ParticleData<float> pf, ff;
+        pf.init(P);
+        for (size_t i = 0; i < P; ++i) pf.data[i] = (float)oparts[i];
+        const std::vector<float> w = {0.25f, 0.5f, 0.25f};
+        std::vector<float> off(P);
+        APRNumerics::face_neighbour_filter(apr, pf, ff, w, 1);
+        orc_face_neighbour_filter_f32(o, pf.data.data(), w.data(), 1, -1, off.data());
+        CHECK(same_bytes(ff.data, off.data(), P));
+        APRNumerics::seperable_face_neighbour_filter(apr, pf, ff, w, 2);
+        orc_face_neighbour_filter_f32(o, pf.data.data(), w.data(), 0, 2, off.data());
+        CHECK(same_bytes(ff.data, off.data(), P));
+    }
+
+    // blocked conversion and sampling: 16-plane blocks with 4 ghost planes
+    {
+        APRConverterBatch bconv(ctx);
+        bconv.par = conv.par;
+        bconv.z_block_size = 16;
+        bconv.ghost_z = 4;
+        APR bapr;
+        CHECK(bconv.get_apr(bapr, img));
+        void* ob = orc_get_apr_by_block_u16(img.mesh.data(), y, x, z, &conv.par, 16, 4, err, sizeof(err));
+        CHECK(ob != nullptr);
+        if (ob) {
+            int64_t binfo[8];
+            orc_apr_info(ob, binfo);
+            const size_t bP = (size_t)binfo[2];
+            CHECK(bapr.total_number_particles() == bP);
+            std::vector<uint64_t> blx, bxz, olx((size_t)binfo[4]), oxz((size_t)binfo[3]);
+            std::vector<uint16_t> byv, oyv(bP);
+            bapr.linear_access(blx, bxz, byv);
+            orc_apr_get(ob, olx.data(), oxz.data(), oyv.data());
+            CHECK(blx == olx && bxz == oxz && byv == oyv);
+            ParticleData<uint16_t> bp;
+            bp.sample_parts_from_img_blocked(bapr, img, 16, 4);
+            std::vector<uint16_t> obp(bP);
+            orc_sample_image_blocked_u16(ob, img.mesh.data(), 16, 4, obp.data());
+            CHECK(same_bytes(bp.data, obp.data(), bP));
+            orc_apr_free(ob);
+        }
     }
 
     // gradients: a constant field has none; a bad dimension throws std::invalid_argument
