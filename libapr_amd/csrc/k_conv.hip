@@ -483,6 +483,8 @@ __global__ __launch_bounds__(CT) void k_conv_sep(ConvArgs<TIn> a) {
     p += (size_t)WX * WYS * 4;
     float* Ab = (float*)p;               /* WX x TY */
     p += (size_t)WX * TY * 4;
+    int32_t* cxo = (int32_t*)p;          /* WX: coarse-plane offset of each window row, -1 none */
+    p += (size_t)((WX + 3) & ~3) * 4;
     uint8_t* pidx = p;                   /* SZR x TX x TY */
 
     const int64_t ntiles = (int64_t)L.ntx * L.nty * L.ntz;
@@ -506,6 +508,23 @@ __global__ __launch_bounds__(CT) void k_conv_sep(ConvArgs<TIn> a) {
         }
     }
     constexpr int nmd = TOP ? WX : 2 * WX;   /* window rows looked up: particle rows, tree rows */
+
+    /* plane-invariant coarse geometry: each window row's offset in a coarse plane (LDS), and
+       this lane's two window columns' coarse y (-1 outside the level) */
+    for (int t = threadIdx.x; t < WX; t += CT) {
+        const int64_t x = x0 - hx + t;
+        const bool ok = a.coarse && (a.reflect || (x >= 0 && x < L.X));
+        cxo[t] = ok ? (int32_t)(((a.reflect ? refl_c(x, L.X) : x) >> 1) * L.cY) : -1;
+    }
+    int32_t cyo[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int yw = lane + 64 * h;
+        const int64_t y = y0 - hy + yw;
+        cyo[h] = (yw < WY && y >= 0 && y < L.Y) ? (int32_t)(y >> 1) : -1;
+    }
+    const int64_t cplane = L.cX * L.cY;
+    __syncthreads();
 
     double n1 = 0.0, n2 = 0.0;
     float breg[RPT][SZ];
@@ -566,21 +585,17 @@ __global__ __launch_bounds__(CT) void k_conv_sep(ConvArgs<TIn> a) {
             if (lane == 0) mpre[nmd] = carry;
         }
         /* the coarse image under the window: a wave per window row, lanes along y (a second
-           pass for the y halo beyond 64 cells), four rows' loads in flight before their stores */
+           pass for the y halo beyond 64 cells), four rows' loads in flight before their stores;
+           offsets within the coarse plane come from the plane-invariant tables */
+        const float* cpl = (plane_zero || !a.coarse) ? nullptr : a.coarse + (zs >> 1) * cplane;
         for (int xb = wv; xb < WX; xb += 4 * NWAVE) {
             float cv[4][2];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int xw = xb + u * NWAVE;
-                const int64_t x = x0 - hx + xw;
-                const bool rin = xw < WX && !plane_zero && a.coarse && (a.reflect || (x >= 0 && x < L.X));
-                const float* crow = rin ? a.coarse + ((zs >> 1) * L.cX + ((a.reflect ? refl_c(x, L.X) : x) >> 1)) * L.cY : nullptr;
+                const int32_t xo = (cpl && xw < WX) ? cxo[xw] : -1;
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int yw = lane + 64 * h;
-                    const int64_t y = y0 - hy + yw;
-                    cv[u][h] = (crow && yw < WY && y >= 0 && y < L.Y) ? crow[y >> 1] : 0.0f;
-                }
+                for (int h = 0; h < 2; ++h) cv[u][h] = (xo >= 0 && cyo[h] >= 0) ? cpl[xo + cyo[h]] : 0.0f;
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
@@ -754,7 +769,7 @@ __global__ __launch_bounds__(CT) void k_conv_sep(ConvArgs<TIn> a) {
 size_t conv_sep_lds_bytes(int tx, int sy, int sx, int sz) {
     const int WX = tx + (sx - 1), WYS = TY + (sy - 1), SZR = sz + 1;
     size_t b = (size_t)SZR * tx * 8 + (size_t)2 * WX * 8 + (size_t)(2 * WX + 2) * 4 + (size_t)WX * WYS * 4 + (size_t)WX * TY * 4 +
-               (size_t)SZR * tx * TY;
+               (size_t)((WX + 3) & ~3) * 4 + (size_t)SZR * tx * TY;
     return std::max<size_t>(b, 2 * CT * sizeof(double));
 }
 
@@ -1075,7 +1090,8 @@ void conv_levels(aprh_context* ctx, aprh_apr& a, const std::vector<HostStencil>&
         L.sz = s.sz;
         pl.sep = off_sep[(size_t)sn] >= 0;
         L.w_off = pl.sep ? off_sep[(size_t)sn] : off_exact[(size_t)sn];
-        pl.regz = pl.sep && sep_regz_size(s.sz) && s.sy == s.sz && s.sx == s.sz;
+        /* the register-ring kernel addresses a coarse plane with 32-bit offsets */
+        pl.regz = pl.sep && sep_regz_size(s.sz) && s.sy == s.sz && s.sx == s.sz && L.cX * L.cY <= 0x7FFFFFFF;
         if (pl.regz) {
             /* 16-row tiles: more blocks per CU hide the plane's global reads better than the
                smaller x halo of 32-row tiles saves (measured on the 9x9x9 RL workload) */
