@@ -94,16 +94,23 @@ __device__ __forceinline__ int64_t refl_c(int64_t q, int64_t n) {
 /* particles of row `rid` (an xz_end index): [b, e), and the part [s0, s1) with y in block yb */
 __device__ __forceinline__ void row_block(const uint64_t* __restrict__ xz, const uint16_t* __restrict__ yidx, int64_t yrow, int nyb,
                                           int64_t rid, int yb, uint64_t& b, uint64_t& e, uint64_t& s0, uint64_t& s1) {
-    b = xz[rid - 1];
-    e = xz[rid];
     if (nyb <= 1) {
+        b = xz[rid - 1];
+        e = xz[rid];
         s0 = b;
         s1 = e;
         return;
     }
+    /* both index entries are read (clamped to the row's nyb - 1 entries) together with the row
+       ends, so the three loads are in flight at once */
     const uint16_t* ix = yidx + yrow * (int64_t)(nyb - 1);
-    s0 = yb > 0 ? b + ix[yb - 1] : b;
-    s1 = (yb + 1 < nyb) ? b + ix[yb] : e;
+    const uint64_t m0 = yb > 0 ? ~(uint64_t)0 : 0, m1 = yb + 1 < nyb ? ~(uint64_t)0 : 0;
+    const uint16_t v0 = ix[yb > 0 ? yb - 1 : 0], v1 = ix[yb < nyb - 2 ? yb : nyb - 2];
+    b = xz[rid - 1];
+    e = xz[rid];
+    /* selections as masks, not branches, so no wait falls between the loads */
+    s0 = b + ((uint64_t)v0 & m0);
+    s1 = e + ((b + v1 - e) & m1);
 }
 
 template <typename T> __device__ __forceinline__ float tof(T v) { return (float)v; }
@@ -457,6 +464,13 @@ __global__ __launch_bounds__(CT) void k_conv_tile(ConvArgs<TIn> a) {
     }
 }
 
+/* smallest power of two above n */
+constexpr int pow2_above(int n) {
+    int p = 1;
+    while (p <= n) p <<= 1;
+    return p;
+}
+
 /* Separable mode with the z pass in registers.  Each thread owns one y (its lane) in TX / 4
    rows of the tile and keeps the x-passed values of the last SZ planes of those cells in
    registers; a per-plane map from interior cells to their particle (offset in the row's block,
@@ -477,8 +491,11 @@ __global__ __launch_bounds__(CT) void k_conv_sep(ConvArgs<TIn> a) {
     p += (size_t)SZR * TX * 8;
     uint64_t* mlo = (uint64_t*)p;        /* 2 WX */
     p += (size_t)2 * WX * 8;
-    int* mpre = (int*)p;                 /* 2 WX + 2 */
-    p += (size_t)(2 * WX + 2) * 4;
+    /* exclusive prefix of the window rows' load counts, then the total, then INT_MAX up to a
+       power of two: the row search runs a fixed number of steps */
+    constexpr int NPM = pow2_above(2 * WX);
+    int* mpre = (int*)p;                 /* NPM + 2 */
+    p += (size_t)(NPM + 2) * 4;
     float* V = (float*)p;                /* WX x WYS */
     p += (size_t)WX * WYS * 4;
     float* Ab = (float*)p;               /* WX x TY */
@@ -508,6 +525,8 @@ __global__ __launch_bounds__(CT) void k_conv_sep(ConvArgs<TIn> a) {
         }
     }
     constexpr int nmd = TOP ? WX : 2 * WX;   /* window rows looked up: particle rows, tree rows */
+    constexpr int NP = pow2_above(nmd);
+    for (int t = nmd + 1 + threadIdx.x; t < NP; t += CT) mpre[t] = 0x7FFFFFFF;
 
     /* plane-invariant coarse geometry: each window row's offset in a coarse plane (LDS), and
        this lane's two window columns' coarse y (-1 outside the level) */
@@ -618,12 +637,11 @@ __global__ __launch_bounds__(CT) void k_conv_sep(ConvArgs<TIn> a) {
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int k = k0 + u * CT;
-                    int lo = 0, hi = nmd;
-                    while (hi - lo > 1) {
-                        const int mid = (lo + hi) >> 1;
-                        if (mpre[mid] <= k) lo = mid;
-                        else hi = mid;
-                    }
+                    /* the last row r with mpre[r] <= k, in log2(NP) fixed steps */
+                    int lo = 0;
+#pragma unroll
+                    for (int st = NP / 2; st >= 1; st >>= 1)
+                        if (mpre[lo + st] <= k) lo += st;
                     rr[u] = k < total ? lo : -1;
                     ii[u] = k < total ? mlo[lo] + (uint64_t)(k - mpre[lo]) : 0;
                 }
@@ -768,7 +786,7 @@ __global__ __launch_bounds__(CT) void k_conv_sep(ConvArgs<TIn> a) {
 
 size_t conv_sep_lds_bytes(int tx, int sy, int sx, int sz) {
     const int WX = tx + (sx - 1), WYS = TY + (sy - 1), SZR = sz + 1;
-    size_t b = (size_t)SZR * tx * 8 + (size_t)2 * WX * 8 + (size_t)(2 * WX + 2) * 4 + (size_t)WX * WYS * 4 + (size_t)WX * TY * 4 +
+    size_t b = (size_t)SZR * tx * 8 + (size_t)2 * WX * 8 + (size_t)(pow2_above(2 * WX) + 2) * 4 + (size_t)WX * WYS * 4 + (size_t)WX * TY * 4 +
                (size_t)((WX + 3) & ~3) * 4 + (size_t)SZR * tx * TY;
     return std::max<size_t>(b, 2 * CT * sizeof(double));
 }
