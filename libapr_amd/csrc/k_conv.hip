@@ -195,7 +195,7 @@ __global__ __launch_bounds__(CT) void k_conv_tile(ConvArgs<TIn> a) {
     const int64_t nq = (z_hi - z_lo) + 2 * hz;
     for (int64_t q = 0; q < nq; ++q) {
         const int64_t zq = z_lo - hz + q;
-        const int slot = (int)(q % sz);
+        const int slot = (int)q % sz;
         float* V = SEP ? s.V : s.ring + (size_t)slot * WX * WYS;
         uint8_t* ylist = s.ylist + (size_t)slot * TX * TY;
         int* cnt = s.cnt + slot * TX;
@@ -378,7 +378,7 @@ __global__ __launch_bounds__(CT) void k_conv_tile(ConvArgs<TIn> a) {
            flow (inactive lanes compute on cell 0 and store nothing) */
         if (q >= 2 * hz) {
             const int64_t qo = q - 2 * hz;           /* window plane of z = zo - hz */
-            const int oslot = (int)((qo + hz) % sz); /* plane zo itself             */
+            const int oslot = ((int)qo + hz) % sz;   /* plane zo itself (32-bit: q < Z + 2 hz) */
             /* the plane's particles numbered over the tile's rows (row r holds cnt[r]), one lane
                each; SEP keeps a wave per row */
             int ntot = 0;
@@ -417,7 +417,7 @@ __global__ __launch_bounds__(CT) void k_conv_tile(ConvArgs<TIn> a) {
                 } else {
                     CWeights w = Wt;
                     for (int iz = 0; iz < sz; ++iz) {
-                        const float* pl = s.ring + (size_t)((qo + iz) % sz) * WX * WYS + r * WYS + yo;
+                        const float* pl = s.ring + (size_t)(((int)qo + iz) % sz) * WX * WYS + r * WYS + yo;
                         for (int ix = 0; ix < L.sx; ++ix) {
                             val = row_taps<SY>(pl + ix * WYS, w, L.sy, val);
                             w += L.sy;
