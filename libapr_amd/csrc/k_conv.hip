@@ -1136,9 +1136,12 @@ void conv_levels(aprh_context* ctx, aprh_apr& a, const std::vector<HostStencil>&
             if (nxy * cdiv(L.Z, zmin) >= 1024) zc = std::max(zc, zmin);
             zc = std::max<int64_t>(1, std::min<int64_t>(zc, L.Z));
         } else {
+            /* about 8192 blocks; runs of at least 4 hz planes unless that leaves fewer than ~2048
+               (a block marches its planes one after another, so a small level wants short runs) */
             zc = cdiv(L.Z * nxy, 8192);
-            zc = std::max<int64_t>(zc, std::min<int64_t>(L.Z, std::max(8, 4 * hz)));
-            zc = std::min<int64_t>(zc, L.Z);
+            const int64_t zmin = std::min<int64_t>(L.Z, std::max(8, 4 * hz));
+            if (nxy * cdiv(L.Z, zmin) >= 2048) zc = std::max(zc, zmin);
+            zc = std::max<int64_t>(1, std::min<int64_t>(zc, L.Z));
         }
         L.zc = (int)zc;
         L.ntz = (int)cdiv(L.Z, zc);
