@@ -1136,11 +1136,12 @@ void conv_levels(aprh_context* ctx, aprh_apr& a, const std::vector<HostStencil>&
             if (nxy * cdiv(L.Z, zmin) >= 1024) zc = std::max(zc, zmin);
             zc = std::max<int64_t>(1, std::min<int64_t>(zc, L.Z));
         } else {
-            /* about 8192 blocks; runs of at least 4 hz planes unless that leaves fewer than ~2048
-               (a block marches its planes one after another, so a small level wants short runs) */
-            zc = cdiv(L.Z * nxy, 8192);
+            /* at most about 8192 blocks, and at least about 1024 unless runs would exceed 4 hz
+               planes: a block marches its planes one after another, so a small level wants short
+               runs, a large one runs long enough to amortise the 2 hz halo planes */
+            const int64_t W = L.Z * nxy;
             const int64_t zmin = std::min<int64_t>(L.Z, std::max(8, 4 * hz));
-            if (nxy * cdiv(L.Z, zmin) >= 2048) zc = std::max(zc, zmin);
+            zc = std::max(cdiv(W, 8192), std::min(zmin, cdiv(W, 1024)));
             zc = std::max<int64_t>(1, std::min<int64_t>(zc, L.Z));
         }
         L.zc = (int)zc;

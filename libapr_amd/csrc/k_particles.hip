@@ -168,15 +168,13 @@ __global__ void k_rows_to_xz(Rows R, const uint64_t* __restrict__ excl, const ui
 }
 
 /* ---------------- dense level images ---------------- */
-/* values (and optionally a type tag) of every particle / tree node of one level into a dense image */
+/* values (and optionally a type tag) of every particle / tree node of one level into a dense
+   image: wave `wid` of `nw` takes rows wid, wid + nw, ..., its lanes the row's entries */
 template <typename T, typename D>
-__global__ __launch_bounds__(256) void k_scatter_rows(Rows R, int li, const uint64_t* __restrict__ xz_end,
-                                                      const uint16_t* __restrict__ y_vec, const T* __restrict__ vals, D* V,
-                                                      uint8_t* Ty, uint8_t type) {
+__device__ __forceinline__ void scatter_rows_body(const Rows& R, int li, const uint64_t* __restrict__ xz_end,
+                                                  const uint16_t* __restrict__ y_vec, const T* __restrict__ vals, D* V, uint8_t* Ty,
+                                                  uint8_t type, int64_t wid, int64_t nw, int lane) {
     const int64_t nrows = R.x_num[li] * R.z_num[li];
-    const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
-    const int lane = threadIdx.x & 63;
-    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / 64;
     for (int64_t row = wid; row < nrows; row += nw) {
         const uint64_t b = xz_end[R.row_begin[li] + row - 1], e = xz_end[R.row_begin[li] + row];
         D* vr = V + row * R.y_num[li];
@@ -188,16 +186,21 @@ __global__ __launch_bounds__(256) void k_scatter_rows(Rows R, int li, const uint
         }
     }
 }
+template <typename T, typename D>
+__global__ __launch_bounds__(256) void k_scatter_rows(Rows R, int li, const uint64_t* __restrict__ xz_end,
+                                                      const uint16_t* __restrict__ y_vec, const T* __restrict__ vals, D* V,
+                                                      uint8_t* Ty, uint8_t type) {
+    scatter_rows_body<T, D>(R, li, xz_end, y_vec, vals, V, Ty, type, ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64,
+                            ((int64_t)gridDim.x * blockDim.x) / 64, threadIdx.x & 63);
+}
 
 /* one tree level of fill_tree_mean: children (level t+1) are in the dense V/Ty scratch */
-__global__ __launch_bounds__(256) void k_tree_mean_level(Rows TR, int ti, const uint64_t* __restrict__ txz, const uint16_t* __restrict__ ty,
-                                                         const float* __restrict__ V, const uint8_t* __restrict__ Ty, int64_t cyn,
-                                                         int64_t cxn, int64_t czn, float* tree) {
+__device__ __forceinline__ void tree_mean_level_body(const Rows& TR, int ti, const uint64_t* __restrict__ txz,
+                                                     const uint16_t* __restrict__ ty, const float* __restrict__ V,
+                                                     const uint8_t* __restrict__ Ty, int64_t cyn, int64_t cxn, int64_t czn, float* tree,
+                                                     int64_t wid, int64_t nw, int lane) {
     const int64_t pxn = TR.x_num[ti], pzn = TR.z_num[ti], pyn = TR.y_num[ti];
     const int64_t nrows = pxn * pzn;
-    const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
-    const int lane = threadIdx.x & 63;
-    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) / 64;
     for (int64_t row = wid; row < nrows; row += nw) {
         const int64_t zp = row / pxn, xp = row % pxn;
         const uint64_t b = txz[TR.row_begin[ti] + row - 1], e = txz[TR.row_begin[ti] + row];
@@ -221,6 +224,42 @@ __global__ __launch_bounds__(256) void k_tree_mean_level(Rows TR, int ti, const 
             }
             tree[i] = acc;
         }
+    }
+}
+__global__ __launch_bounds__(256) void k_tree_mean_level(Rows TR, int ti, const uint64_t* __restrict__ txz, const uint16_t* __restrict__ ty,
+                                                         const float* __restrict__ V, const uint8_t* __restrict__ Ty, int64_t cyn,
+                                                         int64_t cxn, int64_t czn, float* tree) {
+    tree_mean_level_body(TR, ti, txz, ty, V, Ty, cyn, cxn, czn, tree, ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64,
+                         ((int64_t)gridDim.x * blockDim.x) / 64, threadIdx.x & 63);
+}
+
+/* the block's writes to global memory visible to its own later reads: a workgroup-scope fence
+   and barrier (__syncthreads).  All waves of a workgroup run on one CU and share its vector L1,
+   so no cache maintenance is needed at this scope (an agent-scope fence would write back L2) */
+__device__ __forceinline__ void block_sync_global() { __syncthreads(); }
+
+/* the small tree levels t_hi .. 0 in one block of 1024 threads (each level's rows are few, so
+   separate launches would cost more than their work): per level, clear the type tags of the
+   children level, scatter its particles and tree nodes, take the means; the same bodies as the
+   per-level kernels, so the results are identical */
+template <typename T>
+__global__ __launch_bounds__(1024) void k_tree_small(Rows AR, Rows TR, int t_hi, int l_min, int t_l_max, const uint64_t* __restrict__ xz_end,
+                                                     const uint16_t* __restrict__ y_vec, const T* __restrict__ parts,
+                                                     const uint64_t* __restrict__ txz, const uint16_t* __restrict__ ty, float* tree, float* V,
+                                                     uint8_t* Ty) {
+    const int64_t wid = threadIdx.x / 64, nw = blockDim.x / 64;
+    const int lane = threadIdx.x & 63;
+    for (int t = t_hi; t >= 0; --t) {
+        const int c = t + 1;
+        const int64_t cyn = TR.y_num[c], cxn = TR.x_num[c], czn = TR.z_num[c];
+        const int64_t n = cyn * cxn * czn;
+        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) Ty[i] = 0;
+        block_sync_global();
+        if (c >= l_min) scatter_rows_body<T, float>(AR, c - l_min, xz_end, y_vec, parts, V, Ty, (uint8_t)1, wid, nw, lane);
+        if (c <= t_l_max) scatter_rows_body<float, float>(TR, c, txz, ty, (const float*)tree, V, Ty, (uint8_t)2, wid, nw, lane);
+        block_sync_global();
+        tree_mean_level_body(TR, t, txz, ty, V, Ty, cyn, cxn, czn, tree, wid, nw, lane);
+        block_sync_global();
     }
 }
 
@@ -397,7 +436,14 @@ void tree_mean(aprh_context* ctx, aprh_apr& a, const T* parts, float* tree) {
     if (t_first < 0) return;
     const size_t maxn = level_elems(a, t_first + 1);
     Buf V(ctx->pool, maxn * 4), Ty(ctx->pool, maxn);
-    for (int t = t_first; t >= 0; --t) {
+    /* tree levels whose children level has at most 256 rows go to one single-block launch */
+    int t_small = -1;
+    for (int t = t_first; t >= 0; --t)
+        if (a.x_num[t + 1] * a.z_num[t + 1] <= 256) {
+            t_small = t;
+            break;
+        }
+    for (int t = t_first; t > t_small; --t) {
         const int c = t + 1;   /* children level; APR and tree levels share the level geometry */
         const int64_t cyn = a.y_num[c], cxn = a.x_num[c], czn = a.z_num[c];
         HIP_CHECK(hipMemsetAsync(Ty.ptr(), 0, level_elems(a, c), st));
@@ -413,6 +459,12 @@ void tree_mean(aprh_context* ctx, aprh_apr& a, const T* parts, float* tree) {
         }
         hipLaunchKernelGGL(k_tree_mean_level, kGrid, kBlock, 0, st, TR, t, a.t_xz_end.as<uint64_t>(), a.t_y_vec.as<uint16_t>(),
                            V.as<float>(), Ty.as<uint8_t>(), cyn, cxn, czn, tree);
+        HIP_LAUNCH_CHECK();
+    }
+    if (t_small >= 0) {
+        hipLaunchKernelGGL((k_tree_small<T>), dim3(1), dim3(1024), 0, st, AR, TR, t_small, a.l_min, a.t_l_max, a.xz_end.as<uint64_t>(),
+                           a.y_vec.as<uint16_t>(), parts, a.t_xz_end.as<uint64_t>(), a.t_y_vec.as<uint16_t>(), tree, V.as<float>(),
+                           Ty.as<uint8_t>());
         HIP_LAUNCH_CHECK();
     }
 }
