@@ -478,7 +478,7 @@ __device__ __forceinline__ void mean_y_body(int mode, const float* __restrict__ 
         if (O > 0 && p0 >= int_lo && p0 <= int_hi) {
             /* loads go out in batches of LB before the first is used (lines past the end of the
                block's range read its last line and are not stored) */
-            constexpr int LB = 16;
+            constexpr int LB = 32;
             const int64_t ys0 = p0 + O - py + kk;
             const float* arow = A + line0 * yq + p0 + O + kk;
 #pragma unroll 1
