@@ -11,7 +11,7 @@
  *   APRTreeNumerics::fill_tree_mean      numerics/APRTreeNumerics.hpp:19
  *   APRStencil::*                        numerics/APRStencil.hpp:12-244
  *   APRFilter::convolve(_pencil)         numerics/APRFilter.hpp:67-149
- *   APRNumerics::richardson_lucy         numerics/APRNumerics.hpp:126-134
+ *   APRNumerics::richardson_lucy(_tv)    numerics/APRNumerics.hpp:126-148
  *   APRNumerics::gradient_* (cfd, sobel, magnitude)  numerics/APRNumerics.hpp:165-283
  *   APRNumerics::local_std               numerics/APRNumerics.hpp:357-389
  *   APRNumerics::(seperable_)face_neighbour_filter  numerics/APRNumerics.hpp:286-355
@@ -328,6 +328,17 @@ aprh_rl_stats richardson_lucy(APR& apr, ParticleData<In>& in, ParticleData<float
                                        psf.mesh.data(), (int32_t)psf.y_num, (int32_t)psf.x_num, (int32_t)psf.z_num, number_iterations,
                                        use_stencil_downsample ? 1 : 0, normalize ? 1 : 0, resume ? 1 : 0, tol, &st));
     return st;
+}
+
+/* richardson_lucy_tv (APRNumerics.hpp:142-148, :549-572) */
+template <typename In>
+void richardson_lucy_tv(APR& apr, ParticleData<In>& in, ParticleData<float>& out, PixelData<float>& psf, int number_iterations,
+                        float reg_factor, bool use_stencil_downsample, bool normalize, bool resume) {
+    if (!resume) out.init(apr.total_number_particles());
+    detail::check(aprh_richardson_lucy_tv(apr.context().get(), apr.get(), in.data.data(), detail::dtype<In>::code, out.data.data(),
+                                          psf.mesh.data(), (int32_t)psf.y_num, (int32_t)psf.x_num, (int32_t)psf.z_num,
+                                          number_iterations, reg_factor, use_stencil_downsample ? 1 : 0, normalize ? 1 : 0,
+                                          resume ? 1 : 0));
 }
 
 /* gradient_cfd / gradient_sobel (APRNumerics.hpp:165-213); std::invalid_argument on a bad

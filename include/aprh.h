@@ -229,6 +229,13 @@ int aprh_convolve_stencil(aprh_context* ctx, aprh_apr* apr, const float* stencil
 int aprh_richardson_lucy(aprh_context* ctx, aprh_apr* apr, const void* parts_in, int dtype, float* u, const float* psf, int32_t sy,
                          int32_t sx, int32_t sz, int32_t max_iter, int use_stencil_downsample, int normalize, int resume, double tol,
                          aprh_rl_stats* stats);
+/* APRNumerics::richardson_lucy_tv (APRNumerics.hpp:142-148, :489-525, :549-572): Richardson-Lucy
+   with total-variation regularisation; each iteration's update is divided by
+   1 - reg_factor * div(grad u / |grad u|) (div_norm_grad, :431-486, unit pixel sizes).  u
+   (f32[total]) is set to 1 unless resume; exactly number_iterations iterations. */
+int aprh_richardson_lucy_tv(aprh_context* ctx, aprh_apr* apr, const void* parts_in, int dtype, float* u, const float* psf, int32_t sy,
+                            int32_t sx, int32_t sz, int32_t number_iterations, float reg_factor, int use_stencil_downsample, int normalize,
+                            int resume);
 /* APRReconstruction::reconstruct_constant (APRReconstruction.hpp:80-120; with level_delta < 0
    the full-volume patch form :297-394, tree means from fill_tree_mean).  image has the
    dimensions of level l_max + level_delta, (y, x, z) with y fastest. */

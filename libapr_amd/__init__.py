@@ -146,6 +146,8 @@ def lib():
                                         C.c_int]
     L.aprh_richardson_lucy.argtypes = [vp, vp, vp, C.c_int, vp, vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int,
                                        C.c_int, C.c_int, C.c_double, C.POINTER(AprhRLStats)]
+    L.aprh_richardson_lucy_tv.argtypes = [vp, vp, vp, C.c_int, vp, vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_float,
+                                          C.c_int, C.c_int, C.c_int]
     L.aprh_reconstruct_constant.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp, C.c_int]
     L.aprh_reconstruct_level.argtypes = [vp, vp, vp, C.c_int]
     L.aprh_reconstruct_smooth.argtypes = [vp, vp, vp, C.c_int, C.POINTER(C.c_float), vp, C.c_int]
@@ -859,6 +861,25 @@ class APRNumerics:
                                           int(resume), float(tol), C.byref(stats)))
         out = particle_output[:apr.total] if isinstance(particle_output, np.ndarray) else particle_output
         return out, {"iterations": stats.iterations, "last_rel_update": stats.last_rel_update}
+
+    @staticmethod
+    def richardson_lucy_tv(apr, particle_input, particle_output, psf, number_iterations, reg_factor, use_stencil_downsample=True,
+                           normalize=False, resume=False, dtype=None):
+        """APRNumerics::richardson_lucy_tv (APRNumerics.hpp:489-525, :549-572): Richardson-Lucy with
+        total-variation regularisation.  particle_output: float32[total] (host or device) or None;
+        returns the output."""
+        parts = _host_parts(particle_input)
+        code = _dtype_of(parts, dtype)
+        if particle_output is None:
+            if resume:
+                raise ValueError("resume needs the previous estimate")
+            particle_output = np.zeros(max(apr.total, 1), np.float32)
+        st = np.ascontiguousarray(psf, np.float32)
+        z, x, y = st.shape
+        _check(lib().aprh_richardson_lucy_tv(apr.ctx.h, apr.h, _ptr(parts), code, _ptr(particle_output), _ptr(st), y, x, z,
+                                             int(number_iterations), float(reg_factor), int(use_stencil_downsample),
+                                             int(normalize), int(resume)))
+        return particle_output[:apr.total] if isinstance(particle_output, np.ndarray) else particle_output
 
 
 class ReconPatch:

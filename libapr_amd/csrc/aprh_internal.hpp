@@ -348,6 +348,10 @@ void reduce_norm_parts(aprh_context* ctx, const double* part, int64_t nparts, do
 /* elementwise steps of APRNumerics::local_std: mode 0 p = float(u16 in), 1 p = p * p,
    2 p = sqrtf(max(p - m * m, 0)) */
 void local_std_step(aprh_context* ctx, int mode, float* p, const void* in_or_m, uint64_t n);
+/* elementwise steps of APRNumerics::richardson_lucy_tv: mode 0 normalise (p = g_y, q = g_x,
+   r = g_z; q / r may be nullptr), 1 p = q + p, 2 p = p * q / (1 - reg * r) */
+void rl_tv_step(aprh_context* ctx, int mode, float* p, float* q, float* r, float reg, uint64_t n);
+void fill_f32(aprh_context* ctx, float* p, size_t n, float v);
 template <typename T>
 int richardson_lucy(aprh_context* ctx, aprh_apr& a, const T* in, float* u, const std::vector<HostStencil>& psf,
                     const std::vector<HostStencil>& psf_flipped, int max_iter, bool resume, double tol, double* last_rel);

@@ -113,6 +113,46 @@ void gradient_magnitude(aprh_context* ctx, aprh_apr& a, const T* in, const float
     gradient_magnitude_combine(ctx, out, a.org[1] > 1 ? gx.as<float>() : nullptr, a.org[2] > 1 ? gz.as<float>() : nullptr, a.total);
 }
 
+/* APRNumerics::richardson_lucy_tv (APRNumerics.hpp:489-525): Richardson-Lucy as
+   richardson_lucy (the re-blur's quotient in its convolution's epilogue), then div_norm_grad
+   (:431-486, unit pixel sizes) of u: central differences along the axes present, normalised by
+   their magnitude, and the sum of their differences; u = u * correction / (1 - reg * div) */
+template <typename T>
+void richardson_lucy_tv(aprh_context* ctx, aprh_apr& a, const T* in, float* u, const std::vector<HostStencil>& psf,
+                        const std::vector<HostStencil>& psf_flipped, int niter, float reg, bool resume) {
+    const size_t P = a.total;
+    if (!resume) fill_f32(ctx, u, P, 1.0f);
+    if (P == 0) return;
+    ensure_tree(ctx, a);
+    const bool sep = ctx->conv_mode == APRH_CONV_SEPARABLE;
+    const bool in_u16 = std::is_same<T, uint16_t>::value;
+    const bool has_x = a.org[1] > 1, has_z = a.org[2] > 1;
+    Buf blur(ctx->pool, P * 4), err(ctx->pool, P * 4), gy(ctx->pool, P * 4), dv(ctx->pool, P * 4), tmp(ctx->pool, P * 4);
+    Buf gx, gz, tree(ctx->pool, std::max<uint64_t>(a.t_total, 1) * 4);
+    if (has_x) gx.reset(ctx->pool, P * 4);
+    if (has_z) gz.reset(ctx->pool, P * 4);
+    for (int it = 0; it < niter; ++it) {
+        tree_mean<float>(ctx, a, u, tree.as<float>());
+        conv_levels<float>(ctx, a, psf_flipped, sep, u, tree.as<float>(), blur.as<float>(), true, 1, in, in_u16, nullptr, nullptr);
+        tree_mean<float>(ctx, a, blur.as<float>(), tree.as<float>());
+        conv_levels<float>(ctx, a, psf, sep, blur.as<float>(), tree.as<float>(), err.as<float>(), true, 0, nullptr, false, nullptr, nullptr);
+        gradient_along<float>(ctx, a, u, 0, 1.0f, false, gy.as<float>());
+        if (has_x) gradient_along<float>(ctx, a, u, 1, 1.0f, false, gx.as<float>());
+        if (has_z) gradient_along<float>(ctx, a, u, 2, 1.0f, false, gz.as<float>());
+        rl_tv_step(ctx, 0, gy.as<float>(), has_x ? gx.as<float>() : nullptr, has_z ? gz.as<float>() : nullptr, 0.0f, P);
+        gradient_along<float>(ctx, a, gy.as<float>(), 0, 1.0f, false, dv.as<float>());
+        if (has_x) {
+            gradient_along<float>(ctx, a, gx.as<float>(), 1, 1.0f, false, tmp.as<float>());
+            rl_tv_step(ctx, 1, dv.as<float>(), tmp.as<float>(), nullptr, 0.0f, P);
+        }
+        if (has_z) {
+            gradient_along<float>(ctx, a, gz.as<float>(), 2, 1.0f, false, tmp.as<float>());
+            rl_tv_step(ctx, 1, dv.as<float>(), tmp.as<float>(), nullptr, 0.0f, P);
+        }
+        rl_tv_step(ctx, 2, u, err.as<float>(), dv.as<float>(), reg, P);
+    }
+}
+
 void check_apr(aprh_context* ctx, aprh_apr* a) {
     check_ctx(ctx);
     if (!a) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL apr");
@@ -330,6 +370,39 @@ int aprh_richardson_lucy(aprh_context* ctx, aprh_apr* a, const void* parts_in, i
         stats->iterations = it;
         stats->last_rel_update = rel;
     }
+    APRH_CATCH
+}
+
+int aprh_richardson_lucy_tv(aprh_context* ctx, aprh_apr* a, const void* parts_in, int dtype, float* u, const float* psf, int32_t sy,
+                            int32_t sx, int32_t sz, int32_t number_iterations, float reg_factor, int use_stencil_downsample, int normalize,
+                            int resume) {
+    APRH_TRY
+    check_apr(ctx, a);
+    if (!parts_in || !u) throw Error(APRH_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (number_iterations < 0) throw Error(APRH_ERR_INVALID_ARGUMENT, "negative iteration count");
+    const size_t es = dtype_size(dtype);
+    /* APRNumerics.hpp:549-572: flipped PSF and the two stencil stacks */
+    const HostStencil h = host_stencil(psf, sy, sx, sz);
+    HostStencil hf = h;
+    std::reverse(hf.w.begin(), hf.w.end());
+    const int nst = n_stencil_levels(*a, use_stencil_downsample);
+    const std::vector<HostStencil> sv = downsampled_stencils(h, nst, normalize != 0);
+    const std::vector<HostStencil> fv = downsampled_stencils(hf, nst, normalize != 0);
+    DevIn in(ctx, parts_in, std::max<uint64_t>(a->total, 1) * es);
+    const bool u_dev = is_device_ptr(u);
+    Buf ubuf;
+    float* du = u;
+    if (!u_dev) {
+        ubuf.reset(ctx->pool, std::max<uint64_t>(a->total, 1) * 4);
+        du = ubuf.as<float>();
+        if (resume && a->total) HIP_CHECK(hipMemcpyAsync(du, u, a->total * 4, hipMemcpyHostToDevice, ctx->stream));
+    }
+    if (dtype == APRH_DTYPE_U16)
+        richardson_lucy_tv<uint16_t>(ctx, *a, (const uint16_t*)in.ptr, du, sv, fv, number_iterations, reg_factor, resume != 0);
+    else
+        richardson_lucy_tv<float>(ctx, *a, (const float*)in.ptr, du, sv, fv, number_iterations, reg_factor, resume != 0);
+    if (!u_dev && a->total) HIP_CHECK(hipMemcpyAsync(u, du, a->total * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
     APRH_CATCH
 }
 

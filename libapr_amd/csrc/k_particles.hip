@@ -831,6 +831,57 @@ struct LocalStdStep {
 };
 }  // namespace
 
+namespace {
+/* elementwise steps of APRNumerics::richardson_lucy_tv (APRNumerics.hpp:431-525), in the
+   reference's operation order:
+   mode 0, div_norm_grad's normalisation: m = sqrtf(g_z^2 + g_x^2 + g_y^2) (absent axes are
+           zero), each component divided by m where m > 1e-6 (compared in double);
+   mode 1, the divergence sum: a = b + a;
+   mode 2, the update: a = a * b / (1 - reg * c) */
+struct TvStep {
+    struct Args {
+        int mode;
+        float* a;
+        float* b;   /* mode 0: g_x or nullptr; 1, 2: read only */
+        float* c;   /* mode 0: g_z or nullptr; 2: read only     */
+        float reg;
+        uint64_t n;
+        int64_t G;
+    };
+    static constexpr int THREADS = 256;
+    static const char* name() { return "rl_tv_step"; }
+    static __device__ __forceinline__ void run(const Args& a, int64_t t) {
+        for (uint64_t i = (uint64_t)t * blockDim.x + threadIdx.x; i < a.n; i += (uint64_t)a.G * blockDim.x) {
+            if (a.mode == 0) {
+                const float y = a.a[i], x = a.b ? a.b[i] : 0.0f, z = a.c ? a.c[i] : 0.0f;
+                const float m = sqrtf(z * z + x * x + y * y);
+                if ((double)m > 1e-6) {
+                    if (a.c) a.c[i] = z / m;
+                    if (a.b) a.b[i] = x / m;
+                    a.a[i] = y / m;
+                }
+            } else if (a.mode == 1) {
+                a.a[i] = a.b[i] + a.a[i];
+            } else {
+                a.a[i] = a.a[i] * a.b[i] / (1.0f - a.reg * a.c[i]);
+            }
+        }
+    }
+};
+}  // namespace
+
+void rl_tv_step(aprh_context* ctx, int mode, float* p, float* q, float* r, float reg, uint64_t n) {
+    if (n == 0) return;
+    const int64_t G = stride_tiles((int64_t)n, 256 * 4);
+    launch_tiles<TvStep>(ctx, {mode, p, q, r, reg, n, G}, G);
+}
+
+void fill_f32(aprh_context* ctx, float* p, size_t n, float v) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill_f32, kGrid, kBlock, 0, ctx->stream, p, n, v);
+    HIP_LAUNCH_CHECK();
+}
+
 void local_std_step(aprh_context* ctx, int mode, float* p, const void* q, uint64_t n) {
     if (n == 0) return;
     const int64_t G = stride_tiles((int64_t)n, 256 * 4);

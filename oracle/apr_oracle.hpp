@@ -1729,6 +1729,76 @@ int richardson_lucy(const GenInfo& g, const Linear& L, const Tree& t, const InT*
     return it;
 }
 
+/* APRNumerics::gradient_cfd (APRNumerics.hpp:163-184): the central difference
+   [-1, 0, 1] / (2 delta) along dim (0 y, 1 x, 2 z), divided by 2^d at level delta d
+   (APRStencil::get_rescaled_stencils, APRStencil.hpp:135-152), convolve_pencil with reflected
+   ends and the input's tree means */
+template <typename PartT>
+void gradient_cfd(const GenInfo& g, const Linear& L, const Tree& t, const PartT* in, float* out, int dim, float delta) {
+    Mesh<float> s(dim == 0 ? 3 : 1, dim == 1 ? 3 : 1, dim == 2 ? 3 : 1);
+    s.v[0] = -1.0f / (2 * delta);
+    s.v[1] = 0.0f;
+    s.v[2] = 1.0f / (2 * delta);
+    const int nl = std::max(g.l_max - g.l_min, 1);
+    std::vector<Mesh<float>> st((size_t)nl, s);
+    for (int d = 0; d < nl; ++d) {
+        const float step = (float)std::pow(2.0, (double)d);
+        for (auto& w : st[(size_t)d].v) w = w / step;
+    }
+    std::vector<float> tr;
+    fill_tree_mean(g, L, t, in, tr);
+    convolve_pencil(g, L, t, st, in, tr.data(), out, true);
+}
+
+/* APRNumerics::richardson_lucy_tv (APRNumerics.hpp:489-525, :549-572): Richardson-Lucy whose
+   update divides by 1 - reg * div(grad u / |grad u|); the divergence of the normalised
+   gradient as div_norm_grad (:431-486) with unit pixel sizes: central differences of u along
+   each axis present, each normalised by sqrtf(g_z^2 + g_x^2 + g_y^2) where that exceeds 1e-6,
+   then the differences of the normalised components summed y, + x, + z */
+template <typename InT>
+void richardson_lucy_tv(const GenInfo& g, const Linear& L, const Tree& t, const InT* in, float* u, const Mesh<float>& psf, int niter,
+                        float reg, bool use_ds, bool normalize, bool resume) {
+    Mesh<float> flipped = psf;
+    for (size_t i = 0; i < psf.v.size(); ++i) flipped.v[i] = psf.v[psf.v.size() - 1 - i];
+    const int nst = use_ds ? g.l_max - g.l_min : 1;
+    std::vector<Mesh<float>> sv, fv;
+    downsampled_stencils(psf, sv, nst, normalize);
+    downsampled_stencils(flipped, fv, nst, normalize);
+    const size_t P = L.total;
+    if (!resume) std::fill(u, u + P, 1.0f);
+    std::vector<float> blur(P), est(P), gx(P), gy(P), gz(P), div(P), tmp(P), tr;
+    for (int it = 0; it < niter; ++it) {
+        fill_tree_mean(g, L, t, u, tr);
+        convolve_pencil(g, L, t, fv, u, tr.data(), blur.data(), true);
+        for (size_t i = 0; i < P; ++i) blur[i] = (float)in[i] / blur[i];
+        fill_tree_mean(g, L, t, blur.data(), tr);
+        convolve_pencil(g, L, t, sv, blur.data(), tr.data(), est.data(), true);
+        gradient_cfd(g, L, t, u, gy.data(), 0, 1.0f);
+        if (g.org[1] > 1) gradient_cfd(g, L, t, u, gx.data(), 1, 1.0f);
+        else std::fill(gx.begin(), gx.end(), 0.0f);
+        if (g.org[2] > 1) gradient_cfd(g, L, t, u, gz.data(), 2, 1.0f);
+        else std::fill(gz.begin(), gz.end(), 0.0f);
+        for (size_t i = 0; i < P; ++i) {
+            const float m = std::sqrt(gz[i] * gz[i] + gx[i] * gx[i] + gy[i] * gy[i]);
+            if (m > 1e-6) {
+                gz[i] /= m;
+                gx[i] /= m;
+                gy[i] /= m;
+            }
+        }
+        gradient_cfd(g, L, t, gy.data(), div.data(), 0, 1.0f);
+        if (g.x_num[g.l_max] > 1) {
+            gradient_cfd(g, L, t, gx.data(), tmp.data(), 1, 1.0f);
+            for (size_t i = 0; i < P; ++i) div[i] = tmp[i] + div[i];
+        }
+        if (g.z_num[g.l_max] > 1) {
+            gradient_cfd(g, L, t, gz.data(), tmp.data(), 2, 1.0f);
+            for (size_t i = 0; i < P; ++i) div[i] = tmp[i] + div[i];
+        }
+        for (size_t i = 0; i < P; ++i) u[i] = u[i] * est[i] / (1.0f - reg * div[i]);
+    }
+}
+
 /* APRReconstruction::reconstruct_constant (APRReconstruction.hpp:80-120) and the
    full-volume patch form with level_delta <= 0 (:297-394): the image at level
    L = l_max + level_delta; particles of level l <= L fill blocks of 2^(L-l) (clipped

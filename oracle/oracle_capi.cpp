@@ -441,6 +441,22 @@ int orc_richardson_lucy_f32(void* hp, const float* in, float* u, const float* ps
     return richardson_lucy(h->r.info, h->r.lin, h->get_tree(), in, u, m, niter, use_ds != 0, normalize != 0, resume != 0, tol, rel);
 }
 
+/* APRNumerics::richardson_lucy_tv (APRNumerics.hpp:489-525, :549-572) */
+void orc_richardson_lucy_tv_u16(void* hp, const uint16_t* in, float* u, const float* psf, int sy, int sx, int sz, int niter, float reg,
+                                int use_ds, int normalize, int resume) {
+    Handle* h = (Handle*)hp;
+    Mesh<float> m(sy, sx, sz);
+    std::copy(psf, psf + m.size(), m.v.begin());
+    richardson_lucy_tv(h->r.info, h->r.lin, h->get_tree(), in, u, m, niter, reg, use_ds != 0, normalize != 0, resume != 0);
+}
+void orc_richardson_lucy_tv_f32(void* hp, const float* in, float* u, const float* psf, int sy, int sx, int sz, int niter, float reg,
+                                int use_ds, int normalize, int resume) {
+    Handle* h = (Handle*)hp;
+    Mesh<float> m(sy, sx, sz);
+    std::copy(psf, psf + m.size(), m.v.begin());
+    richardson_lucy_tv(h->r.info, h->r.lin, h->get_tree(), in, u, m, niter, reg, use_ds != 0, normalize != 0, resume != 0);
+}
+
 /* APRNumerics::local_std (APRNumerics.hpp:357-389): the box mean (create_mean_filter,
    APRStencil.hpp:236-244, stencils downsampled and normalised) of the float copy of the input
    with its tree means, and of its square with the squared tree means; then

@@ -531,6 +531,31 @@ int orc_richardson_lucy_f32(void* hp, const float* in, float* u, const float* ps
     return niter;
 }
 
+/* APRNumerics::richardson_lucy_tv (APRNumerics.hpp:489-525, :549-572) */
+}  // extern "C"
+template <typename T>
+static void ref_rl_tv(void* hp, const T* in, float* u, const float* psf, int sy, int sx, int sz, int niter, float reg, int use_ds,
+                      int normalize, int resume) {
+    Handle* h = (Handle*)hp;
+    PixelData<float> m(sy, sx, sz);
+    std::copy(psf, psf + (size_t)sy * sx * sz, m.mesh.begin());
+    ParticleData<T> pi;
+    ParticleData<float> po;
+    fill_parts(pi, in, h->apr.total_number_particles());
+    if (resume) fill_parts(po, u, h->apr.total_number_particles());
+    APRNumerics::richardson_lucy_tv(h->apr, pi, po, m, niter, reg, use_ds != 0, normalize != 0, resume != 0);
+    std::copy(po.data.begin(), po.data.end(), u);
+}
+extern "C" {
+void orc_richardson_lucy_tv_u16(void* hp, const uint16_t* in, float* u, const float* psf, int sy, int sx, int sz, int niter, float reg,
+                                int use_ds, int normalize, int resume) {
+    ref_rl_tv(hp, in, u, psf, sy, sx, sz, niter, reg, use_ds, normalize, resume);
+}
+void orc_richardson_lucy_tv_f32(void* hp, const float* in, float* u, const float* psf, int sy, int sx, int sz, int niter, float reg,
+                                int use_ds, int normalize, int resume) {
+    ref_rl_tv(hp, in, u, psf, sy, sx, sz, niter, reg, use_ds, normalize, resume);
+}
+
 /* APRNumerics::gradient_cfd / gradient_sobel (APRNumerics.hpp:165-213); kind 0 cfd, 1 sobel */
 }  // extern "C"
 template <typename T>

@@ -24,6 +24,8 @@ int orc_richardson_lucy_u16(void* h, const uint16_t* in, float* u, const float* 
                             int normalize, int resume, double tol, double* rel);
 void orc_reconstruct(void* h, const void* parts, int pdt, int level_delta, void* img, int idt);
 void orc_local_std_u16(void* h, const uint16_t* in, const int* size, float* out);
+void orc_richardson_lucy_tv_u16(void* hp, const uint16_t* in, float* u, const float* psf, int sy, int sx, int sz, int niter, float reg,
+                                int use_ds, int normalize, int resume);
 void orc_reconstruct_smooth(void* hp, const void* parts, int pdt, const float* scale3, void* out, int idt);
 void orc_face_neighbour_filter_f32(void* hp, const float* in, const float* filter3, int dim, int repeats, float* out);
 void* orc_get_apr_by_block_u16(const uint16_t* img, int64_t y, int64_t x, int64_t z, const aprh_params* par, int64_t z_block_size,
@@ -185,6 +187,16 @@ int main() {
         const int sz3[3] = {3, 3, 3};
         orc_local_std_u16(o, oparts.data(), sz3, ols.data());
         CHECK(same_bytes(ls.data, ols.data(), P));
+    }
+
+    // total-variation Richardson-Lucy, 2 iterations
+    {
+        PixelData<float> psf = APRStencil::create_gaussian_filter({1.0f, 1.0f, 1.5f}, {5, 5, 5}, true);
+        ParticleData<float> tv;
+        APRNumerics::richardson_lucy_tv(apr, parts, tv, psf, 2, 0.01f, true, true, false);
+        std::vector<float> otv(P, 1.0f);
+        orc_richardson_lucy_tv_u16(o, oparts.data(), otv.data(), psf.mesh.data(), 5, 5, 5, 2, 0.01f, 1, 1, 0);
+        CHECK(same_bytes(tv.data, otv.data(), P));
     }
 
     // smooth reconstruction, default scale {2, 2, 2}, float image

@@ -89,6 +89,10 @@ def _bind(lib):
         getattr(lib, n).argtypes = [C.c_void_p, t, f32p, f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_double, C.POINTER(C.c_double)]
         getattr(lib, n).restype = C.c_int
+    for n, t in (("orc_richardson_lucy_tv_u16", u16p), ("orc_richardson_lucy_tv_f32", f32p)):
+        if hasattr(lib, n):
+            getattr(lib, n).argtypes = [C.c_void_p, t, f32p, f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int,
+                                        C.c_int]
     lib.orc_gaussian_stencil.argtypes = [f32p, i32p, C.c_int, f32p]
     if hasattr(lib, "orc_sample_image_f32"):
         lib.orc_sample_image_f32.argtypes = [C.c_void_p, f32p, f32p]
@@ -225,6 +229,17 @@ class OracleAPR:
         it = f(self.h, p, u, np.ascontiguousarray(psf, np.float32).reshape(-1), y, x, z, niter, int(use_ds), int(normalize),
                int(resume), tol, C.byref(rel))
         return u[:self.total], it, rel.value
+
+    def richardson_lucy_tv(self, parts, psf, niter, reg, use_ds=True, normalize=True, u=None):
+        """APRNumerics::richardson_lucy_tv (APRNumerics.hpp:489-525, :549-572)"""
+        resume = u is not None
+        u = np.ones(max(self.total, 1), np.float32) if u is None else np.ascontiguousarray(u, np.float32).copy()
+        z, x, y = psf.shape
+        f = self.lib.orc_richardson_lucy_tv_u16 if parts.dtype == np.uint16 else self.lib.orc_richardson_lucy_tv_f32
+        p = np.ascontiguousarray(parts) if parts.dtype == np.uint16 else np.ascontiguousarray(parts, np.float32)
+        f(self.h, p, u, np.ascontiguousarray(psf, np.float32).reshape(-1), y, x, z, niter, reg, int(use_ds), int(normalize),
+          int(resume))
+        return u[:self.total]
 
     def gradient(self, parts, sobel, dim, delta):
         """APRNumerics::gradient_cfd / gradient_sobel (reference build only)"""

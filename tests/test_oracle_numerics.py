@@ -262,6 +262,29 @@ def test_local_std_equal_reference(shape, nb, seed, size):
     assert np.array_equal(o.local_std(pf, size).view(np.uint32), r.local_std(pf, size).view(np.uint32))
 
 
+@pytest.mark.parametrize("shape,nb,seed", CASES)
+def test_richardson_lucy_tv_equal_reference(shape, nb, seed):
+    """APRNumerics::richardson_lucy_tv: the oracle's restatement (div_norm_grad over the oracle's
+    convolution, the TV update) against the reference build, bit for bit, u16 and f32 input,
+    fresh and resumed; reg = 0 is plain Richardson-Lucy"""
+    img, o = make_apr(shape, nb, seed, "oracle")
+    parts = o.sample(img)
+    psf = O.gaussian_stencil((1.0, 1.0, 1.5), (5, 5, 5), True)
+    got = o.richardson_lucy_tv(parts, psf, 3, 0.01)
+    assert np.isfinite(got).all() and float(got.max()) > 1.0
+    plain, _, _ = o.richardson_lucy(parts, psf, 3)
+    assert np.array_equal(o.richardson_lucy_tv(parts, psf, 3, 0.0).view(np.uint32), plain.view(np.uint32))
+    assert not np.array_equal(got, plain)
+    if O.load("ref") is None:
+        pytest.skip("reference build (oracle/_ref) not available")
+    _, r = make_apr(shape, nb, seed, "ref")
+    assert np.array_equal(got.view(np.uint32), r.richardson_lucy_tv(parts, psf, 3, 0.01).view(np.uint32))
+    pf = parts.astype(np.float32) * np.float32(0.7)
+    a = o.richardson_lucy_tv(pf, psf, 2, 0.02, u=got)
+    b = r.richardson_lucy_tv(pf, psf, 2, 0.02, u=got)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
 def _gradient_stack(o, dim, delta, sobel):
     """gradient_cfd / gradient_sobel's stencils (APRNumerics.hpp:165-213, APRStencil.hpp:135-152,
     :254-297) in float32 [z, x, y], one per level delta, rescaled by 2^-d"""
