@@ -423,6 +423,15 @@ def run_rl(args):
         per_it[name] = (time.perf_counter() - t) * 250.0
     exact_ms = per_it["exact"]
     log(f"[bench] RL iteration: reference tap order {per_it['exact']:.2f} ms, separable {per_it['separable']:.2f} ms")
+    # total-variation RL (APRNumerics::richardson_lucy_tv, reg 0.002): 1 untimed + 3 timed iterations per mode
+    tv_it = {}
+    for name, mode in (("exact", A.CONV_EXACT), ("separable", A.CONV_SEPARABLE)):
+        ctx.set_conv_mode(mode)
+        A.APRNumerics.richardson_lucy_tv(w.apr, w.parts.ptr, u.ptr, psf, 1, 0.002, normalize=True, dtype="uint16")
+        t = time.perf_counter()
+        A.APRNumerics.richardson_lucy_tv(w.apr, w.parts.ptr, u.ptr, psf, 3, 0.002, normalize=True, resume=True, dtype="uint16")
+        tv_it[name] = (time.perf_counter() - t) * 1000.0 / 3
+    log(f"[bench] RL-TV iteration: reference tap order {tv_it['exact']:.2f} ms, separable {tv_it['separable']:.2f} ms")
     Tn = w.apr.total_number_tree_particles()
     R = int(w.apr.xz_end_size) + int(w.apr.tree_info().xz_end_size)
     rl_ab = 2 * conv_algorithmic_bytes(w.P, Tn, R, 4) + w.P * (2 + 12)
@@ -458,6 +467,7 @@ def run_rl(args):
            "iteration_algorithmic_bytes": int(rl_ab),
            "roofline_separable_iteration": hbm_roofline(rl_ab, per_it["separable"]),
            "roofline_exact_iteration": hbm_roofline(rl_ab, per_it["exact"]),
+           "tv_ms_per_iteration": {"reg_factor": 0.002, "exact": round(tv_it["exact"], 2), "separable": round(tv_it["separable"], 2)},
            "warm_start": {"weights": trained, "iterations": ev["warm_iterations"], "time_to_tol_s": ev["warm_time_s"],
                           "network_s": ev["warm_network_time_s"], "final_rel_update": ev["warm_final_rel_update"],
                           "rel_l2_error_vs_gt": ev["warm_rel_l2_error_vs_gt"],

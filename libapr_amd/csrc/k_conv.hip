@@ -507,8 +507,10 @@ __global__ __launch_bounds__(CT) void k_conv_sep(ConvArgs<TIn> a) {
     const int64_t ntiles = (int64_t)L.ntx * L.nty * L.ntz;
     const int64_t tile = xcd_block(blockIdx.x, gridDim.x);
     if (tile >= ntiles) return;
-    const int tx = (int)(tile % L.ntx);
-    const int ty = (int)((tile / L.ntx) % L.nty);
+    /* y tiles fastest: consecutive blocks (one XCD's run) cover whole rows, so the partial
+       cache lines of a row's particle chunks meet in one L2 */
+    const int ty = (int)(tile % L.nty);
+    const int tx = (int)((tile / L.nty) % L.ntx);
     const int tz = (int)(tile / ((int64_t)L.ntx * L.nty));
     const int64_t x0 = (int64_t)tx * TX, y0 = (int64_t)ty * TY;
     const int yb0 = ty;
