@@ -1191,8 +1191,7 @@ void conv_levels(aprh_context* ctx, aprh_apr& a, const std::vector<HostStencil>&
         boff += pl.grid;
         if (pl.grid == 0) continue;
         if (pl.regz) {
-            if (pl.tx == 32) launch_conv_sep_sz<T, 32>(ca, pl.grid, pl.lds, st);
-            else launch_conv_sep_sz<T, 16>(ca, pl.grid, pl.lds, st);
+            launch_conv_sep_sz<T, 16>(ca, pl.grid, pl.lds, st);
             continue;
         }
         launch_conv_level<T>(ca, pl.tx, pl.sep, pl.grid, pl.lds, st);
