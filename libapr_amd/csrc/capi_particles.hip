@@ -131,22 +131,35 @@ void richardson_lucy_tv(aprh_context* ctx, aprh_apr& a, const T* in, float* u, c
     Buf gx, gz, tree(ctx->pool, std::max<uint64_t>(a.t_total, 1) * 4);
     if (has_x) gx.reset(ctx->pool, P * 4);
     if (has_z) gz.reset(ctx->pool, P * 4);
+    /* gradient_cfd's rescaled central differences (unit pixel size), always in the reference's
+       tap order: three taps cost little, and the normalisation below is sensitive to rounding
+       where the gradient is small */
+    std::vector<HostStencil> cfd[3];
+    for (int d = 0; d < 3; ++d) cfd[d] = rescaled_stencils(cfd_stencil(d, 1.0f), a.l_max - a.l_min);
+    float* tr = tree.as<float>();
+    auto grad = [&](const float* src, int d, float* dst) {
+        conv_levels<float>(ctx, a, cfd[d], false, src, tr, dst, true, 0, nullptr, false, nullptr, nullptr);
+    };
     for (int it = 0; it < niter; ++it) {
-        tree_mean<float>(ctx, a, u, tree.as<float>());
-        conv_levels<float>(ctx, a, psf_flipped, sep, u, tree.as<float>(), blur.as<float>(), true, 1, in, in_u16, nullptr, nullptr);
-        tree_mean<float>(ctx, a, blur.as<float>(), tree.as<float>());
-        conv_levels<float>(ctx, a, psf, sep, blur.as<float>(), tree.as<float>(), err.as<float>(), true, 0, nullptr, false, nullptr, nullptr);
-        gradient_along<float>(ctx, a, u, 0, 1.0f, false, gy.as<float>());
-        if (has_x) gradient_along<float>(ctx, a, u, 1, 1.0f, false, gx.as<float>());
-        if (has_z) gradient_along<float>(ctx, a, u, 2, 1.0f, false, gz.as<float>());
+        tree_mean<float>(ctx, a, u, tr);
+        conv_levels<float>(ctx, a, psf_flipped, sep, u, tr, blur.as<float>(), true, 1, in, in_u16, nullptr, nullptr);
+        /* the gradients of u use the same tree means of u (APRFilter.hpp:1088) */
+        grad(u, 0, gy.as<float>());
+        if (has_x) grad(u, 1, gx.as<float>());
+        if (has_z) grad(u, 2, gz.as<float>());
+        tree_mean<float>(ctx, a, blur.as<float>(), tr);
+        conv_levels<float>(ctx, a, psf, sep, blur.as<float>(), tr, err.as<float>(), true, 0, nullptr, false, nullptr, nullptr);
         rl_tv_step(ctx, 0, gy.as<float>(), has_x ? gx.as<float>() : nullptr, has_z ? gz.as<float>() : nullptr, 0.0f, P);
-        gradient_along<float>(ctx, a, gy.as<float>(), 0, 1.0f, false, dv.as<float>());
+        tree_mean<float>(ctx, a, gy.as<float>(), tr);
+        grad(gy.as<float>(), 0, dv.as<float>());
         if (has_x) {
-            gradient_along<float>(ctx, a, gx.as<float>(), 1, 1.0f, false, tmp.as<float>());
+            tree_mean<float>(ctx, a, gx.as<float>(), tr);
+            grad(gx.as<float>(), 1, tmp.as<float>());
             rl_tv_step(ctx, 1, dv.as<float>(), tmp.as<float>(), nullptr, 0.0f, P);
         }
         if (has_z) {
-            gradient_along<float>(ctx, a, gz.as<float>(), 2, 1.0f, false, tmp.as<float>());
+            tree_mean<float>(ctx, a, gz.as<float>(), tr);
+            grad(gz.as<float>(), 2, tmp.as<float>());
             rl_tv_step(ctx, 1, dv.as<float>(), tmp.as<float>(), nullptr, 0.0f, P);
         }
         rl_tv_step(ctx, 2, u, err.as<float>(), dv.as<float>(), reg, P);
