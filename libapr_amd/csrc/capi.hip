@@ -66,7 +66,7 @@ void solve(aprh_context* ctx, aprh_apr& a, const aprh_params& par, const float* 
 }
 
 template <typename G>
-void lis_and_solve(aprh_context* ctx, aprh_apr* a, const aprh_params& par, Buf& mean, G* grad, float* d_off);
+void lis_and_solve(aprh_context* ctx, aprh_apr* a, const aprh_params& par, Buf& mean, G* grad, float* d_off, const int* d_err);
 
 void get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x, int64_t z, const aprh_params& par, aprh_apr* a) {
     reset_times(ctx, ST_OFFSET, ST_SAMPLE);
@@ -130,11 +130,9 @@ void get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x,
     if (npass) tmp.reset(ctx->pool, nd * 4);
     float* mean_first = (npass & 1) ? tmp.as<float>() : mean.as<float>();
     launch_grad_ds_u16(ctx, I, grad.as<uint16_t>(), mean_first, y, x, z, par.dx, par.dy, par.dz, false);
-    int herr[3] = {0, 0, 0};
-    if (ctx->async_mode)
-        launch_copy(ctx, ctx->slot_err, d_err, sizeof(herr));
-    else
-        HIP_CHECK(hipMemcpyAsync(herr, d_err, sizeof(herr), hipMemcpyDeviceToHost, st));
+    /* the overflow counters are read with the B-spline offset at the end of the call (one host
+       synchronisation instead of a stall between the gradient and the inverse B-spline) */
+    if (ctx->async_mode) launch_copy(ctx, ctx->slot_err, d_err, 3 * sizeof(int));
     if (ctx->keep_intermediates) {
         keep_copy(ctx, a->smoothed, I, n * 2);
         keep_copy(ctx, a->grad_raw, grad.ptr(), nd * 2);
@@ -143,19 +141,12 @@ void get_apr_u16(aprh_context* ctx, const uint16_t* image, int64_t y, int64_t x,
     mark(ctx, ST_INV_BSPLINE);
     if (npass) launch_inv_bspline_pp(ctx, mean_first, (npass & 1) ? mean.as<float>() : tmp.as<float>(), yd, xd, zd, y > 2, x > 2, z > 2);
     tmp.release();
-    if (!ctx->async_mode) HIP_CHECK(hipStreamSynchronize(st));
-    static const char* axis_name[3] = {"y", "x", "z"};
-    for (int i = 0; i < 3; ++i)
-        if (herr[i])
-            throw Error(APRH_ERR_OVERFLOW, std::string("integer under-/overflow encountered in ComputeGradient::bspline_filt_rec_") +
-                                               axis_name[i] +
-                                               " - try squashing the input image to a narrower range or use APRConverter<float>");
-    lis_and_solve<uint16_t>(ctx, a, par, mean, grad.as<uint16_t>(), d_off);
+    lis_and_solve<uint16_t>(ctx, a, par, mean, grad.as<uint16_t>(), d_off, d_err);
 }
 
 /* local intensity scale, applyParameters, solveForAPR, generateDatastructures */
 template <typename G>
-void lis_and_solve(aprh_context* ctx, aprh_apr* a, const aprh_params& par, Buf& mean, G* grad, float* d_off) {
+void lis_and_solve(aprh_context* ctx, aprh_apr* a, const aprh_params& par, Buf& mean, G* grad, float* d_off, const int* d_err) {
     hipStream_t st = ctx->stream;
     const int64_t yd = cdiv(a->org[0], 2), xd = cdiv(a->org[1], 2), zd = cdiv(a->org[2], 2);
     const size_t nd = (size_t)(yd * xd * zd);
@@ -191,8 +182,17 @@ void lis_and_solve(aprh_context* ctx, aprh_apr* a, const aprh_params& par, Buf& 
         return;
     }
     HIP_CHECK(hipMemcpyAsync(&a->bspline_offset, d_off, 4, hipMemcpyDeviceToHost, st));
+    int herr[3] = {0, 0, 0};
+    if (d_err) HIP_CHECK(hipMemcpyAsync(herr, d_err, sizeof(herr), hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
     collect_times(ctx, ST_OFFSET, ST_SAMPLE);
+    /* ComputeGradient.hpp:409-412: the u16 smoothing's range check */
+    static const char* axis_name[3] = {"y", "x", "z"};
+    for (int i = 0; i < 3; ++i)
+        if (herr[i])
+            throw Error(APRH_ERR_OVERFLOW, std::string("integer under-/overflow encountered in ComputeGradient::bspline_filt_rec_") +
+                                               axis_name[i] +
+                                               " - try squashing the input image to a narrower range or use APRConverter<float>");
 }
 
 /* APRConverter<float>::get_apr: no B-spline offset (APRConverter.hpp:218-220), float
@@ -238,7 +238,7 @@ void get_apr_f32(aprh_context* ctx, const float* image, int64_t y, int64_t x, in
     mark(ctx, ST_INV_BSPLINE);
     if (npass) launch_inv_bspline_pp(ctx, mean_first, (npass & 1) ? mean.as<float>() : tmp.as<float>(), yd, xd, zd, y > 2, x > 2, z > 2);
     tmp.release();
-    lis_and_solve<float>(ctx, a, par, mean, grad.as<float>(), d_off);
+    lis_and_solve<float>(ctx, a, par, mean, grad.as<float>(), d_off, nullptr);
 }
 
 }  // namespace
