@@ -51,6 +51,8 @@ __host__ __device__ __forceinline__ uint16_t f2bf(float f) {
     u += 0x7fffu + ((u >> 16) & 1u);
     return (uint16_t)(u >> 16);
 }
+/* the same rounding (to nearest even) in one instruction, for finite values */
+__device__ __forceinline__ uint16_t f2bf_hw(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 __device__ __forceinline__ float bf2f(uint16_t h) {
     const uint32_t u = (uint32_t)h << 16;
     float f;
@@ -58,6 +60,13 @@ __device__ __forceinline__ float bf2f(uint16_t h) {
     return f;
 }
 __device__ __forceinline__ uint32_t pack2(uint16_t lo, uint16_t hi) { return (uint32_t)lo | ((uint32_t)hi << 16); }
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+/* two floats -> two bf16 (round to nearest even, finite values) in one word */
+__device__ __forceinline__ uint32_t pack2_bf16(float lo, float hi) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){lo, hi}, bf16x2));
+}
+#define LDS_PTR(p) ((__attribute__((address_space(3))) void*)(p))
 
 __device__ __forceinline__ f32x4 mfma(u32x4 a, u32x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
@@ -78,184 +87,257 @@ __device__ __forceinline__ void decompose(int v, int s, int& z, int& x, int& y) 
 
 /* ---------------- forward ---------------- */
 
-/* layer 1: x (f32, side s0) -> a (bf16, side s1 = s0 - 2), ReLU(conv + b1) */
-__global__ __launch_bounds__(256) void k_l1_fwd(const float* __restrict__ x, int s0, int s1, const u32x4* __restrict__ W1p,
-                                                const float* __restrict__ b1, uint16_t* a_cl, uint16_t* a_pl, int T) {
-    const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
-    const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    const int nv = s1 * s1 * s1, nin = s0 * s0 * s0;
-    const int nbt = (nv + 15) / 16;
-    const int64_t nblk = (int64_t)T * nbt;
-    const u32x4 wb = W1p[lane];
-    const float bias = b1[m];
-    for (int64_t blk = wid; blk < nblk; blk += nw) {
-        const int t = (int)(blk / nbt), b = (int)(blk % nbt);
-        const int v = b * 16 + m;
-        uint16_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        if (v < nv) {
-            int zo, xo, yo;
-            decompose(v, s1, zo, xo, yo);
-            const float* xt = x + (int64_t)t * nin;
+/* The forward layers run over bricks of 16 (y) x 8 (x) x 4 (z) output voxels.  A block stages the
+   brick's input, (brick + 2)^3 voxels, in LDS once, and its four waves take
+   the brick's 32 y rows of 16 voxels in turn: row (kx, kz) is one 16 x 16 x 32 MFMA chain with the
+   row's 16 voxels as M.  Every brick has the full layout, so a voxel's place is a shift and a mask;
+   rows past the output's x or z edge are skipped, lanes past its y edge compute rows that are never
+   stored (an MFMA row depends on its own A row only).  The one tap that does not exist (27, in the
+   odd half of pair 13) reads tap 26's data against zero weights. */
+constexpr int BY = 16, BX = 8, BZ = 4, IY = BY + 2, IX = BX + 2, IZ = BZ + 2, ROWS = BX * BZ;
+
+/* brick br of a launch over T tiles of nby x nbx x nbz bricks, y fastest */
+__device__ __forceinline__ void brick_origin(uint32_t br, uint32_t nby, uint32_t nbx, uint32_t nbz, int& t, int& z0, int& x0, int& y0) {
+    const uint32_t by = br % nby;
+    br /= nby;
+    const uint32_t bx = br % nbx;
+    br /= nbx;
+    const uint32_t bz = br % nbz;
+    t = (int)(br / nbz);
+    y0 = (int)by * BY;
+    x0 = (int)bx * BX;
+    z0 = (int)bz * BZ;
+}
+
+/* LDS offset (in voxels of the staged brick) of tap t */
+__host__ __device__ constexpr int tap_vox(int t) { return ((t / 9) * IX + (t / 3) % 3) * IY + t % 3; }
+__device__ __forceinline__ int tap_lds(int tap) { return tap_vox(min(tap, NT - 1)); }
+/* odd tap 2p + 1 minus even tap 2p in voxels: 1, IY - 2 or IX * IY - 2 IY - 2; 0 for pair 13,
+   whose odd half (tap 27) re-reads tap 26 */
+__host__ __device__ constexpr int pair_step(int p) { return p == NPAIR - 1 ? 0 : tap_vox(2 * p + 1) - tap_vox(2 * p); }
+
+/* one y row of a 16 -> 16 layer: lane (g, m) reads channels (g & 1) * 8 .. + 7 of tap
+   2p + (g >> 1) of voxel m as the A operand of pair p.  lb[k] is the lane's offset in the brick
+   plus its odd-tap step of class k (steps 0, 1, IY - 2, IX * IY - 2 IY - 2), so every read is a
+   per-row base and an immediate offset */
+__device__ __forceinline__ f32x4 row_mfma(const uint16_t* tile, int R, const int (&lb)[4], const u32x4 (&w)[NPAIR]) {
+    const uint16_t* b0 = tile + R + lb[0];
+    const uint16_t* b1 = tile + R + lb[1];
+    const uint16_t* b2 = tile + R + lb[2];
+    const uint16_t* b3 = tile + R + lb[3];
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const int k = g * 8 + j;
-                if (k < NT) {
-                    int dz, dx, dy;
-                    tap_offsets(k, dz, dx, dy);
-                    h[j] = f2bf(xt[((zo + dz) * s0 + xo + dx) * s0 + yo + dy]);
-                }
+    for (int p = 0; p < NPAIR; ++p) {
+        const int st = pair_step(p);
+        const uint16_t* b = st == 0 ? b0 : st == 1 ? b1 : st == IY - 2 ? b2 : b3;
+        acc = mfma(*(const u32x4*)(b + tap_vox(2 * p) * CH), w[p], acc);
+    }
+    return acc;
+}
+__device__ __forceinline__ void row_lane_offsets(int m, int g, int (&lb)[4]) {
+    const int lo = m * CH + (g & 1) * 8, th = g >> 1;
+    lb[0] = lo;
+    lb[1] = lo + th * CH;
+    lb[2] = lo + th * (IY - 2) * CH;
+    lb[3] = lo + th * (IX * IY - 2 * IY - 2) * CH;
+}
+static_assert(pair_step(0) == 1 && pair_step(1) == IY - 2 && pair_step(4) == IX * IY - 2 * IY - 2, "tap steps");
+
+/* stages the brick's input, IZ x IX x IY voxels of 16 channels from (z0, x0, y0) + off of a tile
+   of side sin, as tile[(lz * IX + lx) * IY + ly][16], 16-byte chunk c at tile[8 c].
+   DIRECT (the forward layers, off = 0): every chunk goes straight from global memory to LDS
+   (global_load_lds_dwordx4: lane l of a wave writes the wave's base + 16 l, which is the chunk
+   order), so all of a brick's loads are in flight at once and hold no registers.  Positions past
+   the input's far edges load a clamped position instead of zero: in a valid convolution they feed
+   only outputs past the output's edge, which are never stored.
+   Otherwise (the input gradients, off = -2, zero outside the input) through registers. */
+constexpr int NCH = IZ * IX * IY * 2, NCH_PT = (NCH + 255) / 256;
+template <bool DIRECT>
+__device__ __forceinline__ void stage_brick(uint16_t* tile, const uint16_t* __restrict__ it, int sin, int off, int z0, int x0, int y0) {
+    __syncthreads();   /* the previous brick's waves are done with the tile */
+    if constexpr (DIRECT) {
+        const int wbase = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * 64;
+#pragma unroll
+        for (int j = 0; j < NCH_PT; ++j) {
+            const int c = threadIdx.x + 256 * j;
+            if (c < NCH) {
+                const int q = c >> 1, h = c & 1;
+                const int ly = q % IY, r = q / IY, lx = r % IX, lz = r / IX;
+                const int zi = min(z0 + lz, sin - 1), xi = min(x0 + lx, sin - 1), yi = min(y0 + ly, sin - 1);
+                __builtin_amdgcn_global_load_lds((const void*)(it + (int64_t)((zi * sin + xi) * sin + yi) * CH + h * 8),
+                                                 LDS_PTR(tile + (wbase + 256 * j) * 8), 16, 0, 0);
             }
         }
-        const u32x4 av = {pack2(h[0], h[1]), pack2(h[2], h[3]), pack2(h[4], h[5]), pack2(h[6], h[7])};
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        acc = mfma(av, wb, acc);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    } else {
+        for (int c = threadIdx.x; c < NCH; c += 256) {
+            const int q = c >> 1, h = c & 1;
+            const int ly = q % IY, r = q / IY, lx = r % IX, lz = r / IX;
+            const int zi = z0 + off + lz, xi = x0 + off + lx, yi = y0 + off + ly;
+            u32x4 v = {0u, 0u, 0u, 0u};
+            if ((unsigned)zi < (unsigned)sin && (unsigned)xi < (unsigned)sin && (unsigned)yi < (unsigned)sin)
+                v = *(const u32x4*)(it + (int64_t)((zi * sin + xi) * sin + yi) * CH + h * 8);
+            *(u32x4*)&tile[c * 8] = v;
+        }
+    }
+    __syncthreads();
+}
+
+/* layer 1: x (f32, side s0) -> a (bf16, side s1 = s0 - 2), ReLU(conv + b1).  The brick's input goes
+   straight to LDS as f32 (global_load_lds_dword, clamped at the far edges as in stage_brick) and
+   is rounded to bf16 when the A operand is built; lane (g, m) holds taps g * 8 .. g * 8 + 7 of
+   voxel m */
+constexpr int NIN1 = IZ * IX * IY, NIN1_PT = (NIN1 + 255) / 256;
+__global__ __launch_bounds__(256) void k_l1_lds(const float* __restrict__ x, int s0, int s1, const u32x4* __restrict__ W1p,
+                                                const float* __restrict__ b1, uint16_t* a_cl, uint16_t* a_pl, uint32_t nbricks, uint32_t nby,
+                                                uint32_t nbx, uint32_t nbz) {
+    __shared__ __attribute__((aligned(16))) float tile[NIN1_PT * 256];
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), m = lane & 15, g = lane >> 4;
+    const u32x4 wb = W1p[lane];
+    const float bias = b1[m];
+    int toff[8];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int vo = b * 16 + g * 4 + i;
-            if (vo < nv) {
-                const float val = fmaxf(acc[i] + bias, 0.0f);
-                const uint16_t hv = f2bf(val);
-                a_cl[((int64_t)t * nv + vo) * CH + m] = hv;
-                if (a_pl) a_pl[((int64_t)t * CH + m) * nv + vo] = hv;
+    for (int j = 0; j < 8; ++j) toff[j] = tap_lds(g * 8 + j);
+    const int nv = s1 * s1 * s1, nin = s0 * s0 * s0;
+    for (uint32_t br = blockIdx.x; br < nbricks; br += gridDim.x) {
+        int t, z0, x0, y0;
+        brick_origin(br, nby, nbx, nbz, t, z0, x0, y0);
+        const float* xt = x + (int64_t)t * nin;
+        __syncthreads();   /* the previous brick's waves are done with the tile */
+#pragma unroll
+        for (int j = 0; j < NIN1_PT; ++j) {
+            const int c = threadIdx.x + 256 * j;
+            if (c < NIN1) {
+                const int ly = c % IY, r = c / IY, lx = r % IX, lz = r / IX;
+                const int zi = min(z0 + lz, s0 - 1), xi = min(x0 + lx, s0 - 1), yi = min(y0 + ly, s0 - 1);
+                __builtin_amdgcn_global_load_lds((const void*)(xt + (zi * s0 + xi) * s0 + yi), LDS_PTR(tile + wave * 64 + 256 * j), 4, 0, 0);
+            }
+        }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        for (int row = wave; row < ROWS; row += 4) {
+            const int kx = row % BX, kz = row / BX;
+            if (x0 + kx >= s1 || z0 + kz >= s1) continue;
+            const float* base = &tile[(kz * IX + kx) * IY + m];
+            float f[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) f[j] = base[toff[j]];
+            const u32x4 av = {pack2_bf16(f[0], f[1]), pack2_bf16(f[2], f[3]), pack2_bf16(f[4], f[5]), pack2_bf16(f[6], f[7])};
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = mfma(av, wb, acc);
+            const int vrow = ((z0 + kz) * s1 + x0 + kx) * s1 + y0 + g * 4;
+            const int64_t vc = ((int64_t)t * nv + vrow) * CH + m;
+            const int64_t vp = ((int64_t)t * CH + m) * nv + vrow;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (y0 + g * 4 + i < s1) {
+                    const uint16_t hv = f2bf_hw(fmaxf(acc[i] + bias, 0.0f));
+                    a_cl[vc + i * CH] = hv;
+                    if (a_pl) a_pl[vp + i] = hv;
+                }
             }
         }
     }
 }
 
-/* 16 -> 16 convolution: out(v) = sum_tap W[tap] in(v + tap + off) (zero outside the input),
-   then + bias, ReLU, and / or the mask (ReLU derivative of a stored activation).
-   The input is staged in LDS: a block owns an output brick of up to
-   16 (y) x 8 (x) x 4 (z) voxels, loads the (brick + 2)^3 input voxels (zero outside the input)
-   once, and its four waves take 16-voxel groups of the brick's voxels in turn, reading every
-   tap's A operand from LDS (the brick's linear voxel order, so edge bricks waste no MFMA work) */
-constexpr int BY = 16, BX = 8, BZ = 4, IY = BY + 2, IX = BX + 2, IZ = BZ + 2;
-
+/* 16 -> 16 convolution: out(v) = sum_tap W[tap] in(v + tap + off) (zero outside the input), then
+   + bias, ReLU, and / or the mask (ReLU derivative of a stored activation).  K of an MFMA is a pair
+   of taps x 16 input channels: lane (g, m) holds channels (g & 1) * 8 .. + 7 of tap 2p + (g >> 1) */
+template <bool DIRECT>
 __global__ __launch_bounds__(256) void k_conv16_lds(const uint16_t* __restrict__ in_cl, int sin, int sout, int off,
                                                     const u32x4* __restrict__ Wp, const float* __restrict__ bias,
                                                     const uint16_t* __restrict__ mask_cl, uint16_t* out_cl, uint16_t* out_pl, int relu,
-                                                    int T, int nby, int nbx, int nbz) {
+                                                    uint32_t nbricks, uint32_t nby, uint32_t nbx, uint32_t nbz) {
     __shared__ __attribute__((aligned(16))) uint16_t tile[IZ * IX * IY * CH];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, m = lane & 15, g = lane >> 4;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), m = lane & 15, g = lane >> 4;
     u32x4 w[NPAIR];
 #pragma unroll
     for (int p = 0; p < NPAIR; ++p) w[p] = Wp[p * 64 + lane];
     const float bv = bias ? bias[m] : 0.0f;
-    const int c8 = (g & 1) * 8, th = g >> 1;
-    /* LDS offset (in u16) of each pair's tap for this lane's half of the pair */
-    int toff[NPAIR];
-#pragma unroll
-    for (int p = 0; p < NPAIR; ++p) {
-        const int tap = min(2 * p + th, NT - 1);
-        int dz, dx, dy;
-        tap_offsets(tap, dz, dx, dy);
-        toff[p] = ((dz * IX + dx) * IY + dy) * CH;
-    }
-    const bool last_pair_live = th == 0;   /* tap 27 (pair 13, odd half) does not exist */
+    int lb[4];
+    row_lane_offsets(m, g, lb);
     const int nv = sout * sout * sout;
-    const int64_t nbricks = (int64_t)T * nby * nbx * nbz;
-    for (int64_t br = blockIdx.x; br < nbricks; br += gridDim.x) {
-        int64_t r = br;
-        const int by = (int)(r % nby);
-        r /= nby;
-        const int bx = (int)(r % nbx);
-        r /= nbx;
-        const int bz = (int)(r % nbz);
-        const int t = (int)(r / nbz);
-        const int y0 = by * BY, x0 = bx * BX, z0 = bz * BZ;
-        const int vy = min(BY, sout - y0), vx = min(BX, sout - x0), vz = min(BZ, sout - z0);
-        const int iy = vy + 2, ix = vx + 2, iz = vz + 2;
-        const uint16_t* it = in_cl + (int64_t)t * sin * sin * sin * CH;
-        __syncthreads();   /* the previous brick's waves are done with the tile */
-        const int nchunk = iz * ix * iy * 2;
-        for (int c = threadIdx.x; c < nchunk; c += 256) {
-            const int h = c & 1;
-            int q = c >> 1;
-            const int ly = q % iy;
-            q /= iy;
-            const int lx = q % ix, lz = q / ix;
-            const int zi = z0 + off + lz, xi = x0 + off + lx, yi = y0 + off + ly;
-            u32x4 v = {0u, 0u, 0u, 0u};
-            if (zi >= 0 && zi < sin && xi >= 0 && xi < sin && yi >= 0 && yi < sin)
-                v = *(const u32x4*)(it + (int64_t)((zi * sin + xi) * sin + yi) * CH + h * 8);
-            *(u32x4*)&tile[((lz * IX + lx) * IY + ly) * CH + h * 8] = v;
-        }
-        __syncthreads();
-        const int nvox = vy * vx * vz, ngroups = (nvox + 15) / 16;
-        for (int gi = wave; gi < ngroups; gi += 4) {
-            const int k = gi * 16 + m;
-            const bool valid = k < nvox;
-            int ky = 0, kx = 0, kz = 0;
-            if (valid) {
-                ky = k % vy;
-                const int q = k / vy;
-                kx = q % vx;
-                kz = q / vx;
-            }
-            const uint16_t* base = &tile[((kz * IX + kx) * IY + ky) * CH + c8];
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int p = 0; p < NPAIR; ++p) {
-                u32x4 av = {0u, 0u, 0u, 0u};
-                if (valid && (p < NPAIR - 1 || last_pair_live)) av = *(const u32x4*)(base + toff[p]);
-                acc = mfma(av, w[p], acc);
-            }
+    for (uint32_t br = blockIdx.x; br < nbricks; br += gridDim.x) {
+        int t, z0, x0, y0;
+        brick_origin(br, nby, nbx, nbz, t, z0, x0, y0);
+        stage_brick<DIRECT>(tile, in_cl + (int64_t)t * sin * sin * sin * CH, sin, off, z0, x0, y0);
+        for (int row = wave; row < ROWS; row += 4) {
+            const int kx = row % BX, kz = row / BX;
+            if (x0 + kx >= sout || z0 + kz >= sout) continue;
+            const f32x4 acc = row_mfma(tile, (kz * IX + kx) * IY * CH, lb, w);
+            /* this lane's four outputs: voxels vrow + g * 4 + i, channel m */
+            const int vrow = ((z0 + kz) * sout + x0 + kx) * sout + y0 + g * 4;
+            const int64_t vc = ((int64_t)t * nv + vrow) * CH + m;
+            const int64_t vp = ((int64_t)t * CH + m) * nv + vrow;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int kk = gi * 16 + g * 4 + i;
-                if (kk < nvox) {
-                    const int oy = kk % vy, q = kk / vy, ox = q % vx, oz = q / vx;
-                    const int vo = ((z0 + oz) * sout + (x0 + ox)) * sout + (y0 + oy);
+                if (y0 + g * 4 + i < sout) {
                     float val = acc[i] + bv;
                     if (relu) val = fmaxf(val, 0.0f);
-                    if (mask_cl && !(bf2f(mask_cl[((int64_t)t * nv + vo) * CH + m]) > 0.0f)) val = 0.0f;
-                    const uint16_t hv = f2bf(val);
-                    if (out_cl) out_cl[((int64_t)t * nv + vo) * CH + m] = hv;
-                    if (out_pl) out_pl[((int64_t)t * CH + m) * nv + vo] = hv;
+                    if (mask_cl && !(bf2f(mask_cl[vc + i * CH]) > 0.0f)) val = 0.0f;
+                    const uint16_t hv = f2bf_hw(val);
+                    if (out_cl) out_cl[vc + i * CH] = hv;
+                    if (out_pl) out_pl[vp + i] = hv;
                 }
             }
         }
     }
 }
 
-/* layer 4 + residual (+ loss gradient): y = x(centre) + b4 + sum W4 a3; one thread per output voxel */
-__global__ __launch_bounds__(256) void k_l4_fwd(const uint16_t* __restrict__ a3_cl, int S, const float* __restrict__ x,
-                                                const float* __restrict__ W4, const float* __restrict__ b4, const float* __restrict__ tgt,
-                                                float* y, float* g4, float gscale, double* part, int T) {
-    __shared__ float w[NT * CH];
+/* layer 4 + residual (+ loss gradient): y = x(centre) + b4 + sum W4 a3, on the MFMA.  The weights
+   keep f32 accuracy as two bf16 parts: W4p holds bf16(W4) in output columns 0-7 and
+   bf16(W4 - bf16(W4)) in columns 8-15, so column 0 + column 8 is the layer's output.  Lane
+   (g, m < 4) takes voxel g * 4 + m of its row, and a row's 16 outputs leave in one coalesced store.
+   With a target, also the loss gradient g4 = 2 (y - tgt) gscale and per-block partial sums of
+   (y - tgt)^2 and g4 */
+__global__ __launch_bounds__(256) void k_l4_lds(const uint16_t* __restrict__ a3_cl, int S, const float* __restrict__ x,
+                                                const u32x4* __restrict__ W4p, const float* __restrict__ b4, const float* __restrict__ tgt,
+                                                float* y, float* g4, float gscale, double* part, uint32_t nbricks, uint32_t nby, uint32_t nbx,
+                                                uint32_t nbz) {
+    __shared__ __attribute__((aligned(16))) uint16_t tile[IZ * IX * IY * CH];
     __shared__ double red[2][256];
-    for (int i = threadIdx.x; i < NT * CH; i += blockDim.x) w[i] = W4[i];
-    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), m = lane & 15, g = lane >> 4;
+    u32x4 w[NPAIR];
+#pragma unroll
+    for (int p = 0; p < NPAIR; ++p) w[p] = W4p[p * 64 + lane];
+    const float bv = b4[0];
+    int lb[4];
+    row_lane_offsets(m, g, lb);
     const int s3 = S + 2, s0 = S + 8;
     const int nv = S * S * S, n3 = s3 * s3 * s3, n0 = s0 * s0 * s0;
-    const int64_t total = (int64_t)T * nv;
     double dsum = 0, gsum = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-        const int t = (int)(i / nv), v = (int)(i % nv);
-        int zo, xo, yo;
-        decompose(v, S, zo, xo, yo);
-        const uint16_t* at = a3_cl + (int64_t)t * n3 * CH;
-        float acc = 0.0f;
-        for (int tap = 0; tap < NT; ++tap) {
-            int dz, dx, dy;
-            tap_offsets(tap, dz, dx, dy);
-            const u32x4* p = (const u32x4*)(at + (int64_t)(((zo + dz) * s3 + xo + dx) * s3 + yo + dy) * CH);
-            const u32x4 lo = p[0], hi = p[1];
-            const float* ww = w + tap * CH;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                acc += ww[2 * q] * bf2f((uint16_t)(lo[q] & 0xffffu)) + ww[2 * q + 1] * bf2f((uint16_t)(lo[q] >> 16));
-                acc += ww[8 + 2 * q] * bf2f((uint16_t)(hi[q] & 0xffffu)) + ww[8 + 2 * q + 1] * bf2f((uint16_t)(hi[q] >> 16));
+    for (uint32_t br = blockIdx.x; br < nbricks; br += gridDim.x) {
+        int t, z0, x0, y0;
+        brick_origin(br, nby, nbx, nbz, t, z0, x0, y0);
+        stage_brick<true>(tile, a3_cl + (int64_t)t * n3 * CH, s3, 0, z0, x0, y0);
+        for (int row = wave; row < ROWS; row += 4) {
+            const int kx = row % BX, kz = row / BX;
+            if (x0 + kx >= S || z0 + kz >= S) continue;
+            /* the residual (and target) loads go out before the MFMA chain */
+            const int zo = z0 + kz, xo = x0 + kx, yo = y0 + g * 4 + m;
+            const bool mine = m < 4 && yo < S;
+            const int64_t i = (int64_t)t * nv + (zo * S + xo) * S + yo;
+            float xv = 0.0f, tv = 0.0f;
+            if (mine) {
+                xv = x[(int64_t)t * n0 + ((zo + 4) * s0 + xo + 4) * s0 + yo + 4];
+                if (tgt) tv = tgt[i];
             }
-        }
-        const float out = x[(int64_t)t * n0 + ((zo + 4) * s0 + xo + 4) * s0 + yo + 4] + b4[0] + acc;
-        y[i] = out;
-        if (tgt) {
-            const float d = out - tgt[i];
-            const float gv = 2.0f * d * gscale;
-            g4[i] = gv;
-            dsum += (double)d * (double)d;
-            gsum += (double)gv;
+            const f32x4 acc = row_mfma(tile, (kz * IX + kx) * IY * CH, lb, w);
+            float sum[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sum[k] = acc[k] + __shfl_down(acc[k], 8, 16);
+            if (mine) {
+                const float a = m == 0 ? sum[0] : m == 1 ? sum[1] : m == 2 ? sum[2] : sum[3];
+                const float out = xv + bv + a;
+                y[i] = out;
+                if (tgt) {
+                    const float d = out - tv;
+                    const float gv = 2.0f * d * gscale;
+                    g4[i] = gv;
+                    dsum += (double)d * (double)d;
+                    gsum += (double)gv;
+                }
+            }
         }
     }
     if (!part) return;
@@ -490,9 +572,9 @@ __global__ void k_adam(float* p, const float* __restrict__ g, float* m, float* v
 }
 
 /* lane-packed bf16 weights for the MFMA kernels */
-__global__ void k_pack(const float* __restrict__ P, u32x4* W1p, u32x4* W2p, u32x4* W3p, u32x4* W2t, u32x4* W3t) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;   /* 0 .. 64 + 4 * 14 * 64 */
-    if (idx >= 64 + 4 * NPAIR * 64) return;
+__global__ void k_pack(const float* __restrict__ P, u32x4* W1p, u32x4* W2p, u32x4* W3p, u32x4* W2t, u32x4* W3t, u32x4* W4p) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;   /* 0 .. 64 + 5 * 14 * 64 */
+    if (idx >= 64 + 5 * NPAIR * 64) return;
     uint16_t h[8];
     u32x4* dst;
     if (idx < 64) {
@@ -502,6 +584,17 @@ __global__ void k_pack(const float* __restrict__ P, u32x4* W1p, u32x4* W2p, u32x
             h[j] = k < NT ? f2bf(P[OFF_W1 + k * CH + n]) : (uint16_t)0;
         }
         dst = W1p + l;
+    } else if (idx >= 64 + 4 * NPAIR * 64) {
+        /* layer 4: bf16(W4[tap][cin]) in output columns 0-7, the bf16 remainder in columns 8-15 */
+        const int q = idx - 64 - 4 * NPAIR * 64;
+        const int p = q / 64, l = q % 64, n = l & 15, g = l >> 4;
+        const int tap = 2 * p + (g >> 1), c0 = (g & 1) * 8;
+        for (int j = 0; j < 8; ++j) {
+            const float wv = tap < NT ? P[OFF_W4 + tap * CH + c0 + j] : 0.0f;
+            const uint16_t hi = f2bf(wv);
+            h[j] = n < 8 ? hi : f2bf(wv - bf2f(hi));
+        }
+        dst = W4p + q;
     } else {
         const int r = idx - 64;
         const int which = r / (NPAIR * 64), q = r % (NPAIR * 64);
@@ -589,7 +682,7 @@ const dim3 kBlk(256);
 struct aprh_cnn {
     aprh_context* ctx = nullptr;
     aprh::Buf P, M, V, G;                        /* params, Adam moments, gradient (f32) */
-    aprh::Buf W1p, W2p, W3p, W2t, W3t;           /* packed bf16 weights                  */
+    aprh::Buf W1p, W2p, W3p, W2t, W3t, W4p;      /* packed bf16 weights                  */
     int step = 0;
     float last_loss = 0;
     /* Gaussian applied to the warm-start particles (sizes 1: none) */
@@ -603,20 +696,39 @@ namespace {
 
 constexpr int WAVES = 2048;   /* waves of the weight-gradient kernels (one partial each) */
 
+/* the bricks of T tiles of output side s, and a grid of at most cap blocks over them */
+struct Bricks {
+    uint32_t n, ny, nx, nz;
+    unsigned grid;
+};
+Bricks bricks(int T, int s, int64_t cap) {
+    Bricks b;
+    b.ny = (uint32_t)cdiv(s, BY);
+    b.nx = (uint32_t)cdiv(s, BX);
+    b.nz = (uint32_t)cdiv(s, BZ);
+    const int64_t n = (int64_t)T * b.ny * b.nx * b.nz;
+    if (n > 0x7FFFFFFF) throw Error(APRH_ERR_INVALID_ARGUMENT, "too many tiles in one network pass");
+    b.n = (uint32_t)n;
+    b.grid = (unsigned)std::min<int64_t>(n, cap);
+    return b;
+}
+
 /* 16 -> 16 convolution of T tiles (input side sin, output side sout, input offset off) */
 void conv16(hipStream_t st, const uint16_t* in_cl, int sin, int sout, int off, const u32x4* Wp, const float* bias,
             const uint16_t* mask_cl, uint16_t* out_cl, uint16_t* out_pl, int relu, int T) {
-    const int nby = (int)cdiv(sout, BY), nbx = (int)cdiv(sout, BX), nbz = (int)cdiv(sout, BZ);
-    const int64_t nbricks = (int64_t)T * nby * nbx * nbz;
-    const unsigned grid = (unsigned)std::min<int64_t>(nbricks, 65536);
-    hipLaunchKernelGGL(k_conv16_lds, dim3(grid), kBlk, 0, st, in_cl, sin, sout, off, Wp, bias, mask_cl, out_cl, out_pl, relu, T, nby, nbx,
-                       nbz);
+    const Bricks b = bricks(T, sout, 65536);
+    if (off == 0)
+        hipLaunchKernelGGL(k_conv16_lds<true>, dim3(b.grid), kBlk, 0, st, in_cl, sin, sout, off, Wp, bias, mask_cl, out_cl, out_pl, relu, b.n, b.ny,
+                           b.nx, b.nz);
+    else
+        hipLaunchKernelGGL(k_conv16_lds<false>, dim3(b.grid), kBlk, 0, st, in_cl, sin, sout, off, Wp, bias, mask_cl, out_cl, out_pl, relu, b.n,
+                           b.ny, b.nx, b.nz);
     HIP_LAUNCH_CHECK();
 }
 
 void pack(aprh_cnn* n) {
-    hipLaunchKernelGGL(k_pack, dim3((64 + 4 * NPAIR * 64 + 255) / 256), kBlk, 0, n->ctx->stream, n->P.as<float>(), n->W1p.as<u32x4>(),
-                       n->W2p.as<u32x4>(), n->W3p.as<u32x4>(), n->W2t.as<u32x4>(), n->W3t.as<u32x4>());
+    hipLaunchKernelGGL(k_pack, dim3((64 + 5 * NPAIR * 64 + 255) / 256), kBlk, 0, n->ctx->stream, n->P.as<float>(), n->W1p.as<u32x4>(),
+                       n->W2p.as<u32x4>(), n->W3p.as<u32x4>(), n->W2t.as<u32x4>(), n->W3t.as<u32x4>(), n->W4p.as<u32x4>());
     HIP_LAUNCH_CHECK();
 }
 
@@ -624,9 +736,12 @@ struct Acts {
     Buf a1c, a1p, a2c, a2p, a3c, a3p;
 };
 
-/* forward of T tiles (input side S + 8); planar copies when training */
-void forward(aprh_cnn* n, const float* x, int T, int S, float* y, Acts& A, bool train, const float* tgt, float* g4, double* part,
-             int nblk_l4) {
+/* blocks of the layer-4 kernel (one loss partial each) */
+int l4_blocks(int T, int S) { return (int)bricks(T, S, 4096).grid; }
+
+/* forward of T tiles (input side S + 8); planar copies when training; part holds l4_blocks(T, S)
+   pairs of loss partials */
+void forward(aprh_cnn* n, const float* x, int T, int S, float* y, Acts& A, bool train, const float* tgt, float* g4, double* part) {
     hipStream_t st = n->ctx->stream;
     const int s0 = S + 8, s1 = S + 6, s2 = S + 4, s3 = S + 2;
     auto vox = [&](int s) { return (size_t)T * s * s * s; };
@@ -639,16 +754,19 @@ void forward(aprh_cnn* n, const float* x, int T, int S, float* y, Acts& A, bool 
         A.a3p.reset(n->ctx->pool, vox(s3) * CH * 2);
     }
     const float* P = n->P.as<float>();
-    auto grid = [&](size_t v) { return dim3((unsigned)std::min<size_t>((v / 16 + 3) / 4 + 1, 16384)); };
-    hipLaunchKernelGGL(k_l1_fwd, grid(vox(s1)), kBlk, 0, st, x, s0, s1, (const u32x4*)n->W1p.as<u32x4>(), P + OFF_B1, A.a1c.as<uint16_t>(),
-                       train ? A.a1p.as<uint16_t>() : (uint16_t*)nullptr, T);
-    HIP_LAUNCH_CHECK();
+    {
+        const Bricks b = bricks(T, s1, 65536);
+        hipLaunchKernelGGL(k_l1_lds, dim3(b.grid), kBlk, 0, st, x, s0, s1, (const u32x4*)n->W1p.as<u32x4>(), P + OFF_B1, A.a1c.as<uint16_t>(),
+                           train ? A.a1p.as<uint16_t>() : (uint16_t*)nullptr, b.n, b.ny, b.nx, b.nz);
+        HIP_LAUNCH_CHECK();
+    }
     conv16(st, A.a1c.as<uint16_t>(), s1, s2, 0, n->W2p.as<u32x4>(), P + OFF_B2, nullptr, A.a2c.as<uint16_t>(),
            train ? A.a2p.as<uint16_t>() : nullptr, 1, T);
     conv16(st, A.a2c.as<uint16_t>(), s2, s3, 0, n->W3p.as<u32x4>(), P + OFF_B3, nullptr, A.a3c.as<uint16_t>(),
            train ? A.a3p.as<uint16_t>() : nullptr, 1, T);
-    hipLaunchKernelGGL(k_l4_fwd, dim3(nblk_l4), kBlk, 0, st, (const uint16_t*)A.a3c.as<uint16_t>(), S, x, P + OFF_W4, P + OFF_B4, tgt, y, g4,
-                       1.0f / (float)vox(S), part, T);
+    const Bricks b4 = bricks(T, S, 4096);
+    hipLaunchKernelGGL(k_l4_lds, dim3(b4.grid), kBlk, 0, st, (const uint16_t*)A.a3c.as<uint16_t>(), S, x, (const u32x4*)n->W4p.as<u32x4>(),
+                       P + OFF_B4, tgt, y, g4, 1.0f / (float)vox(S), part, b4.n, b4.ny, b4.nx, b4.nz);
     HIP_LAUNCH_CHECK();
 }
 
@@ -657,10 +775,10 @@ void train_step(aprh_cnn* n, const float* x, const float* tgt, int T, int S, flo
     Pool* pool = n->ctx->pool;
     const int s1 = S + 6, s2 = S + 4, s3 = S + 2;
     auto vox = [&](int s) { return (size_t)T * s * s * s; };
-    const int nblk_l4 = (int)std::min<size_t>((vox(S) + 255) / 256, 4096);
+    const int nblk_l4 = l4_blocks(T, S);
     Buf y(pool, vox(S) * 4), g4(pool, vox(S) * 4), part(pool, (size_t)nblk_l4 * 2 * 8);
     Acts A;
-    forward(n, x, T, S, y.as<float>(), A, true, tgt, g4.as<float>(), part.as<double>(), nblk_l4);
+    forward(n, x, T, S, y.as<float>(), A, true, tgt, g4.as<float>(), part.as<double>());
     float* G = n->G.as<float>();
     const float* P = n->P.as<float>();
     /* loss and db4 */
@@ -775,7 +893,7 @@ void apply_volume(aprh_cnn* n, const void* img, int dt, int64_t yn, int64_t xn, 
                                xin.as<float>());
         HIP_LAUNCH_CHECK();
         Acts A;
-        forward(n, xin.as<float>(), T, S, y.as<float>(), A, false, nullptr, nullptr, nullptr, 4096);
+        forward(n, xin.as<float>(), T, S, y.as<float>(), A, false, nullptr, nullptr, nullptr);
         hipLaunchKernelGGL(k_scatter_tiles, dim3(8192), kBlk, 0, st, (const float*)y.as<float>(), S, o, T, yn, xn, zn, scale, floor_v, out);
         HIP_LAUNCH_CHECK();
     }
@@ -819,6 +937,7 @@ int aprh_cnn_create(aprh_context* ctx, uint64_t seed, aprh_cnn** out) {
         n->W3p.reset(ctx->pool, NPAIR * 64 * 16);
         n->W2t.reset(ctx->pool, NPAIR * 64 * 16);
         n->W3t.reset(ctx->pool, NPAIR * 64 * 16);
+        n->W4p.reset(ctx->pool, NPAIR * 64 * 16);
         /* He initialisation for layers 1-3, small last layer (the network starts near the identity) */
         std::mt19937_64 rng(seed);
         std::vector<float> p(N_PARAMS, 0.0f);
@@ -881,7 +1000,7 @@ int aprh_cnn_forward_tiles(aprh_context* ctx, aprh_cnn* n, const float* in, int3
     DevIn x(ctx, in, ni * 4);
     DevOut y(ctx, out, no * 4);
     Acts A;
-    forward(n, (const float*)x.ptr, T, S, (float*)y.ptr, A, false, nullptr, nullptr, nullptr, (int)std::min<size_t>((no + 255) / 256, 4096));
+    forward(n, (const float*)x.ptr, T, S, (float*)y.ptr, A, false, nullptr, nullptr, nullptr);
     y.finish(ctx);
     APRH_CATCH
 }
