@@ -506,6 +506,27 @@ def run_rl_iterations(args):
     return res
 
 
+def run_warm_network(args):
+    """--steps passes of the warm-start network (tile extraction, four layers, sampling back to
+    particles, no post filter) over the deconvolution workload's particles, after one untimed pass,
+    for profiling"""
+    import libapr_amd as A
+    from libapr_amd import warm_start as W
+    ctx = A.Context(0)
+    w = W.Workload(ctx, RL_SHAPE_ZXY, RL_GEN["seed"], RL_GEN["n_blobs"])
+    net = (A.WarmStartCNN.load(W.DEFAULT_WEIGHTS, ctx, post_filter=None) if os.path.exists(W.DEFAULT_WEIGHTS)
+           else A.WarmStartCNN(ctx, params=None))
+    u = A.DeviceArray(w.P * 4, ctx)
+    net.warm_start(w.apr, w.parts.ptr, out=u.ptr, dtype="uint16")
+    t = time.perf_counter()
+    for _ in range(max(args.steps, 1)):
+        net.warm_start(w.apr, w.parts.ptr, out=u.ptr, dtype="uint16")
+    ms = (time.perf_counter() - t) * 1000.0 / max(args.steps, 1)
+    log(f"[bench] warm-start network: {ms:.2f} ms per pass")
+    ctx.close()
+    return {"particles": w.P, "ms_per_pass": round(ms, 3)}
+
+
 def run_cpu_rl():
     """Reference CPU Richardson-Lucy (oracle/_ref Release build, else the oracle) timed for 3
     iterations on a 256x256x128 volume of the deconvolution generator (same blob density)"""
@@ -622,11 +643,12 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-separate", action="store_true",
                     help="skip the reference timing of get_apr and sample_image as separate calls (profiling runs)")
-    ap.add_argument("--only", choices=["get_apr", "batch", "copy", "conv", "rl"],
+    ap.add_argument("--only", choices=["get_apr", "batch", "copy", "conv", "rl", "warm"],
                     help="run one part (for profiling): the headline conversion steps, the batched volumes, "
                          "the device copy of 4 GiB that calibrates the HBM byte counters, the 5x5x5 convolution "
-                         "of the headline volume's particles (both modes), or --steps Richardson-Lucy iterations "
-                         "on the deconvolution workload (both modes)")
+                         "of the headline volume's particles (both modes), --steps Richardson-Lucy iterations "
+                         "on the deconvolution workload (both modes), or --steps passes of the warm-start "
+                         "network over that workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -634,7 +656,8 @@ def main():
         sys.exit(2)
     if args.only:
         part = {"get_apr": lambda: (lambda r: list(r[:4]) + [r[6]])(run_gpu(args, with_conv=False)), "batch": lambda: run_batch(args, check=False),
-                "copy": lambda: run_copy(args), "conv": lambda: run_gpu(args)[5], "rl": lambda: run_rl_iterations(args)}[args.only]()
+                "copy": lambda: run_copy(args), "conv": lambda: run_gpu(args)[5], "rl": lambda: run_rl_iterations(args),
+                "warm": lambda: run_warm_network(args)}[args.only]()
         print(json.dumps({"only": args.only, "result": part}), flush=True)
         return
     ms, P, stages, roofline, total_ab, conv5, extra = run_gpu(args)

@@ -11,8 +11,9 @@
  * (dz * 3 + dx) * 3 + dy.
  *
  * Precision: layers 1-3 run on v_mfma_f32_16x16x32_bf16 (bf16 weights and
- * activations, f32 accumulation); layer 4 (one output channel) and its input
- * gradient run on the VALU in f32.  Activations are stored channels-last
+ * activations, f32 accumulation); layer 4 (one output channel) runs on the
+ * same MFMA with its f32 weights split into two bf16 parts, and its input
+ * gradient on the VALU in f32.  Activations are stored channels-last
  * (16 bf16 = 32 B per voxel) for the convolutions and, in training, also
  * channel-planar for the weight-gradient GEMMs, whose reduction dimension is
  * the voxels.
