@@ -506,6 +506,29 @@ def run_rl_iterations(args):
     return res
 
 
+def run_tv_iterations(args):
+    """--steps total-variation Richardson-Lucy iterations (reg 0.002) on the deconvolution workload
+    in each convolution mode (after one untimed iteration), for profiling"""
+    import libapr_amd as A
+    from libapr_amd import warm_start as W
+    ctx = A.Context(0)
+    w = W.Workload(ctx, RL_SHAPE_ZXY, RL_GEN["seed"], RL_GEN["n_blobs"])
+    psf = W.psf_stencil()
+    u = A.DeviceArray(w.P * 4, ctx)
+    w.apr.tree_info()
+    res = {"particles": w.P}
+    for name, mode in (("separable", A.CONV_SEPARABLE), ("exact", A.CONV_EXACT)):
+        ctx.set_conv_mode(mode)
+        A.APRNumerics.richardson_lucy_tv(w.apr, w.parts.ptr, u.ptr, psf, 1, 0.002, normalize=True, dtype="uint16")
+        t = time.perf_counter()
+        A.APRNumerics.richardson_lucy_tv(w.apr, w.parts.ptr, u.ptr, psf, max(args.steps, 1), 0.002, normalize=True, resume=True,
+                                         dtype="uint16")
+        res[name + "_ms_per_iteration"] = round((time.perf_counter() - t) * 1000.0 / max(args.steps, 1), 3)
+        log(f"[bench] RL-TV ({name}): {res[name + '_ms_per_iteration']:.2f} ms per iteration")
+    ctx.close()
+    return res
+
+
 def run_warm_network(args):
     """--steps passes of the warm-start network (tile extraction, four layers, sampling back to
     particles, no post filter) over the deconvolution workload's particles, after one untimed pass,
@@ -643,12 +666,12 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-separate", action="store_true",
                     help="skip the reference timing of get_apr and sample_image as separate calls (profiling runs)")
-    ap.add_argument("--only", choices=["get_apr", "batch", "copy", "conv", "rl", "warm"],
+    ap.add_argument("--only", choices=["get_apr", "batch", "copy", "conv", "rl", "tv", "warm"],
                     help="run one part (for profiling): the headline conversion steps, the batched volumes, "
                          "the device copy of 4 GiB that calibrates the HBM byte counters, the 5x5x5 convolution "
                          "of the headline volume's particles (both modes), --steps Richardson-Lucy iterations "
-                         "on the deconvolution workload (both modes), or --steps passes of the warm-start "
-                         "network over that workload")
+                         "on the deconvolution workload (both modes), --steps total-variation RL iterations "
+                         "there (both modes), or --steps passes of the warm-start network over that workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -657,7 +680,7 @@ def main():
     if args.only:
         part = {"get_apr": lambda: (lambda r: list(r[:4]) + [r[6]])(run_gpu(args, with_conv=False)), "batch": lambda: run_batch(args, check=False),
                 "copy": lambda: run_copy(args), "conv": lambda: run_gpu(args)[5], "rl": lambda: run_rl_iterations(args),
-                "warm": lambda: run_warm_network(args)}[args.only]()
+                "tv": lambda: run_tv_iterations(args), "warm": lambda: run_warm_network(args)}[args.only]()
         print(json.dumps({"only": args.only, "result": part}), flush=True)
         return
     ms, P, stages, roofline, total_ab, conv5, extra = run_gpu(args)

@@ -65,6 +65,18 @@ HostStencil cfd_stencil(int dim, float delta) {
     return s;
 }
 
+/* a 3-tap stencil along one axis as a 3x3x3 stencil: the taps on the centre line, zeros elsewhere */
+HostStencil cube3(const HostStencil& s) {
+    HostStencil c;
+    c.sy = c.sx = c.sz = 3;
+    c.w.assign(27, 0.0f);
+    for (int k = 0; k < 3; ++k) {
+        const int y = s.sy == 3 ? k : 1, x = s.sx == 3 ? k : 1, z = s.sz == 3 ? k : 1;
+        c.w[(size_t)(z * 3 + x) * 3 + y] = s.w[(size_t)k];
+    }
+    return c;
+}
+
 /* APRStencil::create_sobel_filter (APRStencil.hpp:254-276) and the 2-D form (:278-297):
    difference along `dim`, [1/4, 1/2, 1/4] smoothing along the others, (y, x, z) products */
 HostStencil sobel_stencil(int dim, float delta, bool two_d) {
@@ -131,14 +143,14 @@ void richardson_lucy_tv(aprh_context* ctx, aprh_apr& a, const T* in, float* u, c
     Buf gx, gz, tree(ctx->pool, std::max<uint64_t>(a.t_total, 1) * 4);
     if (has_x) gx.reset(ctx->pool, P * 4);
     if (has_z) gz.reset(ctx->pool, P * 4);
-    /* gradient_cfd's rescaled central differences (unit pixel size), always in the reference's
-       tap order: three taps cost little, and the normalisation below is sensitive to rounding
-       where the gradient is small */
+    /* gradient_cfd's rescaled central differences (unit pixel size).  In separable mode each is
+       embedded in a 3x3x3 stencil (its taps on the centre line, zeros elsewhere), which the
+       register-ring kernel takes; in the reference's tap order they stay 1-D */
     std::vector<HostStencil> cfd[3];
-    for (int d = 0; d < 3; ++d) cfd[d] = rescaled_stencils(cfd_stencil(d, 1.0f), a.l_max - a.l_min);
+    for (int d = 0; d < 3; ++d) cfd[d] = rescaled_stencils(sep ? cube3(cfd_stencil(d, 1.0f)) : cfd_stencil(d, 1.0f), a.l_max - a.l_min);
     float* tr = tree.as<float>();
     auto grad = [&](const float* src, int d, float* dst) {
-        conv_levels<float>(ctx, a, cfd[d], false, src, tr, dst, true, 0, nullptr, false, nullptr, nullptr);
+        conv_levels<float>(ctx, a, cfd[d], sep, src, tr, dst, true, 0, nullptr, false, nullptr, nullptr);
     };
     for (int it = 0; it < niter; ++it) {
         tree_mean<float>(ctx, a, u, tr);
