@@ -31,6 +31,7 @@
 #include <random>
 
 #include "aprh_internal.hpp"
+#include "tiles.hpp"
 
 namespace aprh {
 
@@ -97,7 +98,9 @@ __device__ __forceinline__ void decompose(int v, int s, int& z, int& x, int& y) 
    odd half of pair 13) reads tap 26's data against zero weights. */
 constexpr int BY = 16, BX = 8, BZ = 4, IY = BY + 2, IX = BX + 2, IZ = BZ + 2, ROWS = BX * BZ;
 
-/* brick br of a launch over T tiles of nby x nbx x nbz bricks, y fastest */
+/* brick br of a launch over T tiles of nby x nbx x nbz bricks, y fastest.  The kernels number
+   their blocks XCD by XCD (xcd_block), so neighbouring bricks, which share their halo voxels,
+   meet in one L2 */
 __device__ __forceinline__ void brick_origin(uint32_t br, uint32_t nby, uint32_t nbx, uint32_t nbz, int& t, int& z0, int& x0, int& y0) {
     const uint32_t by = br % nby;
     br /= nby;
@@ -200,7 +203,7 @@ __global__ __launch_bounds__(256) void k_l1_lds(const float* __restrict__ x, int
 #pragma unroll
     for (int j = 0; j < 8; ++j) toff[j] = tap_lds(g * 8 + j);
     const int nv = s1 * s1 * s1, nin = s0 * s0 * s0;
-    for (uint32_t br = blockIdx.x; br < nbricks; br += gridDim.x) {
+    for (uint32_t br = (uint32_t)xcd_block(blockIdx.x, gridDim.x); br < nbricks; br += gridDim.x) {
         int t, z0, x0, y0;
         brick_origin(br, nby, nbx, nbz, t, z0, x0, y0);
         const float* xt = x + (int64_t)t * nin;
@@ -258,7 +261,7 @@ __global__ __launch_bounds__(256) void k_conv16_lds(const uint16_t* __restrict__
     int lb[4];
     row_lane_offsets(m, g, lb);
     const int nv = sout * sout * sout;
-    for (uint32_t br = blockIdx.x; br < nbricks; br += gridDim.x) {
+    for (uint32_t br = (uint32_t)xcd_block(blockIdx.x, gridDim.x); br < nbricks; br += gridDim.x) {
         int t, z0, x0, y0;
         brick_origin(br, nby, nbx, nbz, t, z0, x0, y0);
         stage_brick<DIRECT>(tile, in_cl + (int64_t)t * sin * sin * sin * CH, sin, off, z0, x0, y0);
@@ -307,7 +310,7 @@ __global__ __launch_bounds__(256) void k_l4_lds(const uint16_t* __restrict__ a3_
     const int s3 = S + 2, s0 = S + 8;
     const int nv = S * S * S, n3 = s3 * s3 * s3, n0 = s0 * s0 * s0;
     double dsum = 0, gsum = 0;
-    for (uint32_t br = blockIdx.x; br < nbricks; br += gridDim.x) {
+    for (uint32_t br = (uint32_t)xcd_block(blockIdx.x, gridDim.x); br < nbricks; br += gridDim.x) {
         int t, z0, x0, y0;
         brick_origin(br, nby, nbx, nbz, t, z0, x0, y0);
         stage_brick<true>(tile, a3_cl + (int64_t)t * n3 * CH, s3, 0, z0, x0, y0);
